@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: ms/frame of the Stage 1-4 render path on a 1,073,930-splat
+synthetic scene at 1920x1080 (120-frame orbit), on one MI355X.
+
+A step is one frame of the orbit: Stage 2 projection + SH, Stage 3 binning +
+radix sort + tile ranges, Stage 4 compositing, with the scene resident in HBM
+and the camera moving every step (the Stage 3 cache never hits).  Prints one
+JSON line.  The CPU baseline (the oracle, a C++ restatement of the reference
+pipeline, OpenMP over the host cores) renders a bounded sample of the same
+frames on the host in the same run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(N: int, S: int, E: int, T: int, P: int) -> dict:
+    """Algorithmic HBM bytes per stage (f32, SH degree 3, 32-bit keys):
+    B2 = 44N + 236S (read pos/cov/opacity of every splat, SH of survivors,
+    write 40 B of attributes + 4 B extents per survivor; 4 B extents per culled
+    splat is inside the 44N), B3 = 20N + 56S + 76E + 16T (count 8N+24S, scan 8N,
+    emit 4N+32S+8E, sort 4 passes x 16 B = 64E, ranges 4E+16T),
+    B4 = 8T + 40E + 4P (reported, ALU-bound)."""
+    return {"stage2": 44 * N + 236 * S, "stage3": 20 * N + 56 * S + 76 * E + 16 * T,
+            "stage4": 8 * T + 40 * E + 4 * P, "sort": 64 * E,
+            "k_project": 44 * N + 236 * S}
+
+
+def cpu_info() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, frames, budget_s: float):
+    """Oracle frames (Stages 2-4) on the host, all cores, bounded to ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, timed here as the CPU baseline
+
+    from cindergs_amd import synthetic as syn
+
+    cores = min(os.cpu_count() or 1, 64)
+    oracle.set_threads(cores)
+    times = []
+    t_start = time.perf_counter()
+    for f in frames:
+        cam, comp = syn.view_orbit(f)
+        t0 = time.perf_counter()
+        oracle.render_frame(scene, cam, comp)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    ms = 1000.0 * float(np.mean(times))
+    return {"value": round(ms, 3), "unit": "ms/frame", "cores": cores, "kind": "port",
+            "sample": f"{len(times)} orbit frames of the 1,073,930-splat 1080p workload "
+                      f"(Stages 2-4), C++ oracle with OpenMP on {cores} threads ({cpu_info()})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--splats", type=int, default=1_073_930)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-detail", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box here has one GPU.",
+              file=sys.stderr)
+        return 2
+
+    from cindergs_amd import FrameParams, GpuPipelineState, StageTimings
+    from cindergs_amd import synthetic as syn
+
+    t0 = time.perf_counter()
+    scene = syn.scene_clusters(args.splats)
+    print(f"# scene {scene.count} splats generated in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    W, H = args.width, args.height
+    st = GpuPipelineState(0)
+    st.sync_scene_and_cages(scene, None)
+    cams = [syn.view_orbit(f, W, H) for f in range(120)]
+    params = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    tim = StageTimings()
+
+    for i in range(args.warmup):
+        cam, comp = cams[i % 120]
+        st.execute_frame_resident(cam, comp, params, tim)
+    st.synchronize()
+    print(f"# warmup done ({args.warmup} frames) on {st.device_name()}", file=sys.stderr, flush=True)
+
+    # timed region: K frames, each waits for its own counters (as the reference)
+    entries, survivors = [], []
+    st.synchronize()
+    t_begin = time.perf_counter()
+    for i in range(args.steps):
+        cam, comp = cams[(args.warmup + i) % 120]
+        st.execute_frame_resident(cam, comp, params, tim)
+        entries.append(tim.total_tile_entries)
+        survivors.append(tim.surviving_gaussians)
+    st.synchronize()
+    wall = time.perf_counter() - t_begin
+    ms_per_frame = 1000.0 * wall / args.steps
+    print(f"# timed {args.steps} frames: {ms_per_frame:.3f} ms/frame", file=sys.stderr, flush=True)
+
+    # per-kernel device times over the same orbit frames (HIP events on the
+    # context stream, one profiled frame per camera)
+    prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 2, 0)
+    per_kernel = {}
+    stage_ms = {"stage2": [], "stage3": [], "stage4": []}
+    nprof = min(args.steps, 24)
+    for i in range(nprof):
+        cam, comp = cams[(args.warmup + i) % 120]
+        st.execute_frame_resident(cam, comp, prof, tim)
+        stage_ms["stage2"].append(tim.stage2_project_ms)
+        stage_ms["stage3"].append(tim.stage3_bin_sort_ms)
+        stage_ms["stage4"].append(tim.stage4_composite_ms)
+        for name, ms in st.kernel_times():
+            per_kernel.setdefault(name, []).append(ms)
+    # kernels launched several times per frame (radix passes) are summed per frame
+    kern = {k: float(np.sum(v)) / nprof for k, v in per_kernel.items()}
+    launches = {k: len(v) / nprof for k, v in per_kernel.items()}
+
+    N = scene.count
+    S = int(np.mean(survivors))
+    E = int(np.mean(entries))
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    P = W * H
+    B = algorithmic_bytes(N, S, E, T, P)
+    # roofline of the HBM-bound kernel with the largest device time
+    hbm_kernels = {"k_project": B["k_project"], "k_radix_onesweep": 16 * E,
+                   "k_bin": 20 * N + 56 * S + 8 * E}
+    dom = max(hbm_kernels, key=lambda k: kern.get(k, 0.0) * launches.get(k, 1.0))
+    per_launch_ms = kern[dom] / launches[dom]
+    achieved = hbm_kernels[dom] / (per_launch_ms * 1e-3) / 1e9
+    stage_gbs = {}
+    for st_name, key in (("stage2", "stage2"), ("stage3", "stage3")):
+        ms = float(np.mean(stage_ms[st_name]))
+        stage_gbs[st_name] = {"ms": round(ms, 4), "GB/s": round(B[key] / (ms * 1e-3) / 1e9, 1),
+                              "frac_of_8TB/s": round(B[key] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+    stage_gbs["stage4"] = {"ms": round(float(np.mean(stage_ms["stage4"])), 4)}
+
+    result = {
+        "metric": "ms/frame (1.07M splats, 1080p)",
+        "value": round(ms_per_frame, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_frame, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 32-cluster mixture + 10% uniform background, SH degree 3)",
+        "config": {"workload": f"Stage 1-4 render, {N} splats, {W}x{H}, 120-frame orbit, "
+                               f"32-bit keys, no cages", "splats": N, "width": W, "height": H,
+                   "tiles": T},
+        "fps": round(1000.0 / ms_per_frame, 1),
+        "splats_per_s": round(N / (ms_per_frame * 1e-3), 1),
+        "counts": {"N": N, "S_mean": S, "E_mean": E, "T": T},
+        "stages": stage_gbs,
+        "kernels_ms_per_frame": {k: round(v, 4) for k, v in sorted(kern.items())},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "bytes_per_launch": int(hbm_kernels[dom]),
+                     "avg_launch_ms": round(per_launch_ms, 5)},
+    }
+    st.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(scene, [(args.warmup + i) % 120 for i in range(8)],
+                                              args.cpu_budget)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,307 @@
+// cgs_internal.h — device-side building blocks shared by the gfx950 kernels
+// and the launcher declarations used by the host runtime.
+//
+// Device data layout (all buffers owned by the runtime, see runtime.cpp):
+//   scene (rest pose)   planar SoA: pos[3][cap], cov[6][cap], opa[cap],
+//                       sh[48][n] f32 coefficient planes (or f16 planes)
+//   deformed scene      same planar SoA with cap = 2N when splitting is on
+//   projected splat     geom[8] f32 = {mu_x, mu_y, a, b, c, alpha, tz, q_cut}
+//                       color[4] f32 = {r, g, b, 0}; extents u32 (ext_x | ext_y<<16)
+//   stage 3             keys/ids u32[E] (+ ping-pong copies), ranges u32[2T]
+// Wave size is 64 (CDNA); every block size below is a multiple of 64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cgs_detmath.h"
+
+#define CGS_WAVE 64
+
+// Flags of the decoupled look-back status words.
+#define CGS_LB_AGG 1u
+#define CGS_LB_INC 2u
+
+namespace cgs {
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// inclusive scan across the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= (uint32_t)off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over a block of NT threads.
+// `s_tmp` must hold NT/64 words.  Returns the exclusive prefix; `total` gets the sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp,
+                                                         uint32_t& total) {
+  constexpr int NW = NT / 64;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / 64;
+  uint32_t inc = wave_inclusive_scan(v);
+  if (lane == 63) s_tmp[wave] = inc;
+  __syncthreads();
+  uint32_t wave_prefix = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    uint32_t s = s_tmp[w];
+    if (w < (int)wave) wave_prefix += s;
+    tot += s;
+  }
+  total = tot;
+  __syncthreads();
+  return wave_prefix + inc - v;
+}
+
+__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u32(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decoupled look-back on 64-bit status words {flag:2 | value:62}.  Partition
+// `part` publishes its aggregate, walks back over predecessors until an
+// inclusive prefix is found, then publishes its own inclusive prefix.
+// Called by one thread.  Predecessors always hold smaller partition ids taken
+// from an atomic ticket, so they are resident or finished: every wait ends.
+__device__ __forceinline__ uint64_t lookback_u64(uint64_t* status, uint32_t part,
+                                                 uint64_t aggregate) {
+  const uint64_t VMASK = (1ull << 62) - 1ull;
+  if (part == 0) {
+    st_release_u64(&status[0], ((uint64_t)CGS_LB_INC << 62) | (aggregate & VMASK));
+    return 0;
+  }
+  st_release_u64(&status[part], ((uint64_t)CGS_LB_AGG << 62) | (aggregate & VMASK));
+  uint64_t excl = 0;
+  int64_t q = (int64_t)part - 1;
+  while (q >= 0) {
+    uint64_t s = ld_acquire_u64(&status[q]);
+    uint32_t flag = (uint32_t)(s >> 62);
+    if (flag == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & VMASK;
+    if (flag == CGS_LB_INC) break;
+    --q;
+  }
+  st_release_u64(&status[part], ((uint64_t)CGS_LB_INC << 62) | ((excl + aggregate) & VMASK));
+  return excl;
+}
+
+// Same on 32-bit words {flag:2 | value:30} (radix digit counts, deform splits).
+__device__ __forceinline__ uint32_t lookback_u32(uint32_t* status, uint32_t part,
+                                                 uint32_t stride, uint32_t aggregate) {
+  const uint32_t VMASK = (1u << 30) - 1u;
+  if (part == 0) {
+    st_release_u32(&status[0], (CGS_LB_INC << 30) | (aggregate & VMASK));
+    return 0;
+  }
+  st_release_u32(&status[(size_t)part * stride], (CGS_LB_AGG << 30) | (aggregate & VMASK));
+  uint32_t excl = 0;
+  int64_t q = (int64_t)part - 1;
+  while (q >= 0) {
+    uint32_t s = ld_acquire_u32(&status[(size_t)q * stride]);
+    uint32_t flag = s >> 30;
+    if (flag == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & VMASK;
+    if (flag == CGS_LB_INC) break;
+    --q;
+  }
+  st_release_u32(&status[(size_t)part * stride],
+                 (CGS_LB_INC << 30) | ((excl + aggregate) & VMASK));
+  return excl;
+}
+
+}  // namespace cgs
+
+// ---------------------------------------------------------------------------
+// Launchers (defined in the k_*.hip files, called by runtime.cpp)
+// ---------------------------------------------------------------------------
+
+struct CgsSceneDev {
+  const float* pos;    // [3][cap]
+  const float* cov;    // [6][cap] (f32) — null when cov_h is used
+  const uint16_t* cov_h;  // [6][cap] f16 bits (storage variant)
+  const float* opa;    // [cap]
+  const float* sh;     // [48][n_rest] f32 planes — null when sh_h is used
+  const uint16_t* sh_h;   // [48][n_rest] f16 planes (degree-trimmed count used)
+  uint32_t cap;        // plane stride of pos/cov/opa
+  uint32_t sh_stride;  // plane stride of sh (n_rest)
+};
+
+struct CgsProjectArgs {
+  CgsSceneDev scene;
+  const uint32_t* bind_tet;     // [n_rest] or null
+  const float* tet_rot;         // [9*num_tets] col-major or null
+  const uint32_t* split_meta;   // [2*n_rest] (parent, tet) or null
+  const uint32_t* split_count;  // device scalar or null (== 0)
+  float camera[24];
+  float cam_pos[3];
+  float mip;
+  float min_opacity;
+  float min_radius;
+  uint32_t sh_degree;
+  uint32_t has_deformation;
+  uint32_t num_tets;
+  uint32_t num_rest;
+  uint32_t max_gaussians;
+  float* geom;       // [8*max]
+  float* color;      // [4*max]
+  uint32_t* extents; // [max]
+  uint32_t* survivors;  // device scalar (accumulated)
+};
+
+struct CgsBinArgs {
+  const float* geom;
+  const uint32_t* extents;
+  const uint32_t* split_count;  // device scalar or null
+  uint32_t num_rest;
+  uint32_t max_gaussians;
+  uint32_t width, height, tiles_x, tiles_y;
+  uint32_t mode64;           // 0: packed 32-bit keys, 1: (tile, depth bits) pairs
+  uint32_t base_depth_bits;  // 32-bit mode
+  uint32_t depth_bits;       // 32-bit mode
+  uint32_t cap;              // capacity of the entry arrays
+  uint32_t* keys;            // 32-bit mode: packed keys; 64-bit mode: tile ids
+  uint32_t* depths;          // 64-bit mode only
+  uint32_t* ids;
+  uint32_t* counts_out;      // optional [max] (per-splat tile counts)
+  uint32_t* offsets_out;     // optional [max] (exclusive scan)
+  uint32_t* total;           // device scalar: total entries (may exceed cap)
+  uint64_t* status;          // [num_parts] zeroed
+  uint32_t* ticket;          // zeroed
+};
+
+// Optional per-launch timing: mark(name) records an event before a launch;
+// the time of launch i is the interval between marks i and i+1.
+struct CgsProf {
+  static constexpr int MAX = 64;
+  hipEvent_t ev[MAX];
+  const char* names[MAX];
+  int n = 0;
+  bool on = false;
+  void mark(const char* name, hipStream_t s) {
+    if (!on || n >= MAX) return;
+    names[n] = name;
+    (void)hipEventRecord(ev[n], s);
+    ++n;
+  }
+};
+
+extern "C++" {
+void cgs_launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+void cgs_launch_deinterleave_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t k,
+                                 uint32_t out_stride, hipStream_t s);
+void cgs_launch_deinterleave_u16(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t k,
+                                 uint32_t out_stride, hipStream_t s);
+void cgs_launch_sh_f16_planes(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t stride,
+                              uint32_t rest, hipStream_t s);
+void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s);
+uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
+void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s);
+void cgs_launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
+                     uint64_t* status, uint32_t* ticket, hipStream_t s);
+uint32_t cgs_scan_num_parts(uint32_t n);
+
+// radix sort of (key, value) u32 pairs on `bits` low key bits, n read from
+// the device scalar `d_n` (or n_static when d_n is null).  `cap` bounds n.
+struct CgsRadixBuffers {
+  uint32_t* keys;
+  uint32_t* vals;
+  uint32_t* keys_alt;
+  uint32_t* vals_alt;
+  uint32_t* hist;      // [4][256]
+  uint32_t* status;    // [4][max_parts][256]
+  uint32_t* tickets;   // [4]
+  uint32_t cap;
+};
+uint32_t cgs_radix_num_parts(uint32_t cap);
+size_t cgs_radix_control_words(uint32_t cap);
+// Returns 1 if the sorted result is in the *_alt buffers, else 0.
+int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
+                          uint32_t bits, bool iota_vals, bool zero_control, hipStream_t s,
+                          CgsProf* prof = nullptr);
+void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
+                        uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
+                        hipStream_t s);
+void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_t cap,
+                            uint32_t shift, uint32_t total_tiles, uint32_t* ranges,
+                            hipStream_t s);
+
+struct CgsCompositeArgs {
+  const float* geom;
+  const float* color;
+  const uint32_t* sorted_ids;
+  const uint32_t* ranges;
+  uint32_t width, height, tiles_x, tiles_y, padded_row;
+  float bg[4];
+  uint32_t* out;
+};
+void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s);
+
+struct CgsTetArgs {
+  const float* verts;     // [3*V] interleaved
+  const uint32_t* elems;  // [4*T]
+  const float* inv_dm;    // [9*T] col-major
+  float* jac;             // [9*T]
+  float* rot;             // [9*T]
+  float* stretch;         // [T]
+  uint32_t num_tets;
+};
+void cgs_launch_tet_transforms(const CgsTetArgs& a, hipStream_t s);
+
+struct CgsDeformArgs {
+  const float* verts;
+  const uint32_t* elems;
+  const float* jac;
+  const float* stretch;
+  const uint32_t* bind_tet;  // [n]
+  const float* bind_w;       // [4n]
+  const float* rest_pos;     // planar [3][n]
+  const float* rest_cov;     // planar [6][n]
+  const float* rest_opa;     // [n]
+  uint32_t rest_stride;
+  float* out_pos;            // planar [3][cap]
+  float* out_cov;            // planar [6][cap]
+  float* out_opa;            // [cap]
+  uint32_t out_stride;
+  uint32_t* split_meta;      // [2n] or null
+  uint32_t* split_count;     // device scalar (written)
+  uint64_t* status;          // [parts] zeroed
+  uint32_t* ticket;          // zeroed
+  float split_threshold;
+  float mip;
+  uint32_t num_gaussians;
+  uint32_t num_tets;
+};
+uint32_t cgs_deform_num_parts(uint32_t n);
+void cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s);
+}

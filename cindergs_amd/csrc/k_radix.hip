@@ -1,0 +1,210 @@
+// k_radix.hip — stable LSD radix sort of (u32 key, u32 value) pairs.
+//
+// Replaces gpu_radix_sort_u32_pairs (src/render/kernels/radix_sort.rs:292-368):
+// the reference runs 4-bit digits, each pass as histogram + 3-level scan +
+// scatter with 16 per-digit block scans.  Here a pass is ONE kernel (the
+// "onesweep" scheme): 8-bit digits; a partition of 4096 keys is ranked with
+// wave64 ballots (8 ballots give the mask of lanes sharing a digit) and
+// per-wave LDS digit counters, partitions chain their per-digit counts with a
+// decoupled look-back, keys are staged in LDS in digit order and written out
+// as contiguous runs.  All digit histograms of all passes come from one
+// upfront read of the keys.  Ranking keeps the input order of equal digits,
+// so each pass and the whole sort are stable, which is the reference's order.
+#include "cgs_internal.h"
+
+namespace {
+
+constexpr int RT = 256;          // threads
+constexpr int RI = 16;           // keys per thread
+constexpr int RP = RT * RI;      // keys per partition
+constexpr int RW = RT / 64;      // waves
+constexpr int RADIX = 256;
+constexpr int MAX_PASSES = 4;
+
+__device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift) {
+  const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
+  return (1u << w) - 1u;
+}
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* d_n,
+                                                    uint32_t n_static, uint32_t cap,
+                                                    uint32_t bits, uint32_t* hist) {
+  __shared__ uint32_t s_h[MAX_PASSES * RADIX];
+  const uint32_t passes = (bits + 7u) / 8u;
+  for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256u) s_h[i] = 0u;
+  __syncthreads();
+  const uint32_t n = min(d_n ? *d_n : n_static, cap);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t k = keys[i];
+    for (uint32_t p = 0; p < passes; ++p) {
+      const uint32_t sh = p * 8u;
+      atomicAdd(&s_h[p * RADIX + ((k >> sh) & digit_mask(bits, sh))], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256u)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+struct PassArgs {
+  const uint32_t* keys_in;
+  const uint32_t* vals_in;  // null: value = input index (iota)
+  uint32_t* keys_out;
+  uint32_t* vals_out;
+  const uint32_t* d_n;
+  uint32_t n_static;
+  uint32_t cap;
+  uint32_t bits;
+  uint32_t shift;
+  const uint32_t* hist;  // this pass's 256 bins
+  uint32_t* status;      // this pass's [parts][256]
+  uint32_t* ticket;
+};
+
+__global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
+  __shared__ uint32_t s_keys[RP];
+  __shared__ uint32_t s_vals[RP];
+  __shared__ uint32_t s_whist[RW][RADIX];
+  __shared__ uint32_t s_lstart[RADIX];
+  __shared__ uint32_t s_gadj[RADIX];
+  __shared__ uint32_t s_tmp[RW];
+  __shared__ uint32_t s_part;
+
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63u, wave = t >> 6;
+  if (t == 0) s_part = atomicAdd(A.ticket, 1u);
+#pragma unroll
+  for (int w = 0; w < RW; ++w) s_whist[w][t] = 0u;
+  __syncthreads();
+  const uint32_t part = s_part;
+  const uint32_t n = min(A.d_n ? *A.d_n : A.n_static, A.cap);
+  const uint32_t base = part * (uint32_t)RP;
+  if (base >= n) return;  // uniform: no later partition waits on this one
+  const uint32_t cnt_part = min((uint32_t)RP, n - base);
+  const uint32_t dmask = digit_mask(A.bits, A.shift);
+
+  // rank: wave w owns keys [base + w*1024, base + (w+1)*1024), item i of lane l
+  // at offset i*64 + l, so items are visited in input order
+  uint32_t keys[RI], vals[RI], offs[RI];
+  const uint32_t wbase = base + wave * (uint32_t)(RI * 64);
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
+    const bool valid = idx < n;
+    const uint32_t k = valid ? A.keys_in[idx] : 0u;
+    const uint32_t v = valid ? (A.vals_in ? A.vals_in[idx] : idx) : 0u;
+    const uint32_t d = (k >> A.shift) & dmask;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t prior = cgs::mask_rank(m);
+    uint32_t wcount = 0;
+    if (valid) wcount = s_whist[wave][d];
+    // every lane has read its counter before the group leader bumps it
+    __builtin_amdgcn_wave_barrier();
+    if (valid && prior == 0u) s_whist[wave][d] = wcount + (uint32_t)__popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    keys[i] = k;
+    vals[i] = v;
+    offs[i] = valid ? wcount + prior : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+
+  // per digit (thread t == digit t): wave prefixes, partition count
+  uint32_t c[RW];
+  uint32_t dtotal = 0;
+#pragma unroll
+  for (int w = 0; w < RW; ++w) {
+    c[w] = s_whist[w][t];
+    s_whist[w][t] = dtotal;
+    dtotal += c[w];
+  }
+  uint32_t htot;
+  const uint32_t lstart = cgs::block_exclusive_scan<RT>(dtotal, s_tmp, htot);
+  const uint32_t hv = A.hist[t];
+  uint32_t hsum;
+  const uint32_t dbase = cgs::block_exclusive_scan<RT>(hv, s_tmp, hsum);
+  s_lstart[t] = lstart;
+  // look-back for this digit over the partitions before this one
+  const uint32_t excl = cgs::lookback_u32(A.status + t, part, RADIX, dtotal);
+  s_gadj[t] = dbase + excl - lstart;
+  __syncthreads();
+
+  // stage in LDS in digit order
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    if (offs[i] != 0xFFFFFFFFu) {
+      const uint32_t d = (keys[i] >> A.shift) & dmask;
+      const uint32_t pos = s_lstart[d] + s_whist[wave][d] + offs[i];
+      s_keys[pos] = keys[i];
+      s_vals[pos] = vals[i];
+    }
+  }
+  __syncthreads();
+
+  // write out: equal digits are consecutive in LDS and in the output
+  for (uint32_t j = t; j < cnt_part; j += RT) {
+    const uint32_t k = s_keys[j];
+    const uint32_t d = (k >> A.shift) & dmask;
+    const uint32_t dst = s_gadj[d] + j;
+    A.keys_out[dst] = k;
+    A.vals_out[dst] = s_vals[j];
+  }
+}
+
+}  // namespace
+
+uint32_t cgs_radix_num_parts(uint32_t cap) { return (cap + (uint32_t)RP - 1u) / (uint32_t)RP; }
+
+size_t cgs_radix_control_words(uint32_t cap) {
+  const size_t parts = cgs_radix_num_parts(cap) ? cgs_radix_num_parts(cap) : 1u;
+  return (size_t)MAX_PASSES * RADIX + (size_t)MAX_PASSES * parts * RADIX + MAX_PASSES;
+}
+
+int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
+                          uint32_t bits, bool iota_vals, bool zero_control, hipStream_t s,
+                          CgsProf* prof) {
+  if (bits == 0 || b.cap == 0) return 0;
+  if (bits > 32) bits = 32;
+  const uint32_t passes = (bits + 7u) / 8u;
+  const uint32_t parts = cgs_radix_num_parts(b.cap);
+  if (zero_control) {
+    // hist, status and tickets are contiguous when allocated by the runtime
+    cgs_launch_fill_u32(b.hist, 0u, (uint64_t)MAX_PASSES * RADIX, s);
+    cgs_launch_fill_u32(b.status, 0u, (uint64_t)MAX_PASSES * parts * RADIX, s);
+    cgs_launch_fill_u32(b.tickets, 0u, MAX_PASSES, s);
+  }
+  uint32_t hblocks = (b.cap + 255u) / 256u;
+  if (hblocks > 2048u) hblocks = 2048u;
+  if (prof) prof->mark("k_radix_hist", s);
+  hipLaunchKernelGGL(k_radix_hist, dim3(hblocks), dim3(256), 0, s, b.keys, d_n, n_static, b.cap,
+                     bits, b.hist);
+  const uint32_t* kin = b.keys;
+  const uint32_t* vin = iota_vals ? nullptr : b.vals;
+  int in_alt = 0;
+  for (uint32_t p = 0; p < passes; ++p) {
+    PassArgs A;
+    A.keys_in = kin;
+    A.vals_in = vin;
+    A.keys_out = in_alt ? b.keys : b.keys_alt;
+    A.vals_out = in_alt ? b.vals : b.vals_alt;
+    A.d_n = d_n;
+    A.n_static = n_static;
+    A.cap = b.cap;
+    A.bits = bits;
+    A.shift = p * 8u;
+    A.hist = b.hist + p * RADIX;
+    A.status = b.status + (size_t)p * parts * RADIX;
+    A.ticket = b.tickets + p;
+    if (prof) prof->mark("k_radix_onesweep", s);
+    hipLaunchKernelGGL(k_radix_onesweep, dim3(parts), dim3(RT), 0, s, A);
+    kin = A.keys_out;
+    vin = A.vals_out;
+    in_alt ^= 1;
+  }
+  return in_alt;
+}

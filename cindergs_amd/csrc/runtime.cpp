@@ -1,0 +1,1221 @@
+// runtime.cpp — host runtime behind include/cindergs.h.
+//
+// Re-creates the state machine of the reference's GpuSceneState /
+// GpuFrameArena / GpuPipelineState (src/render/gpu_state.rs) on HIP:
+//   * scene residency with the same fingerprint rule (count, data pointers,
+//     64-sample FNV-1a; gpu_state.rs:65-79) and cage topology / vertex diffing
+//     (gpu_state.rs:215-343) that drives the Stage 1 dirty flag;
+//   * an arena of device buffers grown by 1.5x with a 256-word floor
+//     (gpu_state.rs:412-429, gpu_pool.rs:43-49);
+//   * the Stage 3 cache keyed on deform generation + camera + size + culling
+//     parameters (gpu_state.rs:454-499, 956-981);
+//   * execute_frame (gpu_state.rs:613-1243).
+// Differences by design: counts (split children, survivors, tile entries)
+// stay on the device for the whole frame and are read once at the end; the
+// tile-entry arena is sized from the previous frames and a frame that
+// overflows it is re-run from Stage 3 with a larger arena.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/cindergs.h"
+#include "cgs_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(CGS_ERR_EXECUTION, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap_words = 0;  // 4-byte words
+};
+
+// 1.5x growth with a 256-word floor (calculate_exponential_growth, gpu_pool.rs:43-49)
+int ensure_words(DevBuf& b, size_t words) {
+  size_t req = words < 1 ? 1 : words;
+  if (b.p && b.cap_words >= req) return CGS_OK;
+  size_t cap;
+  if (!b.p) {
+    cap = req < 256 ? 256 : req;
+  } else {
+    cap = b.cap_words + b.cap_words / 2;
+    if (cap < req) cap = req;
+  }
+  if (b.p) {
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap_words = 0;
+  }
+  HIPCHK(hipMalloc(&b.p, cap * 4));
+  b.cap_words = cap;
+  return CGS_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap_words = 0;
+}
+
+template <class T>
+T* P(DevBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
+  const uint8_t* p = static_cast<const uint8_t*>(data);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 0x100000001B3ull;
+  }
+  return h;
+}
+uint64_t fnv_u64(uint64_t h, uint64_t v) { return fnv1a(h, &v, 8); }
+
+// hash_f32_sample / hash_u32_sample (gpu_state.rs:37-63)
+uint64_t hash_sample(uint64_t h, const uint32_t* data, size_t len) {
+  h = fnv_u64(h, (uint64_t)len);
+  size_t step = len / 64;
+  if (step < 1) step = 1;
+  for (size_t i = 0; i < len; i += step) h = fnv1a(h, &data[i], 4);
+  if (len) h = fnv1a(h, &data[len - 1], 4);
+  return h;
+}
+
+const uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
+
+struct Stage3Key {
+  uint64_t deform_generation;
+  uint32_t view_bits[16];
+  uint32_t fx, fy, cx, cy, width, height, near_bits, mip_bits, min_op_bits, min_rad_bits;
+  bool operator==(const Stage3Key& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+uint32_t fbits(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+
+}  // namespace
+
+struct cgs_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+  char name[256] = {0};
+
+  // ---- scene residency ----
+  bool scene_known = false;
+  uint64_t scene_hash = 0;
+  uint32_t n_rest = 0;
+  bool f16 = false;
+  uint32_t f16_degree = 3;
+  DevBuf rest_pos, rest_cov, rest_cov_h, rest_opa, rest_sh, rest_sh_h, staging;
+
+  // ---- cages ----
+  bool has_cages = false;
+  uint64_t topo_hash = 0;
+  uint32_t num_tets = 0, num_verts = 0;
+  DevBuf elems, inv_dm, bind_tet, bind_w, verts;
+  std::vector<float> last_verts;
+  uint32_t last_params[2] = {0, 0};
+  bool deformed_valid = false;
+  uint64_t deform_generation = 0;
+
+  // ---- arena ----
+  DevBuf jac, rot, stretch, def_pos, def_cov, def_opa, split_meta;
+  uint32_t def_stride = 0;
+  DevBuf ctrl1;  // [split_count, ticket, pad, pad][u64 status...]
+  DevBuf geom, color, extents;
+  DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
+  uint32_t entry_cap = 0;
+  DevBuf ranges, out_rgba, cnt_dbg, off_dbg;
+  DevBuf ctrl3;  // frame control, see frame_layout()
+  uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
+  DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
+
+  // ---- last frame ----
+  uint32_t last_w = 0, last_h = 0;
+  uint32_t max_g = 0;          // splat slots of the projected arrays
+  uint32_t active = 0;         // active splats of the last frame
+  uint32_t tiles_x = 0, tiles_y = 0, padded_row = 0;
+  bool mode64 = false;
+  uint32_t depth_bits = 0;
+  uint32_t n_entries = 0;
+  uint32_t* sorted_keys = nullptr;
+  uint32_t* sorted_ids = nullptr;
+  bool capture = false;
+  bool captured = false;
+  bool s3_valid = false;
+  Stage3Key s3_key{};
+  uint32_t s3_survivors = 0;
+  CgsProf prof;
+  int prof_count = 0;
+  double prof_ms[CgsProf::MAX];
+  const char* prof_names[CgsProf::MAX];
+};
+
+namespace {
+
+// frame control words (zeroed once per frame before Stage 2)
+struct FrameLayout {
+  size_t survivors, bin_ticket, total;
+  size_t rA_tickets, rB_tickets;  // 4 each
+  size_t histA, histB;            // 1024 each
+  size_t bin_status;              // u64 words, 8-byte aligned (index in u32 words)
+  size_t rA_status, rB_status;
+  size_t words;
+  uint32_t bin_parts, r_parts;
+};
+
+FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
+  FrameLayout L;
+  L.survivors = 0;
+  L.bin_ticket = 1;
+  L.total = 2;
+  L.rA_tickets = 4;
+  L.rB_tickets = 8;
+  L.histA = 12;
+  L.histB = 12 + 1024;
+  size_t w = 12 + 2048;
+  if (w & 1) w++;
+  L.bin_status = w;
+  L.bin_parts = cgs_bin_num_parts(max_g);
+  w += 2 * (size_t)(L.bin_parts ? L.bin_parts : 1);
+  L.r_parts = cgs_radix_num_parts(cap);
+  size_t rs = 4 * (size_t)(L.r_parts ? L.r_parts : 1) * 256;
+  L.rA_status = w;
+  w += rs;
+  L.rB_status = w;
+  w += rs;
+  L.words = w;
+  return L;
+}
+
+int ensure_ctx(cgs_context* c) {
+  if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  return CGS_OK;
+}
+
+#define TRY(expr)              \
+  do {                         \
+    int r_ = (expr);           \
+    if (r_ != CGS_OK) return r_; \
+  } while (0)
+
+int upload_planar_f32(cgs_context* c, DevBuf& dst, const float* src, uint32_t n, uint32_t k,
+                      uint32_t stride) {
+  TRY(ensure_words(dst, (size_t)k * stride));
+  if ((size_t)n * k == 0) return CGS_OK;
+  TRY(ensure_words(c->staging, (size_t)n * k));
+  HIPCHK(hipMemcpyAsync(c->staging.p, src, (size_t)n * k * 4, hipMemcpyHostToDevice, c->stream));
+  cgs_launch_deinterleave_u32(P<uint32_t>(c->staging), P<uint32_t>(dst), n, k, stride, c->stream);
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+uint64_t scene_fingerprint(const float* pos, const float* cov, const float* opa, const float* sh,
+                           uint64_t n, size_t cov_words, size_t sh_words) {
+  uint64_t h = FNV_OFFSET;
+  h = fnv_u64(h, n);
+  h = fnv_u64(h, (uint64_t)(uintptr_t)pos);
+  h = fnv_u64(h, (uint64_t)(uintptr_t)cov);
+  h = fnv_u64(h, (uint64_t)(uintptr_t)opa);
+  h = fnv_u64(h, (uint64_t)(uintptr_t)sh);
+  h = hash_sample(h, reinterpret_cast<const uint32_t*>(pos), (size_t)n * 3);
+  h = hash_sample(h, reinterpret_cast<const uint32_t*>(cov), (size_t)n * cov_words);
+  h = hash_sample(h, reinterpret_cast<const uint32_t*>(opa), (size_t)n);
+  h = hash_sample(h, reinterpret_cast<const uint32_t*>(sh), (size_t)n * sh_words);
+  return h;
+}
+
+void scene_changed(cgs_context* c) {
+  c->deformed_valid = false;
+  c->topo_hash = 0;  // force topology re-upload (bindings are per splat)
+  c->has_cages = false;
+  c->deform_generation++;
+}
+
+
+struct FrameGeom {
+  uint32_t W, H, tiles_x, tiles_y, T, padded_row;
+};
+
+FrameGeom frame_geom(const cgs_compositor_uniforms* comp) {
+  FrameGeom g;
+  g.W = comp->screen_width < 1 ? 1 : comp->screen_width;
+  g.H = comp->screen_height < 1 ? 1 : comp->screen_height;
+  g.tiles_x = (g.W + 15) / 16;
+  g.tiles_y = (g.H + 15) / 16;
+  g.T = g.tiles_x * g.tiles_y;
+  g.padded_row = (g.W + 63) / 64 * 64;  // padded_rgba8_row_u32 (gpu_state.rs:25-27)
+  return g;
+}
+
+void init_entry_cap(cgs_context* c, uint32_t max_g) {
+  if (c->entry_cap != 0) return;
+  uint64_t guess = (uint64_t)max_g * 4u;
+  if (guess < (1u << 20)) guess = 1u << 20;
+  if (guess > (1u << 29)) guess = 1u << 29;
+  c->entry_cap = (uint32_t)guess;
+}
+
+// Stage 3 on the projected splats in the arena (geom/extents, max_g slots,
+// active = num_rest + *split_count).  Launch-only; the caller checks the
+// total against the capacity after synchronising.
+int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t num_rest,
+                  const uint32_t* split_count, uint32_t max_g, bool keep_survivors) {
+  c->tiles_x = g.tiles_x;
+  c->tiles_y = g.tiles_y;
+  c->mode64 = g.T > 65535u;
+  init_entry_cap(c, max_g);
+  const uint32_t cap = c->entry_cap;
+  TRY(ensure_words(c->e_keys, cap));
+  TRY(ensure_words(c->e_ids, cap));
+  TRY(ensure_words(c->e_alt_k, cap));
+  TRY(ensure_words(c->e_alt_v, cap));
+  if (c->mode64) {
+    TRY(ensure_words(c->e_depths, cap));
+    TRY(ensure_words(c->e_perm, cap));
+  }
+  TRY(ensure_words(c->ranges, (size_t)g.T * 2));
+  const FrameLayout L = frame_layout(max_g, cap);
+  TRY(ensure_words(c->ctrl3, L.words));
+  uint32_t* ctrl = P<uint32_t>(c->ctrl3);
+  if (keep_survivors)
+    cgs_launch_fill_u32(ctrl + 1, 0u, L.words - 1, c->stream);
+  else
+    cgs_launch_fill_u32(ctrl, 0u, L.words, c->stream);
+  cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, c->stream);
+  c->captured = false;
+  if (c->capture) {
+    TRY(ensure_words(c->cnt_dbg, max_g));
+    TRY(ensure_words(c->off_dbg, max_g));
+    c->captured = true;
+  }
+
+  uint32_t tile_bits, depth_bits;
+  tile_bits = g.T <= 1 ? 1u : 32u - (uint32_t)__builtin_clz(g.T - 1u);
+  depth_bits = 32u - tile_bits;
+  if (depth_bits > 28u) depth_bits = 28u;
+  c->depth_bits = depth_bits;
+
+  CgsBinArgs B;
+  memset(&B, 0, sizeof(B));
+  B.geom = P<float>(c->geom);
+  B.extents = P<uint32_t>(c->extents);
+  B.split_count = split_count;
+  B.num_rest = num_rest;
+  B.max_gaussians = max_g;
+  B.width = g.W;
+  B.height = g.H;
+  B.tiles_x = g.tiles_x;
+  B.tiles_y = g.tiles_y;
+  B.mode64 = c->mode64 ? 1u : 0u;
+  float np = near_plane < 1e-6f ? 1e-6f : near_plane;
+  B.base_depth_bits = fbits(np) & 0x7F800000u;
+  B.depth_bits = depth_bits;
+  B.cap = cap;
+  B.keys = P<uint32_t>(c->e_keys);
+  B.depths = c->mode64 ? P<uint32_t>(c->e_depths) : nullptr;
+  B.ids = P<uint32_t>(c->e_ids);
+  B.counts_out = c->capture ? P<uint32_t>(c->cnt_dbg) : nullptr;
+  B.offsets_out = c->capture ? P<uint32_t>(c->off_dbg) : nullptr;
+  B.total = ctrl + L.total;
+  B.status = reinterpret_cast<uint64_t*>(ctrl + L.bin_status);
+  B.ticket = ctrl + L.bin_ticket;
+  c->prof.mark("k_bin", c->stream);
+  cgs_launch_bin(B, c->stream);
+
+  const uint32_t* d_n = ctrl + L.total;
+  if (!c->mode64) {
+    CgsRadixBuffers R;
+    R.keys = P<uint32_t>(c->e_keys);
+    R.vals = P<uint32_t>(c->e_ids);
+    R.keys_alt = P<uint32_t>(c->e_alt_k);
+    R.vals_alt = P<uint32_t>(c->e_alt_v);
+    R.hist = ctrl + L.histA;
+    R.status = ctrl + L.rA_status;
+    R.tickets = ctrl + L.rA_tickets;
+    R.cap = cap;
+    uint32_t sort_bits = tile_bits + depth_bits;
+    if (sort_bits > 32) sort_bits = 32;
+    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, false, c->stream, &c->prof);
+    c->sorted_keys = alt ? R.keys_alt : R.keys;
+    c->sorted_ids = alt ? R.vals_alt : R.vals;
+    c->prof.mark("k_tile_ranges", c->stream);
+    cgs_launch_tile_ranges(c->sorted_keys, d_n, cap, depth_bits, g.T, P<uint32_t>(c->ranges),
+                           c->stream);
+  } else {
+    // 64-bit fallback (gpu_radix_sort_u64_pairs, radix_sort.rs:377-477): stable
+    // sort by depth with the entry index as payload, gather, stable sort by tile
+    CgsRadixBuffers A;
+    A.keys = P<uint32_t>(c->e_depths);
+    A.vals = P<uint32_t>(c->e_perm);
+    A.keys_alt = P<uint32_t>(c->e_alt_k);
+    A.vals_alt = P<uint32_t>(c->e_alt_v);
+    A.hist = ctrl + L.histA;
+    A.status = ctrl + L.rA_status;
+    A.tickets = ctrl + L.rA_tickets;
+    A.cap = cap;
+    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, false, c->stream, &c->prof);
+    const uint32_t* perm = altA ? A.vals_alt : A.vals;
+    uint32_t* gk = altA ? P<uint32_t>(c->e_depths) : P<uint32_t>(c->e_alt_k);
+    uint32_t* gv = altA ? P<uint32_t>(c->e_perm) : P<uint32_t>(c->e_alt_v);
+    // with 4 passes altA == 0: perm in e_perm, gather into the alt pair
+    c->prof.mark("k_gather2", c->stream);
+    cgs_launch_gather2(perm, P<uint32_t>(c->e_keys), P<uint32_t>(c->e_ids), gk, gv, d_n, cap,
+                       c->stream);
+    CgsRadixBuffers Bt;
+    Bt.keys = gk;
+    Bt.vals = gv;
+    Bt.keys_alt = P<uint32_t>(c->e_keys);
+    Bt.vals_alt = P<uint32_t>(c->e_ids);
+    Bt.hist = ctrl + L.histB;
+    Bt.status = ctrl + L.rB_status;
+    Bt.tickets = ctrl + L.rB_tickets;
+    Bt.cap = cap;
+    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, false, c->stream, &c->prof);
+    c->sorted_keys = altB ? Bt.keys_alt : Bt.keys;
+    c->sorted_ids = altB ? Bt.vals_alt : Bt.vals;
+    c->prof.mark("k_tile_ranges", c->stream);
+    cgs_launch_tile_ranges(c->sorted_keys, d_n, cap, 0, g.T, P<uint32_t>(c->ranges), c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+int launch_stage4(cgs_context* c, const FrameGeom& g, const cgs_compositor_uniforms* comp) {
+  TRY(ensure_words(c->out_rgba, (size_t)g.padded_row * g.H));
+  c->padded_row = g.padded_row;
+  CgsCompositeArgs A;
+  A.geom = P<float>(c->geom);
+  A.color = P<float>(c->color);
+  A.sorted_ids = c->sorted_ids ? c->sorted_ids : P<uint32_t>(c->e_ids);
+  A.ranges = P<uint32_t>(c->ranges);
+  A.width = g.W;
+  A.height = g.H;
+  A.tiles_x = g.tiles_x;
+  A.tiles_y = g.tiles_y;
+  A.padded_row = g.padded_row;
+  for (int i = 0; i < 4; ++i) A.bg[i] = comp->background_color[i];
+  A.out = P<uint32_t>(c->out_rgba);
+  c->prof.mark("k_composite", c->stream);
+  cgs_launch_composite(A, c->stream);
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+// read survivors / split count / entry total (one small copy + wait)
+int read_counters(cgs_context* c, uint32_t max_g, bool with_split) {
+  const FrameLayout L = frame_layout(max_g, c->entry_cap);
+  (void)L;
+  uint32_t* ctrl = P<uint32_t>(c->ctrl3);
+  HIPCHK(hipMemcpyAsync(c->h_counters, ctrl, 3 * 4, hipMemcpyDeviceToHost, c->stream));
+  if (with_split && c->ctrl1.p)
+    HIPCHK(hipMemcpyAsync(c->h_counters + 3, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
+  else
+    c->h_counters[3] = 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+  return ms;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+const char* cgs_last_error(void) { return g_err.c_str(); }
+
+int cgs_create(int device, cgs_context** out) {
+  if (!out) return fail(CGS_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(CGS_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(CGS_ERR_INVALID_ARG, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (prop.warpSize != 64)
+    return fail(CGS_ERR_UNSUPPORTED, std::string("wave64 device required, got ") + prop.gcnArchName);
+  cgs_context* c = new cgs_context();
+  c->device = device;
+  snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(CGS_ERR_RUNTIME, "stream creation failed");
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  for (auto& e : c->prof.ev) (void)hipEventCreate(&e);
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_counters), 16 * 4, hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return fail(CGS_ERR_RUNTIME, "pinned allocation failed");
+  }
+  memset(c->h_counters, 0, 64);
+  *out = c;
+  return CGS_OK;
+}
+
+void cgs_destroy(cgs_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->rest_pos, &c->rest_cov, &c->rest_cov_h, &c->rest_opa, &c->rest_sh,
+                    &c->rest_sh_h, &c->staging, &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w,
+                    &c->verts, &c->jac, &c->rot, &c->stretch, &c->def_pos, &c->def_cov,
+                    &c->def_opa, &c->split_meta, &c->ctrl1, &c->geom, &c->color, &c->extents,
+                    &c->e_keys, &c->e_ids, &c->e_depths, &c->e_alt_k, &c->e_alt_v, &c->e_perm,
+                    &c->ranges, &c->out_rgba, &c->cnt_dbg, &c->off_dbg, &c->ctrl3, &c->prim_ctrl,
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d};
+  for (DevBuf* b : bufs) release(*b);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->prof.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_counters) (void)hipHostFree(c->h_counters);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int cgs_device_name(cgs_context* c, char* buf, size_t len) {
+  if (!c || !buf || len == 0) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  snprintf(buf, len, "%s", c->name);
+  return CGS_OK;
+}
+
+int cgs_synchronize(cgs_context* c) {
+  TRY(ensure_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_invalidate(cgs_context* c) {
+  if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
+  c->scene_known = false;
+  c->topo_hash = 0;
+  c->last_verts.clear();
+  c->deformed_valid = false;
+  c->deform_generation++;
+  c->s3_valid = false;
+  return CGS_OK;
+}
+
+int cgs_invalidate_stage3(cgs_context* c) {
+  if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
+  c->s3_valid = false;
+  return CGS_OK;
+}
+
+int cgs_sync_scene(cgs_context* c, const float* pos, const float* cov, const float* opa,
+                   const float* sh, uint64_t count) {
+  TRY(ensure_ctx(c));
+  if (count > 0 && (!pos || !cov || !opa || !sh))
+    return fail(CGS_ERR_INVALID_ARG, "null scene array");
+  if (count >= (1ull << 28)) return fail(CGS_ERR_INVALID_ARG, "scene too large (>= 2^28 splats)");
+  const uint64_t h = scene_fingerprint(pos, cov, opa, sh, count, 6, 48);
+  if (c->scene_known && !c->f16 && c->scene_hash == h && c->n_rest == count) return CGS_OK;
+  const uint32_t n = (uint32_t)count;
+  TRY(upload_planar_f32(c, c->rest_pos, pos, n, 3, n));
+  TRY(upload_planar_f32(c, c->rest_cov, cov, n, 6, n));
+  TRY(upload_planar_f32(c, c->rest_opa, opa, n, 1, n));
+  TRY(upload_planar_f32(c, c->rest_sh, sh, n, 48, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->n_rest = n;
+  c->f16 = false;
+  c->scene_hash = h;
+  c->scene_known = true;
+  scene_changed(c);
+  return CGS_OK;
+}
+
+int cgs_sync_scene_f16(cgs_context* c, const float* pos, const uint32_t* cov_packed,
+                       const float* opa, const uint32_t* sh_packed, uint32_t sh_degree,
+                       uint64_t count) {
+  TRY(ensure_ctx(c));
+  if (count > 0 && (!pos || !cov_packed || !opa || !sh_packed))
+    return fail(CGS_ERR_INVALID_ARG, "null scene array");
+  if (count >= (1ull << 28)) return fail(CGS_ERR_INVALID_ARG, "scene too large (>= 2^28 splats)");
+  if (sh_degree > 3) sh_degree = 3;
+  const uint32_t bases = sh_degree == 0 ? 1 : sh_degree == 1 ? 4 : sh_degree == 2 ? 9 : 16;
+  const uint32_t sh_words = (bases * 3 + 1) / 2;
+  const uint64_t h = scene_fingerprint(pos, reinterpret_cast<const float*>(cov_packed), opa,
+                                       reinterpret_cast<const float*>(sh_packed), count, 3,
+                                       sh_words) ^ (0x9e3779b97f4a7c15ull * (sh_degree + 1));
+  if (c->scene_known && c->f16 && c->scene_hash == h && c->n_rest == count) return CGS_OK;
+  const uint32_t n = (uint32_t)count;
+  TRY(upload_planar_f32(c, c->rest_pos, pos, n, 3, n));
+  TRY(upload_planar_f32(c, c->rest_opa, opa, n, 1, n));
+  // covariances: 6 halves per splat -> 6 half planes
+  TRY(ensure_words(c->rest_cov_h, ((size_t)6 * n + 1) / 2));
+  TRY(ensure_words(c->rest_sh_h, ((size_t)48 * n + 1) / 2));
+  if (n) {
+    TRY(ensure_words(c->staging, (size_t)n * (sh_words > 3 ? sh_words : 3)));
+    HIPCHK(hipMemcpyAsync(c->staging.p, cov_packed, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    cgs_launch_deinterleave_u16(P<uint16_t>(c->staging), P<uint16_t>(c->rest_cov_h), n, 6, n,
+                                c->stream);
+    HIPCHK(hipMemcpyAsync(c->staging.p, sh_packed, (size_t)n * sh_words * 4, hipMemcpyHostToDevice,
+                          c->stream));
+    cgs_launch_sh_f16_planes(P<uint16_t>(c->staging), P<uint16_t>(c->rest_sh_h), n, sh_words * 2,
+                             bases - 1, c->stream);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->n_rest = n;
+  c->f16 = true;
+  c->f16_degree = sh_degree;
+  c->scene_hash = h;
+  c->scene_known = true;
+  scene_changed(c);
+  return CGS_OK;
+}
+
+int cgs_sync_cages(cgs_context* c, const uint32_t* elems, const float* inv_dm, uint32_t num_tets,
+                   const float* verts, uint32_t num_verts, const uint32_t* bind_tet,
+                   const float* bind_w, uint64_t count) {
+  TRY(ensure_ctx(c));
+  if (num_tets == 0) {
+    // no active cages (gpu_state.rs:215-229)
+    const bool was = c->has_cages;
+    c->has_cages = false;
+    c->num_tets = 0;
+    c->num_verts = 0;
+    c->topo_hash = 0;
+    c->last_verts.clear();
+    c->deformed_valid = !was;
+    if (was) c->deform_generation++;
+    return CGS_OK;
+  }
+  if (!elems || !inv_dm || !verts || (count && (!bind_tet || !bind_w)))
+    return fail(CGS_ERR_INVALID_ARG, "null cage array");
+  if (count != c->n_rest) return fail(CGS_ERR_INVALID_ARG, "binding count != scene count");
+  for (uint64_t i = 0; i < (uint64_t)num_tets * 4; ++i)
+    if (elems[i] >= num_verts) return fail(CGS_ERR_INVALID_ARG, "tet element index out of range");
+  uint64_t th = FNV_OFFSET;
+  th = fnv_u64(th, num_tets);
+  th = fnv_u64(th, num_verts);
+  th = fnv_u64(th, (uint64_t)(uintptr_t)elems);
+  th = fnv_u64(th, (uint64_t)(uintptr_t)inv_dm);
+  th = fnv_u64(th, (uint64_t)(uintptr_t)bind_tet);
+  th = fnv_u64(th, count);
+  th = hash_sample(th, elems, (size_t)num_tets * 4);
+  th = hash_sample(th, reinterpret_cast<const uint32_t*>(inv_dm), (size_t)num_tets * 9);
+  th = hash_sample(th, bind_tet, count);
+  th = hash_sample(th, reinterpret_cast<const uint32_t*>(bind_w), count * 4);
+  if (th == 0) th = 1;
+  if (!c->has_cages || c->topo_hash != th) {
+    TRY(ensure_words(c->elems, (size_t)num_tets * 4));
+    TRY(ensure_words(c->inv_dm, (size_t)num_tets * 9));
+    TRY(ensure_words(c->bind_tet, count));
+    TRY(ensure_words(c->bind_w, count * 4));
+    HIPCHK(hipMemcpyAsync(c->elems.p, elems, (size_t)num_tets * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->inv_dm.p, inv_dm, (size_t)num_tets * 36, hipMemcpyHostToDevice, c->stream));
+    if (count) {
+      HIPCHK(hipMemcpyAsync(c->bind_tet.p, bind_tet, count * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->bind_w.p, bind_w, count * 16, hipMemcpyHostToDevice, c->stream));
+    }
+    c->num_tets = num_tets;
+    c->has_cages = true;
+    c->topo_hash = th;
+    c->last_verts.clear();
+    c->deformed_valid = false;
+  }
+  const size_t nv = (size_t)num_verts * 3;
+  const bool changed = c->num_verts != num_verts || c->last_verts.size() != nv ||
+                       memcmp(c->last_verts.data(), verts, nv * 4) != 0;
+  if (changed) {
+    TRY(ensure_words(c->verts, nv));
+    HIPCHK(hipMemcpyAsync(c->verts.p, verts, nv * 4, hipMemcpyHostToDevice, c->stream));
+    c->last_verts.assign(verts, verts + nv);
+    c->num_verts = num_verts;
+    c->deformed_valid = false;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_set_capture(cgs_context* c, uint32_t enable) {
+  if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
+  c->capture = enable != 0;
+  return CGS_OK;
+}
+
+int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
+                      const cgs_compositor_uniforms* comp, const cgs_frame_params* prm,
+                      cgs_stage_timings* tim) {
+  TRY(ensure_ctx(c));
+  if (!cam || !comp || !prm) return fail(CGS_ERR_INVALID_ARG, "null frame argument");
+  cgs_stage_timings T;
+  memset(&T, 0, sizeof(T));
+  const FrameGeom g = frame_geom(comp);
+  if (g.W != c->last_w || g.H != c->last_h) {
+    c->s3_valid = false;
+    c->last_w = g.W;
+    c->last_h = g.H;
+  }
+  const bool prof = prm->profile_stages != 0;
+  hipStream_t s = c->stream;
+  c->prof.on = prm->profile_stages >= 2;
+  c->prof.n = 0;
+  c->prof_count = 0;
+
+  if (c->n_rest == 0) {
+    // background frame (gpu_state.rs:658-704)
+    TRY(ensure_words(c->ranges, (size_t)g.T * 2));
+    cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, s);
+    TRY(ensure_words(c->geom, 8));
+    TRY(ensure_words(c->color, 4));
+    TRY(ensure_words(c->e_ids, 1));
+    c->sorted_ids = P<uint32_t>(c->e_ids);
+    if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
+    TRY(launch_stage4(c, g, comp));
+    if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (prof) T.stage4_composite_ms = elapsed(c->ev[3], c->ev[4]);
+    T.stage1_skipped = 1;
+    c->max_g = 0;
+    c->active = 0;
+    c->n_entries = 0;
+    c->s3_valid = false;
+    if (tim) *tim = T;
+    return CGS_OK;
+  }
+
+  const uint32_t N = c->n_rest;
+  const bool split_possible = c->has_cages && prm->strain_split_threshold > 0.0f;
+  const uint32_t max_g = split_possible ? 2 * N : N;
+  c->max_g = max_g;
+  if (prof) HIPCHK(hipEventRecord(c->ev[0], s));
+
+  // ---- Stage 1 ----
+  CgsSceneDev S;
+  memset(&S, 0, sizeof(S));
+  const uint32_t* split_count = nullptr;
+  if (c->has_cages) {
+    const uint32_t p0 = fbits(prm->strain_split_threshold), p1 = fbits(prm->mip_filter_3d_var);
+    if (p0 != c->last_params[0] || p1 != c->last_params[1]) {
+      c->last_params[0] = p0;
+      c->last_params[1] = p1;
+      c->deformed_valid = false;
+    }
+    TRY(ensure_words(c->jac, (size_t)c->num_tets * 9));
+    TRY(ensure_words(c->rot, (size_t)c->num_tets * 9));
+    TRY(ensure_words(c->stretch, c->num_tets));
+    if (c->def_stride < max_g) {
+      release(c->def_pos);
+      release(c->def_cov);
+      release(c->def_opa);
+      c->def_stride = max_g;
+      c->deformed_valid = false;
+    }
+    TRY(ensure_words(c->def_pos, (size_t)3 * c->def_stride));
+    TRY(ensure_words(c->def_cov, (size_t)6 * c->def_stride));
+    TRY(ensure_words(c->def_opa, c->def_stride));
+    TRY(ensure_words(c->split_meta, (size_t)2 * N));
+    const uint32_t dparts = cgs_deform_num_parts(N);
+    TRY(ensure_words(c->ctrl1, 4 + 2 * (size_t)(dparts ? dparts : 1)));
+    if (c->f16) return fail(CGS_ERR_UNSUPPORTED, "cage deformation of the f16 storage variant");
+    if (!c->deformed_valid) {
+      cgs_launch_fill_u32(P<uint32_t>(c->ctrl1), 0u, 4 + 2 * (size_t)(dparts ? dparts : 1), s);
+      CgsTetArgs TA;
+      TA.verts = P<float>(c->verts);
+      TA.elems = P<uint32_t>(c->elems);
+      TA.inv_dm = P<float>(c->inv_dm);
+      TA.jac = P<float>(c->jac);
+      TA.rot = P<float>(c->rot);
+      TA.stretch = P<float>(c->stretch);
+      TA.num_tets = c->num_tets;
+      c->prof.mark("k_tet_transforms", s);
+      cgs_launch_tet_transforms(TA, s);
+      CgsDeformArgs D;
+      D.verts = P<float>(c->verts);
+      D.elems = P<uint32_t>(c->elems);
+      D.jac = P<float>(c->jac);
+      D.stretch = P<float>(c->stretch);
+      D.bind_tet = P<uint32_t>(c->bind_tet);
+      D.bind_w = P<float>(c->bind_w);
+      D.rest_pos = P<float>(c->rest_pos);
+      D.rest_cov = P<float>(c->rest_cov);
+      D.rest_opa = P<float>(c->rest_opa);
+      D.rest_stride = N;
+      D.out_pos = P<float>(c->def_pos);
+      D.out_cov = P<float>(c->def_cov);
+      D.out_opa = P<float>(c->def_opa);
+      D.out_stride = c->def_stride;
+      D.split_meta = P<uint32_t>(c->split_meta);
+      D.split_count = P<uint32_t>(c->ctrl1);
+      D.status = reinterpret_cast<uint64_t*>(P<uint32_t>(c->ctrl1) + 4);
+      D.ticket = P<uint32_t>(c->ctrl1) + 1;
+      D.split_threshold = split_possible ? prm->strain_split_threshold : 0.0f;
+      D.mip = prm->mip_filter_3d_var;
+      D.num_gaussians = N;
+      D.num_tets = c->num_tets;
+      c->prof.mark("k_deform", s);
+      cgs_launch_deform(D, s);
+      HIPCHK(hipGetLastError());
+      c->deformed_valid = true;
+      c->deform_generation++;
+      T.stage1_skipped = 0;
+    } else {
+      T.stage1_skipped = 1;
+    }
+    S.pos = P<float>(c->def_pos);
+    S.cov = P<float>(c->def_cov);
+    S.opa = P<float>(c->def_opa);
+    S.cap = c->def_stride;
+    split_count = P<uint32_t>(c->ctrl1);
+  } else {
+    T.stage1_skipped = 1;
+    S.pos = P<float>(c->rest_pos);
+    if (c->f16)
+      S.cov_h = P<uint16_t>(c->rest_cov_h);
+    else
+      S.cov = P<float>(c->rest_cov);
+    S.opa = P<float>(c->rest_opa);
+    S.cap = N;
+  }
+  if (c->f16)
+    S.sh_h = P<uint16_t>(c->rest_sh_h);
+  else
+    S.sh = P<float>(c->rest_sh);
+  S.sh_stride = N;
+  if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
+
+  // ---- Stage 3 cache key (gpu_state.rs:956-981) ----
+  Stage3Key key;
+  memset(&key, 0, sizeof(key));
+  key.deform_generation = c->deform_generation;
+  for (int i = 0; i < 16; ++i) key.view_bits[i] = fbits(cam->view_matrix[i]);
+  key.fx = fbits(cam->focal_x);
+  key.fy = fbits(cam->focal_y);
+  key.cx = fbits(cam->principal_x);
+  key.cy = fbits(cam->principal_y);
+  key.width = g.W;
+  key.height = g.H;
+  key.near_bits = fbits(cam->near_plane);
+  key.mip_bits = fbits(prm->mip_filter_3d_var);
+  key.min_op_bits = fbits(prm->min_opacity);
+  key.min_rad_bits = fbits(prm->min_radius_px);
+  const bool use_cache = !prm->force_stage3 && c->s3_valid && c->s3_key == key;
+
+  // ---- Stage 2 ----
+  TRY(ensure_words(c->geom, (size_t)8 * max_g));
+  TRY(ensure_words(c->color, (size_t)4 * max_g));
+  TRY(ensure_words(c->extents, max_g));
+  init_entry_cap(c, max_g);
+  const FrameLayout L = frame_layout(max_g, c->entry_cap);
+  TRY(ensure_words(c->ctrl3, L.words));
+  cgs_launch_fill_u32(P<uint32_t>(c->ctrl3), 0u, 1, s);  // survivors
+  CgsProjectArgs PA;
+  memset(&PA, 0, sizeof(PA));
+  PA.scene = S;
+  PA.bind_tet = c->has_cages ? P<uint32_t>(c->bind_tet) : nullptr;
+  PA.tet_rot = c->has_cages ? P<float>(c->rot) : nullptr;
+  PA.split_meta = c->has_cages ? P<uint32_t>(c->split_meta) : nullptr;
+  PA.split_count = split_count;
+  memcpy(PA.camera, cam, 96);
+  for (int i = 0; i < 3; ++i) PA.cam_pos[i] = comp->camera_position[i];
+  PA.mip = c->has_cages ? 0.0f : prm->mip_filter_3d_var;  // gpu_state.rs:894
+  PA.min_opacity = prm->min_opacity;
+  PA.min_radius = prm->min_radius_px;
+  PA.sh_degree = comp->sh_degree > 3 ? 3 : comp->sh_degree;
+  if (c->f16 && PA.sh_degree != c->f16_degree)
+    return fail(CGS_ERR_INVALID_ARG, "sh_degree must equal the quantized scene's degree");
+  PA.has_deformation = c->has_cages ? 1u : 0u;
+  PA.num_tets = c->num_tets;
+  PA.num_rest = N;
+  PA.max_gaussians = max_g;
+  PA.geom = P<float>(c->geom);
+  PA.color = P<float>(c->color);
+  PA.extents = P<uint32_t>(c->extents);
+  PA.survivors = P<uint32_t>(c->ctrl3);
+  c->prof.mark("k_project", s);
+  cgs_launch_project(PA, s);
+  HIPCHK(hipGetLastError());
+  if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
+
+  // ---- Stage 3 ----
+  if (!use_cache) TRY(launch_stage3(c, g, cam->near_plane, N, split_count, max_g, true));
+  if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
+  // ---- Stage 4 ----
+  TRY(launch_stage4(c, g, comp));
+  if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
+
+  TRY(read_counters(c, max_g, c->has_cages));
+  uint32_t survivors = c->h_counters[0];
+  const uint32_t splits = c->h_counters[3];
+  if (!use_cache) {
+    uint32_t total = c->h_counters[2];
+    // the tile-entry arena overflowed: grow and re-run Stages 3-4
+    int reruns = 0;
+    while (total > c->entry_cap && reruns < 4) {
+      uint64_t want = (uint64_t)total + total / 2;
+      if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
+      c->entry_cap = (uint32_t)want;
+      TRY(launch_stage3(c, g, cam->near_plane, N, split_count, max_g, false));
+      if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
+      TRY(launch_stage4(c, g, comp));
+      if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
+      TRY(read_counters(c, max_g, c->has_cages));
+      total = c->h_counters[2];
+      reruns++;
+    }
+    c->n_entries = total;
+    c->s3_key = key;
+    c->s3_valid = true;
+    c->s3_survivors = survivors;
+    T.stage3_reruns = (uint32_t)reruns;
+  } else {
+    T.stage3_cached = 1;
+  }
+  c->active = N + (split_possible ? splits : 0u);
+  if (c->prof.on) {
+    c->prof.mark("end", s);
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i + 1 < c->prof.n; ++i) {
+      c->prof_names[i] = c->prof.names[i];
+      c->prof_ms[i] = elapsed(c->prof.ev[i], c->prof.ev[i + 1]);
+    }
+    c->prof_count = c->prof.n > 0 ? c->prof.n - 1 : 0;
+    c->prof.on = false;
+  }
+  if (prof) {
+    T.stage1_deform_ms = elapsed(c->ev[0], c->ev[1]);
+    T.stage2_project_ms = elapsed(c->ev[1], c->ev[2]);
+    T.stage3_bin_sort_ms = use_cache ? 0.0 : elapsed(c->ev[2], c->ev[3]);
+    T.stage4_composite_ms = elapsed(c->ev[3], c->ev[4]);
+    T.used_hardware_timestamps = 1;
+  }
+  T.active_gaussians = c->active;
+  T.surviving_gaussians = survivors;
+  T.total_tile_entries = c->n_entries;
+  if (tim) *tim = T;
+  return CGS_OK;
+}
+
+int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint32_t* count) {
+  if (!c || !count) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  uint32_t n = (uint32_t)c->prof_count;
+  if (n > cap) n = cap;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (names) snprintf(names + 32 * i, 32, "%s", c->prof_names[i]);
+    if (ms) ms[i] = c->prof_ms[i];
+  }
+  *count = n;
+  return CGS_OK;
+}
+
+int cgs_readback_rgba8(cgs_context* c, uint8_t* dst, uint32_t width, uint32_t height) {
+  TRY(ensure_ctx(c));
+  if (!dst) return fail(CGS_ERR_INVALID_ARG, "null destination");
+  if (!c->out_rgba.p || width != c->last_w || height != c->last_h)
+    return fail(CGS_ERR_SHORT_READBACK, "framebuffer size differs from the last frame");
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)width * 4, c->out_rgba.p, (size_t)c->padded_row * 4,
+                          (size_t)width * 4, height, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_framebuffer_device_ptr(cgs_context* c, void** ptr, uint32_t* padded_row_u32) {
+  if (!c || !ptr || !padded_row_u32) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  *ptr = c->out_rgba.p;
+  *padded_row_u32 = c->padded_row;
+  return CGS_OK;
+}
+
+int cgs_read_projected(cgs_context* c, float* attrs10, uint32_t* extents, uint64_t n) {
+  TRY(ensure_ctx(c));
+  if (n > c->max_g) return fail(CGS_ERR_SHORT_READBACK, "more splats requested than projected");
+  std::vector<float> geom((size_t)n * 8), color((size_t)n * 4);
+  std::vector<uint32_t> ext(n);
+  if (n) {
+    HIPCHK(hipMemcpyAsync(geom.data(), c->geom.p, n * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(color.data(), c->color.p, n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ext.data(), c->extents.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint64_t i = 0; i < n; ++i) {
+    if (extents) extents[i] = ext[i];
+    if (!attrs10) continue;
+    float* a = attrs10 + i * 10;
+    if (ext[i] == 0) {
+      for (int k = 0; k < 10; ++k) a[k] = 0.0f;
+      continue;
+    }
+    const float* gg = &geom[i * 8];
+    const float* cc = &color[i * 4];
+    a[0] = gg[0]; a[1] = gg[1]; a[2] = gg[2]; a[3] = gg[3]; a[4] = gg[4]; a[5] = gg[5];
+    a[6] = cc[0]; a[7] = cc[1]; a[8] = cc[2]; a[9] = gg[6];
+  }
+  return CGS_OK;
+}
+
+int cgs_read_stage3(cgs_context* c, uint32_t* counts, uint32_t* offsets, uint32_t* sorted_keys,
+                    uint32_t* sorted_ids, uint32_t* ranges, uint64_t* n_entries) {
+  TRY(ensure_ctx(c));
+  if ((counts || offsets) && !c->captured)
+    return fail(CGS_ERR_INVALID_ARG, "counts/offsets need cgs_set_capture(ctx, 1) before the frame");
+  const uint32_t E = c->n_entries;
+  const size_t T = (size_t)c->tiles_x * c->tiles_y;
+  if (counts && c->max_g)
+    HIPCHK(hipMemcpyAsync(counts, c->cnt_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+  if (offsets && c->max_g)
+    HIPCHK(hipMemcpyAsync(offsets, c->off_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+  if (sorted_keys && E)
+    HIPCHK(hipMemcpyAsync(sorted_keys, c->sorted_keys, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
+  if (sorted_ids && E)
+    HIPCHK(hipMemcpyAsync(sorted_ids, c->sorted_ids, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
+  if (ranges && T)
+    HIPCHK(hipMemcpyAsync(ranges, c->ranges.p, T * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (n_entries) *n_entries = E;
+  return CGS_OK;
+}
+
+int cgs_read_deformed(cgs_context* c, float* pos, float* cov, float* opa, uint32_t* split_meta,
+                      uint32_t* split_count) {
+  TRY(ensure_ctx(c));
+  if (!c->has_cages || !c->def_pos.p) return fail(CGS_ERR_INVALID_ARG, "no deformation has run");
+  uint32_t splits = 0;
+  HIPCHK(hipMemcpyAsync(&splits, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t m = (size_t)c->n_rest + splits;
+  const size_t S = c->def_stride;
+  if (m > S) return fail(CGS_ERR_EXECUTION, "split count exceeds the deformed arena");
+  std::vector<float> p(3 * S), cv(6 * S);
+  HIPCHK(hipMemcpyAsync(p.data(), c->def_pos.p, 3 * S * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(cv.data(), c->def_cov.p, 6 * S * 4, hipMemcpyDeviceToHost, c->stream));
+  if (opa) HIPCHK(hipMemcpyAsync(opa, c->def_opa.p, m * 4, hipMemcpyDeviceToHost, c->stream));
+  if (split_meta && splits)
+    HIPCHK(hipMemcpyAsync(split_meta, c->split_meta.p, (size_t)splits * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < m; ++i) {
+    if (pos)
+      for (int k = 0; k < 3; ++k) pos[i * 3 + k] = p[k * S + i];
+    if (cov)
+      for (int k = 0; k < 6; ++k) cov[i * 6 + k] = cv[k * S + i];
+  }
+  if (split_count) *split_count = splits;
+  return CGS_OK;
+}
+
+int cgs_read_tet_transforms(cgs_context* c, float* jac, float* rot, float* stretch) {
+  TRY(ensure_ctx(c));
+  if (!c->has_cages || !c->jac.p) return fail(CGS_ERR_INVALID_ARG, "no deformation has run");
+  const size_t t = c->num_tets;
+  if (jac) HIPCHK(hipMemcpyAsync(jac, c->jac.p, t * 36, hipMemcpyDeviceToHost, c->stream));
+  if (rot) HIPCHK(hipMemcpyAsync(rot, c->rot.p, t * 36, hipMemcpyDeviceToHost, c->stream));
+  if (stretch) HIPCHK(hipMemcpyAsync(stretch, c->stretch.p, t * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_load_projected(cgs_context* c, const float* attrs10, const uint32_t* extents, uint64_t n) {
+  TRY(ensure_ctx(c));
+  if (n && (!attrs10 || !extents)) return fail(CGS_ERR_INVALID_ARG, "null arrays");
+  if (n >= (1ull << 28)) return fail(CGS_ERR_INVALID_ARG, "too many splats");
+  std::vector<float> geom((size_t)n * 8), color((size_t)n * 4);
+  for (uint64_t i = 0; i < n; ++i) {
+    const float* a = attrs10 + i * 10;
+    float* gg = &geom[i * 8];
+    gg[0] = a[0]; gg[1] = a[1]; gg[2] = a[2]; gg[3] = a[3]; gg[4] = a[4]; gg[5] = a[5];
+    gg[6] = a[9];
+    gg[7] = extents[i] ? cgs_fclamp(2.0f * cgs_logf(255.0f * a[5]), 0.0f, 8.0f) : 0.0f;
+    color[i * 4] = a[6]; color[i * 4 + 1] = a[7]; color[i * 4 + 2] = a[8]; color[i * 4 + 3] = 0.0f;
+  }
+  TRY(ensure_words(c->geom, (size_t)8 * n));
+  TRY(ensure_words(c->color, (size_t)4 * n));
+  TRY(ensure_words(c->extents, n));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(c->geom.p, geom.data(), n * 32, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->color.p, color.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->extents.p, extents, n * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->max_g = (uint32_t)n;
+  c->active = (uint32_t)n;
+  c->s3_valid = false;
+  return CGS_OK;
+}
+
+int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
+                   const cgs_compositor_uniforms* comp, uint64_t* n_entries) {
+  TRY(ensure_ctx(c));
+  if (!cam || !comp) return fail(CGS_ERR_INVALID_ARG, "null argument");
+  const FrameGeom g = frame_geom(comp);
+  c->last_w = g.W;
+  c->last_h = g.H;
+  for (int attempt = 0; attempt < 5; ++attempt) {
+    TRY(launch_stage3(c, g, cam->near_plane, c->active, nullptr, c->max_g, false));
+    TRY(read_counters(c, c->max_g, false));
+    const uint32_t total = c->h_counters[2];
+    c->n_entries = total;
+    if (total <= c->entry_cap) break;
+    uint64_t want = (uint64_t)total + total / 2;
+    if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
+    c->entry_cap = (uint32_t)want;
+  }
+  if (n_entries) *n_entries = c->n_entries;
+  return CGS_OK;
+}
+
+int cgs_run_stage4(cgs_context* c, const cgs_compositor_uniforms* comp) {
+  TRY(ensure_ctx(c));
+  if (!comp) return fail(CGS_ERR_INVALID_ARG, "null argument");
+  const FrameGeom g = frame_geom(comp);
+  if (g.tiles_x != c->tiles_x || g.tiles_y != c->tiles_y)
+    return fail(CGS_ERR_INVALID_ARG, "compositor size differs from Stage 3");
+  TRY(launch_stage4(c, g, comp));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_malloc(cgs_context* c, uint64_t bytes, void** dptr) {
+  TRY(ensure_ctx(c));
+  if (!dptr) return fail(CGS_ERR_INVALID_ARG, "null out pointer");
+  HIPCHK(hipMalloc(dptr, bytes ? bytes : 4));
+  return CGS_OK;
+}
+
+int cgs_free(cgs_context* c, void* dptr) {
+  TRY(ensure_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dptr) HIPCHK(hipFree(dptr));
+  return CGS_OK;
+}
+
+int cgs_memcpy_h2d(cgs_context* c, void* dst, const void* src, uint64_t bytes) {
+  TRY(ensure_ctx(c));
+  if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_memcpy_d2h(cgs_context* c, void* dst, const void* src, uint64_t bytes) {
+  TRY(ensure_ctx(c));
+  if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_exclusive_scan_u32(cgs_context* c, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                           uint32_t* total) {
+  TRY(ensure_ctx(c));
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "n too large");
+  if (n && (!d_in || !d_out)) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  const uint32_t parts = cgs_scan_num_parts((uint32_t)n);
+  const size_t words = 4 + 2 * (size_t)(parts ? parts : 1);
+  TRY(ensure_words(c->prim_ctrl, words));
+  uint32_t* ctrl = P<uint32_t>(c->prim_ctrl);
+  cgs_launch_fill_u32(ctrl, 0u, words, c->stream);
+  cgs_launch_scan(d_in, d_out, (uint32_t)n, ctrl, reinterpret_cast<uint64_t*>(ctrl + 4), ctrl + 1,
+                  c->stream);
+  HIPCHK(hipGetLastError());
+  uint32_t tot = 0;
+  HIPCHK(hipMemcpyAsync(c->h_counters + 8, ctrl, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  tot = c->h_counters[8];
+  if (total) *total = tot;
+  return CGS_OK;
+}
+
+static int radix_prim(cgs_context* c, uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t bits,
+                      bool iota, int* in_alt) {
+  TRY(ensure_words(c->prim_alt_k, n));
+  TRY(ensure_words(c->prim_alt_v, n));
+  const size_t words = cgs_radix_control_words(n);
+  TRY(ensure_words(c->prim_ctrl, words));
+  uint32_t* ctrl = P<uint32_t>(c->prim_ctrl);
+  cgs_launch_fill_u32(ctrl, 0u, words, c->stream);
+  CgsRadixBuffers R;
+  R.keys = keys;
+  R.vals = vals;
+  R.keys_alt = P<uint32_t>(c->prim_alt_k);
+  R.vals_alt = P<uint32_t>(c->prim_alt_v);
+  R.hist = ctrl;
+  R.status = ctrl + 4 * 256;
+  R.tickets = ctrl + words - 4;
+  R.cap = n;
+  *in_alt = cgs_launch_radix_sort(R, nullptr, n, bits, iota, false, c->stream);
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+int cgs_radix_sort_u32_pairs(cgs_context* c, uint32_t* keys, uint32_t* vals, uint64_t n,
+                             uint32_t sort_bits) {
+  TRY(ensure_ctx(c));
+  if (n >= (1ull << 30)) return fail(CGS_ERR_INVALID_ARG, "n too large (>= 2^30)");
+  if (n == 0 || sort_bits == 0) return CGS_OK;
+  if (!keys || !vals) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  int alt = 0;
+  TRY(radix_prim(c, keys, vals, (uint32_t)n, sort_bits, false, &alt));
+  if (alt) {
+    HIPCHK(hipMemcpyAsync(keys, c->prim_alt_k.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(vals, c->prim_alt_v.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+int cgs_radix_sort_u64_pairs(cgs_context* c, uint32_t* tiles, uint32_t* depths, uint32_t* ids,
+                             uint64_t n, uint32_t tile_bits) {
+  TRY(ensure_ctx(c));
+  if (n >= (1ull << 30)) return fail(CGS_ERR_INVALID_ARG, "n too large (>= 2^30)");
+  if (n == 0) return CGS_OK;
+  if (!tiles || !depths || !ids) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  const uint32_t m = (uint32_t)n;
+  // 1) stable sort by depth with the entry index as payload (4 passes: result in place)
+  TRY(ensure_words(c->prim_alt_d, (size_t)m * 3));
+  uint32_t* perm = P<uint32_t>(c->prim_alt_d);
+  uint32_t* t2 = perm + m;
+  uint32_t* i2 = perm + 2 * (size_t)m;
+  int alt = 0;
+  TRY(radix_prim(c, depths, perm, m, 32, true, &alt));
+  if (alt) {
+    HIPCHK(hipMemcpyAsync(depths, c->prim_alt_k.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(perm, c->prim_alt_v.p, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // 2) gather tiles and ids through the permutation
+  TRY(ensure_words(c->staging, 1));
+  uint32_t* d_n = P<uint32_t>(c->staging);
+  cgs_launch_fill_u32(d_n, m, 1, c->stream);
+  cgs_launch_gather2(perm, tiles, ids, t2, i2, d_n, m, c->stream);
+  // 3) stable sort by tile id
+  TRY(radix_prim(c, t2, i2, m, tile_bits ? tile_bits : 1, false, &alt));
+  const uint32_t* st = alt ? P<uint32_t>(c->prim_alt_k) : t2;
+  const uint32_t* si = alt ? P<uint32_t>(c->prim_alt_v) : i2;
+  // depths follow the final order: depth_sorted[perm2]; recompute through ids
+  HIPCHK(hipMemcpyAsync(tiles, st, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(ids, si, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+/* cindergs.h — C ABI of the MI355X (gfx950) CinderGS render / cage-deformation
+ * hot path.  Plain pointers and sizes only; every entry point returns a
+ * cgs_status (0 = OK) and leaves a message for cgs_last_error().
+ *
+ * Each entry point names the reference interface (Rust, CinderGS 2.0.0) it
+ * replaces.  Paths are relative to the reference repository root.
+ *
+ * Threading: a cgs_context is bound to one device and one HIP stream and is not
+ * thread-safe (the reference's GpuPipelineState methods all take &mut self).
+ * Calls are asynchronous on the context stream except the readbacks,
+ * cgs_execute_frame (which waits for its counters) and cgs_synchronize.
+ */
+#ifndef CINDERGS_H
+#define CINDERGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* GpuError {Runtime, NoAdapter, UnsupportedTarget, Execution}
+ * (src/render/gpu.rs:14-30) and GpuBridgeError {Readback, ShortReadback}
+ * (src/render/gpu_bridge.rs:28-48) as status codes. */
+typedef enum cgs_status {
+  CGS_OK = 0,
+  CGS_ERR_RUNTIME = 1,
+  CGS_ERR_NO_DEVICE = 2,
+  CGS_ERR_UNSUPPORTED = 3,
+  CGS_ERR_EXECUTION = 4,
+  CGS_ERR_SHORT_READBACK = 5,
+  CGS_ERR_INVALID_ARG = 6
+} cgs_status;
+
+typedef struct cgs_context cgs_context;
+
+/* CameraUniforms, 96 bytes (src/render/camera.rs:9-36). view_matrix is the
+ * column-major world-to-camera matrix. */
+typedef struct cgs_camera_uniforms {
+  float view_matrix[16];
+  float focal_x, focal_y;
+  float principal_x, principal_y;
+  float viewport_width, viewport_height;
+  float near_plane;
+  float pad;
+} cgs_camera_uniforms;
+
+/* CompositorUniforms, 48 bytes (src/render/rasterizer/mod.rs:16-37). */
+typedef struct cgs_compositor_uniforms {
+  uint32_t screen_width, screen_height;
+  uint32_t tiles_x, tiles_y;
+  float background_color[4];
+  float camera_position[3];
+  uint32_t sh_degree;
+} cgs_compositor_uniforms;
+
+/* The scalar arguments of GpuPipelineState::execute_frame_gpu
+ * (src/render/gpu_state.rs:613-626). */
+typedef struct cgs_frame_params {
+  float strain_split_threshold;
+  float mip_filter_3d_var;
+  float min_opacity;
+  float min_radius_px;
+  uint32_t profile_stages;
+  uint32_t force_stage3;
+} cgs_frame_params;
+
+/* GpuStageTimings (src/render/gpu_state.rs:513-534) plus the number of Stage 3
+ * re-runs caused by growing the tile-entry arena. */
+typedef struct cgs_stage_timings {
+  double stage1_deform_ms;
+  double stage2_project_ms;
+  double stage3_bin_sort_ms;
+  double stage4_composite_ms;
+  uint32_t stage1_skipped;
+  uint32_t stage3_cached;
+  uint32_t used_hardware_timestamps;
+  uint32_t stage3_reruns;
+  uint64_t active_gaussians;
+  uint64_t surviving_gaussians;
+  uint64_t total_tile_entries;
+} cgs_stage_timings;
+
+/* ---- lifetime (GpuCompute::new + GpuPipelineState::new, gpu.rs:63-100,
+ *      gpu_state.rs:553-562) ---- */
+int cgs_create(int device, cgs_context** out);
+void cgs_destroy(cgs_context* ctx);
+const char* cgs_last_error(void);
+int cgs_device_name(cgs_context* ctx, char* buf, size_t len);
+int cgs_synchronize(cgs_context* ctx);
+
+/* GpuPipelineState::invalidate / invalidate_stage3 (gpu_state.rs:565-573) */
+int cgs_invalidate(cgs_context* ctx);
+int cgs_invalidate_stage3(cgs_context* ctx);
+
+/* GpuSceneState::sync_scene_and_cages, scene part (gpu_state.rs:182-213):
+ * GaussianSceneSoa (src/scene/soa.rs:30-45) — positions[3n] xyz, covariances
+ * [6n] (xx,xy,xz,yy,yz,zz), opacities[n] (activated), sh[48n] per splat
+ * [dc r,g,b, rest_r 15, rest_g 15, rest_b 15].  Uploads only when the
+ * fingerprint (count, pointers, 64-sample FNV-1a, gpu_state.rs:65-79) changes. */
+int cgs_sync_scene(cgs_context* ctx, const float* positions, const float* covariances,
+                   const float* opacities, const float* sh_coeffs, uint64_t count);
+
+/* QuantizedGaussianSceneSoa (src/scene/soa.rs:429-500): covariances as 3 u32
+ * (f16 pairs, low half first, pack_f16_pair soa.rs:348-352), SH as
+ * ((bases*3+1)/2) u32 words per splat, channel-major and degree-trimmed.
+ * Rendered by the f16 projection (project.rs:1344-1679). */
+int cgs_sync_scene_f16(cgs_context* ctx, const float* positions, const uint32_t* cov_packed,
+                       const float* opacities, const uint32_t* sh_packed, uint32_t sh_degree,
+                       uint64_t count);
+
+/* GpuSceneState::sync_scene_and_cages, cage part (gpu_state.rs:215-343), with
+ * the CageScene already flattened by the caller: tet_elements[4*num_tets]
+ * (global vertex indices), inv_dm[9*num_tets] (TetPrecomputed, col-major),
+ * vertices[3*num_vertices], bind_tet[count] (global tet index, 0xFFFFFFFF =
+ * unbound), bind_weights[4*count].  num_tets == 0 detaches the cages.
+ * Topology is re-uploaded when its fingerprint changes; vertices when they
+ * differ from the last call (marks Stage 1 dirty). */
+int cgs_sync_cages(cgs_context* ctx, const uint32_t* tet_elements, const float* inv_dm,
+                   uint32_t num_tets, const float* vertices, uint32_t num_vertices,
+                   const uint32_t* bind_tet, const float* bind_weights, uint64_t count);
+
+/* GpuPipelineState::execute_frame_gpu (gpu_state.rs:613-1243): Stages 1-4 into
+ * the device framebuffer (RGBA8 words, rows padded to ceil(W/64)*64). */
+int cgs_execute_frame(cgs_context* ctx, const cgs_camera_uniforms* camera,
+                      const cgs_compositor_uniforms* compositor,
+                      const cgs_frame_params* params, cgs_stage_timings* timings);
+
+/* readback_framebuffer_rgba8 (gpu_state.rs:1296-1328): tightly packed
+ * width*height*4 bytes. */
+int cgs_readback_rgba8(cgs_context* ctx, uint8_t* dst, uint32_t width, uint32_t height);
+
+/* Replaces copy_framebuffer_to_texture (gpu_state.rs:1247-1292): the device
+ * address of the framebuffer and its row stride in u32 words. */
+int cgs_framebuffer_device_ptr(cgs_context* ctx, void** ptr, uint32_t* padded_row_u32);
+
+/* Per-launch device times of the last cgs_execute_frame run with
+ * profile_stages >= 2 (the reference's measure_stage, gpu_state.rs:575-608,
+ * at kernel granularity): names[32*i] (NUL-terminated) and ms[i]. */
+int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, uint32_t* count);
+
+/* ---- per-stage access (the stage wrappers of src/render/gpu_bridge.rs) ---- */
+
+/* Stage 2 output in the reference layout: attrs[10*n] = [mu_x, mu_y, a, b, c,
+ * alpha_2d, r, g, b, tz] and extents[n] (ext_x | ext_y << 16, 0 = culled)
+ * (project.rs:1271-1284); n = active splats of the last frame. */
+int cgs_read_projected(cgs_context* ctx, float* attrs10, uint32_t* extents, uint64_t n);
+
+/* Stage 3 output of the last frame: per-splat tile counts and exclusive
+ * offsets (needs cgs_set_capture(ctx, 1) before the frame), sorted keys
+ * (packed 32-bit keys, or tile ids on the 64-bit path), sorted splat ids and
+ * tile ranges[2*tiles].  Any pointer may be null.  *n_entries gets E. */
+int cgs_read_stage3(cgs_context* ctx, uint32_t* counts, uint32_t* offsets, uint32_t* sorted_keys,
+                    uint32_t* sorted_ids, uint32_t* ranges, uint64_t* n_entries);
+int cgs_set_capture(cgs_context* ctx, uint32_t enable);
+
+/* Stage 1 output of the last deformation (deform_and_split_compact_kernel,
+ * deformation.rs:872-1211): positions[3*m], covariances[6*m], opacities[m],
+ * split_meta[2*splits] (parent, tet) with m = count + splits. */
+int cgs_read_deformed(cgs_context* ctx, float* positions, float* covariances, float* opacities,
+                      uint32_t* split_meta, uint32_t* split_count);
+/* compute_tet_transforms_kernel outputs (deformation.rs:682-865). */
+int cgs_read_tet_transforms(cgs_context* ctx, float* jacobians, float* rotations,
+                            float* max_stretch);
+
+/* Inject a projected splat set (reference layout, as cgs_read_projected) and
+ * run Stage 3 / Stage 4 on it (the split of execute_frame_gpu used by
+ * test_gpu_stage3_binning_and_ranges_match_cpu, gpu_parity_tests.rs:664-917). */
+int cgs_load_projected(cgs_context* ctx, const float* attrs10, const uint32_t* extents,
+                       uint64_t n);
+int cgs_run_stage3(cgs_context* ctx, const cgs_camera_uniforms* camera,
+                   const cgs_compositor_uniforms* compositor, uint64_t* n_entries);
+int cgs_run_stage4(cgs_context* ctx, const cgs_compositor_uniforms* compositor);
+
+/* ---- device-buffer primitives ---- */
+int cgs_malloc(cgs_context* ctx, uint64_t bytes, void** dptr);
+int cgs_free(cgs_context* ctx, void* dptr);
+int cgs_memcpy_h2d(cgs_context* ctx, void* dst, const void* src, uint64_t bytes);
+int cgs_memcpy_d2h(cgs_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* gpu_exclusive_scan_total (src/render/kernels/scan.rs:312-333). */
+int cgs_exclusive_scan_u32(cgs_context* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                           uint32_t* total);
+/* gpu_radix_sort_u32_pairs (radix_sort.rs:292-368): stable sort of (key, value)
+ * on the low sort_bits key bits, in place. */
+int cgs_radix_sort_u32_pairs(cgs_context* ctx, uint32_t* d_keys, uint32_t* d_vals, uint64_t n,
+                             uint32_t sort_bits);
+/* gpu_radix_sort_u64_pairs (radix_sort.rs:377-477): stable sort of entries by
+ * (tile id on tile_bits, depth bits).  d_tile_ids and d_splat_ids receive the
+ * sorted order (what u64_sorted_outputs returns, radix_sort.rs:270-286);
+ * d_depths is left sorted by depth alone (it serves as the first key). */
+int cgs_radix_sort_u64_pairs(cgs_context* ctx, uint32_t* d_tile_ids, uint32_t* d_depths,
+                             uint32_t* d_splat_ids, uint64_t n, uint32_t tile_bits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CINDERGS_H */

@@ -1,0 +1,720 @@
+// cgs_oracle.cpp — CPU restatement of CinderGS's render and cage-deformation
+// path, used ONLY as the checker by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py.  Nothing in the product links or calls it.
+//
+// It restates, statement by statement and in the same floating-point operation
+// order, the reference's v2 GPU pipeline (the bit-exact target for keys, scan
+// offsets, tile counts and ranges) so that it can run on the host:
+//   Stage 1  src/render/kernels/deformation.rs:682-865  (tet transforms)
+//            src/render/kernels/deformation.rs:872-1211 (deform + split)
+//            with children appended in parent order, as the CPU path does
+//            (src/deformation/cpu_reference.rs:316-377), instead of the
+//            reference GPU's atomic (non-deterministic) order.
+//   Stage 2  src/render/kernels/project.rs:1018-1340 (+ eval_sh_f32 797-895,
+//            f16 storage variant 1344-1679)
+//   Stage 3  src/render/kernels/binning.rs:12-417, scan.rs (exclusive scan),
+//            radix_sort.rs (stable LSD sort == stable sort by key)
+//   Stage 4  src/render/kernels/compositor.rs:721-883
+//   MVC      src/deformation/mvc.rs:340-541 (inside test, exact 3D MVC,
+//            central-difference gradients)
+//
+// Every routine is a template over the scalar type: the float instantiation is
+// the FP32 oracle (elementary functions from cgs_detmath.h so that the result
+// is defined bit for bit), the double instantiation is the FP64 twin used to
+// state each FP32 error against an FP64 run.
+//
+// Parity pinning status (what checks each function besides the GPU):
+//   count/emit/ellipse test : brute-force per-pixel coverage on random ellipses
+//                              (tiles/mod.rs:302-401 procedure), closed forms
+//   scan / sort / ranges    : reference vectors (gpu_parity_tests.rs:512-662),
+//                              std::stable_sort, structural invariants
+//   projection              : closed forms (projection.rs:556-740 analogues),
+//                              SH constants/special directions (sh.rs:144-182),
+//                              FP64 twin
+//   compositor              : hand-computed single/double splat fixtures,
+//                              early-termination bound (T < 1e-3), FP64 twin
+//   tet transforms / deform : identity, uniform scale 2, pure rotation,
+//                              reflection closed forms (integration_tests.rs
+//                              1146-1319), polar identities, FP64 twin
+//   MVC                     : partition of unity, linear reproduction,
+//                              FP64 twin, finite-difference consistency
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <algorithm>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../cindergs_amd/csrc/cgs_detmath.h"
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+template <class T> struct Mth;
+template <> struct Mth<float> {
+  static float ln(float x) { return cgs_logf(x); }
+  static float ex(float x) { return cgs_expf(x); }
+  static float sq(float x) { return cgs_sqrtf(x); }
+  static float pw(float x, float y) { return cgs_powf_pos(x, y); }
+  static float ac(float x) { return cgs_acosf(x); }
+  static float co(float x) { return cgs_cosf(x); }
+  static float mx(float a, float b) { return cgs_fmax(a, b); }
+  static float mn(float a, float b) { return cgs_fmin(a, b); }
+  static float cl(float x, float lo, float hi) { return cgs_fclamp(x, lo, hi); }
+  static float fl(float x) { return floorf(x); }
+  static float ce(float x) { return ceilf(x); }
+  static float rd(float x) { return roundf(x); }
+  static float ab(float x) { return fabsf(x); }
+  // compositor exponent: libm (the GPU uses the hardware exp2 within 1/255)
+  static float cex(float x) { return expf(x); }
+};
+template <> struct Mth<double> {
+  static double ln(double x) { return log(x); }
+  static double ex(double x) { return exp(x); }
+  static double sq(double x) { return sqrt(x); }
+  static double pw(double x, double y) { return pow(x, y); }
+  static double ac(double x) { return acos(x); }
+  static double co(double x) { return cos(x); }
+  static double mx(double a, double b) { return fmax(a, b); }
+  static double mn(double a, double b) { return fmin(a, b); }
+  static double cl(double x, double lo, double hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+  }
+  static double fl(double x) { return floor(x); }
+  static double ce(double x) { return ceil(x); }
+  static double rd(double x) { return round(x); }
+  static double ab(double x) { return fabs(x); }
+  static double cex(double x) { return exp(x); }
+};
+
+static int g_threads = 1;
+
+// ---------------------------------------------------------------------------
+// Stage 2: project_gaussians_v2_kernel (project.rs:1018-1340)
+// ---------------------------------------------------------------------------
+
+// eval_sh_f32 (project.rs:797-895); `c` is the 48-coefficient record
+// [dc r,g,b, rest_r 15, rest_g 15, rest_b 15] (sh.rs:56-61).
+template <class T>
+static void eval_sh(const T* c, T dx, T dy, T dz, uint32_t deg, T out[3]) {
+  typedef Mth<T> m;
+  // K(x): the reference's f32 literal; the FP64 twin uses the same (f32) values.
+#define K(x) ((T)(x##f))
+  T cr = K(0.28209479177387814) * c[0];
+  T cg = K(0.28209479177387814) * c[1];
+  T cb = K(0.28209479177387814) * c[2];
+  if (deg >= 1) {
+    T y1m1 = K(-0.4886025119029199) * dy;
+    T y10 = K(0.4886025119029199) * dz;
+    T y1p1 = K(-0.4886025119029199) * dx;
+    cr += y1m1 * c[3] + y10 * c[4] + y1p1 * c[5];
+    cg += y1m1 * c[18] + y10 * c[19] + y1p1 * c[20];
+    cb += y1m1 * c[33] + y10 * c[34] + y1p1 * c[35];
+    if (deg >= 2) {
+      T xx = dx * dx, yy = dy * dy, zz = dz * dz;
+      T xy = dx * dy, yz = dy * dz, xz = dx * dz;
+      T y2m2 = K(1.0925484305920792) * xy;
+      T y2m1 = K(-1.0925484305920792) * yz;
+      T y20 = K(0.31539156525252005) * ((T)2 * zz - xx - yy);
+      T y2p1 = K(-1.0925484305920792) * xz;
+      T y2p2 = K(0.5462742152960396) * (xx - yy);
+      cr += y2m2 * c[6] + y2m1 * c[7] + y20 * c[8] + y2p1 * c[9] + y2p2 * c[10];
+      cg += y2m2 * c[21] + y2m1 * c[22] + y20 * c[23] + y2p1 * c[24] + y2p2 * c[25];
+      cb += y2m2 * c[36] + y2m1 * c[37] + y20 * c[38] + y2p1 * c[39] + y2p2 * c[40];
+      if (deg >= 3) {
+        T y3m3 = K(-0.5900435899266435) * dy * ((T)3 * xx - yy);
+        T y3m2 = K(2.890611442640554) * xy * dz;
+        T y3m1 = K(-0.4570457994644658) * dy * ((T)4 * zz - xx - yy);
+        T y30 = K(0.3731763325901154) * dz * ((T)2 * zz - (T)3 * xx - (T)3 * yy);
+        T y3p1 = K(-0.4570457994644658) * dx * ((T)4 * zz - xx - yy);
+        T y3p2 = K(1.445305721320277) * dz * (xx - yy);
+        T y3p3 = K(-0.5900435899266435) * dx * (xx - (T)3 * yy);
+        cr += y3m3 * c[11] + y3m2 * c[12] + y3m1 * c[13] + y30 * c[14] + y3p1 * c[15] +
+              y3p2 * c[16] + y3p3 * c[17];
+        cg += y3m3 * c[26] + y3m2 * c[27] + y3m1 * c[28] + y30 * c[29] + y3p1 * c[30] +
+              y3p2 * c[31] + y3p3 * c[32];
+        cb += y3m3 * c[41] + y3m2 * c[42] + y3m1 * c[43] + y30 * c[44] + y3p1 * c[45] +
+              y3p2 * c[46] + y3p3 * c[47];
+      }
+    }
+  }
+#undef K
+  out[0] = m::cl(cr + (T)0.5, (T)0, (T)1);
+  out[1] = m::cl(cg + (T)0.5, (T)0, (T)1);
+  out[2] = m::cl(cb + (T)0.5, (T)0, (T)1);
+}
+
+struct ProjParams {
+  const float* camera;  // 24 floats (CameraUniforms, camera.rs:9-36)
+  float cam_pos[3];
+  float mip;
+  float min_opacity;
+  float min_radius;
+  uint32_t sh_degree;
+  uint32_t has_deformation;
+  uint32_t num_tets;
+  uint32_t num_rest;
+};
+
+// One splat of project_gaussians_v2_kernel. Returns true when the splat
+// survives; `attrs` = [mu_x, mu_y, a, b, c, alpha_2d, r, g, b, tz].
+template <class T>
+static bool project_one(const ProjParams& P, const T pos[3], const T cov[6], T op,
+                        const T* sh48, const float* rot9, T attrs[10], uint32_t* ext_packed) {
+  typedef Mth<T> m;
+  const float* cam = P.camera;
+  T w00 = cam[0], w10 = cam[1], w20 = cam[2];
+  T w01 = cam[4], w11 = cam[5], w21 = cam[6];
+  T w02 = cam[8], w12 = cam[9], w22 = cam[10];
+  T w03 = cam[12], w13 = cam[13], w23 = cam[14];
+  T focal_x = cam[16], focal_y = cam[17], principal_x = cam[18], principal_y = cam[19];
+  T vw = cam[20], vh = cam[21], near_plane = cam[22];
+
+  T px = pos[0], py = pos[1], pz = pos[2];
+  T tx = w00 * px + w01 * py + w02 * pz + w03;
+  T ty = w10 * px + w11 * py + w12 * pz + w13;
+  T tz = w20 * px + w21 * py + w22 * pz + w23;
+  if (tz <= near_plane) return false;
+
+  if (op < (T)P.min_opacity) return false;
+  T v_xx = cov[0], v_xy = cov[1], v_xz = cov[2], v_yy = cov[3], v_yz = cov[4], v_zz = cov[5];
+  T alpha_3d = op;
+  T mip = (T)P.mip;
+  if (mip > (T)0) {
+    T det_orig = m::mx(v_xx * (v_yy * v_zz - v_yz * v_yz) - v_xy * (v_xy * v_zz - v_xz * v_yz) +
+                           v_xz * (v_xy * v_yz - v_xz * v_yy),
+                       (T)1e-24f);
+    v_xx += mip;
+    v_yy += mip;
+    v_zz += mip;
+    T det_reg = m::mx(v_xx * (v_yy * v_zz - v_yz * v_yz) - v_xy * (v_xy * v_zz - v_xz * v_yz) +
+                          v_xz * (v_xy * v_yz - v_xz * v_yy),
+                      det_orig);
+    T comp = m::sq(m::cl(det_orig / det_reg, (T)0, (T)1));
+    alpha_3d = m::cl(op * comp, (T)0, (T)1);
+    if (alpha_3d < (T)P.min_opacity) return false;
+  }
+
+  T k00 = w00 * v_xx + w01 * v_xy + w02 * v_xz;
+  T k01 = w00 * v_xy + w01 * v_yy + w02 * v_yz;
+  T k02 = w00 * v_xz + w01 * v_yz + w02 * v_zz;
+  T k10 = w10 * v_xx + w11 * v_xy + w12 * v_xz;
+  T k11 = w10 * v_xy + w11 * v_yy + w12 * v_yz;
+  T k12 = w10 * v_xz + w11 * v_yz + w12 * v_zz;
+  T k20 = w20 * v_xx + w21 * v_xy + w22 * v_xz;
+  T k21 = w20 * v_xy + w21 * v_yy + w22 * v_yz;
+  T k22 = w20 * v_xz + w21 * v_yz + w22 * v_zz;
+
+  T cam_xx = k00 * w00 + k01 * w01 + k02 * w02;
+  T cam_xy = k00 * w10 + k01 * w11 + k02 * w12;
+  T cam_xz = k00 * w20 + k01 * w21 + k02 * w22;
+  T cam_yy = k10 * w10 + k11 * w11 + k12 * w12;
+  T cam_yz = k10 * w20 + k11 * w21 + k12 * w22;
+  T cam_zz = k20 * w20 + k21 * w21 + k22 * w22;
+  (void)k21;
+  (void)k22;
+
+  T inv_tz = (T)1 / tz;
+  T inv_tz2 = inv_tz * inv_tz;
+  T j00 = focal_x * inv_tz;
+  T j02 = -focal_x * tx * inv_tz2;
+  T j11 = focal_y * inv_tz;
+  T j12 = -focal_y * ty * inv_tz2;
+
+  T m00 = j00 * cam_xx + j02 * cam_xz;
+  T m01 = j00 * cam_xy + j02 * cam_yz;
+  T m02 = j00 * cam_xz + j02 * cam_zz;
+  T m11 = j11 * cam_yy + j12 * cam_yz;
+  T m12 = j11 * cam_yz + j12 * cam_zz;
+
+  T s_xx = m00 * j00 + m02 * j02;
+  T s_xy = m01 * j11 + m02 * j12;
+  T s_yy = m11 * j11 + m12 * j12;
+
+  T raw_mid = (T)0.5 * (s_xx + s_yy);
+  T raw_det = m::mx(s_xx * s_yy - s_xy * s_xy, (T)0);
+  T raw_discr = m::sq(m::mx(raw_mid * raw_mid - raw_det, (T)0));
+  T raw_lmax = m::mx(raw_mid + raw_discr, (T)0);
+  if ((T)3 * m::sq(raw_lmax) < (T)P.min_radius) return false;
+
+  T sp_xx = s_xx + (T)0.3f;
+  T sp_xy = s_xy;
+  T sp_yy = s_yy + (T)0.3f;
+  T det = sp_xx * sp_yy - sp_xy * sp_xy;
+  if (det <= (T)1e-6f) return false;
+  T comp2 = m::sq(m::cl(raw_det / det, (T)0, (T)1));
+  T alpha_2d = m::cl(alpha_3d * comp2, (T)0, (T)1);
+  if (alpha_2d < (T)P.min_opacity || alpha_2d <= (T)(1.0f / 255.0f)) return false;
+  T q_max = m::cl((T)2 * m::ln((T)255 * alpha_2d), (T)0, (T)8);
+  T sigma_cut = m::sq(q_max);
+  T ext_x = m::cl(m::ce(sigma_cut * m::sq(m::mx(sp_xx, (T)0))), (T)1, (T)1024);
+  T ext_y = m::cl(m::ce(sigma_cut * m::sq(m::mx(sp_yy, (T)0))), (T)1, (T)1024);
+  T mu_x = focal_x * tx * inv_tz + principal_x;
+  T mu_y = focal_y * ty * inv_tz + principal_y;
+  if (mu_x + ext_x < (T)0 || mu_x - ext_x >= vw || mu_y + ext_y < (T)0 || mu_y - ext_y >= vh)
+    return false;
+  T inv_det = (T)1 / det;
+  T a = sp_yy * inv_det;
+  T b = -sp_xy * inv_det;
+  T c = sp_xx * inv_det;
+
+  T vx = px - (T)P.cam_pos[0];
+  T vy = py - (T)P.cam_pos[1];
+  T vz = pz - (T)P.cam_pos[2];
+  T inv_len = (T)1 / m::mx((T)0.0001f, m::sq(vx * vx + vy * vy + vz * vz));
+  T dx = vx * inv_len, dy = vy * inv_len, dz = vz * inv_len;
+  if (rot9) {
+    T r00 = rot9[0], r10 = rot9[1], r20 = rot9[2];
+    T r01 = rot9[3], r11 = rot9[4], r21 = rot9[5];
+    T r02 = rot9[6], r12 = rot9[7], r22 = rot9[8];
+    T ldx = r00 * dx + r10 * dy + r20 * dz;
+    T ldy = r01 * dx + r11 * dy + r21 * dz;
+    T ldz = r02 * dx + r12 * dy + r22 * dz;
+    dx = ldx;
+    dy = ldy;
+    dz = ldz;
+  }
+  T rgb[3];
+  eval_sh<T>(sh48, dx, dy, dz, P.sh_degree, rgb);
+  attrs[0] = mu_x;
+  attrs[1] = mu_y;
+  attrs[2] = a;
+  attrs[3] = b;
+  attrs[4] = c;
+  attrs[5] = alpha_2d;
+  attrs[6] = rgb[0];
+  attrs[7] = rgb[1];
+  attrs[8] = rgb[2];
+  attrs[9] = tz;
+  uint32_t ex = (uint32_t)ext_x, ey = (uint32_t)ext_y;
+  *ext_packed = (ex & 0xFFFFu) | ((ey & 0xFFFFu) << 16);
+  return true;
+}
+
+// IEEE binary16 -> binary32, exact (the reference's native f16 path casts
+// with f32::cast_from, project.rs:899-1009).
+static float half_to_float(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  uint32_t e = (h >> 10) & 0x1fu;
+  uint32_t mant = h & 0x3ffu;
+  if (e == 0) {
+    if (mant == 0) return cgs_u2f(sign);
+    float f = (float)mant * 5.9604644775390625e-8f;  // mant * 2^-24, exact
+    return cgs_u2f(cgs_f2u(f) | sign);
+  }
+  if (e == 31) return cgs_u2f(sign | 0x7f800000u | (mant << 13));
+  return cgs_u2f(sign | ((e + 112u) << 23) | (mant << 13));
+}
+
+template <class T, class OutT>
+static uint32_t project_all(const ProjParams& P, const float* pos, const float* cov,
+                            const uint16_t* cov_h, const float* opa, const float* sh,
+                            const uint16_t* sh_h, const uint32_t* bind_tet,
+                            const float* tet_rot, const uint32_t* split_meta,
+                            uint32_t split_count, OutT* attrs10, uint32_t* extents,
+                            uint32_t max_gaussians) {
+  uint32_t active = P.num_rest + split_count;
+  uint32_t survivors = 0;
+  uint32_t bases = P.sh_degree == 0 ? 1 : P.sh_degree == 1 ? 4 : P.sh_degree == 2 ? 9 : 16;
+  uint32_t h_stride = ((bases * 3 + 1) / 2) * 2;
+#pragma omp parallel for schedule(static, 2048) reduction(+ : survivors) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)max_gaussians; ++ii) {
+    uint32_t idx = (uint32_t)ii;
+    if (idx >= active) {
+      extents[idx] = 0;
+      continue;
+    }
+    T p[3] = {(T)pos[idx * 3 + 0], (T)pos[idx * 3 + 1], (T)pos[idx * 3 + 2]};
+    T cv[6];
+    for (int k = 0; k < 6; ++k)
+      cv[k] = cov_h ? (T)half_to_float(cov_h[(size_t)idx * 6 + k]) : (T)cov[(size_t)idx * 6 + k];
+    uint32_t sh_idx = idx;
+    uint32_t tet_raw = P.num_tets;
+    if (idx < P.num_rest) {
+      if (P.has_deformation) tet_raw = bind_tet[idx];
+    } else {
+      uint32_t off = idx - P.num_rest;
+      sh_idx = split_meta[off * 2];
+      tet_raw = split_meta[off * 2 + 1];
+    }
+    T shv[48];
+    if (sh_h) {
+      // channel-major trimmed f16 record -> 48-coefficient record
+      for (int k = 0; k < 48; ++k) shv[k] = 0;
+      const uint16_t* s = sh_h + (size_t)sh_idx * h_stride;
+      shv[0] = half_to_float(s[0]);
+      shv[1] = half_to_float(s[1]);
+      shv[2] = half_to_float(s[2]);
+      uint32_t rest = bases - 1;
+      for (uint32_t k = 0; k < rest; ++k) {
+        shv[3 + k] = half_to_float(s[3 + k]);
+        shv[18 + k] = half_to_float(s[3 + rest + k]);
+        shv[33 + k] = half_to_float(s[3 + 2 * rest + k]);
+      }
+    } else {
+      for (int k = 0; k < 48; ++k) shv[k] = (T)sh[(size_t)sh_idx * 48 + k];
+    }
+    const float* rot =
+        (P.has_deformation && tet_raw < P.num_tets) ? tet_rot + (size_t)tet_raw * 9 : nullptr;
+    T at[10];
+    uint32_t ep = 0;
+    bool alive = project_one<T>(P, p, cv, (T)opa[idx], shv, rot, at, &ep);
+    extents[idx] = alive ? ep : 0u;
+    if (alive) {
+      for (int k = 0; k < 10; ++k) attrs10[(size_t)idx * 10 + k] = (OutT)at[k];
+      survivors += 1;
+    }
+  }
+  return survivors;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 3: binning.rs
+// ---------------------------------------------------------------------------
+
+// ellipse_intersects_tile_gpu (binning.rs:35-83) with the per-splat cutoff
+// q_cutoff = clamp(2 ln(255 alpha), 0, 8) hoisted (a pure function of alpha).
+template <class T>
+static bool ellipse_tile(T mu_x, T mu_y, T a, T b, T c, T alpha, T q_cut, uint32_t tx, uint32_t ty,
+                         uint32_t W, uint32_t H) {
+  typedef Mth<T> m;
+  if (alpha < (T)(1.0f / 255.0f) || W == 0 || H == 0) return false;
+  uint32_t xs = tx * 16u, ys = ty * 16u;
+  if (xs >= W || ys >= H) return false;
+  uint32_t xe = std::min(xs + 15u, W - 1u), ye = std::min(ys + 15u, H - 1u);
+  T tminx = (T)(float)xs + (T)0.5, tmaxx = (T)(float)xe + (T)0.5;
+  T tminy = (T)(float)ys + (T)0.5, tmaxy = (T)(float)ye + (T)0.5;
+  T dx0 = tminx - mu_x, dx1 = tmaxx - mu_x, dy0 = tminy - mu_y, dy1 = tmaxy - mu_y;
+  T x_near = m::cl((T)0, dx0, dx1);
+  T y1 = m::cl(-b * x_near / m::mx(c, (T)1e-12f), dy0, dy1);
+  T q1 = a * x_near * x_near + (T)2 * b * x_near * y1 + c * y1 * y1;
+  T y_near = m::cl((T)0, dy0, dy1);
+  T x2 = m::cl(-b * y_near / m::mx(a, (T)1e-12f), dx0, dx1);
+  T q2 = a * x2 * x2 + (T)2 * b * x2 * y_near + c * y_near * y_near;
+  T min_quad = m::mn(q1, q2);
+  return min_quad <= q_cut + (T)1e-4f;
+}
+
+struct Bounds {
+  uint32_t tx0, tx1, ty0, ty1;
+};
+
+static Bounds tile_bounds(float mu_x, float mu_y, uint32_t packed, uint32_t tiles_x,
+                          uint32_t tiles_y) {
+  float ext_x = (float)(packed & 0xFFFFu);
+  float ext_y = (float)((packed >> 16) & 0xFFFFu);
+  int32_t mtx = (int32_t)tiles_x - 1, mty = (int32_t)tiles_y - 1;
+  Bounds r;
+  r.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / 16.0f)), 0, mtx);
+  r.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / 16.0f)), 0, mtx);
+  r.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / 16.0f)), 0, mty);
+  r.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / 16.0f)), 0, mty);
+  return r;
+}
+
+static inline float qcut_of(float alpha) {
+  return cgs_fclamp(2.0f * cgs_logf(255.0f * alpha), 0.0f, 8.0f);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Exported C interface (ctypes).  All arrays are in the reference's layouts.
+// ===========================================================================
+
+ORC_API void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+ORC_API int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+ORC_API float orc_logf(float x) { return cgs_logf(x); }
+ORC_API float orc_expf(float x) { return cgs_expf(x); }
+ORC_API float orc_powf(float x, float y) { return cgs_powf_pos(x, y); }
+ORC_API float orc_acosf(float x) { return cgs_acosf(x); }
+ORC_API float orc_cosf(float x) { return cgs_cosf(x); }
+ORC_API float orc_half_to_float(uint16_t h) { return half_to_float(h); }
+
+// Stage 2, FP32 or FP64 (attrs written as float or double).  cov_h / sh_h select
+// the f16 storage variant (project.rs:1344-1679) when non-null.
+ORC_API uint32_t orc_project_v2(const float* pos, const float* cov, const uint16_t* cov_h,
+                                const float* opa, const float* sh, const uint16_t* sh_h,
+                                const uint32_t* bind_tet, const float* tet_rot,
+                                const uint32_t* split_meta, uint32_t split_count,
+                                const float* camera24, const float* cam_pos3, float mip,
+                                float min_opacity, float min_radius, uint32_t sh_degree,
+                                uint32_t has_deformation, uint32_t num_tets, uint32_t num_rest,
+                                uint32_t max_gaussians, int fp64, void* attrs10,
+                                uint32_t* extents) {
+  ProjParams P;
+  P.camera = camera24;
+  P.cam_pos[0] = cam_pos3[0];
+  P.cam_pos[1] = cam_pos3[1];
+  P.cam_pos[2] = cam_pos3[2];
+  P.mip = mip;
+  P.min_opacity = min_opacity;
+  P.min_radius = min_radius;
+  P.sh_degree = sh_degree > 3 ? 3 : sh_degree;
+  P.has_deformation = has_deformation;
+  P.num_tets = num_tets;
+  P.num_rest = num_rest;
+  if (fp64)
+    return project_all<double, double>(P, pos, cov, cov_h, opa, sh, sh_h, bind_tet, tet_rot,
+                                       split_meta, split_count, (double*)attrs10, extents,
+                                       max_gaussians);
+  return project_all<float, float>(P, pos, cov, cov_h, opa, sh, sh_h, bind_tet, tet_rot,
+                                   split_meta, split_count, (float*)attrs10, extents,
+                                   max_gaussians);
+}
+
+// compute_packed_key_bits (binning.rs:12-20)
+ORC_API void orc_packed_key_bits(uint32_t total_tiles, uint32_t* tile_bits, uint32_t* depth_bits) {
+  uint32_t tb = total_tiles <= 1 ? 1u : 32u - (uint32_t)__builtin_clz(total_tiles - 1u);
+  uint32_t db = 32u - tb;
+  if (db > 28u) db = 28u;
+  *tile_bits = tb;
+  *depth_bits = db;
+}
+
+// count_tiles_kernel (binning.rs:90-164)
+ORC_API void orc_count_tiles(const float* attrs10, const uint32_t* extents, uint32_t active,
+                             uint32_t max_gaussians, uint32_t W, uint32_t H, uint32_t tiles_x,
+                             uint32_t tiles_y, uint32_t* counts) {
+#pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)max_gaussians; ++ii) {
+    uint32_t idx = (uint32_t)ii;
+    if (idx >= active || tiles_x == 0 || tiles_y == 0 || extents[idx] == 0) {
+      counts[idx] = 0;
+      continue;
+    }
+    const float* a = attrs10 + (size_t)idx * 10;
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    float qc = qcut_of(a[5]);
+    uint32_t cnt = 0;
+    for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
+      for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) cnt++;
+    counts[idx] = cnt;
+  }
+}
+
+// gpu_exclusive_scan_total (scan.rs:198-333) == serial exclusive prefix sum
+// (tiles/cpu_reference.rs:100-115); u32 wrap-around as on the GPU.
+ORC_API uint32_t orc_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n) {
+  uint32_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t v = in[i];
+    out[i] = acc;
+    acc += v;
+  }
+  return acc;
+}
+
+// emit_tile_keys_kernel (binning.rs:168-250)
+ORC_API void orc_emit_keys32(const float* attrs10, const uint32_t* extents,
+                             const uint32_t* offsets, uint32_t active, uint32_t W, uint32_t H,
+                             uint32_t tiles_x, uint32_t tiles_y, uint32_t base_depth_bits,
+                             uint32_t depth_bits, uint32_t* keys, uint32_t* ids) {
+#pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)active; ++ii) {
+    uint32_t idx = (uint32_t)ii;
+    if (tiles_x == 0 || tiles_y == 0 || extents[idx] == 0) continue;
+    const float* a = attrs10 + (size_t)idx * 10;
+    uint32_t raw = cgs_f2u(cgs_fmax(a[9], 1e-6f));
+    uint32_t rel = raw > base_depth_bits ? std::min(raw - base_depth_bits, 0x0FFFFFFFu) : 0u;
+    uint32_t dq = rel >> (28u - depth_bits);
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    float qc = qcut_of(a[5]);
+    uint32_t cur = offsets[idx];
+    for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
+      for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) {
+          uint32_t tile = ty * tiles_x + tx;
+          keys[cur] = (tile << depth_bits) | dq;
+          ids[cur] = idx;
+          cur++;
+        }
+  }
+}
+
+// emit_tile_keys_u64_kernel (binning.rs:254-328)
+ORC_API void orc_emit_keys64(const float* attrs10, const uint32_t* extents,
+                             const uint32_t* offsets, uint32_t active, uint32_t W, uint32_t H,
+                             uint32_t tiles_x, uint32_t tiles_y, uint32_t* tile_ids,
+                             uint32_t* depths, uint32_t* ids) {
+#pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)active; ++ii) {
+    uint32_t idx = (uint32_t)ii;
+    if (tiles_x == 0 || tiles_y == 0 || extents[idx] == 0) continue;
+    const float* a = attrs10 + (size_t)idx * 10;
+    uint32_t dbits = cgs_f2u(cgs_fmax(a[9], 0.0f));
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    float qc = qcut_of(a[5]);
+    uint32_t cur = offsets[idx];
+    for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
+      for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) {
+          tile_ids[cur] = ty * tiles_x + tx;
+          depths[cur] = dbits;
+          ids[cur] = idx;
+          cur++;
+        }
+  }
+}
+
+// gpu_radix_sort_u32_pairs (radix_sort.rs:292-368): a stable LSD radix sort
+// over `bits` key bits == a stable sort by (key & mask).
+ORC_API void orc_sort_pairs32(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t bits) {
+  uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+  std::vector<uint64_t> tmp(n);
+  for (uint64_t i = 0; i < n; ++i) tmp[i] = ((uint64_t)(keys[i] & mask) << 32) | i;
+  std::sort(tmp.begin(), tmp.end());  // (key, original index) total order == stable
+  std::vector<uint32_t> k2(n), v2(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t src = tmp[i] & 0xFFFFFFFFull;
+    k2[i] = keys[src];
+    v2[i] = vals[src];
+  }
+  if (n) {
+    memcpy(keys, k2.data(), n * 4);
+    memcpy(vals, v2.data(), n * 4);
+  }
+}
+
+// gpu_radix_sort_u64_pairs (radix_sort.rs:377-477): stable by depth, then
+// stable by tile == stable sort by (tile & tile_mask, depth).
+ORC_API void orc_sort64(uint32_t* tile_ids, uint32_t* depths, uint32_t* ids, uint64_t n,
+                        uint32_t tile_bits) {
+  uint32_t tmask = tile_bits >= 32 ? 0xFFFFFFFFu : ((1u << tile_bits) - 1u);
+  std::vector<uint64_t> order(n);
+  std::iota(order.begin(), order.end(), 0ull);
+  std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+    uint32_t tx = tile_ids[x] & tmask, ty = tile_ids[y] & tmask;
+    if (tx != ty) return tx < ty;
+    return depths[x] < depths[y];
+  });
+  std::vector<uint32_t> t2(n), d2(n), i2(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    t2[i] = tile_ids[order[i]];
+    d2[i] = depths[order[i]];
+    i2[i] = ids[order[i]];
+  }
+  if (n) {
+    memcpy(tile_ids, t2.data(), n * 4);
+    memcpy(depths, d2.data(), n * 4);
+    memcpy(ids, i2.data(), n * 4);
+  }
+}
+
+// identify_tile_ranges_kernel (binning.rs:344-379); tile of entry i =
+// keys[i] >> shift (shift = depth_bits, or 0 for sorted tile ids).
+ORC_API void orc_tile_ranges(const uint32_t* sorted, uint64_t n, uint32_t shift,
+                             uint32_t total_tiles, uint32_t* ranges) {
+  memset(ranges, 0, (size_t)total_tiles * 8);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t t = sorted[i] >> shift;
+    if (t >= total_tiles) continue;
+    if (i == 0) {
+      ranges[t * 2] = 0;
+    } else {
+      uint32_t pt = sorted[i - 1] >> shift;
+      if (t != pt) {
+        if (pt < total_tiles) ranges[pt * 2 + 1] = (uint32_t)i;
+        ranges[t * 2] = (uint32_t)i;
+      }
+    }
+    if (i + 1 == n) ranges[t * 2 + 1] = (uint32_t)n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4: tile_compositor_v2_kernel (compositor.rs:721-883).  The per-pixel
+// result is the sequential front-to-back blend over the tile's range; the
+// batching and tile-wide early exit of the kernel do not change it.
+// ---------------------------------------------------------------------------
+template <class T, class AT>
+static void composite_impl(const AT* attrs10, const uint32_t* sorted_ids, const uint32_t* ranges,
+                           uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t padded_row,
+                           const float* bg, uint32_t* out, float* out_rgba_f) {
+  typedef Mth<T> m;
+  uint32_t tiles_y = (H + 15) / 16;
+  (void)tiles_y;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
+  for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
+    uint32_t py = (uint32_t)yy;
+    for (uint32_t px = 0; px < W; ++px) {
+      uint32_t tile = (py / 16) * tiles_x + (px / 16);
+      uint32_t rs = ranges[tile * 2], re = ranges[tile * 2 + 1];
+      bool empty = rs >= re;
+      T cr = 0, cg = 0, cb = 0, t = 1;
+      T pcx = (T)(float)px + (T)0.5, pcy = (T)(float)py + (T)0.5;
+      if (!empty) {
+        for (uint32_t e = rs; e < re; ++e) {
+          const AT* a = attrs10 + (size_t)sorted_ids[e] * 10;
+          T op = (T)a[5];
+          if (!(op > (T)0.003921569f)) continue;
+          T dx = pcx - (T)a[0], dy = pcy - (T)a[1];
+          T q = (T)-0.5 * ((T)a[2] * dx * dx + (T)2 * (T)a[3] * dx * dy + (T)a[4] * dy * dy);
+          if (q > (T)0) q = 0;
+          if (q >= (T)-4) {
+            T alpha = m::mn((T)0.99f, op * m::cex(q));
+            if (alpha >= (T)0.003921569f) {
+              T w = alpha * t;
+              cr += w * (T)a[6];
+              cg += w * (T)a[7];
+              cb += w * (T)a[8];
+              t *= (T)1 - alpha;
+              if (t < (T)0.001f) {
+                t = 0;
+                break;
+              }
+            }
+          }
+        }
+      }
+      T fr = m::cl(cr + t * (T)bg[0], (T)0, (T)1);
+      T fg = m::cl(cg + t * (T)bg[1], (T)0, (T)1);
+      T fb = m::cl(cb + t * (T)bg[2], (T)0, (T)1);
+      T fa = m::cl((T)1 - t, (T)0, (T)1);
+      if (empty) fa = (T)bg[3];
+      if (out_rgba_f) {
+        float* o = out_rgba_f + ((size_t)py * W + px) * 4;
+        o[0] = (float)fr;
+        o[1] = (float)fg;
+        o[2] = (float)fb;
+        o[3] = (float)fa;
+      }
+      if (out) {
+        uint32_t r8 = (uint32_t)m::cl(m::rd(fr * (T)255), (T)0, (T)255);
+        uint32_t g8 = (uint32_t)m::cl(m::rd(fg * (T)255), (T)0, (T)255);
+        uint32_t b8 = (uint32_t)m::cl(m::rd(fb * (T)255), (T)0, (T)255);
+        uint32_t a8 = (uint32_t)m::cl(m::rd(fa * (T)255), (T)0, (T)255);
+        out[(size_t)py * padded_row + px] = r8 | (g8 << 8) | (b8 << 16) | (a8 << 24);
+      }
+    }
+  }
+}
+
+// out: RGBA8 words with row stride padded_row (may be null); out_f: unrounded
+// RGBA floats, W*H*4 (may be null).  fp64 selects the FP64 twin (attrs must
+// then be double).
+ORC_API void orc_composite_v2(const void* attrs10, const uint32_t* sorted_ids,
+                              const uint32_t* ranges, uint32_t W, uint32_t H, uint32_t tiles_x,
+                              uint32_t padded_row, const float* bg4, int fp64, uint32_t* out,
+                              float* out_f) {
+  if (fp64)
+    composite_impl<double, double>((const double*)attrs10, sorted_ids, ranges, W, H, tiles_x,
+                                   padded_row, bg4, out, out_f);
+  else
+    composite_impl<float, float>((const float*)attrs10, sorted_ids, ranges, W, H, tiles_x,
+                                 padded_row, bg4, out, out_f);
+}

@@ -1,0 +1,221 @@
+"""ctypes wrapper of the CPU oracle (oracle/cgs_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+
+`render_frame` chains the restated stages exactly as execute_frame_gpu does
+(src/render/gpu_state.rs:613-1243) for a scene without cages.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_f = C.c_float
+_i = C.c_int
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_set_threads": ([_i], None),
+            "orc_max_threads": ([], _i),
+            "orc_logf": ([_f], _f),
+            "orc_expf": ([_f], _f),
+            "orc_powf": ([_f, _f], _f),
+            "orc_acosf": ([_f], _f),
+            "orc_cosf": ([_f], _f),
+            "orc_half_to_float": ([C.c_uint16], _f),
+            "orc_project_v2": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _f,
+                                _f, _f, _u32, _u32, _u32, _u32, _u32, _i, _vp, _vp], _u32),
+            "orc_packed_key_bits": ([_u32, C.POINTER(_u32), C.POINTER(_u32)], None),
+            "orc_count_tiles": ([_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp], None),
+            "orc_exclusive_scan": ([_vp, _vp, _u64], _u32),
+            "orc_emit_keys32": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp,
+                                 _vp], None),
+            "orc_emit_keys64": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp],
+                                None),
+            "orc_sort_pairs32": ([_vp, _vp, _u64, _u32], None),
+            "orc_sort64": ([_vp, _vp, _vp, _u64, _u32], None),
+            "orc_tile_ranges": ([_vp, _u64, _u32, _u32, _vp], None),
+            "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _vp, _i, _vp, _vp],
+                                 None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_vp)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def packed_key_bits(total_tiles: int):
+    tb, db = _u32(), _u32()
+    lib().orc_packed_key_bits(total_tiles, C.byref(tb), C.byref(db))
+    return tb.value, db.value
+
+
+def base_depth_bits(near: float) -> int:
+    return int(np.float32(max(near, 1e-6)).view(np.uint32)) & 0x7F800000
+
+
+def project(scene, cam_arr, cam_pos, sh_degree=3, mip=0.0, min_opacity=1.0 / 255.0,
+            min_radius=0.5, fp64=False, f16=None, bind_tet=None, tet_rot=None,
+            split_meta=None, split_count=0, num_tets=0, max_gaussians=None, positions=None,
+            covariances=None, opacities=None):
+    """project_gaussians_v2_kernel restated; returns (attrs[n,10], extents[n], survivors).
+
+    `f16` = (cov_halves[n,6] u16, sh_halves[n,stride] u16) selects the f16 variant.
+    """
+    n_rest = scene.count
+    pos = positions if positions is not None else scene.positions
+    cov = covariances if covariances is not None else scene.covariances_3d
+    opa = opacities if opacities is not None else scene.opacities
+    maxg = max_gaussians if max_gaussians is not None else n_rest
+    attrs = np.zeros((maxg, 10), dtype=np.float64 if fp64 else np.float32)
+    ext = np.zeros(maxg, dtype=np.uint32)
+    cov_h = sh_h = None
+    if f16 is not None:
+        cov_h, sh_h = f16
+    cam_pos = np.asarray(cam_pos, dtype=np.float32)
+    has_def = 1 if bind_tet is not None else 0
+    surv = lib().orc_project_v2(
+        _p(np.ascontiguousarray(pos, dtype=np.float32)),
+        None if cov_h is not None else _p(np.ascontiguousarray(cov, dtype=np.float32)),
+        _p(cov_h), _p(np.ascontiguousarray(opa, dtype=np.float32)),
+        None if sh_h is not None else _p(scene.sh_coeffs), _p(sh_h), _p(bind_tet), _p(tet_rot),
+        _p(split_meta), split_count, _p(cam_arr), _p(cam_pos), mip, min_opacity, min_radius,
+        sh_degree, has_def, num_tets, n_rest, maxg, 1 if fp64 else 0, _p(attrs), _p(ext))
+    return attrs, ext, surv
+
+
+def count_tiles(attrs, ext, W, H, active=None):
+    n = ext.shape[0]
+    active = n if active is None else active
+    tx, ty = (W + 15) // 16, (H + 15) // 16
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_count_tiles(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), active, n,
+                          W, H, tx, ty, _p(out))
+    return out
+
+
+def exclusive_scan(v):
+    v = np.ascontiguousarray(v, dtype=np.uint32)
+    out = np.zeros_like(v)
+    tot = lib().orc_exclusive_scan(_p(v), _p(out), v.shape[0])
+    return out, tot
+
+
+def emit32(attrs, ext, offsets, total, W, H, near, active=None):
+    n = ext.shape[0]
+    active = n if active is None else active
+    tx, ty = (W + 15) // 16, (H + 15) // 16
+    _, db = packed_key_bits(tx * ty)
+    keys = np.zeros(max(total, 1), dtype=np.uint32)
+    ids = np.zeros(max(total, 1), dtype=np.uint32)
+    lib().orc_emit_keys32(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), _p(offsets),
+                          active, W, H, tx, ty, base_depth_bits(near), db, _p(keys), _p(ids))
+    return keys[:total], ids[:total], db
+
+
+def emit64(attrs, ext, offsets, total, W, H, active=None):
+    n = ext.shape[0]
+    active = n if active is None else active
+    tx, ty = (W + 15) // 16, (H + 15) // 16
+    tiles = np.zeros(max(total, 1), dtype=np.uint32)
+    depths = np.zeros(max(total, 1), dtype=np.uint32)
+    ids = np.zeros(max(total, 1), dtype=np.uint32)
+    lib().orc_emit_keys64(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), _p(offsets),
+                          active, W, H, tx, ty, _p(tiles), _p(depths), _p(ids))
+    return tiles[:total], depths[:total], ids[:total]
+
+
+def sort_pairs32(keys, vals, bits=32):
+    k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    v = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    lib().orc_sort_pairs32(_p(k), _p(v), k.shape[0], bits)
+    return k, v
+
+
+def sort64(tiles, depths, ids, tile_bits):
+    t = np.ascontiguousarray(tiles, dtype=np.uint32).copy()
+    d = np.ascontiguousarray(depths, dtype=np.uint32).copy()
+    i = np.ascontiguousarray(ids, dtype=np.uint32).copy()
+    lib().orc_sort64(_p(t), _p(d), _p(i), t.shape[0], tile_bits)
+    return t, d, i
+
+
+def tile_ranges(sorted_keys, shift, total_tiles):
+    r = np.zeros(total_tiles * 2, dtype=np.uint32)
+    s = np.ascontiguousarray(sorted_keys, dtype=np.uint32)
+    lib().orc_tile_ranges(_p(s), s.shape[0], shift, total_tiles, _p(r))
+    return r.reshape(-1, 2)
+
+
+def composite(attrs, sorted_ids, ranges, W, H, bg, fp64=False, want_float=False):
+    tx = (W + 15) // 16
+    pad = (W + 63) // 64 * 64
+    out = np.zeros(pad * H, dtype=np.uint32)
+    outf = np.zeros((H, W, 4), dtype=np.float32) if want_float else None
+    a = np.ascontiguousarray(attrs, dtype=np.float64 if fp64 else np.float32)
+    ids = np.ascontiguousarray(sorted_ids, dtype=np.uint32)
+    if ids.size == 0:
+        ids = np.zeros(1, dtype=np.uint32)
+    lib().orc_composite_v2(_p(a), _p(ids), _p(np.ascontiguousarray(ranges, dtype=np.uint32)), W,
+                           H, tx, pad, _p(np.asarray(bg, dtype=np.float32)), 1 if fp64 else 0,
+                           _p(out), _p(outf))
+    rgba = out.reshape(H, pad)[:, :W].copy().view(np.uint8).reshape(H, W, 4)
+    return (rgba, outf) if want_float else rgba
+
+
+def render_frame(scene, cam, comp, sh_degree=None, mip=0.0, min_opacity=1.0 / 255.0,
+                 min_radius=0.5, f16=None):
+    """Stages 2-4 of execute_frame_gpu without cages, all intermediates returned."""
+    from cindergs_amd.synthetic import camera_array  # plain data helper
+
+    W, H = max(1, comp.screen_width), max(1, comp.screen_height)
+    cam_arr = camera_array(cam)
+    deg = comp.sh_degree if sh_degree is None else sh_degree
+    attrs, ext, surv = project(scene, cam_arr, list(comp.camera_position), deg, mip, min_opacity,
+                               min_radius, f16=f16)
+    counts = count_tiles(attrs, ext, W, H)
+    offsets, total = exclusive_scan(counts)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    tb, _ = packed_key_bits(T)
+    out = {"attrs": attrs, "extents": ext, "survivors": surv, "counts": counts,
+           "offsets": offsets, "total": total}
+    if T <= 65535:
+        keys, ids, db = emit32(attrs, ext, offsets, total, W, H, cam.near_plane)
+        sk, sid = sort_pairs32(keys, ids, min(32, tb + db))
+        ranges = tile_ranges(sk, db, T)
+    else:
+        tiles, depths, ids = emit64(attrs, ext, offsets, total, W, H)
+        sk, _, sid = sort64(tiles, depths, ids, tb)
+        ranges = tile_ranges(sk, 0, T)
+    out.update(sorted_keys=sk, sorted_ids=sid, ranges=ranges)
+    out["rgba"] = composite(attrs, sid, ranges, W, H, list(comp.background_color))
+    return out

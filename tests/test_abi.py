@@ -1,0 +1,57 @@
+"""CPU-side checks of the C ABI library: it loads, exports every entry point
+declared in include/cindergs.h, and reports the error taxonomy of the
+reference (GpuError, src/render/gpu.rs:14-30) without a device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from cindergs_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "cindergs.h")).read()
+    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(cgs_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert header_functions() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_reference_sizes():
+    assert C.sizeof(_lib.CameraUniforms) == 96   # camera.rs:9-36
+    assert C.sizeof(_lib.CompositorUniforms) == 48  # rasterizer/mod.rs:91-94
+
+
+def test_invalid_arguments_fail_loudly():
+    lib = _lib.load()
+    assert lib.cgs_create(0, None) == _lib.CGS_ERR_INVALID_ARG
+    assert b"null" in lib.cgs_last_error()
+    assert lib.cgs_invalidate(None) == _lib.CGS_ERR_INVALID_ARG
+    n = C.c_uint32()
+    assert lib.cgs_kernel_times(None, None, None, 0, C.byref(n)) == _lib.CGS_ERR_INVALID_ARG
+
+
+def test_no_device_maps_to_noadapter():
+    """Without a visible GPU, cgs_create reports NoAdapter (the reference caches
+    the acquisition failure, pipeline.rs:211-213, and callers fall back above
+    the ABI)."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    code = _lib.load().cgs_create(0, C.byref(h))
+    assert code == _lib.CGS_ERR_NO_DEVICE
+    with pytest.raises(_lib.GpuError) as ei:
+        _lib.check(code)
+    assert ei.value.kind == "NoAdapter"
