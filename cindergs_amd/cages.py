@@ -1,0 +1,180 @@
+"""Volumetric tetrahedral cages on the host: construction, splat binding and the
+spring simulator that drives the cage vertices frame by frame.
+
+Follows src/deformation/mod.rs (TetMesh::new 202-227: D_m = [X1-X0, X2-X0,
+X3-X0], inverse stored column-major, identity when |det| <= 1e-8;
+bind_point 267-310: the first tetrahedron in index order whose barycentric
+penalty sum(max(-w_k, 0)) is <= 1e-4, else the one with the smallest penalty)
+and src/deformation/dynamics.rs (CageSpringSimulator::step 249-277:
+semi-implicit Euler, a = k (x_rest - x) / m, v = (v + a dt) * exp(-c dt)).
+
+The grid cage is a jittered Freudenthal (Kuhn) subdivision: each cell of an
+nx x ny x nz grid over the scene's bounds is split into 6 tetrahedra sharing
+the cell diagonal; 10 x 10 x 10 cells give 6,000 tetrahedra.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+
+from .pipeline import TetCage
+
+UNBOUND = 0xFFFFFFFF
+
+# the 6 tetrahedra of a Kuhn-subdivided cube (corner index bit0=x, bit1=y, bit2=z),
+# all containing the diagonal 0 -> 7 and positively oriented
+_KUHN = []
+for perm in itertools.permutations((1, 2, 4)):
+    a = perm[0]
+    b = a | perm[1]
+    _KUHN.append((0, a, b, 7))
+
+
+class TetMesh:
+    """TetMesh (src/deformation/mod.rs:191-227) with a cell index for binding."""
+
+    def __init__(self, rest_vertices: np.ndarray, elements: np.ndarray, cells=None, lo=None,
+                 hi=None):
+        self.rest_vertices = np.ascontiguousarray(rest_vertices, dtype=np.float32)
+        self.elements = np.ascontiguousarray(elements, dtype=np.uint32)
+        self.cells = cells
+        self.lo = lo
+        self.hi = hi
+        X = self.rest_vertices.astype(np.float64)
+        e = self.elements.astype(np.int64)
+        Dm = np.stack([X[e[:, 1]] - X[e[:, 0]], X[e[:, 2]] - X[e[:, 0]], X[e[:, 3]] - X[e[:, 0]]],
+                      axis=2)  # columns
+        det = np.linalg.det(Dm)
+        inv = np.linalg.inv(np.where(np.abs(det)[:, None, None] > 1e-8, Dm, np.eye(3)))
+        inv[np.abs(det) <= 1e-8] = np.eye(3)
+        # column-major 9 floats per tet (Mat3::to_cols_array)
+        self.inv_dm = np.ascontiguousarray(np.transpose(inv, (0, 2, 1)).reshape(-1, 9),
+                                           dtype=np.float32)
+
+    @property
+    def num_tets(self) -> int:
+        return int(self.elements.shape[0])
+
+    @property
+    def num_vertices(self) -> int:
+        return int(self.rest_vertices.shape[0])
+
+    @staticmethod
+    def jittered_grid(lo, hi, cells=(10, 10, 10), jitter: float = 0.15, seed: int = 3,
+                      pad: float = 1e-3) -> "TetMesh":
+        lo = np.asarray(lo, dtype=np.float64)
+        hi = np.asarray(hi, dtype=np.float64)
+        ext = np.maximum(hi - lo, 1e-3)
+        lo = lo - ext * pad
+        hi = hi + ext * pad
+        nx, ny, nz = cells
+        gx = np.linspace(lo[0], hi[0], nx + 1)
+        gy = np.linspace(lo[1], hi[1], ny + 1)
+        gz = np.linspace(lo[2], hi[2], nz + 1)
+        V = np.stack(np.meshgrid(gx, gy, gz, indexing="ij"), -1).reshape(-1, 3)
+        h = (hi - lo) / np.array(cells)
+        rng = np.random.Generator(np.random.PCG64(0xC1DE5EED00000000 + seed))
+        ijk = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                                   indexing="ij"), -1).reshape(-1, 3)
+        interior = np.all((ijk > 0) & (ijk < np.array(cells)), axis=1)
+        V[interior] += rng.uniform(-jitter, jitter, size=(interior.sum(), 3)) * h
+
+        def vid(i, j, k):
+            return (i * (ny + 1) + j) * (nz + 1) + k
+
+        elems = []
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    corner = [vid(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1))
+                              for c in range(8)]
+                    for t in _KUHN:
+                        elems.append([corner[t[0]], corner[t[1]], corner[t[2]], corner[t[3]]])
+        return TetMesh(V.astype(np.float32), np.array(elems, dtype=np.uint32), cells, lo, hi)
+
+    def barycentric(self, p: np.ndarray, tets: np.ndarray) -> np.ndarray:
+        """[w0..w3] of points p[n,3] in tets[n] (bind_point's f32 arithmetic)."""
+        X0 = self.rest_vertices[self.elements[tets, 0]]
+        M = self.inv_dm[tets].reshape(-1, 3, 3).transpose(0, 2, 1)  # row-major view
+        d = (p.astype(np.float32) - X0).astype(np.float32)
+        w123 = np.einsum("nij,nj->ni", M, d).astype(np.float32)
+        w0 = (np.float32(1.0) - (w123[:, 0] + w123[:, 1] + w123[:, 2])).astype(np.float32)
+        return np.concatenate([w0[:, None], w123], 1)
+
+    def bind_gaussians(self, positions: np.ndarray):
+        """bind_point for every position: first tet in index order with penalty
+        <= 1e-4, else minimal penalty.  On a grid cage only the tetrahedra of
+        the 27 cells around the point's cell can contain it (jitter < half a
+        cell), so those are the candidates searched, in index order."""
+        assert self.cells is not None, "cell search needs a grid cage"
+        pos = np.asarray(positions, dtype=np.float32)
+        n = pos.shape[0]
+        nx, ny, nz = self.cells
+        cell = np.floor((pos - self.lo) / ((self.hi - self.lo) / np.array(self.cells))).astype(np.int64)
+        cell = np.clip(cell, 0, np.array(self.cells) - 1)
+        best_t = np.zeros(n, dtype=np.int64)
+        best_pen = np.full(n, np.inf, dtype=np.float32)
+        best_w = np.zeros((n, 4), dtype=np.float32)
+        first_t = np.full(n, np.iinfo(np.int64).max, dtype=np.int64)
+        first_w = np.zeros((n, 4), dtype=np.float32)
+        for di, dj, dk in itertools.product((-1, 0, 1), repeat=3):
+            c = cell + np.array([di, dj, dk])
+            ok = np.all((c >= 0) & (c < np.array(self.cells)), axis=1)
+            cidx = (c[:, 0] * ny + c[:, 1]) * nz + c[:, 2]
+            for s in range(6):
+                t = np.where(ok, cidx * 6 + s, 0)
+                w = self.barycentric(pos, t)
+                pen = (np.maximum(-w, 0).sum(1)).astype(np.float32)
+                hit = ok & (pen <= 1e-4) & (t < first_t)
+                first_t = np.where(hit, t, first_t)
+                first_w[hit] = w[hit]
+                better = ok & ((pen < best_pen) | ((pen == best_pen) & (t < best_t)))
+                best_pen = np.where(better, pen, best_pen)
+                best_t = np.where(better, t, best_t)
+                best_w[better] = w[better]
+        found = first_t != np.iinfo(np.int64).max
+        tet = np.where(found, first_t, best_t).astype(np.uint32)
+        w = np.where(found[:, None], first_w, best_w).astype(np.float32)
+        return tet, np.ascontiguousarray(w)
+
+    def cage(self, vertices: np.ndarray, bind_tet: np.ndarray, bind_w: np.ndarray) -> TetCage:
+        return TetCage(self.elements, self.inv_dm, np.ascontiguousarray(vertices, dtype=np.float32),
+                       np.ascontiguousarray(bind_tet, dtype=np.uint32),
+                       np.ascontiguousarray(bind_w, dtype=np.float32))
+
+
+class CageSpringSimulator:
+    """CageSpringSimulator::step (src/deformation/dynamics.rs:249-277), f32."""
+
+    MAX_DT = 0.033
+
+    def __init__(self, rest: np.ndarray, stiffness: float = 40.0, damping_c: float = 4.0,
+                 mass: float = 1.0):
+        self.rest = np.asarray(rest, dtype=np.float32).copy()
+        self.x = self.rest.copy()
+        self.v = np.zeros_like(self.rest)
+        self.k = np.float32(stiffness)
+        self.c = np.float32(damping_c)  # continuous damping coefficient (1/s)
+        self.mass = np.float32(mass)
+
+    def impulse(self, magnitude: float = 0.2, fraction: float = 0.05, seed: int = 3):
+        """Outward radial velocity pulse on a seeded subset of the vertices."""
+        rng = np.random.Generator(np.random.PCG64(0xC1DE5EED00000000 + seed + 100))
+        n = self.rest.shape[0]
+        pick = rng.choice(n, size=max(1, int(round(fraction * n))), replace=False)
+        centroid = self.rest.mean(0)
+        d = self.rest[pick] - centroid
+        ln = np.linalg.norm(d, axis=1, keepdims=True)
+        d = np.where(ln > 1e-2, d / np.maximum(ln, 1e-12), np.array([0, 1, 0], dtype=np.float32))
+        self.v[pick] += (d * magnitude).astype(np.float32)
+
+    def step(self, dt: float = 1.0 / 60.0):
+        dt = np.float32(min(dt, self.MAX_DT))
+        gamma = np.float32(math.exp(-float(self.c) * float(dt)))
+        inv_m = np.float32(1.0) / max(self.mass, np.float32(0.001))
+        accel = (self.rest - self.x) * (self.k * inv_m)
+        self.v = ((self.v + accel * dt) * gamma).astype(np.float32)
+        self.x = (self.x + self.v * dt).astype(np.float32)
+        return self.x

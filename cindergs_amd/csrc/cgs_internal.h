@@ -2,9 +2,15 @@
 // and the launcher declarations used by the host runtime.
 //
 // Device data layout (all buffers owned by the runtime, see runtime.cpp):
-//   scene (rest pose)   planar SoA: pos[3][cap], cov[6][cap], opa[cap],
-//                       sh[48][n] f32 coefficient planes (or f16 planes)
-//   deformed scene      same planar SoA with cap = 2N when splitting is on
+//   scene (rest pose)   wave-blocked SoA: splats in blocks of 64 (one wave),
+//                       channel-major inside a block, element (i, k) of a
+//                       K-channel array at ((i/64)*K + k)*64 + i%64.
+//                       geo K=10 {x, y, z, cov xx,xy,xz,yy,yz,zz, opacity},
+//                       sh K=48 (f32, or f16 for the storage variant),
+//                       cov_h K=6 f16 (storage variant).  A wave's loads are
+//                       256 contiguous bytes each and stay inside one 2.5 KB
+//                       (geo) + 12 KB (sh) region: few pages, few TLB entries.
+//   deformed scene      geo layout with capacity 2N when splitting is on
 //   projected splat     geom[8] f32 = {mu_x, mu_y, a, b, c, alpha, tz, q_cut}
 //                       color[4] f32 = {r, g, b, 0}; extents u32 (ext_x | ext_y<<16)
 //   stage 3             keys/ids u32[E] (+ ping-pong copies), ranges u32[2T]
@@ -21,6 +27,21 @@
 // Flags of the decoupled look-back status words.
 #define CGS_LB_AGG 1u
 #define CGS_LB_INC 2u
+
+#define CGS_GEO_K 10
+#define CGS_CH_X 0
+#define CGS_CH_COV 3
+#define CGS_CH_OPA 9
+#define CGS_SH_K 48
+
+// index of element (i, k) of a K-channel wave-blocked array
+__host__ __device__ __forceinline__ size_t cgs_blk(uint32_t i, uint32_t k, uint32_t K) {
+  return ((size_t)(i >> 6) * K + k) * 64u + (i & 63u);
+}
+// words of a K-channel wave-blocked array holding n elements
+__host__ __device__ __forceinline__ size_t cgs_blk_words(uint32_t n, uint32_t K) {
+  return (size_t)((n + 63u) / 64u) * K * 64u;
+}
 
 namespace cgs {
 
@@ -147,14 +168,10 @@ __device__ __forceinline__ uint32_t lookback_u32(uint32_t* status, uint32_t part
 // ---------------------------------------------------------------------------
 
 struct CgsSceneDev {
-  const float* pos;    // [3][cap]
-  const float* cov;    // [6][cap] (f32) — null when cov_h is used
-  const uint16_t* cov_h;  // [6][cap] f16 bits (storage variant)
-  const float* opa;    // [cap]
-  const float* sh;     // [48][n_rest] f32 planes — null when sh_h is used
-  const uint16_t* sh_h;   // [48][n_rest] f16 planes (degree-trimmed count used)
-  uint32_t cap;        // plane stride of pos/cov/opa
-  uint32_t sh_stride;  // plane stride of sh (n_rest)
+  const float* geo;       // blocked K=10: x,y,z, cov[6], opacity
+  const uint16_t* cov_h;  // blocked K=6 f16 covariances (storage variant) or null
+  const float* sh;        // blocked K=48 f32, or null when sh_h is used
+  const uint16_t* sh_h;   // blocked K=48 f16 (storage variant)
 };
 
 struct CgsProjectArgs {
@@ -218,12 +235,14 @@ struct CgsProf {
 
 extern "C++" {
 void cgs_launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
-void cgs_launch_deinterleave_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t k,
-                                 uint32_t out_stride, hipStream_t s);
-void cgs_launch_deinterleave_u16(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t k,
-                                 uint32_t out_stride, hipStream_t s);
-void cgs_launch_sh_f16_planes(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t stride,
-                              uint32_t rest, hipStream_t s);
+// per-splat records in[i*k_in + c] -> blocked out[(i, chan0 + c) of k_out channels]
+void cgs_launch_to_blocked_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t k_in,
+                               uint32_t k_out, uint32_t chan0, hipStream_t s);
+void cgs_launch_to_blocked_u16(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t k_in,
+                               uint32_t k_out, uint32_t chan0, hipStream_t s);
+// trimmed channel-major f16 SH records (stride halves) -> blocked K=48 halves
+void cgs_launch_sh_f16_blocked(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t stride,
+                               uint32_t rest, hipStream_t s);
 void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s);
 uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
 void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s);
@@ -285,14 +304,8 @@ struct CgsDeformArgs {
   const float* stretch;
   const uint32_t* bind_tet;  // [n]
   const float* bind_w;       // [4n]
-  const float* rest_pos;     // planar [3][n]
-  const float* rest_cov;     // planar [6][n]
-  const float* rest_opa;     // [n]
-  uint32_t rest_stride;
-  float* out_pos;            // planar [3][cap]
-  float* out_cov;            // planar [6][cap]
-  float* out_opa;            // [cap]
-  uint32_t out_stride;
+  const float* rest_geo;     // blocked K=10
+  float* out_geo;            // blocked K=10, capacity 2n when splitting
   uint32_t* split_meta;      // [2n] or null
   uint32_t* split_count;     // device scalar (written)
   uint64_t* status;          // [parts] zeroed

@@ -7,33 +7,65 @@
 // RGBA8 = round(255 x) packed r | g<<8 | b<<16 | a<<24 into rows padded to
 // 64 pixels (gpu_state.rs:25-27).
 //
-// MI355X design: one 256-thread block (4 wave64s) per tile; each batch of 256
-// splats is gathered once into LDS as two float4 records + one float and read
-// back as wave-wide broadcasts (no bank conflicts); the tile stops as soon as
-// no pixel is alive (__syncthreads_or instead of the reference's
-// double-buffered shared atomic flag).  The conic is stored pre-scaled by
-// -1/2 (and b by -1), which is exact, so the Mahalanobis term q has the same
-// bits as the reference expression; e^q uses the hardware exp2 (the parity
-// bound for colour is 1/255).  Tiles are dealt to blocks so that each of the
-// 8 XCDs (round-robin block placement) gets one contiguous band of tiles,
-// keeping neighbouring tiles' shared splats in the same L2.
+// MI355X design:
+//  * one 256-thread block (4 wave64s) per tile; wave w owns the 8x8 quadrant
+//    (w&1, w>>1) of the tile, one pixel per lane;
+//  * a batch of 256 splats is gathered once into LDS (two float4 + one float
+//    per splat) and, by the loading thread, tested against the 4 quadrants
+//    with the exact ellipse-vs-rectangle test the binning uses
+//    (binning.rs:35-83, on the quadrant's pixel-centre rectangle); the four
+//    results become 4 ballot masks per loading wave;
+//  * each wave then walks only the set bits of its 256-bit mask, in order
+//    (front to back), with a scalar loop: no divergent branches, and the
+//    blend itself is branch-free (select instead of if/break);
+//  * a wave stops when none of its pixels is alive, the tile stops after a
+//    batch in which no pixel is alive (__syncthreads_or);
+//  * the conic is stored pre-scaled by -1/2 (and b by -1), which is exact, so
+//    q has the same bits as the reference expression; e^q uses the hardware
+//    exp2 (the colour parity bound is 1/255);
+//  * tiles are dealt so that each of the 8 XCDs (round-robin block placement)
+//    gets one contiguous band of tiles, keeping shared splats in one L2.
 #include "cgs_internal.h"
 
 namespace {
 
 constexpr int CT = 256;
+constexpr float ALPHA_MIN = 0.003921569f;  // the reference's literal
+
+// ellipse_intersects_tile_gpu (binning.rs:35-83) on an arbitrary pixel rectangle
+__device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float a, float b,
+                                                  float c, float alpha, float qcut, uint32_t xs,
+                                                  uint32_t ys, uint32_t w, uint32_t h,
+                                                  uint32_t W, uint32_t H) {
+  if (alpha < (1.0f / 255.0f) || xs >= W || ys >= H) return false;
+  const uint32_t xe = min(xs + w - 1u, W - 1u), ye = min(ys + h - 1u, H - 1u);
+  const float dx0 = ((float)xs + 0.5f) - mu_x;
+  const float dx1 = ((float)xe + 0.5f) - mu_x;
+  const float dy0 = ((float)ys + 0.5f) - mu_y;
+  const float dy1 = ((float)ye + 0.5f) - mu_y;
+  const float x_near = cgs_fclamp(0.0f, dx0, dx1);
+  const float y1 = cgs_fclamp(-b * x_near / cgs_fmax(c, 1e-12f), dy0, dy1);
+  const float q1 = a * x_near * x_near + 2.0f * b * x_near * y1 + c * y1 * y1;
+  const float y_near = cgs_fclamp(0.0f, dy0, dy1);
+  const float x2 = cgs_fclamp(-b * y_near / cgs_fmax(a, 1e-12f), dx0, dx1);
+  const float q2 = a * x2 * x2 + 2.0f * b * x2 * y_near + c * y_near * y_near;
+  return cgs_fmin(q1, q2) <= qcut + 1e-4f;
+}
 
 __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t per_xcd) {
-  __shared__ float4 s_g[CT];   // -a/2, -b, -c/2, opacity
-  __shared__ float4 s_m[CT];   // mu_x, mu_y, r, g
-  __shared__ float s_b[CT];    // blue
+  __shared__ float4 s_g[CT];  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
+  __shared__ float4 s_m[CT];  // mu_x, mu_y, r, g
+  __shared__ float s_b[CT];   // blue
+  __shared__ uint64_t s_mask[4][CT / 64];  // [quadrant][loading wave]
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
   const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
   if (tile >= total_tiles) return;
   const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
   const uint32_t t = threadIdx.x;
-  const uint32_t px = tx * 16u + (t & 15u), py = ty * 16u + (t >> 4);
+  const uint32_t wave = t >> 6, lane = t & 63u;
+  const uint32_t qx0 = tx * 16u + (wave & 1u) * 8u, qy0 = ty * 16u + (wave >> 1) * 8u;
+  const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
   const bool inside = px < A.width && py < A.height;
 
   const uint32_t rs = A.ranges[tile * 2u], re = A.ranges[tile * 2u + 1u];
@@ -44,45 +76,72 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t p
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   bool alive = inside;
   for (uint32_t b0 = 0; b0 < n; b0 += CT) {
+    // ---- gather the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
+    bool q0 = false, q1 = false, q2 = false, q3 = false;
     if (e < re) {
       const uint32_t g = A.sorted_ids[e];
       const float4* gp = reinterpret_cast<const float4*>(A.geom + (size_t)g * 8);
-      const float4 g0 = gp[0];
-      const float4 g1 = gp[1];
+      const float4 g0 = gp[0];  // mu_x, mu_y, a, b
+      const float4 g1 = gp[1];  // c, alpha, tz, q_cut
       const float4 col = reinterpret_cast<const float4*>(A.color)[g];
-      s_g[t] = make_float4(-0.5f * g0.z, -g0.w, -0.5f * g1.x, g1.y);
+      const float op = g1.y > ALPHA_MIN ? g1.y : 0.0f;
+      s_g[t] = make_float4(-0.5f * g0.z, -g0.w, -0.5f * g1.x, op);
       s_m[t] = make_float4(g0.x, g0.y, col.x, col.y);
       s_b[t] = col.z;
+      const uint32_t bx = tx * 16u, by = ty * 16u;
+      q0 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx, by, 8u, 8u,
+                             A.width, A.height);
+      q1 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx + 8u, by, 8u, 8u,
+                             A.width, A.height);
+      q2 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx, by + 8u, 8u, 8u,
+                             A.width, A.height);
+      q3 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx + 8u, by + 8u, 8u,
+                             8u, A.width, A.height);
+    }
+    const uint64_t m0 = __ballot(q0), m1 = __ballot(q1), m2 = __ballot(q2), m3 = __ballot(q3);
+    if (lane == 0) {
+      s_mask[0][wave] = m0;
+      s_mask[1][wave] = m1;
+      s_mask[2][wave] = m2;
+      s_mask[3][wave] = m3;
     }
     __syncthreads();
-    const uint32_t cnt = min((uint32_t)CT, n - b0);
-    if (alive) {
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const float4 gk = s_g[k];
-        const float op = gk.w;
-        if (!(op > 0.003921569f)) continue;
-        const float4 mk = s_m[k];
-        const float dx = pcx - mk.x;
-        const float dy = pcy - mk.y;
-        // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
-        float q = gk.x * dx * dx + gk.y * dx * dy + gk.z * dy * dy;
-        if (q > 0.0f) q = 0.0f;
-        if (q >= -4.0f) {
-          const float alpha = fminf(0.99f, op * __expf(q));
-          if (alpha >= 0.003921569f) {
-            const float w = alpha * T;
-            cr = __builtin_fmaf(w, mk.z, cr);
-            cg = __builtin_fmaf(w, mk.w, cg);
-            cb = __builtin_fmaf(w, s_b[k], cb);
-            T = T * (1.0f - alpha);
-            if (T < 0.001f) {
-              T = 0.0f;
-              alive = false;
-              break;
-            }
-          }
+
+    // ---- this wave's quadrant: only the splats that can touch it ----
+    if (__ballot(alive) != 0ull) {
+#pragma unroll 1
+      for (uint32_t v = 0; v < (uint32_t)(CT / 64); ++v) {
+        // the mask is wave-uniform; readfirstlane returns int, so widen through
+        // uint32_t (a sign extension would set the upper 32 bits)
+        const uint64_t mv = s_mask[wave][v];
+        const uint32_t mlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mv);
+        const uint32_t mhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(mv >> 32));
+        uint64_t m = ((uint64_t)mhi << 32) | (uint64_t)mlo;
+        while (m != 0ull) {
+          const uint32_t k = v * 64u + (uint32_t)__builtin_ctzll(m);
+          m &= m - 1ull;
+          const float4 gk = s_g[k];
+          const float4 mk = s_m[k];
+          const float bk = s_b[k];
+          const float dx = pcx - mk.x;
+          const float dy = pcy - mk.y;
+          // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
+          float q = gk.x * dx * dx + gk.y * dx * dy + gk.z * dy * dy;
+          q = q > 0.0f ? 0.0f : q;
+          const float alpha = fminf(0.99f, gk.w * __expf(q));
+          const bool ok = alive && q >= -4.0f && alpha >= ALPHA_MIN;
+          const float w = ok ? alpha * T : 0.0f;
+          cr = __builtin_fmaf(w, mk.z, cr);
+          cg = __builtin_fmaf(w, mk.w, cg);
+          cb = __builtin_fmaf(w, bk, cb);
+          const float Tn = T * (1.0f - alpha);
+          const bool sat = ok && Tn < 0.001f;
+          T = ok ? (sat ? 0.0f : Tn) : T;
+          alive = alive && !sat;
+          if (__ballot(alive) == 0ull) break;
         }
+        if (__ballot(alive) == 0ull) break;
       }
     }
     if (!__syncthreads_or(alive)) break;

@@ -11,7 +11,7 @@
 // totals with a decoupled look-back, so child k of the frame is always the
 // k-th splitting parent in index order — the order of the CPU path
 // (src/deformation/cpu_reference.rs:316-377) — and keys stay reproducible.
-// Outputs are planar (one plane per component) for coalesced Stage 2 loads.
+// Outputs use the wave-blocked geo layout read by Stage 2.
 #include "cgs_internal.h"
 
 namespace {
@@ -124,24 +124,20 @@ __device__ __forceinline__ float det3s(float xx, float xy, float xz, float yy, f
   return xx * (yy * zz - yz * yz) - xy * (xy * zz - xz * yz) + xz * (xy * yz - xz * yy);
 }
 
-struct Out7 {
-  float x, y, z, c0, c1, c2, c3, c4, c5, o;
-};
-
 __device__ __forceinline__ void store_splat(const CgsDeformArgs& A, uint32_t i, float x, float y,
                                             float z, float c0, float c1, float c2, float c3,
                                             float c4, float c5, float o) {
-  const size_t S = A.out_stride;
-  A.out_pos[i] = x;
-  A.out_pos[S + i] = y;
-  A.out_pos[2 * S + i] = z;
-  A.out_cov[i] = c0;
-  A.out_cov[S + i] = c1;
-  A.out_cov[2 * S + i] = c2;
-  A.out_cov[3 * S + i] = c3;
-  A.out_cov[4 * S + i] = c4;
-  A.out_cov[5 * S + i] = c5;
-  A.out_opa[i] = o;
+  float* g = A.out_geo;
+  g[cgs_blk(i, 0, CGS_GEO_K)] = x;
+  g[cgs_blk(i, 1, CGS_GEO_K)] = y;
+  g[cgs_blk(i, 2, CGS_GEO_K)] = z;
+  g[cgs_blk(i, 3, CGS_GEO_K)] = c0;
+  g[cgs_blk(i, 4, CGS_GEO_K)] = c1;
+  g[cgs_blk(i, 5, CGS_GEO_K)] = c2;
+  g[cgs_blk(i, 6, CGS_GEO_K)] = c3;
+  g[cgs_blk(i, 7, CGS_GEO_K)] = c4;
+  g[cgs_blk(i, 8, CGS_GEO_K)] = c5;
+  g[cgs_blk(i, 9, CGS_GEO_K)] = o;
 }
 
 __global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
@@ -154,7 +150,7 @@ __global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
   const uint32_t part = s_part;
   const uint32_t idx = part * 256u + t;
   const bool valid = idx < A.num_gaussians;
-  const size_t RS = A.rest_stride;
+  const float* RG = A.rest_geo;
 
   bool split = false;
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 0;
@@ -162,13 +158,14 @@ __global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
   uint32_t tet_raw = 0xFFFFFFFFu;
   if (valid) {
     tet_raw = A.bind_tet[idx];
-    const float s_xx = A.rest_cov[idx], s_xy = A.rest_cov[RS + idx];
-    const float s_xz = A.rest_cov[2 * RS + idx], s_yy = A.rest_cov[3 * RS + idx];
-    const float s_yz = A.rest_cov[4 * RS + idx], s_zz = A.rest_cov[5 * RS + idx];
-    const float alpha = A.rest_opa[idx];
+    const float s_xx = RG[cgs_blk(idx, 3, CGS_GEO_K)], s_xy = RG[cgs_blk(idx, 4, CGS_GEO_K)];
+    const float s_xz = RG[cgs_blk(idx, 5, CGS_GEO_K)], s_yy = RG[cgs_blk(idx, 6, CGS_GEO_K)];
+    const float s_yz = RG[cgs_blk(idx, 7, CGS_GEO_K)], s_zz = RG[cgs_blk(idx, 8, CGS_GEO_K)];
+    const float alpha = RG[cgs_blk(idx, 9, CGS_GEO_K)];
     const float mip = A.mip;
     if (tet_raw >= A.num_tets) {
-      const float x = A.rest_pos[idx], y = A.rest_pos[RS + idx], z = A.rest_pos[2 * RS + idx];
+      const float x = RG[cgs_blk(idx, 0, CGS_GEO_K)], y = RG[cgs_blk(idx, 1, CGS_GEO_K)];
+      const float z = RG[cgs_blk(idx, 2, CGS_GEO_K)];
       if (mip > 0.0f) {
         const float det_orig = cgs_fmax(det3s(s_xx, s_xy, s_xz, s_yy, s_yz, s_zz), 1e-24f);
         const float r_xx = s_xx + mip, r_yy = s_yy + mip, r_zz = s_zz + mip;

@@ -6,8 +6,9 @@
 // for operation (compiled with -ffp-contract=off; ln via cgs_logf) so that the
 // outputs are bit-identical to the CPU restatement.
 //
-// MI355X layout: the scene is planar (one plane per component), so every
-// load instruction of a wave reads 256 contiguous bytes; the outputs are a
+// MI355X layout: the scene is wave-blocked (64 splats per block, channel-major
+// inside), so every load instruction of a wave reads 256 contiguous bytes of
+// one small region; the outputs are a
 // 32-byte geometry record and a 16-byte colour record per splat, both aligned
 // so the compositor's gathers never straddle a cache line.  The reference's
 // surviving_ids compaction is never read downstream (gpu_state.rs:900-938);
@@ -18,15 +19,15 @@ namespace {
 
 template <bool F16>
 __device__ __forceinline__ float ld_cov(const CgsSceneDev& s, uint32_t k, uint32_t i) {
-  if (F16) return (float)__builtin_bit_cast(_Float16, s.cov_h[(size_t)k * s.cap + i]);
-  return s.cov[(size_t)k * s.cap + i];
+  if (F16) return (float)__builtin_bit_cast(_Float16, s.cov_h[cgs_blk(i, k, 6u)]);
+  return s.geo[cgs_blk(i, CGS_CH_COV + k, CGS_GEO_K)];
 }
 
 // SH coefficient k (0..47 in the f32 record order) of splat i.
 template <bool F16>
 __device__ __forceinline__ float ld_sh(const CgsSceneDev& s, uint32_t k, uint32_t i) {
-  if (F16) return (float)__builtin_bit_cast(_Float16, s.sh_h[(size_t)k * s.sh_stride + i]);
-  return s.sh[(size_t)k * s.sh_stride + i];
+  if (F16) return (float)__builtin_bit_cast(_Float16, s.sh_h[cgs_blk(i, k, CGS_SH_K)]);
+  return s.sh[cgs_blk(i, k, CGS_SH_K)];
 }
 
 template <bool F16>
@@ -107,11 +108,13 @@ __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
   const float vw = cam[20], vh = cam[21], near_plane = cam[22];
 
   uint32_t ext_packed = 0u;
-  const float px = S.pos[idx], py = S.pos[(size_t)S.cap + idx], pz = S.pos[2 * (size_t)S.cap + idx];
+  const float px = S.geo[cgs_blk(idx, 0, CGS_GEO_K)];
+  const float py = S.geo[cgs_blk(idx, 1, CGS_GEO_K)];
+  const float pz = S.geo[cgs_blk(idx, 2, CGS_GEO_K)];
   const float tx = w00 * px + w01 * py + w02 * pz + w03;
   const float ty = w10 * px + w11 * py + w12 * pz + w13;
   const float tz = w20 * px + w21 * py + w22 * pz + w23;
-  const float op = S.opa[idx];
+  const float op = S.geo[cgs_blk(idx, CGS_CH_OPA, CGS_GEO_K)];
   do {
     if (tz <= near_plane) break;
     if (op < A.min_opacity) break;
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
 void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s) {
   if (a.max_gaussians == 0) return;
   const uint32_t blocks = (a.max_gaussians + 255u) / 256u;
-  if (a.scene.cov_h || a.scene.sh_h)
+  if (a.scene.sh_h)
     hipLaunchKernelGGL(k_project<true>, dim3(blocks), dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(k_project<false>, dim3(blocks), dim3(256), 0, s, a);

@@ -34,11 +34,28 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
   for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256u) s_h[i] = 0u;
   __syncthreads();
   const uint32_t n = min(d_n ? *d_n : n_static, cap);
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint32_t k = keys[i];
+  // Equal digits within a wave are merged first (8 ballots give the lanes that
+  // share a digit), so each distinct digit costs one LDS atomic per wave:
+  // sorted-ish keys would otherwise serialise on a few bins.
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += stride) {
+    const uint32_t i = base + lane;
+    const bool valid = i < n;
+    const uint32_t k = valid ? keys[i] : 0u;
+    const uint64_t vmask = __ballot(valid);
     for (uint32_t p = 0; p < passes; ++p) {
       const uint32_t sh = p * 8u;
-      atomicAdd(&s_h[p * RADIX + ((k >> sh) & digit_mask(bits, sh))], 1u);
+      const uint32_t d = (k >> sh) & digit_mask(bits, sh);
+      uint64_t m = vmask;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+      }
+      if (valid && cgs::mask_rank(m) == 0u)
+        atomicAdd(&s_h[p * RADIX + d], (uint32_t)__popcll(m));
     }
   }
   __syncthreads();

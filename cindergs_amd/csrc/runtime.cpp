@@ -128,7 +128,7 @@ struct cgs_context {
   uint32_t n_rest = 0;
   bool f16 = false;
   uint32_t f16_degree = 3;
-  DevBuf rest_pos, rest_cov, rest_cov_h, rest_opa, rest_sh, rest_sh_h, staging;
+  DevBuf rest_geo, rest_cov_h, rest_sh, rest_sh_h, staging;  // wave-blocked
 
   // ---- cages ----
   bool has_cages = false;
@@ -141,8 +141,8 @@ struct cgs_context {
   uint64_t deform_generation = 0;
 
   // ---- arena ----
-  DevBuf jac, rot, stretch, def_pos, def_cov, def_opa, split_meta;
-  uint32_t def_stride = 0;
+  DevBuf jac, rot, stretch, def_geo, split_meta;
+  uint32_t def_cap = 0;  // splat capacity of def_geo
   DevBuf ctrl1;  // [split_count, ticket, pad, pad][u64 status...]
   DevBuf geom, color, extents;
   DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
@@ -222,13 +222,15 @@ int ensure_ctx(cgs_context* c) {
     if (r_ != CGS_OK) return r_; \
   } while (0)
 
-int upload_planar_f32(cgs_context* c, DevBuf& dst, const float* src, uint32_t n, uint32_t k,
-                      uint32_t stride) {
-  TRY(ensure_words(dst, (size_t)k * stride));
-  if ((size_t)n * k == 0) return CGS_OK;
-  TRY(ensure_words(c->staging, (size_t)n * k));
-  HIPCHK(hipMemcpyAsync(c->staging.p, src, (size_t)n * k * 4, hipMemcpyHostToDevice, c->stream));
-  cgs_launch_deinterleave_u32(P<uint32_t>(c->staging), P<uint32_t>(dst), n, k, stride, c->stream);
+// host records src[i*k_in + c] -> channels chan0.. of a k_out-channel blocked array
+int upload_blocked_u32(cgs_context* c, DevBuf& dst, const void* src, uint32_t n, uint32_t k_in,
+                       uint32_t k_out, uint32_t chan0) {
+  TRY(ensure_words(dst, cgs_blk_words(n, k_out)));
+  if ((size_t)n * k_in == 0) return CGS_OK;
+  TRY(ensure_words(c->staging, (size_t)n * k_in));
+  HIPCHK(hipMemcpyAsync(c->staging.p, src, (size_t)n * k_in * 4, hipMemcpyHostToDevice, c->stream));
+  cgs_launch_to_blocked_u32(P<uint32_t>(c->staging), P<uint32_t>(dst), n, k_in, k_out, chan0,
+                            c->stream);
   HIPCHK(hipGetLastError());
   return CGS_OK;
 }
@@ -488,10 +490,10 @@ void cgs_destroy(cgs_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->rest_pos, &c->rest_cov, &c->rest_cov_h, &c->rest_opa, &c->rest_sh,
-                    &c->rest_sh_h, &c->staging, &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w,
-                    &c->verts, &c->jac, &c->rot, &c->stretch, &c->def_pos, &c->def_cov,
-                    &c->def_opa, &c->split_meta, &c->ctrl1, &c->geom, &c->color, &c->extents,
+  DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
+                    &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
+                    &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->geom, &c->color,
+                    &c->extents,
                     &c->e_keys, &c->e_ids, &c->e_depths, &c->e_alt_k, &c->e_alt_v, &c->e_perm,
                     &c->ranges, &c->out_rgba, &c->cnt_dbg, &c->off_dbg, &c->ctrl3, &c->prim_ctrl,
                     &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d};
@@ -543,10 +545,10 @@ int cgs_sync_scene(cgs_context* c, const float* pos, const float* cov, const flo
   const uint64_t h = scene_fingerprint(pos, cov, opa, sh, count, 6, 48);
   if (c->scene_known && !c->f16 && c->scene_hash == h && c->n_rest == count) return CGS_OK;
   const uint32_t n = (uint32_t)count;
-  TRY(upload_planar_f32(c, c->rest_pos, pos, n, 3, n));
-  TRY(upload_planar_f32(c, c->rest_cov, cov, n, 6, n));
-  TRY(upload_planar_f32(c, c->rest_opa, opa, n, 1, n));
-  TRY(upload_planar_f32(c, c->rest_sh, sh, n, 48, n));
+  TRY(upload_blocked_u32(c, c->rest_geo, pos, n, 3, CGS_GEO_K, CGS_CH_X));
+  TRY(upload_blocked_u32(c, c->rest_geo, cov, n, 6, CGS_GEO_K, CGS_CH_COV));
+  TRY(upload_blocked_u32(c, c->rest_geo, opa, n, 1, CGS_GEO_K, CGS_CH_OPA));
+  TRY(upload_blocked_u32(c, c->rest_sh, sh, n, 48, CGS_SH_K, 0));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->n_rest = n;
   c->f16 = false;
@@ -571,20 +573,20 @@ int cgs_sync_scene_f16(cgs_context* c, const float* pos, const uint32_t* cov_pac
                                        sh_words) ^ (0x9e3779b97f4a7c15ull * (sh_degree + 1));
   if (c->scene_known && c->f16 && c->scene_hash == h && c->n_rest == count) return CGS_OK;
   const uint32_t n = (uint32_t)count;
-  TRY(upload_planar_f32(c, c->rest_pos, pos, n, 3, n));
-  TRY(upload_planar_f32(c, c->rest_opa, opa, n, 1, n));
-  // covariances: 6 halves per splat -> 6 half planes
-  TRY(ensure_words(c->rest_cov_h, ((size_t)6 * n + 1) / 2));
-  TRY(ensure_words(c->rest_sh_h, ((size_t)48 * n + 1) / 2));
+  TRY(upload_blocked_u32(c, c->rest_geo, pos, n, 3, CGS_GEO_K, CGS_CH_X));
+  TRY(upload_blocked_u32(c, c->rest_geo, opa, n, 1, CGS_GEO_K, CGS_CH_OPA));
+  // covariances: 6 halves per splat -> 6 blocked half channels
+  TRY(ensure_words(c->rest_cov_h, (cgs_blk_words(n, 6) + 1) / 2));
+  TRY(ensure_words(c->rest_sh_h, (cgs_blk_words(n, CGS_SH_K) + 1) / 2));
   if (n) {
     TRY(ensure_words(c->staging, (size_t)n * (sh_words > 3 ? sh_words : 3)));
     HIPCHK(hipMemcpyAsync(c->staging.p, cov_packed, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
-    cgs_launch_deinterleave_u16(P<uint16_t>(c->staging), P<uint16_t>(c->rest_cov_h), n, 6, n,
-                                c->stream);
+    cgs_launch_to_blocked_u16(P<uint16_t>(c->staging), P<uint16_t>(c->rest_cov_h), n, 6, 6, 0,
+                              c->stream);
     HIPCHK(hipMemcpyAsync(c->staging.p, sh_packed, (size_t)n * sh_words * 4, hipMemcpyHostToDevice,
                           c->stream));
-    cgs_launch_sh_f16_planes(P<uint16_t>(c->staging), P<uint16_t>(c->rest_sh_h), n, sh_words * 2,
-                             bases - 1, c->stream);
+    cgs_launch_sh_f16_blocked(P<uint16_t>(c->staging), P<uint16_t>(c->rest_sh_h), n, sh_words * 2,
+                              bases - 1, c->stream);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -728,16 +730,11 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
     TRY(ensure_words(c->jac, (size_t)c->num_tets * 9));
     TRY(ensure_words(c->rot, (size_t)c->num_tets * 9));
     TRY(ensure_words(c->stretch, c->num_tets));
-    if (c->def_stride < max_g) {
-      release(c->def_pos);
-      release(c->def_cov);
-      release(c->def_opa);
-      c->def_stride = max_g;
+    if (c->def_cap < max_g) {
+      c->def_cap = max_g;
       c->deformed_valid = false;
     }
-    TRY(ensure_words(c->def_pos, (size_t)3 * c->def_stride));
-    TRY(ensure_words(c->def_cov, (size_t)6 * c->def_stride));
-    TRY(ensure_words(c->def_opa, c->def_stride));
+    TRY(ensure_words(c->def_geo, cgs_blk_words(c->def_cap, CGS_GEO_K)));
     TRY(ensure_words(c->split_meta, (size_t)2 * N));
     const uint32_t dparts = cgs_deform_num_parts(N);
     TRY(ensure_words(c->ctrl1, 4 + 2 * (size_t)(dparts ? dparts : 1)));
@@ -761,14 +758,8 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
       D.stretch = P<float>(c->stretch);
       D.bind_tet = P<uint32_t>(c->bind_tet);
       D.bind_w = P<float>(c->bind_w);
-      D.rest_pos = P<float>(c->rest_pos);
-      D.rest_cov = P<float>(c->rest_cov);
-      D.rest_opa = P<float>(c->rest_opa);
-      D.rest_stride = N;
-      D.out_pos = P<float>(c->def_pos);
-      D.out_cov = P<float>(c->def_cov);
-      D.out_opa = P<float>(c->def_opa);
-      D.out_stride = c->def_stride;
+      D.rest_geo = P<float>(c->rest_geo);
+      D.out_geo = P<float>(c->def_geo);
       D.split_meta = P<uint32_t>(c->split_meta);
       D.split_count = P<uint32_t>(c->ctrl1);
       D.status = reinterpret_cast<uint64_t*>(P<uint32_t>(c->ctrl1) + 4);
@@ -786,26 +777,17 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
     } else {
       T.stage1_skipped = 1;
     }
-    S.pos = P<float>(c->def_pos);
-    S.cov = P<float>(c->def_cov);
-    S.opa = P<float>(c->def_opa);
-    S.cap = c->def_stride;
+    S.geo = P<float>(c->def_geo);
     split_count = P<uint32_t>(c->ctrl1);
   } else {
     T.stage1_skipped = 1;
-    S.pos = P<float>(c->rest_pos);
-    if (c->f16)
-      S.cov_h = P<uint16_t>(c->rest_cov_h);
-    else
-      S.cov = P<float>(c->rest_cov);
-    S.opa = P<float>(c->rest_opa);
-    S.cap = N;
+    S.geo = P<float>(c->rest_geo);
+    if (c->f16) S.cov_h = P<uint16_t>(c->rest_cov_h);
   }
   if (c->f16)
     S.sh_h = P<uint16_t>(c->rest_sh_h);
   else
     S.sh = P<float>(c->rest_sh);
-  S.sh_stride = N;
   if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
 
   // ---- Stage 3 cache key (gpu_state.rs:956-981) ----
@@ -1002,25 +984,24 @@ int cgs_read_stage3(cgs_context* c, uint32_t* counts, uint32_t* offsets, uint32_
 int cgs_read_deformed(cgs_context* c, float* pos, float* cov, float* opa, uint32_t* split_meta,
                       uint32_t* split_count) {
   TRY(ensure_ctx(c));
-  if (!c->has_cages || !c->def_pos.p) return fail(CGS_ERR_INVALID_ARG, "no deformation has run");
+  if (!c->has_cages || !c->def_geo.p) return fail(CGS_ERR_INVALID_ARG, "no deformation has run");
   uint32_t splits = 0;
   HIPCHK(hipMemcpyAsync(&splits, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  const size_t m = (size_t)c->n_rest + splits;
-  const size_t S = c->def_stride;
-  if (m > S) return fail(CGS_ERR_EXECUTION, "split count exceeds the deformed arena");
-  std::vector<float> p(3 * S), cv(6 * S);
-  HIPCHK(hipMemcpyAsync(p.data(), c->def_pos.p, 3 * S * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(cv.data(), c->def_cov.p, 6 * S * 4, hipMemcpyDeviceToHost, c->stream));
-  if (opa) HIPCHK(hipMemcpyAsync(opa, c->def_opa.p, m * 4, hipMemcpyDeviceToHost, c->stream));
+  const uint32_t m = c->n_rest + splits;
+  if (m > c->def_cap) return fail(CGS_ERR_EXECUTION, "split count exceeds the deformed arena");
+  std::vector<float> g(cgs_blk_words(m, CGS_GEO_K));
+  if (!g.empty())
+    HIPCHK(hipMemcpyAsync(g.data(), c->def_geo.p, g.size() * 4, hipMemcpyDeviceToHost, c->stream));
   if (split_meta && splits)
     HIPCHK(hipMemcpyAsync(split_meta, c->split_meta.p, (size_t)splits * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < m; ++i) {
+  for (uint32_t i = 0; i < m; ++i) {
     if (pos)
-      for (int k = 0; k < 3; ++k) pos[i * 3 + k] = p[k * S + i];
+      for (uint32_t k = 0; k < 3; ++k) pos[(size_t)i * 3 + k] = g[cgs_blk(i, k, CGS_GEO_K)];
     if (cov)
-      for (int k = 0; k < 6; ++k) cov[i * 6 + k] = cv[k * S + i];
+      for (uint32_t k = 0; k < 6; ++k) cov[(size_t)i * 6 + k] = g[cgs_blk(i, 3 + k, CGS_GEO_K)];
+    if (opa) opa[i] = g[cgs_blk(i, CGS_CH_OPA, CGS_GEO_K)];
   }
   if (split_count) *split_count = splits;
   return CGS_OK;

@@ -718,3 +718,313 @@ ORC_API void orc_composite_v2(const void* attrs10, const uint32_t* sorted_ids,
     composite_impl<float, float>((const float*)attrs10, sorted_ids, ranges, W, H, tiles_x,
                                  padded_row, bg4, out, out_f);
 }
+
+// ---------------------------------------------------------------------------
+// Stage 1: compute_tet_transforms_kernel (deformation.rs:682-865) and
+// deform_and_split_compact_kernel (deformation.rs:872-1211), children appended
+// in parent order (cpu_reference.rs:316-377 ordering).
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T>
+static T det3s(T xx, T xy, T xz, T yy, T yz, T zz) {
+  return xx * (yy * zz - yz * yz) - xy * (xy * zz - xz * yz) + xz * (xy * yz - xz * yy);
+}
+
+template <class T>
+static void tet_one(const float* V, const uint32_t* e, const float* M, T J[9], T R[9],
+                    T* stretch) {
+  typedef Mth<T> m;
+  const uint32_t v0 = e[0], v1 = e[1], v2 = e[2], v3 = e[3];
+  T x0x = V[v0 * 3], x0y = V[v0 * 3 + 1], x0z = V[v0 * 3 + 2];
+  T ds00 = (T)V[v1 * 3] - x0x, ds10 = (T)V[v1 * 3 + 1] - x0y, ds20 = (T)V[v1 * 3 + 2] - x0z;
+  T ds01 = (T)V[v2 * 3] - x0x, ds11 = (T)V[v2 * 3 + 1] - x0y, ds21 = (T)V[v2 * 3 + 2] - x0z;
+  T ds02 = (T)V[v3 * 3] - x0x, ds12 = (T)V[v3 * 3 + 1] - x0y, ds22 = (T)V[v3 * 3 + 2] - x0z;
+  T a00 = M[0], a10 = M[1], a20 = M[2], a01 = M[3], a11 = M[4], a21 = M[5];
+  T a02 = M[6], a12 = M[7], a22 = M[8];
+  T j00 = ds00 * a00 + ds01 * a10 + ds02 * a20;
+  T j10 = ds10 * a00 + ds11 * a10 + ds12 * a20;
+  T j20 = ds20 * a00 + ds21 * a10 + ds22 * a20;
+  T j01 = ds00 * a01 + ds01 * a11 + ds02 * a21;
+  T j11 = ds10 * a01 + ds11 * a11 + ds12 * a21;
+  T j21 = ds20 * a01 + ds21 * a11 + ds22 * a21;
+  T j02 = ds00 * a02 + ds01 * a12 + ds02 * a22;
+  T j12 = ds10 * a02 + ds11 * a12 + ds12 * a22;
+  T j22 = ds20 * a02 + ds21 * a12 + ds22 * a22;
+  J[0] = j00; J[1] = j10; J[2] = j20; J[3] = j01; J[4] = j11; J[5] = j21;
+  J[6] = j02; J[7] = j12; J[8] = j22;
+  T c00 = j00 * j00 + j10 * j10 + j20 * j20;
+  T c01 = j00 * j01 + j10 * j11 + j20 * j21;
+  T c02 = j00 * j02 + j10 * j12 + j20 * j22;
+  T c11 = j01 * j01 + j11 * j11 + j21 * j21;
+  T c12 = j01 * j02 + j11 * j12 + j21 * j22;
+  T c22 = j02 * j02 + j12 * j12 + j22 * j22;
+  T p1 = c01 * c01 + c02 * c02 + c12 * c12;
+  if (p1 <= (T)1e-10f) {
+    *stretch = m::sq(m::mx(m::mx(c00, m::mx(c11, c22)), (T)0));
+  } else {
+    T q = (c00 + c11 + c22) / (T)3;
+    T d0 = c00 - q, d1 = c11 - q, d2 = c22 - q;
+    T p2 = d0 * d0 + d1 * d1 + d2 * d2 + (T)2 * p1;
+    T p = m::sq(m::mx(p2 / (T)6, (T)1e-12f));
+    T ip = (T)1 / p;
+    T b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip, b01 = c01 * ip, b02 = c02 * ip, b12 = c12 * ip;
+    T det_b = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
+              b02 * (b01 * b12 - b11 * b02);
+    T r = m::cl((T)0.5 * det_b, (T)-1, (T)1);
+    T phi = m::ac(r) / (T)3;
+    T emax = q + (T)2 * p * m::co(phi);
+    *stretch = m::sq(m::mx(emax, (T)0));
+  }
+  T det_j = j00 * (j11 * j22 - j12 * j21) - j01 * (j10 * j22 - j12 * j20) +
+            j02 * (j10 * j21 - j11 * j20);
+  T adet = m::ab(det_j);
+  if (adet < (T)1e-8f) {
+    for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? (T)1 : (T)0;
+    return;
+  }
+  T gamma = m::ex(-m::ln(adet) / (T)3);
+  T r00 = j00 * gamma, r10 = j10 * gamma, r20 = j20 * gamma;
+  T r01 = j01 * gamma, r11 = j11 * gamma, r21 = j21 * gamma;
+  T r02 = j02 * gamma, r12 = j12 * gamma, r22 = j22 * gamma;
+  for (int it = 0; it < 5; ++it) {
+    T f00 = r11 * r22 - r21 * r12, f10 = r21 * r02 - r01 * r22, f20 = r01 * r12 - r11 * r02;
+    T f01 = r12 * r20 - r22 * r10, f11 = r22 * r00 - r02 * r20, f21 = r02 * r10 - r12 * r00;
+    T f02 = r10 * r21 - r20 * r11, f12 = r20 * r01 - r00 * r21, f22 = r00 * r11 - r10 * r01;
+    T det_r = r00 * f00 + r10 * f10 + r20 * f20;
+    if (m::ab(det_r) > (T)1e-12f) {
+      T inv = (T)1 / det_r;
+      r00 = (T)0.5 * (r00 + f00 * inv);
+      r10 = (T)0.5 * (r10 + f10 * inv);
+      r20 = (T)0.5 * (r20 + f20 * inv);
+      r01 = (T)0.5 * (r01 + f01 * inv);
+      r11 = (T)0.5 * (r11 + f11 * inv);
+      r21 = (T)0.5 * (r21 + f21 * inv);
+      r02 = (T)0.5 * (r02 + f02 * inv);
+      r12 = (T)0.5 * (r12 + f12 * inv);
+      r22 = (T)0.5 * (r22 + f22 * inv);
+    }
+  }
+  T det_f = r00 * (r11 * r22 - r21 * r12) + r10 * (r21 * r02 - r01 * r22) +
+            r20 * (r01 * r12 - r11 * r02);
+  if (det_f < (T)0) {
+    r02 = -r02;
+    r12 = -r12;
+    r22 = -r22;
+  }
+  R[0] = r00; R[1] = r10; R[2] = r20; R[3] = r01; R[4] = r11; R[5] = r21;
+  R[6] = r02; R[7] = r12; R[8] = r22;
+}
+
+template <class T>
+struct Splat10 {
+  T x, y, z, c[6], o;
+};
+
+// One splat of deform_and_split_compact_kernel.  Returns true when it splits;
+// `a` receives the splat (or first child), `b` the second child.
+template <class T>
+static bool deform_one(const float* V, const uint32_t* elems, const T* jac, const T* stretch,
+                       uint32_t tet_raw, const float* w4, const float* pos, const float* cov,
+                       float opa, float tau, float mip, uint32_t num_tets, Splat10<T>& a,
+                       Splat10<T>& b) {
+  typedef Mth<T> m;
+  T s_xx = cov[0], s_xy = cov[1], s_xz = cov[2], s_yy = cov[3], s_yz = cov[4], s_zz = cov[5];
+  T alpha = opa;
+  T mp = (T)mip;
+  if (tet_raw >= num_tets) {
+    a.x = pos[0];
+    a.y = pos[1];
+    a.z = pos[2];
+    if (mp > (T)0) {
+      T det_orig = m::mx(det3s(s_xx, s_xy, s_xz, s_yy, s_yz, s_zz), (T)1e-24f);
+      T r_xx = s_xx + mp, r_yy = s_yy + mp, r_zz = s_zz + mp;
+      T det_reg = m::mx(det3s(r_xx, s_xy, s_xz, r_yy, s_yz, r_zz), det_orig);
+      T comp = m::sq(m::cl(det_orig / det_reg, (T)0, (T)1));
+      T cc[6] = {r_xx, s_xy, s_xz, r_yy, s_yz, r_zz};
+      for (int k = 0; k < 6; ++k) a.c[k] = cc[k];
+      a.o = m::cl(alpha * comp, (T)0, (T)1);
+    } else {
+      T cc[6] = {s_xx, s_xy, s_xz, s_yy, s_yz, s_zz};
+      for (int k = 0; k < 6; ++k) a.c[k] = cc[k];
+      a.o = alpha;
+    }
+    return false;
+  }
+  T w0 = w4[0], w1 = w4[1], w2 = w4[2], w3 = w4[3];
+  const uint32_t* E = elems + (size_t)tet_raw * 4;
+  uint32_t v0 = E[0], v1 = E[1], v2 = E[2], v3 = E[3];
+  T px = w0 * (T)V[v0 * 3] + w1 * (T)V[v1 * 3] + w2 * (T)V[v2 * 3] + w3 * (T)V[v3 * 3];
+  T py = w0 * (T)V[v0 * 3 + 1] + w1 * (T)V[v1 * 3 + 1] + w2 * (T)V[v2 * 3 + 1] + w3 * (T)V[v3 * 3 + 1];
+  T pz = w0 * (T)V[v0 * 3 + 2] + w1 * (T)V[v1 * 3 + 2] + w2 * (T)V[v2 * 3 + 2] + w3 * (T)V[v3 * 3 + 2];
+  const T* J = jac + (size_t)tet_raw * 9;
+  T j00 = J[0], j10 = J[1], j20 = J[2], j01 = J[3], j11 = J[4], j21 = J[5];
+  T j02 = J[6], j12 = J[7], j22 = J[8];
+  T k00 = j00 * s_xx + j01 * s_xy + j02 * s_xz;
+  T k01 = j00 * s_xy + j01 * s_yy + j02 * s_yz;
+  T k02 = j00 * s_xz + j01 * s_yz + j02 * s_zz;
+  T k10 = j10 * s_xx + j11 * s_xy + j12 * s_xz;
+  T k11 = j10 * s_xy + j11 * s_yy + j12 * s_yz;
+  T k12 = j10 * s_xz + j11 * s_yz + j12 * s_zz;
+  T k20 = j20 * s_xx + j21 * s_xy + j22 * s_xz;
+  T k21 = j20 * s_xy + j21 * s_yy + j22 * s_yz;
+  T k22 = j20 * s_xz + j21 * s_yz + j22 * s_zz;
+  T m00 = m::mx((T)1e-4f, k00 * j00 + k01 * j01 + k02 * j02);
+  T m01 = (T)0.5 * ((k00 * j10 + k01 * j11 + k02 * j12) + (k10 * j00 + k11 * j01 + k12 * j02));
+  T m02 = (T)0.5 * ((k00 * j20 + k01 * j21 + k02 * j22) + (k20 * j00 + k21 * j01 + k22 * j02));
+  T m11 = m::mx((T)1e-4f, k10 * j10 + k11 * j11 + k12 * j12);
+  T m12 = (T)0.5 * ((k10 * j20 + k11 * j21 + k12 * j22) + (k20 * j10 + k21 * j11 + k22 * j12));
+  T m22 = m::mx((T)1e-4f, k20 * j20 + k21 * j21 + k22 * j22);
+  T det_j = j00 * (j11 * j22 - j12 * j21) - j01 * (j10 * j22 - j12 * j20) +
+            j02 * (j10 * j21 - j11 * j20);
+  T s_det = m::cl(det_j, (T)0.6f, (T)1.8f);
+  T safe_base = m::cl((T)1 - alpha, (T)1e-4f, (T)0.999f);
+  T alpha_p = m::cl((T)1 - m::pw(safe_base, (T)1 / s_det), (T)0.05f, (T)0.99f);
+  bool split = tau > 0.0f && stretch[tet_raw] > (T)tau;
+  if (!split) {
+    a.x = px;
+    a.y = py;
+    a.z = pz;
+    if (mp > (T)0) {
+      T det_orig = m::mx(det3s(m00, m01, m02, m11, m12, m22), (T)1e-24f);
+      T r00 = m00 + mp, r11 = m11 + mp, r22 = m22 + mp;
+      T det_reg = m::mx(det3s(r00, m01, m02, r11, m12, r22), det_orig);
+      T comp = m::sq(m::cl(det_orig / det_reg, (T)0, (T)1));
+      T cc[6] = {r00, m01, m02, r11, m12, r22};
+      for (int k = 0; k < 6; ++k) a.c[k] = cc[k];
+      a.o = m::cl(alpha_p * comp, (T)0, (T)1);
+    } else {
+      T cc[6] = {m00, m01, m02, m11, m12, m22};
+      for (int k = 0; k < 6; ++k) a.c[k] = cc[k];
+      a.o = alpha_p;
+    }
+    return false;
+  }
+  T p1 = m01 * m01 + m02 * m02 + m12 * m12;
+  T lambda0 = m::mx(m00, m::mx(m11, m22));
+  T ux = 1, uy = 0, uz = 0;
+  if (p1 <= (T)1e-10f) {
+    if (m11 >= m00 && m11 >= m22) {
+      ux = 0; uy = 1; uz = 0;
+    } else if (m22 >= m00 && m22 >= m11) {
+      ux = 0; uy = 0; uz = 1;
+    }
+  } else {
+    T q = (m00 + m11 + m22) / (T)3;
+    T d0 = m00 - q, d1 = m11 - q, d2 = m22 - q;
+    T p2 = d0 * d0 + d1 * d1 + d2 * d2 + (T)2 * p1;
+    T p = m::sq(m::mx(p2 / (T)6, (T)1e-12f));
+    T ip = (T)1 / p;
+    T b00 = d0 * ip, b11 = d1 * ip, b22 = d2 * ip, b01 = m01 * ip, b02 = m02 * ip, b12 = m12 * ip;
+    T det_b = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
+              b02 * (b01 * b12 - b11 * b02);
+    T r = m::cl((T)0.5 * det_b, (T)-1, (T)1);
+    T phi = m::ac(r) / (T)3;
+    lambda0 = q + (T)2 * p * m::co(phi);
+    T r0x = m00 - lambda0, r0y = m01, r0z = m02;
+    T r1x = m01, r1y = m11 - lambda0, r1z = m12;
+    T r2x = m02, r2y = m12, r2z = m22 - lambda0;
+    T c01x = r0y * r1z - r0z * r1y, c01y = r0z * r1x - r0x * r1z, c01z = r0x * r1y - r0y * r1x;
+    T n01 = c01x * c01x + c01y * c01y + c01z * c01z;
+    T c12x = r1y * r2z - r1z * r2y, c12y = r1z * r2x - r1x * r2z, c12z = r1x * r2y - r1y * r2x;
+    T n12 = c12x * c12x + c12y * c12y + c12z * c12z;
+    T c02x = r0y * r2z - r0z * r2y, c02y = r0z * r2x - r0x * r2z, c02z = r0x * r2y - r0y * r2x;
+    T n02 = c02x * c02x + c02y * c02y + c02z * c02z;
+    if (n01 >= n12 && n01 >= n02 && n01 > (T)1e-12f) {
+      T inv = (T)1 / m::sq(n01);
+      ux = c01x * inv; uy = c01y * inv; uz = c01z * inv;
+    } else if (n12 >= n02 && n12 > (T)1e-12f) {
+      T inv = (T)1 / m::sq(n12);
+      ux = c12x * inv; uy = c12y * inv; uz = c12z * inv;
+    } else if (n02 > (T)1e-12f) {
+      T inv = (T)1 / m::sq(n02);
+      ux = c02x * inv; uy = c02y * inv; uz = c02z * inv;
+    }
+  }
+  T safe_l0 = m::mx(lambda0, (T)1e-8f);
+  T mag = (T)0.5 * m::sq(safe_l0);
+  T dx = ux * mag, dy = uy * mag, dz = uz * mag;
+  T shrink = (T)0.75 * safe_l0;
+  T c00 = m::mx((T)1e-8f, m00 - shrink * ux * ux);
+  T c01 = m01 - shrink * ux * uy;
+  T c02 = m02 - shrink * ux * uz;
+  T c11 = m::mx((T)1e-8f, m11 - shrink * uy * uy);
+  T c12 = m12 - shrink * uy * uz;
+  T c22 = m::mx((T)1e-8f, m22 - shrink * uz * uz);
+  T child_alpha = alpha_p;
+  if (mp > (T)0) {
+    T det_orig = m::mx(det3s(c00, c01, c02, c11, c12, c22), (T)1e-24f);
+    c00 += mp;
+    c11 += mp;
+    c22 += mp;
+    T det_reg = m::mx(det3s(c00, c01, c02, c11, c12, c22), det_orig);
+    T comp = m::sq(m::cl(det_orig / det_reg, (T)0, (T)1));
+    child_alpha = m::cl(child_alpha * comp, (T)0, (T)1);
+  }
+  T cc[6] = {c00, c01, c02, c11, c12, c22};
+  a.x = px - dx; a.y = py - dy; a.z = pz - dz;
+  b.x = px + dx; b.y = py + dy; b.z = pz + dz;
+  for (int k = 0; k < 6; ++k) a.c[k] = b.c[k] = cc[k];
+  a.o = b.o = child_alpha;
+  return true;
+}
+
+template <class T>
+static uint32_t deform_all(const float* V, const uint32_t* elems, const float* inv_dm,
+                           uint32_t num_tets, const uint32_t* bind_tet, const float* bind_w,
+                           const float* pos, const float* cov, const float* opa, uint32_t n,
+                           float tau, float mip, T* J, T* R, T* stretch, T* out_pos, T* out_cov,
+                           T* out_opa, uint32_t* split_meta) {
+#pragma omp parallel for schedule(static, 64) num_threads(g_threads)
+  for (int64_t t = 0; t < (int64_t)num_tets; ++t)
+    tet_one<T>(V, elems + t * 4, inv_dm + t * 9, J + t * 9, R + t * 9, stretch + t);
+  std::vector<uint8_t> flag(n);
+  std::vector<Splat10<T>> child(n);
+#pragma omp parallel for schedule(static, 1024) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    Splat10<T> a, b;
+    bool sp = deform_one<T>(V, elems, J, stretch, bind_tet[ii], bind_w + ii * 4, pos + ii * 3,
+                            cov + ii * 6, opa[ii], tau, mip, num_tets, a, b);
+    out_pos[ii * 3] = a.x;
+    out_pos[ii * 3 + 1] = a.y;
+    out_pos[ii * 3 + 2] = a.z;
+    for (int k = 0; k < 6; ++k) out_cov[ii * 6 + k] = a.c[k];
+    out_opa[ii] = a.o;
+    flag[ii] = sp ? 1 : 0;
+    if (sp) child[ii] = b;
+  }
+  uint32_t splits = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!flag[i]) continue;
+    const size_t c = (size_t)n + splits;
+    out_pos[c * 3] = child[i].x;
+    out_pos[c * 3 + 1] = child[i].y;
+    out_pos[c * 3 + 2] = child[i].z;
+    for (int k = 0; k < 6; ++k) out_cov[c * 6 + k] = child[i].c[k];
+    out_opa[c] = child[i].o;
+    split_meta[splits * 2] = i;
+    split_meta[splits * 2 + 1] = bind_tet[i];
+    splits++;
+  }
+  return splits;
+}
+
+}  // namespace
+
+// Stage 1.  Outputs in the reference layouts, sized for 2n splats (positions
+// [3*2n], covariances [6*2n], opacities [2n], split_meta [2n]); J/R/stretch
+// per tet.  fp64 selects the FP64 twin (all float outputs are then double).
+// Returns the number of split children.
+ORC_API uint32_t orc_deform(const float* verts, const uint32_t* elems, const float* inv_dm,
+                            uint32_t num_tets, const uint32_t* bind_tet, const float* bind_w,
+                            const float* pos, const float* cov, const float* opa, uint32_t n,
+                            float tau, float mip, int fp64, void* J, void* R, void* stretch,
+                            void* out_pos, void* out_cov, void* out_opa, uint32_t* split_meta) {
+  if (fp64)
+    return deform_all<double>(verts, elems, inv_dm, num_tets, bind_tet, bind_w, pos, cov, opa, n,
+                              tau, mip, (double*)J, (double*)R, (double*)stretch,
+                              (double*)out_pos, (double*)out_cov, (double*)out_opa, split_meta);
+  return deform_all<float>(verts, elems, inv_dm, num_tets, bind_tet, bind_w, pos, cov, opa, n, tau,
+                           mip, (float*)J, (float*)R, (float*)stretch, (float*)out_pos,
+                           (float*)out_cov, (float*)out_opa, split_meta);
+}

@@ -219,3 +219,72 @@ def render_frame(scene, cam, comp, sh_degree=None, mip=0.0, min_opacity=1.0 / 25
     out.update(sorted_keys=sk, sorted_ids=sid, ranges=ranges)
     out["rgba"] = composite(attrs, sid, ranges, W, H, list(comp.background_color))
     return out
+
+
+def deform(cage, scene, tau=0.0, mip=0.0, fp64=False):
+    """Stage 1 restated (tet transforms + deform/split, children in parent order).
+
+    Returns dict(J, R, stretch, pos[m,3], cov[m,6], opa[m], split_meta[s,2], splits)."""
+    L = lib()
+    if not hasattr(L, "_deform_sig"):
+        L.orc_deform.argtypes = [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _f, _f, _i,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.orc_deform.restype = _u32
+        L._deform_sig = True
+    dt = np.float64 if fp64 else np.float32
+    n = scene.count
+    T = cage.elements.shape[0]
+    J = np.zeros((T, 9), dtype=dt)
+    R = np.zeros((T, 9), dtype=dt)
+    st = np.zeros(T, dtype=dt)
+    pos = np.zeros((2 * n, 3), dtype=dt)
+    cov = np.zeros((2 * n, 6), dtype=dt)
+    opa = np.zeros(2 * n, dtype=dt)
+    meta = np.zeros((max(n, 1), 2), dtype=np.uint32)
+    s = L.orc_deform(_p(np.ascontiguousarray(cage.vertices, dtype=np.float32)),
+                     _p(np.ascontiguousarray(cage.elements, dtype=np.uint32)),
+                     _p(np.ascontiguousarray(cage.inv_dm, dtype=np.float32)), T,
+                     _p(np.ascontiguousarray(cage.bind_tet, dtype=np.uint32)),
+                     _p(np.ascontiguousarray(cage.bind_weights, dtype=np.float32)),
+                     _p(scene.positions), _p(scene.covariances_3d), _p(scene.opacities), n, tau,
+                     mip, 1 if fp64 else 0, _p(J), _p(R), _p(st), _p(pos), _p(cov), _p(opa),
+                     _p(meta))
+    m = n + s
+    return {"J": J, "R": R, "stretch": st, "pos": pos[:m], "cov": cov[:m], "opa": opa[:m],
+            "split_meta": meta[:s], "splits": s}
+
+
+def render_frame_deformed(scene, cage, cam, comp, tau=0.0, mip=0.0, min_opacity=1.0 / 255.0,
+                          min_radius=0.5):
+    """Stages 1-4 of execute_frame_gpu with a cage (Stage 2 mip = 0 with cages,
+    gpu_state.rs:894; max_gaussians = 2N iff tau > 0, 706-713)."""
+    from cindergs_amd.synthetic import camera_array
+
+    d = deform(cage, scene, tau, mip)
+    n = scene.count
+    maxg = 2 * n if tau > 0 else n
+    W, H = max(1, comp.screen_width), max(1, comp.screen_height)
+    attrs, ext, surv = project(scene, camera_array(cam), list(comp.camera_position),
+                               comp.sh_degree, 0.0, min_opacity, min_radius,
+                               bind_tet=np.ascontiguousarray(cage.bind_tet, dtype=np.uint32),
+                               tet_rot=np.ascontiguousarray(d["R"], dtype=np.float32),
+                               split_meta=np.ascontiguousarray(d["split_meta"].reshape(-1)) if d["splits"] else np.zeros(2, np.uint32),
+                               split_count=d["splits"], num_tets=cage.elements.shape[0],
+                               max_gaussians=maxg, positions=np.ascontiguousarray(
+                                   np.concatenate([d["pos"], np.zeros((maxg - d["pos"].shape[0], 3), np.float32)])),
+                               covariances=np.ascontiguousarray(
+                                   np.concatenate([d["cov"], np.zeros((maxg - d["cov"].shape[0], 6), np.float32)])),
+                               opacities=np.ascontiguousarray(
+                                   np.concatenate([d["opa"], np.zeros(maxg - d["opa"].shape[0], np.float32)])))
+    active = n + d["splits"]
+    counts = count_tiles(attrs, ext, W, H, active)
+    offsets, total = exclusive_scan(counts)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    tb, _ = packed_key_bits(T)
+    keys, ids, db = emit32(attrs, ext, offsets, total, W, H, cam.near_plane, active)
+    sk, sid = sort_pairs32(keys, ids, min(32, tb + db))
+    ranges = tile_ranges(sk, db, T)
+    rgba = composite(attrs, sid, ranges, W, H, list(comp.background_color))
+    return {"deform": d, "attrs": attrs, "extents": ext, "survivors": surv, "counts": counts,
+            "offsets": offsets, "total": total, "sorted_keys": sk, "sorted_ids": sid,
+            "ranges": ranges, "rgba": rgba}

@@ -1,0 +1,118 @@
+"""GPU parity of Stage 1 (cage deformation) and of full cage-deformed frames.
+
+Bar: per-tet J/R/stretch and deformed positions/covariances/opacities
+bit-exact against the FP32 oracle (same f32 operation order, deterministic
+split order); the FP32 error against the FP64 twin stays within 1e-5
+relative; keys/ranges bit-exact and RGBA within 1/255 on the deformed frame.
+Mirrors test_deformation_kernel_matches_cpu_reference /
+test_gpu_deform_with_splitting_matches_cpu_strain_detection
+(tests/gpu_parity_tests.rs:165-228, 314-371) with tighter bars.
+"""
+import numpy as np
+import pytest
+
+from cindergs_amd import synthetic as syn
+from cindergs_amd.cages import CageSpringSimulator, TetMesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _cage_for(scene, steps=10, impulse=0.2):
+    lo, hi = scene.positions.min(0), scene.positions.max(0)
+    mesh = TetMesh.jittered_grid(lo, hi, (10, 10, 10))
+    tet, w = mesh.bind_gaussians(scene.positions)
+    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim.impulse(impulse, 0.05)
+    for _ in range(steps):
+        sim.step()
+    return mesh, mesh.cage(sim.x, tet, w)
+
+
+@pytest.mark.parametrize("tau,mip", [(0.0, 0.0), (1.02, 0.0), (1.02, 1e-4), (0.0, 2e-4)])
+def test_deformation_matches_oracle(gpu, oracle_lib, tau, mip):
+    import oracle
+
+    scene = syn.scene_gaussian_ball(40_000, k=21)
+    mesh, cage = _cage_for(scene)
+    cam, comp = syn.view_config1()
+    print(f"[case] deform tau={tau} mip={mip}", flush=True)
+    gpu.set_capture(True)
+    t = gpu.execute_frame_gpu(scene, cage, cam, comp, strain_split_threshold=tau,
+                              mip_filter_3d_var=mip, force_stage3=True)
+    gpu.set_capture(False)
+    assert t.stage1_skipped == 0
+    ref = oracle.render_frame_deformed(scene, cage, cam, comp, tau=tau, mip=mip)
+    d = ref["deform"]
+    J, R, s = gpu.read_tet_transforms(mesh.num_tets)
+    np.testing.assert_array_equal(J.view(np.uint32), d["J"].view(np.uint32))
+    np.testing.assert_array_equal(R.view(np.uint32), d["R"].view(np.uint32))
+    np.testing.assert_array_equal(s.view(np.uint32), d["stretch"].view(np.uint32))
+    pos, cov, opa, meta = gpu.read_deformed(scene.count)
+    assert meta.shape[0] == d["splits"]
+    if tau > 0:
+        assert d["splits"] > 0
+    np.testing.assert_array_equal(meta, d["split_meta"])
+    np.testing.assert_array_equal(pos.view(np.uint32), d["pos"].view(np.uint32))
+    np.testing.assert_array_equal(cov.view(np.uint32), d["cov"].view(np.uint32))
+    np.testing.assert_array_equal(opa.view(np.uint32), d["opa"].view(np.uint32))
+    # FP32 against the FP64 twin: within 1e-5 relative (the contract's bar)
+    d64 = oracle.deform(cage, scene, tau, mip, fp64=True)
+    if d64["splits"] == d["splits"]:
+        scale_p = np.abs(d64["pos"]).max()
+        assert np.abs(pos - d64["pos"]).max() / scale_p < 1e-5
+        cn = np.abs(d64["cov"]).max(1, keepdims=True)
+        assert (np.abs(cov - d64["cov"]) / cn).max() < 1e-5
+    # the frame rendered from the deformed splats
+    n_act = scene.count + d["splits"]
+    attrs, ext = gpu.read_projected(n_act)
+    np.testing.assert_array_equal(ext, ref["extents"][:n_act])
+    alive = ext != 0
+    np.testing.assert_array_equal(attrs[alive].view(np.uint32),
+                                  ref["attrs"][:n_act][alive].view(np.uint32))
+    s3 = gpu.read_stage3(2 * scene.count if tau > 0 else scene.count,
+                         comp.tiles_x * comp.tiles_y, with_counts=True)
+    np.testing.assert_array_equal(s3["keys"], ref["sorted_keys"])
+    np.testing.assert_array_equal(s3["ids"], ref["sorted_ids"])
+    np.testing.assert_array_equal(s3["ranges"], ref["ranges"])
+    img = gpu.readback_framebuffer_rgba8(800, 600)
+    assert np.abs(img.astype(np.int16) - ref["rgba"].astype(np.int16)).max() <= 1
+
+
+def test_identity_cage_leaves_splats_in_place(gpu, oracle_lib):
+    """integration_tests.rs:1146-1200: a cage at rest gives J = I, R = I and the
+    rest positions / covariances (with the 1e-4 diagonal floor), opacity by the
+    density law at det J = 1 (alpha' = clamp(alpha, .05, .99) up to pow rounding)."""
+    scene = syn.scene_gaussian_ball(5_000, k=22)
+    mesh, _ = _cage_for(scene, steps=0, impulse=0.0)
+    tet, w = mesh.bind_gaussians(scene.positions)
+    cage = mesh.cage(mesh.rest_vertices, tet, w)
+    cam, comp = syn.view_config1()
+    gpu.execute_frame_gpu(scene, cage, cam, comp)
+    J, R, _ = gpu.read_tet_transforms(mesh.num_tets)
+    eye = np.eye(3, dtype=np.float32).T.reshape(-1)
+    assert np.abs(J - eye).max() < 1e-5
+    assert np.abs(R - eye).max() < 1e-5
+    pos, cov, opa, _ = gpu.read_deformed(scene.count)
+    assert np.abs(pos - scene.positions).max() < 1e-5
+    expect = scene.covariances_3d.copy()
+    for k in (0, 3, 5):
+        expect[:, k] = np.maximum(expect[:, k], 1e-4)
+    assert np.abs(cov - expect).max() < 1e-8
+    np.testing.assert_allclose(opa, np.clip(scene.opacities, 0.05, 0.99), rtol=2e-5, atol=1e-6)
+
+
+def test_stage1_skipped_when_cage_static(gpu):
+    """gpu_state.rs:300-343, 852-859: unchanged cage vertices and (tau, mip) skip
+    Stage 1; moving one vertex re-runs it."""
+    scene = syn.scene_gaussian_ball(5_000, k=23)
+    mesh, cage = _cage_for(scene, steps=3)
+    cam, comp = syn.view_config1()
+    t1 = gpu.execute_frame_gpu(scene, cage, cam, comp)
+    t2 = gpu.execute_frame_gpu(scene, cage, cam, comp)
+    assert t2.stage1_skipped == 1 and t2.stage3_cached == 1
+    moved = cage.vertices.copy()
+    moved[5] += 0.01
+    cage2 = mesh.cage(moved, cage.bind_tet, cage.bind_weights)
+    t3 = gpu.execute_frame_gpu(scene, cage2, cam, comp)
+    assert t3.stage1_skipped == 0 and t3.stage3_cached == 0
+    assert t1.stage1_skipped == 0
