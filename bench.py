@@ -83,7 +83,8 @@ def main() -> int:
     ap.add_argument("--splats", type=int, default=1_073_930)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel-detail", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -93,6 +94,21 @@ def main() -> int:
 
     from cindergs_amd import FrameParams, GpuPipelineState, StageTimings
     from cindergs_amd import synthetic as syn
+
+    if args.calibrate_copy:
+        import ctypes as C
+
+        st = GpuPipelineState(0)
+        nbytes = args.calibrate_copy << 20
+        src = st.to_device(np.zeros(nbytes // 4, dtype=np.uint32))
+        dst = st.to_device(np.zeros(nbytes // 4, dtype=np.uint32))
+        for _ in range(5):
+            st._lib.cgs_memcpy_d2d(st._h, dst.ptr, src.ptr, C.c_uint64(nbytes))
+        print(json.dumps({"calibration_copy_bytes": nbytes, "copies": 5}), flush=True)
+        src.free()
+        dst.free()
+        st.close()
+        return 0
 
     t0 = time.perf_counter()
     scene = syn.scene_clusters(args.splats)
@@ -152,7 +168,7 @@ def main() -> int:
     # roofline of the HBM-bound kernel with the largest device time
     hbm_kernels = {"k_project": B["k_project"], "k_radix_onesweep": 16 * E,
                    "k_bin": 20 * N + 56 * S + 8 * E}
-    dom = max(hbm_kernels, key=lambda k: kern.get(k, 0.0) * launches.get(k, 1.0))
+    dom = max(hbm_kernels, key=lambda k: kern.get(k, 0.0))
     per_launch_ms = kern[dom] / launches[dom]
     achieved = hbm_kernels[dom] / (per_launch_ms * 1e-3) / 1e9
     stage_gbs = {}

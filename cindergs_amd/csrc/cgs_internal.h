@@ -193,7 +193,6 @@ struct CgsProjectArgs {
   float* geom;       // [8*max]
   float* color;      // [4*max]
   uint32_t* extents; // [max]
-  uint32_t* survivors;  // device scalar (accumulated)
 };
 
 struct CgsBinArgs {
@@ -212,9 +211,16 @@ struct CgsBinArgs {
   uint32_t* ids;
   uint32_t* counts_out;      // optional [max] (per-splat tile counts)
   uint32_t* offsets_out;     // optional [max] (exclusive scan)
-  uint32_t* total;           // device scalar: total entries (may exceed cap)
+  uint32_t* total;           // device scalars: [0] total entries (may exceed cap),
+                             // [1] survivors (splats with nonzero extents)
   uint64_t* status;          // [num_parts] zeroed
   uint32_t* ticket;          // zeroed
+  uint32_t* hist_partial;    // optional [num_parts][hist_words]: digit histograms of the
+                             // emitted keys (32-bit mode: 4 passes of the key; 64-bit
+                             // mode: 4 passes of the depth bits, then the tile passes)
+  uint32_t hist_words;       // 1024 (32-bit) or 2048 (64-bit)
+  uint32_t sort_bits;        // 32-bit mode: bits of the packed key
+  uint32_t tile_bits;        // 64-bit mode
 };
 
 // Optional per-launch timing: mark(name) records an event before a launch;
@@ -257,16 +263,22 @@ struct CgsRadixBuffers {
   uint32_t* vals;
   uint32_t* keys_alt;
   uint32_t* vals_alt;
-  uint32_t* hist;      // [4][256]
-  uint32_t* status;    // [4][max_parts][256]
-  uint32_t* tickets;   // [4]
+  uint32_t* hist;          // [4][256] digit histograms
+  uint32_t* hist_partial;  // [cgs_radix_hist_blocks()][4*256] (when computed here)
+  uint32_t* status;        // [4][max_parts][256] zeroed
+  uint32_t* tickets;       // [4] zeroed
   uint32_t cap;
 };
 uint32_t cgs_radix_num_parts(uint32_t cap);
+uint32_t cgs_radix_hist_blocks();
 size_t cgs_radix_control_words(uint32_t cap);
-// Returns 1 if the sorted result is in the *_alt buffers, else 0.
+// hist[b] = sum_r partial[r*row_words + b] for b < bins
+void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row_words,
+                            uint32_t bins, uint32_t* hist, hipStream_t s);
+// Returns 1 if the sorted result is in the *_alt buffers, else 0.  With
+// hist_given the caller has already filled b.hist for the passes.
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
-                          uint32_t bits, bool iota_vals, bool zero_control, hipStream_t s,
+                          uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
                           CgsProf* prof = nullptr);
 void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,

@@ -20,7 +20,7 @@
 namespace {
 
 constexpr int BT = 256;  // threads per partition
-constexpr int BI = 8;    // splats per thread
+constexpr int BI = 16;   // splats per thread
 constexpr int BP = BT * BI;
 
 struct TileBox {
@@ -79,14 +79,28 @@ __device__ __forceinline__ Splat2D load_splat(const float* geom, uint32_t i) {
   return s;
 }
 
+__device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift) {
+  const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
+  return (1u << w) - 1u;
+}
+
+// digit histograms of one emitted entry for every radix pass that will sort it
+__device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_t bits) {
+  for (uint32_t p = 0; p * 8u < bits; ++p)
+    atomicAdd(&s_hist[p * 256u + ((key >> (p * 8u)) & dmask(bits, p * 8u))], 1u);
+}
+
 __global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
   __shared__ uint32_t s_part;
   __shared__ uint32_t s_val[BP];
   __shared__ uint32_t s_tmp[BT / 64];
   __shared__ uint64_t s_base;
+  __shared__ uint32_t s_hist[2048];
 
   const uint32_t t = threadIdx.x;
   if (t == 0) s_part = atomicAdd(A.ticket, 1u);
+  if (A.hist_partial)
+    for (uint32_t i = t; i < A.hist_words; i += BT) s_hist[i] = 0u;
   __syncthreads();
   const uint32_t part = s_part;
   const uint32_t base = part * (uint32_t)BP;
@@ -94,13 +108,15 @@ __global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
   const bool tiles_ok = A.tiles_x != 0u && A.tiles_y != 0u;
 
   // 1) counts, striped so that the loads of a wave are contiguous
+  uint32_t surv = 0;
 #pragma unroll 1
   for (int i = 0; i < BI; ++i) {
     const uint32_t s = base + (uint32_t)i * BT + t;
     uint32_t cnt = 0;
-    if (s < active && tiles_ok) {
+    if (s < active) {
       const uint32_t packed = A.extents[s];
-      if (packed != 0u) {
+      surv += packed != 0u ? 1u : 0u;
+      if (packed != 0u && tiles_ok) {
         const Splat2D sp = load_splat(A.geom, s);
         const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
         for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
@@ -121,12 +137,18 @@ __global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
     loc[j] = s_val[t * BI + j];
     sum += loc[j];
   }
-  uint32_t total;
+  uint32_t total, surv_total;
   const uint32_t texcl = cgs::block_exclusive_scan<BT>(sum, s_tmp, total);
+  (void)cgs::block_exclusive_scan<BT>(surv, s_tmp, surv_total);
   if (t == 0) {
-    const uint64_t excl = cgs::lookback_u64(A.status, part, (uint64_t)total);
-    s_base = excl;
-    if (part == gridDim.x - 1) *A.total = (uint32_t)(excl + total);
+    // one look-back carries both sums: entries in the low 32 bits, survivors above
+    const uint64_t agg = ((uint64_t)surv_total << 32) | (uint64_t)total;
+    const uint64_t excl = cgs::lookback_u64(A.status, part, agg);
+    s_base = excl & 0xFFFFFFFFull;
+    if (part == gridDim.x - 1) {
+      A.total[0] = (uint32_t)((excl + agg) & 0xFFFFFFFFull);
+      A.total[1] = (uint32_t)((excl + agg) >> 32);
+    }
   }
   __syncthreads();
   const uint64_t pbase = s_base;
@@ -158,9 +180,13 @@ __global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
       for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
         for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
           if (hits(sp, tx, ty, A.width, A.height)) {
+            const uint32_t key = ((ty * A.tiles_x + tx) << A.depth_bits) | dq;
+            // only stored entries are histogrammed: the sort reads min(total, cap)
+            // entries, and its digit offsets must add up to exactly that many
             if (cur < A.cap) {
-              A.keys[cur] = ((ty * A.tiles_x + tx) << A.depth_bits) | dq;
+              A.keys[cur] = key;
               A.ids[cur] = s;
+              if (A.hist_partial) hist_key(s_hist, key, A.sort_bits);
             }
             ++cur;
           }
@@ -169,14 +195,24 @@ __global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
       for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
         for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
           if (hits(sp, tx, ty, A.width, A.height)) {
+            const uint32_t tile = ty * A.tiles_x + tx;
             if (cur < A.cap) {
-              A.keys[cur] = ty * A.tiles_x + tx;
+              A.keys[cur] = tile;
               A.depths[cur] = dbits;
               A.ids[cur] = s;
+              if (A.hist_partial) {
+                hist_key(s_hist, dbits, 32u);
+                hist_key(s_hist + 1024, tile, A.tile_bits);
+              }
             }
             ++cur;
           }
     }
+  }
+  if (A.hist_partial) {
+    __syncthreads();
+    for (uint32_t i = t; i < A.hist_words; i += BT)
+      A.hist_partial[(size_t)part * A.hist_words + i] = s_hist[i];
   }
 }
 

@@ -7,7 +7,7 @@
 // operation (-ffp-contract=off; ln/exp/pow/acos/cos from cgs_detmath.h), with
 // one deliberate change: the reference hands out child slots with an atomic
 // counter (deformation.rs:1191), so child order varies from run to run.  Here
-// each 256-splat partition scans its split flags and chains the partition
+// each 1024-splat partition scans its split flags and chains the partition
 // totals with a decoupled look-back, so child k of the frame is always the
 // k-th splitting parent in index order — the order of the CPU path
 // (src/deformation/cpu_reference.rs:316-377) — and keys stay reproducible.
@@ -140,18 +140,16 @@ __device__ __forceinline__ void store_splat(const CgsDeformArgs& A, uint32_t i, 
   g[cgs_blk(i, 9, CGS_GEO_K)] = o;
 }
 
-__global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
-  __shared__ uint32_t s_part;
-  __shared__ uint32_t s_tmp[4];
-  __shared__ uint64_t s_base;
-  const uint32_t t = threadIdx.x;
-  if (t == 0) s_part = atomicAdd(A.ticket, 1u);
-  __syncthreads();
-  const uint32_t part = s_part;
-  const uint32_t idx = part * 256u + t;
-  const bool valid = idx < A.num_gaussians;
-  const float* RG = A.rest_geo;
+struct Child {
+  float x, y, z, c00, c01, c02, c11, c12, c22, alpha;
+  uint32_t tet;
+};
 
+// One splat of deform_and_split_compact_kernel: writes the splat (or its first
+// child) at idx and returns true with the second child in `ch` when it splits.
+__device__ __forceinline__ bool deform_splat(const CgsDeformArgs& A, uint32_t idx, Child& ch) {
+  const bool valid = true;
+  const float* RG = A.rest_geo;
   bool split = false;
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 0;
   float c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0, child_alpha = 0;
@@ -298,21 +296,62 @@ __global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
     }
   }
 
-  // deterministic child slots: scan of split flags across the whole launch
-  uint32_t total;
-  const uint32_t local = cgs::block_exclusive_scan<256>(split ? 1u : 0u, s_tmp, total);
+  ch.x = px + dx;
+  ch.y = py + dy;
+  ch.z = pz + dz;
+  ch.c00 = c00;
+  ch.c01 = c01;
+  ch.c02 = c02;
+  ch.c11 = c11;
+  ch.c12 = c12;
+  ch.c22 = c22;
+  ch.alpha = child_alpha;
+  ch.tet = tet_raw;
+  return split;
+}
+
+constexpr int DI = 4;  // splats per thread: a partition is 1024 splats
+
+__global__ __launch_bounds__(256) void k_deform(CgsDeformArgs A) {
+  __shared__ uint32_t s_part;
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint64_t s_base;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) s_part = atomicAdd(A.ticket, 1u);
+  __syncthreads();
+  const uint32_t part = s_part;
+  Child ch[DI];
+  bool sp[DI];
+#pragma unroll
+  for (int i = 0; i < DI; ++i) {
+    const uint32_t idx = part * (256u * DI) + (uint32_t)i * 256u + t;
+    sp[i] = idx < A.num_gaussians ? deform_splat(A, idx, ch[i]) : false;
+  }
+  // deterministic child slots: split flags scanned in splat-index order
+  // (row i of the partition before row i+1), partitions chained by look-back
+  uint32_t loc[DI];
+  uint32_t run = 0;
+#pragma unroll
+  for (int i = 0; i < DI; ++i) {
+    uint32_t tot;
+    loc[i] = run + cgs::block_exclusive_scan<256>(sp[i] ? 1u : 0u, s_tmp, tot);
+    run += tot;
+  }
   if (t == 0) {
-    const uint64_t excl = cgs::lookback_u64(A.status, part, (uint64_t)total);
+    const uint64_t excl = cgs::lookback_u64(A.status, part, (uint64_t)run);
     s_base = excl;
-    if (part == gridDim.x - 1) *A.split_count = (uint32_t)(excl + total);
+    if (part == gridDim.x - 1) *A.split_count = (uint32_t)(excl + run);
   }
   __syncthreads();
-  if (split) {
-    const uint32_t off = (uint32_t)s_base + local;
-    const uint32_t child = A.num_gaussians + off;
-    store_splat(A, child, px + dx, py + dy, pz + dz, c00, c01, c02, c11, c12, c22, child_alpha);
+#pragma unroll
+  for (int i = 0; i < DI; ++i) {
+    if (!sp[i]) continue;
+    const uint32_t idx = part * (256u * DI) + (uint32_t)i * 256u + t;
+    const uint32_t off = (uint32_t)s_base + loc[i];
+    store_splat(A, A.num_gaussians + off, ch[i].x, ch[i].y, ch[i].z, ch[i].c00, ch[i].c01,
+                ch[i].c02, ch[i].c11, ch[i].c12, ch[i].c22, ch[i].alpha);
     A.split_meta[2 * off] = idx;
-    A.split_meta[2 * off + 1] = tet_raw;
+    A.split_meta[2 * off + 1] = ch[i].tet;
   }
 }
 
@@ -323,7 +362,7 @@ void cgs_launch_tet_transforms(const CgsTetArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_tet_transforms, dim3((a.num_tets + 63u) / 64u), dim3(64), 0, s, a);
 }
 
-uint32_t cgs_deform_num_parts(uint32_t n) { return (n + 255u) / 256u; }
+uint32_t cgs_deform_num_parts(uint32_t n) { return (n + 256u * DI - 1u) / (256u * DI); }
 
 void cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s) {
   const uint32_t parts = cgs_deform_num_parts(a.num_gaussians);

@@ -8,35 +8,39 @@
 // per-wave LDS digit counters, partitions chain their per-digit counts with a
 // decoupled look-back, keys are staged in LDS in digit order and written out
 // as contiguous runs.  All digit histograms of all passes come from one
-// upfront read of the keys.  Ranking keeps the input order of equal digits,
+// upfront read of the keys (or from the binning pass that emitted them) as
+// per-block partials summed by a small reduction: no global atomics besides
+// one partition ticket per live 8192-key partition.  Ranking keeps the input order of equal digits,
 // so each pass and the whole sort are stable, which is the reference's order.
 #include "cgs_internal.h"
 
 namespace {
 
-constexpr int RT = 256;          // threads
+constexpr int RT = 512;          // threads (8 wave64s)
 constexpr int RI = 16;           // keys per thread
-constexpr int RP = RT * RI;      // keys per partition
+constexpr int RP = RT * RI;      // keys per partition (8192)
 constexpr int RW = RT / 64;      // waves
 constexpr int RADIX = 256;
 constexpr int MAX_PASSES = 4;
+constexpr int HIST_BLOCKS = 128; // partial histograms of the standalone sort
 
 __device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift) {
   const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
   return (1u << w) - 1u;
 }
 
+// Partial digit histograms of all passes, one row of MAX_PASSES*256 words per
+// block (no global atomics: same-address device atomics serialise).
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* d_n,
                                                     uint32_t n_static, uint32_t cap,
-                                                    uint32_t bits, uint32_t* hist) {
+                                                    uint32_t bits, uint32_t* partial) {
   __shared__ uint32_t s_h[MAX_PASSES * RADIX];
   const uint32_t passes = (bits + 7u) / 8u;
-  for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256u) s_h[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += 256u) s_h[i] = 0u;
   __syncthreads();
   const uint32_t n = min(d_n ? *d_n : n_static, cap);
   // Equal digits within a wave are merged first (8 ballots give the lanes that
-  // share a digit), so each distinct digit costs one LDS atomic per wave:
-  // sorted-ish keys would otherwise serialise on a few bins.
+  // share a digit), so each distinct digit costs one LDS atomic per wave.
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t stride = gridDim.x * 256u;
   for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += stride) {
@@ -59,8 +63,30 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256u)
-    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+  for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += 256u)
+    partial[(size_t)blockIdx.x * (MAX_PASSES * RADIX) + i] = s_h[i];
+}
+
+// hist[b] = sum over rows of partial[row][b], b < bins: each block owns 16 bins,
+// its 16 row-chunks are summed in parallel and combined in LDS.
+__global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* partial, uint32_t rows,
+                                                     uint32_t row_words, uint32_t bins,
+                                                     uint32_t* hist) {
+  __shared__ uint32_t s_sum[16][17];
+  const uint32_t t = threadIdx.x;
+  const uint32_t bin = blockIdx.x * 16u + (t & 15u);
+  const uint32_t chunk = t >> 4;
+  uint32_t acc = 0;
+  if (bin < bins)
+    for (uint32_t r = chunk; r < rows; r += 16u) acc += partial[(size_t)r * row_words + bin];
+  s_sum[chunk][t & 15u] = acc;
+  __syncthreads();
+  if (t < 16u && blockIdx.x * 16u + t < bins) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v += s_sum[c][t];
+    hist[blockIdx.x * 16u + t] = v;
+  }
 }
 
 struct PassArgs {
@@ -89,28 +115,32 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
   const uint32_t t = threadIdx.x;
   const uint32_t lane = t & 63u, wave = t >> 6;
+  const uint32_t n = min(A.d_n ? *A.d_n : A.n_static, A.cap);
+  // blocks beyond the live entries leave before taking a ticket (each ticket
+  // is a same-address device atomic); the tickets taken are exactly 0..P-1
+  if (blockIdx.x * (uint32_t)RP >= n) return;
   if (t == 0) s_part = atomicAdd(A.ticket, 1u);
-#pragma unroll
-  for (int w = 0; w < RW; ++w) s_whist[w][t] = 0u;
+  for (uint32_t i = t; i < (uint32_t)(RW * RADIX); i += RT) (&s_whist[0][0])[i] = 0u;
   __syncthreads();
   const uint32_t part = s_part;
-  const uint32_t n = min(A.d_n ? *A.d_n : A.n_static, A.cap);
   const uint32_t base = part * (uint32_t)RP;
-  if (base >= n) return;  // uniform: no later partition waits on this one
   const uint32_t cnt_part = min((uint32_t)RP, n - base);
   const uint32_t dmask = digit_mask(A.bits, A.shift);
 
   // rank: wave w owns keys [base + w*1024, base + (w+1)*1024), item i of lane l
   // at offset i*64 + l, so items are visited in input order
-  uint32_t keys[RI], vals[RI], offs[RI];
+  uint32_t keys[RI], offs[RI];
   const uint32_t wbase = base + wave * (uint32_t)(RI * 64);
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
+    keys[i] = idx < n ? A.keys_in[idx] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     const bool valid = idx < n;
-    const uint32_t k = valid ? A.keys_in[idx] : 0u;
-    const uint32_t v = valid ? (A.vals_in ? A.vals_in[idx] : idx) : 0u;
-    const uint32_t d = (k >> A.shift) & dmask;
+    const uint32_t d = (keys[i] >> A.shift) & dmask;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -125,40 +155,42 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
     __builtin_amdgcn_wave_barrier();
     if (valid && prior == 0u) s_whist[wave][d] = wcount + (uint32_t)__popcll(m);
     __builtin_amdgcn_wave_barrier();
-    keys[i] = k;
-    vals[i] = v;
     offs[i] = valid ? wcount + prior : 0xFFFFFFFFu;
   }
   __syncthreads();
 
-  // per digit (thread t == digit t): wave prefixes, partition count
-  uint32_t c[RW];
+  // per digit (thread t == digit t < 256): wave prefixes, partition count
   uint32_t dtotal = 0;
+  if (t < (uint32_t)RADIX) {
 #pragma unroll
-  for (int w = 0; w < RW; ++w) {
-    c[w] = s_whist[w][t];
-    s_whist[w][t] = dtotal;
-    dtotal += c[w];
+    for (int w = 0; w < RW; ++w) {
+      const uint32_t c = s_whist[w][t];
+      s_whist[w][t] = dtotal;
+      dtotal += c;
+    }
   }
   uint32_t htot;
   const uint32_t lstart = cgs::block_exclusive_scan<RT>(dtotal, s_tmp, htot);
-  const uint32_t hv = A.hist[t];
+  const uint32_t hv = t < (uint32_t)RADIX ? A.hist[t] : 0u;
   uint32_t hsum;
   const uint32_t dbase = cgs::block_exclusive_scan<RT>(hv, s_tmp, hsum);
-  s_lstart[t] = lstart;
-  // look-back for this digit over the partitions before this one
-  const uint32_t excl = cgs::lookback_u32(A.status + t, part, RADIX, dtotal);
-  s_gadj[t] = dbase + excl - lstart;
+  if (t < (uint32_t)RADIX) {
+    s_lstart[t] = lstart;
+    // look-back for this digit over the partitions before this one
+    const uint32_t excl = cgs::lookback_u32(A.status + t, part, RADIX, dtotal);
+    s_gadj[t] = dbase + excl - lstart;
+  }
   __syncthreads();
 
-  // stage in LDS in digit order
+  // stage keys and values in LDS in digit order
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
     if (offs[i] != 0xFFFFFFFFu) {
+      const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
       const uint32_t d = (keys[i] >> A.shift) & dmask;
       const uint32_t pos = s_lstart[d] + s_whist[wave][d] + offs[i];
       s_keys[pos] = keys[i];
-      s_vals[pos] = vals[i];
+      s_vals[pos] = A.vals_in ? A.vals_in[idx] : idx;
     }
   }
   __syncthreads();
@@ -168,38 +200,46 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
     const uint32_t k = s_keys[j];
     const uint32_t d = (k >> A.shift) & dmask;
     const uint32_t dst = s_gadj[d] + j;
-    A.keys_out[dst] = k;
-    A.vals_out[dst] = s_vals[j];
+    if (dst < n) {  // always true when the histogram matches the n keys read
+      A.keys_out[dst] = k;
+      A.vals_out[dst] = s_vals[j];
+    }
   }
 }
 
 }  // namespace
 
 uint32_t cgs_radix_num_parts(uint32_t cap) { return (cap + (uint32_t)RP - 1u) / (uint32_t)RP; }
+uint32_t cgs_radix_hist_blocks() { return HIST_BLOCKS; }
 
 size_t cgs_radix_control_words(uint32_t cap) {
   const size_t parts = cgs_radix_num_parts(cap) ? cgs_radix_num_parts(cap) : 1u;
-  return (size_t)MAX_PASSES * RADIX + (size_t)MAX_PASSES * parts * RADIX + MAX_PASSES;
+  return (size_t)MAX_PASSES * RADIX + (size_t)MAX_PASSES * parts * RADIX + MAX_PASSES +
+         (size_t)HIST_BLOCKS * MAX_PASSES * RADIX;
+}
+
+void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row_words,
+                            uint32_t bins, uint32_t* hist, hipStream_t s) {
+  if (rows == 0 || bins == 0) return;
+  hipLaunchKernelGGL(k_hist_reduce, dim3((bins + 15u) / 16u), dim3(256), 0, s, partial, rows,
+                     row_words, bins, hist);
 }
 
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
-                          uint32_t bits, bool iota_vals, bool zero_control, hipStream_t s,
+                          uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
                           CgsProf* prof) {
   if (bits == 0 || b.cap == 0) return 0;
   if (bits > 32) bits = 32;
   const uint32_t passes = (bits + 7u) / 8u;
   const uint32_t parts = cgs_radix_num_parts(b.cap);
-  if (zero_control) {
-    // hist, status and tickets are contiguous when allocated by the runtime
-    cgs_launch_fill_u32(b.hist, 0u, (uint64_t)MAX_PASSES * RADIX, s);
-    cgs_launch_fill_u32(b.status, 0u, (uint64_t)MAX_PASSES * parts * RADIX, s);
-    cgs_launch_fill_u32(b.tickets, 0u, MAX_PASSES, s);
+  if (!hist_given) {
+    if (prof) prof->mark("k_radix_hist", s);
+    hipLaunchKernelGGL(k_radix_hist, dim3(HIST_BLOCKS), dim3(256), 0, s, b.keys, d_n, n_static,
+                       b.cap, bits, b.hist_partial);
+    if (prof) prof->mark("k_hist_reduce", s);
+    cgs_launch_hist_reduce(b.hist_partial, HIST_BLOCKS, MAX_PASSES * RADIX, passes * RADIX,
+                           b.hist, s);
   }
-  uint32_t hblocks = (b.cap + 255u) / 256u;
-  if (hblocks > 2048u) hblocks = 2048u;
-  if (prof) prof->mark("k_radix_hist", s);
-  hipLaunchKernelGGL(k_radix_hist, dim3(hblocks), dim3(256), 0, s, b.keys, d_n, n_static, b.cap,
-                     bits, b.hist);
   const uint32_t* kin = b.keys;
   const uint32_t* vin = iota_vals ? nullptr : b.vals;
   int in_alt = 0;

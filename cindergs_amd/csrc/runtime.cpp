@@ -176,19 +176,22 @@ struct cgs_context {
 namespace {
 
 // frame control words (zeroed once per frame before Stage 2)
+// Stage 3 control words: [0, zero_words) is cleared before every Stage 3 run;
+// the per-partition histogram rows after it are fully written by the binning.
 struct FrameLayout {
-  size_t survivors, bin_ticket, total;
+  size_t bin_ticket, total;       // total: [entries, survivors]
   size_t rA_tickets, rB_tickets;  // 4 each
-  size_t histA, histB;            // 1024 each
+  size_t histA, histB;            // 1024 each, contiguous (64-bit path reduces both at once)
   size_t bin_status;              // u64 words, 8-byte aligned (index in u32 words)
   size_t rA_status, rB_status;
+  size_t zero_words;
+  size_t bin_hist;                // [bin_parts][hist_words]
   size_t words;
   uint32_t bin_parts, r_parts;
 };
 
 FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
   FrameLayout L;
-  L.survivors = 0;
   L.bin_ticket = 1;
   L.total = 2;
   L.rA_tickets = 4;
@@ -206,6 +209,9 @@ FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
   w += rs;
   L.rB_status = w;
   w += rs;
+  L.zero_words = w;
+  L.bin_hist = w;
+  w += (size_t)(L.bin_parts ? L.bin_parts : 1) * 2048;
   L.words = w;
   return L;
 }
@@ -303,10 +309,8 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
   const FrameLayout L = frame_layout(max_g, cap);
   TRY(ensure_words(c->ctrl3, L.words));
   uint32_t* ctrl = P<uint32_t>(c->ctrl3);
-  if (keep_survivors)
-    cgs_launch_fill_u32(ctrl + 1, 0u, L.words - 1, c->stream);
-  else
-    cgs_launch_fill_u32(ctrl, 0u, L.words, c->stream);
+  (void)keep_survivors;
+  cgs_launch_fill_u32(ctrl, 0u, L.zero_words, c->stream);
   cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, c->stream);
   c->captured = false;
   if (c->capture) {
@@ -345,8 +349,20 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
   B.total = ctrl + L.total;
   B.status = reinterpret_cast<uint64_t*>(ctrl + L.bin_status);
   B.ticket = ctrl + L.bin_ticket;
+  B.hist_partial = ctrl + L.bin_hist;
+  B.hist_words = c->mode64 ? 2048u : 1024u;
+  uint32_t sort_bits = tile_bits + depth_bits;
+  if (sort_bits > 32) sort_bits = 32;
+  B.sort_bits = sort_bits;
+  B.tile_bits = tile_bits;
   c->prof.mark("k_bin", c->stream);
   cgs_launch_bin(B, c->stream);
+  // digit histograms of every pass, summed over the binning partitions
+  const uint32_t hist_bins =
+      c->mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u) : 256u * ((sort_bits + 7u) / 8u);
+  c->prof.mark("k_hist_reduce", c->stream);
+  cgs_launch_hist_reduce(ctrl + L.bin_hist, L.bin_parts, B.hist_words, hist_bins, ctrl + L.histA,
+                         c->stream);
 
   const uint32_t* d_n = ctrl + L.total;
   if (!c->mode64) {
@@ -356,12 +372,11 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
     R.keys_alt = P<uint32_t>(c->e_alt_k);
     R.vals_alt = P<uint32_t>(c->e_alt_v);
     R.hist = ctrl + L.histA;
+    R.hist_partial = nullptr;
     R.status = ctrl + L.rA_status;
     R.tickets = ctrl + L.rA_tickets;
     R.cap = cap;
-    uint32_t sort_bits = tile_bits + depth_bits;
-    if (sort_bits > 32) sort_bits = 32;
-    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, false, c->stream, &c->prof);
+    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, c->stream, &c->prof);
     c->sorted_keys = alt ? R.keys_alt : R.keys;
     c->sorted_ids = alt ? R.vals_alt : R.vals;
     c->prof.mark("k_tile_ranges", c->stream);
@@ -376,10 +391,11 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
     A.keys_alt = P<uint32_t>(c->e_alt_k);
     A.vals_alt = P<uint32_t>(c->e_alt_v);
     A.hist = ctrl + L.histA;
+    A.hist_partial = nullptr;
     A.status = ctrl + L.rA_status;
     A.tickets = ctrl + L.rA_tickets;
     A.cap = cap;
-    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, false, c->stream, &c->prof);
+    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, true, c->stream, &c->prof);
     const uint32_t* perm = altA ? A.vals_alt : A.vals;
     uint32_t* gk = altA ? P<uint32_t>(c->e_depths) : P<uint32_t>(c->e_alt_k);
     uint32_t* gv = altA ? P<uint32_t>(c->e_perm) : P<uint32_t>(c->e_alt_v);
@@ -393,10 +409,11 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
     Bt.keys_alt = P<uint32_t>(c->e_keys);
     Bt.vals_alt = P<uint32_t>(c->e_ids);
     Bt.hist = ctrl + L.histB;
+    Bt.hist_partial = nullptr;
     Bt.status = ctrl + L.rB_status;
     Bt.tickets = ctrl + L.rB_tickets;
     Bt.cap = cap;
-    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, false, c->stream, &c->prof);
+    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, c->stream, &c->prof);
     c->sorted_keys = altB ? Bt.keys_alt : Bt.keys;
     c->sorted_ids = altB ? Bt.vals_alt : Bt.vals;
     c->prof.mark("k_tile_ranges", c->stream);
@@ -427,16 +444,15 @@ int launch_stage4(cgs_context* c, const FrameGeom& g, const cgs_compositor_unifo
   return CGS_OK;
 }
 
-// read survivors / split count / entry total (one small copy + wait)
-int read_counters(cgs_context* c, uint32_t max_g, bool with_split) {
-  const FrameLayout L = frame_layout(max_g, c->entry_cap);
-  (void)L;
+// read the entry total, the survivors and the split count (one small copy + wait):
+// h_counters[2] = entries, [3] = survivors, [4] = split children
+int read_counters(cgs_context* c, bool with_split) {
   uint32_t* ctrl = P<uint32_t>(c->ctrl3);
-  HIPCHK(hipMemcpyAsync(c->h_counters, ctrl, 3 * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_counters, ctrl, 4 * 4, hipMemcpyDeviceToHost, c->stream));
   if (with_split && c->ctrl1.p)
-    HIPCHK(hipMemcpyAsync(c->h_counters + 3, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_counters + 4, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
   else
-    c->h_counters[3] = 0;
+    c->h_counters[4] = 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
 }
@@ -814,7 +830,6 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   init_entry_cap(c, max_g);
   const FrameLayout L = frame_layout(max_g, c->entry_cap);
   TRY(ensure_words(c->ctrl3, L.words));
-  cgs_launch_fill_u32(P<uint32_t>(c->ctrl3), 0u, 1, s);  // survivors
   CgsProjectArgs PA;
   memset(&PA, 0, sizeof(PA));
   PA.scene = S;
@@ -837,7 +852,6 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   PA.geom = P<float>(c->geom);
   PA.color = P<float>(c->color);
   PA.extents = P<uint32_t>(c->extents);
-  PA.survivors = P<uint32_t>(c->ctrl3);
   c->prof.mark("k_project", s);
   cgs_launch_project(PA, s);
   HIPCHK(hipGetLastError());
@@ -850,9 +864,9 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   TRY(launch_stage4(c, g, comp));
   if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
 
-  TRY(read_counters(c, max_g, c->has_cages));
-  uint32_t survivors = c->h_counters[0];
-  const uint32_t splits = c->h_counters[3];
+  TRY(read_counters(c, c->has_cages));
+  const uint32_t splits = c->h_counters[4];
+  uint32_t survivors = use_cache ? c->s3_survivors : c->h_counters[3];
   if (!use_cache) {
     uint32_t total = c->h_counters[2];
     // the tile-entry arena overflowed: grow and re-run Stages 3-4
@@ -865,8 +879,9 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
       if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
       TRY(launch_stage4(c, g, comp));
       if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
-      TRY(read_counters(c, max_g, c->has_cages));
+      TRY(read_counters(c, c->has_cages));
       total = c->h_counters[2];
+      survivors = c->h_counters[3];
       reruns++;
     }
     c->n_entries = total;
@@ -1055,7 +1070,7 @@ int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
   c->last_h = g.H;
   for (int attempt = 0; attempt < 5; ++attempt) {
     TRY(launch_stage3(c, g, cam->near_plane, c->active, nullptr, c->max_g, false));
-    TRY(read_counters(c, c->max_g, false));
+    TRY(read_counters(c, false));
     const uint32_t total = c->h_counters[2];
     c->n_entries = total;
     if (total <= c->entry_cap) break;
@@ -1106,6 +1121,13 @@ int cgs_memcpy_d2h(cgs_context* c, void* dst, const void* src, uint64_t bytes) {
   return CGS_OK;
 }
 
+int cgs_memcpy_d2d(cgs_context* c, void* dst, const void* src, uint64_t bytes) {
+  TRY(ensure_ctx(c));
+  if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
 int cgs_exclusive_scan_u32(cgs_context* c, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
                            uint32_t* total) {
   TRY(ensure_ctx(c));
@@ -1134,15 +1156,19 @@ static int radix_prim(cgs_context* c, uint32_t* keys, uint32_t* vals, uint32_t n
   const size_t words = cgs_radix_control_words(n);
   TRY(ensure_words(c->prim_ctrl, words));
   uint32_t* ctrl = P<uint32_t>(c->prim_ctrl);
-  cgs_launch_fill_u32(ctrl, 0u, words, c->stream);
+  const size_t parts = cgs_radix_num_parts(n) ? cgs_radix_num_parts(n) : 1u;
+  const size_t status_words = 4 * parts * 256;
+  // layout: hist[1024] | status | tickets[4] | partial histograms
+  cgs_launch_fill_u32(ctrl + 1024, 0u, status_words + 4, c->stream);
   CgsRadixBuffers R;
   R.keys = keys;
   R.vals = vals;
   R.keys_alt = P<uint32_t>(c->prim_alt_k);
   R.vals_alt = P<uint32_t>(c->prim_alt_v);
   R.hist = ctrl;
-  R.status = ctrl + 4 * 256;
-  R.tickets = ctrl + words - 4;
+  R.status = ctrl + 1024;
+  R.tickets = ctrl + 1024 + status_words;
+  R.hist_partial = ctrl + 1024 + status_words + 4;
   R.cap = n;
   *in_alt = cgs_launch_radix_sort(R, nullptr, n, bits, iota, false, c->stream);
   HIPCHK(hipGetLastError());

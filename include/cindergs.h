@@ -178,6 +178,7 @@ int cgs_malloc(cgs_context* ctx, uint64_t bytes, void** dptr);
 int cgs_free(cgs_context* ctx, void* dptr);
 int cgs_memcpy_h2d(cgs_context* ctx, void* dst, const void* src, uint64_t bytes);
 int cgs_memcpy_d2h(cgs_context* ctx, void* dst, const void* src, uint64_t bytes);
+int cgs_memcpy_d2d(cgs_context* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* gpu_exclusive_scan_total (src/render/kernels/scan.rs:312-333). */
 int cgs_exclusive_scan_u32(cgs_context* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
