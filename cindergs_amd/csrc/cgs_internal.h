@@ -209,12 +209,12 @@ struct CgsBinArgs {
   uint32_t* keys;            // 32-bit mode: packed keys; 64-bit mode: tile ids
   uint32_t* depths;          // 64-bit mode only
   uint32_t* ids;
-  uint32_t* counts_out;      // optional [max] (per-splat tile counts)
+  uint32_t* counts;          // [max] per-splat tile counts
   uint32_t* offsets_out;     // optional [max] (exclusive scan)
   uint32_t* total;           // device scalars: [0] total entries (may exceed cap),
                              // [1] survivors (splats with nonzero extents)
-  uint64_t* status;          // [num_parts] zeroed
-  uint32_t* ticket;          // zeroed
+  uint32_t* block_sums;      // [num_parts] entries per block
+  uint32_t* block_surv;      // [num_parts] survivors per block
   uint32_t* hist_partial;    // optional [num_parts][hist_words]: digit histograms of the
                              // emitted keys (32-bit mode: 4 passes of the key; 64-bit
                              // mode: 4 passes of the depth bits, then the tile passes)
@@ -251,7 +251,7 @@ void cgs_launch_sh_f16_blocked(const uint16_t* in, uint16_t* out, uint32_t n, ui
                                uint32_t rest, hipStream_t s);
 void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s);
 uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
-void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s);
+void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof = nullptr);
 void cgs_launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
                      uint64_t* status, uint32_t* ticket, hipStream_t s);
 uint32_t cgs_scan_num_parts(uint32_t n);

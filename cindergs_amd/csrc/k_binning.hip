@@ -8,20 +8,22 @@
 // identify_tile_ranges(_u32_tile)_kernel (binning.rs:344-417).
 //
 // MI355X design: the reference runs count -> 3-level scan -> host readback of
-// the total -> emit.  Here one kernel does count, a block scan, a decoupled
-// look-back across partitions (partition ids from an atomic ticket, so the
-// chain always progresses) and the emission; the total stays on the device
-// and emission is bounded by the arena capacity (an overflowing frame is
-// re-run by the host with a larger arena).  The per-splat cutoff
+// the total -> emit.  Here two kernels: k_count writes per-splat counts and
+// per-block sums; k_emit derives its block prefix by reducing the sums of the
+// blocks before it (a few loads per thread: no look-back chain, no ticket, no
+// same-address atomics), scans locally and emits; the total stays on the
+// device and emission is bounded by the arena capacity (an overflowing frame
+// is re-run by the host with a larger arena).  The emitting blocks also build
+// the radix digit histograms of the entries they store.  The per-splat cutoff
 // q_cut = clamp(2 ln(255 alpha), 0, 8) was computed once by Stage 2 with the
 // same expression the reference evaluates per tile, so the test is bit-equal.
 #include "cgs_internal.h"
 
 namespace {
 
-constexpr int BT = 256;  // threads per partition
-constexpr int BI = 16;   // splats per thread
-constexpr int BP = BT * BI;
+constexpr int BT = 256;  // threads per block
+constexpr int BS = 4;    // consecutive splats per thread
+constexpr int BP = BT * BS;
 
 struct TileBox {
   uint32_t tx0, tx1, ty0, ty1;
@@ -79,140 +81,150 @@ __device__ __forceinline__ Splat2D load_splat(const float* geom, uint32_t i) {
   return s;
 }
 
+// Splats are handled in blocks of BT*BS consecutive splats, thread t owning
+// splats base + t*BS .. + BS-1 (so a thread's entries are contiguous).
+__device__ __forceinline__ uint32_t count_splat(const CgsBinArgs& A, uint32_t s, uint32_t active,
+                                                bool& survived) {
+  survived = false;
+  if (s >= active) return 0u;
+  const uint32_t packed = A.extents[s];
+  if (packed == 0u) return 0u;
+  survived = true;
+  if (A.tiles_x == 0u || A.tiles_y == 0u) return 0u;
+  const Splat2D sp = load_splat(A.geom, s);
+  const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
+  uint32_t cnt = 0;
+  for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
+    for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx) cnt += hits(sp, tx, ty, A.width, A.height) ? 1u : 0u;
+  return cnt;
+}
+
+// count_tiles_kernel (binning.rs:90-164): per-splat tile counts + per-block sums
+__global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
+  __shared__ uint32_t s_tmp[BT / 64];
+  const uint32_t t = threadIdx.x;
+  const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
+  const uint32_t s0 = blockIdx.x * (uint32_t)BP + t * (uint32_t)BS;
+  uint32_t sum = 0, surv = 0;
+#pragma unroll
+  for (int j = 0; j < BS; ++j) {
+    const uint32_t s = s0 + (uint32_t)j;
+    bool sv;
+    const uint32_t c = count_splat(A, s, active, sv);
+    if (s < A.max_gaussians) A.counts[s] = c;
+    sum += c;
+    surv += sv ? 1u : 0u;
+  }
+  uint32_t tot, stot;
+  (void)cgs::block_exclusive_scan<BT>(sum, s_tmp, tot);
+  (void)cgs::block_exclusive_scan<BT>(surv, s_tmp, stot);
+  if (t == 0) {
+    A.block_sums[blockIdx.x] = tot;
+    A.block_surv[blockIdx.x] = stot;
+  }
+}
+
+// digit histograms of one emitted entry for every radix pass that will sort it
 __device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift) {
   const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
   return (1u << w) - 1u;
 }
-
-// digit histograms of one emitted entry for every radix pass that will sort it
 __device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_t bits) {
   for (uint32_t p = 0; p * 8u < bits; ++p)
     atomicAdd(&s_hist[p * 256u + ((key >> (p * 8u)) & dmask(bits, p * 8u))], 1u);
 }
 
-__global__ __launch_bounds__(BT) void k_bin(CgsBinArgs A) {
-  __shared__ uint32_t s_part;
-  __shared__ uint32_t s_val[BP];
+// gpu_exclusive_scan_total + emit_tile_keys(_u64)_kernel (scan.rs:198-333,
+// binning.rs:168-328).  The prefix of block b is the sum of the block sums
+// before it, reduced here by the block itself (no look-back chain, no ticket).
+__global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
   __shared__ uint32_t s_tmp[BT / 64];
-  __shared__ uint64_t s_base;
   __shared__ uint32_t s_hist[2048];
-
   const uint32_t t = threadIdx.x;
-  if (t == 0) s_part = atomicAdd(A.ticket, 1u);
+  const uint32_t b = blockIdx.x;
   if (A.hist_partial)
     for (uint32_t i = t; i < A.hist_words; i += BT) s_hist[i] = 0u;
-  __syncthreads();
-  const uint32_t part = s_part;
-  const uint32_t base = part * (uint32_t)BP;
+  // prefix of this block and, in the last block, the totals
+  uint32_t pre = 0, all_surv = 0;
+  const bool last = b == gridDim.x - 1;
+  for (uint32_t j = t; j < b; j += BT) pre += A.block_sums[j];
+  if (last)
+    for (uint32_t j = t; j < gridDim.x; j += BT) all_surv += A.block_surv[j];
+  uint32_t pre_tot, surv_tot;
+  (void)cgs::block_exclusive_scan<BT>(pre, s_tmp, pre_tot);
+  (void)cgs::block_exclusive_scan<BT>(all_surv, s_tmp, surv_tot);
+  // thread-blocked offsets inside the block
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
-  const bool tiles_ok = A.tiles_x != 0u && A.tiles_y != 0u;
-
-  // 1) counts, striped so that the loads of a wave are contiguous
-  uint32_t surv = 0;
-#pragma unroll 1
-  for (int i = 0; i < BI; ++i) {
-    const uint32_t s = base + (uint32_t)i * BT + t;
-    uint32_t cnt = 0;
-    if (s < active) {
-      const uint32_t packed = A.extents[s];
-      surv += packed != 0u ? 1u : 0u;
-      if (packed != 0u && tiles_ok) {
-        const Splat2D sp = load_splat(A.geom, s);
-        const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
-        for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-          for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
-            cnt += hits(sp, tx, ty, A.width, A.height) ? 1u : 0u;
-      }
-    }
-    s_val[i * BT + t] = cnt;
-    if (A.counts_out && s < A.max_gaussians) A.counts_out[s] = cnt;
-  }
-  __syncthreads();
-
-  // 2) blocked scan: thread t owns splats base + t*BI .. + BI-1
-  uint32_t loc[BI];
+  const uint32_t s0 = b * (uint32_t)BP + t * (uint32_t)BS;
+  uint32_t cnt[BS];
   uint32_t sum = 0;
 #pragma unroll
-  for (int j = 0; j < BI; ++j) {
-    loc[j] = s_val[t * BI + j];
-    sum += loc[j];
+  for (int j = 0; j < BS; ++j) {
+    const uint32_t s = s0 + (uint32_t)j;
+    cnt[j] = s < A.max_gaussians ? A.counts[s] : 0u;
+    sum += cnt[j];
   }
-  uint32_t total, surv_total;
-  const uint32_t texcl = cgs::block_exclusive_scan<BT>(sum, s_tmp, total);
-  (void)cgs::block_exclusive_scan<BT>(surv, s_tmp, surv_total);
-  if (t == 0) {
-    // one look-back carries both sums: entries in the low 32 bits, survivors above
-    const uint64_t agg = ((uint64_t)surv_total << 32) | (uint64_t)total;
-    const uint64_t excl = cgs::lookback_u64(A.status, part, agg);
-    s_base = excl & 0xFFFFFFFFull;
-    if (part == gridDim.x - 1) {
-      A.total[0] = (uint32_t)((excl + agg) & 0xFFFFFFFFull);
-      A.total[1] = (uint32_t)((excl + agg) >> 32);
-    }
+  uint32_t btot;
+  uint32_t off = pre_tot + cgs::block_exclusive_scan<BT>(sum, s_tmp, btot);
+  if (last && t == 0) {
+    A.total[0] = pre_tot + btot;
+    A.total[1] = surv_tot;
   }
-  __syncthreads();
-  const uint64_t pbase = s_base;
-  uint32_t run = texcl;
-#pragma unroll
-  for (int j = 0; j < BI; ++j) {
-    s_val[t * BI + j] = run;  // partition-local exclusive offset
-    run += loc[j];
-  }
-  __syncthreads();
-
-  // 3) emission (striped again), bounded by the capacity
+  const bool tiles_ok = A.tiles_x != 0u && A.tiles_y != 0u;
 #pragma unroll 1
-  for (int i = 0; i < BI; ++i) {
-    const uint32_t s = base + (uint32_t)i * BT + t;
-    if (s >= A.max_gaussians) continue;
-    const uint64_t off = pbase + s_val[i * BT + t];
-    if (A.offsets_out) A.offsets_out[s] = (uint32_t)off;
-    if (s >= active || !tiles_ok) continue;
-    const uint32_t packed = A.extents[s];
-    if (packed == 0u) continue;
-    const Splat2D sp = load_splat(A.geom, s);
-    const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
-    uint64_t cur = off;
-    if (!A.mode64) {
-      const uint32_t raw = cgs_f2u(cgs_fmax(sp.tz, 1e-6f));
-      const uint32_t rel = raw > A.base_depth_bits ? min(raw - A.base_depth_bits, 0x0FFFFFFFu) : 0u;
-      const uint32_t dq = rel >> (28u - A.depth_bits);
-      for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-        for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
-          if (hits(sp, tx, ty, A.width, A.height)) {
-            const uint32_t key = ((ty * A.tiles_x + tx) << A.depth_bits) | dq;
-            // only stored entries are histogrammed: the sort reads min(total, cap)
-            // entries, and its digit offsets must add up to exactly that many
-            if (cur < A.cap) {
-              A.keys[cur] = key;
-              A.ids[cur] = s;
-              if (A.hist_partial) hist_key(s_hist, key, A.sort_bits);
-            }
-            ++cur;
-          }
-    } else {
-      const uint32_t dbits = cgs_f2u(cgs_fmax(sp.tz, 0.0f));
-      for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-        for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
-          if (hits(sp, tx, ty, A.width, A.height)) {
-            const uint32_t tile = ty * A.tiles_x + tx;
-            if (cur < A.cap) {
-              A.keys[cur] = tile;
-              A.depths[cur] = dbits;
-              A.ids[cur] = s;
-              if (A.hist_partial) {
-                hist_key(s_hist, dbits, 32u);
-                hist_key(s_hist + 1024, tile, A.tile_bits);
+  for (int j = 0; j < BS; ++j) {
+    const uint32_t s = s0 + (uint32_t)j;
+    if (s >= A.max_gaussians) break;
+    if (A.offsets_out) A.offsets_out[s] = off;
+    const uint32_t c = cnt[j];
+    if (c != 0u && s < active && tiles_ok) {
+      const uint32_t packed = A.extents[s];
+      const Splat2D sp = load_splat(A.geom, s);
+      const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
+      uint32_t cur = off;
+      if (!A.mode64) {
+        const uint32_t raw = cgs_f2u(cgs_fmax(sp.tz, 1e-6f));
+        const uint32_t rel =
+            raw > A.base_depth_bits ? min(raw - A.base_depth_bits, 0x0FFFFFFFu) : 0u;
+        const uint32_t dq = rel >> (28u - A.depth_bits);
+        for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
+          for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
+            if (hits(sp, tx, ty, A.width, A.height)) {
+              const uint32_t key = ((ty * A.tiles_x + tx) << A.depth_bits) | dq;
+              // only stored entries are histogrammed: the sort reads
+              // min(total, cap) entries and its digit offsets must add up to that
+              if (cur < A.cap) {
+                A.keys[cur] = key;
+                A.ids[cur] = s;
+                if (A.hist_partial) hist_key(s_hist, key, A.sort_bits);
               }
+              ++cur;
             }
-            ++cur;
-          }
+      } else {
+        const uint32_t dbits = cgs_f2u(cgs_fmax(sp.tz, 0.0f));
+        for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
+          for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
+            if (hits(sp, tx, ty, A.width, A.height)) {
+              const uint32_t tile = ty * A.tiles_x + tx;
+              if (cur < A.cap) {
+                A.keys[cur] = tile;
+                A.depths[cur] = dbits;
+                A.ids[cur] = s;
+                if (A.hist_partial) {
+                  hist_key(s_hist, dbits, 32u);
+                  hist_key(s_hist + 1024, tile, A.tile_bits);
+                }
+              }
+              ++cur;
+            }
+      }
     }
+    off += c;
   }
   if (A.hist_partial) {
     __syncthreads();
     for (uint32_t i = t; i < A.hist_words; i += BT)
-      A.hist_partial[(size_t)part * A.hist_words + i] = s_hist[i];
+      A.hist_partial[(size_t)b * A.hist_words + i] = s_hist[i];
   }
 }
 
@@ -269,10 +281,13 @@ uint32_t cgs_bin_num_parts(uint32_t max_gaussians) {
   return (max_gaussians + (uint32_t)BP - 1u) / (uint32_t)BP;
 }
 
-void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s) {
+void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof) {
   const uint32_t parts = cgs_bin_num_parts(a.max_gaussians);
   if (parts == 0) return;
-  hipLaunchKernelGGL(k_bin, dim3(parts), dim3(BT), 0, s, a);
+  if (prof) prof->mark("k_count", s);
+  hipLaunchKernelGGL(k_count, dim3(parts), dim3(BT), 0, s, a);
+  if (prof) prof->mark("k_emit", s);
+  hipLaunchKernelGGL(k_emit, dim3(parts), dim3(BT), 0, s, a);
 }
 
 void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_t cap,

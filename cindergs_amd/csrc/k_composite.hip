@@ -118,27 +118,48 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t p
         const uint32_t mlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mv);
         const uint32_t mhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(mv >> 32));
         uint64_t m = ((uint64_t)mhi << 32) | (uint64_t)mlo;
+        // two splats per iteration: their LDS reads and q / e^q chains are
+        // independent, only the blend is sequential (front to back)
         while (m != 0ull) {
-          const uint32_t k = v * 64u + (uint32_t)__builtin_ctzll(m);
+          const uint32_t k0 = v * 64u + (uint32_t)__builtin_ctzll(m);
           m &= m - 1ull;
-          const float4 gk = s_g[k];
-          const float4 mk = s_m[k];
-          const float bk = s_b[k];
-          const float dx = pcx - mk.x;
-          const float dy = pcy - mk.y;
+          const bool two = m != 0ull;  // wave-uniform
+          const uint32_t k1 = two ? v * 64u + (uint32_t)__builtin_ctzll(m) : k0;
+          m = two ? (m & (m - 1ull)) : m;
+          const float4 g0 = s_g[k0], m0 = s_m[k0];
+          const float4 g1 = s_g[k1], m1 = s_m[k1];
+          const float b0 = s_b[k0], b1 = s_b[k1];
+          const float dx0 = pcx - m0.x, dy0 = pcy - m0.y;
+          const float dx1 = pcx - m1.x, dy1 = pcy - m1.y;
           // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
-          float q = gk.x * dx * dx + gk.y * dx * dy + gk.z * dy * dy;
-          q = q > 0.0f ? 0.0f : q;
-          const float alpha = fminf(0.99f, gk.w * __expf(q));
-          const bool ok = alive && q >= -4.0f && alpha >= ALPHA_MIN;
-          const float w = ok ? alpha * T : 0.0f;
-          cr = __builtin_fmaf(w, mk.z, cr);
-          cg = __builtin_fmaf(w, mk.w, cg);
-          cb = __builtin_fmaf(w, bk, cb);
-          const float Tn = T * (1.0f - alpha);
-          const bool sat = ok && Tn < 0.001f;
-          T = ok ? (sat ? 0.0f : Tn) : T;
-          alive = alive && !sat;
+          float q0 = g0.x * dx0 * dx0 + g0.y * dx0 * dy0 + g0.z * dy0 * dy0;
+          float q1 = g1.x * dx1 * dx1 + g1.y * dx1 * dy1 + g1.z * dy1 * dy1;
+          q0 = q0 > 0.0f ? 0.0f : q0;
+          q1 = q1 > 0.0f ? 0.0f : q1;
+          const float a0 = fminf(0.99f, g0.w * __expf(q0));
+          const float a1 = fminf(0.99f, g1.w * __expf(q1));
+          {
+            const bool ok = alive && q0 >= -4.0f && a0 >= ALPHA_MIN;
+            const float w = ok ? a0 * T : 0.0f;
+            cr = __builtin_fmaf(w, m0.z, cr);
+            cg = __builtin_fmaf(w, m0.w, cg);
+            cb = __builtin_fmaf(w, b0, cb);
+            const float Tn = T * (1.0f - a0);
+            const bool sat = ok && Tn < 0.001f;
+            T = ok ? (sat ? 0.0f : Tn) : T;
+            alive = alive && !sat;
+          }
+          {
+            const bool ok = two && alive && q1 >= -4.0f && a1 >= ALPHA_MIN;
+            const float w = ok ? a1 * T : 0.0f;
+            cr = __builtin_fmaf(w, m1.z, cr);
+            cg = __builtin_fmaf(w, m1.w, cg);
+            cb = __builtin_fmaf(w, b1, cb);
+            const float Tn = T * (1.0f - a1);
+            const bool sat = ok && Tn < 0.001f;
+            T = ok ? (sat ? 0.0f : Tn) : T;
+            alive = alive && !sat;
+          }
           if (__ballot(alive) == 0ull) break;
         }
         if (__ballot(alive) == 0ull) break;

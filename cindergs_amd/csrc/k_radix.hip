@@ -77,8 +77,18 @@ __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* partial, ui
   const uint32_t bin = blockIdx.x * 16u + (t & 15u);
   const uint32_t chunk = t >> 4;
   uint32_t acc = 0;
-  if (bin < bins)
-    for (uint32_t r = chunk; r < rows; r += 16u) acc += partial[(size_t)r * row_words + bin];
+  if (bin < bins) {
+    uint32_t r = chunk;
+    // 8 independent loads in flight per iteration
+    for (; r + 7u * 16u < rows; r += 8u * 16u) {
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(r + u * 16u) * row_words + bin];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; r < rows; r += 16u) acc += partial[(size_t)r * row_words + bin];
+  }
   s_sum[chunk][t & 15u] = acc;
   __syncthreads();
   if (t < 16u && blockIdx.x * 16u + t < bins) {

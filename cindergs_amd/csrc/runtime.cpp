@@ -179,10 +179,10 @@ namespace {
 // Stage 3 control words: [0, zero_words) is cleared before every Stage 3 run;
 // the per-partition histogram rows after it are fully written by the binning.
 struct FrameLayout {
-  size_t bin_ticket, total;       // total: [entries, survivors]
+  size_t total;                   // [entries, survivors]
   size_t rA_tickets, rB_tickets;  // 4 each
   size_t histA, histB;            // 1024 each, contiguous (64-bit path reduces both at once)
-  size_t bin_status;              // u64 words, 8-byte aligned (index in u32 words)
+  size_t block_sums, block_surv;  // [bin_parts] each, fully written by the binning
   size_t rA_status, rB_status;
   size_t zero_words;
   size_t bin_hist;                // [bin_parts][hist_words]
@@ -192,7 +192,6 @@ struct FrameLayout {
 
 FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
   FrameLayout L;
-  L.bin_ticket = 1;
   L.total = 2;
   L.rA_tickets = 4;
   L.rB_tickets = 8;
@@ -200,9 +199,11 @@ FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
   L.histB = 12 + 1024;
   size_t w = 12 + 2048;
   if (w & 1) w++;
-  L.bin_status = w;
   L.bin_parts = cgs_bin_num_parts(max_g);
-  w += 2 * (size_t)(L.bin_parts ? L.bin_parts : 1);
+  L.block_sums = w;
+  w += (size_t)(L.bin_parts ? L.bin_parts : 1);
+  L.block_surv = w;
+  w += (size_t)(L.bin_parts ? L.bin_parts : 1);
   L.r_parts = cgs_radix_num_parts(cap);
   size_t rs = 4 * (size_t)(L.r_parts ? L.r_parts : 1) * 256;
   L.rA_status = w;
@@ -312,9 +313,9 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
   (void)keep_survivors;
   cgs_launch_fill_u32(ctrl, 0u, L.zero_words, c->stream);
   cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, c->stream);
+  TRY(ensure_words(c->cnt_dbg, max_g));  // per-splat tile counts (always needed)
   c->captured = false;
   if (c->capture) {
-    TRY(ensure_words(c->cnt_dbg, max_g));
     TRY(ensure_words(c->off_dbg, max_g));
     c->captured = true;
   }
@@ -344,19 +345,18 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
   B.keys = P<uint32_t>(c->e_keys);
   B.depths = c->mode64 ? P<uint32_t>(c->e_depths) : nullptr;
   B.ids = P<uint32_t>(c->e_ids);
-  B.counts_out = c->capture ? P<uint32_t>(c->cnt_dbg) : nullptr;
+  B.counts = P<uint32_t>(c->cnt_dbg);
   B.offsets_out = c->capture ? P<uint32_t>(c->off_dbg) : nullptr;
   B.total = ctrl + L.total;
-  B.status = reinterpret_cast<uint64_t*>(ctrl + L.bin_status);
-  B.ticket = ctrl + L.bin_ticket;
+  B.block_sums = ctrl + L.block_sums;
+  B.block_surv = ctrl + L.block_surv;
   B.hist_partial = ctrl + L.bin_hist;
   B.hist_words = c->mode64 ? 2048u : 1024u;
   uint32_t sort_bits = tile_bits + depth_bits;
   if (sort_bits > 32) sort_bits = 32;
   B.sort_bits = sort_bits;
   B.tile_bits = tile_bits;
-  c->prof.mark("k_bin", c->stream);
-  cgs_launch_bin(B, c->stream);
+  cgs_launch_bin(B, c->stream, &c->prof);
   // digit histograms of every pass, summed over the binning partitions
   const uint32_t hist_bins =
       c->mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u) : 256u * ((sort_bits + 7u) / 8u);
