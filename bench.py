@@ -12,10 +12,14 @@ frames on the host in the same run.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -36,6 +40,62 @@ def algorithmic_bytes(N: int, S: int, E: int, T: int, P: int) -> dict:
     return {"stage2": 44 * N + 236 * S, "stage3": 20 * N + 56 * S + 76 * E + 16 * T,
             "stage4": 8 * T + 40 * E + 4 * P, "sort": 64 * E,
             "k_project": 44 * N + 236 * S}
+
+
+def _kernel_key(name: str) -> str:
+    """Kernel name as rocprofv3 prints it -> bare function name."""
+    name = name.replace("void ", "").replace("(anonymous namespace)::", "")
+    return name.split("(")[0].split("<")[0].strip()
+
+
+def pmc_per_dispatch(counter: str, bench_args: list, timeout_s: int = 90) -> dict:
+    """Mean value of one rocprofv3 PMC counter per dispatch, per kernel, over a
+    child run of this script (the program itself after `--`, as rocprofv3
+    requires).  Empty dict when rocprofv3 is absent or the run fails."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return {}
+    out = tempfile.mkdtemp(prefix="cgs_pmc_")
+    try:
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "c", "--",
+               sys.executable, os.path.abspath(__file__)] + bench_args
+        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=timeout_s, check=True)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        vals = {}
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") == counter:
+                        vals.setdefault(_kernel_key(row["Kernel_Name"]), []).append(
+                            float(row["Counter_Value"]))
+        return {k: float(np.mean(v)) for k, v in vals.items()}
+    except (subprocess.SubprocessError, OSError, KeyError, ValueError):
+        return {}
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def hbm_traffic(kernel: str, calib_mib: int = 256) -> dict | None:
+    """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by what the same counter reads for a device-to-device
+    copy of calib_mib MiB (calib bytes read / written = calib_mib MiB)."""
+    run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
+           "--no-traffic"]
+    cal = ["--calibrate-copy", str(calib_mib)]
+    nbytes = float(calib_mib << 20)
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c = pmc_per_dispatch(counter, cal)
+        copy = [v for k, v in c.items() if k.startswith("__amd_rocclr_copyBuffer")]
+        k = pmc_per_dispatch(counter, run)
+        if not copy or copy[0] <= 0.0 or kernel not in k:
+            return None
+        factor = nbytes / (copy[0] * 1024.0)
+        res[counter] = {"kib_per_dispatch": round(k[kernel], 1), "copy_factor": round(factor, 4),
+                        "bytes": k[kernel] * 1024.0 * factor}
+    res["bytes"] = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
+    return res
 
 
 def cpu_info() -> str:
@@ -83,6 +143,9 @@ def main() -> int:
     ap.add_argument("--splats", type=int, default=1_073_930)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 PMC child runs that measure HBM traffic")
+    ap.add_argument("--profile-frames", type=int, default=24)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -146,7 +209,7 @@ def main() -> int:
     prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 2, 0)
     per_kernel = {}
     stage_ms = {"stage2": [], "stage3": [], "stage4": []}
-    nprof = min(args.steps, 24)
+    nprof = max(1, min(args.steps, args.profile_frames))
     for i in range(nprof):
         cam, comp = cams[(args.warmup + i) % 120]
         st.execute_frame_resident(cam, comp, prof, tim)
@@ -167,7 +230,7 @@ def main() -> int:
     B = algorithmic_bytes(N, S, E, T, P)
     # roofline of the HBM-bound kernel with the largest device time
     hbm_kernels = {"k_project": B["k_project"], "k_radix_onesweep": 16 * E,
-                   "k_bin": 20 * N + 56 * S + 8 * E}
+                   "k_count": 8 * N + 24 * S, "k_emit": 8 * N + 32 * S + 8 * E}
     dom = max(hbm_kernels, key=lambda k: kern.get(k, 0.0))
     per_launch_ms = kern[dom] / launches[dom]
     achieved = hbm_kernels[dom] / (per_launch_ms * 1e-3) / 1e9
@@ -206,6 +269,11 @@ def main() -> int:
                      "avg_launch_ms": round(per_launch_ms, 5)},
     }
     st.close()
+    if not args.no_traffic:
+        tr = hbm_traffic(dom)
+        if tr is not None:
+            result["roofline"]["traffic"] = int(tr["bytes"])
+            result["roofline"]["traffic_pmc"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, [(args.warmup + i) % 120 for i in range(8)],
                                               args.cpu_budget)

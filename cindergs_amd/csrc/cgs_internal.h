@@ -104,61 +104,71 @@ __device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back on 64-bit status words {flag:2 | value:62}.  Partition
-// `part` publishes its aggregate, walks back over predecessors until an
-// inclusive prefix is found, then publishes its own inclusive prefix.
-// Called by one thread.  Predecessors always hold smaller partition ids taken
-// from an atomic ticket, so they are resident or finished: every wait ends.
-__device__ __forceinline__ uint64_t lookback_u64(uint64_t* status, uint32_t part,
-                                                 uint64_t aggregate) {
-  const uint64_t VMASK = (1ull << 62) - 1ull;
+// Decoupled look-back, windowed.  Partition `part` publishes its aggregate,
+// then reads the statuses of up to LB_WINDOW predecessors at once and consumes
+// them nearest first: an aggregate is added and the walk goes on, an
+// inclusive prefix is added and ends it, a status not yet published stops the
+// window and is polled again.  On MI355X each status read is a cross-XCD
+// coherent load (~125 ns), so reading a window per round trip lets the prefix
+// travel LB_WINDOW partitions per memory latency instead of one.  Predecessors
+// hold smaller partition ids taken from an atomic ticket (or launched earlier),
+// so they are resident or finished and every wait ends.  Called by one thread
+// per status column.
+#define CGS_LB_WINDOW 16
+
+template <class U, int VBITS>
+__device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_t stride,
+                                               U aggregate) {
+  const U VMASK = ((U)1 << VBITS) - (U)1;
   if (part == 0) {
-    st_release_u64(&status[0], ((uint64_t)CGS_LB_INC << 62) | (aggregate & VMASK));
+    __hip_atomic_store(&status[0], ((U)CGS_LB_INC << VBITS) | (aggregate & VMASK),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  st_release_u64(&status[part], ((uint64_t)CGS_LB_AGG << 62) | (aggregate & VMASK));
-  uint64_t excl = 0;
-  int64_t q = (int64_t)part - 1;
-  while (q >= 0) {
-    uint64_t s = ld_acquire_u64(&status[q]);
-    uint32_t flag = (uint32_t)(s >> 62);
-    if (flag == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
+  __hip_atomic_store(&status[(size_t)part * stride], ((U)CGS_LB_AGG << VBITS) | (aggregate & VMASK),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  U excl = 0;
+  int64_t q = (int64_t)part - 1;  // nearest predecessor not yet accounted for
+  while (true) {
+    U s[CGS_LB_WINDOW];
+#pragma unroll
+    for (int k = 0; k < CGS_LB_WINDOW; ++k) {
+      const int64_t qq = q - k;
+      s[k] = qq >= 0 ? __hip_atomic_load(&status[(size_t)qq * stride], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT)
+                     : ((U)CGS_LB_INC << VBITS);  // never consumed: partition 0 is inclusive
     }
-    excl += s & VMASK;
-    if (flag == CGS_LB_INC) break;
-    --q;
+    int used = 0;
+    bool done = false;
+#pragma unroll
+    for (int k = 0; k < CGS_LB_WINDOW; ++k) {
+      if (done || used != k) continue;
+      const uint32_t flag = (uint32_t)(s[k] >> VBITS);
+      if (flag == 0) continue;  // not published yet: poll again from here
+      excl += s[k] & VMASK;
+      used = k + 1;
+      if (flag == CGS_LB_INC) done = true;
+    }
+    if (done) break;
+    q -= used;
+    if (used < CGS_LB_WINDOW) __builtin_amdgcn_s_sleep(1);
   }
-  st_release_u64(&status[part], ((uint64_t)CGS_LB_INC << 62) | ((excl + aggregate) & VMASK));
+  __hip_atomic_store(&status[(size_t)part * stride],
+                     ((U)CGS_LB_INC << VBITS) | ((excl + aggregate) & VMASK), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
   return excl;
 }
 
-// Same on 32-bit words {flag:2 | value:30} (radix digit counts, deform splits).
+// 64-bit status words {flag:2 | value:62}
+__device__ __forceinline__ uint64_t lookback_u64(uint64_t* status, uint32_t part,
+                                                 uint64_t aggregate) {
+  return lookback_windowed<uint64_t, 62>(status, part, 1u, aggregate);
+}
+
+// 32-bit status words {flag:2 | value:30} (radix digit counts)
 __device__ __forceinline__ uint32_t lookback_u32(uint32_t* status, uint32_t part,
                                                  uint32_t stride, uint32_t aggregate) {
-  const uint32_t VMASK = (1u << 30) - 1u;
-  if (part == 0) {
-    st_release_u32(&status[0], (CGS_LB_INC << 30) | (aggregate & VMASK));
-    return 0;
-  }
-  st_release_u32(&status[(size_t)part * stride], (CGS_LB_AGG << 30) | (aggregate & VMASK));
-  uint32_t excl = 0;
-  int64_t q = (int64_t)part - 1;
-  while (q >= 0) {
-    uint32_t s = ld_acquire_u32(&status[(size_t)q * stride]);
-    uint32_t flag = s >> 30;
-    if (flag == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    excl += s & VMASK;
-    if (flag == CGS_LB_INC) break;
-    --q;
-  }
-  st_release_u32(&status[(size_t)part * stride],
-                 (CGS_LB_INC << 30) | ((excl + aggregate) & VMASK));
-  return excl;
+  return lookback_windowed<uint32_t, 30>(status, part, stride, aggregate);
 }
 
 }  // namespace cgs
