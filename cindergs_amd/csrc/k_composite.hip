@@ -23,8 +23,12 @@
 //  * the conic is stored pre-scaled by -1/2 (and b by -1), which is exact, so
 //    q has the same bits as the reference expression; e^q uses the hardware
 //    exp2 (the colour parity bound is 1/255);
-//  * tiles are dealt so that each of the 8 XCDs (round-robin block placement)
-//    gets one contiguous band of tiles, keeping shared splats in one L2.
+//  * block b composites tile b: the hardware deals consecutive blocks to the 8
+//    XCDs in turn, so XCD k gets the tile columns x = k (mod 8) (the row
+//    length 120 at 1080p is a multiple of 8) and the dense middle of the
+//    image is spread over all eight; vertical neighbours share an L2;
+//  * the splat ids of the next batch are loaded while the current batch is
+//    composited, so a batch waits for one memory round trip, not two.
 #include "cgs_internal.h"
 
 namespace {
@@ -52,14 +56,14 @@ __device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float 
   return cgs_fmin(q1, q2) <= qcut + 1e-4f;
 }
 
-__global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t per_xcd) {
+__global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   __shared__ float4 s_g[CT];  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
   __shared__ float4 s_m[CT];  // mu_x, mu_y, r, g
   __shared__ float s_b[CT];   // blue
   __shared__ uint64_t s_mask[4][CT / 64];  // [quadrant][loading wave]
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
-  const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  const uint32_t tile = blockIdx.x;
   if (tile >= total_tiles) return;
   const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
   const uint32_t t = threadIdx.x;
@@ -75,12 +79,14 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t p
 
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   bool alive = inside;
+  uint32_t g_next = (rs + t < re) ? A.sorted_ids[rs + t] : 0u;
   for (uint32_t b0 = 0; b0 < n; b0 += CT) {
     // ---- gather the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
+    const uint32_t g = g_next;
+    if (e + (uint32_t)CT < re) g_next = A.sorted_ids[e + CT];  // next batch's id
     bool q0 = false, q1 = false, q2 = false, q3 = false;
     if (e < re) {
-      const uint32_t g = A.sorted_ids[e];
       const float4* gp = reinterpret_cast<const float4*>(A.geom + (size_t)g * 8);
       const float4 g0 = gp[0];  // mu_x, mu_y, a, b
       const float4 g1 = gp[1];  // c, alpha, tz, q_cut
@@ -187,6 +193,5 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A, uint32_t p
 void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s) {
   const uint32_t total = a.tiles_x * a.tiles_y;
   if (total == 0) return;
-  const uint32_t per_xcd = (total + 7u) / 8u;
-  hipLaunchKernelGGL(k_composite, dim3(per_xcd * 8u), dim3(CT), 0, s, a, per_xcd);
+  hipLaunchKernelGGL(k_composite, dim3(total), dim3(CT), 0, s, a);
 }

@@ -151,13 +151,17 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     const bool valid = idx < n;
     const uint32_t d = (keys[i] >> A.shift) & dmask;
-    uint64_t m = __ballot(valid);
+    // lanes with the same digit: for each bit, keep the lanes whose bit equals
+    // this lane's (sx = all ones when the bit is set, so ~(bb ^ sx) keeps them)
+    uint32_t mlo = (uint32_t)__ballot(valid), mhi = (uint32_t)(__ballot(valid) >> 32);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      m &= bit ? bb : ~bb;
+      const uint32_t sx = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+      const uint64_t bb = __ballot(sx != 0u);
+      mlo &= ~((uint32_t)bb ^ sx);
+      mhi &= ~((uint32_t)(bb >> 32) ^ sx);
     }
+    const uint64_t m = ((uint64_t)mhi << 32) | mlo;
     const uint32_t prior = cgs::mask_rank(m);
     uint32_t wcount = 0;
     if (valid) wcount = s_whist[wave][d];
