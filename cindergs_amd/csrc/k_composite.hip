@@ -15,9 +15,10 @@
 //    with the exact ellipse-vs-rectangle test the binning uses
 //    (binning.rs:35-83, on the quadrant's pixel-centre rectangle); the four
 //    results become 4 ballot masks per loading wave;
-//  * each wave then walks only the set bits of its 256-bit mask, in order
-//    (front to back), with a scalar loop: no divergent branches, and the
-//    blend itself is branch-free (select instead of if/break);
+//  * each wave compacts the set bits of its 256-bit mask into an LDS list of
+//    splat slots (order kept) and walks the list front to back four splats
+//    per step: the four LDS reads and q / e^q chains are independent, only
+//    the blend is sequential; the blend is branch-free (select, not if/break);
 //  * a wave stops when none of its pixels is alive, the tile stops after a
 //    batch in which no pixel is alive (__syncthreads_or);
 //  * the conic is stored pre-scaled by -1/2 (and b by -1), which is exact, so
@@ -27,8 +28,9 @@
 //    XCDs in turn, so XCD k gets the tile columns x = k (mod 8) (the row
 //    length 120 at 1080p is a multiple of 8) and the dense middle of the
 //    image is spread over all eight; vertical neighbours share an L2;
-//  * the splat ids of the next batch are loaded while the current batch is
-//    composited, so a batch waits for one memory round trip, not two.
+//  * the records of the next batch are loaded (and the ids of the one after)
+//    while the current batch is composited, so the memory round trips of a
+//    batch overlap the previous batch's blending.
 #include "cgs_internal.h"
 
 namespace {
@@ -56,11 +58,40 @@ __device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float 
   return cgs_fmin(q1, q2) <= qcut + 1e-4f;
 }
 
+// One splat's compositor inputs as staged in LDS.
+struct Splat {
+  float4 g;  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
+  float4 m;  // mu_x, mu_y, r, g
+  float b;   // blue
+};
+
+__device__ __forceinline__ float splat_q(const Splat& s, float pcx, float pcy) {
+  const float dx = pcx - s.m.x, dy = pcy - s.m.y;
+  // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
+  const float q = s.g.x * dx * dx + s.g.y * dx * dy + s.g.z * dy * dy;
+  return q > 0.0f ? 0.0f : q;
+}
+
+// Front-to-back blend of one splat into the pixel state (branch-free).
+__device__ __forceinline__ void blend(const Splat& s, float q, float a, bool valid, bool& alive,
+                                      float& T, float& cr, float& cg, float& cb) {
+  const bool ok = valid && alive && q >= -4.0f && a >= ALPHA_MIN;
+  const float w = ok ? a * T : 0.0f;
+  cr = __builtin_fmaf(w, s.m.z, cr);
+  cg = __builtin_fmaf(w, s.m.w, cg);
+  cb = __builtin_fmaf(w, s.b, cb);
+  const float Tn = T * (1.0f - a);
+  const bool sat = ok && Tn < 0.001f;
+  T = ok ? (sat ? 0.0f : Tn) : T;
+  alive = alive && !sat;
+}
+
 __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
-  __shared__ float4 s_g[CT];  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
-  __shared__ float4 s_m[CT];  // mu_x, mu_y, r, g
-  __shared__ float s_b[CT];   // blue
-  __shared__ uint64_t s_mask[4][CT / 64];  // [quadrant][loading wave]
+  __shared__ float4 s_g[CT];
+  __shared__ float4 s_m[CT];
+  __shared__ float s_b[CT];
+  __shared__ uint64_t s_mask[4][CT / 64];     // [quadrant][loading wave]
+  __shared__ __align__(16) uint32_t s_list[4][CT];  // [quadrant] splats that can touch it
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
   const uint32_t tile = blockIdx.x;
@@ -71,38 +102,48 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   const uint32_t qx0 = tx * 16u + (wave & 1u) * 8u, qy0 = ty * 16u + (wave >> 1) * 8u;
   const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
   const bool inside = px < A.width && py < A.height;
+  const uint32_t bx = tx * 16u, by = ty * 16u;
 
   const uint32_t rs = A.ranges[tile * 2u], re = A.ranges[tile * 2u + 1u];
   const bool empty = rs >= re;
   const uint32_t n = empty ? 0u : re - rs;
   const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
+  const float4* geom4 = reinterpret_cast<const float4*>(A.geom);
+  const float4* color4 = reinterpret_cast<const float4*>(A.color);
+
+  // records of the batch about to be staged, loaded one batch ahead.  The
+  // loads are unconditional, at indexes clamped to the tile's last entry, so
+  // that nothing waits for them before they are used a batch later.
+  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, rc = r0;
+  uint32_t g_ahead = 0u;
+  const uint32_t last = empty ? rs : re - 1u;
+  if (!empty) {
+    const uint32_t g = A.sorted_ids[min(rs + t, last)];
+    r0 = geom4[(size_t)g * 2];
+    r1 = geom4[(size_t)g * 2 + 1];
+    rc = color4[g];
+    g_ahead = A.sorted_ids[min(rs + t + (uint32_t)CT, last)];
+  }
 
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   bool alive = inside;
-  uint32_t g_next = (rs + t < re) ? A.sorted_ids[rs + t] : 0u;
   for (uint32_t b0 = 0; b0 < n; b0 += CT) {
-    // ---- gather the batch and test it against the 4 quadrants ----
+    // ---- stage the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
-    const uint32_t g = g_next;
-    if (e + (uint32_t)CT < re) g_next = A.sorted_ids[e + CT];  // next batch's id
     bool q0 = false, q1 = false, q2 = false, q3 = false;
     if (e < re) {
-      const float4* gp = reinterpret_cast<const float4*>(A.geom + (size_t)g * 8);
-      const float4 g0 = gp[0];  // mu_x, mu_y, a, b
-      const float4 g1 = gp[1];  // c, alpha, tz, q_cut
-      const float4 col = reinterpret_cast<const float4*>(A.color)[g];
-      const float op = g1.y > ALPHA_MIN ? g1.y : 0.0f;
-      s_g[t] = make_float4(-0.5f * g0.z, -g0.w, -0.5f * g1.x, op);
-      s_m[t] = make_float4(g0.x, g0.y, col.x, col.y);
-      s_b[t] = col.z;
-      const uint32_t bx = tx * 16u, by = ty * 16u;
-      q0 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx, by, 8u, 8u,
+      // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
+      const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
+      s_g[t] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
+      s_m[t] = make_float4(r0.x, r0.y, rc.x, rc.y);
+      s_b[t] = rc.z;
+      q0 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by, 8u, 8u,
                              A.width, A.height);
-      q1 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx + 8u, by, 8u, 8u,
+      q1 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + 8u, by, 8u, 8u,
                              A.width, A.height);
-      q2 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx, by + 8u, 8u, 8u,
+      q2 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by + 8u, 8u, 8u,
                              A.width, A.height);
-      q3 = ellipse_hits_rect(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.w, bx + 8u, by + 8u, 8u,
+      q3 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + 8u, by + 8u, 8u,
                              8u, A.width, A.height);
     }
     const uint64_t m0 = __ballot(q0), m1 = __ballot(q1), m2 = __ballot(q2), m3 = __ballot(q3);
@@ -112,62 +153,46 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
       s_mask[2][wave] = m2;
       s_mask[3][wave] = m3;
     }
+    // the next batch's records are loaded while this batch is composited
+    const uint32_t en = e + CT;
+    r0 = geom4[(size_t)g_ahead * 2];
+    r1 = geom4[(size_t)g_ahead * 2 + 1];
+    rc = color4[g_ahead];
+    g_ahead = A.sorted_ids[min(en + (uint32_t)CT, last)];
     __syncthreads();
 
-    // ---- this wave's quadrant: only the splats that can touch it ----
+    // ---- compact this wave's quadrant list (splat order kept) ----
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < (uint32_t)(CT / 64); ++v) {
+      const uint64_t mv = s_mask[wave][v];
+      if ((mv >> lane) & 1ull) s_list[wave][cnt + cgs::mask_rank(mv)] = v * 64u + lane;
+      cnt += (uint32_t)__popcll(mv);
+    }
+    cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- walk it four splats at a time, front to back ----
     if (__ballot(alive) != 0ull) {
 #pragma unroll 1
-      for (uint32_t v = 0; v < (uint32_t)(CT / 64); ++v) {
-        // the mask is wave-uniform; readfirstlane returns int, so widen through
-        // uint32_t (a sign extension would set the upper 32 bits)
-        const uint64_t mv = s_mask[wave][v];
-        const uint32_t mlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mv);
-        const uint32_t mhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(mv >> 32));
-        uint64_t m = ((uint64_t)mhi << 32) | (uint64_t)mlo;
-        // two splats per iteration: their LDS reads and q / e^q chains are
-        // independent, only the blend is sequential (front to back)
-        while (m != 0ull) {
-          const uint32_t k0 = v * 64u + (uint32_t)__builtin_ctzll(m);
-          m &= m - 1ull;
-          const bool two = m != 0ull;  // wave-uniform
-          const uint32_t k1 = two ? v * 64u + (uint32_t)__builtin_ctzll(m) : k0;
-          m = two ? (m & (m - 1ull)) : m;
-          const float4 g0 = s_g[k0], m0 = s_m[k0];
-          const float4 g1 = s_g[k1], m1 = s_m[k1];
-          const float b0 = s_b[k0], b1 = s_b[k1];
-          const float dx0 = pcx - m0.x, dy0 = pcy - m0.y;
-          const float dx1 = pcx - m1.x, dy1 = pcy - m1.y;
-          // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
-          float q0 = g0.x * dx0 * dx0 + g0.y * dx0 * dy0 + g0.z * dy0 * dy0;
-          float q1 = g1.x * dx1 * dx1 + g1.y * dx1 * dy1 + g1.z * dy1 * dy1;
-          q0 = q0 > 0.0f ? 0.0f : q0;
-          q1 = q1 > 0.0f ? 0.0f : q1;
-          const float a0 = fminf(0.99f, g0.w * __expf(q0));
-          const float a1 = fminf(0.99f, g1.w * __expf(q1));
-          {
-            const bool ok = alive && q0 >= -4.0f && a0 >= ALPHA_MIN;
-            const float w = ok ? a0 * T : 0.0f;
-            cr = __builtin_fmaf(w, m0.z, cr);
-            cg = __builtin_fmaf(w, m0.w, cg);
-            cb = __builtin_fmaf(w, b0, cb);
-            const float Tn = T * (1.0f - a0);
-            const bool sat = ok && Tn < 0.001f;
-            T = ok ? (sat ? 0.0f : Tn) : T;
-            alive = alive && !sat;
-          }
-          {
-            const bool ok = two && alive && q1 >= -4.0f && a1 >= ALPHA_MIN;
-            const float w = ok ? a1 * T : 0.0f;
-            cr = __builtin_fmaf(w, m1.z, cr);
-            cg = __builtin_fmaf(w, m1.w, cg);
-            cb = __builtin_fmaf(w, b1, cb);
-            const float Tn = T * (1.0f - a1);
-            const bool sat = ok && Tn < 0.001f;
-            T = ok ? (sat ? 0.0f : Tn) : T;
-            alive = alive && !sat;
-          }
-          if (__ballot(alive) == 0ull) break;
-        }
+      for (uint32_t i = 0; i < cnt; i += 4u) {
+        const uint4 id4 = *reinterpret_cast<const uint4*>(&s_list[wave][i]);
+        // wave-uniform: entries past cnt repeat the first (always valid) one
+        const bool v1 = i + 1u < cnt, v2 = i + 2u < cnt, v3 = i + 3u < cnt;
+        const uint32_t k0 = id4.x, k1 = v1 ? id4.y : id4.x, k2 = v2 ? id4.z : id4.x,
+                       k3 = v3 ? id4.w : id4.x;
+        Splat S0{s_g[k0], s_m[k0], s_b[k0]}, S1{s_g[k1], s_m[k1], s_b[k1]};
+        Splat S2{s_g[k2], s_m[k2], s_b[k2]}, S3{s_g[k3], s_m[k3], s_b[k3]};
+        const float qa = splat_q(S0, pcx, pcy), qb = splat_q(S1, pcx, pcy);
+        const float qc = splat_q(S2, pcx, pcy), qd = splat_q(S3, pcx, pcy);
+        const float aa = fminf(0.99f, S0.g.w * __expf(qa));
+        const float ab = fminf(0.99f, S1.g.w * __expf(qb));
+        const float ac = fminf(0.99f, S2.g.w * __expf(qc));
+        const float ad = fminf(0.99f, S3.g.w * __expf(qd));
+        blend(S0, qa, aa, true, alive, T, cr, cg, cb);
+        blend(S1, qb, ab, v1, alive, T, cr, cg, cb);
+        blend(S2, qc, ac, v2, alive, T, cr, cg, cb);
+        blend(S3, qd, ad, v3, alive, T, cr, cg, cb);
         if (__ballot(alive) == 0ull) break;
       }
     }
