@@ -305,8 +305,9 @@ struct CgsCompositeArgs {
   uint32_t width, height, tiles_x, tiles_y, padded_row;
   float bg[4];
   uint32_t* out;
+  uint32_t* order;  // [tiles] dispatch order scratch (null: tile order)
 };
-void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s);
+void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof = nullptr);
 
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved

@@ -21,9 +21,10 @@
 //    the blend is sequential; the blend is branch-free (select, not if/break);
 //  * a wave stops when none of its pixels is alive, the tile stops after a
 //    batch in which no pixel is alive (__syncthreads_or);
-//  * the conic is stored pre-scaled by -1/2 (and b by -1), which is exact, so
-//    q has the same bits as the reference expression; e^q uses the hardware
-//    exp2 (the colour parity bound is 1/255);
+//  * the conic is stored pre-scaled by -log2(e)/2 (b by -log2(e)) so that
+//    q2 = log2(e) q takes two FMAs and e^q is one hardware exp2; the blend
+//    folds T (1 - alpha) into T - alpha T.  These change only rounding (the
+//    colour parity bound is 1/255);
 //  * block b composites tile b: the hardware deals consecutive blocks to the 8
 //    XCDs in turn, so XCD k gets the tile columns x = k (mod 8) (the row
 //    length 120 at 1080p is a multiple of 8) and the dense middle of the
@@ -60,42 +61,96 @@ __device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float 
 
 // One splat's compositor inputs as staged in LDS.
 struct Splat {
-  float4 g;  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
+  float4 g;  // -a/2, -b, -c/2 scaled by log2(e), opacity (0 when <= 1/255)
   float4 m;  // mu_x, mu_y, r, g
   float b;   // blue
 };
 
-__device__ __forceinline__ float splat_q(const Splat& s, float pcx, float pcy) {
+constexpr float LOG2E = 1.4426950408889634f;
+constexpr float Q2_MIN = -4.0f * LOG2E;  // q >= -4 in base-2 units
+
+// log2(e) * min(q, 0) with q = -1/2 (a dx^2 + 2 b dx dy + c dy^2), as
+// dx (a' dx + b' dy) + c' dy^2 (two fused multiply-adds)
+__device__ __forceinline__ float splat_q2(const Splat& s, float pcx, float pcy) {
   const float dx = pcx - s.m.x, dy = pcy - s.m.y;
-  // == -0.5f * (a*dx*dx + 2*b*dx*dy + c*dy*dy) bit for bit
-  const float q = s.g.x * dx * dx + s.g.y * dx * dy + s.g.z * dy * dy;
-  return q > 0.0f ? 0.0f : q;
+  const float u = __builtin_fmaf(s.g.y, dy, s.g.x * dx);
+  const float q = __builtin_fmaf(u, dx, s.g.z * dy * dy);
+  return fminf(q, 0.0f);
 }
 
-// Front-to-back blend of one splat into the pixel state (branch-free).
-__device__ __forceinline__ void blend(const Splat& s, float q, float a, bool valid, bool& alive,
+// Front-to-back blend of one splat into the pixel state (branch-free):
+// w = alpha T, C += w c, T -= w (= T (1 - alpha)); T < 1e-3 saturates the
+// pixel (T = 0) after the splat is added.  A pixel that is not alive has
+// ok = false, so w = 0 and its T stays 0 (saturated) or 1 (off screen).
+__device__ __forceinline__ void blend(const Splat& s, float q2, float a, bool valid, bool& alive,
                                       float& T, float& cr, float& cg, float& cb) {
-  const bool ok = valid && alive && q >= -4.0f && a >= ALPHA_MIN;
-  const float w = ok ? a * T : 0.0f;
+  const bool ok = valid && alive && q2 >= Q2_MIN && a >= ALPHA_MIN;
+  const float w = (ok ? a : 0.0f) * T;
   cr = __builtin_fmaf(w, s.m.z, cr);
   cg = __builtin_fmaf(w, s.m.w, cg);
   cb = __builtin_fmaf(w, s.b, cb);
-  const float Tn = T * (1.0f - a);
-  const bool sat = ok && Tn < 0.001f;
-  T = ok ? (sat ? 0.0f : Tn) : T;
+  const float Tn = T - w;
+  const bool sat = Tn < 0.001f;
+  T = sat ? 0.0f : Tn;
   alive = alive && !sat;
 }
 
+__device__ __forceinline__ Splat staged(const float4 (*st)[CT], uint32_t off) {
+  const char* p = reinterpret_cast<const char*>(&st[0][0]) + off;
+  Splat s;
+  s.g = *reinterpret_cast<const float4*>(p);
+  s.m = *reinterpret_cast<const float4*>(p + CT * 16);
+  s.b = *reinterpret_cast<const float*>(p + 2 * CT * 16);
+  return s;
+}
+
+// Dispatch order of the tiles: the heaviest first (entries > 2048, then
+// > 512, then > 64, then the rest), so that the few tiles that take longest
+// start at the beginning of the launch.  One 1024-thread block; thread t owns
+// the contiguous tiles [t*per, (t+1)*per) (per <= 32), so one block scan of the
+// per-thread class counts orders every class by tile index.  Counts of two
+// classes share a word (tiles <= 32768 < 2^16).
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* ranges, uint32_t tiles,
+                                                     uint32_t* order) {
+  __shared__ uint32_t s_tmp[16];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (tiles + 1023u) / 1024u;
+  const uint32_t t0 = min(t * per, tiles), t1 = min(t0 + per, tiles);
+  uint64_t cls = 0;  // 2 bits per owned tile
+  uint32_t c01 = 0, c23 = 0;
+  for (uint32_t i = t0; i < t1; ++i) {
+    const uint2 r = reinterpret_cast<const uint2*>(ranges)[i];
+    const uint32_t n = r.y > r.x ? r.y - r.x : 0u;
+    const uint32_t k = n > 2048u ? 0u : n > 512u ? 1u : n > 64u ? 2u : 3u;
+    cls |= (uint64_t)k << (2u * (i - t0));
+    c01 += k == 0u ? 1u : k == 1u ? 0x10000u : 0u;
+    c23 += k == 2u ? 1u : k == 3u ? 0x10000u : 0u;
+  }
+  uint32_t a, c;
+  const uint32_t o01 = cgs::block_exclusive_scan<1024>(c01, s_tmp, a);
+  const uint32_t o23 = cgs::block_exclusive_scan<1024>(c23, s_tmp, c);
+  uint32_t pos[4];
+  pos[0] = o01 & 0xFFFFu;
+  pos[1] = (a & 0xFFFFu) + (o01 >> 16);
+  pos[2] = (a & 0xFFFFu) + (a >> 16) + (o23 & 0xFFFFu);
+  pos[3] = (a & 0xFFFFu) + (a >> 16) + (c & 0xFFFFu) + (o23 >> 16);
+  for (uint32_t i = t0; i < t1; ++i) {
+    const uint32_t k = (uint32_t)(cls >> (2u * (i - t0))) & 3u;
+    order[pos[k]++] = i;
+  }
+}
+
 __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
-  __shared__ float4 s_g[CT];
-  __shared__ float4 s_m[CT];
-  __shared__ float s_b[CT];
+  // staged batch: plane 0 = Splat::g, 1 = Splat::m, 2 = (blue, -, -, -); the
+  // quadrant lists hold byte offsets into a plane, so a splat's three reads
+  // share one address register
+  __shared__ float4 s_stage[3][CT];
   __shared__ uint64_t s_mask[4][CT / 64];     // [quadrant][loading wave]
   __shared__ __align__(16) uint32_t s_list[4][CT];  // [quadrant] splats that can touch it
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
-  const uint32_t tile = blockIdx.x;
-  if (tile >= total_tiles) return;
+  if (blockIdx.x >= total_tiles) return;
+  const uint32_t tile = A.order ? A.order[blockIdx.x] : blockIdx.x;
   const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
   const uint32_t t = threadIdx.x;
   const uint32_t wave = t >> 6, lane = t & 63u;
@@ -134,9 +189,9 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
     if (e < re) {
       // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
       const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
-      s_g[t] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
-      s_m[t] = make_float4(r0.x, r0.y, rc.x, rc.y);
-      s_b[t] = rc.z;
+      s_stage[0][t] = make_float4(-0.5f * LOG2E * r0.z, -LOG2E * r0.w, -0.5f * LOG2E * r1.x, op);
+      s_stage[1][t] = make_float4(r0.x, r0.y, rc.x, rc.y);
+      reinterpret_cast<float*>(&s_stage[2][t])[0] = rc.z;
       q0 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by, 8u, 8u,
                              A.width, A.height);
       q1 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + 8u, by, 8u, 8u,
@@ -166,7 +221,7 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
 #pragma unroll
     for (uint32_t v = 0; v < (uint32_t)(CT / 64); ++v) {
       const uint64_t mv = s_mask[wave][v];
-      if ((mv >> lane) & 1ull) s_list[wave][cnt + cgs::mask_rank(mv)] = v * 64u + lane;
+      if ((mv >> lane) & 1ull) s_list[wave][cnt + cgs::mask_rank(mv)] = (v * 64u + lane) * 16u;
       cnt += (uint32_t)__popcll(mv);
     }
     cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
@@ -181,14 +236,14 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
         const bool v1 = i + 1u < cnt, v2 = i + 2u < cnt, v3 = i + 3u < cnt;
         const uint32_t k0 = id4.x, k1 = v1 ? id4.y : id4.x, k2 = v2 ? id4.z : id4.x,
                        k3 = v3 ? id4.w : id4.x;
-        Splat S0{s_g[k0], s_m[k0], s_b[k0]}, S1{s_g[k1], s_m[k1], s_b[k1]};
-        Splat S2{s_g[k2], s_m[k2], s_b[k2]}, S3{s_g[k3], s_m[k3], s_b[k3]};
-        const float qa = splat_q(S0, pcx, pcy), qb = splat_q(S1, pcx, pcy);
-        const float qc = splat_q(S2, pcx, pcy), qd = splat_q(S3, pcx, pcy);
-        const float aa = fminf(0.99f, S0.g.w * __expf(qa));
-        const float ab = fminf(0.99f, S1.g.w * __expf(qb));
-        const float ac = fminf(0.99f, S2.g.w * __expf(qc));
-        const float ad = fminf(0.99f, S3.g.w * __expf(qd));
+        const Splat S0 = staged(s_stage, k0), S1 = staged(s_stage, k1);
+        const Splat S2 = staged(s_stage, k2), S3 = staged(s_stage, k3);
+        const float qa = splat_q2(S0, pcx, pcy), qb = splat_q2(S1, pcx, pcy);
+        const float qc = splat_q2(S2, pcx, pcy), qd = splat_q2(S3, pcx, pcy);
+        const float aa = fminf(0.99f, S0.g.w * __builtin_amdgcn_exp2f(qa));
+        const float ab = fminf(0.99f, S1.g.w * __builtin_amdgcn_exp2f(qb));
+        const float ac = fminf(0.99f, S2.g.w * __builtin_amdgcn_exp2f(qc));
+        const float ad = fminf(0.99f, S3.g.w * __builtin_amdgcn_exp2f(qd));
         blend(S0, qa, aa, true, alive, T, cr, cg, cb);
         blend(S1, qb, ab, v1, alive, T, cr, cg, cb);
         blend(S2, qc, ac, v2, alive, T, cr, cg, cb);
@@ -215,8 +270,16 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
 
 }  // namespace
 
-void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s) {
+void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof) {
   const uint32_t total = a.tiles_x * a.tiles_y;
   if (total == 0) return;
-  hipLaunchKernelGGL(k_composite, dim3(total), dim3(CT), 0, s, a);
+  CgsCompositeArgs A = a;
+  if (A.order && total <= 32768u) {
+    if (prof) prof->mark("k_tile_order", s);
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, A.ranges, total, A.order);
+  } else {
+    A.order = nullptr;  // tile order (8K frames: one block would take too long)
+  }
+  if (prof) prof->mark("k_composite", s);
+  hipLaunchKernelGGL(k_composite, dim3(total), dim3(CT), 0, s, A);
 }

@@ -147,7 +147,7 @@ struct cgs_context {
   DevBuf geom, color, extents;
   DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
   uint32_t entry_cap = 0;
-  DevBuf ranges, out_rgba, cnt_dbg, off_dbg;
+  DevBuf ranges, out_rgba, cnt_dbg, off_dbg, comp_order;
   DevBuf ctrl3;  // frame control, see frame_layout()
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
@@ -438,8 +438,9 @@ int launch_stage4(cgs_context* c, const FrameGeom& g, const cgs_compositor_unifo
   A.padded_row = g.padded_row;
   for (int i = 0; i < 4; ++i) A.bg[i] = comp->background_color[i];
   A.out = P<uint32_t>(c->out_rgba);
-  c->prof.mark("k_composite", c->stream);
-  cgs_launch_composite(A, c->stream);
+  TRY(ensure_words(c->comp_order, g.T));
+  A.order = P<uint32_t>(c->comp_order);
+  cgs_launch_composite(A, c->stream, &c->prof);
   HIPCHK(hipGetLastError());
   return CGS_OK;
 }
@@ -512,7 +513,7 @@ void cgs_destroy(cgs_context* c) {
                     &c->extents,
                     &c->e_keys, &c->e_ids, &c->e_depths, &c->e_alt_k, &c->e_alt_v, &c->e_perm,
                     &c->ranges, &c->out_rgba, &c->cnt_dbg, &c->off_dbg, &c->ctrl3, &c->prim_ctrl,
-                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d};
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->comp_order};
   for (DevBuf* b : bufs) release(*b);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
