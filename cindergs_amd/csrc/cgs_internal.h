@@ -105,18 +105,16 @@ __device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
 }
 
 // Decoupled look-back, windowed.  Partition `part` publishes its aggregate,
-// then reads the statuses of up to LB_WINDOW predecessors at once and consumes
+// then reads the statuses of up to WIN predecessors at once and consumes
 // them nearest first: an aggregate is added and the walk goes on, an
 // inclusive prefix is added and ends it, a status not yet published stops the
 // window and is polled again.  On MI355X each status read is a cross-XCD
 // coherent load (~125 ns), so reading a window per round trip lets the prefix
-// travel LB_WINDOW partitions per memory latency instead of one.  Predecessors
+// travel WIN partitions per memory latency instead of one.  Predecessors
 // hold smaller partition ids taken from an atomic ticket (or launched earlier),
 // so they are resident or finished and every wait ends.  Called by one thread
 // per status column.
-#define CGS_LB_WINDOW 16
-
-template <class U, int VBITS>
+template <class U, int VBITS, int WIN>
 __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_t stride,
                                                U aggregate) {
   const U VMASK = ((U)1 << VBITS) - (U)1;
@@ -130,9 +128,9 @@ __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_
   U excl = 0;
   int64_t q = (int64_t)part - 1;  // nearest predecessor not yet accounted for
   while (true) {
-    U s[CGS_LB_WINDOW];
+    U s[WIN];
 #pragma unroll
-    for (int k = 0; k < CGS_LB_WINDOW; ++k) {
+    for (int k = 0; k < WIN; ++k) {
       const int64_t qq = q - k;
       s[k] = qq >= 0 ? __hip_atomic_load(&status[(size_t)qq * stride], __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT)
@@ -141,7 +139,7 @@ __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_
     int used = 0;
     bool done = false;
 #pragma unroll
-    for (int k = 0; k < CGS_LB_WINDOW; ++k) {
+    for (int k = 0; k < WIN; ++k) {
       if (done || used != k) continue;
       const uint32_t flag = (uint32_t)(s[k] >> VBITS);
       if (flag == 0) continue;  // not published yet: poll again from here
@@ -151,7 +149,7 @@ __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_
     }
     if (done) break;
     q -= used;
-    if (used < CGS_LB_WINDOW) __builtin_amdgcn_s_sleep(1);
+    if (used < WIN) __builtin_amdgcn_s_sleep(1);
   }
   __hip_atomic_store(&status[(size_t)part * stride],
                      ((U)CGS_LB_INC << VBITS) | ((excl + aggregate) & VMASK), __ATOMIC_RELAXED,
@@ -162,13 +160,13 @@ __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_
 // 64-bit status words {flag:2 | value:62}
 __device__ __forceinline__ uint64_t lookback_u64(uint64_t* status, uint32_t part,
                                                  uint64_t aggregate) {
-  return lookback_windowed<uint64_t, 62>(status, part, 1u, aggregate);
+  return lookback_windowed<uint64_t, 62, 16>(status, part, 1u, aggregate);
 }
 
 // 32-bit status words {flag:2 | value:30} (radix digit counts)
 __device__ __forceinline__ uint32_t lookback_u32(uint32_t* status, uint32_t part,
                                                  uint32_t stride, uint32_t aggregate) {
-  return lookback_windowed<uint32_t, 30>(status, part, stride, aggregate);
+  return lookback_windowed<uint32_t, 30, 16>(status, part, stride, aggregate);
 }
 
 }  // namespace cgs

@@ -139,12 +139,19 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
   // rank: wave w owns keys [base + w*1024, base + (w+1)*1024), item i of lane l
   // at offset i*64 + l, so items are visited in input order
-  uint32_t keys[RI], offs[RI];
+  // (the values are loaded here too, so that their latency overlaps the
+  // ranking instead of following the look-back)
+  uint32_t keys[RI], vals[RI], offs[RI];
   const uint32_t wbase = base + wave * (uint32_t)(RI * 64);
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     keys[i] = idx < n ? A.keys_in[idx] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
+    vals[i] = A.vals_in ? (idx < n ? A.vals_in[idx] : 0u) : idx;
   }
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
@@ -200,11 +207,10 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
     if (offs[i] != 0xFFFFFFFFu) {
-      const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
       const uint32_t d = (keys[i] >> A.shift) & dmask;
       const uint32_t pos = s_lstart[d] + s_whist[wave][d] + offs[i];
       s_keys[pos] = keys[i];
-      s_vals[pos] = A.vals_in ? A.vals_in[idx] : idx;
+      s_vals[pos] = vals[i];
     }
   }
   __syncthreads();
