@@ -16,7 +16,7 @@
 
 namespace {
 
-constexpr int RT = 512;          // threads (8 wave64s)
+constexpr int RT = 1024;         // threads (16 wave64s)
 constexpr int RI = 16;           // keys per thread
 constexpr int RP = RT * RI;      // keys per partition (8192)
 constexpr int RW = RT / 64;      // waves
