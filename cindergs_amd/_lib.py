@@ -122,6 +122,7 @@ SIGNATURES = {
     "cgs_set_capture": [_vp, _u32],
     "cgs_read_deformed": [_vp, _vp, _vp, _vp, _vp, C.POINTER(_u32)],
     "cgs_read_tet_transforms": [_vp, _vp, _vp, _vp],
+    "cgs_bind_tets": [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_load_projected": [_vp, _vp, _vp, _u64],
     "cgs_run_stage3": [_vp, C.POINTER(CameraUniforms), C.POINTER(CompositorUniforms),
                        C.POINTER(_u64)],

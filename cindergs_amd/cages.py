@@ -32,6 +32,37 @@ for perm in itertools.permutations((1, 2, 4)):
     _KUHN.append((0, a, b, 7))
 
 
+def _cross(a, b):
+    """glam Vec3::cross, f32, the crate's operation order."""
+    return np.stack([a[:, 1] * b[:, 2] - b[:, 1] * a[:, 2], a[:, 2] * b[:, 0] - b[:, 2] * a[:, 0],
+                     a[:, 0] * b[:, 1] - b[:, 0] * a[:, 1]], 1)
+
+
+def _dot(a, b):
+    return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+
+def inverse_rest_matrices(rest_vertices: np.ndarray, elements: np.ndarray) -> np.ndarray:
+    """D_m^-1 per tet as TetMesh::new computes it (src/deformation/mod.rs:202-227):
+    D_m = [X1-X0, X2-X0, X3-X0] in f32, glam 0.28's Mat3::inverse (column cross
+    products, det = z . (x cross y), times 1/det, transposed) when |det| > 1e-8,
+    else the identity; [T, 9] column-major (Mat3::to_cols_array)."""
+    X = np.asarray(rest_vertices, dtype=np.float32)
+    e = np.asarray(elements, dtype=np.int64)
+    x0 = X[e[:, 0]]
+    cx, cy, cz = X[e[:, 1]] - x0, X[e[:, 2]] - x0, X[e[:, 3]] - x0
+    t0, t1, t2 = _cross(cy, cz), _cross(cz, cx), _cross(cx, cy)
+    det = _dot(cz, t2)
+    ok = np.abs(det) > np.float32(1e-8)
+    inv = (np.float32(1.0) / np.where(ok, det, np.float32(1.0))).astype(np.float32)
+    r0, r1, r2 = t0 * inv[:, None], t1 * inv[:, None], t2 * inv[:, None]
+    # transposed: column j of the inverse holds component j of r0, r1, r2
+    out = np.stack([r0[:, 0], r1[:, 0], r2[:, 0], r0[:, 1], r1[:, 1], r2[:, 1],
+                    r0[:, 2], r1[:, 2], r2[:, 2]], 1).astype(np.float32)
+    out[~ok] = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1], np.float32)
+    return np.ascontiguousarray(out)
+
+
 class TetMesh:
     """TetMesh (src/deformation/mod.rs:191-227) with a cell index for binding."""
 
@@ -42,16 +73,7 @@ class TetMesh:
         self.cells = cells
         self.lo = lo
         self.hi = hi
-        X = self.rest_vertices.astype(np.float64)
-        e = self.elements.astype(np.int64)
-        Dm = np.stack([X[e[:, 1]] - X[e[:, 0]], X[e[:, 2]] - X[e[:, 0]], X[e[:, 3]] - X[e[:, 0]]],
-                      axis=2)  # columns
-        det = np.linalg.det(Dm)
-        inv = np.linalg.inv(np.where(np.abs(det)[:, None, None] > 1e-8, Dm, np.eye(3)))
-        inv[np.abs(det) <= 1e-8] = np.eye(3)
-        # column-major 9 floats per tet (Mat3::to_cols_array)
-        self.inv_dm = np.ascontiguousarray(np.transpose(inv, (0, 2, 1)).reshape(-1, 9),
-                                           dtype=np.float32)
+        self.inv_dm = inverse_rest_matrices(self.rest_vertices, self.elements)
 
     @property
     def num_tets(self) -> int:
@@ -95,13 +117,27 @@ class TetMesh:
         return TetMesh(V.astype(np.float32), np.array(elems, dtype=np.uint32), cells, lo, hi)
 
     def barycentric(self, p: np.ndarray, tets: np.ndarray) -> np.ndarray:
-        """[w0..w3] of points p[n,3] in tets[n] (bind_point's f32 arithmetic)."""
+        """[w0..w3] of points p[n,3] in tets[n] with bind_point's f32 operation
+        order (mod.rs:280-290): w_r = (m[0,r] dx + m[1,r] dy) + m[2,r] dz."""
         X0 = self.rest_vertices[self.elements[tets, 0]]
-        M = self.inv_dm[tets].reshape(-1, 3, 3).transpose(0, 2, 1)  # row-major view
+        m = self.inv_dm[tets]  # column-major: m[:, col*3 + row]
         d = (p.astype(np.float32) - X0).astype(np.float32)
-        w123 = np.einsum("nij,nj->ni", M, d).astype(np.float32)
-        w0 = (np.float32(1.0) - (w123[:, 0] + w123[:, 1] + w123[:, 2])).astype(np.float32)
-        return np.concatenate([w0[:, None], w123], 1)
+        w = [(m[:, r] * d[:, 0] + m[:, 3 + r] * d[:, 1]) + m[:, 6 + r] * d[:, 2]
+             for r in range(3)]
+        w0 = np.float32(1.0) - ((w[0] + w[1]) + w[2])
+        return np.stack([w0, w[0], w[1], w[2]], 1).astype(np.float32)
+
+    def bounds(self):
+        """TetMesh::bounds (mod.rs:242-256): AABB padded by 1e-3 of its size."""
+        lo = self.rest_vertices.min(0)
+        hi = self.rest_vertices.max(0)
+        pad = np.maximum(hi - lo, np.float32(1e-3)) * np.float32(1e-3)
+        return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+
+    def bind_gaussians_gpu(self, state, positions: np.ndarray, bounds=None):
+        """bind_gaussians(_with_bounds) on the GPU (grid-accelerated, same result
+        as the reference's index-order scan)."""
+        return state.bind_tets(self.rest_vertices, self.elements, self.inv_dm, positions, bounds)
 
     def bind_gaussians(self, positions: np.ndarray):
         """bind_point for every position: first tet in index order with penalty
@@ -178,3 +214,22 @@ class CageSpringSimulator:
         self.v = ((self.v + accel * dt) * gamma).astype(np.float32)
         self.x = (self.x + self.v * dt).astype(np.float32)
         return self.x
+
+
+def auto_bind_gpu(state, meshes, positions: np.ndarray):
+    """CageScene::auto_bind (src/deformation/mod.rs:444-464): each cage binds the
+    splats inside its padded rest bounds; the first cage in the list keeps a
+    splat bound by several.  Returns (cage[n] u32, tet[n] u32, weights[n,4]);
+    cage = 0xFFFFFFFF for an unbound splat."""
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    n = p.shape[0]
+    cage = np.full(n, UNBOUND, np.uint32)
+    tet = np.zeros(n, np.uint32)
+    w = np.zeros((n, 4), np.float32)
+    for ci, m in enumerate(meshes):
+        t, ww = m.bind_gaussians_gpu(state, p, m.bounds())
+        take = (cage == UNBOUND) & (t != UNBOUND)
+        cage[take] = ci
+        tet[take] = t[take]
+        w[take] = ww[take]
+    return cage, tet, w

@@ -307,6 +307,23 @@ struct CgsCompositeArgs {
 };
 void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof = nullptr);
 
+struct CgsBindArgs {
+  const float* tet_rec;  // [T][12]: X0 xyz, inv_dm column-major
+  uint32_t num_tets;
+  const float* pos;      // [n][3]
+  uint32_t n;
+  uint32_t has_bounds;
+  float bmin[3], bmax[3];
+  // uniform grid: cell (x, y, z) = ((z * dims[1] + y) * dims[0] + x); null = none
+  const uint32_t* cell_start;  // [cells + 1]
+  const uint32_t* cell_tets;   // tet ids per cell, ascending
+  float lo[3], inv_cell[3];
+  uint32_t dims[3];
+  uint32_t* out_tet;  // [n], 0xFFFFFFFF = unbound
+  float* out_w;       // [n][4]
+};
+void cgs_launch_bind_tets(const CgsBindArgs& a, hipStream_t s);
+
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved
   const uint32_t* elems;  // [4*T]

@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -1224,6 +1225,191 @@ int cgs_radix_sort_u64_pairs(cgs_context* c, uint32_t* tiles, uint32_t* depths, 
   HIPCHK(hipMemcpyAsync(ids, si, n * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
+}
+
+// TetMesh::bind_gaussians / bind_gaussians_with_bounds (src/deformation/
+// mod.rs:263-323) on the GPU.  Host arrays in, host arrays out.
+int cgs_bind_tets(cgs_context* c, const float* rest_vertices, uint32_t num_vertices,
+                  const uint32_t* elements, const float* inv_dm, uint32_t num_tets,
+                  const float* positions, uint64_t n, const float* bounds, uint32_t* out_tet,
+                  float* out_weights) {
+  TRY(ensure_ctx(c));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!rest_vertices || !elements || !inv_dm || !positions || !out_tet || !out_weights)
+    return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  if (num_tets == 0) return fail(CGS_ERR_INVALID_ARG, "a cage needs at least one tetrahedron");
+  for (uint32_t t = 0; t < num_tets; ++t)
+    for (int k = 0; k < 4; ++k)
+      if (elements[(size_t)t * 4 + k] >= num_vertices)
+        return fail(CGS_ERR_INVALID_ARG, "tet vertex index out of range");
+  // per-tet record (X0, inv_dm) and the box that holds every point the tet can
+  // accept: p = X0 + A w with A = inv_dm^-1, and a penalty <= 1e-4 keeps p
+  // within 1e-4 of a diameter of the corners X0 + A {0, e1, e2, e3}
+  std::vector<float> rec((size_t)num_tets * 12);
+  std::vector<double> box((size_t)num_tets * 6);
+  bool grid_ok = true;
+  double glo[3] = {1e300, 1e300, 1e300}, ghi[3] = {-1e300, -1e300, -1e300};
+  for (uint32_t t = 0; t < num_tets; ++t) {
+    const float* x0 = rest_vertices + (size_t)elements[(size_t)t * 4] * 3;
+    const float* m = inv_dm + (size_t)t * 9;  // column-major: m[col*3 + row]
+    float* r = rec.data() + (size_t)t * 12;
+    for (int k = 0; k < 3; ++k) r[k] = x0[k];
+    for (int k = 0; k < 9; ++k) r[3 + k] = m[k];
+    // A = inverse of M (row-major M[row][col] = m[col*3 + row]), in double
+    double M[3][3];
+    for (int row = 0; row < 3; ++row)
+      for (int col = 0; col < 3; ++col) M[row][col] = m[col * 3 + row];
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                       M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) {
+      grid_ok = false;  // a tet that accepts an unbounded region: scan everything
+      continue;
+    }
+    double Ai[3][3];
+    Ai[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    Ai[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    Ai[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    Ai[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    Ai[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    Ai[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    Ai[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    Ai[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    Ai[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = x0[k];
+    for (int e = 0; e < 3; ++e)
+      for (int k = 0; k < 3; ++k) {
+        const double v = x0[k] + Ai[k][e];
+        lo[k] = std::min(lo[k], v);
+        hi[k] = std::max(hi[k], v);
+      }
+    double diam = 0.0, mag = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      diam = std::max(diam, hi[k] - lo[k]);
+      mag = std::max(mag, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+    }
+    const double pad = 1e-3 * diam + 1e-5 * mag + 1e-7;
+    for (int k = 0; k < 3; ++k) {
+      box[(size_t)t * 6 + k] = lo[k] - pad;
+      box[(size_t)t * 6 + 3 + k] = hi[k] + pad;
+      if (!std::isfinite(lo[k] - pad) || !std::isfinite(hi[k] + pad)) grid_ok = false;
+      glo[k] = std::min(glo[k], lo[k] - pad);
+      ghi[k] = std::max(ghi[k], hi[k] + pad);
+    }
+  }
+  // grid of about 2 cells per tet edge length, at most 128 per axis
+  uint32_t dims[3] = {1, 1, 1};
+  std::vector<uint32_t> cell_start, cell_tets;
+  double cell[3] = {1, 1, 1};
+  if (grid_ok) {
+    const double cells_target = std::max(1.0, 8.0 * num_tets);
+    const double vol = std::max((ghi[0] - glo[0]) * (ghi[1] - glo[1]) * (ghi[2] - glo[2]), 1e-30);
+    const double h = std::cbrt(vol / cells_target);
+    for (int k = 0; k < 3; ++k) {
+      dims[k] = (uint32_t)std::min(128.0, std::max(1.0, std::ceil((ghi[k] - glo[k]) / h)));
+      cell[k] = (ghi[k] - glo[k]) / dims[k];
+      if (!(cell[k] > 0.0)) cell[k] = 1.0;
+    }
+    const size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<uint32_t> cnt(ncell + 1, 0u);
+    auto range = [&](uint32_t t, int k, uint32_t& a, uint32_t& b) {
+      const double fa = std::floor((box[(size_t)t * 6 + k] - glo[k]) / cell[k]);
+      const double fb = std::floor((box[(size_t)t * 6 + 3 + k] - glo[k]) / cell[k]);
+      a = (uint32_t)std::min<double>(dims[k] - 1, std::max(0.0, fa));
+      b = (uint32_t)std::min<double>(dims[k] - 1, std::max(0.0, fb));
+    };
+    for (int pass = 0; pass < 2 && grid_ok; ++pass) {
+      if (pass == 1) {
+        size_t total = 0;
+        for (size_t i = 0; i < ncell; ++i) total += cnt[i];
+        if (total > (64u << 20)) {  // boxes too large to be worth a grid
+          grid_ok = false;
+          break;
+        }
+        cell_start.assign(ncell + 1, 0u);
+        for (size_t i = 0; i < ncell; ++i) cell_start[i + 1] = cell_start[i] + cnt[i];
+        cell_tets.assign(cell_start[ncell], 0u);
+        std::fill(cnt.begin(), cnt.end(), 0u);
+      }
+      for (uint32_t t = 0; t < num_tets; ++t) {  // index order keeps lists ascending
+        uint32_t a[3], b[3];
+        for (int k = 0; k < 3; ++k) range(t, k, a[k], b[k]);
+        for (uint32_t z = a[2]; z <= b[2]; ++z)
+          for (uint32_t y = a[1]; y <= b[1]; ++y)
+            for (uint32_t x = a[0]; x <= b[0]; ++x) {
+              const size_t cidx = ((size_t)z * dims[1] + y) * dims[0] + x;
+              if (pass == 1) cell_tets[cell_start[cidx] + cnt[cidx]] = t;
+              cnt[cidx]++;
+            }
+      }
+    }
+  }
+  // device buffers (freed before returning: binding is a one-off per cage)
+  void *d_rec = nullptr, *d_pos = nullptr, *d_tet = nullptr, *d_w = nullptr;
+  void *d_start = nullptr, *d_cells = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {d_rec, d_pos, d_tet, d_w, d_start, d_cells})
+      if (p) (void)hipFree(p);
+  };
+  int rc = CGS_OK;
+  do {
+    if (hipMalloc(&d_rec, rec.size() * 4) != hipSuccess ||
+        hipMalloc(&d_pos, n * 12) != hipSuccess || hipMalloc(&d_tet, n * 4) != hipSuccess ||
+        hipMalloc(&d_w, n * 16) != hipSuccess) {
+      rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (binding buffers)");
+      break;
+    }
+    if (grid_ok && (hipMalloc(&d_start, cell_start.size() * 4) != hipSuccess ||
+                    hipMalloc(&d_cells, std::max<size_t>(cell_tets.size(), 1) * 4) != hipSuccess)) {
+      rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (binding grid)");
+      break;
+    }
+    hipError_t e = hipMemcpyAsync(d_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_pos, positions, n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && grid_ok) {
+      e = hipMemcpyAsync(d_start, cell_start.data(), cell_start.size() * 4, hipMemcpyHostToDevice,
+                         c->stream);
+      if (e == hipSuccess && !cell_tets.empty())
+        e = hipMemcpyAsync(d_cells, cell_tets.data(), cell_tets.size() * 4,
+                           hipMemcpyHostToDevice, c->stream);
+    }
+    if (e != hipSuccess) {
+      rc = fail(CGS_ERR_RUNTIME, std::string("upload: ") + hipGetErrorString(e));
+      break;
+    }
+    CgsBindArgs A;
+    memset(&A, 0, sizeof(A));
+    A.tet_rec = (const float*)d_rec;
+    A.num_tets = num_tets;
+    A.pos = (const float*)d_pos;
+    A.n = (uint32_t)n;
+    A.has_bounds = bounds ? 1u : 0u;
+    for (int k = 0; k < 3; ++k) {
+      A.bmin[k] = bounds ? bounds[k] : 0.0f;
+      A.bmax[k] = bounds ? bounds[3 + k] : 0.0f;
+      A.lo[k] = (float)glo[k];
+      A.inv_cell[k] = (float)(1.0 / cell[k]);
+      A.dims[k] = dims[k];
+    }
+    A.cell_start = grid_ok ? (const uint32_t*)d_start : nullptr;
+    A.cell_tets = grid_ok ? (const uint32_t*)d_cells : nullptr;
+    A.out_tet = (uint32_t*)d_tet;
+    A.out_w = (float*)d_w;
+    cgs_launch_bind_tets(A, c->stream);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out_tet, d_tet, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out_weights, d_w, n * 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(CGS_ERR_EXECUTION, std::string("bind: ") + hipGetErrorString(e));
+  } while (false);
+  (void)hipStreamSynchronize(c->stream);
+  cleanup();
+  return rc;
 }
 
 }  // extern "C"

@@ -236,6 +236,25 @@ class GpuPipelineState:
         check(self._lib.cgs_read_tet_transforms(self._h, _ptr(J), _ptr(R), _ptr(s)))
         return J, R, s
 
+    def bind_tets(self, rest_vertices: np.ndarray, elements: np.ndarray, inv_dm: np.ndarray,
+                  positions: np.ndarray, bounds=None):
+        """TetMesh::bind_gaussians(_with_bounds) (mod.rs:263-323) on the GPU.
+        Returns (tet[n] u32, 0xFFFFFFFF = unbound; weights[n,4] f32)."""
+        v = np.ascontiguousarray(rest_vertices, dtype=np.float32)
+        e = np.ascontiguousarray(elements, dtype=np.uint32)
+        m = np.ascontiguousarray(inv_dm, dtype=np.float32)
+        p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        b = None
+        if bounds is not None:
+            b = np.ascontiguousarray(np.concatenate([np.asarray(bounds[0], np.float32),
+                                                     np.asarray(bounds[1], np.float32)]))
+        tet = np.zeros(p.shape[0], np.uint32)
+        w = np.zeros((p.shape[0], 4), np.float32)
+        check(self._lib.cgs_bind_tets(self._h, _ptr(v), v.shape[0], _ptr(e), _ptr(m),
+                                      e.shape[0], _ptr(p), p.shape[0], _ptr(b), _ptr(tet),
+                                      _ptr(w)))
+        return tet, w
+
     def load_projected(self, attrs10: np.ndarray, extents: np.ndarray):
         a = np.ascontiguousarray(attrs10, dtype=np.float32)
         e = np.ascontiguousarray(extents, dtype=np.uint32)

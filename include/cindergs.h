@@ -164,6 +164,20 @@ int cgs_read_deformed(cgs_context* ctx, float* positions, float* covariances, fl
 int cgs_read_tet_transforms(cgs_context* ctx, float* jacobians, float* rotations,
                             float* max_stretch);
 
+/* TetMesh::bind_gaussians / bind_gaussians_with_bounds (src/deformation/
+ * mod.rs:263-323, bind_point 267-310) on the GPU: for each of the n positions
+ * (xyz), the first tetrahedron in index order whose barycentric penalty
+ * sum(max(-w_k, 0)) is <= 1e-4, else the one of smallest penalty; with bounds
+ * (min xyz, max xyz; NULL for none) a position outside them, or whose best
+ * penalty exceeds 0.15, is unbound (tet 0xFFFFFFFF, weights 0).  rest_vertices
+ * [3*num_vertices], elements [4*num_tets], inv_dm [9*num_tets] column-major
+ * (TetMesh::new, 202-227); out_tet [n], out_weights [4*n] (w0..w3).  Host
+ * arrays; returns when the results are written. */
+int cgs_bind_tets(cgs_context* ctx, const float* rest_vertices, uint32_t num_vertices,
+                  const uint32_t* elements, const float* inv_dm, uint32_t num_tets,
+                  const float* positions, uint64_t n, const float* bounds, uint32_t* out_tet,
+                  float* out_weights);
+
 /* Inject a projected splat set (reference layout, as cgs_read_projected) and
  * run Stage 3 / Stage 4 on it (the split of execute_frame_gpu used by
  * test_gpu_stage3_binning_and_ranges_match_cpu, gpu_parity_tests.rs:664-917). */

@@ -1028,3 +1028,118 @@ ORC_API uint32_t orc_deform(const float* verts, const uint32_t* elems, const flo
                            mip, (float*)J, (float*)R, (float*)stretch, (float*)out_pos,
                            (float*)out_cov, (float*)out_opa, split_meta);
 }
+
+// ---------------------------------------------------------------------------
+// Cage construction and splat binding (src/deformation/mod.rs)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// glam 0.28 Vec3::cross / dot (scalar f32, operation order of the crate)
+struct V3 {
+  float x, y, z;
+};
+static inline V3 v3_sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 v3_cross(V3 a, V3 b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+static inline float v3_dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// One point against one tet: bind_point's barycentric weights
+// [w1 w2 w3] = inv_dm (p - X0) (glam Mat3 * Vec3: x_axis*v.x + y_axis*v.y +
+// z_axis*v.z, left to right), w0 = 1 - (w1 + w2 + w3), and the penalty
+// sum of max(-w_k, 0) (mod.rs:280-293).
+static inline float tet_weights(const float* X0, const float* m, const float p[3], float w[4]) {
+  const float dx = p[0] - X0[0], dy = p[1] - X0[1], dz = p[2] - X0[2];
+  w[1] = (m[0] * dx + m[3] * dy) + m[6] * dz;
+  w[2] = (m[1] * dx + m[4] * dy) + m[7] * dz;
+  w[3] = (m[2] * dx + m[5] * dy) + m[8] * dz;
+  w[0] = 1.0f - ((w[1] + w[2]) + w[3]);
+  return ((cgs_fmax(-w[0], 0.0f) + cgs_fmax(-w[1], 0.0f)) + cgs_fmax(-w[2], 0.0f)) +
+         cgs_fmax(-w[3], 0.0f);
+}
+
+}  // namespace
+
+// TetMesh::new (mod.rs:202-227): D_m = [X1-X0, X2-X0, X3-X0] (columns),
+// inverse by glam's Mat3::inverse (cofactors from column cross products,
+// det = z . (x cross y), scaled by 1/det, transposed) when |det| > 1e-8, else
+// identity; written column-major (to_cols_array).
+ORC_API void orc_tet_inv_dm(const float* verts, const uint32_t* elems, uint32_t num_tets,
+                            float* inv_dm) {
+  for (uint32_t t = 0; t < num_tets; ++t) {
+    const float* a = verts + (size_t)elems[t * 4 + 0] * 3;
+    const float* b = verts + (size_t)elems[t * 4 + 1] * 3;
+    const float* c = verts + (size_t)elems[t * 4 + 2] * 3;
+    const float* d = verts + (size_t)elems[t * 4 + 3] * 3;
+    const V3 x0 = {a[0], a[1], a[2]};
+    const V3 cx = v3_sub({b[0], b[1], b[2]}, x0);
+    const V3 cy = v3_sub({c[0], c[1], c[2]}, x0);
+    const V3 cz = v3_sub({d[0], d[1], d[2]}, x0);
+    float* o = inv_dm + (size_t)t * 9;
+    const float det = v3_dot(cz, v3_cross(cx, cy));
+    if (!(fabsf(det) > 1e-8f)) {
+      for (int k = 0; k < 9; ++k) o[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+      continue;
+    }
+    const V3 t0 = v3_cross(cy, cz), t1 = v3_cross(cz, cx), t2 = v3_cross(cx, cy);
+    const float det2 = v3_dot(cz, t2);
+    const float inv = 1.0f / det2;
+    // from_cols(t0*inv, t1*inv, t2*inv).transpose(): column j of the result
+    // holds component j of t0, t1, t2
+    const V3 r0 = {t0.x * inv, t0.y * inv, t0.z * inv};
+    const V3 r1 = {t1.x * inv, t1.y * inv, t1.z * inv};
+    const V3 r2 = {t2.x * inv, t2.y * inv, t2.z * inv};
+    o[0] = r0.x; o[1] = r1.x; o[2] = r2.x;
+    o[3] = r0.y; o[4] = r1.y; o[5] = r2.y;
+    o[6] = r0.z; o[7] = r1.z; o[8] = r2.z;
+  }
+}
+
+// TetMesh::bind_point (mod.rs:267-310) for every point, brute force over the
+// tets in index order: the first tet with penalty <= 1e-4 wins, else the tet
+// of smallest penalty (first on ties); with bounds (min xyz, max xyz) a point
+// outside them, or whose best penalty exceeds 0.15, is unbound (tet = ~0,
+// weights 0) (bind_gaussians_with_bounds, 315-323).
+ORC_API void orc_bind_tets(const float* verts, const uint32_t* elems, const float* inv_dm,
+                           uint32_t num_tets, const float* pos, uint64_t n, const float* bounds,
+                           uint32_t* out_tet, float* out_w) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    const float p[3] = {pos[ii * 3 + 0], pos[ii * 3 + 1], pos[ii * 3 + 2]};
+    float* ow = out_w + ii * 4;
+    if (bounds && (p[0] < bounds[0] || p[0] > bounds[3] || p[1] < bounds[1] ||
+                   p[1] > bounds[4] || p[2] < bounds[2] || p[2] > bounds[5])) {
+      out_tet[ii] = 0xFFFFFFFFu;
+      ow[0] = ow[1] = ow[2] = ow[3] = 0.0f;
+      continue;
+    }
+    uint32_t best_t = 0;
+    float best_pen = INFINITY;
+    float best_w[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    bool found = false;
+    for (uint32_t t = 0; t < num_tets; ++t) {
+      float w[4];
+      const float pen = tet_weights(verts + (size_t)elems[t * 4] * 3, inv_dm + (size_t)t * 9, p, w);
+      if (pen <= 1e-4f) {
+        out_tet[ii] = t;
+        for (int k = 0; k < 4; ++k) ow[k] = w[k];
+        found = true;
+        break;
+      }
+      if (pen < best_pen) {
+        best_pen = pen;
+        best_t = t;
+        for (int k = 0; k < 4; ++k) best_w[k] = w[k];
+      }
+    }
+    if (found) continue;
+    if (bounds && best_pen > 0.15f) {
+      out_tet[ii] = 0xFFFFFFFFu;
+      ow[0] = ow[1] = ow[2] = ow[3] = 0.0f;
+    } else {
+      out_tet[ii] = best_t;
+      for (int k = 0; k < 4; ++k) ow[k] = best_w[k];
+    }
+  }
+}

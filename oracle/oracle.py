@@ -288,3 +288,37 @@ def render_frame_deformed(scene, cage, cam, comp, tau=0.0, mip=0.0, min_opacity=
     return {"deform": d, "attrs": attrs, "extents": ext, "survivors": surv, "counts": counts,
             "offsets": offsets, "total": total, "sorted_keys": sk, "sorted_ids": sid,
             "ranges": ranges, "rgba": rgba}
+
+
+def tet_inv_dm(verts, elems):
+    """TetMesh::new's D_m^-1 per tet (mod.rs:202-227), column-major [T, 9] f32."""
+    L = lib()
+    if not hasattr(L, "_inv_sig"):
+        L.orc_tet_inv_dm.argtypes = [_vp, _vp, _u32, _vp]
+        L.orc_tet_inv_dm.restype = None
+        L._inv_sig = True
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    e = np.ascontiguousarray(elems, dtype=np.uint32)
+    out = np.zeros((e.shape[0], 9), np.float32)
+    L.orc_tet_inv_dm(_p(v), _p(e), e.shape[0], _p(out))
+    return out
+
+
+def bind_tets(verts, elems, inv_dm, positions, bounds=None):
+    """TetMesh::bind_gaussians(_with_bounds) (mod.rs:263-323), brute force in
+    tet index order.  Returns (tet[n] u32 (0xFFFFFFFF unbound), weights[n,4])."""
+    L = lib()
+    if not hasattr(L, "_bind_sig"):
+        L.orc_bind_tets.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp]
+        L.orc_bind_tets.restype = None
+        L._bind_sig = True
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    e = np.ascontiguousarray(elems, dtype=np.uint32)
+    m = np.ascontiguousarray(inv_dm, dtype=np.float32)
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    b = None if bounds is None else np.ascontiguousarray(
+        np.concatenate([np.asarray(bounds[0], np.float32), np.asarray(bounds[1], np.float32)]))
+    tet = np.zeros(p.shape[0], np.uint32)
+    w = np.zeros((p.shape[0], 4), np.float32)
+    L.orc_bind_tets(_p(v), _p(e), _p(m), e.shape[0], _p(p), p.shape[0], _p(b), _p(tet), _p(w))
+    return tet, w

@@ -517,3 +517,48 @@ def test_polar_decomposition_identities(oracle_lib, fp64):
     assert abs(np.linalg.det(R) - 1.0) < 1e-5
     R, _ = _polar_of(oracle_lib, np.zeros((3, 3)), fp64)
     np.testing.assert_array_equal(R, np.eye(3))
+
+
+# ---------------------------------------------------------------------------
+# Binding (mod.rs:202-323): the oracle's brute-force index-order scan against
+# the reference's tests and an independent host restatement (grid search)
+# ---------------------------------------------------------------------------
+
+def test_binding_reference_cases(oracle_lib):
+    mesh = _box_cage([-1, -1, -1], [1, 1, 1])
+    np.testing.assert_array_equal(oracle_lib.tet_inv_dm(mesh.rest_vertices, mesh.elements)
+                                  .view(np.uint32), mesh.inv_dm.view(np.uint32))
+    t, w = oracle_lib.bind_tets(mesh.rest_vertices, mesh.elements, mesh.inv_dm,
+                                np.array([[0, 0, 0], [0.5, -0.5, 0.5]], np.float32))
+    assert np.all(np.abs(w.sum(1) - 1.0) < 1e-5) and np.all(w >= -1e-4)
+    mesh = _box_cage([0, 0, 0], [2, 2, 2])
+    pt = np.array([[0.7, 1.3, 0.4]], np.float32)
+    t, w = oracle_lib.bind_tets(mesh.rest_vertices, mesh.elements, mesh.inv_dm, pt)
+    rec = (w[0][:, None] * mesh.rest_vertices[mesh.elements[t[0]]]).sum(0)
+    assert np.all(np.abs(rec - pt[0]) < 1e-4)
+    # outside the bounds: unbound; outside the cage without bounds: best penalty
+    far = np.array([[5.0, 5.0, 5.0]], np.float32)
+    t, w = oracle_lib.bind_tets(mesh.rest_vertices, mesh.elements, mesh.inv_dm, far,
+                                mesh.bounds())
+    assert t[0] == 0xFFFFFFFF and np.all(w == 0)
+    t, w = oracle_lib.bind_tets(mesh.rest_vertices, mesh.elements, mesh.inv_dm, far)
+    assert t[0] != 0xFFFFFFFF
+
+
+def test_binding_oracle_equals_grid_search_restatement(oracle_lib):
+    """The oracle scans every tet; the host TetMesh.bind_gaussians searches
+    only the 27 neighbouring cells of a grid cage.  Same tets and weights."""
+    from cindergs_amd.cages import TetMesh
+
+    mesh = TetMesh.jittered_grid([-1, -1, -1], [1, 1, 1], cells=(6, 6, 6))
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-0.98, 0.98, (8000, 3)).astype(np.float32)
+    t, w = oracle_lib.bind_tets(mesh.rest_vertices, mesh.elements, mesh.inv_dm, p)
+    ht, hw = mesh.bind_gaussians(p)
+    np.testing.assert_array_equal(t, ht)
+    np.testing.assert_array_equal(w.view(np.uint32), hw.view(np.uint32))
+    # the first-fit rule: no tet of smaller index accepts the point
+    for i in range(0, 8000, 397):
+        for tt in range(int(t[i])):
+            ww = mesh.barycentric(p[i:i + 1], np.array([tt]))[0]
+            assert np.maximum(-ww, 0).sum() > 1e-4
