@@ -123,6 +123,9 @@ SIGNATURES = {
     "cgs_read_deformed": [_vp, _vp, _vp, _vp, _vp, C.POINTER(_u32)],
     "cgs_read_tet_transforms": [_vp, _vp, _vp, _vp],
     "cgs_bind_tets": [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
+    "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
+    "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
+    "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
     "cgs_load_projected": [_vp, _vp, _vp, _u64],
     "cgs_run_stage3": [_vp, C.POINTER(CameraUniforms), C.POINTER(CompositorUniforms),
                        C.POINTER(_u64)],

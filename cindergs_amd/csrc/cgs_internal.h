@@ -324,6 +324,27 @@ struct CgsBindArgs {
 };
 void cgs_launch_bind_tets(const CgsBindArgs& a, hipStream_t s);
 
+struct CgsMvcArgs {
+  const float* verts;       // [nv][3] rest cage vertices
+  uint32_t nv;
+  const uint32_t* tris;     // [nt][3] outward triangles
+  uint32_t nt;
+  const uint32_t* vt_start; // [nv + 1] incidence lists: entries tri*3 + corner,
+  const uint32_t* vt_list;  //   ascending triangle order per vertex
+  const float* pts;         // [n][3]
+  uint32_t n;
+  float h;                  // central-difference step (jacobian_step_size)
+  float* out_w;             // [n][nv]
+  float* out_g;             // [n][nv][3] or null
+  uint32_t* mask;           // [n] inside flags or null
+};
+size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt);
+void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s);
+void cgs_launch_mvc_winding(const CgsMvcArgs& a, float* out, hipStream_t s);
+void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
+                           const float* rest_pos, const float* dverts, uint32_t nv, uint32_t n,
+                           float* out_pos, float* out_F, hipStream_t s);
+
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved
   const uint32_t* elems;  // [4*T]

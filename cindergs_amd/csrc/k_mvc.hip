@@ -1,0 +1,307 @@
+// k_mvc.hip — exact 3D mean value coordinates on a closed triangle cage.
+//
+// Follows PolyCageMesh (src/deformation/mvc.rs): winding_number / contains_point
+// (326-363), compute_mvc_weights (366-471, Ju-Schaefer-Warren with the vertex,
+// on-face and degenerate-triangle branches), bind_gaussians (498-541: weights
+// plus central-difference gradients with the step jacobian_step_size,
+// 485-495), evaluate_position (473-481) and MvcBindings::local_jacobian (66-78).
+//
+// MI355X design: one wave64 per point.  The point's unit directions and
+// distances to every cage vertex live in LDS (float4 per vertex); lanes take
+// triangles t = lane + 64k and write each triangle's three mu terms to LDS;
+// each vertex then sums its incident triangles' terms in triangle order (a
+// CSR incidence list built on the host), which is the order in which the
+// reference adds them, and the normaliser is a fixed-order wave reduction, so
+// the weights are deterministic.  The first vertex closer than 1e-6 and the
+// first triangle on which the point lies (|pi - h| < 1e-5) are found with wave
+// minima, matching the reference's early returns.  The seven evaluations of a
+// bind (the point and p +- h e_axis) run back to back in the same wave.
+// Transcendentals are the device library's asinf / sinf / atan2f, so the
+// weights agree with the reference to rounding, not bit for bit.
+#include "cgs_internal.h"
+
+namespace {
+
+constexpr float MVC_PI = 3.14159265358979323846f;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+struct V3f {
+  float x, y, z;
+};
+__device__ __forceinline__ V3f vsub(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float vdot(V3f a, V3f b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3f vcross(V3f a, V3f b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ float vlen(V3f a) { return sqrtf(vdot(a, a)); }
+__device__ __forceinline__ V3f vert(const float* V, uint32_t i) {
+  return {V[i * 3], V[i * 3 + 1], V[i * 3 + 2]};
+}
+
+// winding_number (mvc.rs:326-347); triangles split over the lanes
+__device__ float winding(const CgsMvcArgs& A, V3f p) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float part = 0.0f;
+  uint32_t on_vertex = 0u;
+  for (uint32_t t = lane; t < A.nt; t += 64u) {
+    const V3f a = vsub(vert(A.verts, A.tris[t * 3]), p);
+    const V3f b = vsub(vert(A.verts, A.tris[t * 3 + 1]), p);
+    const V3f c = vsub(vert(A.verts, A.tris[t * 3 + 2]), p);
+    const float la = vlen(a), lb = vlen(b), lc = vlen(c);
+    if (la < 1e-7f || lb < 1e-7f || lc < 1e-7f) {
+      on_vertex = 1u;
+      continue;
+    }
+    const float det = vdot(a, vcross(b, c));
+    const float den = ((la * lb * lc + vdot(a, b) * lc) + vdot(b, c) * la) + vdot(c, a) * lb;
+    part += 2.0f * atan2f(det, den);
+  }
+  if (__ballot(on_vertex != 0u) != 0ull) return 1.0f;
+  return wave_sum(part) / (4.0f * MVC_PI);
+}
+
+// The weights of one point, left in s_w[v] (v < nv).  s_dir holds the
+// directions and distances, s_mu the per-triangle terms (3 per triangle).
+__device__ void mvc_eval(const CgsMvcArgs& A, V3f x, float4* s_dir, float* s_mu, float* s_w) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nv = A.nv, nt = A.nt;
+  // vertices: directions, distances, first coincident vertex
+  uint32_t hit_v = 0xFFFFFFFFu;
+  for (uint32_t v = lane; v < nv; v += 64u) {
+    const V3f diff = vsub(vert(A.verts, v), x);
+    const float d = vlen(diff);
+    if (d < 1e-6f) hit_v = min(hit_v, v);
+    s_dir[v] = make_float4(diff.x / d, diff.y / d, diff.z / d, d);
+  }
+  hit_v = wave_min_u32(hit_v);
+  if (hit_v != 0xFFFFFFFFu) {
+    for (uint32_t v = lane; v < nv; v += 64u) s_w[v] = v == hit_v ? 1.0f : 0.0f;
+    return;
+  }
+  __builtin_amdgcn_wave_barrier();
+  // triangles: mu terms, first triangle the point lies on
+  float part = 0.0f;
+  uint32_t hit_t = 0xFFFFFFFFu;
+  for (uint32_t t = lane; t < nt; t += 64u) {
+    const uint32_t i0 = A.tris[t * 3], i1 = A.tris[t * 3 + 1], i2 = A.tris[t * 3 + 2];
+    const float4 d0 = s_dir[i0], d1 = s_dir[i1], d2 = s_dir[i2];
+    const V3f u0 = {d0.x, d0.y, d0.z}, u1 = {d1.x, d1.y, d1.z}, u2 = {d2.x, d2.y, d2.z};
+    const float l0 = vlen(vsub(u1, u2)), l1 = vlen(vsub(u2, u0)), l2 = vlen(vsub(u0, u1));
+    const float th0 = 2.0f * asinf(cgs_fclamp(l0 * 0.5f, 0.0f, 1.0f));
+    const float th1 = 2.0f * asinf(cgs_fclamp(l1 * 0.5f, 0.0f, 1.0f));
+    const float th2 = 2.0f * asinf(cgs_fclamp(l2 * 0.5f, 0.0f, 1.0f));
+    const float h = 0.5f * ((th0 + th1) + th2);
+    float mu0 = 0.0f, mu1 = 0.0f, mu2 = 0.0f;
+    if (fabsf(MVC_PI - h) < 1e-5f) {
+      hit_t = min(hit_t, t);
+    } else {
+      const float st0 = sinf(th0), st1 = sinf(th1), st2 = sinf(th2);
+      if (!(fabsf(st0) < 1e-6f || fabsf(st1) < 1e-6f || fabsf(st2) < 1e-6f)) {
+        const float sh = sinf(h);
+        const float c0 = cgs_fclamp((2.0f * sh * sinf(h - th0)) / (st1 * st2) - 1.0f, -1.0f, 1.0f);
+        const float c1 = cgs_fclamp((2.0f * sh * sinf(h - th1)) / (st2 * st0) - 1.0f, -1.0f, 1.0f);
+        const float c2 = cgs_fclamp((2.0f * sh * sinf(h - th2)) / (st0 * st1) - 1.0f, -1.0f, 1.0f);
+        const float det = vdot(u2, vcross(u0, u1));
+        const float sg = det >= 0.0f ? 1.0f : -1.0f;
+        const float s0 = sg * sqrtf(cgs_fmax(1.0f - c0 * c0, 0.0f));
+        const float s1 = sg * sqrtf(cgs_fmax(1.0f - c1 * c1, 0.0f));
+        const float s2 = sg * sqrtf(cgs_fmax(1.0f - c2 * c2, 0.0f));
+        if (!(fabsf(s0) <= 1e-6f || fabsf(s1) <= 1e-6f || fabsf(s2) <= 1e-6f)) {
+          mu0 = (th0 - c1 * th2 - c2 * th1) / (2.0f * d0.w * st1 * s2);
+          mu1 = (th1 - c2 * th0 - c0 * th2) / (2.0f * d1.w * st2 * s0);
+          mu2 = (th2 - c0 * th1 - c1 * th0) / (2.0f * d2.w * st0 * s1);
+        }
+      }
+    }
+    s_mu[t * 3] = mu0;
+    s_mu[t * 3 + 1] = mu1;
+    s_mu[t * 3 + 2] = mu2;
+    part += (mu0 + mu1) + mu2;
+  }
+  hit_t = wave_min_u32(hit_t);
+  __builtin_amdgcn_wave_barrier();
+  if (hit_t != 0xFFFFFFFFu) {
+    // the point is on triangle hit_t: weights from the opposite-angle sines
+    const uint32_t i0 = A.tris[hit_t * 3], i1 = A.tris[hit_t * 3 + 1], i2 = A.tris[hit_t * 3 + 2];
+    const float4 d0 = s_dir[i0], d1 = s_dir[i1], d2 = s_dir[i2];
+    const V3f u0 = {d0.x, d0.y, d0.z}, u1 = {d1.x, d1.y, d1.z}, u2 = {d2.x, d2.y, d2.z};
+    const float th0 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u1, u2)) * 0.5f, 0.0f, 1.0f));
+    const float th1 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u2, u0)) * 0.5f, 0.0f, 1.0f));
+    const float th2 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u0, u1)) * 0.5f, 0.0f, 1.0f));
+    const float w0 = sinf(th0) * d1.w * d2.w;
+    const float w1 = sinf(th1) * d2.w * d0.w;
+    const float w2 = sinf(th2) * d0.w * d1.w;
+    const float sum = (w0 + w1) + w2;
+    const bool ok = fabsf(sum) > 1e-12f;
+    for (uint32_t v = lane; v < nv; v += 64u)
+      s_w[v] = !ok ? 0.0f : v == i0 ? w0 / sum : v == i1 ? w1 / sum : v == i2 ? w2 / sum : 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    return;
+  }
+  const float total = wave_sum(part);
+  if (fabsf(total) > 1e-12f) {
+    const float inv = 1.0f / total;
+    for (uint32_t v = lane; v < nv; v += 64u) {
+      float acc = 0.0f;  // incident triangles in triangle order
+      for (uint32_t j = A.vt_start[v]; j < A.vt_start[v + 1]; ++j) acc += s_mu[A.vt_list[j]];
+      s_w[v] = acc * inv;
+    }
+  } else {
+    // nearest vertex (first on ties)
+    float best = __builtin_inff();
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t v = lane; v < nv; v += 64u)
+      if (s_dir[v].w < best) {
+        best = s_dir[v].w;
+        bi = v;
+      }
+    // wave arg-min: smallest distance, then smallest index
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const float ob = __shfl_xor(best, o, 64);
+      const uint32_t oi = (uint32_t)__shfl_xor((int)bi, o, 64);
+      if (ob < best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    for (uint32_t v = lane; v < nv; v += 64u) s_w[v] = v == bi ? 1.0f : 0.0f;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// bind_gaussians (mvc.rs:498-541): one wave per point.
+__global__ __launch_bounds__(64) void k_mvc_bind(CgsMvcArgs A) {
+  extern __shared__ float4 s_dyn[];
+  float4* s_dir = s_dyn;
+  float* s_mu = reinterpret_cast<float*>(s_dir + A.nv);
+  float* s_w = s_mu + 3 * A.nt;
+  float* s_w2 = s_w + A.nv;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = blockIdx.x;
+  if (i >= A.n) return;
+  const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
+  float* row = A.out_w + (size_t)i * A.nv;
+  float* grow = A.out_g ? A.out_g + (size_t)i * A.nv * 3 : nullptr;
+  const bool inside = fabsf(winding(A, p)) >= 0.5f;
+  if (A.mask && lane == 0) A.mask[i] = inside ? 1u : 0u;
+  if (!inside) {
+    for (uint32_t v = lane; v < A.nv; v += 64u) row[v] = 0.0f;
+    if (grow)
+      for (uint32_t v = lane; v < 3 * A.nv; v += 64u) grow[v] = 0.0f;
+    return;
+  }
+  mvc_eval(A, p, s_dir, s_mu, s_w);
+  for (uint32_t v = lane; v < A.nv; v += 64u) row[v] = s_w[v];
+  if (!grow) return;
+  const float inv_2h = 0.5f / A.h;
+  for (int axis = 0; axis < 3; ++axis) {
+    V3f pp = p, pm = p;
+    if (axis == 0) {
+      pp.x += A.h;
+      pm.x -= A.h;
+    } else if (axis == 1) {
+      pp.y += A.h;
+      pm.y -= A.h;
+    } else {
+      pp.z += A.h;
+      pm.z -= A.h;
+    }
+    mvc_eval(A, pp, s_dir, s_mu, s_w2);
+    mvc_eval(A, pm, s_dir, s_mu, s_w);
+    for (uint32_t v = lane; v < A.nv; v += 64u) grow[(size_t)v * 3 + axis] = (s_w2[v] - s_w[v]) * inv_2h;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// winding numbers only (contains_point = |w| >= 0.5)
+__global__ __launch_bounds__(64) void k_mvc_winding(CgsMvcArgs A, float* out) {
+  const uint32_t i = blockIdx.x;
+  if (i >= A.n) return;
+  const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
+  const float w = winding(A, p);
+  if (threadIdx.x == 0) out[i] = w;
+}
+
+// evaluate_position (mvc.rs:473-481) and MvcBindings::local_jacobian (66-78)
+// for every bound point: x' = sum_j w_j v'_j, F = sum_j v'_j g_j^T, in vertex
+// order; one wave per point, lanes over vertices, fixed-order reduction.
+__global__ __launch_bounds__(64) void k_mvc_deform(const float* w, const float* g,
+                                                   const uint32_t* mask, const float* rest_pos,
+                                                   const float* dverts, uint32_t nv, uint32_t n,
+                                                   float* out_pos, float* out_F) {
+  const uint32_t i = blockIdx.x, lane = threadIdx.x;
+  if (i >= n) return;
+  if (mask && mask[i] == 0u) {
+    if (lane < 3) out_pos[(size_t)i * 3 + lane] = rest_pos[(size_t)i * 3 + lane];
+    if (out_F && lane < 9) out_F[(size_t)i * 9 + lane] = (lane % 4 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  const float* wr = w + (size_t)i * nv;
+  const float* gr = g ? g + (size_t)i * nv * 3 : nullptr;
+  float px = 0, py = 0, pz = 0;
+  float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t v = lane; v < nv; v += 64u) {
+    const float vx = dverts[v * 3], vy = dverts[v * 3 + 1], vz = dverts[v * 3 + 2];
+    const float wv = wr[v];
+    px += vx * wv;
+    py += vy * wv;
+    pz += vz * wv;
+    if (gr) {
+      const float g0 = gr[(size_t)v * 3], g1 = gr[(size_t)v * 3 + 1], g2 = gr[(size_t)v * 3 + 2];
+      // column-major F: col k = sum v' * g[k]
+      F[0] += vx * g0; F[1] += vy * g0; F[2] += vz * g0;
+      F[3] += vx * g1; F[4] += vy * g1; F[5] += vz * g1;
+      F[6] += vx * g2; F[7] += vy * g2; F[8] += vz * g2;
+    }
+  }
+  px = wave_sum(px);
+  py = wave_sum(py);
+  pz = wave_sum(pz);
+  if (lane == 0) {
+    out_pos[(size_t)i * 3] = px;
+    out_pos[(size_t)i * 3 + 1] = py;
+    out_pos[(size_t)i * 3 + 2] = pz;
+  }
+  if (out_F && gr) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const float s = wave_sum(F[k]);
+      if (lane == 0) out_F[(size_t)i * 9 + k] = s;
+    }
+  }
+}
+
+}  // namespace
+
+size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt) {
+  return (size_t)nv * 16 + (size_t)nt * 12 + (size_t)nv * 8;
+}
+
+void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_mvc_bind, dim3(a.n), dim3(64), cgs_mvc_lds_bytes(a.nv, a.nt), s, a);
+}
+
+void cgs_launch_mvc_winding(const CgsMvcArgs& a, float* out, hipStream_t s) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_mvc_winding, dim3(a.n), dim3(64), 0, s, a, out);
+}
+
+void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
+                           const float* rest_pos, const float* dverts, uint32_t nv, uint32_t n,
+                           float* out_pos, float* out_F, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mvc_deform, dim3(n), dim3(64), 0, s, w, g, mask, rest_pos, dverts, nv, n,
+                     out_pos, out_F);
+}

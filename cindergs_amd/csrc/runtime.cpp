@@ -1412,4 +1412,146 @@ int cgs_bind_tets(cgs_context* c, const float* rest_vertices, uint32_t num_verti
   return rc;
 }
 
+// ---- mean value coordinates (src/deformation/mvc.rs) ----------------------
+
+// Uploads a triangle cage and its vertex->triangle incidence lists (entries
+// tri*3 + corner in ascending triangle order) into scratch device memory.
+struct MvcCage {
+  void* d_verts = nullptr;
+  void* d_tris = nullptr;
+  void* d_start = nullptr;
+  void* d_list = nullptr;
+  float h = 0.0f;
+  ~MvcCage() {
+    for (void* p : {d_verts, d_tris, d_start, d_list})
+      if (p) (void)hipFree(p);
+  }
+};
+
+static int mvc_upload(cgs_context* c, const float* verts, uint32_t nv, const uint32_t* tris,
+                      uint32_t nt, MvcCage& M) {
+  if (!verts || !tris) return fail(CGS_ERR_INVALID_ARG, "null cage");
+  if (nv < 4 || nt < 4) return fail(CGS_ERR_INVALID_ARG, "a cage needs >= 4 vertices and triangles");
+  if (cgs_mvc_lds_bytes(nv, nt) > 64 * 1024)
+    return fail(CGS_ERR_INVALID_ARG, "cage too large for one wave's LDS (nv*24 + nt*12 > 64 KiB)");
+  std::vector<uint32_t> start(nv + 1, 0u), list((size_t)nt * 3);
+  for (uint32_t t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t v = tris[(size_t)t * 3 + k];
+      if (v >= nv) return fail(CGS_ERR_INVALID_ARG, "triangle vertex index out of range");
+      start[v + 1]++;
+    }
+  for (uint32_t v = 0; v < nv; ++v) start[v + 1] += start[v];
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+  for (uint32_t t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k) list[fill[tris[(size_t)t * 3 + k]]++] = t * 3 + (uint32_t)k;
+  // jacobian_step_size (mvc.rs:485-495), f32
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = cgs_fmin(lo[k], verts[(size_t)v * 3 + k]);
+      hi[k] = cgs_fmax(hi[k], verts[(size_t)v * 3 + k]);
+    }
+  const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  const float diag = cgs_fmax(sqrtf((dx * dx + dy * dy) + dz * dz), 1e-3f);
+  M.h = cgs_fclamp(diag * 1e-3f, 1e-5f, 1e-2f);
+  HIPCHK(hipMalloc(&M.d_verts, (size_t)nv * 12));
+  HIPCHK(hipMalloc(&M.d_tris, (size_t)nt * 12));
+  HIPCHK(hipMalloc(&M.d_start, (size_t)(nv + 1) * 4));
+  HIPCHK(hipMalloc(&M.d_list, (size_t)nt * 12));
+  HIPCHK(hipMemcpyAsync(M.d_verts, verts, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(M.d_tris, tris, (size_t)nt * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(M.d_start, start.data(), (size_t)(nv + 1) * 4, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(M.d_list, list.data(), (size_t)nt * 12, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+  return CGS_OK;
+}
+
+// PolyCageMesh::bind_gaussians (mvc.rs:498-541).  Cage on the host; points
+// and outputs in device memory: d_points [3n], d_weights [n*nv],
+// d_gradients [n*nv*3] or NULL, d_inside [n] u32 or NULL.
+int cgs_mvc_bind_exact(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                       const uint32_t* triangles, uint32_t num_triangles, const float* d_points,
+                       uint64_t n, float* d_weights, float* d_gradients, uint32_t* d_inside) {
+  TRY(ensure_ctx(c));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!d_points || !d_weights) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  MvcCage M;
+  TRY(mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M));
+  CgsMvcArgs A;
+  A.verts = (const float*)M.d_verts;
+  A.nv = num_vertices;
+  A.tris = (const uint32_t*)M.d_tris;
+  A.nt = num_triangles;
+  A.vt_start = (const uint32_t*)M.d_start;
+  A.vt_list = (const uint32_t*)M.d_list;
+  A.pts = d_points;
+  A.n = (uint32_t)n;
+  A.h = M.h;
+  A.out_w = d_weights;
+  A.out_g = d_gradients;
+  A.mask = d_inside;
+  cgs_launch_mvc_bind(A, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+// PolyCageMesh::winding_number for n device points into d_out [n] f32.
+int cgs_mvc_winding(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                    const uint32_t* triangles, uint32_t num_triangles, const float* d_points,
+                    uint64_t n, float* d_out) {
+  TRY(ensure_ctx(c));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!d_points || !d_out) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  MvcCage M;
+  TRY(mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M));
+  CgsMvcArgs A;
+  memset(&A, 0, sizeof(A));
+  A.verts = (const float*)M.d_verts;
+  A.nv = num_vertices;
+  A.tris = (const uint32_t*)M.d_tris;
+  A.nt = num_triangles;
+  A.pts = d_points;
+  A.n = (uint32_t)n;
+  cgs_launch_mvc_winding(A, d_out, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
+// evaluate_position + MvcBindings::local_jacobian (mvc.rs:66-78, 473-481) for
+// n bound points: d_out_positions [3n] (rest position where not inside),
+// d_out_jacobians [9n] column-major or NULL (needs d_gradients).  The deformed
+// cage vertices come from the host (num_vertices x 3).
+int cgs_mvc_deform(cgs_context* c, const float* d_weights, const float* d_gradients,
+                   const uint32_t* d_inside, const float* d_rest_positions,
+                   const float* deformed_vertices, uint32_t num_vertices, uint64_t n,
+                   float* d_out_positions, float* d_out_jacobians) {
+  TRY(ensure_ctx(c));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!d_weights || !d_rest_positions || !deformed_vertices || !d_out_positions)
+    return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  if (d_out_jacobians && !d_gradients)
+    return fail(CGS_ERR_INVALID_ARG, "jacobians need the weight gradients");
+  void* dv = nullptr;
+  HIPCHK(hipMalloc(&dv, (size_t)num_vertices * 12));
+  hipError_t e = hipMemcpyAsync(dv, deformed_vertices, (size_t)num_vertices * 12,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    cgs_launch_mvc_deform(d_weights, d_gradients, d_inside, d_rest_positions, (const float*)dv,
+                          num_vertices, (uint32_t)n, d_out_positions, d_out_jacobians, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dv);
+  if (e != hipSuccess) return fail(CGS_ERR_EXECUTION, std::string("mvc deform: ") + hipGetErrorString(e));
+  return CGS_OK;
+}
+
 }  // extern "C"

@@ -178,6 +178,31 @@ int cgs_bind_tets(cgs_context* ctx, const float* rest_vertices, uint32_t num_ver
                   const float* positions, uint64_t n, const float* bounds, uint32_t* out_tet,
                   float* out_weights);
 
+/* PolyCageMesh::bind_gaussians (src/deformation/mvc.rs:498-541): exact 3D mean
+ * value coordinates (compute_mvc_weights, 366-471) of n points against a
+ * closed outward-wound triangle cage, their central-difference gradients
+ * (step clamp(1e-3 * bbox diagonal, 1e-5, 1e-2), 485-495) and the inside test
+ * |winding number| >= 0.5 (326-363); rows of points outside are zero.  Cage
+ * arrays on the host (vertices [3*nv], triangles [3*nt], nv*24 + nt*12 <= 64 KiB);
+ * d_points [3n], d_weights [n*nv], d_gradients [n*nv*3] (or NULL), d_inside
+ * [n] u32 (or NULL) in device memory (cgs_malloc). */
+int cgs_mvc_bind_exact(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
+                       const uint32_t* triangles, uint32_t num_triangles, const float* d_points,
+                       uint64_t n, float* d_weights, float* d_gradients, uint32_t* d_inside);
+/* PolyCageMesh::winding_number (mvc.rs:326-347) of n device points into d_out [n]. */
+int cgs_mvc_winding(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
+                    const uint32_t* triangles, uint32_t num_triangles, const float* d_points,
+                    uint64_t n, float* d_out);
+/* evaluate_position (mvc.rs:473-481) and MvcBindings::local_jacobian (66-78):
+ * x' = sum_j w_j v'_j and F = sum_j v'_j grad(w_j)^T (column-major) for n
+ * points of a binding table; a point whose d_inside flag is 0 keeps its rest
+ * position (and F = I).  deformed_vertices on the host; the rest in device
+ * memory; d_gradients / d_out_jacobians may be NULL together. */
+int cgs_mvc_deform(cgs_context* ctx, const float* d_weights, const float* d_gradients,
+                   const uint32_t* d_inside, const float* d_rest_positions,
+                   const float* deformed_vertices, uint32_t num_vertices, uint64_t n,
+                   float* d_out_positions, float* d_out_jacobians);
+
 /* Inject a projected splat set (reference layout, as cgs_read_projected) and
  * run Stage 3 / Stage 4 on it (the split of execute_frame_gpu used by
  * test_gpu_stage3_binning_and_ranges_match_cpu, gpu_parity_tests.rs:664-917). */

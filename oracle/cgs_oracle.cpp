@@ -1143,3 +1143,230 @@ ORC_API void orc_bind_tets(const float* verts, const uint32_t* elems, const floa
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Mean value coordinates on a closed triangle cage (src/deformation/mvc.rs)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+template <class T> struct Tr;
+template <> struct Tr<float> {
+  static float asin_(float x) { return asinf(x); }
+  static float sin_(float x) { return sinf(x); }
+  static float atan2_(float y, float x) { return atan2f(y, x); }
+  static float sqrt_(float x) { return sqrtf(x); }
+};
+template <> struct Tr<double> {
+  static double asin_(double x) { return asin(x); }
+  static double sin_(double x) { return sin(x); }
+  static double atan2_(double y, double x) { return atan2(y, x); }
+  static double sqrt_(double x) { return sqrt(x); }
+};
+
+template <class T> struct W3 {
+  T x, y, z;
+};
+template <class T> static inline W3<T> w_sub(W3<T> a, W3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> static inline T w_dot(W3<T> a, W3<T> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class T> static inline W3<T> w_cross(W3<T> a, W3<T> b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class T> static inline T w_len(W3<T> a) { return Tr<T>::sqrt_(w_dot(a, a)); }
+template <class T> static inline W3<T> w_at(const float* v, uint32_t i) {
+  return {(T)v[i * 3], (T)v[i * 3 + 1], (T)v[i * 3 + 2]};
+}
+template <class T> static inline T w_max(T a, T b) { return a > b ? a : (b != b ? a : b); }
+template <class T> static inline T w_clamp(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// PolyCageMesh::winding_number (mvc.rs:326-347)
+template <class T>
+static T winding_number(const float* V, const uint32_t* F, uint32_t nt, W3<T> p) {
+  T sum = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const W3<T> a = w_sub(w_at<T>(V, F[t * 3]), p), b = w_sub(w_at<T>(V, F[t * 3 + 1]), p),
+                c = w_sub(w_at<T>(V, F[t * 3 + 2]), p);
+    const T la = w_len(a), lb = w_len(b), lc = w_len(c);
+    if (la < (T)1e-7f || lb < (T)1e-7f || lc < (T)1e-7f) return 1;
+    const T det = w_dot(a, w_cross(b, c));
+    const T denom = ((la * lb * lc + w_dot(a, b) * lc) + w_dot(b, c) * la) + w_dot(c, a) * lb;
+    sum += (T)2 * Tr<T>::atan2_(det, denom);
+  }
+  return sum / ((T)4 * (T)3.14159265358979323846f);
+}
+
+// PolyCageMesh::compute_mvc_weights (mvc.rs:366-471), operation order of the
+// reference (left to right, glam's dot / cross / length / determinant)
+template <class T>
+static void mvc_weights(const float* V, uint32_t nv, const uint32_t* F, uint32_t nt,
+                        const T x[3], T* w, T* dist, W3<T>* dir) {
+  typedef Tr<T> m;
+  const T PI_ = (T)3.14159265358979323846f;
+  const W3<T> X = {x[0], x[1], x[2]};
+  for (uint32_t i = 0; i < nv; ++i) {
+    const W3<T> diff = w_sub(w_at<T>(V, i), X);
+    const T d = w_len(diff);
+    if (d < (T)1e-6f) {
+      for (uint32_t k = 0; k < nv; ++k) w[k] = 0;
+      w[i] = 1;
+      return;
+    }
+    dist[i] = d;
+    dir[i] = {diff.x / d, diff.y / d, diff.z / d};
+  }
+  for (uint32_t k = 0; k < nv; ++k) w[k] = 0;
+  T total = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t i0 = F[t * 3], i1 = F[t * 3 + 1], i2 = F[t * 3 + 2];
+    const W3<T> u0 = dir[i0], u1 = dir[i1], u2 = dir[i2];
+    const T l0 = w_len(w_sub(u1, u2)), l1 = w_len(w_sub(u2, u0)), l2 = w_len(w_sub(u0, u1));
+    const T th0 = (T)2 * m::asin_(w_clamp(l0 * (T)0.5, (T)0, (T)1));
+    const T th1 = (T)2 * m::asin_(w_clamp(l1 * (T)0.5, (T)0, (T)1));
+    const T th2 = (T)2 * m::asin_(w_clamp(l2 * (T)0.5, (T)0, (T)1));
+    const T h = (T)0.5 * ((th0 + th1) + th2);
+    if (fabs((double)(PI_ - h)) < (double)(T)1e-5f) {
+      const T w0 = m::sin_(th0) * dist[i1] * dist[i2];
+      const T w1 = m::sin_(th1) * dist[i2] * dist[i0];
+      const T w2 = m::sin_(th2) * dist[i0] * dist[i1];
+      const T sum = (w0 + w1) + w2;
+      for (uint32_t k = 0; k < nv; ++k) w[k] = 0;
+      if (fabs((double)sum) > (double)(T)1e-12f) {
+        w[i0] = w0 / sum;
+        w[i1] = w1 / sum;
+        w[i2] = w2 / sum;
+      }
+      return;
+    }
+    const T s_t0 = m::sin_(th0), s_t1 = m::sin_(th1), s_t2 = m::sin_(th2);
+    if (fabs((double)s_t0) < (double)(T)1e-6f || fabs((double)s_t1) < (double)(T)1e-6f ||
+        fabs((double)s_t2) < (double)(T)1e-6f)
+      continue;
+    const T sin_h = m::sin_(h);
+    const T c0 = w_clamp(((T)2 * sin_h * m::sin_(h - th0)) / (s_t1 * s_t2) - (T)1, (T)-1, (T)1);
+    const T c1 = w_clamp(((T)2 * sin_h * m::sin_(h - th1)) / (s_t2 * s_t0) - (T)1, (T)-1, (T)1);
+    const T c2 = w_clamp(((T)2 * sin_h * m::sin_(h - th2)) / (s_t0 * s_t1) - (T)1, (T)-1, (T)1);
+    const T det = w_dot(u2, w_cross(u0, u1));
+    const T sgn = det >= (T)0 ? (T)1 : (T)-1;
+    const T s0 = sgn * m::sqrt_(w_max((T)1 - c0 * c0, (T)0));
+    const T s1 = sgn * m::sqrt_(w_max((T)1 - c1 * c1, (T)0));
+    const T s2 = sgn * m::sqrt_(w_max((T)1 - c2 * c2, (T)0));
+    if (fabs((double)s0) <= (double)(T)1e-6f || fabs((double)s1) <= (double)(T)1e-6f ||
+        fabs((double)s2) <= (double)(T)1e-6f)
+      continue;
+    const T mu0 = (th0 - c1 * th2 - c2 * th1) / ((T)2 * dist[i0] * s_t1 * s2);
+    const T mu1 = (th1 - c2 * th0 - c0 * th2) / ((T)2 * dist[i1] * s_t2 * s0);
+    const T mu2 = (th2 - c0 * th1 - c1 * th0) / ((T)2 * dist[i2] * s_t0 * s1);
+    w[i0] += mu0;
+    w[i1] += mu1;
+    w[i2] += mu2;
+    total += (mu0 + mu1) + mu2;
+  }
+  if (fabs((double)total) > (double)(T)1e-12f) {
+    const T inv = (T)1 / total;
+    for (uint32_t k = 0; k < nv; ++k) w[k] *= inv;
+  } else {
+    uint32_t best = 0;
+    for (uint32_t k = 1; k < nv; ++k)
+      if (dist[k] < dist[best]) best = k;
+    w[best] = 1;
+  }
+}
+
+// jacobian_step_size (mvc.rs:485-495)
+template <class T> static T mvc_step(const float* V, uint32_t nv) {
+  T lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < nv; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], (T)V[i * 3 + k]);
+      hi[k] = std::max(hi[k], (T)V[i * 3 + k]);
+    }
+  const W3<T> d = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  const T diag = w_max(w_len(d), (T)1e-3f);
+  return w_clamp(diag * (T)1e-3f, (T)1e-5f, (T)1e-2f);
+}
+
+template <class T>
+static void mvc_bind_all(const float* V, uint32_t nv, const uint32_t* F, uint32_t nt,
+                         const float* pts, uint64_t n, T* out_w, T* out_g, uint8_t* mask) {
+  const T h = mvc_step<T>(V, nv);
+  const T inv_2h = (T)0.5 / h;
+#pragma omp parallel num_threads(g_threads)
+  {
+    std::vector<T> dist(nv), wp(nv), wm(nv);
+    std::vector<W3<T>> dir(nv);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+      const T p[3] = {(T)pts[ii * 3], (T)pts[ii * 3 + 1], (T)pts[ii * 3 + 2]};
+      T* w = out_w + (size_t)ii * nv;
+      T* g = out_g ? out_g + (size_t)ii * nv * 3 : nullptr;
+      const T wn = winding_number<T>(V, F, nt, {p[0], p[1], p[2]});
+      const bool inside = fabs((double)wn) >= 0.5;
+      if (mask) mask[ii] = inside ? 1 : 0;
+      if (!inside) {
+        for (uint32_t k = 0; k < nv; ++k) w[k] = 0;
+        if (g)
+          for (uint32_t k = 0; k < nv * 3; ++k) g[k] = 0;
+        continue;
+      }
+      mvc_weights<T>(V, nv, F, nt, p, w, dist.data(), dir.data());
+      if (!g) continue;
+      for (int axis = 0; axis < 3; ++axis) {
+        T pp[3] = {p[0], p[1], p[2]}, pm[3] = {p[0], p[1], p[2]};
+        pp[axis] += h;
+        pm[axis] -= h;
+        mvc_weights<T>(V, nv, F, nt, pp, wp.data(), dist.data(), dir.data());
+        mvc_weights<T>(V, nv, F, nt, pm, wm.data(), dist.data(), dir.data());
+        for (uint32_t j = 0; j < nv; ++j) g[(size_t)j * 3 + axis] = (wp[j] - wm[j]) * inv_2h;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// PolyCageMesh::bind_gaussians (mvc.rs:498-541): per point the inside mask
+// (|winding number| >= 0.5), the MVC weights [n*nv] and their central
+// differences [n*nv*3] (step = clamp(1e-3 diag, 1e-5, 1e-2)); zeros outside.
+// out_grad may be null (weights only).  fp64 selects the FP64 twin (double
+// outputs).
+ORC_API void orc_mvc_bind(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                          const float* pts, uint64_t n, int fp64, void* out_w, void* out_grad,
+                          uint8_t* mask) {
+  if (fp64)
+    mvc_bind_all<double>(verts, nv, tris, nt, pts, n, (double*)out_w, (double*)out_grad, mask);
+  else
+    mvc_bind_all<float>(verts, nv, tris, nt, pts, n, (float*)out_w, (float*)out_grad, mask);
+}
+
+// winding number only (contains_point = |w| >= 0.5)
+ORC_API void orc_mvc_winding(const float* verts, const uint32_t* tris, uint32_t nt,
+                             const float* pts, uint64_t n, int fp64, double* out) {
+#pragma omp parallel for schedule(static, 64) num_threads(g_threads)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    if (fp64)
+      out[ii] = winding_number<double>(verts, tris, nt,
+                                       {(double)pts[ii * 3], (double)pts[ii * 3 + 1],
+                                        (double)pts[ii * 3 + 2]});
+    else
+      out[ii] = winding_number<float>(verts, tris, nt,
+                                      {pts[ii * 3], pts[ii * 3 + 1], pts[ii * 3 + 2]});
+  }
+}
+
+// compute_mvc_weights alone (no inside test), per point: [n*nv]
+ORC_API void orc_mvc_weights(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                             const float* pts, uint64_t n, int fp64, void* out_w) {
+  for (uint64_t i = 0; i < n; ++i) {
+    if (fp64) {
+      std::vector<double> dist(nv);
+      std::vector<W3<double>> dir(nv);
+      const double p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_weights<double>(verts, nv, tris, nt, p, (double*)out_w + i * nv, dist.data(), dir.data());
+    } else {
+      std::vector<float> dist(nv);
+      std::vector<W3<float>> dir(nv);
+      const float p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_weights<float>(verts, nv, tris, nt, p, (float*)out_w + i * nv, dist.data(), dir.data());
+    }
+  }
+}

@@ -322,3 +322,53 @@ def bind_tets(verts, elems, inv_dm, positions, bounds=None):
     w = np.zeros((p.shape[0], 4), np.float32)
     L.orc_bind_tets(_p(v), _p(e), _p(m), e.shape[0], _p(p), p.shape[0], _p(b), _p(tet), _p(w))
     return tet, w
+
+
+def mvc_bind(verts, tris, points, fp64=False, gradients=True):
+    """PolyCageMesh::bind_gaussians (mvc.rs:498-541) restated: (weights [n,nv],
+    gradients [n,nv,3] or None, inside [n] bool)."""
+    L = lib()
+    if not hasattr(L, "_mvc_sig"):
+        L.orc_mvc_bind.argtypes = [_vp, _u32, _vp, _u32, _vp, _u64, _i, _vp, _vp, _vp]
+        L.orc_mvc_bind.restype = None
+        L.orc_mvc_winding.argtypes = [_vp, _vp, _u32, _vp, _u64, _i, _vp]
+        L.orc_mvc_winding.restype = None
+        L._mvc_sig = True
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    n, nv = p.shape[0], v.shape[0]
+    dt = np.float64 if fp64 else np.float32
+    w = np.zeros((n, nv), dt)
+    g = np.zeros((n, nv, 3), dt) if gradients else None
+    m = np.zeros(n, np.uint8)
+    L.orc_mvc_bind(_p(v), nv, _p(t), t.shape[0], _p(p), n, 1 if fp64 else 0, _p(w), _p(g), _p(m))
+    return w, g, m.astype(bool)
+
+
+def mvc_winding(verts, tris, points, fp64=False):
+    """PolyCageMesh::winding_number (mvc.rs:326-347) per point (as float64)."""
+    mvc_bind(verts, tris, np.zeros((0, 3), np.float32))  # signatures
+    L = lib()
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(p.shape[0], np.float64)
+    L.orc_mvc_winding(_p(v), _p(t), t.shape[0], _p(p), p.shape[0], 1 if fp64 else 0, _p(out))
+    return out
+
+
+def mvc_weights(verts, tris, points, fp64=False):
+    """PolyCageMesh::compute_mvc_weights (mvc.rs:366-471) without the inside test."""
+    L = lib()
+    if not hasattr(L, "_mvcw_sig"):
+        L.orc_mvc_weights.argtypes = [_vp, _u32, _vp, _u32, _vp, _u64, _i, _vp]
+        L.orc_mvc_weights.restype = None
+        L._mvcw_sig = True
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    w = np.zeros((p.shape[0], v.shape[0]), np.float64 if fp64 else np.float32)
+    L.orc_mvc_weights(_p(v), v.shape[0], _p(t), t.shape[0], _p(p), p.shape[0], 1 if fp64 else 0,
+                      _p(w))
+    return w

@@ -562,3 +562,116 @@ def test_binding_oracle_equals_grid_search_restatement(oracle_lib):
         for tt in range(int(t[i])):
             ww = mesh.barycentric(p[i:i + 1], np.array([tt]))[0]
             assert np.maximum(-ww, 0).sum() > 1e-4
+
+
+# ---------------------------------------------------------------------------
+# Mean value coordinates (mvc.rs:366-541) against the reference's tests and
+# the identities that define them
+# ---------------------------------------------------------------------------
+
+def _mvc_cage(kind, a, b):
+    m = FIX["mvc_cages"]
+    if kind == "box":
+        lo, hi = a, b
+        V = [[lo[0], lo[1], lo[2]], [hi[0], lo[1], lo[2]], [hi[0], hi[1], lo[2]],
+             [lo[0], hi[1], lo[2]], [lo[0], lo[1], hi[2]], [hi[0], lo[1], hi[2]],
+             [hi[0], hi[1], hi[2]], [lo[0], hi[1], hi[2]]]
+        return np.array(V, np.float32), np.array(m["box_triangles"], np.uint32)
+    c, r = np.array(a, np.float32), np.float32(max(abs(b), 1e-4))
+    if kind == "octahedron":
+        d = [[r, 0, 0], [-r, 0, 0], [0, r, 0], [0, -r, 0], [0, 0, r], [0, 0, -r]]
+        return (c + np.array(d, np.float32)).astype(np.float32), \
+            np.array(m["octahedron_triangles"], np.uint32)
+    phi = (1.0 + math.sqrt(5.0)) * 0.5
+    raw = np.array([[-1, phi, 0], [1, phi, 0], [-1, -phi, 0], [1, -phi, 0], [0, -1, phi],
+                    [0, 1, phi], [0, -1, -phi], [0, 1, -phi], [phi, 0, -1], [phi, 0, 1],
+                    [-phi, 0, -1], [-phi, 0, 1]], np.float32)
+    raw = raw / np.linalg.norm(raw, axis=1, keepdims=True)
+    return (c + raw * r).astype(np.float32), np.array(m["icosahedron_triangles"], np.uint32)
+
+
+def test_mvc_presets_and_icosphere_are_closed():
+    from cindergs_amd.mvc import PolyCage, icosahedron, icosphere
+
+    V, T = _mvc_cage("box", [-1.0] * 3, [1.0] * 3)
+    box = PolyCage(V, T)
+    assert box.is_closed_manifold() and abs(box.signed_volume() - 8.0) < 1e-4
+    for kind, a, b in (("octahedron", [0, 0, 0], 1.5), ("icosahedron", [0, 0, 0], 2.0)):
+        c = PolyCage(*_mvc_cage(kind, a, b))
+        assert c.is_closed_manifold() and c.signed_volume() > 0
+    ico = icosahedron()
+    assert ico.is_closed_manifold() and ico.signed_volume() > 0
+    sph = icosphere()
+    assert sph.num_vertices == 642 and sph.triangles.shape[0] == 1280
+    assert sph.is_closed_manifold() and sph.signed_volume() > 0
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_mvc_reference_reproduction(oracle_lib, fp64):
+    case = FIX["mvc"][1]
+    for kind, a, b in case["cages"]:
+        V, T = _mvc_cage(kind, a, b)
+        w, _, inside = oracle_lib.mvc_bind(V, T, np.array([case["point"]], np.float32), fp64)
+        assert inside[0]
+        assert abs(w[0].sum() - 1.0) < case["sum_tol"]
+        assert np.linalg.norm(w[0] @ V.astype(np.float64) - case["point"]) < case["reproduction_tol"]
+        if kind != "box":  # convex cages: positive coordinates
+            assert w[0].min() > -1e-6
+    c = FIX["mvc"][2]
+    V, T = _mvc_cage("octahedron", *c["octahedron"])
+    w = oracle_lib.mvc_weights(V, T, np.array([c["vertex"], c["face_point"]], np.float32), fp64)
+    assert abs(w[0][c["vertex_index"]] - 1.0) < 1e-6
+    assert np.linalg.norm(w[1] @ V.astype(np.float64) - c["face_point"]) < c["tol"]
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_mvc_gradients_match_local_jacobian(oracle_lib, fp64):
+    """mvc.rs:885-907: sum_j v'_j grad(w_j)^T equals the central difference of
+    the deformed position; for the undeformed cage both are the identity
+    (linear precision)."""
+    c = FIX["mvc"][3]
+    V, T = _mvc_cage("octahedron", *c["octahedron"])
+    pts = np.array(c["points"], np.float32)
+    w, g, inside = oracle_lib.mvc_bind(V, T, pts, fp64)
+    assert inside.all()
+    Vd = V.astype(np.float64).copy()
+    for vi, ax, dv in c["deform"]:
+        Vd[vi, ax] += dv
+    lo, hi = V.min(0), V.max(0)
+    h = min(max(np.float32(np.linalg.norm(hi - lo)) * np.float32(1e-3), 1e-5), 1e-2)
+    for i, p in enumerate(pts):
+        F = np.einsum("jk,jl->lk", g[i], Vd)  # column k = sum_j v'_j g_jk
+        cols = []
+        for ax in range(3):
+            pp, pm = p.copy(), p.copy()
+            pp[ax] += np.float32(h)
+            pm[ax] -= np.float32(h)
+            wp, _, _ = oracle_lib.mvc_bind(V, T, pp[None], fp64, gradients=False)
+            wm, _, _ = oracle_lib.mvc_bind(V, T, pm[None], fp64, gradients=False)
+            cols.append((wp[0] @ Vd - wm[0] @ Vd) * (0.5 / h))
+        assert np.abs(F - np.stack(cols, 1)).max() < c["tol"]
+        Fi = np.einsum("jk,jl->lk", g[i], V.astype(np.float64))
+        assert np.abs(Fi - np.eye(3)).max() < (1e-3 if not fp64 else 1e-6)
+
+
+def test_mvc_fp32_matches_fp64_on_the_icosphere(oracle_lib):
+    from cindergs_amd.mvc import icosphere
+
+    cage = icosphere()
+    rng = np.random.default_rng(9)
+    d = rng.normal(size=(64, 3))
+    p = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(0, 0.95, (64, 1))
+         ).astype(np.float32)  # inside the radius 1.2 - 0.15 ball
+    w32, _, in32 = oracle_lib.mvc_bind(cage.vertices, cage.triangles, p, False, gradients=False)
+    w64, _, in64 = oracle_lib.mvc_bind(cage.vertices, cage.triangles, p, True, gradients=False)
+    assert in32.all() and in64.all()
+    # near the surface the small angles lose relative precision in f32: the
+    # weights agree to 1e-4 absolute, the reproduced positions to 1e-5 of the
+    # cage diagonal
+    assert np.abs(w32 - w64).max() < 1e-4
+    diag = cage.diagonal()
+    assert np.abs(w32.astype(np.float64) @ cage.vertices.astype(np.float64) - p).max() < 1e-5 * diag
+    assert np.abs(w64.sum(1) - 1).max() < 1e-12
+    assert np.abs(w64 @ cage.vertices.astype(np.float64) - p).max() < 1e-12
+    far = np.array([[3.0, 0, 0], [0, -2.5, 0.1]], np.float32)
+    assert np.all(np.abs(oracle_lib.mvc_winding(cage.vertices, cage.triangles, far, True)) < 1e-6)
