@@ -126,6 +126,17 @@ SIGNATURES = {
     "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
     "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
+    "cgs_mlp_create": [_vp, _u32, C.c_float, _u64, C.POINTER(_vp)],
+    "cgs_mlp_destroy": [_vp],
+    "cgs_mlp_param_count": [_vp],
+    "cgs_mlp_layout": [_vp, _vp, _vp, _vp],
+    "cgs_mlp_get_params": [_vp, _vp, _vp],
+    "cgs_mlp_set_params": [_vp, _vp, _vp],
+    "cgs_mlp_train": [_vp, _vp, _vp, _u64, _u32, _u32, C.c_float, _vp],
+    "cgs_mlp_gradient": [_vp, _vp, _vp, _u32, _vp, _vp],
+    "cgs_mlp_infer": [_vp, _vp, _u64, _vp, _vp],
+    "cgs_mlp_bind": [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _u64, C.c_float, _vp, _vp, _vp,
+                     C.POINTER(_u64)],
     "cgs_load_projected": [_vp, _vp, _vp, _u64],
     "cgs_run_stage3": [_vp, C.POINTER(CameraUniforms), C.POINTER(CompositorUniforms),
                        C.POINTER(_u64)],
@@ -158,8 +169,10 @@ def load() -> C.CDLL:
         fn.argtypes = args
         if name == "cgs_last_error":
             fn.restype = C.c_char_p
-        elif name == "cgs_destroy":
+        elif name in ("cgs_destroy", "cgs_mlp_destroy"):
             fn.restype = None
+        elif name == "cgs_mlp_param_count":
+            fn.restype = _u64
         else:
             fn.restype = C.c_int
     _lib = lib

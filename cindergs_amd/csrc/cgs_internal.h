@@ -337,13 +337,86 @@ struct CgsMvcArgs {
   float* out_w;             // [n][nv]
   float* out_g;             // [n][nv][3] or null
   uint32_t* mask;           // [n] inside flags or null
+  const uint32_t* only;     // [n] points to bind (others untouched) or null = all
 };
 size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt);
 void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s);
 void cgs_launch_mvc_winding(const CgsMvcArgs& a, float* out, hipStream_t s);
+// inside = |winding| >= 0.5; flag = inside && residual > tol; rows of points
+// outside zeroed (w [n x nv], g [n x nv x 3] or null); block counts of flags
+void cgs_launch_mvc_select(const float* winding, const float* residual, float tol, uint32_t n,
+                           uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
+                           uint32_t* block_counts, hipStream_t s);
+uint32_t cgs_mvc_select_blocks(uint32_t n);
 void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
                            const float* rest_pos, const float* dverts, uint32_t nv, uint32_t n,
                            float* out_pos, float* out_F, hipStream_t s);
+
+// ---- surrogate network (k_mlp.hip, mlp.cpp) ----
+#define CGS_MLP_LAYERS 5
+struct CgsGemmArgs {       // C = A B^T, A [M x K], B [N x K] bf16 (K % 32 == 0, ld % 8 == 0)
+  const uint16_t* A;
+  uint32_t lda;
+  const uint16_t* B;
+  uint32_t ldb;
+  uint32_t M, N, K;
+  const float* bias;       // bias[col * bias_stride] or null
+  uint32_t bias_stride;
+  uint32_t relu;
+  const uint16_t* mask;    // C *= (mask[(row / mask_div) * ldm + col] > 0) or null
+  uint32_t ldm, mask_div;
+  float* Cf;               // f32 [M x ldcf] or null
+  uint32_t ldcf;
+  uint16_t* Cb;            // bf16 [M x ldcb] or null
+  uint32_t ldcb;
+  uint16_t* Ct;            // bf16 transposed [N x ldct] or null (M % 4 == 0)
+  uint32_t ldct;
+};
+struct CgsAdamArgs {
+  float* w;                // flat master parameters (layer l: rows x (cols + 32), bias in col `cols`)
+  float* m;
+  float* v;
+  const float* grad;
+  uint64_t total;
+  uint64_t off[CGS_MLP_LAYERS];
+  uint32_t rows[CGS_MLP_LAYERS], cols[CGS_MLP_LAYERS];
+  uint16_t* wb[CGS_MLP_LAYERS];  // bf16 [rows x cols]
+  uint16_t* wt[CGS_MLP_LAYERS];  // bf16 [cols x rows]
+  float b1, b2, lr_t, eps_t;
+};
+void cgs_launch_gemm_nt(const CgsGemmArgs& g, hipStream_t s);
+void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const float* Bm,
+                        uint16_t* A0, uint16_t* A0t, uint32_t ldt, uint16_t* T0, hipStream_t s);
+void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,
+                         uint32_t P, uint32_t V, uint16_t* dZ, uint16_t* dZt, uint32_t ldt,
+                         float* row_loss, hipStream_t s);
+void cgs_launch_sum(const float* v, uint32_t n, float div, float* out, hipStream_t s);
+void cgs_launch_adam(const CgsAdamArgs& a, bool update, hipStream_t s);
+void cgs_launch_sample(uint32_t* idx, uint32_t P, uint64_t n, uint64_t seed, uint32_t step,
+                       hipStream_t s);
+struct CgsMlpPostArgs {
+  const float* Z;       // [P x ldz]
+  const float* TZ;      // [3P x ldz] tangents or null
+  uint32_t ldz, P, V;
+  const float* X;       // [P x 3]
+  const float* cage;    // [V x 3] rest cage or null (no residual, no projection)
+  float minv[16];       // (A A^T)^-1, A = [1; v] (4 x V), row-major
+  float* out_w;         // [P x V]
+  float* out_g;         // [P x V x 3] or null
+  float* residual;      // [P] or null
+};
+void cgs_launch_mlp_post(const CgsMlpPostArgs& a, hipStream_t s);
+
+struct cgs_mlp;
+int cgs_mlp_infer_residual(struct cgs_mlp* M, const float* d_x, uint64_t n, float* d_w,
+                           float* d_grad, const float* d_cage, const float* minv16,
+                           float* d_residual);
+hipStream_t cgs_mlp_stream(struct cgs_mlp* M);
+uint32_t cgs_mlp_outputs(struct cgs_mlp* M);
+
+// error reporting and context access shared by the host files
+int cgs_fail(int code, const char* msg);
+hipStream_t cgs_context_stream(struct cgs_context* c);
 
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved

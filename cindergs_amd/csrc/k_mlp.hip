@@ -1,0 +1,376 @@
+// k_mlp.hip — the MVC surrogate network: a coordinate MLP, trained and run
+// with hand-written BF16 MFMA kernels (FP32 accumulation, FP32 master weights).
+//
+// Network (no counterpart in the reference, which always evaluates exact MVC,
+// src/deformation/mvc.rs:366-541): x (3) -> Fourier features
+// [sin 2pi Bx, cos 2pi Bx] (B: 32 x 3, seeded N(0, sigma^2)) -> 4 x (Linear 256
+// + ReLU) -> Linear n_out -> affine normalisation w = z + (1 - sum z) / n_out, so
+// the weights sum to 1 as mean value coordinates do.  Loss: mean squared error
+// against exact MVC weights.
+//
+// gfx950 design:
+//  * one GEMM kernel, C = A B^T with A [M x K] and B [N x K] both K-contiguous
+//    bf16, v_mfma_f32_16x16x32_bf16 tiles (a 64 x 64 block tile, four waves of
+//    32 x 32), fragments loaded straight from global memory as 16-byte rows;
+//    epilogues fuse bias, ReLU, the ReLU mask of the backward pass, f32 or
+//    bf16 stores and a transposed bf16 copy (the layout the next GEMM needs);
+//  * every layer's weights are stored with 32 extra columns, column `cols`
+//    holding the bias: the activations' transposed copies carry a row of ones
+//    there, so the weight-gradient GEMM produces the bias gradient too;
+//  * all parameters live in one flat f32 buffer, updated by one Adam kernel
+//    that also refreshes the bf16 weights and their transposes;
+//  * inference propagates the three input tangents through the same GEMMs
+//    (forward-mode derivative, masked by the primal ReLU pattern), which gives
+//    the weight gradients dw/dx that a binding needs.
+#include "cgs_internal.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ __launch_bounds__(256) void k_gemm_nt(CgsGemmArgs G) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t m0 = blockIdx.y * 64u + (wave >> 1) * 32u;
+  const uint32_t n0 = blockIdx.x * 64u + (wave & 1u) * 32u;
+  const uint32_t r = lane & 15u, kq = (lane >> 4) * 8u;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero = {};
+  for (uint32_t k = 0; k < G.K; k += 32u) {
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const uint32_t row = m0 + i * 16u + r;
+      a[i] = row < G.M ? *reinterpret_cast<const bf16x8*>(G.A + (size_t)row * G.lda + k + kq)
+                       : zero;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t col = n0 + j * 16u + r;
+      b[j] = col < G.N ? *reinterpret_cast<const bf16x8*>(G.B + (size_t)col * G.ldb + k + kq)
+                       : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  // lane holds C[row0 + q][col] for q = 0..3
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t col = n0 + j * 16u + (lane & 15u);
+      const uint32_t row0 = m0 + i * 16u + (lane >> 4) * 4u;
+      if (col >= G.N) continue;
+      float v[4];
+      const float bias = G.bias ? G.bias[(size_t)col * G.bias_stride] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float x = acc[i][j][q] + bias;
+        if (G.relu) x = fmaxf(x, 0.0f);
+        if (G.mask) {
+          const uint32_t row = row0 + q;
+          const uint16_t mv = row < G.M ? G.mask[(size_t)(row / G.mask_div) * G.ldm + col] : 0;
+          x = ((mv & 0x8000u) == 0u && mv != 0u) ? x : 0.0f;
+        }
+        v[q] = x;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t row = row0 + q;
+        if (row >= G.M) continue;
+        if (G.Cf) G.Cf[(size_t)row * G.ldcf + col] = v[q];
+        if (G.Cb) G.Cb[(size_t)row * G.ldcb + col] = f2bf(v[q]);
+      }
+      if (G.Ct && row0 + 3u < G.M) {
+        const uint64_t packed = (uint64_t)f2bf(v[0]) | ((uint64_t)f2bf(v[1]) << 16) |
+                                ((uint64_t)f2bf(v[2]) << 32) | ((uint64_t)f2bf(v[3]) << 48);
+        *reinterpret_cast<uint64_t*>(G.Ct + (size_t)col * G.ldct + row0) = packed;
+      }
+    }
+}
+
+// Fourier features of P points (gathered through idx when given): A0 [P x 64]
+// and its transpose [64(+32) x P]; tangents T0 [3P x 64] (row 3p + axis).
+__global__ __launch_bounds__(256) void k_fourier(const float* X, const uint32_t* idx, uint32_t P,
+                                                 const float* Bm, uint16_t* A0, uint16_t* A0t,
+                                                 uint32_t ldt, uint16_t* T0) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t p = t >> 5, k = t & 31u;
+  if (p >= P) return;
+  const uint32_t src = idx ? idx[p] : p;
+  const float x = X[(size_t)src * 3], y = X[(size_t)src * 3 + 1], z = X[(size_t)src * 3 + 2];
+  const float b0 = Bm[k * 3], b1 = Bm[k * 3 + 1], b2 = Bm[k * 3 + 2];
+  const float TWO_PI = 6.283185307179586f;
+  const float s = TWO_PI * ((b0 * x + b1 * y) + b2 * z);
+  float sn, cs;
+  sincosf(s, &sn, &cs);
+  A0[(size_t)p * 64 + k] = f2bf(sn);
+  A0[(size_t)p * 64 + 32 + k] = f2bf(cs);
+  if (A0t) {
+    A0t[(size_t)k * ldt + p] = f2bf(sn);
+    A0t[(size_t)(32 + k) * ldt + p] = f2bf(cs);
+  }
+  if (T0) {
+    const float bb[3] = {b0, b1, b2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      T0[((size_t)p * 3 + a) * 64 + k] = f2bf(TWO_PI * bb[a] * cs);
+      T0[((size_t)p * 3 + a) * 64 + 32 + k] = f2bf(-TWO_PI * bb[a] * sn);
+    }
+  }
+}
+
+__device__ __forceinline__ float wave_sum64(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Affine normalisation, squared error and its gradient, one wave per row:
+// w = z + (1 - sum z)/V, e = w - target, loss_row = sum e^2,
+// dL/dz_j = g_j - mean(g) with g = 2 e / (P V).  Writes dZ [P x ldz] and its
+// transpose [ldz(+32) x P] in bf16 (columns >= V stay zero).
+__global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
+                                                  const float* targets, const uint32_t* idx,
+                                                  uint32_t P, uint32_t V, uint16_t* dZ,
+                                                  uint16_t* dZt, uint32_t ldt, float* row_loss) {
+  const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (p >= P) return;
+  const float* z = Z + (size_t)p * ldz;
+  const float* tg = targets + (size_t)(idx ? idx[p] : p) * V;
+  float s = 0.0f;
+  for (uint32_t j = lane; j < V; j += 64u) s += z[j];
+  s = wave_sum64(s);
+  const float shift = (1.0f - s) / (float)V;
+  const float scale = 2.0f / ((float)P * (float)V);
+  float gsum = 0.0f, l2 = 0.0f;
+  for (uint32_t j = lane; j < V; j += 64u) {
+    const float e = (z[j] + shift) - tg[j];
+    l2 += e * e;
+    gsum += scale * e;
+  }
+  gsum = wave_sum64(gsum);
+  l2 = wave_sum64(l2);
+  const float gmean = gsum / (float)V;
+  for (uint32_t j = lane; j < V; j += 64u) {
+    const float e = (z[j] + shift) - tg[j];
+    const uint16_t d = f2bf(scale * e - gmean);
+    dZ[(size_t)p * ldz + j] = d;
+    dZt[(size_t)j * ldt + p] = d;
+  }
+  if (lane == 0 && row_loss) row_loss[p] = l2;
+}
+
+// one block: out = sum(v[0..n)) / div
+__global__ __launch_bounds__(1024) void k_sum(const float* v, uint32_t n, float div, float* out) {
+  __shared__ float s[16];
+  float a = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024u) a += v[i];
+  a = wave_sum64(a);
+  if ((threadIdx.x & 63u) == 0) s[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int w = 0; w < 16; ++w) t += s[w];
+    *out = t / div;
+  }
+}
+
+// Adam on the flat parameter buffer, then the bf16 weights [rows x cols] and
+// transposed [cols x rows] of each layer (bias column excluded) refreshed.
+__global__ __launch_bounds__(256) void k_adam(CgsAdamArgs A) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (e >= A.total) return;
+  const float g = A.grad[e];
+  const float m = A.b1 * A.m[e] + (1.0f - A.b1) * g;
+  const float v = A.b2 * A.v[e] + (1.0f - A.b2) * g * g;
+  A.m[e] = m;
+  A.v[e] = v;
+  const float w = A.w[e] - A.lr_t * m / (sqrtf(v) + A.eps_t);
+  A.w[e] = w;
+  int l = 0;
+#pragma unroll
+  for (int k = 1; k < CGS_MLP_LAYERS; ++k)
+    if (e >= A.off[k]) l = k;
+  const uint64_t r = e - A.off[l];
+  const uint32_t ld = A.cols[l] + 32u;
+  const uint32_t row = (uint32_t)(r / ld), col = (uint32_t)(r % ld);
+  if (col < A.cols[l]) {
+    const uint16_t b = f2bf(w);
+    A.wb[l][(size_t)row * A.cols[l] + col] = b;
+    A.wt[l][(size_t)col * A.rows[l] + row] = b;
+  }
+}
+
+// bf16 weights and transposes from the master weights (no update)
+__global__ __launch_bounds__(256) void k_refresh(CgsAdamArgs A) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (e >= A.total) return;
+  int l = 0;
+#pragma unroll
+  for (int k = 1; k < CGS_MLP_LAYERS; ++k)
+    if (e >= A.off[k]) l = k;
+  const uint64_t r = e - A.off[l];
+  const uint32_t ld = A.cols[l] + 32u;
+  const uint32_t row = (uint32_t)(r / ld), col = (uint32_t)(r % ld);
+  if (col < A.cols[l]) {
+    const uint16_t b = f2bf(A.w[e]);
+    A.wb[l][(size_t)row * A.cols[l] + col] = b;
+    A.wt[l][(size_t)col * A.rows[l] + row] = b;
+  }
+}
+
+// batch indices: a 64-bit mix of (seed, step, i) modulo n
+__global__ __launch_bounds__(256) void k_sample(uint32_t* idx, uint32_t P, uint64_t n,
+                                                uint64_t seed, uint32_t step) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= P) return;
+  uint64_t x = seed ^ ((uint64_t)step << 32) ^ (uint64_t)i * 0x9E3779B97F4A7C15ull;
+  x ^= x >> 33;
+  x *= 0xFF51AFD7ED558CCDull;
+  x ^= x >> 33;
+  x *= 0xC4CEB9FE1A85EC53ull;
+  x ^= x >> 33;
+  idx[i] = (uint32_t)(x % n);
+}
+
+// Surrogate outputs of P points (one wave per point): w = z + (1 - sum z)/V
+// and, from the tangent rows, dw/dx_a = t_a - (sum t_a)/V.  With a cage: the
+// reconstruction residual |sum_j w_j v_j - x| of these weights (the fallback
+// test), then the weights projected onto linear precision,
+// w' = w + A^T Minv ([1; x] - A w) with A = [1 ... 1; v_1 ... v_V] and
+// Minv = (A A^T)^-1 (so sum w' = 1 and sum w' v = x), and their derivatives
+// g'_a = g_a + A^T Minv (e_{a+1} - A g_a).
+__global__ __launch_bounds__(256) void k_mlp_post(CgsMlpPostArgs A) {
+  const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (p >= A.P) return;
+  const uint32_t V = A.V;
+  const float* z = A.Z + (size_t)p * A.ldz;
+  float s = 0.0f;
+  for (uint32_t j = lane; j < V; j += 64u) s += z[j];
+  s = wave_sum64(s);
+  const float shift = (1.0f - s) / (float)V;
+  float* wrow = A.out_w + (size_t)p * V;
+  float* grow = A.out_g ? A.out_g + (size_t)p * V * 3 : nullptr;
+  if (A.TZ && grow) {
+    for (int a = 0; a < 3; ++a) {
+      const float* t = A.TZ + ((size_t)p * 3 + a) * A.ldz;
+      float ts = 0.0f;
+      for (uint32_t j = lane; j < V; j += 64u) ts += t[j];
+      ts = wave_sum64(ts) / (float)V;
+      for (uint32_t j = lane; j < V; j += 64u) grow[(size_t)j * 3 + a] = t[j] - ts;
+    }
+  }
+  if (!A.cage) {
+    for (uint32_t j = lane; j < V; j += 64u) wrow[j] = z[j] + shift;
+    return;
+  }
+  const float* C = A.cage;
+  float s0 = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (uint32_t j = lane; j < V; j += 64u) {
+    const float w = z[j] + shift;
+    s0 += w;
+    sx += w * C[j * 3];
+    sy += w * C[j * 3 + 1];
+    sz += w * C[j * 3 + 2];
+  }
+  s0 = wave_sum64(s0);
+  sx = wave_sum64(sx);
+  sy = wave_sum64(sy);
+  sz = wave_sum64(sz);
+  const float x = A.X[(size_t)p * 3], y = A.X[(size_t)p * 3 + 1], zz = A.X[(size_t)p * 3 + 2];
+  if (A.residual && lane == 0)
+    A.residual[p] = sqrtf(((sx - x) * (sx - x) + (sy - y) * (sy - y)) + (sz - zz) * (sz - zz));
+  const float r[4] = {1.0f - s0, x - sx, y - sy, zz - sz};
+  float lam[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    lam[i] = ((A.minv[i * 4] * r[0] + A.minv[i * 4 + 1] * r[1]) + A.minv[i * 4 + 2] * r[2]) +
+             A.minv[i * 4 + 3] * r[3];
+  for (uint32_t j = lane; j < V; j += 64u)
+    wrow[j] = (z[j] + shift) + ((lam[0] + lam[1] * C[j * 3]) + lam[2] * C[j * 3 + 1]) +
+              lam[3] * C[j * 3 + 2];
+  if (!(A.TZ && grow)) return;
+  for (int a = 0; a < 3; ++a) {
+    float c0 = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (uint32_t j = lane; j < V; j += 64u) {
+      const float g = grow[(size_t)j * 3 + a];
+      c0 += g;
+      cx += g * C[j * 3];
+      cy += g * C[j * 3 + 1];
+      cz += g * C[j * 3 + 2];
+    }
+    c0 = wave_sum64(c0);
+    cx = wave_sum64(cx);
+    cy = wave_sum64(cy);
+    cz = wave_sum64(cz);
+    const float e[4] = {-c0, (a == 0 ? 1.0f : 0.0f) - cx, (a == 1 ? 1.0f : 0.0f) - cy,
+                        (a == 2 ? 1.0f : 0.0f) - cz};
+    float la[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      la[i] = ((A.minv[i * 4] * e[0] + A.minv[i * 4 + 1] * e[1]) + A.minv[i * 4 + 2] * e[2]) +
+              A.minv[i * 4 + 3] * e[3];
+    for (uint32_t j = lane; j < V; j += 64u)
+      grow[(size_t)j * 3 + a] += ((la[0] + la[1] * C[j * 3]) + la[2] * C[j * 3 + 1]) +
+                                 la[3] * C[j * 3 + 2];
+  }
+}
+
+}  // namespace
+
+void cgs_launch_gemm_nt(const CgsGemmArgs& g, hipStream_t s) {
+  if (g.M == 0 || g.N == 0) return;
+  hipLaunchKernelGGL(k_gemm_nt, dim3((g.N + 63u) / 64u, (g.M + 63u) / 64u), dim3(256), 0, s, g);
+}
+
+void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const float* Bm,
+                        uint16_t* A0, uint16_t* A0t, uint32_t ldt, uint16_t* T0, hipStream_t s) {
+  if (P == 0) return;
+  hipLaunchKernelGGL(k_fourier, dim3((P * 32u + 255u) / 256u), dim3(256), 0, s, X, idx, P, Bm, A0,
+                     A0t, ldt, T0);
+}
+
+void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,
+                         uint32_t P, uint32_t V, uint16_t* dZ, uint16_t* dZt, uint32_t ldt,
+                         float* row_loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_mlp_loss, dim3((P + 3u) / 4u), dim3(256), 0, s, Z, ldz, targets, idx, P, V,
+                     dZ, dZt, ldt, row_loss);
+}
+
+void cgs_launch_sum(const float* v, uint32_t n, float div, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, v, n, div, out);
+}
+
+void cgs_launch_adam(const CgsAdamArgs& a, bool update, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((a.total + 255u) / 256u);
+  if (update)
+    hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, a);
+}
+
+void cgs_launch_sample(uint32_t* idx, uint32_t P, uint64_t n, uint64_t seed, uint32_t step,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_sample, dim3((P + 255u) / 256u), dim3(256), 0, s, idx, P, n, seed, step);
+}
+
+void cgs_launch_mlp_post(const CgsMlpPostArgs& a, hipStream_t s) {
+  if (a.P == 0) return;
+  hipLaunchKernelGGL(k_mlp_post, dim3((a.P + 3u) / 4u), dim3(256), 0, s, a);
+}

@@ -190,6 +190,7 @@ __global__ __launch_bounds__(64) void k_mvc_bind(CgsMvcArgs A) {
   const uint32_t lane = threadIdx.x;
   const uint32_t i = blockIdx.x;
   if (i >= A.n) return;
+  if (A.only && A.only[i] == 0u) return;
   const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
   float* row = A.out_w + (size_t)i * A.nv;
   float* grow = A.out_g ? A.out_g + (size_t)i * A.nv * 3 : nullptr;
@@ -282,7 +283,41 @@ __global__ __launch_bounds__(64) void k_mvc_deform(const float* w, const float* 
   }
 }
 
+__global__ __launch_bounds__(256) void k_mvc_select(const float* winding, const float* residual,
+                                                    float tol, uint32_t n, uint32_t nv,
+                                                    uint32_t* inside, uint32_t* flags, float* w,
+                                                    float* g, uint32_t* block_counts) {
+  __shared__ uint32_t s_c[4];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  uint32_t f = 0u;
+  if (i < n) {
+    const bool in = fabsf(winding[i]) >= 0.5f;
+    f = (in && residual[i] > tol) ? 1u : 0u;
+    inside[i] = in ? 1u : 0u;
+    flags[i] = f;
+    if (!in) {
+      for (uint32_t v = 0; v < nv; ++v) w[(size_t)i * nv + v] = 0.0f;
+      if (g)
+        for (uint32_t v = 0; v < 3 * nv; ++v) g[(size_t)i * nv * 3 + v] = 0.0f;
+    }
+  }
+  const uint32_t c = (uint32_t)__popcll(__ballot(f != 0u));
+  if ((threadIdx.x & 63u) == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
 }  // namespace
+
+uint32_t cgs_mvc_select_blocks(uint32_t n) { return (n + 255u) / 256u; }
+
+void cgs_launch_mvc_select(const float* winding, const float* residual, float tol, uint32_t n,
+                           uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
+                           uint32_t* block_counts, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mvc_select, dim3(cgs_mvc_select_blocks(n)), dim3(256), 0, s, winding,
+                     residual, tol, n, nv, inside, flags, w, g, block_counts);
+}
 
 size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt) {
   return (size_t)nv * 16 + (size_t)nt * 12 + (size_t)nv * 8;

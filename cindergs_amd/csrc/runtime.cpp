@@ -1494,6 +1494,7 @@ int cgs_mvc_bind_exact(cgs_context* c, const float* cage_vertices, uint32_t num_
   A.out_w = d_weights;
   A.out_g = d_gradients;
   A.mask = d_inside;
+  A.only = nullptr;
   cgs_launch_mvc_bind(A, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1522,6 +1523,125 @@ int cgs_mvc_winding(cgs_context* c, const float* cage_vertices, uint32_t num_ver
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
+}
+
+// The surrogate binding: the network's weights and gradients for every
+// point, exact MVC (the reference routine) for every point whose
+// reconstruction residual |sum_j w_j v_j - x| exceeds residual_tol times the
+// cage's bounding-box diagonal, zero rows outside the cage (same inside test
+// as the exact binding).  Same outputs as cgs_mvc_bind_exact.
+int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices,
+                 uint32_t num_vertices, const uint32_t* triangles, uint32_t num_triangles,
+                 const float* d_points, uint64_t n, float residual_tol, float* d_weights,
+                 float* d_gradients, uint32_t* d_inside, uint64_t* fallback_count) {
+  TRY(ensure_ctx(c));
+  if (!net || !d_points || !d_weights || !d_inside)
+    return fail(CGS_ERR_INVALID_ARG, "null argument");
+  if (cgs_mlp_outputs(net) != num_vertices)
+    return fail(CGS_ERR_INVALID_ARG, "the network's outputs do not match the cage's vertices");
+  if (n == 0) {
+    if (fallback_count) *fallback_count = 0;
+    return CGS_OK;
+  }
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  MvcCage M;
+  TRY(mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M));
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < num_vertices; ++v)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = cgs_fmin(lo[k], cage_vertices[(size_t)v * 3 + k]);
+      hi[k] = cgs_fmax(hi[k], cage_vertices[(size_t)v * 3 + k]);
+    }
+  const float diag = sqrtf(((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1])) +
+                           (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  const uint32_t nb = cgs_mvc_select_blocks((uint32_t)n);
+  void *d_wind = nullptr, *d_res = nullptr, *d_flags = nullptr, *d_counts = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {d_wind, d_res, d_flags, d_counts})
+      if (p) (void)hipFree(p);
+  };
+  int rc = CGS_OK;
+  do {
+    if (hipMalloc(&d_wind, n * 4) != hipSuccess || hipMalloc(&d_res, n * 4) != hipSuccess ||
+        hipMalloc(&d_flags, n * 4) != hipSuccess || hipMalloc(&d_counts, (size_t)nb * 4) != hipSuccess) {
+      rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (surrogate binding)");
+      break;
+    }
+    CgsMvcArgs A;
+    memset(&A, 0, sizeof(A));
+    A.verts = (const float*)M.d_verts;
+    A.nv = num_vertices;
+    A.tris = (const uint32_t*)M.d_tris;
+    A.nt = num_triangles;
+    A.vt_start = (const uint32_t*)M.d_start;
+    A.vt_list = (const uint32_t*)M.d_list;
+    A.pts = d_points;
+    A.n = (uint32_t)n;
+    A.h = M.h;
+    cgs_launch_mvc_winding(A, (float*)d_wind, c->stream);
+    // (A A^T)^-1 for the linear-precision projection, A = [1; v] (4 x nv)
+    double G[4][4] = {{0}};
+    for (uint32_t v = 0; v < num_vertices; ++v) {
+      const double a[4] = {1.0, cage_vertices[(size_t)v * 3], cage_vertices[(size_t)v * 3 + 1],
+                           cage_vertices[(size_t)v * 3 + 2]};
+      for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) G[i][k] += a[i] * a[k];
+    }
+    double aug[4][8];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 8; ++k) aug[i][k] = k < 4 ? G[i][k] : (k - 4 == i ? 1.0 : 0.0);
+    bool singular = false;
+    for (int col = 0; col < 4 && !singular; ++col) {
+      int piv = col;
+      for (int r2 = col + 1; r2 < 4; ++r2)
+        if (std::fabs(aug[r2][col]) > std::fabs(aug[piv][col])) piv = r2;
+      if (std::fabs(aug[piv][col]) < 1e-12) {
+        singular = true;
+        break;
+      }
+      for (int k = 0; k < 8; ++k) std::swap(aug[col][k], aug[piv][k]);
+      const double d = aug[col][col];
+      for (int k = 0; k < 8; ++k) aug[col][k] /= d;
+      for (int r2 = 0; r2 < 4; ++r2)
+        if (r2 != col) {
+          const double f = aug[r2][col];
+          for (int k = 0; k < 8; ++k) aug[r2][k] -= f * aug[col][k];
+        }
+    }
+    if (singular) {
+      rc = fail(CGS_ERR_INVALID_ARG, "cage vertices are coplanar");
+      break;
+    }
+    float minv[16];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) minv[i * 4 + k] = (float)aug[i][4 + k];
+    rc = cgs_mlp_infer_residual(net, d_points, n, d_weights, d_gradients,
+                                (const float*)M.d_verts, minv, (float*)d_res);
+    if (rc != CGS_OK) break;
+    cgs_launch_mvc_select((const float*)d_wind, (const float*)d_res, residual_tol * diag,
+                          (uint32_t)n, num_vertices, d_inside, (uint32_t*)d_flags, d_weights,
+                          d_gradients, (uint32_t*)d_counts, c->stream);
+    A.out_w = d_weights;
+    A.out_g = d_gradients;
+    A.mask = d_inside;
+    A.only = (const uint32_t*)d_flags;
+    cgs_launch_mvc_bind(A, c->stream);
+    std::vector<uint32_t> counts(nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(counts.data(), d_counts, (size_t)nb * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      rc = fail(CGS_ERR_EXECUTION, std::string("surrogate binding: ") + hipGetErrorString(e));
+      break;
+    }
+    uint64_t fb = 0;
+    for (uint32_t k = 0; k < nb; ++k) fb += counts[k];
+    if (fallback_count) *fallback_count = fb;
+  } while (false);
+  (void)hipStreamSynchronize(c->stream);
+  cleanup();
+  return rc;
 }
 
 // evaluate_position + MvcBindings::local_jacobian (mvc.rs:66-78, 473-481) for
@@ -1554,4 +1674,11 @@ int cgs_mvc_deform(cgs_context* c, const float* d_weights, const float* d_gradie
   return CGS_OK;
 }
 
+
+}  // extern "C"
+
+int cgs_fail(int code, const char* msg) { return fail(code, msg ? msg : "error"); }
+hipStream_t cgs_context_stream(cgs_context* c) { return c ? c->stream : nullptr; }
+
+extern "C" {
 }  // extern "C"

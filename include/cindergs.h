@@ -203,6 +203,44 @@ int cgs_mvc_deform(cgs_context* ctx, const float* d_weights, const float* d_grad
                    const float* deformed_vertices, uint32_t num_vertices, uint64_t n,
                    float* d_out_positions, float* d_out_jacobians);
 
+/* ---- MVC surrogate network (no counterpart in the reference) ------------------
+ * A coordinate MLP that stands in for exact MVC when binding many points to one
+ * cage: x -> [sin 2pi Bx, cos 2pi Bx] (B 32 x 3, N(0, sigma^2)) -> 4 x
+ * (Linear 256 + ReLU) -> Linear n_out -> w = z + (1 - sum z) / n_out.  BF16 MFMA
+ * GEMMs with FP32 accumulation, FP32 master weights, Adam.  Flat parameter
+ * layout: layer l (0..4) is a row-major [rows_l x (cols_l + 32)] block at
+ * offsets_l, column cols_l holding the bias (see cgs_mlp_layout); rows =
+ * {256, 256, 256, 256, n_out rounded up to 32}, cols = {64, 256, 256, 256, 256}. */
+typedef struct cgs_mlp cgs_mlp;
+int cgs_mlp_create(cgs_context* ctx, uint32_t n_out, float fourier_sigma, uint64_t seed,
+                   cgs_mlp** out);
+void cgs_mlp_destroy(cgs_mlp* net);
+uint64_t cgs_mlp_param_count(const cgs_mlp* net);
+int cgs_mlp_layout(const cgs_mlp* net, uint32_t* rows, uint32_t* cols, uint64_t* offsets);
+/* flat parameters [param_count] and the Fourier matrix [32 x 3] (either NULL) */
+int cgs_mlp_get_params(cgs_mlp* net, float* params, float* fourier);
+int cgs_mlp_set_params(cgs_mlp* net, const float* params, const float* fourier);
+/* steps of Adam (lr, betas 0.9/0.999, eps 1e-8) on batches of `batch` (a
+ * multiple of 64) drawn from the n device pairs (d_x [3n], d_targets
+ * [n*n_out]); losses [steps] receives each step's mean squared error (or NULL) */
+int cgs_mlp_train(cgs_mlp* net, const float* d_x, const float* d_targets, uint64_t n,
+                  uint32_t steps, uint32_t batch, float lr, float* losses);
+/* loss and parameter gradient (flat layout, host) of the first n pairs (n a
+ * multiple of 64), without an update */
+int cgs_mlp_gradient(cgs_mlp* net, const float* d_x, const float* d_targets, uint32_t n,
+                     float* grad, float* loss);
+/* weights [n*n_out] and, if d_grad, their input derivatives [n*n_out*3] */
+int cgs_mlp_infer(cgs_mlp* net, const float* d_x, uint64_t n, float* d_w, float* d_grad);
+/* The surrogate for PolyCageMesh::bind_gaussians (mvc.rs:498-541): same outputs
+ * as cgs_mvc_bind_exact (zero rows outside), the network's weights and input
+ * derivatives for inside points whose residual |sum_j w_j v_j - x| is at most
+ * residual_tol x (cage bounding-box diagonal), exact MVC for the others;
+ * fallback_count receives how many points took the exact path. */
+int cgs_mlp_bind(cgs_context* ctx, cgs_mlp* net, const float* cage_vertices,
+                 uint32_t num_vertices, const uint32_t* triangles, uint32_t num_triangles,
+                 const float* d_points, uint64_t n, float residual_tol, float* d_weights,
+                 float* d_gradients, uint32_t* d_inside, uint64_t* fallback_count);
+
 /* Inject a projected splat set (reference layout, as cgs_read_projected) and
  * run Stage 3 / Stage 4 on it (the split of execute_frame_gpu used by
  * test_gpu_stage3_binning_and_ranges_match_cpu, gpu_parity_tests.rs:664-917). */

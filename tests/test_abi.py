@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(ROOT, "include", "cindergs.h")).read()
-    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(cgs_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|uint64_t|const char\*)\s+(cgs_\w+)\s*\(", text, re.M)))
 
 
 def test_header_and_binding_agree():

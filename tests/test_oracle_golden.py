@@ -675,3 +675,42 @@ def test_mvc_fp32_matches_fp64_on_the_icosphere(oracle_lib):
     assert np.abs(w64 @ cage.vertices.astype(np.float64) - p).max() < 1e-12
     far = np.array([[3.0, 0, 0], [0, -2.5, 0.1]], np.float32)
     assert np.all(np.abs(oracle_lib.mvc_winding(cage.vertices, cage.triangles, far, True)) < 1e-6)
+
+
+# ---------------------------------------------------------------------------
+# The surrogate network's CPU restatement: its backward pass against central
+# differences of its own loss (float64, no bf16 rounding)
+# ---------------------------------------------------------------------------
+
+def test_mlp_oracle_backward_matches_finite_differences():
+    import mlp_oracle as mo
+
+    rng = np.random.default_rng(11)
+    n_out = 20
+    rows = [256, 256, 256, 256, 32]
+    cols = [64, 256, 256, 256, 256]
+    offs = list(np.cumsum([0] + [r * (c + 32) for r, c in zip(rows, cols)])[:-1])
+    params = np.zeros(sum(r * (c + 32) for r, c in zip(rows, cols)))
+    for r, c, o in zip(rows, cols, offs):
+        blk = params[o:o + r * (c + 32)].reshape(r, c + 32)
+        blk[:, :c + 1] = rng.normal(0, 1.0 / math.sqrt(c), (r, c + 1))
+        if r == 32:
+            blk[n_out:] = 0.0
+    fourier = rng.normal(0, 1.0, (32, 3))
+    x = rng.uniform(-1, 1, (16, 3))
+    t = rng.dirichlet(np.ones(n_out), 16)
+    loss, grad = mo.loss_and_grad(params, fourier, rows, cols, offs, x, t, n_out, use_bf16=False)
+    for l, (r, c, o) in enumerate(zip(rows, cols, offs)):
+        for (i, j) in [(0, 0), (r // 2 if r > n_out else n_out - 1, c // 3), (1, c)]:  # c = bias
+            k = o + i * (c + 32) + j
+            eps = 1e-6
+            p1, p2 = params.copy(), params.copy()
+            p1[k] += eps
+            p2[k] -= eps
+            l1, _ = mo.loss_and_grad(p1, fourier, rows, cols, offs, x, t, n_out, use_bf16=False)
+            l2, _ = mo.loss_and_grad(p2, fourier, rows, cols, offs, x, t, n_out, use_bf16=False)
+            fd = (l1 - l2) / (2 * eps)
+            assert abs(fd - grad[k]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (l, i, j, fd, grad[k])
+    # the normalised weights sum to one whatever the network
+    w = mo.weights(params, fourier, rows, cols, offs, x, n_out, use_bf16=False)
+    assert np.abs(w.sum(1) - 1).max() < 1e-12
