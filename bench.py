@@ -133,6 +133,76 @@ def cpu_baseline(scene, frames, budget_s: float):
                       f"(Stages 2-4), C++ oracle with OpenMP on {cores} threads ({cpu_info()})"}
 
 
+def batched_views(n_views: int, reps: int, lanes_list, splats=None):
+    """Views/s of the view batch (one HIP graph per batch) against per-view
+    cgs_execute_frame calls over the same views: 3 scenes (200k, 1.07M, 4M
+    splats, Gaussian-ball distribution), n_views seeded orbit cameras with
+    resolutions drawn from 320x240 / 800x600 / 1920x1080 / 3840x2160."""
+    from cindergs_amd import FrameParams, GpuPipelineState, StageTimings, ViewBatch
+    from cindergs_amd import synthetic as syn
+
+    splats = splats or syn.BATCH_SCENE_SPLATS
+    t0 = time.perf_counter()
+    scenes = [syn.scene_gaussian_ball(n, k=50 + i) for i, n in enumerate(splats)]
+    states = []
+    for sc in scenes:
+        stt = GpuPipelineState(0)
+        stt.sync_scene_and_cages(sc, None)
+        states.append(stt)
+    view_scene, cams, comps = syn.batch_views(n_views, len(scenes), seed=5)
+    print(f"# batch scenes {[s.count for s in scenes]} ready in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    params = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    tim = StageTimings()
+
+    def per_view_pass():
+        for v in range(n_views):
+            states[view_scene[v]].execute_frame_resident(cams[v], comps[v], params, tim)
+
+    per_view_pass()  # warm-up (arena growth)
+    for stt in states:
+        stt.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        per_view_pass()
+    for stt in states:
+        stt.synchronize()
+    per_view_s = (time.perf_counter() - t) / reps
+    print(f"# per-view: {1000 * per_view_s:.2f} ms per {n_views} views", file=sys.stderr, flush=True)
+
+    sweep = {}
+    best = None
+    for lanes in lanes_list:
+        with ViewBatch(0, lanes=lanes) as batch:
+            batch.execute(states, view_scene, cams, comps, params)  # capture + arena sizing
+            batch.execute(states, view_scene, cams, comps, params)
+            t = time.perf_counter()
+            gpu_ms = 0.0
+            for _ in range(reps):
+                stats = batch.execute(states, view_scene, cams, comps, params)
+                gpu_ms += stats.gpu_ms
+            wall = (time.perf_counter() - t) / reps
+            sweep[lanes] = {"ms_per_batch": round(1000 * wall, 3),
+                            "gpu_ms_per_batch": round(gpu_ms / reps, 3),
+                            "views_per_s": round(n_views / wall, 1),
+                            "rebuilt": int(stats.graph_built)}
+            print(f"# batched lanes={lanes}: {1000 * wall:.2f} ms per {n_views} views",
+                  file=sys.stderr, flush=True)
+            if best is None or wall < best[1]:
+                best = (lanes, wall, int(stats.total_tile_entries))
+    for stt in states:
+        stt.close()
+    res = {"views": n_views, "reps": reps,
+           "scenes": [s.count for s in scenes],
+           "resolutions": sorted({f"{c.screen_width}x{c.screen_height}" for c in comps}),
+           "per_view_views_per_s": round(n_views / per_view_s, 1),
+           "batched_views_per_s": round(n_views / best[1], 1),
+           "speedup": round(per_view_s / best[1], 3),
+           "lanes": best[0], "tile_entries_per_batch": best[2],
+           "lanes_sweep": sweep}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +216,11 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 PMC child runs that measure HBM traffic")
     ap.add_argument("--profile-frames", type=int, default=24)
+    ap.add_argument("--batch-views", type=int, default=64,
+                    help="views of the batched multi-view measurement (0 = skip it)")
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--batch-lanes", type=str, default="1,2,4,8")
+    ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -171,6 +246,12 @@ def main() -> int:
         src.free()
         dst.free()
         st.close()
+        return 0
+
+    if args.batch_only:
+        lanes = [int(x) for x in args.batch_lanes.split(",") if x]
+        print(json.dumps({"batched": batched_views(args.batch_views, args.batch_reps, lanes)}),
+              flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -274,6 +355,9 @@ def main() -> int:
         if tr is not None:
             result["roofline"]["traffic"] = int(tr["bytes"])
             result["roofline"]["traffic_pmc"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+    if args.batch_views > 0:
+        lanes = [int(x) for x in args.batch_lanes.split(",") if x]
+        result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, [(args.warmup + i) % 120 for i in range(8)],
                                               args.cpu_budget)

@@ -6,4 +6,4 @@ package is its host-side mirror of the reference API (src/render/gpu_state.rs).
 """
 from .pipeline import (CameraUniforms, CompositorUniforms, DeviceBuffer, FrameParams,  # noqa: F401
                        GaussianSceneSoa, GpuError, GpuPipelineState, QuantizedGaussianSceneSoa,
-                       StageTimings, TetCage, padded_rgba8_row_u32)
+                       StageTimings, TetCage, ViewBatch, padded_rgba8_row_u32)

@@ -92,6 +92,22 @@ class StageTimings(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchStats(C.Structure):
+    """cgs_batch_stats: one cgs_execute_views_batched call."""
+
+    _fields_ = [
+        ("views", C.c_uint32),
+        ("lanes", C.c_uint32),
+        ("graph_built", C.c_uint32),
+        ("reruns", C.c_uint32),
+        ("total_tile_entries", C.c_uint64),
+        ("gpu_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 assert C.sizeof(CameraUniforms) == 96
 assert C.sizeof(CompositorUniforms) == 48
 
@@ -117,6 +133,13 @@ SIGNATURES = {
     "cgs_readback_rgba8": [_vp, _vp, _u32, _u32],
     "cgs_framebuffer_device_ptr": [_vp, C.POINTER(_vp), C.POINTER(_u32)],
     "cgs_kernel_times": [_vp, C.c_char_p, C.POINTER(C.c_double), _u32, C.POINTER(_u32)],
+    "cgs_batch_create": [_int, _u32, C.POINTER(_vp)],
+    "cgs_batch_destroy": [_vp],
+    "cgs_execute_views_batched": [_vp, _vp, _u32, _vp, _vp, _vp, _u32, C.POINTER(FrameParams),
+                                  C.POINTER(BatchStats)],
+    "cgs_batch_framebuffer": [_vp, _u32, C.POINTER(_vp), C.POINTER(_u32)],
+    "cgs_batch_readback_rgba8": [_vp, _u32, _vp, _u32, _u32],
+    "cgs_batch_view_counts": [_vp, _u32, C.POINTER(_u32), C.POINTER(_u32)],
     "cgs_read_projected": [_vp, _vp, _vp, _u64],
     "cgs_read_stage3": [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_u64)],
     "cgs_set_capture": [_vp, _u32],
@@ -169,7 +192,7 @@ def load() -> C.CDLL:
         fn.argtypes = args
         if name == "cgs_last_error":
             fn.restype = C.c_char_p
-        elif name in ("cgs_destroy", "cgs_mlp_destroy"):
+        elif name in ("cgs_destroy", "cgs_mlp_destroy", "cgs_batch_destroy"):
             fn.restype = None
         elif name == "cgs_mlp_param_count":
             fn.restype = _u64

@@ -249,6 +249,8 @@ struct CgsProf {
 
 extern "C++" {
 void cgs_launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+// dst[0, n) = src[0, n) with one small block (a frame's control words)
+void cgs_launch_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
 // per-splat records in[i*k_in + c] -> blocked out[(i, chan0 + c) of k_out channels]
 void cgs_launch_to_blocked_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t k_in,
                                uint32_t k_out, uint32_t chan0, hipStream_t s);

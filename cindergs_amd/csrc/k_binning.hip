@@ -21,9 +21,9 @@
 
 namespace {
 
-constexpr int BT = 256;  // threads per block
-constexpr int BS = 4;    // consecutive splats per thread
-constexpr int BP = BT * BS;
+constexpr int BT = 1024;      // threads per block (16 wave64s), one splat per thread
+constexpr int BP = BT;        // splats per block
+constexpr uint32_t SMALL_BOX = 8;  // boxes of at most this many tiles: walked by their thread
 
 struct TileBox {
   uint32_t tx0, tx1, ty0, ty1;
@@ -42,89 +42,218 @@ __device__ __forceinline__ TileBox tile_box(float mu_x, float mu_y, uint32_t pac
   return b;
 }
 
-struct Splat2D {
-  float mu_x, mu_y, a, b, c, alpha, tz, qcut;
-};
-
-// ellipse_intersects_tile_gpu (binning.rs:35-83)
-__device__ __forceinline__ bool hits(const Splat2D& s, uint32_t tx, uint32_t ty, uint32_t W,
-                                     uint32_t H) {
-  if (s.alpha < (1.0f / 255.0f) || W == 0u || H == 0u) return false;
+// ellipse_intersects_tile_gpu (binning.rs:35-83); d0 = (mu_x, mu_y, a, b),
+// d1 = (c, alpha, q_cut, -)
+__device__ __forceinline__ bool hits(const float4& d0, const float4& d1, uint32_t tx,
+                                     uint32_t ty, uint32_t W, uint32_t H) {
+  const float mu_x = d0.x, mu_y = d0.y, a = d0.z, b = d0.w, c = d1.x;
+  if (d1.y < (1.0f / 255.0f) || W == 0u || H == 0u) return false;
   const uint32_t xs = tx * 16u, ys = ty * 16u;
   if (xs >= W || ys >= H) return false;
   const uint32_t xe = min(xs + 15u, W - 1u), ye = min(ys + 15u, H - 1u);
-  const float dx0 = ((float)xs + 0.5f) - s.mu_x;
-  const float dx1 = ((float)xe + 0.5f) - s.mu_x;
-  const float dy0 = ((float)ys + 0.5f) - s.mu_y;
-  const float dy1 = ((float)ye + 0.5f) - s.mu_y;
+  const float dx0 = ((float)xs + 0.5f) - mu_x;
+  const float dx1 = ((float)xe + 0.5f) - mu_x;
+  const float dy0 = ((float)ys + 0.5f) - mu_y;
+  const float dy1 = ((float)ye + 0.5f) - mu_y;
   const float x_near = cgs_fclamp(0.0f, dx0, dx1);
-  const float y1 = cgs_fclamp(-s.b * x_near / cgs_fmax(s.c, 1e-12f), dy0, dy1);
-  const float q1 = s.a * x_near * x_near + 2.0f * s.b * x_near * y1 + s.c * y1 * y1;
+  const float y1 = cgs_fclamp(-b * x_near / cgs_fmax(c, 1e-12f), dy0, dy1);
+  const float q1 = a * x_near * x_near + 2.0f * b * x_near * y1 + c * y1 * y1;
   const float y_near = cgs_fclamp(0.0f, dy0, dy1);
-  const float x2 = cgs_fclamp(-s.b * y_near / cgs_fmax(s.a, 1e-12f), dx0, dx1);
-  const float q2 = s.a * x2 * x2 + 2.0f * s.b * x2 * y_near + s.c * y_near * y_near;
-  return cgs_fmin(q1, q2) <= s.qcut + 1e-4f;
+  const float x2 = cgs_fclamp(-b * y_near / cgs_fmax(a, 1e-12f), dx0, dx1);
+  const float q2 = a * x2 * x2 + 2.0f * b * x2 * y_near + c * y_near * y_near;
+  return cgs_fmin(q1, q2) <= d1.z + 1e-4f;
 }
 
-__device__ __forceinline__ Splat2D load_splat(const float* geom, uint32_t i) {
-  const float4* g = reinterpret_cast<const float4*>(geom + (size_t)i * 8);
-  const float4 g0 = g[0], g1 = g[1];
-  Splat2D s;
-  s.mu_x = g0.x;
-  s.mu_y = g0.y;
-  s.a = g0.z;
-  s.b = g0.w;
-  s.c = g1.x;
-  s.alpha = g1.y;
-  s.tz = g1.z;
-  s.qcut = g1.w;
-  return s;
+// 32-bit mode key bits of a splat: the depth digit of the packed key
+// (emit_tile_keys_kernel, binning.rs:168-247); 64-bit mode: raw depth bits
+__device__ __forceinline__ uint32_t splat_key_bits(const CgsBinArgs& A, float tz) {
+  if (!A.mode64) {
+    const uint32_t raw = cgs_f2u(cgs_fmax(tz, 1e-6f));
+    const uint32_t rel = raw > A.base_depth_bits ? min(raw - A.base_depth_bits, 0x0FFFFFFFu) : 0u;
+    return rel >> (28u - A.depth_bits);
+  }
+  return cgs_f2u(cgs_fmax(tz, 0.0f));
 }
 
-// Splats are handled in blocks of BT*BS consecutive splats, thread t owning
-// splats base + t*BS .. + BS-1 (so a thread's entries are contiguous).
-__device__ __forceinline__ uint32_t count_splat(const CgsBinArgs& A, uint32_t s, uint32_t active,
-                                                bool& survived) {
-  survived = false;
-  if (s >= active) return 0u;
+// One splat as the binning sees it (thread-owned)
+struct BinSplat {
+  float4 d0, d1;  // (mu_x, mu_y, a, b), (c, alpha, q_cut, key bits)
+  TileBox box;
+  uint32_t area;  // tiles of the box (0: hits nothing)
+  bool surv;      // extents != 0 and index < active
+};
+
+__device__ __forceinline__ BinSplat load_bin_splat(const CgsBinArgs& A, uint32_t s,
+                                                   uint32_t active) {
+  BinSplat b;
+  b.area = 0;
+  b.surv = false;
+  if (s >= active) return b;
   const uint32_t packed = A.extents[s];
-  if (packed == 0u) return 0u;
-  survived = true;
-  if (A.tiles_x == 0u || A.tiles_y == 0u) return 0u;
-  const Splat2D sp = load_splat(A.geom, s);
-  const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
-  uint32_t cnt = 0;
-  for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-    for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx) cnt += hits(sp, tx, ty, A.width, A.height) ? 1u : 0u;
-  return cnt;
+  if (packed == 0u) return b;
+  b.surv = true;
+  if (A.tiles_x == 0u || A.tiles_y == 0u) return b;
+  const float4* g = reinterpret_cast<const float4*>(A.geom + (size_t)s * 8);
+  const float4 g0 = g[0], g1 = g[1];
+  b.d0 = g0;
+  b.d1 = make_float4(g1.x, g1.y, g1.w, cgs_u2f(splat_key_bits(A, g1.z)));
+  b.box = tile_box(g0.x, g0.y, packed, A.tiles_x, A.tiles_y);
+  // a splat below the opacity cut hits no tile (hits() rejects it first)
+  if (g1.y >= (1.0f / 255.0f))
+    b.area = (b.box.tx1 - b.box.tx0 + 1u) * (b.box.ty1 - b.box.ty0 + 1u);
+  return b;
+}
+
+// A wave's splats with more than SMALL_BOX tiles ("big"), in splat order, and
+// the start of each one's tiles in the wave's flattened (splat, tile) item
+// list (tiles row-major inside the box: the reference's emission order).  The
+// wave walks that list 64 items at a time, so a few large splats do not hold
+// the other lanes of their wave.
+struct WaveBig {
+  uint32_t start[65];
+  float4 d0[64];
+  float4 d1[64];
+  uint4 box[64];     // tx0, ty0, box width, splat index
+  uint32_t acc[64];  // count: hits so far; emit: next output slot
+  uint32_t mark[64];
+};
+
+// stages this lane's splat if it is big (in slot `slot`); returns the wave's
+// item total
+__device__ __forceinline__ uint32_t stage_big(WaveBig& W, const BinSplat& b, uint32_t s,
+                                              uint32_t acc0, uint32_t lane, uint32_t& nv,
+                                              uint32_t& slot) {
+  const bool big = b.area > SMALL_BOX;
+  const uint64_t mb = __ballot(big);
+  nv = (uint32_t)__popcll(mb);
+  const uint32_t a = big ? b.area : 0u;
+  const uint32_t inc = cgs::wave_inclusive_scan(a);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+  slot = cgs::mask_rank(mb);
+  if (big) {
+    const uint32_t p = slot;
+    W.start[p] = inc - a;
+    W.d0[p] = b.d0;
+    W.d1[p] = b.d1;
+    W.box[p] = make_uint4(b.box.tx0, b.box.ty0, b.box.tx1 - b.box.tx0 + 1u, s);
+    W.acc[p] = acc0;
+  }
+  if (lane == 0) W.start[nv] = total;
+  W.mark[lane] = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return total;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One 64-item chunk of the flattened walk starting at item `base`: finds
+// this lane's big splat (slot `sp`) and tile, and whether the tile is hit.
+// `cp` is the first big splat whose items start at or after `base`.  Returns
+// the lanes that start a splat run in the chunk, lane 0 always included.
+__device__ __forceinline__ uint64_t walk_chunk(const CgsBinArgs& A, WaveBig& W, uint32_t base,
+                                               uint32_t total, uint32_t nv, uint32_t lane,
+                                               uint32_t& cp, uint32_t& sp, uint32_t& tile_x,
+                                               uint32_t& tile_y, bool& hit) {
+  // big splats whose first item falls in the chunk mark their first lane
+  const uint32_t j = cp + lane;
+  const uint32_t st = j < nv ? W.start[j] : 0xFFFFFFFFu;
+  const bool head = st < base + 64u;
+  if (head) W.mark[st - base] = 1u;
+  wave_sync_lds();
+  const uint64_t M = __ballot(W.mark[lane] != 0u);
+  W.mark[lane] = 0u;
+  const uint32_t heads = (uint32_t)__popcll(__ballot(head));
+  const uint32_t upto = cgs::mask_rank(M) + (uint32_t)((M >> lane) & 1ull);
+  sp = cp + upto - 1u;  // cp - 1 = the splat continued from the previous chunk
+  cp += heads;
+  const uint32_t item = base + lane;
+  hit = false;
+  tile_x = tile_y = 0u;
+  if (item < total) {
+    const uint32_t k = item - W.start[sp];
+    const uint4 bx = W.box[sp];
+    // k / width through the reciprocal, corrected to the exact quotient
+    uint32_t q = (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)bx.z));
+    int32_t r = (int32_t)k - (int32_t)(q * bx.z);
+    if (r < 0) {
+      q--;
+      r += (int32_t)bx.z;
+    } else if (r >= (int32_t)bx.z) {
+      q++;
+      r -= (int32_t)bx.z;
+    }
+    tile_x = bx.x + (uint32_t)r;
+    tile_y = bx.y + q;
+    hit = hits(W.d0[sp], W.d1[sp], tile_x, tile_y, A.width, A.height);
+  }
+  return M | 1ull;
+}
+
+// lanes [first lane of this lane's run, this lane)
+__device__ __forceinline__ uint64_t run_before(uint64_t R, uint32_t lane) {
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint64_t le = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
+  const uint32_t h = 63u - (uint32_t)__builtin_clzll(R & le);
+  return lt & ~((1ull << h) - 1ull);
+}
+
+// the whole run of a run-head lane
+__device__ __forceinline__ uint64_t run_of_head(uint64_t R, uint32_t lane) {
+  const uint64_t above = lane == 63u ? 0ull : R & (~0ull << (lane + 1u));
+  const uint32_t nxt = above ? (uint32_t)__builtin_ctzll(above) : 64u;
+  const uint64_t lo = nxt == 64u ? ~0ull : ((1ull << nxt) - 1ull);
+  return lo & (~0ull << lane);
 }
 
 // count_tiles_kernel (binning.rs:90-164): per-splat tile counts + per-block sums
 __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
+  __shared__ WaveBig s_w[BT / 64];
   __shared__ uint32_t s_tmp[BT / 64];
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  WaveBig& W = s_w[wave];
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
-  const uint32_t s0 = blockIdx.x * (uint32_t)BP + t * (uint32_t)BS;
-  uint32_t sum = 0, surv = 0;
-#pragma unroll
-  for (int j = 0; j < BS; ++j) {
-    const uint32_t s = s0 + (uint32_t)j;
-    bool sv;
-    const uint32_t c = count_splat(A, s, active, sv);
-    if (s < A.max_gaussians) A.counts[s] = c;
-    sum += c;
-    surv += sv ? 1u : 0u;
+  const uint32_t s = blockIdx.x * (uint32_t)BP + t;
+  const BinSplat b = load_bin_splat(A, s, active);
+  // small boxes: the thread walks them
+  uint32_t cnt = 0;
+  if (b.area != 0u && b.area <= SMALL_BOX)
+    for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
+      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx)
+        cnt += hits(b.d0, b.d1, tx, ty, A.width, A.height) ? 1u : 0u;
+  // big boxes: the wave walks them together
+  uint32_t nv, slot;
+  const uint32_t total = stage_big(W, b, s, 0u, lane, nv, slot);
+  uint32_t cp = 0;
+  for (uint32_t base = 0; base < total; base += 64u) {
+    uint32_t sp, tx, ty;
+    bool hit;
+    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
+    const uint64_t Hm = __ballot(hit);
+    if (((R >> lane) & 1ull) && base + lane < total)
+      W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
+    wave_sync_lds();
   }
+  if (b.area > SMALL_BOX) cnt = W.acc[slot];
+  if (s < A.max_gaussians) A.counts[s] = cnt;
   uint32_t tot, stot;
-  (void)cgs::block_exclusive_scan<BT>(sum, s_tmp, tot);
-  (void)cgs::block_exclusive_scan<BT>(surv, s_tmp, stot);
+  (void)cgs::block_exclusive_scan<BT>(cnt, s_tmp, tot);
+  (void)cgs::block_exclusive_scan<BT>(b.surv ? 1u : 0u, s_tmp, stot);
   if (t == 0) {
     A.block_sums[blockIdx.x] = tot;
     A.block_surv[blockIdx.x] = stot;
   }
 }
 
-// digit histograms of one emitted entry for every radix pass that will sort it
+// digit histograms of the emitted entries.  Entries of the thread walks: one
+// LDS atomic per digit.  Entries of the wave walk (consecutive entries of a few
+// splats, sharing their top digit): the top digit (tile bits, or the exponent
+// of a raw depth) is summed over the wave first, one atomic per distinct digit.
 __device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift) {
   const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
   return (1u << w) - 1u;
@@ -133,98 +262,135 @@ __device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_
   for (uint32_t p = 0; p * 8u < bits; ++p)
     atomicAdd(&s_hist[p * 256u + ((key >> (p * 8u)) & dmask(bits, p * 8u))], 1u);
 }
+__device__ __forceinline__ void hist_wave(uint32_t* s_hist, uint32_t key, uint32_t bits,
+                                          bool store, uint32_t lane) {
+  const uint32_t passes = (bits + 7u) / 8u;
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t d = (key >> (p * 8u)) & dmask(bits, p * 8u);
+    if (p + 1u < passes) {
+      if (store) atomicAdd(&s_hist[p * 256u + d], 1u);
+    } else {
+      uint64_t m = __ballot(store);
+      while (m) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)leader);
+        const uint64_t same = __ballot(store && d == d0);
+        if (lane == leader) atomicAdd(&s_hist[p * 256u + d0], (uint32_t)__popcll(same));
+        m &= ~same;
+      }
+    }
+  }
+}
+
+// one entry: 32-bit packed key (tile << depth_bits | depth digit), or the
+// (tile, raw depth) pair of the 64-bit mode
+__device__ __forceinline__ void store_entry(const CgsBinArgs& A, uint32_t pos, uint32_t tile,
+                                            uint32_t kb, uint32_t id) {
+  if (!A.mode64) {
+    A.keys[pos] = (tile << A.depth_bits) | kb;
+  } else {
+    A.keys[pos] = tile;
+    A.depths[pos] = kb;
+  }
+  A.ids[pos] = id;
+}
 
 // gpu_exclusive_scan_total + emit_tile_keys(_u64)_kernel (scan.rs:198-333,
 // binning.rs:168-328).  The prefix of block b is the sum of the block sums
 // before it, reduced here by the block itself (no look-back chain, no ticket).
+// Only stored entries are histogrammed: the sort reads min(total, cap)
+// entries and its digit offsets must add up to that.
 __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
+  __shared__ WaveBig s_w[BT / 64];
   __shared__ uint32_t s_tmp[BT / 64];
   __shared__ uint32_t s_hist[2048];
-  const uint32_t t = threadIdx.x;
-  const uint32_t b = blockIdx.x;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t blk = blockIdx.x;
+  WaveBig& W = s_w[wave];
   if (A.hist_partial)
     for (uint32_t i = t; i < A.hist_words; i += BT) s_hist[i] = 0u;
   // prefix of this block and, in the last block, the totals
   uint32_t pre = 0, all_surv = 0;
-  const bool last = b == gridDim.x - 1;
-  for (uint32_t j = t; j < b; j += BT) pre += A.block_sums[j];
+  const bool last = blk == gridDim.x - 1;
+  for (uint32_t j = t; j < blk; j += BT) pre += A.block_sums[j];
   if (last)
     for (uint32_t j = t; j < gridDim.x; j += BT) all_surv += A.block_surv[j];
   uint32_t pre_tot, surv_tot;
   (void)cgs::block_exclusive_scan<BT>(pre, s_tmp, pre_tot);
   (void)cgs::block_exclusive_scan<BT>(all_surv, s_tmp, surv_tot);
-  // thread-blocked offsets inside the block
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
-  const uint32_t s0 = b * (uint32_t)BP + t * (uint32_t)BS;
-  uint32_t cnt[BS];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int j = 0; j < BS; ++j) {
-    const uint32_t s = s0 + (uint32_t)j;
-    cnt[j] = s < A.max_gaussians ? A.counts[s] : 0u;
-    sum += cnt[j];
-  }
+  const uint32_t s = blk * (uint32_t)BP + t;
+  const uint32_t c = s < A.max_gaussians ? A.counts[s] : 0u;
   uint32_t btot;
-  uint32_t off = pre_tot + cgs::block_exclusive_scan<BT>(sum, s_tmp, btot);
+  const uint32_t off = pre_tot + cgs::block_exclusive_scan<BT>(c, s_tmp, btot);
   if (last && t == 0) {
     A.total[0] = pre_tot + btot;
     A.total[1] = surv_tot;
   }
-  const bool tiles_ok = A.tiles_x != 0u && A.tiles_y != 0u;
-#pragma unroll 1
-  for (int j = 0; j < BS; ++j) {
-    const uint32_t s = s0 + (uint32_t)j;
-    if (s >= A.max_gaussians) break;
-    if (A.offsets_out) A.offsets_out[s] = off;
-    const uint32_t c = cnt[j];
-    if (c != 0u && s < active && tiles_ok) {
-      const uint32_t packed = A.extents[s];
-      const Splat2D sp = load_splat(A.geom, s);
-      const TileBox bx = tile_box(sp.mu_x, sp.mu_y, packed, A.tiles_x, A.tiles_y);
-      uint32_t cur = off;
+  if (A.offsets_out && s < A.max_gaussians) A.offsets_out[s] = off;
+  const bool hist = A.hist_partial != nullptr;
+  const uint32_t hbits = A.mode64 ? 32u : A.sort_bits;
+  BinSplat b;
+  b.area = 0;
+  if (c != 0u) b = load_bin_splat(A, s, active);
+  // small boxes: the thread walks them
+  if (b.area != 0u && b.area <= SMALL_BOX) {
+    uint32_t cur = off;
+    const uint32_t kb = cgs_f2u(b.d1.w);
+    for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
+      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx)
+        if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+          const uint32_t tile = ty * A.tiles_x + tx;
+          if (cur < A.cap) {
+            store_entry(A, cur, tile, kb, s);
+            if (hist) {
+              if (!A.mode64) {
+                hist_key(s_hist, (tile << A.depth_bits) | kb, hbits);
+              } else {
+                hist_key(s_hist, kb, 32u);
+                hist_key(s_hist + 1024, tile, A.tile_bits);
+              }
+            }
+          }
+          ++cur;
+        }
+  }
+  // big boxes: the wave walks them together; a hit goes to its splat's next
+  // slot plus the hits of the same splat on the lanes before it
+  uint32_t nv, slot;
+  const uint32_t total = stage_big(W, b, s, off, lane, nv, slot);
+  uint32_t cp = 0;
+  for (uint32_t base = 0; base < total; base += 64u) {
+    uint32_t sp, tx, ty;
+    bool hit;
+    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
+    const uint64_t Hm = __ballot(hit);
+    uint32_t pos = 0, id = 0, kb = 0;
+    if (hit) {
+      pos = W.acc[sp] + (uint32_t)__popcll(Hm & run_before(R, lane));
+      id = W.box[sp].w;
+      kb = cgs_f2u(W.d1[sp].w);
+    }
+    wave_sync_lds();
+    if (((R >> lane) & 1ull) && base + lane < total)
+      W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
+    const bool store = hit && pos < A.cap;
+    const uint32_t tile = ty * A.tiles_x + tx;
+    if (store) store_entry(A, pos, tile, kb, id);
+    if (hist) {
       if (!A.mode64) {
-        const uint32_t raw = cgs_f2u(cgs_fmax(sp.tz, 1e-6f));
-        const uint32_t rel =
-            raw > A.base_depth_bits ? min(raw - A.base_depth_bits, 0x0FFFFFFFu) : 0u;
-        const uint32_t dq = rel >> (28u - A.depth_bits);
-        for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-          for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
-            if (hits(sp, tx, ty, A.width, A.height)) {
-              const uint32_t key = ((ty * A.tiles_x + tx) << A.depth_bits) | dq;
-              // only stored entries are histogrammed: the sort reads
-              // min(total, cap) entries and its digit offsets must add up to that
-              if (cur < A.cap) {
-                A.keys[cur] = key;
-                A.ids[cur] = s;
-                if (A.hist_partial) hist_key(s_hist, key, A.sort_bits);
-              }
-              ++cur;
-            }
+        hist_wave(s_hist, (tile << A.depth_bits) | kb, hbits, store, lane);
       } else {
-        const uint32_t dbits = cgs_f2u(cgs_fmax(sp.tz, 0.0f));
-        for (uint32_t ty = bx.ty0; ty <= bx.ty1; ++ty)
-          for (uint32_t tx = bx.tx0; tx <= bx.tx1; ++tx)
-            if (hits(sp, tx, ty, A.width, A.height)) {
-              const uint32_t tile = ty * A.tiles_x + tx;
-              if (cur < A.cap) {
-                A.keys[cur] = tile;
-                A.depths[cur] = dbits;
-                A.ids[cur] = s;
-                if (A.hist_partial) {
-                  hist_key(s_hist, dbits, 32u);
-                  hist_key(s_hist + 1024, tile, A.tile_bits);
-                }
-              }
-              ++cur;
-            }
+        hist_wave(s_hist, kb, 32u, store, lane);
+        hist_wave(s_hist + 1024, tile, A.tile_bits, store, lane);
       }
     }
-    off += c;
+    wave_sync_lds();
   }
-  if (A.hist_partial) {
+  if (hist) {
     __syncthreads();
     for (uint32_t i = t; i < A.hist_words; i += BT)
-      A.hist_partial[(size_t)b * A.hist_words + i] = s_hist[i];
+      A.hist_partial[(size_t)blk * A.hist_words + i] = s_hist[i];
   }
 }
 

@@ -40,6 +40,11 @@ __global__ __launch_bounds__(256) void k_sh_f16_blocked(const uint16_t* in, uint
   }
 }
 
+// dst[i] = src[i], i < n (n small: a frame's control words)
+__global__ __launch_bounds__(64) void k_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += 64u) dst[i] = src[i];
+}
+
 uint32_t grid_for(uint64_t total) {
   uint64_t b = (total + 255u) / 256u;
   if (b > 8192u) b = 8192u;
@@ -68,4 +73,9 @@ void cgs_launch_sh_f16_blocked(const uint16_t* in, uint16_t* out, uint32_t n, ui
   if (n == 0) return;
   hipLaunchKernelGGL(k_sh_f16_blocked, dim3(grid_for((uint64_t)n * 48u)), dim3(256), 0, s, in,
                      out, n, stride, rest);
+}
+
+void cgs_launch_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_copy_words, dim3(1), dim3(64), 0, s, src, dst, n);
 }

@@ -30,6 +30,9 @@
 namespace {
 
 thread_local std::string g_err;
+// set while a view batch is being captured into a HIP graph: buffers are sized
+// before the capture, so a growth inside it is an internal error
+thread_local bool g_no_alloc = false;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -52,6 +55,7 @@ struct DevBuf {
 int ensure_words(DevBuf& b, size_t words) {
   size_t req = words < 1 ? 1 : words;
   if (b.p && b.cap_words >= req) return CGS_OK;
+  if (g_no_alloc) return fail(CGS_ERR_EXECUTION, "internal: buffer growth during graph capture");
   size_t cap;
   if (!b.p) {
     cap = req < 256 ? 256 : req;
@@ -77,6 +81,10 @@ void release(DevBuf& b) {
 
 template <class T>
 T* P(DevBuf& b) {
+  return static_cast<T*>(b.p);
+}
+template <typename T>
+T* P(const DevBuf& b) {
   return static_cast<T*>(b.p);
 }
 
@@ -117,6 +125,33 @@ uint32_t fbits(float f) {
 
 }  // namespace
 
+// Per-view frame buffers (GpuFrameArena, gpu_state.rs:384-429): the projected
+// splats, the tile entries and their sort buffers, the tile ranges, the frame
+// control words and the framebuffer.  A context owns one; a view batch owns one
+// per concurrent lane.
+struct FrameArena {
+  DevBuf geom, color, extents;
+  DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
+  uint32_t entry_cap = 0;
+  DevBuf ranges, out_rgba, cnt_dbg, off_dbg, comp_order;
+  DevBuf ctrl3;  // frame control, see frame_layout()
+  uint32_t tiles_x = 0, tiles_y = 0, padded_row = 0;
+  bool mode64 = false;
+  uint32_t depth_bits = 0;
+  uint32_t* sorted_keys = nullptr;
+  uint32_t* sorted_ids = nullptr;
+  bool capture = false;
+  bool captured = false;
+
+  void release_all() {
+    DevBuf* bufs[] = {&geom, &color, &extents, &e_keys, &e_ids, &e_depths, &e_alt_k,
+                      &e_alt_v, &e_perm, &ranges, &out_rgba, &cnt_dbg, &off_dbg, &comp_order,
+                      &ctrl3};
+    for (DevBuf* b : bufs) release(*b);
+    entry_cap = 0;
+  }
+};
+
 struct cgs_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -145,11 +180,7 @@ struct cgs_context {
   DevBuf jac, rot, stretch, def_geo, split_meta;
   uint32_t def_cap = 0;  // splat capacity of def_geo
   DevBuf ctrl1;  // [split_count, ticket, pad, pad][u64 status...]
-  DevBuf geom, color, extents;
-  DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
-  uint32_t entry_cap = 0;
-  DevBuf ranges, out_rgba, cnt_dbg, off_dbg, comp_order;
-  DevBuf ctrl3;  // frame control, see frame_layout()
+  FrameArena fa;
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
 
@@ -157,14 +188,7 @@ struct cgs_context {
   uint32_t last_w = 0, last_h = 0;
   uint32_t max_g = 0;          // splat slots of the projected arrays
   uint32_t active = 0;         // active splats of the last frame
-  uint32_t tiles_x = 0, tiles_y = 0, padded_row = 0;
-  bool mode64 = false;
-  uint32_t depth_bits = 0;
   uint32_t n_entries = 0;
-  uint32_t* sorted_keys = nullptr;
-  uint32_t* sorted_ids = nullptr;
-  bool capture = false;
-  bool captured = false;
   bool s3_valid = false;
   Stage3Key s3_key{};
   uint32_t s3_survivors = 0;
@@ -281,56 +305,56 @@ FrameGeom frame_geom(const cgs_compositor_uniforms* comp) {
   return g;
 }
 
-void init_entry_cap(cgs_context* c, uint32_t max_g) {
-  if (c->entry_cap != 0) return;
+void init_entry_cap(FrameArena& a, uint32_t max_g) {
+  if (a.entry_cap != 0) return;
   uint64_t guess = (uint64_t)max_g * 4u;
   if (guess < (1u << 20)) guess = 1u << 20;
   if (guess > (1u << 29)) guess = 1u << 29;
-  c->entry_cap = (uint32_t)guess;
+  a.entry_cap = (uint32_t)guess;
 }
 
 // Stage 3 on the projected splats in the arena (geom/extents, max_g slots,
 // active = num_rest + *split_count).  Launch-only; the caller checks the
 // total against the capacity after synchronising.
-int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t num_rest,
-                  const uint32_t* split_count, uint32_t max_g, bool keep_survivors) {
-  c->tiles_x = g.tiles_x;
-  c->tiles_y = g.tiles_y;
-  c->mode64 = g.T > 65535u;
-  init_entry_cap(c, max_g);
-  const uint32_t cap = c->entry_cap;
-  TRY(ensure_words(c->e_keys, cap));
-  TRY(ensure_words(c->e_ids, cap));
-  TRY(ensure_words(c->e_alt_k, cap));
-  TRY(ensure_words(c->e_alt_v, cap));
-  if (c->mode64) {
-    TRY(ensure_words(c->e_depths, cap));
-    TRY(ensure_words(c->e_perm, cap));
+int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& g,
+                  float near_plane, uint32_t num_rest, const uint32_t* split_count,
+                  uint32_t max_g) {
+  a.tiles_x = g.tiles_x;
+  a.tiles_y = g.tiles_y;
+  a.mode64 = g.T > 65535u;
+  init_entry_cap(a, max_g);
+  const uint32_t cap = a.entry_cap;
+  TRY(ensure_words(a.e_keys, cap));
+  TRY(ensure_words(a.e_ids, cap));
+  TRY(ensure_words(a.e_alt_k, cap));
+  TRY(ensure_words(a.e_alt_v, cap));
+  if (a.mode64) {
+    TRY(ensure_words(a.e_depths, cap));
+    TRY(ensure_words(a.e_perm, cap));
   }
-  TRY(ensure_words(c->ranges, (size_t)g.T * 2));
+  TRY(ensure_words(a.ranges, (size_t)g.T * 2));
   const FrameLayout L = frame_layout(max_g, cap);
-  TRY(ensure_words(c->ctrl3, L.words));
-  uint32_t* ctrl = P<uint32_t>(c->ctrl3);
-  (void)keep_survivors;
-  cgs_launch_fill_u32(ctrl, 0u, L.zero_words, c->stream);
-  cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, c->stream);
-  TRY(ensure_words(c->cnt_dbg, max_g));  // per-splat tile counts (always needed)
-  c->captured = false;
-  if (c->capture) {
-    TRY(ensure_words(c->off_dbg, max_g));
-    c->captured = true;
+  TRY(ensure_words(a.ctrl3, L.words));
+  uint32_t* ctrl = P<uint32_t>(a.ctrl3);
+  cgs_launch_fill_u32(ctrl, 0u, L.zero_words, s);
+  cgs_launch_fill_u32(P<uint32_t>(a.ranges), 0u, (uint64_t)g.T * 2, s);
+  TRY(ensure_words(a.cnt_dbg, max_g));  // per-splat tile counts (always needed)
+  a.captured = false;
+  if (a.capture) {
+    TRY(ensure_words(a.off_dbg, max_g));
+    a.captured = true;
   }
 
   uint32_t tile_bits, depth_bits;
   tile_bits = g.T <= 1 ? 1u : 32u - (uint32_t)__builtin_clz(g.T - 1u);
   depth_bits = 32u - tile_bits;
   if (depth_bits > 28u) depth_bits = 28u;
-  c->depth_bits = depth_bits;
+  a.depth_bits = depth_bits;
 
   CgsBinArgs B;
   memset(&B, 0, sizeof(B));
-  B.geom = P<float>(c->geom);
-  B.extents = P<uint32_t>(c->extents);
+  B.geom = P<float>(a.geom);
+  B.extents = P<uint32_t>(a.extents);
   B.split_count = split_count;
   B.num_rest = num_rest;
   B.max_gaussians = max_g;
@@ -338,110 +362,116 @@ int launch_stage3(cgs_context* c, const FrameGeom& g, float near_plane, uint32_t
   B.height = g.H;
   B.tiles_x = g.tiles_x;
   B.tiles_y = g.tiles_y;
-  B.mode64 = c->mode64 ? 1u : 0u;
+  B.mode64 = a.mode64 ? 1u : 0u;
   float np = near_plane < 1e-6f ? 1e-6f : near_plane;
   B.base_depth_bits = fbits(np) & 0x7F800000u;
   B.depth_bits = depth_bits;
   B.cap = cap;
-  B.keys = P<uint32_t>(c->e_keys);
-  B.depths = c->mode64 ? P<uint32_t>(c->e_depths) : nullptr;
-  B.ids = P<uint32_t>(c->e_ids);
-  B.counts = P<uint32_t>(c->cnt_dbg);
-  B.offsets_out = c->capture ? P<uint32_t>(c->off_dbg) : nullptr;
+  B.keys = P<uint32_t>(a.e_keys);
+  B.depths = a.mode64 ? P<uint32_t>(a.e_depths) : nullptr;
+  B.ids = P<uint32_t>(a.e_ids);
+  B.counts = P<uint32_t>(a.cnt_dbg);
+  B.offsets_out = a.capture ? P<uint32_t>(a.off_dbg) : nullptr;
   B.total = ctrl + L.total;
   B.block_sums = ctrl + L.block_sums;
   B.block_surv = ctrl + L.block_surv;
   B.hist_partial = ctrl + L.bin_hist;
-  B.hist_words = c->mode64 ? 2048u : 1024u;
+  B.hist_words = a.mode64 ? 2048u : 1024u;
   uint32_t sort_bits = tile_bits + depth_bits;
   if (sort_bits > 32) sort_bits = 32;
   B.sort_bits = sort_bits;
   B.tile_bits = tile_bits;
-  cgs_launch_bin(B, c->stream, &c->prof);
+  cgs_launch_bin(B, s, prof);
   // digit histograms of every pass, summed over the binning partitions
   const uint32_t hist_bins =
-      c->mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u) : 256u * ((sort_bits + 7u) / 8u);
-  c->prof.mark("k_hist_reduce", c->stream);
+      a.mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u) : 256u * ((sort_bits + 7u) / 8u);
+  prof->mark("k_hist_reduce", s);
   cgs_launch_hist_reduce(ctrl + L.bin_hist, L.bin_parts, B.hist_words, hist_bins, ctrl + L.histA,
-                         c->stream);
+                         s);
 
   const uint32_t* d_n = ctrl + L.total;
-  if (!c->mode64) {
+  if (!a.mode64) {
     CgsRadixBuffers R;
-    R.keys = P<uint32_t>(c->e_keys);
-    R.vals = P<uint32_t>(c->e_ids);
-    R.keys_alt = P<uint32_t>(c->e_alt_k);
-    R.vals_alt = P<uint32_t>(c->e_alt_v);
+    R.keys = P<uint32_t>(a.e_keys);
+    R.vals = P<uint32_t>(a.e_ids);
+    R.keys_alt = P<uint32_t>(a.e_alt_k);
+    R.vals_alt = P<uint32_t>(a.e_alt_v);
     R.hist = ctrl + L.histA;
     R.hist_partial = nullptr;
     R.status = ctrl + L.rA_status;
     R.tickets = ctrl + L.rA_tickets;
     R.cap = cap;
-    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, c->stream, &c->prof);
-    c->sorted_keys = alt ? R.keys_alt : R.keys;
-    c->sorted_ids = alt ? R.vals_alt : R.vals;
-    c->prof.mark("k_tile_ranges", c->stream);
-    cgs_launch_tile_ranges(c->sorted_keys, d_n, cap, depth_bits, g.T, P<uint32_t>(c->ranges),
-                           c->stream);
+    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, s, prof);
+    a.sorted_keys = alt ? R.keys_alt : R.keys;
+    a.sorted_ids = alt ? R.vals_alt : R.vals;
+    prof->mark("k_tile_ranges", s);
+    cgs_launch_tile_ranges(a.sorted_keys, d_n, cap, depth_bits, g.T, P<uint32_t>(a.ranges),
+                           s);
   } else {
     // 64-bit fallback (gpu_radix_sort_u64_pairs, radix_sort.rs:377-477): stable
     // sort by depth with the entry index as payload, gather, stable sort by tile
     CgsRadixBuffers A;
-    A.keys = P<uint32_t>(c->e_depths);
-    A.vals = P<uint32_t>(c->e_perm);
-    A.keys_alt = P<uint32_t>(c->e_alt_k);
-    A.vals_alt = P<uint32_t>(c->e_alt_v);
+    A.keys = P<uint32_t>(a.e_depths);
+    A.vals = P<uint32_t>(a.e_perm);
+    A.keys_alt = P<uint32_t>(a.e_alt_k);
+    A.vals_alt = P<uint32_t>(a.e_alt_v);
     A.hist = ctrl + L.histA;
     A.hist_partial = nullptr;
     A.status = ctrl + L.rA_status;
     A.tickets = ctrl + L.rA_tickets;
     A.cap = cap;
-    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, true, c->stream, &c->prof);
+    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, true, s, prof);
     const uint32_t* perm = altA ? A.vals_alt : A.vals;
-    uint32_t* gk = altA ? P<uint32_t>(c->e_depths) : P<uint32_t>(c->e_alt_k);
-    uint32_t* gv = altA ? P<uint32_t>(c->e_perm) : P<uint32_t>(c->e_alt_v);
+    uint32_t* gk = altA ? P<uint32_t>(a.e_depths) : P<uint32_t>(a.e_alt_k);
+    uint32_t* gv = altA ? P<uint32_t>(a.e_perm) : P<uint32_t>(a.e_alt_v);
     // with 4 passes altA == 0: perm in e_perm, gather into the alt pair
-    c->prof.mark("k_gather2", c->stream);
-    cgs_launch_gather2(perm, P<uint32_t>(c->e_keys), P<uint32_t>(c->e_ids), gk, gv, d_n, cap,
-                       c->stream);
+    prof->mark("k_gather2", s);
+    cgs_launch_gather2(perm, P<uint32_t>(a.e_keys), P<uint32_t>(a.e_ids), gk, gv, d_n, cap,
+                       s);
     CgsRadixBuffers Bt;
     Bt.keys = gk;
     Bt.vals = gv;
-    Bt.keys_alt = P<uint32_t>(c->e_keys);
-    Bt.vals_alt = P<uint32_t>(c->e_ids);
+    Bt.keys_alt = P<uint32_t>(a.e_keys);
+    Bt.vals_alt = P<uint32_t>(a.e_ids);
     Bt.hist = ctrl + L.histB;
     Bt.hist_partial = nullptr;
     Bt.status = ctrl + L.rB_status;
     Bt.tickets = ctrl + L.rB_tickets;
     Bt.cap = cap;
-    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, c->stream, &c->prof);
-    c->sorted_keys = altB ? Bt.keys_alt : Bt.keys;
-    c->sorted_ids = altB ? Bt.vals_alt : Bt.vals;
-    c->prof.mark("k_tile_ranges", c->stream);
-    cgs_launch_tile_ranges(c->sorted_keys, d_n, cap, 0, g.T, P<uint32_t>(c->ranges), c->stream);
+    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, s, prof);
+    a.sorted_keys = altB ? Bt.keys_alt : Bt.keys;
+    a.sorted_ids = altB ? Bt.vals_alt : Bt.vals;
+    prof->mark("k_tile_ranges", s);
+    cgs_launch_tile_ranges(a.sorted_keys, d_n, cap, 0, g.T, P<uint32_t>(a.ranges), s);
   }
   HIPCHK(hipGetLastError());
   return CGS_OK;
 }
 
-int launch_stage4(cgs_context* c, const FrameGeom& g, const cgs_compositor_uniforms* comp) {
-  TRY(ensure_words(c->out_rgba, (size_t)g.padded_row * g.H));
-  c->padded_row = g.padded_row;
+// Stage 4 into out (padded rows of g.padded_row words), or into the arena's
+// framebuffer when out is null
+int launch_stage4(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& g,
+                  const cgs_compositor_uniforms* comp, uint32_t* out = nullptr) {
+  if (!out) {
+    TRY(ensure_words(a.out_rgba, (size_t)g.padded_row * g.H));
+    out = P<uint32_t>(a.out_rgba);
+  }
+  a.padded_row = g.padded_row;
   CgsCompositeArgs A;
-  A.geom = P<float>(c->geom);
-  A.color = P<float>(c->color);
-  A.sorted_ids = c->sorted_ids ? c->sorted_ids : P<uint32_t>(c->e_ids);
-  A.ranges = P<uint32_t>(c->ranges);
+  A.geom = P<float>(a.geom);
+  A.color = P<float>(a.color);
+  A.sorted_ids = a.sorted_ids ? a.sorted_ids : P<uint32_t>(a.e_ids);
+  A.ranges = P<uint32_t>(a.ranges);
   A.width = g.W;
   A.height = g.H;
   A.tiles_x = g.tiles_x;
   A.tiles_y = g.tiles_y;
   A.padded_row = g.padded_row;
   for (int i = 0; i < 4; ++i) A.bg[i] = comp->background_color[i];
-  A.out = P<uint32_t>(c->out_rgba);
-  TRY(ensure_words(c->comp_order, g.T));
-  A.order = P<uint32_t>(c->comp_order);
-  cgs_launch_composite(A, c->stream, &c->prof);
+  A.out = out;
+  TRY(ensure_words(a.comp_order, g.T));
+  A.order = P<uint32_t>(a.comp_order);
+  cgs_launch_composite(A, s, prof);
   HIPCHK(hipGetLastError());
   return CGS_OK;
 }
@@ -449,7 +479,7 @@ int launch_stage4(cgs_context* c, const FrameGeom& g, const cgs_compositor_unifo
 // read the entry total, the survivors and the split count (one small copy + wait):
 // h_counters[2] = entries, [3] = survivors, [4] = split children
 int read_counters(cgs_context* c, bool with_split) {
-  uint32_t* ctrl = P<uint32_t>(c->ctrl3);
+  const uint32_t* ctrl = P<uint32_t>(c->fa.ctrl3);
   HIPCHK(hipMemcpyAsync(c->h_counters, ctrl, 4 * 4, hipMemcpyDeviceToHost, c->stream));
   if (with_split && c->ctrl1.p)
     HIPCHK(hipMemcpyAsync(c->h_counters + 4, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -463,6 +493,169 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
   return ms;
+}
+
+
+// ---------------------------------------------------------------------------
+// frame pieces shared by the single-frame path and the view batch
+// ---------------------------------------------------------------------------
+
+// the scene side of one frame after Stage 1: what Stages 2-3 read
+struct SceneFrame {
+  CgsSceneDev S;
+  const uint32_t* split_count = nullptr;  // device word: split children, or null
+  uint32_t N = 0, max_g = 0;
+  bool split_possible = false;
+};
+
+// Stage 1 (gpu_state.rs:706-713, 852-894): the scene is re-deformed when the
+// cage vertices or (tau, mip) changed since the last deformation; fills the
+// scene pointers Stage 2 reads.  deformed = Stage 1 ran.
+int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* prof,
+           SceneFrame& F, bool& deformed) {
+  deformed = false;
+  const uint32_t N = c->n_rest;
+  F.N = N;
+  F.split_possible = c->has_cages && prm->strain_split_threshold > 0.0f;
+  F.max_g = F.split_possible ? 2 * N : N;
+  memset(&F.S, 0, sizeof(F.S));
+  F.split_count = nullptr;
+  if (c->has_cages) {
+    if (c->f16) return fail(CGS_ERR_UNSUPPORTED, "cage deformation of the f16 storage variant");
+    const uint32_t p0 = fbits(prm->strain_split_threshold), p1 = fbits(prm->mip_filter_3d_var);
+    if (p0 != c->last_params[0] || p1 != c->last_params[1]) {
+      c->last_params[0] = p0;
+      c->last_params[1] = p1;
+      c->deformed_valid = false;
+    }
+    TRY(ensure_words(c->jac, (size_t)c->num_tets * 9));
+    TRY(ensure_words(c->rot, (size_t)c->num_tets * 9));
+    TRY(ensure_words(c->stretch, c->num_tets));
+    if (c->def_cap < F.max_g) {
+      c->def_cap = F.max_g;
+      c->deformed_valid = false;
+    }
+    TRY(ensure_words(c->def_geo, cgs_blk_words(c->def_cap, CGS_GEO_K)));
+    TRY(ensure_words(c->split_meta, (size_t)2 * N));
+    const uint32_t dparts = cgs_deform_num_parts(N);
+    TRY(ensure_words(c->ctrl1, 4 + 2 * (size_t)(dparts ? dparts : 1)));
+    if (!c->deformed_valid) {
+      cgs_launch_fill_u32(P<uint32_t>(c->ctrl1), 0u, 4 + 2 * (size_t)(dparts ? dparts : 1), s);
+      CgsTetArgs TA;
+      TA.verts = P<float>(c->verts);
+      TA.elems = P<uint32_t>(c->elems);
+      TA.inv_dm = P<float>(c->inv_dm);
+      TA.jac = P<float>(c->jac);
+      TA.rot = P<float>(c->rot);
+      TA.stretch = P<float>(c->stretch);
+      TA.num_tets = c->num_tets;
+      prof->mark("k_tet_transforms", s);
+      cgs_launch_tet_transforms(TA, s);
+      CgsDeformArgs D;
+      D.verts = P<float>(c->verts);
+      D.elems = P<uint32_t>(c->elems);
+      D.jac = P<float>(c->jac);
+      D.stretch = P<float>(c->stretch);
+      D.bind_tet = P<uint32_t>(c->bind_tet);
+      D.bind_w = P<float>(c->bind_w);
+      D.rest_geo = P<float>(c->rest_geo);
+      D.out_geo = P<float>(c->def_geo);
+      D.split_meta = P<uint32_t>(c->split_meta);
+      D.split_count = P<uint32_t>(c->ctrl1);
+      D.status = reinterpret_cast<uint64_t*>(P<uint32_t>(c->ctrl1) + 4);
+      D.ticket = P<uint32_t>(c->ctrl1) + 1;
+      D.split_threshold = F.split_possible ? prm->strain_split_threshold : 0.0f;
+      D.mip = prm->mip_filter_3d_var;
+      D.num_gaussians = N;
+      D.num_tets = c->num_tets;
+      prof->mark("k_deform", s);
+      cgs_launch_deform(D, s);
+      HIPCHK(hipGetLastError());
+      c->deformed_valid = true;
+      c->deform_generation++;
+      deformed = true;
+    }
+    F.S.geo = P<float>(c->def_geo);
+    F.split_count = P<uint32_t>(c->ctrl1);
+  } else {
+    F.S.geo = P<float>(c->rest_geo);
+    if (c->f16) F.S.cov_h = P<uint16_t>(c->rest_cov_h);
+  }
+  if (c->f16)
+    F.S.sh_h = P<uint16_t>(c->rest_sh_h);
+  else
+    F.S.sh = P<float>(c->rest_sh);
+  return CGS_OK;
+}
+
+// the Stage 3 cache key fields (gpu_state.rs:956-981)
+Stage3Key stage3_key(const cgs_context* c, const cgs_camera_uniforms* cam, const FrameGeom& g,
+                     const cgs_frame_params* prm) {
+  Stage3Key key;
+  memset(&key, 0, sizeof(key));
+  key.deform_generation = c->deform_generation;
+  for (int i = 0; i < 16; ++i) key.view_bits[i] = fbits(cam->view_matrix[i]);
+  key.fx = fbits(cam->focal_x);
+  key.fy = fbits(cam->focal_y);
+  key.cx = fbits(cam->principal_x);
+  key.cy = fbits(cam->principal_y);
+  key.width = g.W;
+  key.height = g.H;
+  key.near_bits = fbits(cam->near_plane);
+  key.mip_bits = fbits(prm->mip_filter_3d_var);
+  key.min_op_bits = fbits(prm->min_opacity);
+  key.min_rad_bits = fbits(prm->min_radius_px);
+  return key;
+}
+
+// Stage 2 of one view into the arena a (project_v2 dispatch, gpu_state.rs:882-948)
+int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t s,
+           CgsProf* prof, const cgs_camera_uniforms* cam, const cgs_compositor_uniforms* comp,
+           const cgs_frame_params* prm) {
+  const uint32_t max_g = F.max_g;
+  TRY(ensure_words(a.geom, (size_t)8 * max_g));
+  TRY(ensure_words(a.color, (size_t)4 * max_g));
+  TRY(ensure_words(a.extents, max_g));
+  init_entry_cap(a, max_g);
+  const FrameLayout L = frame_layout(max_g, a.entry_cap);
+  TRY(ensure_words(a.ctrl3, L.words));
+  CgsProjectArgs PA;
+  memset(&PA, 0, sizeof(PA));
+  PA.scene = F.S;
+  PA.bind_tet = c->has_cages ? P<uint32_t>(c->bind_tet) : nullptr;
+  PA.tet_rot = c->has_cages ? P<float>(c->rot) : nullptr;
+  PA.split_meta = c->has_cages ? P<uint32_t>(c->split_meta) : nullptr;
+  PA.split_count = F.split_count;
+  memcpy(PA.camera, cam, 96);
+  for (int i = 0; i < 3; ++i) PA.cam_pos[i] = comp->camera_position[i];
+  PA.mip = c->has_cages ? 0.0f : prm->mip_filter_3d_var;  // gpu_state.rs:894
+  PA.min_opacity = prm->min_opacity;
+  PA.min_radius = prm->min_radius_px;
+  PA.sh_degree = comp->sh_degree > 3 ? 3 : comp->sh_degree;
+  if (c->f16 && PA.sh_degree != c->f16_degree)
+    return fail(CGS_ERR_INVALID_ARG, "sh_degree must equal the quantized scene's degree");
+  PA.has_deformation = c->has_cages ? 1u : 0u;
+  PA.num_tets = c->num_tets;
+  PA.num_rest = F.N;
+  PA.max_gaussians = max_g;
+  PA.geom = P<float>(a.geom);
+  PA.color = P<float>(a.color);
+  PA.extents = P<uint32_t>(a.extents);
+  prof->mark("k_project", s);
+  cgs_launch_project(PA, s);
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+// the arena of an empty scene's frame: no tile has entries
+int background_arena(FrameArena& a, const FrameGeom& g, hipStream_t s) {
+  TRY(ensure_words(a.ranges, (size_t)g.T * 2));
+  cgs_launch_fill_u32(P<uint32_t>(a.ranges), 0u, (uint64_t)g.T * 2, s);
+  TRY(ensure_words(a.geom, 8));
+  TRY(ensure_words(a.color, 4));
+  TRY(ensure_words(a.e_ids, 1));
+  a.sorted_ids = P<uint32_t>(a.e_ids);
+  return CGS_OK;
 }
 
 }  // namespace
@@ -510,12 +703,10 @@ void cgs_destroy(cgs_context* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
-                    &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->geom, &c->color,
-                    &c->extents,
-                    &c->e_keys, &c->e_ids, &c->e_depths, &c->e_alt_k, &c->e_alt_v, &c->e_perm,
-                    &c->ranges, &c->out_rgba, &c->cnt_dbg, &c->off_dbg, &c->ctrl3, &c->prim_ctrl,
-                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->comp_order};
+                    &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d};
   for (DevBuf* b : bufs) release(*b);
+  c->fa.release_all();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->prof.ev)
@@ -683,7 +874,7 @@ int cgs_sync_cages(cgs_context* c, const uint32_t* elems, const float* inv_dm, u
 
 int cgs_set_capture(cgs_context* c, uint32_t enable) {
   if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
-  c->capture = enable != 0;
+  c->fa.capture = enable != 0;
   return CGS_OK;
 }
 
@@ -708,14 +899,9 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
 
   if (c->n_rest == 0) {
     // background frame (gpu_state.rs:658-704)
-    TRY(ensure_words(c->ranges, (size_t)g.T * 2));
-    cgs_launch_fill_u32(P<uint32_t>(c->ranges), 0u, (uint64_t)g.T * 2, s);
-    TRY(ensure_words(c->geom, 8));
-    TRY(ensure_words(c->color, 4));
-    TRY(ensure_words(c->e_ids, 1));
-    c->sorted_ids = P<uint32_t>(c->e_ids);
+    TRY(background_arena(c->fa, g, s));
     if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
-    TRY(launch_stage4(c, g, comp));
+    TRY(launch_stage4(c->fa, s, &c->prof, g, comp));
     if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));
     if (prof) T.stage4_composite_ms = elapsed(c->ev[3], c->ev[4]);
@@ -728,142 +914,30 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
     return CGS_OK;
   }
 
-  const uint32_t N = c->n_rest;
-  const bool split_possible = c->has_cages && prm->strain_split_threshold > 0.0f;
-  const uint32_t max_g = split_possible ? 2 * N : N;
-  c->max_g = max_g;
   if (prof) HIPCHK(hipEventRecord(c->ev[0], s));
-
   // ---- Stage 1 ----
-  CgsSceneDev S;
-  memset(&S, 0, sizeof(S));
-  const uint32_t* split_count = nullptr;
-  if (c->has_cages) {
-    const uint32_t p0 = fbits(prm->strain_split_threshold), p1 = fbits(prm->mip_filter_3d_var);
-    if (p0 != c->last_params[0] || p1 != c->last_params[1]) {
-      c->last_params[0] = p0;
-      c->last_params[1] = p1;
-      c->deformed_valid = false;
-    }
-    TRY(ensure_words(c->jac, (size_t)c->num_tets * 9));
-    TRY(ensure_words(c->rot, (size_t)c->num_tets * 9));
-    TRY(ensure_words(c->stretch, c->num_tets));
-    if (c->def_cap < max_g) {
-      c->def_cap = max_g;
-      c->deformed_valid = false;
-    }
-    TRY(ensure_words(c->def_geo, cgs_blk_words(c->def_cap, CGS_GEO_K)));
-    TRY(ensure_words(c->split_meta, (size_t)2 * N));
-    const uint32_t dparts = cgs_deform_num_parts(N);
-    TRY(ensure_words(c->ctrl1, 4 + 2 * (size_t)(dparts ? dparts : 1)));
-    if (c->f16) return fail(CGS_ERR_UNSUPPORTED, "cage deformation of the f16 storage variant");
-    if (!c->deformed_valid) {
-      cgs_launch_fill_u32(P<uint32_t>(c->ctrl1), 0u, 4 + 2 * (size_t)(dparts ? dparts : 1), s);
-      CgsTetArgs TA;
-      TA.verts = P<float>(c->verts);
-      TA.elems = P<uint32_t>(c->elems);
-      TA.inv_dm = P<float>(c->inv_dm);
-      TA.jac = P<float>(c->jac);
-      TA.rot = P<float>(c->rot);
-      TA.stretch = P<float>(c->stretch);
-      TA.num_tets = c->num_tets;
-      c->prof.mark("k_tet_transforms", s);
-      cgs_launch_tet_transforms(TA, s);
-      CgsDeformArgs D;
-      D.verts = P<float>(c->verts);
-      D.elems = P<uint32_t>(c->elems);
-      D.jac = P<float>(c->jac);
-      D.stretch = P<float>(c->stretch);
-      D.bind_tet = P<uint32_t>(c->bind_tet);
-      D.bind_w = P<float>(c->bind_w);
-      D.rest_geo = P<float>(c->rest_geo);
-      D.out_geo = P<float>(c->def_geo);
-      D.split_meta = P<uint32_t>(c->split_meta);
-      D.split_count = P<uint32_t>(c->ctrl1);
-      D.status = reinterpret_cast<uint64_t*>(P<uint32_t>(c->ctrl1) + 4);
-      D.ticket = P<uint32_t>(c->ctrl1) + 1;
-      D.split_threshold = split_possible ? prm->strain_split_threshold : 0.0f;
-      D.mip = prm->mip_filter_3d_var;
-      D.num_gaussians = N;
-      D.num_tets = c->num_tets;
-      c->prof.mark("k_deform", s);
-      cgs_launch_deform(D, s);
-      HIPCHK(hipGetLastError());
-      c->deformed_valid = true;
-      c->deform_generation++;
-      T.stage1_skipped = 0;
-    } else {
-      T.stage1_skipped = 1;
-    }
-    S.geo = P<float>(c->def_geo);
-    split_count = P<uint32_t>(c->ctrl1);
-  } else {
-    T.stage1_skipped = 1;
-    S.geo = P<float>(c->rest_geo);
-    if (c->f16) S.cov_h = P<uint16_t>(c->rest_cov_h);
-  }
-  if (c->f16)
-    S.sh_h = P<uint16_t>(c->rest_sh_h);
-  else
-    S.sh = P<float>(c->rest_sh);
+  SceneFrame F;
+  bool deformed = false;
+  TRY(stage1(c, prm, s, &c->prof, F, deformed));
+  T.stage1_skipped = deformed ? 0u : 1u;
+  const uint32_t N = F.N, max_g = F.max_g;
+  c->max_g = max_g;
   if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
 
   // ---- Stage 3 cache key (gpu_state.rs:956-981) ----
-  Stage3Key key;
-  memset(&key, 0, sizeof(key));
-  key.deform_generation = c->deform_generation;
-  for (int i = 0; i < 16; ++i) key.view_bits[i] = fbits(cam->view_matrix[i]);
-  key.fx = fbits(cam->focal_x);
-  key.fy = fbits(cam->focal_y);
-  key.cx = fbits(cam->principal_x);
-  key.cy = fbits(cam->principal_y);
-  key.width = g.W;
-  key.height = g.H;
-  key.near_bits = fbits(cam->near_plane);
-  key.mip_bits = fbits(prm->mip_filter_3d_var);
-  key.min_op_bits = fbits(prm->min_opacity);
-  key.min_rad_bits = fbits(prm->min_radius_px);
+  const Stage3Key key = stage3_key(c, cam, g, prm);
   const bool use_cache = !prm->force_stage3 && c->s3_valid && c->s3_key == key;
 
   // ---- Stage 2 ----
-  TRY(ensure_words(c->geom, (size_t)8 * max_g));
-  TRY(ensure_words(c->color, (size_t)4 * max_g));
-  TRY(ensure_words(c->extents, max_g));
-  init_entry_cap(c, max_g);
-  const FrameLayout L = frame_layout(max_g, c->entry_cap);
-  TRY(ensure_words(c->ctrl3, L.words));
-  CgsProjectArgs PA;
-  memset(&PA, 0, sizeof(PA));
-  PA.scene = S;
-  PA.bind_tet = c->has_cages ? P<uint32_t>(c->bind_tet) : nullptr;
-  PA.tet_rot = c->has_cages ? P<float>(c->rot) : nullptr;
-  PA.split_meta = c->has_cages ? P<uint32_t>(c->split_meta) : nullptr;
-  PA.split_count = split_count;
-  memcpy(PA.camera, cam, 96);
-  for (int i = 0; i < 3; ++i) PA.cam_pos[i] = comp->camera_position[i];
-  PA.mip = c->has_cages ? 0.0f : prm->mip_filter_3d_var;  // gpu_state.rs:894
-  PA.min_opacity = prm->min_opacity;
-  PA.min_radius = prm->min_radius_px;
-  PA.sh_degree = comp->sh_degree > 3 ? 3 : comp->sh_degree;
-  if (c->f16 && PA.sh_degree != c->f16_degree)
-    return fail(CGS_ERR_INVALID_ARG, "sh_degree must equal the quantized scene's degree");
-  PA.has_deformation = c->has_cages ? 1u : 0u;
-  PA.num_tets = c->num_tets;
-  PA.num_rest = N;
-  PA.max_gaussians = max_g;
-  PA.geom = P<float>(c->geom);
-  PA.color = P<float>(c->color);
-  PA.extents = P<uint32_t>(c->extents);
-  c->prof.mark("k_project", s);
-  cgs_launch_project(PA, s);
-  HIPCHK(hipGetLastError());
+  TRY(stage2(c, F, c->fa, s, &c->prof, cam, comp, prm));
   if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
 
   // ---- Stage 3 ----
-  if (!use_cache) TRY(launch_stage3(c, g, cam->near_plane, N, split_count, max_g, true));
+  if (!use_cache)
+    TRY(launch_stage3(c->fa, s, &c->prof, g, cam->near_plane, N, F.split_count, max_g));
   if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
   // ---- Stage 4 ----
-  TRY(launch_stage4(c, g, comp));
+  TRY(launch_stage4(c->fa, s, &c->prof, g, comp));
   if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
 
   TRY(read_counters(c, c->has_cages));
@@ -873,13 +947,13 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
     uint32_t total = c->h_counters[2];
     // the tile-entry arena overflowed: grow and re-run Stages 3-4
     int reruns = 0;
-    while (total > c->entry_cap && reruns < 4) {
+    while (total > c->fa.entry_cap && reruns < 4) {
       uint64_t want = (uint64_t)total + total / 2;
       if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
-      c->entry_cap = (uint32_t)want;
-      TRY(launch_stage3(c, g, cam->near_plane, N, split_count, max_g, false));
+      c->fa.entry_cap = (uint32_t)want;
+      TRY(launch_stage3(c->fa, s, &c->prof, g, cam->near_plane, N, F.split_count, max_g));
       if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
-      TRY(launch_stage4(c, g, comp));
+      TRY(launch_stage4(c->fa, s, &c->prof, g, comp));
       if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
       TRY(read_counters(c, c->has_cages));
       total = c->h_counters[2];
@@ -894,7 +968,7 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   } else {
     T.stage3_cached = 1;
   }
-  c->active = N + (split_possible ? splits : 0u);
+  c->active = N + (F.split_possible ? splits : 0u);
   if (c->prof.on) {
     c->prof.mark("end", s);
     HIPCHK(hipStreamSynchronize(s));
@@ -934,9 +1008,9 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 int cgs_readback_rgba8(cgs_context* c, uint8_t* dst, uint32_t width, uint32_t height) {
   TRY(ensure_ctx(c));
   if (!dst) return fail(CGS_ERR_INVALID_ARG, "null destination");
-  if (!c->out_rgba.p || width != c->last_w || height != c->last_h)
+  if (!c->fa.out_rgba.p || width != c->last_w || height != c->last_h)
     return fail(CGS_ERR_SHORT_READBACK, "framebuffer size differs from the last frame");
-  HIPCHK(hipMemcpy2DAsync(dst, (size_t)width * 4, c->out_rgba.p, (size_t)c->padded_row * 4,
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)width * 4, c->fa.out_rgba.p, (size_t)c->fa.padded_row * 4,
                           (size_t)width * 4, height, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
@@ -944,8 +1018,8 @@ int cgs_readback_rgba8(cgs_context* c, uint8_t* dst, uint32_t width, uint32_t he
 
 int cgs_framebuffer_device_ptr(cgs_context* c, void** ptr, uint32_t* padded_row_u32) {
   if (!c || !ptr || !padded_row_u32) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
-  *ptr = c->out_rgba.p;
-  *padded_row_u32 = c->padded_row;
+  *ptr = c->fa.out_rgba.p;
+  *padded_row_u32 = c->fa.padded_row;
   return CGS_OK;
 }
 
@@ -955,9 +1029,9 @@ int cgs_read_projected(cgs_context* c, float* attrs10, uint32_t* extents, uint64
   std::vector<float> geom((size_t)n * 8), color((size_t)n * 4);
   std::vector<uint32_t> ext(n);
   if (n) {
-    HIPCHK(hipMemcpyAsync(geom.data(), c->geom.p, n * 32, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(color.data(), c->color.p, n * 16, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(ext.data(), c->extents.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(geom.data(), c->fa.geom.p, n * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(color.data(), c->fa.color.p, n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ext.data(), c->fa.extents.p, n * 4, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   for (uint64_t i = 0; i < n; ++i) {
@@ -979,20 +1053,20 @@ int cgs_read_projected(cgs_context* c, float* attrs10, uint32_t* extents, uint64
 int cgs_read_stage3(cgs_context* c, uint32_t* counts, uint32_t* offsets, uint32_t* sorted_keys,
                     uint32_t* sorted_ids, uint32_t* ranges, uint64_t* n_entries) {
   TRY(ensure_ctx(c));
-  if ((counts || offsets) && !c->captured)
+  if ((counts || offsets) && !c->fa.captured)
     return fail(CGS_ERR_INVALID_ARG, "counts/offsets need cgs_set_capture(ctx, 1) before the frame");
   const uint32_t E = c->n_entries;
-  const size_t T = (size_t)c->tiles_x * c->tiles_y;
+  const size_t T = (size_t)c->fa.tiles_x * c->fa.tiles_y;
   if (counts && c->max_g)
-    HIPCHK(hipMemcpyAsync(counts, c->cnt_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(counts, c->fa.cnt_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
   if (offsets && c->max_g)
-    HIPCHK(hipMemcpyAsync(offsets, c->off_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(offsets, c->fa.off_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
   if (sorted_keys && E)
-    HIPCHK(hipMemcpyAsync(sorted_keys, c->sorted_keys, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(sorted_keys, c->fa.sorted_keys, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
   if (sorted_ids && E)
-    HIPCHK(hipMemcpyAsync(sorted_ids, c->sorted_ids, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(sorted_ids, c->fa.sorted_ids, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
   if (ranges && T)
-    HIPCHK(hipMemcpyAsync(ranges, c->ranges.p, T * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ranges, c->fa.ranges.p, T * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (n_entries) *n_entries = E;
   return CGS_OK;
@@ -1048,13 +1122,13 @@ int cgs_load_projected(cgs_context* c, const float* attrs10, const uint32_t* ext
     gg[7] = extents[i] ? cgs_fclamp(2.0f * cgs_logf(255.0f * a[5]), 0.0f, 8.0f) : 0.0f;
     color[i * 4] = a[6]; color[i * 4 + 1] = a[7]; color[i * 4 + 2] = a[8]; color[i * 4 + 3] = 0.0f;
   }
-  TRY(ensure_words(c->geom, (size_t)8 * n));
-  TRY(ensure_words(c->color, (size_t)4 * n));
-  TRY(ensure_words(c->extents, n));
+  TRY(ensure_words(c->fa.geom, (size_t)8 * n));
+  TRY(ensure_words(c->fa.color, (size_t)4 * n));
+  TRY(ensure_words(c->fa.extents, n));
   if (n) {
-    HIPCHK(hipMemcpyAsync(c->geom.p, geom.data(), n * 32, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->color.p, color.data(), n * 16, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->extents.p, extents, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->fa.geom.p, geom.data(), n * 32, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->fa.color.p, color.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->fa.extents.p, extents, n * 4, hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   c->max_g = (uint32_t)n;
@@ -1071,14 +1145,14 @@ int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
   c->last_w = g.W;
   c->last_h = g.H;
   for (int attempt = 0; attempt < 5; ++attempt) {
-    TRY(launch_stage3(c, g, cam->near_plane, c->active, nullptr, c->max_g, false));
+    TRY(launch_stage3(c->fa, c->stream, &c->prof, g, cam->near_plane, c->active, nullptr, c->max_g));
     TRY(read_counters(c, false));
     const uint32_t total = c->h_counters[2];
     c->n_entries = total;
-    if (total <= c->entry_cap) break;
+    if (total <= c->fa.entry_cap) break;
     uint64_t want = (uint64_t)total + total / 2;
     if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
-    c->entry_cap = (uint32_t)want;
+    c->fa.entry_cap = (uint32_t)want;
   }
   if (n_entries) *n_entries = c->n_entries;
   return CGS_OK;
@@ -1088,9 +1162,9 @@ int cgs_run_stage4(cgs_context* c, const cgs_compositor_uniforms* comp) {
   TRY(ensure_ctx(c));
   if (!comp) return fail(CGS_ERR_INVALID_ARG, "null argument");
   const FrameGeom g = frame_geom(comp);
-  if (g.tiles_x != c->tiles_x || g.tiles_y != c->tiles_y)
+  if (g.tiles_x != c->fa.tiles_x || g.tiles_y != c->fa.tiles_y)
     return fail(CGS_ERR_INVALID_ARG, "compositor size differs from Stage 3");
-  TRY(launch_stage4(c, g, comp));
+  TRY(launch_stage4(c->fa, c->stream, &c->prof, g, comp));
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
 }
@@ -1680,5 +1754,394 @@ int cgs_mvc_deform(cgs_context* c, const float* d_weights, const float* d_gradie
 int cgs_fail(int code, const char* msg) { return fail(code, msg ? msg : "error"); }
 hipStream_t cgs_context_stream(cgs_context* c) { return c ? c->stream : nullptr; }
 
+
+// ===========================================================================
+// View batch: many views of several resident scenes from one HIP graph
+// ===========================================================================
+//
+// The reference renders one view per execute_frame_gpu call (gpu_state.rs:
+// 613-1243), waiting for its counts between stages.  A batch renders n views
+// of up to n scenes with no host round trip between them:
+//   * Stage 1 of every scene runs first, on the scene's own stream (it is
+//     shared by all views of the scene);
+//   * Stages 2-4 of every view are captured once into a HIP graph and
+//     replayed while the views, the scenes' device buffers and the batch's own
+//     buffers are unchanged (a fingerprint of all of them keys the graph);
+//   * the views are dealt to `lanes` arenas, longest first (an estimate of
+//     splats + pixels/2), and each lane is a chain of views on its own branch
+//     of the graph, so the latency-bound steps of one view (look-back sort
+//     passes, the tile-order block, launch gaps) overlap the others;
+//   * each view composites into its own framebuffer and leaves its tile-entry
+//     total and survivor count in a device slot; these are read once after
+//     the replay, and a view whose entries overflowed its lane's arena has
+//     the arena grown (1.5x) and the batch replayed.
+// A view's pixels are those cgs_execute_frame gives for the same camera: the
+// kernels and their inputs are the same.
+
+struct cgs_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  std::vector<hipStream_t> lane_stream;
+  std::vector<hipEvent_t> lane_done;
+  std::vector<FrameArena> lanes;
+  DevBuf fb;        // every view's framebuffer, 256-byte aligned slices
+  DevBuf counters;  // [view][4]: control words 0-3 of the view's Stage 3
+  uint32_t* h_counters = nullptr;
+  size_t h_counter_words = 0;
+  std::vector<size_t> fb_off;  // words
+  std::vector<uint32_t> fb_w, fb_h, fb_row, view_lane;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t key = 0;
+  CgsProf noprof;  // never on
+};
+
+namespace {
+
+void batch_drop_graph(cgs_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->key = 0;
+}
+
+// every buffer a view of (max_g splat slots, T tiles) touches in Stages 2-4
+int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64) {
+  init_entry_cap(a, max_g);
+  const uint32_t cap = a.entry_cap;
+  TRY(ensure_words(a.geom, (size_t)8 * max_g));
+  TRY(ensure_words(a.color, (size_t)4 * max_g));
+  TRY(ensure_words(a.extents, max_g));
+  TRY(ensure_words(a.cnt_dbg, max_g));
+  TRY(ensure_words(a.e_keys, cap));
+  TRY(ensure_words(a.e_ids, cap));
+  TRY(ensure_words(a.e_alt_k, cap));
+  TRY(ensure_words(a.e_alt_v, cap));
+  if (mode64) {
+    TRY(ensure_words(a.e_depths, cap));
+    TRY(ensure_words(a.e_perm, cap));
+  }
+  TRY(ensure_words(a.ranges, (size_t)T * 2));
+  TRY(ensure_words(a.comp_order, T));
+  TRY(ensure_words(a.ctrl3, frame_layout(max_g, cap).words));
+  return CGS_OK;
+}
+
+uint64_t fnv_bytes(uint64_t h, const void* p, size_t n) { return fnv1a(h, p, n); }
+
+uint64_t fnv_buf(uint64_t h, const DevBuf& d) {
+  h = fnv_u64(h, (uint64_t)(uintptr_t)d.p);
+  return fnv_u64(h, d.cap_words);
+}
+
+// Stages 2-4 of view v on lane stream s (no host synchronisation)
+int batch_enqueue_view(cgs_batch* b, cgs_context* c, const SceneFrame& F, FrameArena& a,
+                       hipStream_t s, const cgs_camera_uniforms* cam,
+                       const cgs_compositor_uniforms* comp, const cgs_frame_params* prm,
+                       uint32_t v) {
+  const FrameGeom g = frame_geom(comp);
+  uint32_t* out = P<uint32_t>(b->fb) + b->fb_off[v];
+  uint32_t* slot = P<uint32_t>(b->counters) + 4 * (size_t)v;
+  if (F.N == 0) {
+    // background frame (gpu_state.rs:658-704)
+    TRY(background_arena(a, g, s));
+    cgs_launch_fill_u32(slot, 0u, 4, s);
+    return launch_stage4(a, s, &b->noprof, g, comp, out);
+  }
+  TRY(stage2(c, F, a, s, &b->noprof, cam, comp, prm));
+  TRY(launch_stage3(a, s, &b->noprof, g, cam->near_plane, F.N, F.split_count, F.max_g));
+  cgs_launch_copy_words(P<uint32_t>(a.ctrl3), slot, 4, s);
+  return launch_stage4(a, s, &b->noprof, g, comp, out);
+}
+
+int batch_capture(cgs_batch* b, cgs_context* const* scenes, const std::vector<SceneFrame>& SF,
+                  const uint32_t* view_scene, const cgs_camera_uniforms* cams,
+                  const cgs_compositor_uniforms* comps, uint32_t n_views,
+                  const cgs_frame_params* prm) {
+  batch_drop_graph(b);
+  const uint32_t L = (uint32_t)b->lanes.size();
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  g_no_alloc = true;
+  int rc = CGS_OK;
+  hipError_t e = hipEventRecord(b->fork, b->stream);
+  for (uint32_t k = 0; k < L && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->lane_stream[k], b->fork, 0);
+  if (e != hipSuccess) rc = fail(CGS_ERR_RUNTIME, std::string("graph fork: ") + hipGetErrorString(e));
+  for (uint32_t v = 0; v < n_views && rc == CGS_OK; ++v) {
+    const uint32_t k = b->view_lane[v], si = view_scene[v];
+    rc = batch_enqueue_view(b, scenes[si], SF[si], b->lanes[k], b->lane_stream[k], &cams[v],
+                            &comps[v], prm, v);
+  }
+  for (uint32_t k = 0; k < L && rc == CGS_OK; ++k) {
+    e = hipEventRecord(b->lane_done[k], b->lane_stream[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->lane_done[k], 0);
+    if (e != hipSuccess) rc = fail(CGS_ERR_RUNTIME, std::string("graph join: ") + hipGetErrorString(e));
+  }
+  g_no_alloc = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
+  if (rc != CGS_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ee != hipSuccess || !graph)
+    return fail(CGS_ERR_RUNTIME, std::string("graph capture: ") + hipGetErrorString(ee));
+  b->graph = graph;
+  const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    batch_drop_graph(b);
+    return fail(CGS_ERR_RUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  }
+  return CGS_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int cgs_batch_create(int device, uint32_t lanes, cgs_batch** out) {
+  if (!out) return fail(CGS_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (lanes < 1 || lanes > 16) return fail(CGS_ERR_INVALID_ARG, "lanes must be in [1, 16]");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(CGS_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(CGS_ERR_INVALID_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  cgs_batch* b = new cgs_batch();
+  b->device = device;
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreate(&b->t0) == hipSuccess && hipEventCreate(&b->t1) == hipSuccess;
+  b->lanes.resize(lanes);
+  b->lane_stream.assign(lanes, nullptr);
+  b->lane_done.assign(lanes, nullptr);
+  for (uint32_t k = 0; k < lanes && ok; ++k)
+    ok = hipStreamCreateWithFlags(&b->lane_stream[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->lane_done[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    cgs_batch_destroy(b);
+    return fail(CGS_ERR_RUNTIME, "stream or event creation failed");
+  }
+  *out = b;
+  return CGS_OK;
+}
+
+void cgs_batch_destroy(cgs_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (FrameArena& a : b->lanes) a.release_all();
+  release(b->fb);
+  release(b->counters);
+  if (b->h_counters) (void)hipHostFree(b->h_counters);
+  for (hipStream_t s : b->lane_stream)
+    if (s) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->lane_done)
+    if (e) (void)hipEventDestroy(e);
+  hipEvent_t evs[] = {b->fork, b->t0, b->t1};
+  for (hipEvent_t e : evs)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t n_scenes,
+                              const uint32_t* view_scene, const cgs_camera_uniforms* cams,
+                              const cgs_compositor_uniforms* comps, uint32_t n_views,
+                              const cgs_frame_params* prm, cgs_batch_stats* stats) {
+  if (!b || !scenes || !prm || (n_views && (!view_scene || !cams || !comps)))
+    return fail(CGS_ERR_INVALID_ARG, "null batch argument");
+  if (n_scenes == 0 && n_views) return fail(CGS_ERR_INVALID_ARG, "views without scenes");
+  HIPCHK(hipSetDevice(b->device));
+  cgs_batch_stats St;
+  memset(&St, 0, sizeof(St));
+  St.views = n_views;
+  St.lanes = (uint32_t)b->lanes.size();
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    if (!scenes[i]) return fail(CGS_ERR_INVALID_ARG, "null scene context");
+    if (scenes[i]->device != b->device) return fail(CGS_ERR_INVALID_ARG, "scene on another device");
+  }
+  for (uint32_t v = 0; v < n_views; ++v)
+    if (view_scene[v] >= n_scenes) return fail(CGS_ERR_INVALID_ARG, "view scene index out of range");
+
+  // ---- Stage 1 of every scene, on its own stream ----
+  std::vector<SceneFrame> SF(n_scenes);
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    cgs_context* c = scenes[i];
+    if (c->n_rest == 0) continue;
+    bool deformed = false;
+    TRY(stage1(c, prm, c->stream, &b->noprof, SF[i], deformed));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+
+  // ---- views -> lanes (longest first onto the least loaded lane) ----
+  const uint32_t L = (uint32_t)b->lanes.size();
+  std::vector<FrameGeom> G(n_views);
+  std::vector<double> cost(n_views);
+  std::vector<uint32_t> order(n_views);
+  for (uint32_t v = 0; v < n_views; ++v) {
+    G[v] = frame_geom(&comps[v]);
+    cost[v] = (double)SF[view_scene[v]].max_g + 0.5 * (double)G[v].W * G[v].H;
+    order[v] = v;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+  std::vector<double> load(L, 0.0);
+  b->view_lane.assign(n_views, 0);
+  for (uint32_t v : order) {
+    uint32_t best = 0;
+    for (uint32_t k = 1; k < L; ++k)
+      if (load[k] < load[best]) best = k;
+    b->view_lane[v] = best;
+    load[best] += cost[v];
+  }
+
+  // ---- framebuffers and counter slots ----
+  b->fb_off.assign(n_views, 0);
+  b->fb_w.assign(n_views, 0);
+  b->fb_h.assign(n_views, 0);
+  b->fb_row.assign(n_views, 0);
+  size_t words = 0;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    b->fb_off[v] = words;
+    b->fb_w[v] = G[v].W;
+    b->fb_h[v] = G[v].H;
+    b->fb_row[v] = G[v].padded_row;
+    words += ((size_t)G[v].padded_row * G[v].H + 63) / 64 * 64;
+  }
+  TRY(ensure_words(b->fb, words));
+  TRY(ensure_words(b->counters, 4 * (size_t)(n_views ? n_views : 1)));
+  if (b->h_counter_words < 4 * (size_t)n_views) {
+    if (b->h_counters) (void)hipHostFree(b->h_counters);
+    b->h_counters = nullptr;
+    b->h_counter_words = 0;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_counters), 16 * (size_t)n_views,
+                         hipHostMallocDefault));
+    b->h_counter_words = 4 * (size_t)n_views;
+  }
+  if (n_views == 0) {
+    if (stats) *stats = St;
+    return CGS_OK;
+  }
+
+  for (int attempt = 0;; ++attempt) {
+    // ---- size every lane for its largest view ----
+    for (uint32_t k = 0; k < L; ++k) {
+      uint32_t mg = 1, T = 1;
+      bool m64 = false;
+      for (uint32_t v = 0; v < n_views; ++v) {
+        if (b->view_lane[v] != k) continue;
+        mg = std::max(mg, SF[view_scene[v]].max_g);
+        T = std::max(T, G[v].T);
+        m64 = m64 || G[v].T > 65535u;
+      }
+      TRY(size_arena(b->lanes[k], mg, T, m64));
+    }
+    // ---- the graph's fingerprint: every pointer and value it bakes in ----
+    uint64_t h = FNV_OFFSET;
+    h = fnv_u64(h, n_views);
+    h = fnv_u64(h, L);
+    h = fnv_bytes(h, view_scene, 4 * (size_t)n_views);
+    h = fnv_bytes(h, cams, sizeof(cgs_camera_uniforms) * n_views);
+    h = fnv_bytes(h, comps, sizeof(cgs_compositor_uniforms) * n_views);
+    const float pv[4] = {prm->strain_split_threshold, prm->mip_filter_3d_var, prm->min_opacity,
+                         prm->min_radius_px};
+    h = fnv_bytes(h, pv, sizeof(pv));
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+      const cgs_context* c = scenes[i];
+      h = fnv_u64(h, (uint64_t)(uintptr_t)c);
+      h = fnv_bytes(h, &SF[i].S, sizeof(SF[i].S));
+      h = fnv_u64(h, (uint64_t)(uintptr_t)SF[i].split_count);
+      h = fnv_u64(h, ((uint64_t)SF[i].N << 32) | SF[i].max_g);
+      h = fnv_u64(h, ((uint64_t)c->num_tets << 8) | (c->has_cages ? 1u : 0u) |
+                         (c->f16 ? 2u : 0u) | ((uint64_t)c->f16_degree << 2));
+      h = fnv_buf(h, c->bind_tet);
+      h = fnv_buf(h, c->rot);
+      h = fnv_buf(h, c->split_meta);
+    }
+    for (const FrameArena& a : b->lanes) {
+      const DevBuf* bufs[] = {&a.geom, &a.color, &a.extents, &a.cnt_dbg, &a.e_keys, &a.e_ids,
+                              &a.e_alt_k, &a.e_alt_v, &a.e_depths, &a.e_perm, &a.ranges,
+                              &a.comp_order, &a.ctrl3};
+      for (const DevBuf* d : bufs) h = fnv_buf(h, *d);
+      h = fnv_u64(h, a.entry_cap);
+    }
+    h = fnv_buf(h, b->fb);
+    h = fnv_buf(h, b->counters);
+    if (h == 0) h = 1;
+    if (!b->exec || b->key != h) {
+      TRY(batch_capture(b, scenes, SF, view_scene, cams, comps, n_views, prm));
+      b->key = h;
+      St.graph_built = 1;
+    }
+    // ---- replay ----
+    HIPCHK(hipEventRecord(b->t0, b->stream));
+    HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->t1, b->stream));
+    HIPCHK(hipMemcpyAsync(b->h_counters, b->counters.p, 16 * (size_t)n_views,
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    St.gpu_ms += elapsed(b->t0, b->t1);
+    // ---- arena overflow: grow the lane and replay ----
+    std::vector<uint32_t> need(L, 0);
+    bool overflow = false;
+    uint64_t entries = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+      const uint32_t total = b->h_counters[4 * v + 2];
+      entries += total;
+      FrameArena& a = b->lanes[b->view_lane[v]];
+      need[b->view_lane[v]] = std::max(need[b->view_lane[v]], total);
+      if (total > a.entry_cap) overflow = true;
+    }
+    St.total_tile_entries = entries;
+    if (!overflow) break;
+    if (attempt >= 4) return fail(CGS_ERR_EXECUTION, "tile-entry arena still overflowing");
+    for (uint32_t k = 0; k < L; ++k) {
+      FrameArena& a = b->lanes[k];
+      if (need[k] <= a.entry_cap) continue;
+      const uint64_t want = (uint64_t)need[k] + need[k] / 2;
+      if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
+      a.entry_cap = (uint32_t)want;
+    }
+    St.reruns++;
+  }
+  if (stats) *stats = St;
+  return CGS_OK;
+}
+
+int cgs_batch_framebuffer(cgs_batch* b, uint32_t view, void** ptr, uint32_t* padded_row_u32) {
+  if (!b || !ptr || !padded_row_u32) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  if (view >= b->fb_off.size()) return fail(CGS_ERR_INVALID_ARG, "view index out of range");
+  *ptr = P<uint32_t>(b->fb) + b->fb_off[view];
+  *padded_row_u32 = b->fb_row[view];
+  return CGS_OK;
+}
+
+int cgs_batch_readback_rgba8(cgs_batch* b, uint32_t view, uint8_t* dst, uint32_t width,
+                             uint32_t height) {
+  if (!b || !dst) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  if (view >= b->fb_off.size()) return fail(CGS_ERR_INVALID_ARG, "view index out of range");
+  if (width != b->fb_w[view] || height != b->fb_h[view])
+    return fail(CGS_ERR_SHORT_READBACK, "size differs from the view's framebuffer");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)width * 4, P<uint32_t>(b->fb) + b->fb_off[view],
+                          (size_t)b->fb_row[view] * 4, (size_t)width * 4, height,
+                          hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return CGS_OK;
+}
+
+int cgs_batch_view_counts(cgs_batch* b, uint32_t view, uint32_t* tile_entries,
+                          uint32_t* survivors) {
+  if (!b) return fail(CGS_ERR_INVALID_ARG, "null batch");
+  if (view >= b->fb_off.size() || 4 * (size_t)view >= b->h_counter_words)
+    return fail(CGS_ERR_INVALID_ARG, "view index out of range");
+  if (tile_entries) *tile_entries = b->h_counters[4 * view + 2];
+  if (survivors) *survivors = b->h_counters[4 * view + 3];
+  return CGS_OK;
+}
+
 }  // extern "C"

@@ -322,8 +322,79 @@ class DeviceBuffer:
         self.ptr = None
 
 
+class ViewBatch:
+    """Many views of several resident scenes from one captured HIP graph.
+
+    The reference renders one view per ``execute_frame_gpu`` call
+    (gpu_state.rs:613-1243) in a per-view loop of its callers
+    (benchmark.rs:231-366, headless.rs:140-210); this renders a list of views
+    in one call.  Each view's pixels equal ``execute_frame_gpu`` on the same
+    scene, camera and parameters.  ``lanes`` views run concurrently.
+    """
+
+    def __init__(self, device: int = 0, lanes: int = 4):
+        lib = _lib.load()
+        h = C.c_void_p()
+        check(lib.cgs_batch_create(device, lanes, C.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cgs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def execute(self, states, view_scene, cameras, compositors, params: FrameParams):
+        """Render view v of ``states[view_scene[v]]`` (synchronised GpuPipelineStates)."""
+        n = len(cameras)
+        if len(compositors) != n or len(view_scene) != n:
+            raise ValueError("cameras, compositors and view_scene differ in length")
+        scenes = (C.c_void_p * max(1, len(states)))(*[st._h for st in states])
+        vs = np.ascontiguousarray(np.asarray(view_scene, dtype=np.uint32))
+        cams = (CameraUniforms * max(1, n))(*cameras)
+        comps = (CompositorUniforms * max(1, n))(*compositors)
+        stats = _lib.BatchStats()
+        check(self._lib.cgs_execute_views_batched(
+            self._h, scenes, len(states), _ptr(vs), cams, comps, n, C.byref(params),
+            C.byref(stats)))
+        self.sizes = [(max(1, c.screen_width), max(1, c.screen_height)) for c in compositors]
+        return stats
+
+    def readback_rgba8(self, view: int) -> np.ndarray:
+        w, h = self.sizes[view]
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        check(self._lib.cgs_batch_readback_rgba8(self._h, view, _ptr(out), w, h))
+        return out
+
+    def framebuffer_device_ptr(self, view: int):
+        p = C.c_void_p()
+        row = C.c_uint32()
+        check(self._lib.cgs_batch_framebuffer(self._h, view, C.byref(p), C.byref(row)))
+        return p.value, row.value
+
+    def view_counts(self, view: int):
+        """(tile entries, surviving splats) of a view of the last batch."""
+        e = C.c_uint32()
+        s = C.c_uint32()
+        check(self._lib.cgs_batch_view_counts(self._h, view, C.byref(e), C.byref(s)))
+        return e.value, s.value
+
+
 __all__ = [
     "CameraUniforms", "CompositorUniforms", "FrameParams", "StageTimings", "GpuError",
     "GaussianSceneSoa", "QuantizedGaussianSceneSoa", "TetCage", "GpuPipelineState",
-    "DeviceBuffer", "padded_rgba8_row_u32",
+    "DeviceBuffer", "ViewBatch", "padded_rgba8_row_u32",
 ]

@@ -185,3 +185,27 @@ def quantize_f16(scene: GaussianSceneSoa, sh_degree: int = 3) -> QuantizedGaussi
     sh_p = pack_f16_pair(tmp[:, 0::2], tmp[:, 1::2]).astype(np.uint32)
     return QuantizedGaussianSceneSoa(scene.positions.copy(), np.ascontiguousarray(cov_p),
                                      scene.opacities.copy(), np.ascontiguousarray(sh_p), deg)
+
+
+BATCH_RESOLUTIONS = ((320, 240), (800, 600), (1920, 1080), (3840, 2160))
+BATCH_SCENE_SPLATS = (200_000, CONFIG2_SPLATS, 4_000_000)
+
+
+def batch_views(n_views: int = 64, n_scenes: int = 3, seed: int = 5,
+                resolutions=BATCH_RESOLUTIONS):
+    """Seeded orbit views for batched rendering: view v looks at the origin of
+    scene v % n_scenes from azimuth ~ U(0, 2 pi), elevation ~ U(-0.4, 0.4) rad
+    and distance ~ U(2.5, 5), with a resolution drawn uniformly from
+    ``resolutions`` (fov 60 deg).  Returns (view_scene, cameras, compositors)."""
+    rng = _rng(seed)
+    view_scene, cams, comps = [], [], []
+    for v in range(n_views):
+        W, H = resolutions[int(rng.integers(0, len(resolutions)))]
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        el = rng.uniform(-0.4, 0.4)
+        r = rng.uniform(2.5, 5.0)
+        eye = (r * math.cos(el) * math.sin(az), r * math.sin(el), -r * math.cos(el) * math.cos(az))
+        view_scene.append(v % n_scenes)
+        cams.append(camera_look_at(eye, (0, 0, 0), (0, 1, 0), 60.0, W, H))
+        comps.append(compositor(W, H, eye))
+    return view_scene, cams, comps

@@ -140,6 +140,49 @@ int cgs_framebuffer_device_ptr(cgs_context* ctx, void** ptr, uint32_t* padded_ro
  * at kernel granularity): names[32*i] (NUL-terminated) and ms[i]. */
 int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, uint32_t* count);
 
+/* ---- batched multi-view rendering ----
+ * The reference renders one view per GpuPipelineState::execute_frame_gpu call
+ * (gpu_state.rs:613-1243) from a per-view loop in its callers
+ * (benchmark.rs:231-366, headless.rs:140-210).  A cgs_batch renders many views
+ * of several resident scenes (each a cgs_context after cgs_sync_scene /
+ * cgs_sync_cages) in one call: Stage 1 once per scene, then Stages 2-4 of
+ * every view from one captured HIP graph, replayed while the views and the
+ * buffers are unchanged.  Each view's RGBA8 equals what cgs_execute_frame
+ * gives for the same scene, camera and parameters.  A batch is bound to one
+ * device and is not thread-safe; it must not run concurrently with frames of
+ * the contexts it reads. */
+typedef struct cgs_batch cgs_batch;
+
+typedef struct cgs_batch_stats {
+  uint32_t views;              /* views rendered */
+  uint32_t lanes;              /* concurrent branches of the graph */
+  uint32_t graph_built;        /* 1 when this call captured the graph */
+  uint32_t reruns;             /* replays after a lane's tile-entry arena grew */
+  uint64_t total_tile_entries; /* summed over the views */
+  double gpu_ms;               /* device time of the graph replay(s) */
+} cgs_batch_stats;
+
+/* lanes in [1, 16]: views run on that many concurrent graph branches, each
+ * with its own frame arena. */
+int cgs_batch_create(int device, uint32_t lanes, cgs_batch** out);
+void cgs_batch_destroy(cgs_batch* batch);
+
+/* View v renders scenes[view_scene[v]] with cameras[v] and compositors[v];
+ * params are shared (profile_stages and force_stage3 are ignored: every view
+ * runs Stage 3).  Waits for the views to finish. */
+int cgs_execute_views_batched(cgs_batch* batch, cgs_context* const* scenes, uint32_t n_scenes,
+                              const uint32_t* view_scene, const cgs_camera_uniforms* cameras,
+                              const cgs_compositor_uniforms* compositors, uint32_t n_views,
+                              const cgs_frame_params* params, cgs_batch_stats* stats);
+
+/* View v of the last batch: device framebuffer and row stride in u32 words,
+ * tightly packed readback, and its tile-entry / survivor counts. */
+int cgs_batch_framebuffer(cgs_batch* batch, uint32_t view, void** ptr, uint32_t* padded_row_u32);
+int cgs_batch_readback_rgba8(cgs_batch* batch, uint32_t view, uint8_t* dst, uint32_t width,
+                             uint32_t height);
+int cgs_batch_view_counts(cgs_batch* batch, uint32_t view, uint32_t* tile_entries,
+                          uint32_t* survivors);
+
 /* ---- per-stage access (the stage wrappers of src/render/gpu_bridge.rs) ---- */
 
 /* Stage 2 output in the reference layout: attrs[10*n] = [mu_x, mu_y, a, b, c,
