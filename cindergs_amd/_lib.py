@@ -92,6 +92,20 @@ class StageTimings(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """cgs_tuning: launch shapes that leave every output bit-identical."""
+
+    _fields_ = [
+        ("radix_partition_keys", C.c_uint32),
+        ("radix_digit_bits", C.c_uint32),
+        ("composite_batch", C.c_uint32),
+        ("tile_order", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class BatchStats(C.Structure):
     """cgs_batch_stats: one cgs_execute_views_batched call."""
 
@@ -133,6 +147,13 @@ SIGNATURES = {
     "cgs_readback_rgba8": [_vp, _vp, _u32, _u32],
     "cgs_framebuffer_device_ptr": [_vp, C.POINTER(_vp), C.POINTER(_u32)],
     "cgs_kernel_times": [_vp, C.c_char_p, C.POINTER(C.c_double), _u32, C.POINTER(_u32)],
+    "cgs_set_tuning": [_vp, C.POINTER(Tuning)],
+    "cgs_get_tuning": [_vp, C.POINTER(Tuning)],
+    "cgs_autotune": [_vp, C.POINTER(CameraUniforms), C.POINTER(CompositorUniforms),
+                     C.POINTER(FrameParams), _u32, C.POINTER(Tuning), C.POINTER(C.c_double), _u32],
+    "cgs_tuning_candidates": [],
+    "cgs_tuning_candidate": [_u32, C.POINTER(Tuning)],
+    "cgs_batch_set_tuning": [_vp, C.POINTER(Tuning)],
     "cgs_batch_create": [_int, _u32, C.POINTER(_vp)],
     "cgs_batch_destroy": [_vp],
     "cgs_execute_views_batched": [_vp, _vp, _u32, _vp, _vp, _vp, _u32, C.POINTER(FrameParams),
@@ -196,6 +217,8 @@ def load() -> C.CDLL:
             fn.restype = None
         elif name == "cgs_mlp_param_count":
             fn.restype = _u64
+        elif name == "cgs_tuning_candidates":
+            fn.restype = _u32
         else:
             fn.restype = C.c_int
     _lib = lib

@@ -229,6 +229,7 @@ struct CgsBinArgs {
   uint32_t hist_words;       // 1024 (32-bit) or 2048 (64-bit)
   uint32_t sort_bits;        // 32-bit mode: bits of the packed key
   uint32_t tile_bits;        // 64-bit mode
+  uint32_t digit_bits;       // 32-bit mode: digit width of the sort (8 or 4); 64-bit: 8
 };
 
 // Optional per-launch timing: mark(name) records an event before a launch;
@@ -286,10 +287,15 @@ size_t cgs_radix_control_words(uint32_t cap);
 void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row_words,
                             uint32_t bins, uint32_t* hist, hipStream_t s);
 // Returns 1 if the sorted result is in the *_alt buffers, else 0.  With
-// hist_given the caller has already filled b.hist for the passes.
+// hist_given the caller has already filled b.hist for the passes (pass p's
+// 2^digit_bits bins at b.hist + p * 2^digit_bits).  part_keys: 16384 (one
+// 1024-thread block per CU) or 8192 (two 512-thread blocks per CU);
+// digit_bits: 8 (4 passes over 32 bits) or 4 (8 passes).  Every choice gives
+// the same stable order.
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
-                          CgsProf* prof = nullptr);
+                          CgsProf* prof = nullptr, uint32_t part_keys = 16384,
+                          uint32_t digit_bits = 8);
 void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
                         hipStream_t s);
@@ -307,7 +313,10 @@ struct CgsCompositeArgs {
   uint32_t* out;
   uint32_t* order;  // [tiles] dispatch order scratch (null: tile order)
 };
-void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof = nullptr);
+// batch: splats staged per compositor step (64, 128 or 256; the same pixels
+// for every choice)
+void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof = nullptr,
+                          uint32_t batch = 256);
 
 struct CgsBindArgs {
   const float* tet_rec;  // [T][12]: X0 xyz, inv_dm column-major

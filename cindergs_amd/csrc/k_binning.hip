@@ -254,28 +254,30 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
 // LDS atomic per digit.  Entries of the wave walk (consecutive entries of a few
 // splats, sharing their top digit): the top digit (tile bits, or the exponent
 // of a raw depth) is summed over the wave first, one atomic per distinct digit.
-__device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift) {
-  const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
+// pass p of a sort with db-bit digits: bins [p << db, (p + 1) << db)
+__device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift, uint32_t db) {
+  const uint32_t w = bits - shift >= db ? db : bits - shift;
   return (1u << w) - 1u;
 }
-__device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_t bits) {
-  for (uint32_t p = 0; p * 8u < bits; ++p)
-    atomicAdd(&s_hist[p * 256u + ((key >> (p * 8u)) & dmask(bits, p * 8u))], 1u);
+__device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_t bits,
+                                         uint32_t db) {
+  for (uint32_t p = 0; p * db < bits; ++p)
+    atomicAdd(&s_hist[(p << db) + ((key >> (p * db)) & dmask(bits, p * db, db))], 1u);
 }
 __device__ __forceinline__ void hist_wave(uint32_t* s_hist, uint32_t key, uint32_t bits,
-                                          bool store, uint32_t lane) {
-  const uint32_t passes = (bits + 7u) / 8u;
+                                          uint32_t db, bool store, uint32_t lane) {
+  const uint32_t passes = (bits + db - 1u) / db;
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t d = (key >> (p * 8u)) & dmask(bits, p * 8u);
+    const uint32_t d = (key >> (p * db)) & dmask(bits, p * db, db);
     if (p + 1u < passes) {
-      if (store) atomicAdd(&s_hist[p * 256u + d], 1u);
+      if (store) atomicAdd(&s_hist[(p << db) + d], 1u);
     } else {
       uint64_t m = __ballot(store);
       while (m) {
         const uint32_t leader = (uint32_t)__builtin_ctzll(m);
         const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)leader);
         const uint64_t same = __ballot(store && d == d0);
-        if (lane == leader) atomicAdd(&s_hist[p * 256u + d0], (uint32_t)__popcll(same));
+        if (lane == leader) atomicAdd(&s_hist[(p << db) + d0], (uint32_t)__popcll(same));
         m &= ~same;
       }
     }
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
   if (A.offsets_out && s < A.max_gaussians) A.offsets_out[s] = off;
   const bool hist = A.hist_partial != nullptr;
   const uint32_t hbits = A.mode64 ? 32u : A.sort_bits;
+  const uint32_t db = A.digit_bits == 4u ? 4u : 8u;
   BinSplat b;
   b.area = 0;
   if (c != 0u) b = load_bin_splat(A, s, active);
@@ -345,10 +348,10 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
             store_entry(A, cur, tile, kb, s);
             if (hist) {
               if (!A.mode64) {
-                hist_key(s_hist, (tile << A.depth_bits) | kb, hbits);
+                hist_key(s_hist, (tile << A.depth_bits) | kb, hbits, db);
               } else {
-                hist_key(s_hist, kb, 32u);
-                hist_key(s_hist + 1024, tile, A.tile_bits);
+                hist_key(s_hist, kb, 32u, 8u);
+                hist_key(s_hist + 1024, tile, A.tile_bits, 8u);
               }
             }
           }
@@ -379,10 +382,10 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     if (store) store_entry(A, pos, tile, kb, id);
     if (hist) {
       if (!A.mode64) {
-        hist_wave(s_hist, (tile << A.depth_bits) | kb, hbits, store, lane);
+        hist_wave(s_hist, (tile << A.depth_bits) | kb, hbits, db, store, lane);
       } else {
-        hist_wave(s_hist, kb, 32u, store, lane);
-        hist_wave(s_hist + 1024, tile, A.tile_bits, store, lane);
+        hist_wave(s_hist, kb, 32u, 8u, store, lane);
+        hist_wave(s_hist + 1024, tile, A.tile_bits, 8u, store, lane);
       }
     }
     wave_sync_lds();

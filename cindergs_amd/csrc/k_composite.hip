@@ -36,7 +36,7 @@
 
 namespace {
 
-constexpr int CT = 256;
+constexpr int CT = 256;  // threads: one per pixel of the 16x16 tile
 constexpr float ALPHA_MIN = 0.003921569f;  // the reference's literal
 
 // ellipse_intersects_tile_gpu (binning.rs:35-83) on an arbitrary pixel rectangle
@@ -95,12 +95,13 @@ __device__ __forceinline__ void blend(const Splat& s, float q2, float a, bool va
   alive = alive && !sat;
 }
 
-__device__ __forceinline__ Splat staged(const float4 (*st)[CT], uint32_t off) {
+template <int CB>
+__device__ __forceinline__ Splat staged(const float4 (*st)[CB], uint32_t off) {
   const char* p = reinterpret_cast<const char*>(&st[0][0]) + off;
   Splat s;
   s.g = *reinterpret_cast<const float4*>(p);
-  s.m = *reinterpret_cast<const float4*>(p + CT * 16);
-  s.b = *reinterpret_cast<const float*>(p + 2 * CT * 16);
+  s.m = *reinterpret_cast<const float4*>(p + CB * 16);
+  s.b = *reinterpret_cast<const float*>(p + 2 * CB * 16);
   return s;
 }
 
@@ -140,13 +141,16 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* ranges, uin
   }
 }
 
+// CB splats per batch (64, 128 or 256), staged by threads t < CB
+template <int CB>
 __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   // staged batch: plane 0 = Splat::g, 1 = Splat::m, 2 = (blue, -, -, -); the
   // quadrant lists hold byte offsets into a plane, so a splat's three reads
   // share one address register
-  __shared__ float4 s_stage[3][CT];
-  __shared__ uint64_t s_mask[4][CT / 64];     // [quadrant][loading wave]
-  __shared__ __align__(16) uint32_t s_list[4][CT];  // [quadrant] splats that can touch it
+  __shared__ float4 s_stage[3][CB];
+  __shared__ uint64_t s_mask[4][CB / 64];     // [quadrant][loading wave]
+  __shared__ __align__(16) uint32_t s_list[4][CB];  // [quadrant] splats that can touch it
+  const bool loader = threadIdx.x < (uint32_t)CB;
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
   if (blockIdx.x >= total_tiles) return;
@@ -172,21 +176,21 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, rc = r0;
   uint32_t g_ahead = 0u;
   const uint32_t last = empty ? rs : re - 1u;
-  if (!empty) {
+  if (!empty && loader) {
     const uint32_t g = A.sorted_ids[min(rs + t, last)];
     r0 = geom4[(size_t)g * 2];
     r1 = geom4[(size_t)g * 2 + 1];
     rc = color4[g];
-    g_ahead = A.sorted_ids[min(rs + t + (uint32_t)CT, last)];
+    g_ahead = A.sorted_ids[min(rs + t + (uint32_t)CB, last)];
   }
 
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   bool alive = inside;
-  for (uint32_t b0 = 0; b0 < n; b0 += CT) {
+  for (uint32_t b0 = 0; b0 < n; b0 += CB) {
     // ---- stage the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
     bool q0 = false, q1 = false, q2 = false, q3 = false;
-    if (e < re) {
+    if (loader && e < re) {
       // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
       const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
       s_stage[0][t] = make_float4(-0.5f * LOG2E * r0.z, -LOG2E * r0.w, -0.5f * LOG2E * r1.x, op);
@@ -202,24 +206,26 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
                              8u, A.width, A.height);
     }
     const uint64_t m0 = __ballot(q0), m1 = __ballot(q1), m2 = __ballot(q2), m3 = __ballot(q3);
-    if (lane == 0) {
+    if (lane == 0 && wave < (uint32_t)(CB / 64)) {
       s_mask[0][wave] = m0;
       s_mask[1][wave] = m1;
       s_mask[2][wave] = m2;
       s_mask[3][wave] = m3;
     }
     // the next batch's records are loaded while this batch is composited
-    const uint32_t en = e + CT;
-    r0 = geom4[(size_t)g_ahead * 2];
-    r1 = geom4[(size_t)g_ahead * 2 + 1];
-    rc = color4[g_ahead];
-    g_ahead = A.sorted_ids[min(en + (uint32_t)CT, last)];
+    if (loader) {
+      const uint32_t en = e + CB;
+      r0 = geom4[(size_t)g_ahead * 2];
+      r1 = geom4[(size_t)g_ahead * 2 + 1];
+      rc = color4[g_ahead];
+      g_ahead = A.sorted_ids[min(en + (uint32_t)CB, last)];
+    }
     __syncthreads();
 
     // ---- compact this wave's quadrant list (splat order kept) ----
     uint32_t cnt = 0;
 #pragma unroll
-    for (uint32_t v = 0; v < (uint32_t)(CT / 64); ++v) {
+    for (uint32_t v = 0; v < (uint32_t)(CB / 64); ++v) {
       const uint64_t mv = s_mask[wave][v];
       if ((mv >> lane) & 1ull) s_list[wave][cnt + cgs::mask_rank(mv)] = (v * 64u + lane) * 16u;
       cnt += (uint32_t)__popcll(mv);
@@ -236,8 +242,8 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
         const bool v1 = i + 1u < cnt, v2 = i + 2u < cnt, v3 = i + 3u < cnt;
         const uint32_t k0 = id4.x, k1 = v1 ? id4.y : id4.x, k2 = v2 ? id4.z : id4.x,
                        k3 = v3 ? id4.w : id4.x;
-        const Splat S0 = staged(s_stage, k0), S1 = staged(s_stage, k1);
-        const Splat S2 = staged(s_stage, k2), S3 = staged(s_stage, k3);
+        const Splat S0 = staged<CB>(s_stage, k0), S1 = staged<CB>(s_stage, k1);
+        const Splat S2 = staged<CB>(s_stage, k2), S3 = staged<CB>(s_stage, k3);
         const float qa = splat_q2(S0, pcx, pcy), qb = splat_q2(S1, pcx, pcy);
         const float qc = splat_q2(S2, pcx, pcy), qd = splat_q2(S3, pcx, pcy);
         const float aa = fminf(0.99f, S0.g.w * __builtin_amdgcn_exp2f(qa));
@@ -270,7 +276,8 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
 
 }  // namespace
 
-void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof) {
+void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof,
+                          uint32_t batch) {
   const uint32_t total = a.tiles_x * a.tiles_y;
   if (total == 0) return;
   CgsCompositeArgs A = a;
@@ -281,5 +288,10 @@ void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* pro
     A.order = nullptr;  // tile order (8K frames: one block would take too long)
   }
   if (prof) prof->mark("k_composite", s);
-  hipLaunchKernelGGL(k_composite, dim3(total), dim3(CT), 0, s, A);
+  if (batch == 64u)
+    hipLaunchKernelGGL(k_composite<64>, dim3(total), dim3(CT), 0, s, A);
+  else if (batch == 128u)
+    hipLaunchKernelGGL(k_composite<128>, dim3(total), dim3(CT), 0, s, A);
+  else
+    hipLaunchKernelGGL(k_composite<256>, dim3(total), dim3(CT), 0, s, A);
 }

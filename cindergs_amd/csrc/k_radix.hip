@@ -16,16 +16,14 @@
 
 namespace {
 
-constexpr int RT = 1024;         // threads (16 wave64s)
 constexpr int RI = 16;           // keys per thread
-constexpr int RP = RT * RI;      // keys per partition (8192)
-constexpr int RW = RT / 64;      // waves
-constexpr int RADIX = 256;
-constexpr int MAX_PASSES = 4;
+constexpr int RP_MIN = 512 * RI; // smallest partition (status arrays are sized for it)
+constexpr int RADIX = 256;       // digits of the widest digit (8 bits)
+constexpr int MAX_PASSES = 4;    // passes of 8-bit digits over 32 bits
 constexpr int HIST_BLOCKS = 128; // partial histograms of the standalone sort
 
-__device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift) {
-  const uint32_t w = bits - shift >= 8u ? 8u : bits - shift;
+__device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift, uint32_t db) {
+  const uint32_t w = bits - shift >= db ? db : bits - shift;
   return (1u << w) - 1u;
 }
 
@@ -33,9 +31,11 @@ __device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift) {
 // block (no global atomics: same-address device atomics serialise).
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* d_n,
                                                     uint32_t n_static, uint32_t cap,
-                                                    uint32_t bits, uint32_t* partial) {
+                                                    uint32_t bits, uint32_t db,
+                                                    uint32_t* partial) {
   __shared__ uint32_t s_h[MAX_PASSES * RADIX];
-  const uint32_t passes = (bits + 7u) / 8u;
+  const uint32_t passes = (bits + db - 1u) / db;
+  const uint32_t radix = 1u << db;
   for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += 256u) s_h[i] = 0u;
   __syncthreads();
   const uint32_t n = min(d_n ? *d_n : n_static, cap);
@@ -49,17 +49,16 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
     const uint32_t k = valid ? keys[i] : 0u;
     const uint64_t vmask = __ballot(valid);
     for (uint32_t p = 0; p < passes; ++p) {
-      const uint32_t sh = p * 8u;
-      const uint32_t d = (k >> sh) & digit_mask(bits, sh);
+      const uint32_t sh = p * db;
+      const uint32_t d = (k >> sh) & digit_mask(bits, sh, db);
       uint64_t m = vmask;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (uint32_t b = 0; b < db; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bb = __ballot(bit);
         m &= bit ? bb : ~bb;
       }
       if (valid && cgs::mask_rank(m) == 0u)
-        atomicAdd(&s_h[p * RADIX + d], (uint32_t)__popcll(m));
+        atomicAdd(&s_h[p * radix + d], (uint32_t)__popcll(m));
     }
   }
   __syncthreads();
@@ -114,12 +113,17 @@ struct PassArgs {
   uint32_t* ticket;
 };
 
+// One pass over partitions of RT*RI keys with DB-bit digits.
+template <int RT, int DB>
 __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
+  constexpr int RP = RT * RI;
+  constexpr int RW = RT / 64;
+  constexpr int NB = 1 << DB;  // digits
   __shared__ uint32_t s_keys[RP];
   __shared__ uint32_t s_vals[RP];
-  __shared__ uint32_t s_whist[RW][RADIX];
-  __shared__ uint32_t s_lstart[RADIX];
-  __shared__ uint32_t s_gadj[RADIX];
+  __shared__ uint32_t s_whist[RW][NB];
+  __shared__ uint32_t s_lstart[NB];
+  __shared__ uint32_t s_gadj[NB];
   __shared__ uint32_t s_tmp[RW];
   __shared__ uint32_t s_part;
 
@@ -130,12 +134,12 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   // is a same-address device atomic); the tickets taken are exactly 0..P-1
   if (blockIdx.x * (uint32_t)RP >= n) return;
   if (t == 0) s_part = atomicAdd(A.ticket, 1u);
-  for (uint32_t i = t; i < (uint32_t)(RW * RADIX); i += RT) (&s_whist[0][0])[i] = 0u;
+  for (uint32_t i = t; i < (uint32_t)(RW * NB); i += RT) (&s_whist[0][0])[i] = 0u;
   __syncthreads();
   const uint32_t part = s_part;
   const uint32_t base = part * (uint32_t)RP;
   const uint32_t cnt_part = min((uint32_t)RP, n - base);
-  const uint32_t dmask = digit_mask(A.bits, A.shift);
+  const uint32_t dmask = digit_mask(A.bits, A.shift, (uint32_t)DB);
 
   // rank: wave w owns keys [base + w*1024, base + (w+1)*1024), item i of lane l
   // at offset i*64 + l, so items are visited in input order
@@ -162,7 +166,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
     // this lane's (sx = all ones when the bit is set, so ~(bb ^ sx) keeps them)
     uint32_t mlo = (uint32_t)__ballot(valid), mhi = (uint32_t)(__ballot(valid) >> 32);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DB; ++b) {
       const uint32_t sx = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
       const uint64_t bb = __ballot(sx != 0u);
       mlo &= ~((uint32_t)bb ^ sx);
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
   // per digit (thread t == digit t < 256): wave prefixes, partition count
   uint32_t dtotal = 0;
-  if (t < (uint32_t)RADIX) {
+  if (t < (uint32_t)NB) {
 #pragma unroll
     for (int w = 0; w < RW; ++w) {
       const uint32_t c = s_whist[w][t];
@@ -192,13 +196,13 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   }
   uint32_t htot;
   const uint32_t lstart = cgs::block_exclusive_scan<RT>(dtotal, s_tmp, htot);
-  const uint32_t hv = t < (uint32_t)RADIX ? A.hist[t] : 0u;
+  const uint32_t hv = t < (uint32_t)NB ? A.hist[t] : 0u;
   uint32_t hsum;
   const uint32_t dbase = cgs::block_exclusive_scan<RT>(hv, s_tmp, hsum);
-  if (t < (uint32_t)RADIX) {
+  if (t < (uint32_t)NB) {
     s_lstart[t] = lstart;
     // look-back for this digit over the partitions before this one
-    const uint32_t excl = cgs::lookback_u32(A.status + t, part, RADIX, dtotal);
+    const uint32_t excl = cgs::lookback_u32(A.status + t, part, NB, dtotal);
     s_gadj[t] = dbase + excl - lstart;
   }
   __syncthreads();
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
 }  // namespace
 
-uint32_t cgs_radix_num_parts(uint32_t cap) { return (cap + (uint32_t)RP - 1u) / (uint32_t)RP; }
+uint32_t cgs_radix_num_parts(uint32_t cap) { return (cap + (uint32_t)RP_MIN - 1u) / (uint32_t)RP_MIN; }
 uint32_t cgs_radix_hist_blocks() { return HIST_BLOCKS; }
 
 size_t cgs_radix_control_words(uint32_t cap) {
@@ -247,17 +251,21 @@ void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row
 
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
-                          CgsProf* prof) {
+                          CgsProf* prof, uint32_t part_keys, uint32_t digit_bits) {
   if (bits == 0 || b.cap == 0) return 0;
   if (bits > 32) bits = 32;
-  const uint32_t passes = (bits + 7u) / 8u;
-  const uint32_t parts = cgs_radix_num_parts(b.cap);
+  const uint32_t db = digit_bits == 4u ? 4u : 8u;
+  const uint32_t radix = 1u << db;
+  const uint32_t passes = (bits + db - 1u) / db;
+  const uint32_t parts_alloc = cgs_radix_num_parts(b.cap);  // status rows per pass
+  const uint32_t pk = part_keys == 8192u ? 8192u : 16384u;
+  const uint32_t grid = (b.cap + pk - 1u) / pk;
   if (!hist_given) {
     if (prof) prof->mark("k_radix_hist", s);
     hipLaunchKernelGGL(k_radix_hist, dim3(HIST_BLOCKS), dim3(256), 0, s, b.keys, d_n, n_static,
-                       b.cap, bits, b.hist_partial);
+                       b.cap, bits, db, b.hist_partial);
     if (prof) prof->mark("k_hist_reduce", s);
-    cgs_launch_hist_reduce(b.hist_partial, HIST_BLOCKS, MAX_PASSES * RADIX, passes * RADIX,
+    cgs_launch_hist_reduce(b.hist_partial, HIST_BLOCKS, MAX_PASSES * RADIX, passes * radix,
                            b.hist, s);
   }
   const uint32_t* kin = b.keys;
@@ -273,12 +281,22 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
     A.n_static = n_static;
     A.cap = b.cap;
     A.bits = bits;
-    A.shift = p * 8u;
-    A.hist = b.hist + p * RADIX;
-    A.status = b.status + (size_t)p * parts * RADIX;
+    A.shift = p * db;
+    A.hist = b.hist + p * radix;
+    A.status = b.status + (size_t)p * parts_alloc * radix;
     A.ticket = b.tickets + p;
     if (prof) prof->mark("k_radix_onesweep", s);
-    hipLaunchKernelGGL(k_radix_onesweep, dim3(parts), dim3(RT), 0, s, A);
+    if (db == 8u) {
+      if (pk == 16384u)
+        hipLaunchKernelGGL((k_radix_onesweep<1024, 8>), dim3(grid), dim3(1024), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_radix_onesweep<512, 8>), dim3(grid), dim3(512), 0, s, A);
+    } else {
+      if (pk == 16384u)
+        hipLaunchKernelGGL((k_radix_onesweep<1024, 4>), dim3(grid), dim3(1024), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_radix_onesweep<512, 4>), dim3(grid), dim3(512), 0, s, A);
+    }
     kin = A.keys_out;
     vin = A.vals_out;
     in_alt ^= 1;

@@ -142,6 +142,7 @@ struct FrameArena {
   uint32_t* sorted_ids = nullptr;
   bool capture = false;
   bool captured = false;
+  cgs_tuning tune = {16384u, 8u, 256u, 1u};
 
   void release_all() {
     DevBuf* bufs[] = {&geom, &color, &extents, &e_keys, &e_ids, &e_depths, &e_alt_k,
@@ -205,7 +206,7 @@ namespace {
 // the per-partition histogram rows after it are fully written by the binning.
 struct FrameLayout {
   size_t total;                   // [entries, survivors]
-  size_t rA_tickets, rB_tickets;  // 4 each
+  size_t rA_tickets, rB_tickets;  // 8 each
   size_t histA, histB;            // 1024 each, contiguous (64-bit path reduces both at once)
   size_t block_sums, block_surv;  // [bin_parts] each, fully written by the binning
   size_t rA_status, rB_status;
@@ -218,11 +219,11 @@ struct FrameLayout {
 FrameLayout frame_layout(uint32_t max_g, uint32_t cap) {
   FrameLayout L;
   L.total = 2;
-  L.rA_tickets = 4;
-  L.rB_tickets = 8;
-  L.histA = 12;
-  L.histB = 12 + 1024;
-  size_t w = 12 + 2048;
+  L.rA_tickets = 4;   // 8 words: up to 8 passes of 4-bit digits
+  L.rB_tickets = 12;
+  L.histA = 20;
+  L.histB = 20 + 1024;
+  size_t w = 20 + 2048;
   if (w & 1) w++;
   L.bin_parts = cgs_bin_num_parts(max_g);
   L.block_sums = w;
@@ -381,10 +382,12 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   if (sort_bits > 32) sort_bits = 32;
   B.sort_bits = sort_bits;
   B.tile_bits = tile_bits;
+  const uint32_t db = a.mode64 ? 8u : a.tune.radix_digit_bits;
+  B.digit_bits = db;
   cgs_launch_bin(B, s, prof);
   // digit histograms of every pass, summed over the binning partitions
-  const uint32_t hist_bins =
-      a.mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u) : 256u * ((sort_bits + 7u) / 8u);
+  const uint32_t hist_bins = a.mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u)
+                                      : (1u << db) * ((sort_bits + db - 1u) / db);
   prof->mark("k_hist_reduce", s);
   cgs_launch_hist_reduce(ctrl + L.bin_hist, L.bin_parts, B.hist_words, hist_bins, ctrl + L.histA,
                          s);
@@ -401,7 +404,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
     R.status = ctrl + L.rA_status;
     R.tickets = ctrl + L.rA_tickets;
     R.cap = cap;
-    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, s, prof);
+    int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, s, prof,
+                                    a.tune.radix_partition_keys, db);
     a.sorted_keys = alt ? R.keys_alt : R.keys;
     a.sorted_ids = alt ? R.vals_alt : R.vals;
     prof->mark("k_tile_ranges", s);
@@ -420,7 +424,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
     A.status = ctrl + L.rA_status;
     A.tickets = ctrl + L.rA_tickets;
     A.cap = cap;
-    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, true, s, prof);
+    int altA = cgs_launch_radix_sort(A, d_n, 0, 32, true, true, s, prof,
+                                     a.tune.radix_partition_keys, 8u);
     const uint32_t* perm = altA ? A.vals_alt : A.vals;
     uint32_t* gk = altA ? P<uint32_t>(a.e_depths) : P<uint32_t>(a.e_alt_k);
     uint32_t* gv = altA ? P<uint32_t>(a.e_perm) : P<uint32_t>(a.e_alt_v);
@@ -438,7 +443,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
     Bt.status = ctrl + L.rB_status;
     Bt.tickets = ctrl + L.rB_tickets;
     Bt.cap = cap;
-    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, s, prof);
+    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, s, prof,
+                                     a.tune.radix_partition_keys, 8u);
     a.sorted_keys = altB ? Bt.keys_alt : Bt.keys;
     a.sorted_ids = altB ? Bt.vals_alt : Bt.vals;
     prof->mark("k_tile_ranges", s);
@@ -470,8 +476,8 @@ int launch_stage4(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   for (int i = 0; i < 4; ++i) A.bg[i] = comp->background_color[i];
   A.out = out;
   TRY(ensure_words(a.comp_order, g.T));
-  A.order = P<uint32_t>(a.comp_order);
-  cgs_launch_composite(A, s, prof);
+  A.order = a.tune.tile_order ? P<uint32_t>(a.comp_order) : nullptr;
+  cgs_launch_composite(A, s, prof, a.tune.composite_batch);
   HIPCHK(hipGetLastError());
   return CGS_OK;
 }
@@ -644,6 +650,17 @@ int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t
   prof->mark("k_project", s);
   cgs_launch_project(PA, s);
   HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
+int check_tuning(const cgs_tuning* t) {
+  if (t->radix_partition_keys != 16384u && t->radix_partition_keys != 8192u)
+    return fail(CGS_ERR_INVALID_ARG, "radix_partition_keys must be 16384 or 8192");
+  if (t->radix_digit_bits != 8u && t->radix_digit_bits != 4u)
+    return fail(CGS_ERR_INVALID_ARG, "radix_digit_bits must be 8 or 4");
+  if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u)
+    return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128 or 256");
+  if (t->tile_order > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_order must be 0 or 1");
   return CGS_OK;
 }
 
@@ -1002,6 +1019,95 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
     if (ms) ms[i] = c->prof_ms[i];
   }
   *count = n;
+  return CGS_OK;
+}
+
+// ---- launch tuning ----
+
+uint32_t cgs_tuning_candidates(void) { return 24u; }
+
+int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
+  if (!out || i >= 24u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  static const uint32_t parts[2] = {16384u, 8192u}, digits[2] = {8u, 4u},
+                        batches[3] = {256u, 128u, 64u}, orders[2] = {1u, 0u};
+  out->radix_partition_keys = parts[i % 2u];
+  out->radix_digit_bits = digits[(i / 2u) % 2u];
+  out->composite_batch = batches[(i / 4u) % 3u];
+  out->tile_order = orders[i / 12u];
+  return CGS_OK;
+}
+
+int cgs_set_tuning(cgs_context* c, const cgs_tuning* t) {
+  if (!c || !t) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  TRY(check_tuning(t));
+  c->fa.tune = *t;
+  return CGS_OK;
+}
+
+int cgs_get_tuning(cgs_context* c, cgs_tuning* t) {
+  if (!c || !t) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  *t = c->fa.tune;
+  return CGS_OK;
+}
+
+int cgs_autotune(cgs_context* c, const cgs_camera_uniforms* cam,
+                 const cgs_compositor_uniforms* comp, const cgs_frame_params* prm, uint32_t reps,
+                 cgs_tuning* best, double* ms, uint32_t n_ms) {
+  TRY(ensure_ctx(c));
+  if (!cam || !comp || !prm) return fail(CGS_ERR_INVALID_ARG, "null frame argument");
+  if (reps == 0) reps = 1;
+  cgs_frame_params p = *prm;
+  p.force_stage3 = 1;
+  p.profile_stages = 0;
+  const cgs_tuning start = c->fa.tune;
+  const FrameGeom g = frame_geom(comp);
+  // the current setting's frame is the reference every candidate must equal
+  cgs_stage_timings T;
+  TRY(cgs_execute_frame(c, cam, comp, &p, &T));
+  const uint64_t ref_entries = T.total_tile_entries;
+  std::vector<uint8_t> ref((size_t)g.W * g.H * 4), got(ref.size());
+  std::vector<uint32_t> ref_ranges((size_t)g.T * 2), got_ranges(ref_ranges.size());
+  TRY(cgs_readback_rgba8(c, ref.data(), g.W, g.H));
+  if (c->n_rest) {
+    HIPCHK(hipMemcpy(ref_ranges.data(), c->fa.ranges.p, ref_ranges.size() * 4,
+                     hipMemcpyDeviceToHost));
+  }
+  const uint32_t n = cgs_tuning_candidates();
+  double best_ms = 1e300;
+  cgs_tuning best_t = start;
+  for (uint32_t i = 0; i < n; ++i) {
+    cgs_tuning cand;
+    TRY(cgs_tuning_candidate(i, &cand));
+    c->fa.tune = cand;
+    int rc = cgs_execute_frame(c, cam, comp, &p, &T);
+    if (rc == CGS_OK) rc = cgs_readback_rgba8(c, got.data(), g.W, g.H);
+    if (rc == CGS_OK && c->n_rest) {
+      if (hipMemcpy(got_ranges.data(), c->fa.ranges.p, got_ranges.size() * 4,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(CGS_ERR_EXECUTION, "range readback failed");
+    }
+    if (rc != CGS_OK) {
+      c->fa.tune = start;
+      return rc;
+    }
+    if (T.total_tile_entries != ref_entries || got != ref ||
+        (c->n_rest && got_ranges != ref_ranges)) {
+      c->fa.tune = start;
+      return fail(CGS_ERR_EXECUTION, "a tuning candidate changed the frame (internal error)");
+    }
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    for (uint32_t r = 0; r < reps; ++r) TRY(cgs_execute_frame(c, cam, comp, &p, &T));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    const double t_ms = elapsed(c->ev[0], c->ev[1]) / reps;
+    if (ms && i < n_ms) ms[i] = t_ms;
+    if (t_ms < best_ms) {
+      best_ms = t_ms;
+      best_t = cand;
+    }
+  }
+  c->fa.tune = best_t;
+  if (best) *best = best_t;
   return CGS_OK;
 }
 
@@ -2109,6 +2215,14 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     St.reruns++;
   }
   if (stats) *stats = St;
+  return CGS_OK;
+}
+
+int cgs_batch_set_tuning(cgs_batch* b, const cgs_tuning* t) {
+  if (!b || !t) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
+  TRY(check_tuning(t));
+  for (FrameArena& a : b->lanes) a.tune = *t;
+  batch_drop_graph(b);  // the launches change
   return CGS_OK;
 }
 

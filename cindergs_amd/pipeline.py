@@ -170,6 +170,25 @@ class GpuPipelineState:
         check(self._lib.cgs_execute_frame(self._h, C.byref(camera), C.byref(compositor),
                                           C.byref(params), C.byref(timings)))
 
+    # ---- launch tuning ------------------------------------------------------
+    def set_tuning(self, tuning: "_lib.Tuning"):
+        check(self._lib.cgs_set_tuning(self._h, C.byref(tuning)))
+
+    def get_tuning(self) -> "_lib.Tuning":
+        t = _lib.Tuning()
+        check(self._lib.cgs_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def autotune(self, camera, compositor, params: FrameParams, reps: int = 3):
+        """Time every tuning candidate on this view (each checked to give the
+        same frame) and keep the fastest.  Returns (best, [(candidate, ms)])."""
+        n = tuning_candidates_count()
+        ms = (C.c_double * n)()
+        best = _lib.Tuning()
+        check(self._lib.cgs_autotune(self._h, C.byref(camera), C.byref(compositor),
+                                     C.byref(params), reps, C.byref(best), ms, n))
+        return best, [(tuning_candidate(i), ms[i]) for i in range(n)]
+
     def kernel_times(self):
         """[(kernel name, ms)] of the last frame run with profile_stages=2."""
         names = C.create_string_buffer(32 * 64)
@@ -322,6 +341,16 @@ class DeviceBuffer:
         self.ptr = None
 
 
+def tuning_candidates_count() -> int:
+    return int(_lib.load().cgs_tuning_candidates())
+
+
+def tuning_candidate(i: int) -> "_lib.Tuning":
+    t = _lib.Tuning()
+    check(_lib.load().cgs_tuning_candidate(i, C.byref(t)))
+    return t
+
+
 class ViewBatch:
     """Many views of several resident scenes from one captured HIP graph.
 
@@ -356,6 +385,9 @@ class ViewBatch:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_tuning(self, tuning: "_lib.Tuning"):
+        check(self._lib.cgs_batch_set_tuning(self._h, C.byref(tuning)))
 
     def execute(self, states, view_scene, cameras, compositors, params: FrameParams):
         """Render view v of ``states[view_scene[v]]`` (synchronised GpuPipelineStates)."""
@@ -396,5 +428,6 @@ class ViewBatch:
 __all__ = [
     "CameraUniforms", "CompositorUniforms", "FrameParams", "StageTimings", "GpuError",
     "GaussianSceneSoa", "QuantizedGaussianSceneSoa", "TetCage", "GpuPipelineState",
-    "DeviceBuffer", "ViewBatch", "padded_rgba8_row_u32",
+    "DeviceBuffer", "ViewBatch", "padded_rgba8_row_u32", "tuning_candidate",
+    "tuning_candidates_count",
 ]

@@ -140,6 +140,36 @@ int cgs_framebuffer_device_ptr(cgs_context* ctx, void** ptr, uint32_t* padded_ro
  * at kernel granularity): names[32*i] (NUL-terminated) and ms[i]. */
 int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, uint32_t* count);
 
+/* ---- launch tuning ----
+ * Launch shapes that leave every output bit-identical (keys, ranges, RGBA8):
+ * radix partition size and digit width of the Stage 3 sort, splats staged
+ * per compositor step, and the heaviest-tiles-first compositor order.  The
+ * reference fixes its shapes in code (radix_sort.rs:292-368: 4-bit digits,
+ * 256-key blocks; compositor.rs:721-883: batches of 256); here they are
+ * per-context settings that cgs_autotune picks by timing.  Defaults:
+ * {16384, 8, 256, 1}. */
+typedef struct cgs_tuning {
+  uint32_t radix_partition_keys; /* 16384 or 8192 keys per sort partition */
+  uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
+  uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step */
+  uint32_t tile_order;           /* 1: heaviest tiles composited first */
+} cgs_tuning;
+
+int cgs_set_tuning(cgs_context* ctx, const cgs_tuning* tuning);
+int cgs_get_tuning(cgs_context* ctx, cgs_tuning* tuning);
+
+/* Times every cgs_tuning choice on `reps` frames of the given view (after one
+ * untimed frame each), checks that each gives the same framebuffer and tile
+ * entries as the current setting, and keeps the fastest.  ms[i] (optional,
+ * n_ms entries) gets the mean device time of choice i in the order of
+ * cgs_tuning_candidate; *best gets the chosen setting. */
+int cgs_autotune(cgs_context* ctx, const cgs_camera_uniforms* camera,
+                 const cgs_compositor_uniforms* compositor, const cgs_frame_params* params,
+                 uint32_t reps, cgs_tuning* best, double* ms, uint32_t n_ms);
+/* candidate i (< cgs_tuning_candidates()) of the autotuner's sweep */
+uint32_t cgs_tuning_candidates(void);
+int cgs_tuning_candidate(uint32_t i, cgs_tuning* out);
+
 /* ---- batched multi-view rendering ----
  * The reference renders one view per GpuPipelineState::execute_frame_gpu call
  * (gpu_state.rs:613-1243) from a per-view loop in its callers
@@ -174,6 +204,9 @@ int cgs_execute_views_batched(cgs_batch* batch, cgs_context* const* scenes, uint
                               const uint32_t* view_scene, const cgs_camera_uniforms* cameras,
                               const cgs_compositor_uniforms* compositors, uint32_t n_views,
                               const cgs_frame_params* params, cgs_batch_stats* stats);
+
+/* launch tuning of every lane (see cgs_tuning) */
+int cgs_batch_set_tuning(cgs_batch* batch, const cgs_tuning* tuning);
 
 /* View v of the last batch: device framebuffer and row stride in u32 words,
  * tightly packed readback, and its tile-entry / survivor counts. */

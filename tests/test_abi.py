@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(ROOT, "include", "cindergs.h")).read()
-    return sorted(set(re.findall(r"^(?:int|void|uint64_t|const char\*)\s+(cgs_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|uint32_t|uint64_t|const char\*)\s+(cgs_\w+)\s*\(", text, re.M)))
 
 
 def test_header_and_binding_agree():
@@ -55,3 +55,20 @@ def test_no_device_maps_to_noadapter():
     with pytest.raises(_lib.GpuError) as ei:
         _lib.check(code)
     assert ei.value.kind == "NoAdapter"
+
+
+def test_tuning_candidates_cover_every_knob():
+    """The autotuner's sweep (host-only entry points, no GPU needed): every
+    combination of partition size, digit width, compositor batch and tile
+    order exactly once."""
+    from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
+
+    n = tuning_candidates_count()
+    combos = {tuple(tuning_candidate(i).as_dict().values()) for i in range(n)}
+    assert n == len(combos) == 2 * 2 * 3 * 2
+    assert {c[0] for c in combos} == {16384, 8192}
+    assert {c[1] for c in combos} == {8, 4}
+    assert {c[2] for c in combos} == {256, 128, 64}
+    assert {c[3] for c in combos} == {0, 1}
+    assert tuning_candidate(0).as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
+                                             "composite_batch": 256, "tile_order": 1}

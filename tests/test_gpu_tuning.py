@@ -1,0 +1,97 @@
+"""Launch tuning (cgs_tuning, cgs_autotune) on the GPU.
+
+Bar: every tuning candidate (radix partition 16384/8192 keys, 8- or 4-bit
+digits, compositor batch 256/128/64, heaviest-tiles-first order on/off) gives
+bit-identical tile counts, scan offsets, sorted keys, sorted ids, tile ranges
+and RGBA8 to the default setting, which the parity tests hold to the oracle;
+the 64-bit key path (8K) likewise for the partition sizes; the autotuner keeps
+one of the candidates and the frame it renders is unchanged.
+"""
+import numpy as np
+import pytest
+
+from cindergs_amd import synthetic as syn
+from cindergs_amd._lib import StageTimings, Tuning
+from cindergs_amd.pipeline import FrameParams, tuning_candidate, tuning_candidates_count
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(gpu, cam, comp, n):
+    p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 1)
+    t = StageTimings()
+    gpu.set_capture(True)
+    gpu.execute_frame_resident(cam, comp, p, t)
+    gpu.set_capture(False)
+    W, H = comp.screen_width, comp.screen_height
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    s3 = gpu.read_stage3(n, T, with_counts=True)
+    return s3, gpu.readback_framebuffer_rgba8(W, H)
+
+
+def _same(a, b, label):
+    (s3a, ia), (s3b, ib) = a, b
+    assert s3a["n"] == s3b["n"], f"{label}: entries"
+    for k in ("counts", "offsets", "keys", "ids", "ranges"):
+        np.testing.assert_array_equal(s3a[k], s3b[k], err_msg=f"{label}: {k}")
+    assert np.array_equal(ia, ib), f"{label}: RGBA8"
+
+
+def test_every_candidate_gives_the_same_frame(gpu):
+    scene = syn.scene_clusters(150_000, k=41)
+    gpu.sync_scene_and_cages(scene, None)
+    cam, comp = syn.view_orbit(7, 1280, 720)
+    default = gpu.get_tuning()
+    assert default.as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
+                                 "composite_batch": 256, "tile_order": 1}
+    ref = _frame(gpu, cam, comp, scene.count)
+    n = tuning_candidates_count()
+    seen = set()
+    for i in range(n):
+        c = tuning_candidate(i)
+        seen.add(tuple(c.as_dict().values()))
+        print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
+        gpu.set_tuning(c)
+        _same(ref, _frame(gpu, cam, comp, scene.count), f"candidate {i}")
+    assert len(seen) == n == 24
+    gpu.set_tuning(default)
+
+
+def test_partition_sizes_on_64bit_keys(gpu):
+    scene = syn.scene_gaussian_ball(40_000, k=42)
+    gpu.sync_scene_and_cages(scene, None)
+    cam = syn.camera_look_at((0.0, 0.0, -3.0), (0, 0, 0), (0, 1, 0), 60.0, 7680, 4320)
+    comp = syn.compositor(7680, 4320, (0.0, 0.0, -3.0))
+    default = gpu.get_tuning()
+    ref = _frame(gpu, cam, comp, scene.count)
+    for pk, batch in ((8192, 128), (16384, 64)):
+        t = Tuning(pk, 8, batch, 0)
+        print(f"[case] 8K partition {pk} batch {batch}", flush=True)
+        gpu.set_tuning(t)
+        _same(ref, _frame(gpu, cam, comp, scene.count), f"8K partition {pk}")
+    gpu.set_tuning(default)
+
+
+def test_autotune_keeps_a_candidate_and_the_frame(gpu):
+    scene = syn.scene_clusters(120_000, k=43)
+    gpu.sync_scene_and_cages(scene, None)
+    cam, comp = syn.view_orbit(3, 960, 540)
+    p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    default = gpu.get_tuning()
+    ref = _frame(gpu, cam, comp, scene.count)
+    best, table = gpu.autotune(cam, comp, p, reps=2)
+    print(f"[case] autotune best {best.as_dict()}", flush=True)
+    assert any(c.as_dict() == best.as_dict() for c, _ in table)
+    assert all(ms > 0 for _, ms in table)
+    assert gpu.get_tuning().as_dict() == best.as_dict()
+    _same(ref, _frame(gpu, cam, comp, scene.count), "autotuned")
+    gpu.set_tuning(default)
+
+
+def test_invalid_tuning_is_rejected(gpu):
+    from cindergs_amd._lib import GpuError
+
+    for bad in (Tuning(4096, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
+                Tuning(16384, 8, 256, 2)):
+        with pytest.raises(GpuError):
+            gpu.set_tuning(bad)
