@@ -170,10 +170,13 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
     per_view_s = (time.perf_counter() - t) / reps
     print(f"# per-view: {1000 * per_view_s:.2f} ms per {n_views} views", file=sys.stderr, flush=True)
 
-    sweep = {}
-    best = None
-    for lanes in lanes_list:
+    from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
+
+    def run_batch(lanes, tuning, shared=True):
         with ViewBatch(0, lanes=lanes) as batch:
+            batch.set_shared_projection(shared)
+            if tuning is not None:
+                batch.set_tuning(tuning)
             batch.execute(states, view_scene, cams, comps, params)  # capture + arena sizing
             batch.execute(states, view_scene, cams, comps, params)
             t = time.perf_counter()
@@ -182,14 +185,35 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
                 stats = batch.execute(states, view_scene, cams, comps, params)
                 gpu_ms += stats.gpu_ms
             wall = (time.perf_counter() - t) / reps
-            sweep[lanes] = {"ms_per_batch": round(1000 * wall, 3),
-                            "gpu_ms_per_batch": round(gpu_ms / reps, 3),
-                            "views_per_s": round(n_views / wall, 1),
-                            "rebuilt": int(stats.graph_built)}
-            print(f"# batched lanes={lanes}: {1000 * wall:.2f} ms per {n_views} views",
-                  file=sys.stderr, flush=True)
-            if best is None or wall < best[1]:
-                best = (lanes, wall, int(stats.total_tile_entries))
+        return wall, gpu_ms / reps, stats
+
+    # autotuning of the batch: lanes with the default launch shapes, then the
+    # 8-bit-digit launch shapes at the best lane count
+    sweep = {}
+    best = None
+    for lanes in lanes_list:
+        wall, gms, stats = run_batch(lanes, None)
+        sweep[f"lanes={lanes}"] = round(1000 * wall, 3)
+        print(f"# batched lanes={lanes}: {1000 * wall:.2f} ms per {n_views} views",
+              file=sys.stderr, flush=True)
+        if best is None or wall < best[1]:
+            best = (lanes, wall, int(stats.total_tile_entries), None)
+    for i in range(tuning_candidates_count()):
+        tu = tuning_candidate(i)
+        if tu.radix_digit_bits != 8 or i == 0:
+            continue
+        wall, gms, stats = run_batch(best[0], tu)
+        key = "lanes={} part={} batch={} order={}".format(
+            best[0], tu.radix_partition_keys, tu.composite_batch, tu.tile_order)
+        sweep[key] = round(1000 * wall, 3)
+        print(f"# batched {key}: {1000 * wall:.2f} ms", file=sys.stderr, flush=True)
+        if wall < best[1]:
+            best = (best[0], wall, int(stats.total_tile_entries), tu)
+    # the same batch with one projection per view (the shared pass switched off)
+    wall_np, _, _ = run_batch(best[0], best[3], shared=False)
+    sweep["best, one projection per view"] = round(1000 * wall_np, 3)
+    print(f"# batched without the shared projection: {1000 * wall_np:.2f} ms", file=sys.stderr,
+          flush=True)
     for stt in states:
         stt.close()
     res = {"views": n_views, "reps": reps,
@@ -199,7 +223,8 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
            "batched_views_per_s": round(n_views / best[1], 1),
            "speedup": round(per_view_s / best[1], 3),
            "lanes": best[0], "tile_entries_per_batch": best[2],
-           "lanes_sweep": sweep}
+           "tuning": best[3].as_dict() if best[3] is not None else "default",
+           "sweep_ms_per_batch": sweep}
     return res
 
 

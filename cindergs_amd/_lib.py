@@ -154,6 +154,7 @@ SIGNATURES = {
     "cgs_tuning_candidates": [],
     "cgs_tuning_candidate": [_u32, C.POINTER(Tuning)],
     "cgs_batch_set_tuning": [_vp, C.POINTER(Tuning)],
+    "cgs_batch_set_shared_projection": [_vp, _u32],
     "cgs_batch_create": [_int, _u32, C.POINTER(_vp)],
     "cgs_batch_destroy": [_vp],
     "cgs_execute_views_batched": [_vp, _vp, _u32, _vp, _vp, _vp, _u32, C.POINTER(FrameParams),

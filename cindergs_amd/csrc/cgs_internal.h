@@ -203,6 +203,17 @@ struct CgsProjectArgs {
   uint32_t* extents; // [max]
 };
 
+// One view of a multi-view projection: camera (CameraUniforms words),
+// camera position, and the view's output records
+struct CgsViewProjection {
+  float camera[24];
+  float cam_pos[3];
+  float pad;
+  float* geom;        // [8*max]
+  float* color;       // [4*max]
+  uint32_t* extents;  // [max]
+};
+
 struct CgsBinArgs {
   const float* geom;
   const uint32_t* extents;
@@ -261,6 +272,10 @@ void cgs_launch_to_blocked_u16(const uint16_t* in, uint16_t* out, uint32_t n, ui
 void cgs_launch_sh_f16_blocked(const uint16_t* in, uint16_t* out, uint32_t n, uint32_t stride,
                                uint32_t rest, hipStream_t s);
 void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s);
+// Stage 2 of n_views <= 64 views of the scene in `a` (its camera, cam_pos and
+// outputs unused) into each view's records; `views` in device memory
+void cgs_launch_project_views(const CgsProjectArgs& a, const CgsViewProjection* views,
+                              uint32_t n_views, hipStream_t s);
 uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
 void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof = nullptr);
 void cgs_launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,

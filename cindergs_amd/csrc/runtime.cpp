@@ -143,6 +143,11 @@ struct FrameArena {
   bool capture = false;
   bool captured = false;
   cgs_tuning tune = {16384u, 8u, 256u, 1u};
+  // projected records read by Stages 3-4 when they live outside the arena
+  // (the view batch's multi-view projection); null: geom / color / extents
+  const float* geom_in = nullptr;
+  const float* color_in = nullptr;
+  const uint32_t* extents_in = nullptr;
 
   void release_all() {
     DevBuf* bufs[] = {&geom, &color, &extents, &e_keys, &e_ids, &e_depths, &e_alt_k,
@@ -354,8 +359,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
 
   CgsBinArgs B;
   memset(&B, 0, sizeof(B));
-  B.geom = P<float>(a.geom);
-  B.extents = P<uint32_t>(a.extents);
+  B.geom = a.geom_in ? a.geom_in : P<float>(a.geom);
+  B.extents = a.extents_in ? a.extents_in : P<uint32_t>(a.extents);
   B.split_count = split_count;
   B.num_rest = num_rest;
   B.max_gaussians = max_g;
@@ -464,8 +469,8 @@ int launch_stage4(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   }
   a.padded_row = g.padded_row;
   CgsCompositeArgs A;
-  A.geom = P<float>(a.geom);
-  A.color = P<float>(a.color);
+  A.geom = a.geom_in ? a.geom_in : P<float>(a.geom);
+  A.color = a.color_in ? a.color_in : P<float>(a.color);
   A.sorted_ids = a.sorted_ids ? a.sorted_ids : P<uint32_t>(a.e_ids);
   A.ranges = P<uint32_t>(a.ranges);
   A.width = g.W;
@@ -614,7 +619,30 @@ Stage3Key stage3_key(const cgs_context* c, const cgs_camera_uniforms* cam, const
   return key;
 }
 
-// Stage 2 of one view into the arena a (project_v2 dispatch, gpu_state.rs:882-948)
+// The Stage 2 arguments of the scene side of a frame (project_v2 dispatch,
+// gpu_state.rs:882-948); camera and outputs are filled by the caller
+int project_args(const cgs_context* c, const SceneFrame& F, const cgs_compositor_uniforms* comp,
+                 const cgs_frame_params* prm, CgsProjectArgs& PA) {
+  memset(&PA, 0, sizeof(PA));
+  PA.scene = F.S;
+  PA.bind_tet = c->has_cages ? P<uint32_t>(c->bind_tet) : nullptr;
+  PA.tet_rot = c->has_cages ? P<float>(c->rot) : nullptr;
+  PA.split_meta = c->has_cages ? P<uint32_t>(c->split_meta) : nullptr;
+  PA.split_count = F.split_count;
+  PA.mip = c->has_cages ? 0.0f : prm->mip_filter_3d_var;  // gpu_state.rs:894
+  PA.min_opacity = prm->min_opacity;
+  PA.min_radius = prm->min_radius_px;
+  PA.sh_degree = comp->sh_degree > 3 ? 3 : comp->sh_degree;
+  if (c->f16 && PA.sh_degree != c->f16_degree)
+    return fail(CGS_ERR_INVALID_ARG, "sh_degree must equal the quantized scene's degree");
+  PA.has_deformation = c->has_cages ? 1u : 0u;
+  PA.num_tets = c->num_tets;
+  PA.num_rest = F.N;
+  PA.max_gaussians = F.max_g;
+  return CGS_OK;
+}
+
+// Stage 2 of one view into the arena a
 int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t s,
            CgsProf* prof, const cgs_camera_uniforms* cam, const cgs_compositor_uniforms* comp,
            const cgs_frame_params* prm) {
@@ -626,24 +654,9 @@ int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t
   const FrameLayout L = frame_layout(max_g, a.entry_cap);
   TRY(ensure_words(a.ctrl3, L.words));
   CgsProjectArgs PA;
-  memset(&PA, 0, sizeof(PA));
-  PA.scene = F.S;
-  PA.bind_tet = c->has_cages ? P<uint32_t>(c->bind_tet) : nullptr;
-  PA.tet_rot = c->has_cages ? P<float>(c->rot) : nullptr;
-  PA.split_meta = c->has_cages ? P<uint32_t>(c->split_meta) : nullptr;
-  PA.split_count = F.split_count;
+  TRY(project_args(c, F, comp, prm, PA));
   memcpy(PA.camera, cam, 96);
   for (int i = 0; i < 3; ++i) PA.cam_pos[i] = comp->camera_position[i];
-  PA.mip = c->has_cages ? 0.0f : prm->mip_filter_3d_var;  // gpu_state.rs:894
-  PA.min_opacity = prm->min_opacity;
-  PA.min_radius = prm->min_radius_px;
-  PA.sh_degree = comp->sh_degree > 3 ? 3 : comp->sh_degree;
-  if (c->f16 && PA.sh_degree != c->f16_degree)
-    return fail(CGS_ERR_INVALID_ARG, "sh_degree must equal the quantized scene's degree");
-  PA.has_deformation = c->has_cages ? 1u : 0u;
-  PA.num_tets = c->num_tets;
-  PA.num_rest = F.N;
-  PA.max_gaussians = max_g;
   PA.geom = P<float>(a.geom);
   PA.color = P<float>(a.color);
   PA.extents = P<uint32_t>(a.extents);
@@ -1901,6 +1914,17 @@ struct cgs_batch {
   hipGraphExec_t exec = nullptr;
   uint64_t key = 0;
   CgsProf noprof;  // never on
+  // multi-view projection: every view's projected records (geom 8, color 4,
+  // extents 1 words per splat slot) and the per-view descriptors, grouped by
+  // (scene, SH degree) in runs of at most 64 views
+  bool shared_projection = true;
+  DevBuf proj, vproj;
+  std::vector<size_t> proj_off;  // words, per view
+  struct Group {
+    uint32_t scene, first, count, view0;  // descriptors [first, first + count)
+  };
+  std::vector<Group> groups;
+  std::vector<uint32_t> vproj_index;  // view -> descriptor
 };
 
 namespace {
@@ -1914,12 +1938,12 @@ void batch_drop_graph(cgs_batch* b) {
 }
 
 // every buffer a view of (max_g splat slots, T tiles) touches in Stages 2-4
-int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64) {
+int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_records) {
   init_entry_cap(a, max_g);
   const uint32_t cap = a.entry_cap;
-  TRY(ensure_words(a.geom, (size_t)8 * max_g));
-  TRY(ensure_words(a.color, (size_t)4 * max_g));
-  TRY(ensure_words(a.extents, max_g));
+  TRY(ensure_words(a.geom, own_records ? (size_t)8 * max_g : 8));
+  TRY(ensure_words(a.color, own_records ? (size_t)4 * max_g : 4));
+  TRY(ensure_words(a.extents, own_records ? max_g : 1));
   TRY(ensure_words(a.cnt_dbg, max_g));
   TRY(ensure_words(a.e_keys, cap));
   TRY(ensure_words(a.e_ids, cap));
@@ -1956,10 +1980,23 @@ int batch_enqueue_view(cgs_batch* b, cgs_context* c, const SceneFrame& F, FrameA
     cgs_launch_fill_u32(slot, 0u, 4, s);
     return launch_stage4(a, s, &b->noprof, g, comp, out);
   }
-  TRY(stage2(c, F, a, s, &b->noprof, cam, comp, prm));
-  TRY(launch_stage3(a, s, &b->noprof, g, cam->near_plane, F.N, F.split_count, F.max_g));
-  cgs_launch_copy_words(P<uint32_t>(a.ctrl3), slot, 4, s);
-  return launch_stage4(a, s, &b->noprof, g, comp, out);
+  if (b->shared_projection) {
+    // the view's records were written by the multi-view projection
+    uint32_t* rec = P<uint32_t>(b->proj) + b->proj_off[v];
+    a.geom_in = reinterpret_cast<const float*>(rec);
+    a.color_in = reinterpret_cast<const float*>(rec + (size_t)8 * F.max_g);
+    a.extents_in = rec + (size_t)12 * F.max_g;
+  } else {
+    TRY(stage2(c, F, a, s, &b->noprof, cam, comp, prm));
+  }
+  int rc = launch_stage3(a, s, &b->noprof, g, cam->near_plane, F.N, F.split_count, F.max_g);
+  if (rc == CGS_OK) {
+    cgs_launch_copy_words(P<uint32_t>(a.ctrl3), slot, 4, s);
+    rc = launch_stage4(a, s, &b->noprof, g, comp, out);
+  }
+  a.geom_in = a.color_in = nullptr;
+  a.extents_in = nullptr;
+  return rc;
 }
 
 int batch_capture(cgs_batch* b, cgs_context* const* scenes, const std::vector<SceneFrame>& SF,
@@ -1971,9 +2008,22 @@ int batch_capture(cgs_batch* b, cgs_context* const* scenes, const std::vector<Sc
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   g_no_alloc = true;
   int rc = CGS_OK;
-  hipError_t e = hipEventRecord(b->fork, b->stream);
-  for (uint32_t k = 0; k < L && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->lane_stream[k], b->fork, 0);
-  if (e != hipSuccess) rc = fail(CGS_ERR_RUNTIME, std::string("graph fork: ") + hipGetErrorString(e));
+  hipError_t e = hipSuccess;
+  if (b->shared_projection) {
+    for (const cgs_batch::Group& G : b->groups) {
+      if (rc != CGS_OK) break;
+      CgsProjectArgs PA;
+      rc = project_args(scenes[G.scene], SF[G.scene], &comps[G.view0], prm, PA);
+      if (rc == CGS_OK)
+        cgs_launch_project_views(PA, reinterpret_cast<const CgsViewProjection*>(b->vproj.p) + G.first,
+                                 G.count, b->stream);
+    }
+  }
+  if (rc == CGS_OK) e = hipEventRecord(b->fork, b->stream);
+  for (uint32_t k = 0; k < L && e == hipSuccess && rc == CGS_OK; ++k)
+    e = hipStreamWaitEvent(b->lane_stream[k], b->fork, 0);
+  if (e != hipSuccess && rc == CGS_OK)
+    rc = fail(CGS_ERR_RUNTIME, std::string("graph fork: ") + hipGetErrorString(e));
   for (uint32_t v = 0; v < n_views && rc == CGS_OK; ++v) {
     const uint32_t k = b->view_lane[v], si = view_scene[v];
     rc = batch_enqueue_view(b, scenes[si], SF[si], b->lanes[k], b->lane_stream[k], &cams[v],
@@ -2042,6 +2092,8 @@ void cgs_batch_destroy(cgs_batch* b) {
   for (FrameArena& a : b->lanes) a.release_all();
   release(b->fb);
   release(b->counters);
+  release(b->proj);
+  release(b->vproj);
   if (b->h_counters) (void)hipHostFree(b->h_counters);
   for (hipStream_t s : b->lane_stream)
     if (s) (void)hipStreamDestroy(s);
@@ -2133,6 +2185,52 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     return CGS_OK;
   }
 
+  // ---- multi-view projection: records per view, descriptors per group ----
+  std::vector<CgsViewProjection> desc;
+  if (b->shared_projection) {
+    b->proj_off.assign(n_views, 0);
+    size_t pw = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+      b->proj_off[v] = pw;
+      const uint32_t mg = SF[view_scene[v]].max_g;
+      pw += ((size_t)13 * mg + 63) / 64 * 64;
+    }
+    TRY(ensure_words(b->proj, pw));
+    b->groups.clear();
+    b->vproj_index.assign(n_views, 0);
+    // views of one scene and SH degree, in view order, runs of <= 64
+    std::vector<uint32_t> byg(n_views);
+    for (uint32_t v = 0; v < n_views; ++v) byg[v] = v;
+    std::stable_sort(byg.begin(), byg.end(), [&](uint32_t x, uint32_t y) {
+      const uint32_t dx = std::min(comps[x].sh_degree, 3u), dy = std::min(comps[y].sh_degree, 3u);
+      return view_scene[x] != view_scene[y] ? view_scene[x] < view_scene[y] : dx < dy;
+    });
+    for (uint32_t v : byg) {
+      const uint32_t si = view_scene[v];
+      if (SF[si].N == 0) continue;
+      const uint32_t deg = std::min(comps[v].sh_degree, 3u);
+      cgs_batch::Group* G = b->groups.empty() ? nullptr : &b->groups.back();
+      if (!G || G->scene != si || std::min(comps[G->view0].sh_degree, 3u) != deg ||
+          G->count == 64u) {
+        b->groups.push_back({si, (uint32_t)desc.size(), 0u, v});
+        G = &b->groups.back();
+      }
+      CgsViewProjection d;
+      memset(&d, 0, sizeof(d));
+      memcpy(d.camera, &cams[v], 96);
+      for (int i = 0; i < 3; ++i) d.cam_pos[i] = comps[v].camera_position[i];
+      uint32_t* rec = P<uint32_t>(b->proj) + b->proj_off[v];
+      const uint32_t mg = SF[si].max_g;
+      d.geom = reinterpret_cast<float*>(rec);
+      d.color = reinterpret_cast<float*>(rec + (size_t)8 * mg);
+      d.extents = rec + (size_t)12 * mg;
+      b->vproj_index[v] = (uint32_t)desc.size();
+      desc.push_back(d);
+      G->count++;
+    }
+    TRY(ensure_words(b->vproj, desc.size() * sizeof(CgsViewProjection) / 4 + 1));
+  }
+
   for (int attempt = 0;; ++attempt) {
     // ---- size every lane for its largest view ----
     for (uint32_t k = 0; k < L; ++k) {
@@ -2144,7 +2242,7 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
         T = std::max(T, G[v].T);
         m64 = m64 || G[v].T > 65535u;
       }
-      TRY(size_arena(b->lanes[k], mg, T, m64));
+      TRY(size_arena(b->lanes[k], mg, T, m64, !b->shared_projection));
     }
     // ---- the graph's fingerprint: every pointer and value it bakes in ----
     uint64_t h = FNV_OFFSET;
@@ -2177,8 +2275,14 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     }
     h = fnv_buf(h, b->fb);
     h = fnv_buf(h, b->counters);
+    h = fnv_u64(h, b->shared_projection ? 1u : 0u);
+    h = fnv_buf(h, b->proj);
+    h = fnv_buf(h, b->vproj);
     if (h == 0) h = 1;
     if (!b->exec || b->key != h) {
+      if (!desc.empty())
+        HIPCHK(hipMemcpy(b->vproj.p, desc.data(), desc.size() * sizeof(CgsViewProjection),
+                         hipMemcpyHostToDevice));
       TRY(batch_capture(b, scenes, SF, view_scene, cams, comps, n_views, prm));
       b->key = h;
       St.graph_built = 1;
@@ -2223,6 +2327,13 @@ int cgs_batch_set_tuning(cgs_batch* b, const cgs_tuning* t) {
   TRY(check_tuning(t));
   for (FrameArena& a : b->lanes) a.tune = *t;
   batch_drop_graph(b);  // the launches change
+  return CGS_OK;
+}
+
+int cgs_batch_set_shared_projection(cgs_batch* b, uint32_t enable) {
+  if (!b) return fail(CGS_ERR_INVALID_ARG, "null batch");
+  b->shared_projection = enable != 0;
+  batch_drop_graph(b);
   return CGS_OK;
 }
 

@@ -389,6 +389,10 @@ class ViewBatch:
     def set_tuning(self, tuning: "_lib.Tuning"):
         check(self._lib.cgs_batch_set_tuning(self._h, C.byref(tuning)))
 
+    def set_shared_projection(self, enable: bool):
+        """Stage 2 of all views of a scene in one pass over the scene (default)."""
+        check(self._lib.cgs_batch_set_shared_projection(self._h, int(enable)))
+
     def execute(self, states, view_scene, cameras, compositors, params: FrameParams):
         """Render view v of ``states[view_scene[v]]`` (synchronised GpuPipelineStates)."""
         n = len(cameras)

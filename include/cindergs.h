@@ -207,6 +207,11 @@ int cgs_execute_views_batched(cgs_batch* batch, cgs_context* const* scenes, uint
 
 /* launch tuning of every lane (see cgs_tuning) */
 int cgs_batch_set_tuning(cgs_batch* batch, const cgs_tuning* tuning);
+/* 1 (default): Stage 2 of all views of a scene runs as one pass over the
+ * scene (each splat's record read once, its SH once), writing every view's
+ * projected records (13 words per splat slot per view); 0: one projection per
+ * view in its lane.  The pixels are the same either way. */
+int cgs_batch_set_shared_projection(cgs_batch* batch, uint32_t enable);
 
 /* View v of the last batch: device framebuffer and row stride in u32 words,
  * tightly packed readback, and its tile-entry / survivor counts. */

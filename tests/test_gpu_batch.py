@@ -109,6 +109,28 @@ def test_batch_lane_counts_agree():
         st.close()
 
 
+def test_shared_projection_on_and_off_agree():
+    """Stage 2 as one pass over each scene (default) or one pass per view: the
+    same pixels and counts."""
+    scenes = [syn.scene_gaussian_ball(20_000, k=38), syn.scene_clusters(30_000, k=39)]
+    states = _states(scenes)
+    view_scene, cams, comps = syn.batch_views(9, 2, seed=15, resolutions=SMALL)
+    p = _params()
+    out = {}
+    for shared in (True, False):
+        with ViewBatch(0, lanes=3) as batch:
+            batch.set_shared_projection(shared)
+            st = batch.execute(states, view_scene, cams, comps, p)
+            print(f"[case] shared_projection={shared}: {st.as_dict()}", flush=True)
+            out[shared] = ([batch.readback_rgba8(v) for v in range(len(cams))],
+                           [batch.view_counts(v) for v in range(len(cams))])
+    for v in range(len(cams)):
+        assert np.array_equal(out[True][0][v], out[False][0][v]), f"view {v}"
+        assert out[True][1][v] == out[False][1][v], f"view {v} counts"
+    for st in states:
+        st.close()
+
+
 def test_batch_view_against_oracle(oracle_lib):
     import oracle
 
