@@ -250,10 +250,8 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
   }
 }
 
-// digit histograms of the emitted entries.  Entries of the thread walks: one
-// LDS atomic per digit.  Entries of the wave walk (consecutive entries of a few
-// splats, sharing their top digit): the top digit (tile bits, or the exponent
-// of a raw depth) is summed over the wave first, one atomic per distinct digit.
+// digit histograms of the emitted entries; entries of the thread walks: one
+// LDS atomic per digit (the lanes of a wave hold different splats)
 // pass p of a sort with db-bit digits: bins [p << db, (p + 1) << db)
 __device__ __forceinline__ uint32_t dmask(uint32_t bits, uint32_t shift, uint32_t db) {
   const uint32_t w = bits - shift >= db ? db : bits - shift;
@@ -264,15 +262,24 @@ __device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_
   for (uint32_t p = 0; p * db < bits; ++p)
     atomicAdd(&s_hist[(p << db) + ((key >> (p * db)) & dmask(bits, p * db, db))], 1u);
 }
-__device__ __forceinline__ void hist_wave(uint32_t* s_hist, uint32_t key, uint32_t bits,
-                                          uint32_t db, bool store, uint32_t lane) {
+// Histograms of the wave walk's stored entries.  The lanes of a chunk hold
+// tiles of one or a few splats, so the digits made only of a splat's depth
+// bits (the low `const_bits` of the key) are shared by a whole run of lanes:
+// the run's first lane adds the run's stored count once.  Digits with tile
+// bits get one atomic per entry, the top one summed over the wave first.
+__device__ __forceinline__ void hist_walk(uint32_t* s_hist, uint32_t key, uint32_t bits,
+                                          uint32_t const_bits, uint32_t db, bool store,
+                                          uint64_t Hs, bool head, uint64_t run, uint32_t lane) {
   const uint32_t passes = (bits + db - 1u) / db;
   for (uint32_t p = 0; p < passes; ++p) {
     const uint32_t d = (key >> (p * db)) & dmask(bits, p * db, db);
-    if (p + 1u < passes) {
+    if ((p + 1u) * db <= const_bits) {
+      const uint32_t c = head ? (uint32_t)__popcll(Hs & run) : 0u;
+      if (c != 0u) atomicAdd(&s_hist[(p << db) + d], c);
+    } else if (p + 1u < passes) {
       if (store) atomicAdd(&s_hist[(p << db) + d], 1u);
     } else {
-      uint64_t m = __ballot(store);
+      uint64_t m = Hs;
       while (m) {
         const uint32_t leader = (uint32_t)__builtin_ctzll(m);
         const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)leader);
@@ -368,24 +375,28 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     bool hit;
     const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
     const uint64_t Hm = __ballot(hit);
+    const bool act = base + lane < total;
     uint32_t pos = 0, id = 0, kb = 0;
+    if (act) kb = cgs_f2u(W.d1[sp].w);  // the splat's depth bits, valid on every active lane
     if (hit) {
       pos = W.acc[sp] + (uint32_t)__popcll(Hm & run_before(R, lane));
       id = W.box[sp].w;
-      kb = cgs_f2u(W.d1[sp].w);
     }
+    const bool head = ((R >> lane) & 1ull) && act;
+    const uint64_t run = head ? run_of_head(R, lane) : 0ull;
     wave_sync_lds();
-    if (((R >> lane) & 1ull) && base + lane < total)
-      W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
+    if (head) W.acc[sp] += (uint32_t)__popcll(Hm & run);
     const bool store = hit && pos < A.cap;
+    const uint64_t Hs = __ballot(store);
     const uint32_t tile = ty * A.tiles_x + tx;
     if (store) store_entry(A, pos, tile, kb, id);
     if (hist) {
       if (!A.mode64) {
-        hist_wave(s_hist, (tile << A.depth_bits) | kb, hbits, db, store, lane);
+        hist_walk(s_hist, (tile << A.depth_bits) | kb, hbits, A.depth_bits, db, store, Hs, head,
+                  run, lane);
       } else {
-        hist_wave(s_hist, kb, 32u, 8u, store, lane);
-        hist_wave(s_hist + 1024, tile, A.tile_bits, 8u, store, lane);
+        hist_walk(s_hist, kb, 32u, 32u, 8u, store, Hs, head, run, lane);
+        hist_walk(s_hist + 1024, tile, A.tile_bits, 0u, 8u, store, Hs, head, run, lane);
       }
     }
     wave_sync_lds();

@@ -1925,6 +1925,10 @@ struct cgs_batch {
   };
   std::vector<Group> groups;
   std::vector<uint32_t> vproj_index;  // view -> descriptor
+  // tile entries of each view in the last replay of the same views, used to
+  // deal the views to lanes by their measured weight
+  uint64_t views_hash = 0;
+  std::vector<uint32_t> last_entries;
 };
 
 namespace {
@@ -2136,13 +2140,25 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
   }
 
   // ---- views -> lanes (longest first onto the least loaded lane) ----
+  // weight of a view in tile-entry units: its entries (measured in the last
+  // replay of the same views, else estimated as 3 per splat), plus pixels / 2
+  // and splats / 5 for the per-pixel and per-splat work
+  uint64_t vh = FNV_OFFSET;
+  vh = fnv_u64(vh, n_views);
+  vh = fnv_bytes(vh, view_scene, 4 * (size_t)n_views);
+  vh = fnv_bytes(vh, cams, sizeof(cgs_camera_uniforms) * n_views);
+  vh = fnv_bytes(vh, comps, sizeof(cgs_compositor_uniforms) * n_views);
+  for (uint32_t i = 0; i < n_scenes; ++i) vh = fnv_u64(vh, SF[i].max_g);
+  const bool measured = vh == b->views_hash && b->last_entries.size() == n_views;
   const uint32_t L = (uint32_t)b->lanes.size();
   std::vector<FrameGeom> G(n_views);
   std::vector<double> cost(n_views);
   std::vector<uint32_t> order(n_views);
   for (uint32_t v = 0; v < n_views; ++v) {
     G[v] = frame_geom(&comps[v]);
-    cost[v] = (double)SF[view_scene[v]].max_g + 0.5 * (double)G[v].W * G[v].H;
+    const double mg = (double)SF[view_scene[v]].max_g;
+    const double e = measured ? (double)b->last_entries[v] : 3.0 * mg;
+    cost[v] = e + 0.5 * (double)G[v].W * G[v].H + 0.2 * mg;
     order[v] = v;
   }
   std::stable_sort(order.begin(), order.end(),
@@ -2156,7 +2172,6 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     b->view_lane[v] = best;
     load[best] += cost[v];
   }
-
   // ---- framebuffers and counter slots ----
   b->fb_off.assign(n_views, 0);
   b->fb_w.assign(n_views, 0);
@@ -2249,6 +2264,7 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     h = fnv_u64(h, n_views);
     h = fnv_u64(h, L);
     h = fnv_bytes(h, view_scene, 4 * (size_t)n_views);
+    h = fnv_bytes(h, b->view_lane.data(), 4 * (size_t)n_views);
     h = fnv_bytes(h, cams, sizeof(cgs_camera_uniforms) * n_views);
     h = fnv_bytes(h, comps, sizeof(cgs_compositor_uniforms) * n_views);
     const float pv[4] = {prm->strain_split_threshold, prm->mip_filter_3d_var, prm->min_opacity,
@@ -2307,6 +2323,9 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
       if (total > a.entry_cap) overflow = true;
     }
     St.total_tile_entries = entries;
+    b->views_hash = vh;
+    b->last_entries.resize(n_views);
+    for (uint32_t v = 0; v < n_views; ++v) b->last_entries[v] = b->h_counters[4 * v + 2];
     if (!overflow) break;
     if (attempt >= 4) return fail(CGS_ERR_EXECUTION, "tile-entry arena still overflowing");
     for (uint32_t k = 0; k < L; ++k) {

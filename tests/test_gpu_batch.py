@@ -77,6 +77,9 @@ def test_batch_matches_per_view_frames():
     with ViewBatch(0, lanes=3) as batch:
         s1 = _check_batch(batch, states, view_scene, cams, comps, p, "batch first")
         assert s1.graph_built == 1 and s1.lanes == 3
+        # the second call may re-deal the views to lanes by their measured
+        # weights (one more capture); from then on the graph is replayed
+        _check_batch(batch, states, view_scene, cams, comps, p, "batch second")
         s2 = _check_batch(batch, states, view_scene, cams, comps, p, "batch replay")
         assert s2.graph_built == 0, "unchanged batch re-captured its graph"
         cams2 = list(cams)
