@@ -133,6 +133,228 @@ def cpu_baseline(scene, frames, budget_s: float):
                       f"(Stages 2-4), C++ oracle with OpenMP on {cores} threads ({cpu_info()})"}
 
 
+def _kernel_means(st, frames, params, tim):
+    """Per-kernel device ms per frame (HIP events on the context stream)."""
+    acc = {}
+    for cam, comp in frames:
+        st.execute_frame_resident(cam, comp, params, tim)
+        for name, ms in st.kernel_times():
+            acc.setdefault(name, []).append(ms)
+    return {k: round(float(np.sum(v)) / len(frames), 4) for k, v in sorted(acc.items())}
+
+
+def frame_8k(scene, frames: int = 6):
+    """One 7680x4320 frame of the 1.07M-splat scene (129,600 tiles: 64-bit keys)."""
+    from cindergs_amd import FrameParams, GpuPipelineState, StageTimings
+    from cindergs_amd import synthetic as syn
+
+    st = GpuPipelineState(0)
+    st.sync_scene_and_cages(scene, None)
+    params = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    tim = StageTimings()
+    views = [syn.view_orbit(f * 20, 7680, 4320) for f in range(frames)]
+    st.execute_frame_resident(*views[0], params, tim)
+    st.synchronize()
+    t = time.perf_counter()
+    ents = []
+    for cam, comp in views:
+        st.execute_frame_resident(cam, comp, params, tim)
+        ents.append(tim.total_tile_entries)
+    ms = 1000 * (time.perf_counter() - t) / frames
+    kern = _kernel_means(st, views[:3], FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 2, 0), tim)
+    st.close()
+    return {"workload": f"{scene.count} splats, 7680x4320 (129600 tiles, 64-bit keys), "
+                        f"{frames} orbit frames", "ms_per_frame": round(ms, 3),
+            "tile_entries_mean": int(np.mean(ents)), "kernels_ms_per_frame": kern}
+
+
+def f16_orbit(scene, steps: int, warmup: int):
+    """The 1080p orbit on the f16 quantised storage (QuantizedGaussianSceneSoa)."""
+    from cindergs_amd import FrameParams, GpuPipelineState, StageTimings
+    from cindergs_amd import synthetic as syn
+
+    q = syn.quantize_f16(scene, 3)
+    st = GpuPipelineState(0)
+    st.sync_scene_and_cages(q, None)
+    params = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    tim = StageTimings()
+    cams = [syn.view_orbit(f, 1920, 1080) for f in range(120)]
+    for i in range(warmup):
+        st.execute_frame_resident(*cams[i % 120], params, tim)
+    st.synchronize()
+    t = time.perf_counter()
+    surv = []
+    for i in range(steps):
+        st.execute_frame_resident(*cams[(warmup + i) % 120], params, tim)
+        surv.append(tim.surviving_gaussians)
+    ms = 1000 * (time.perf_counter() - t) / steps
+    prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 1, 0)
+    s2 = []
+    for i in range(12):
+        st.execute_frame_resident(*cams[(warmup + i) % 120], prof, tim)
+        s2.append(tim.stage2_project_ms)
+    st.close()
+    N, S = scene.count, int(np.mean(surv))
+    b2 = 32 * N + 140 * S  # f16 record: 12-byte covariance, 96-byte SH (degree 3)
+    s2ms = float(np.mean(s2))
+    return {"workload": f"{N} splats f16 storage (SH degree 3), 1920x1080 orbit",
+            "ms_per_frame": round(ms, 4), "stage2_ms": round(s2ms, 4),
+            "stage2_GB/s": round(b2 / (s2ms * 1e-3) / 1e9, 1),
+            "stage2_frac_of_8TB/s": round(b2 / (s2ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+
+
+def cage_frames(scene, frames: int = 300):
+    """The scene bound to a 6,000-tet jittered-grid cage (10x10x10 cells x 6
+    tets), vertices stepped by the spring solver (stiffness 40, damping 4,
+    dt 1/60) after a 0.2 impulse on 5% of them; each frame Stage 1-4 at 1080p."""
+    from cindergs_amd import FrameParams, GpuPipelineState, StageTimings
+    from cindergs_amd import synthetic as syn
+    from cindergs_amd.cages import CageSpringSimulator, TetMesh
+
+    st = GpuPipelineState(0)
+    lo, hi = scene.positions.min(0), scene.positions.max(0)
+    mesh = TetMesh.jittered_grid(lo, hi, (10, 10, 10))
+    st.sync_scene_and_cages(scene, None)
+    t = time.perf_counter()
+    tet, w = mesh.bind_gaussians_gpu(st, scene.positions)
+    bind_ms = 1000 * (time.perf_counter() - t)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping_c=4.0)
+    sim.impulse(0.2, 0.05)
+    params = FrameParams(1.02, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    tim = StageTimings()
+    cams = [syn.view_orbit(f, 1920, 1080) for f in range(120)]
+    # warm-up frame (arena growth), then the timed frames: host spring step +
+    # vertex upload + Stage 1-4
+    st.sync_cages(mesh.cage(sim.x, tet, w), scene.count)
+    st.execute_frame_resident(*cams[0], params, tim)
+    st.synchronize()
+    splits, ran = [], 0
+    t = time.perf_counter()
+    for f in range(frames):
+        sim.step(1.0 / 60.0)
+        st.sync_cages(mesh.cage(sim.x, tet, w), scene.count)
+        st.execute_frame_resident(*cams[f % 120], params, tim)
+        splits.append(int(tim.active_gaussians) - scene.count)
+        ran += 0 if tim.stage1_skipped else 1
+    ms = 1000 * (time.perf_counter() - t) / frames
+    prof = FrameParams(1.02, 0.0, 1.0 / 255.0, 0.5, 1, 0)
+    s1 = []
+    for f in range(10):
+        sim.step(1.0 / 60.0)
+        st.sync_cages(mesh.cage(sim.x, tet, w), scene.count)
+        st.execute_frame_resident(*cams[f % 120], prof, tim)
+        s1.append(tim.stage1_deform_ms)
+    st.close()
+    return {"workload": f"{scene.count} splats bound to {mesh.num_tets} tets, spring-driven "
+                        f"cage, strain split 1.02, 1920x1080, {frames} frames",
+            "ms_per_frame": round(ms, 4), "stage1_ms": round(float(np.mean(s1)), 4),
+            "stage1_ran_frames": ran, "split_children_mean": int(np.mean(splits)),
+            "gpu_tet_binding_ms": round(bind_ms, 1)}
+
+
+def mvc_binding(n_points: int, train_steps: int, n_train: int):
+    """Exact MVC binding (weights + gradients) of n_points inside a 642-vertex
+    icosphere cage, and the surrogate network trained on exact weights, with
+    the residual fallback (tolerance 1e-3 of the cage diagonal).  Output
+    buffers are allocated before the timed calls."""
+    import ctypes as C
+
+    from cindergs_amd import GpuPipelineState
+    from cindergs_amd._lib import check
+    from cindergs_amd.pipeline import _ptr
+    from cindergs_amd.mvc import MvcGpu, icosphere, sample_inside
+    from cindergs_amd.surrogate import MvcSurrogate
+
+    st = GpuPipelineState(0)
+    lib = st._lib
+    cage = icosphere(3, 1.2, 0.15)
+    ex = MvcGpu(st, cage)
+    nv, nt = cage.num_vertices, cage.triangles.shape[0]
+    pts = sample_inside(cage, n_points + n_train, inside_fn=lambda p: _winding(ex, st, p))
+
+    def dalloc(nbytes):
+        p = C.c_void_p()
+        check(lib.cgs_malloc(st._h, nbytes, C.byref(p)))
+        return p
+
+    d_pts = st.to_device(pts[:n_points])
+    d_w, d_g, d_m = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12), dalloc(n_points * 4)
+    d_ws, d_gs, d_ms = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12), dalloc(n_points * 4)
+    d_wi, d_gi = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12)
+    cv, ct = _ptr(cage.vertices), _ptr(cage.triangles)
+    st.synchronize()
+    t = time.perf_counter()
+    check(lib.cgs_mvc_bind_exact(st._h, cv, nv, ct, nt, d_pts.ptr, n_points, d_w, d_g, d_m))
+    st.synchronize()
+    exact_ms = 1000 * (time.perf_counter() - t)
+    d_tr = st.to_device(pts[n_points:])
+    d_wt = dalloc(n_train * nv * 4)
+    check(lib.cgs_mvc_bind_exact(st._h, cv, nv, ct, nt, d_tr.ptr, n_train, d_wt, None, None))
+    net = MvcSurrogate(st, nv, fourier_sigma=1.0, seed=7)
+    st.synchronize()
+    t = time.perf_counter()
+    check(lib.cgs_mlp_train(net._h, d_tr.ptr, d_wt, n_train, train_steps, 8192, 1e-3, None))
+    st.synchronize()
+    train_s = time.perf_counter() - t
+    fb = C.c_uint64()
+    t = time.perf_counter()
+    check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, n_points, 1e-3, d_ws, d_gs,
+                           d_ms, C.byref(fb)))
+    st.synchronize()
+    sur_ms = 1000 * (time.perf_counter() - t)
+    t = time.perf_counter()
+    check(lib.cgs_mlp_infer(net._h, d_pts.ptr, n_points, d_wi, d_gi))
+    st.synchronize()
+    inf_ms = 1000 * (time.perf_counter() - t)
+
+    def host(d, n):
+        out = np.empty((n, nv), np.float32)
+        check(lib.cgs_memcpy_d2h(st._h, _ptr(out), d, out.nbytes))
+        return out
+
+    we, wsu, wnet = host(d_w, n_points), host(d_ws, n_points), host(d_wi, n_points)
+    # deformed-position error under a seeded 0.1 perturbation of the cage vertices
+    rng = np.random.Generator(np.random.PCG64(99))
+    dv = (cage.vertices + rng.uniform(-0.1, 0.1, cage.vertices.shape)).astype(np.float32)
+    pe = we @ dv
+    diag = cage.diagonal()
+
+    def stats(wx):
+        e = np.linalg.norm(pe - wx @ dv, axis=1) / diag
+        return {"max": float(e.max()), "p99": float(np.quantile(e, 0.99)),
+                "mean": float(e.mean())}
+
+    # the network's weights after the linear-precision projection
+    # w + A^T (A A^T)^-1 ([1; x] - A w), A = [1^T; V^T] (what the bind applies to
+    # the points it keeps)
+    A = np.concatenate([np.ones((1, nv)), cage.vertices.T.astype(np.float64)], 0)
+    Minv = np.linalg.inv(A @ A.T)
+    rhs = np.concatenate([np.ones((n_points, 1)), pts[:n_points].astype(np.float64)], 1)
+    wproj = (wnet + ((rhs - wnet.astype(np.float64) @ A.T) @ Minv @ A)).astype(np.float32)
+    res = {"workload": f"{n_points} points in a {nv}-vertex cage, weights + gradients",
+           "exact_ms": round(exact_ms, 2), "surrogate_bind_ms": round(sur_ms, 2),
+           "surrogate_inference_ms": round(inf_ms, 2),
+           "fallback_fraction": round(fb.value / n_points, 4), "train_steps": train_steps,
+           "train_samples": n_train, "train_s": round(train_s, 2),
+           "position_error_over_diag": {"bind (with fallback)": stats(wsu),
+                                        "network alone": stats(wnet),
+                                        "network + projection": stats(wproj)}}
+    for p in (d_w, d_g, d_m, d_ws, d_gs, d_ms, d_wi, d_gi, d_wt):
+        check(lib.cgs_free(st._h, p))
+    d_pts.free()
+    d_tr.free()
+    net.close()
+    st.close()
+    return res
+
+
+def _winding(ex, st, p):
+    d = st.to_device(p)
+    w = ex.winding(d, p.shape[0])
+    d.free()
+    return w
+
+
 def batched_views(n_views: int, reps: int, lanes_list, splats=None):
     """Views/s of the view batch (one HIP graph per batch) against per-view
     cgs_execute_frame calls over the same views: 3 scenes (200k, 1.07M, 4M
@@ -246,6 +468,13 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--batch-lanes", type=str, default="1,2,4,8")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 8K, f16 and cage-deformation measurements")
+    ap.add_argument("--cage-frames", type=int, default=300)
+    ap.add_argument("--mvc", type=int, default=0, metavar="POINTS",
+                    help="also time exact vs surrogate MVC binding of POINTS points")
+    ap.add_argument("--mvc-steps", type=int, default=20000)
+    ap.add_argument("--mvc-train", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -380,9 +609,18 @@ def main() -> int:
         if tr is not None:
             result["roofline"]["traffic"] = int(tr["bytes"])
             result["roofline"]["traffic_pmc"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+    if not args.no_extras:
+        result["frame_8k"] = frame_8k(scene)
+        print(f"# 8K: {result['frame_8k']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
+        result["f16"] = f16_orbit(scene, min(args.steps, 120), args.warmup)
+        print(f"# f16: {result['f16']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
+        result["cage"] = cage_frames(scene, args.cage_frames)
+        print(f"# cage: {result['cage']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
     if args.batch_views > 0:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes)
+    if args.mvc > 0:
+        result["mvc"] = mvc_binding(args.mvc, args.mvc_steps, args.mvc_train)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, [(args.warmup + i) % 120 for i in range(8)],
                                               args.cpu_budget)
