@@ -80,8 +80,10 @@ def hbm_traffic(kernel: str, calib_mib: int = 256) -> dict | None:
     """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by what the same counter reads for a device-to-device
     copy of calib_mib MiB (calib bytes read / written = calib_mib MiB)."""
+    # the headline workload only: the other legs launch the same kernels on
+    # other sizes and would enter the per-dispatch mean
     run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
-           "--no-traffic"]
+           "--no-traffic", "--no-extras", "--batch-views", "0"]
     cal = ["--calibrate-copy", str(calib_mib)]
     nbytes = float(calib_mib << 20)
     res = {}
@@ -355,7 +357,7 @@ def _winding(ex, st, p):
     return w
 
 
-def batched_views(n_views: int, reps: int, lanes_list, splats=None):
+def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
     """Views/s of the view batch (one HIP graph per batch) against per-view
     cgs_execute_frame calls over the same views: 3 scenes (200k, 1.07M, 4M
     splats, Gaussian-ball distribution), n_views seeded orbit cameras with
@@ -413,6 +415,15 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
     # 8-bit-digit launch shapes at the best lane count
     sweep = {}
     best = None
+    if fixed is not None:
+        from cindergs_amd._lib import Tuning
+
+        tu = Tuning(fixed[1], 8, fixed[2], fixed[3])
+        wall, gms, stats = run_batch(fixed[0], tu)
+        best = (fixed[0], wall, int(stats.total_tile_entries), tu)
+        sweep[f"lanes={fixed[0]} part={fixed[1]} batch={fixed[2]} order={fixed[3]}"] = round(
+            1000 * wall, 3)
+        lanes_list = []
     for lanes in lanes_list:
         wall, gms, stats = run_batch(lanes, None)
         sweep[f"lanes={lanes}"] = round(1000 * wall, 3)
@@ -420,7 +431,7 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
               file=sys.stderr, flush=True)
         if best is None or wall < best[1]:
             best = (lanes, wall, int(stats.total_tile_entries), None)
-    for i in range(tuning_candidates_count()):
+    for i in range(tuning_candidates_count() if fixed is None else 0):
         tu = tuning_candidate(i)
         if tu.radix_digit_bits != 8 or i == 0:
             continue
@@ -431,11 +442,12 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None):
         print(f"# batched {key}: {1000 * wall:.2f} ms", file=sys.stderr, flush=True)
         if wall < best[1]:
             best = (best[0], wall, int(stats.total_tile_entries), tu)
-    # the same batch with one projection per view (the shared pass switched off)
-    wall_np, _, _ = run_batch(best[0], best[3], shared=False)
-    sweep["best, one projection per view"] = round(1000 * wall_np, 3)
-    print(f"# batched without the shared projection: {1000 * wall_np:.2f} ms", file=sys.stderr,
-          flush=True)
+    if fixed is None:
+        # the same batch with one projection per view (the shared pass off)
+        wall_np, _, _ = run_batch(best[0], best[3], shared=False)
+        sweep["best, one projection per view"] = round(1000 * wall_np, 3)
+        print(f"# batched without the shared projection: {1000 * wall_np:.2f} ms",
+              file=sys.stderr, flush=True)
     for stt in states:
         stt.close()
     res = {"views": n_views, "reps": reps,
@@ -466,7 +478,9 @@ def main() -> int:
     ap.add_argument("--batch-views", type=int, default=64,
                     help="views of the batched multi-view measurement (0 = skip it)")
     ap.add_argument("--batch-reps", type=int, default=3)
-    ap.add_argument("--batch-lanes", type=str, default="1,2,4,8")
+    ap.add_argument("--batch-lanes", type=str, default="4,8,16")
+    ap.add_argument("--batch-fixed", type=str, default="", metavar="LANES,PART,BATCH,ORDER",
+                    help="batched views: time only this lane count and tuning (no sweep)")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 8K, f16 and cage-deformation measurements")
@@ -502,10 +516,11 @@ def main() -> int:
         st.close()
         return 0
 
+    fixed = [int(x) for x in args.batch_fixed.split(",")] if args.batch_fixed else None
     if args.batch_only:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
-        print(json.dumps({"batched": batched_views(args.batch_views, args.batch_reps, lanes)}),
-              flush=True)
+        print(json.dumps({"batched": batched_views(args.batch_views, args.batch_reps, lanes,
+                                                   fixed=fixed)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -618,7 +633,7 @@ def main() -> int:
         print(f"# cage: {result['cage']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
     if args.batch_views > 0:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
-        result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes)
+        result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes, fixed=fixed)
     if args.mvc > 0:
         result["mvc"] = mvc_binding(args.mvc, args.mvc_steps, args.mvc_train)
     if not args.no_cpu_baseline:

@@ -143,19 +143,15 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
   // rank: wave w owns keys [base + w*1024, base + (w+1)*1024), item i of lane l
   // at offset i*64 + l, so items are visited in input order
-  // (the values are loaded here too, so that their latency overlaps the
-  // ranking instead of following the look-back)
-  uint32_t keys[RI], vals[RI], offs[RI];
+  // offs: each key's rank among the wave's keys of its digit (< 1024, or
+  // 0xFFFF for no key), two per register, so that keys, values and ranks of
+  // the 1024-thread shape fit its 128 VGPRs without spilling
+  uint32_t keys[RI], vals[RI], offs[RI / 2];
   const uint32_t wbase = base + wave * (uint32_t)(RI * 64);
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     keys[i] = idx < n ? A.keys_in[idx] : 0u;
-  }
-#pragma unroll
-  for (int i = 0; i < RI; ++i) {
-    const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
-    vals[i] = A.vals_in ? (idx < n ? A.vals_in[idx] : 0u) : idx;
   }
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
@@ -180,7 +176,18 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
     __builtin_amdgcn_wave_barrier();
     if (valid && prior == 0u) s_whist[wave][d] = wcount + (uint32_t)__popcll(m);
     __builtin_amdgcn_wave_barrier();
-    offs[i] = valid ? wcount + prior : 0xFFFFFFFFu;
+    const uint32_t rank = valid ? wcount + prior : 0xFFFFu;
+    if (i & 1)
+      offs[i >> 1] |= rank << 16;
+    else
+      offs[i >> 1] = rank;
+  }
+  // the values are loaded once the keys are ranked: their latency overlaps the
+  // digit scans and the look-back, and they are not live during the ranking
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
+    vals[i] = A.vals_in ? (idx < n ? A.vals_in[idx] : 0u) : idx;
   }
   __syncthreads();
 
@@ -210,9 +217,10 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   // stage keys and values in LDS in digit order
 #pragma unroll
   for (int i = 0; i < RI; ++i) {
-    if (offs[i] != 0xFFFFFFFFu) {
+    const uint32_t rank = (offs[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+    if (rank != 0xFFFFu) {
       const uint32_t d = (keys[i] >> A.shift) & dmask;
-      const uint32_t pos = s_lstart[d] + s_whist[wave][d] + offs[i];
+      const uint32_t pos = s_lstart[d] + s_whist[wave][d] + rank;
       s_keys[pos] = keys[i];
       s_vals[pos] = vals[i];
     }
