@@ -241,6 +241,10 @@ struct CgsBinArgs {
   uint32_t sort_bits;        // 32-bit mode: bits of the packed key
   uint32_t tile_bits;        // 64-bit mode
   uint32_t digit_bits;       // 32-bit mode: digit width of the sort (8 or 4); 64-bit: 8
+  // hit results k_count leaves for k_emit (both null: k_emit re-tests):
+  uint8_t* small_masks;      // [max] per splat with a box of <= 8 tiles: bit k = tile k hit
+  uint64_t* chunk_masks;     // [num_parts * 16 * 32] per wave: the hit masks of its first
+                             // 32 wave-walk chunks
 };
 
 // Optional per-launch timing: mark(name) records an event before a launch;
@@ -277,6 +281,11 @@ void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s);
 void cgs_launch_project_views(const CgsProjectArgs& a, const CgsViewProjection* views,
                               uint32_t n_views, hipStream_t s);
 uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
+// words of the hit-mask buffer k_count fills for k_emit (small masks, then
+// chunk masks 8-byte aligned); cgs_bin_masks splits it
+size_t cgs_bin_mask_words(uint32_t max_gaussians);
+void cgs_bin_masks(uint32_t* buf, uint32_t max_gaussians, uint8_t** small_masks,
+                   uint64_t** chunk_masks);
 void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof = nullptr);
 void cgs_launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
                      uint64_t* status, uint32_t* ticket, hipStream_t s);

@@ -24,6 +24,7 @@ namespace {
 constexpr int BT = 1024;      // threads per block (16 wave64s), one splat per thread
 constexpr int BP = BT;        // splats per block
 constexpr uint32_t SMALL_BOX = 8;  // boxes of at most this many tiles: walked by their thread
+constexpr uint32_t KCHUNK = 32;    // hit masks kept per wave for the wave walk
 
 struct TileBox {
   uint32_t tx0, tx1, ty0, ty1;
@@ -156,10 +157,13 @@ __device__ __forceinline__ void wave_sync_lds() {
 // this lane's big splat (slot `sp`) and tile, and whether the tile is hit.
 // `cp` is the first big splat whose items start at or after `base`.  Returns
 // the lanes that start a splat run in the chunk, lane 0 always included.
+// With `test` false the hit is taken from `known` (the mask k_count recorded
+// for this chunk) instead of the ellipse test.
 __device__ __forceinline__ uint64_t walk_chunk(const CgsBinArgs& A, WaveBig& W, uint32_t base,
                                                uint32_t total, uint32_t nv, uint32_t lane,
                                                uint32_t& cp, uint32_t& sp, uint32_t& tile_x,
-                                               uint32_t& tile_y, bool& hit) {
+                                               uint32_t& tile_y, bool& hit, bool test = true,
+                                               uint64_t known = 0ull) {
   // big splats whose first item falls in the chunk mark their first lane
   const uint32_t j = cp + lane;
   const uint32_t st = j < nv ? W.start[j] : 0xFFFFFFFFu;
@@ -190,7 +194,8 @@ __device__ __forceinline__ uint64_t walk_chunk(const CgsBinArgs& A, WaveBig& W, 
     }
     tile_x = bx.x + (uint32_t)r;
     tile_y = bx.y + q;
-    hit = hits(W.d0[sp], W.d1[sp], tile_x, tile_y, A.width, A.height);
+    hit = test ? hits(W.d0[sp], W.d1[sp], tile_x, tile_y, A.width, A.height)
+               : ((known >> lane) & 1ull) != 0ull;
   }
   return M | 1ull;
 }
@@ -220,21 +225,30 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
   const uint32_t s = blockIdx.x * (uint32_t)BP + t;
   const BinSplat b = load_bin_splat(A, s, active);
-  // small boxes: the thread walks them
+  // small boxes: the thread walks them; bit k of the splat's mask = box tile
+  // k (row-major) is hit
   uint32_t cnt = 0;
-  if (b.area != 0u && b.area <= SMALL_BOX)
+  if (b.area != 0u && b.area <= SMALL_BOX) {
+    uint32_t mask = 0u, k = 0u;
     for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
-      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx)
-        cnt += hits(b.d0, b.d1, tx, ty, A.width, A.height) ? 1u : 0u;
-  // big boxes: the wave walks them together
+      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
+        if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) mask |= 1u << k;
+    cnt = (uint32_t)__popc(mask);
+    if (A.small_masks) A.small_masks[s] = (uint8_t)mask;
+  }
+  // big boxes: the wave walks them together; the hit masks of its first
+  // KCHUNK chunks are kept for k_emit
   uint32_t nv, slot;
   const uint32_t total = stage_big(W, b, s, 0u, lane, nv, slot);
+  uint64_t* masks =
+      A.chunk_masks ? A.chunk_masks + ((size_t)blockIdx.x * (BT / 64) + wave) * KCHUNK : nullptr;
   uint32_t cp = 0;
   for (uint32_t base = 0; base < total; base += 64u) {
     uint32_t sp, tx, ty;
     bool hit;
     const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
     const uint64_t Hm = __ballot(hit);
+    if (masks && base / 64u < KCHUNK && lane == 0u) masks[base / 64u] = Hm;
     if (((R >> lane) & 1ull) && base + lane < total)
       W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
     wave_sync_lds();
@@ -340,16 +354,20 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
   const bool hist = A.hist_partial != nullptr;
   const uint32_t hbits = A.mode64 ? 32u : A.sort_bits;
   const uint32_t db = A.digit_bits == 4u ? 4u : 8u;
-  BinSplat b;
-  b.area = 0;
-  if (c != 0u) b = load_bin_splat(A, s, active);
-  // small boxes: the thread walks them
+  // staged exactly as k_count staged it (its walk's chunks, whose hit masks
+  // are reused here, depend on every big box, hit or not)
+  const BinSplat b = load_bin_splat(A, s, active);
+  // small boxes: the thread walks them (the tiles k_count found hit, when it
+  // kept their mask)
   if (b.area != 0u && b.area <= SMALL_BOX) {
     uint32_t cur = off;
     const uint32_t kb = cgs_f2u(b.d1.w);
+    const uint32_t mask = A.small_masks ? (uint32_t)A.small_masks[s] : 0u;
+    uint32_t k = 0u;
     for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
-      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx)
-        if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
+        if (A.small_masks ? ((mask >> k) & 1u) != 0u
+                          : hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
           const uint32_t tile = ty * A.tiles_x + tx;
           if (cur < A.cap) {
             store_entry(A, cur, tile, kb, s);
@@ -369,11 +387,15 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
   // slot plus the hits of the same splat on the lanes before it
   uint32_t nv, slot;
   const uint32_t total = stage_big(W, b, s, off, lane, nv, slot);
+  const uint64_t* masks =
+      A.chunk_masks ? A.chunk_masks + ((size_t)blk * (BT / 64) + wave) * KCHUNK : nullptr;
   uint32_t cp = 0;
   for (uint32_t base = 0; base < total; base += 64u) {
     uint32_t sp, tx, ty;
     bool hit;
-    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
+    const bool known = masks && base / 64u < KCHUNK;
+    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit, !known,
+                                  known ? masks[base / 64u] : 0ull);
     const uint64_t Hm = __ballot(hit);
     const bool act = base + lane < total;
     uint32_t pos = 0, id = 0, kb = 0;
@@ -459,6 +481,19 @@ void cgs_launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s)
 
 uint32_t cgs_bin_num_parts(uint32_t max_gaussians) {
   return (max_gaussians + (uint32_t)BP - 1u) / (uint32_t)BP;
+}
+
+size_t cgs_bin_mask_words(uint32_t max_gaussians) {
+  const size_t small = ((size_t)max_gaussians + 7u) / 8u * 2u;  // bytes -> words, 8-byte aligned
+  const size_t chunk = (size_t)cgs_bin_num_parts(max_gaussians) * (BT / 64) * KCHUNK * 2u;
+  return small + chunk;
+}
+
+void cgs_bin_masks(uint32_t* buf, uint32_t max_gaussians, uint8_t** small_masks,
+                   uint64_t** chunk_masks) {
+  const size_t small = ((size_t)max_gaussians + 7u) / 8u * 2u;
+  *small_masks = reinterpret_cast<uint8_t*>(buf);
+  *chunk_masks = reinterpret_cast<uint64_t*>(buf + small);
 }
 
 void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof) {

@@ -134,6 +134,7 @@ struct FrameArena {
   DevBuf e_keys, e_ids, e_depths, e_alt_k, e_alt_v, e_perm;
   uint32_t entry_cap = 0;
   DevBuf ranges, out_rgba, cnt_dbg, off_dbg, comp_order;
+  DevBuf hitm;   // hit masks from k_count to k_emit
   DevBuf ctrl3;  // frame control, see frame_layout()
   uint32_t tiles_x = 0, tiles_y = 0, padded_row = 0;
   bool mode64 = false;
@@ -152,7 +153,7 @@ struct FrameArena {
   void release_all() {
     DevBuf* bufs[] = {&geom, &color, &extents, &e_keys, &e_ids, &e_depths, &e_alt_k,
                       &e_alt_v, &e_perm, &ranges, &out_rgba, &cnt_dbg, &off_dbg, &comp_order,
-                      &ctrl3};
+                      &hitm, &ctrl3};
     for (DevBuf* b : bufs) release(*b);
     entry_cap = 0;
   }
@@ -345,6 +346,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   cgs_launch_fill_u32(ctrl, 0u, L.zero_words, s);
   cgs_launch_fill_u32(P<uint32_t>(a.ranges), 0u, (uint64_t)g.T * 2, s);
   TRY(ensure_words(a.cnt_dbg, max_g));  // per-splat tile counts (always needed)
+  TRY(ensure_words(a.hitm, cgs_bin_mask_words(max_g)));
   a.captured = false;
   if (a.capture) {
     TRY(ensure_words(a.off_dbg, max_g));
@@ -378,6 +380,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   B.ids = P<uint32_t>(a.e_ids);
   B.counts = P<uint32_t>(a.cnt_dbg);
   B.offsets_out = a.capture ? P<uint32_t>(a.off_dbg) : nullptr;
+  cgs_bin_masks(P<uint32_t>(a.hitm), max_g, &B.small_masks, &B.chunk_masks);
   B.total = ctrl + L.total;
   B.block_sums = ctrl + L.block_sums;
   B.block_surv = ctrl + L.block_surv;
@@ -1949,6 +1952,7 @@ int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_
   TRY(ensure_words(a.color, own_records ? (size_t)4 * max_g : 4));
   TRY(ensure_words(a.extents, own_records ? max_g : 1));
   TRY(ensure_words(a.cnt_dbg, max_g));
+  TRY(ensure_words(a.hitm, cgs_bin_mask_words(max_g)));
   TRY(ensure_words(a.e_keys, cap));
   TRY(ensure_words(a.e_ids, cap));
   TRY(ensure_words(a.e_alt_k, cap));
@@ -2285,7 +2289,7 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     for (const FrameArena& a : b->lanes) {
       const DevBuf* bufs[] = {&a.geom, &a.color, &a.extents, &a.cnt_dbg, &a.e_keys, &a.e_ids,
                               &a.e_alt_k, &a.e_alt_v, &a.e_depths, &a.e_perm, &a.ranges,
-                              &a.comp_order, &a.ctrl3};
+                              &a.comp_order, &a.hitm, &a.ctrl3};
       for (const DevBuf* d : bufs) h = fnv_buf(h, *d);
       h = fnv_u64(h, a.entry_cap);
     }
