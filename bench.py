@@ -76,7 +76,7 @@ def pmc_per_dispatch(counter: str, bench_args: list, timeout_s: int = 90) -> dic
         shutil.rmtree(out, ignore_errors=True)
 
 
-def hbm_traffic(kernel: str, calib_mib: int = 256) -> dict | None:
+def hbm_traffic(kernel: str, calib_mib: int = 256, tuning: str = "") -> dict | None:
     """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by what the same counter reads for a device-to-device
     copy of calib_mib MiB (calib bytes read / written = calib_mib MiB)."""
@@ -84,6 +84,7 @@ def hbm_traffic(kernel: str, calib_mib: int = 256) -> dict | None:
     # other sizes and would enter the per-dispatch mean
     run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
            "--no-traffic", "--no-extras", "--batch-views", "0"]
+    run += ["--tuning", tuning] if tuning else ["--no-autotune"]
     cal = ["--calibrate-copy", str(calib_mib)]
     nbytes = float(calib_mib << 20)
     res = {}
@@ -482,6 +483,10 @@ def main() -> int:
     ap.add_argument("--batch-fixed", type=str, default="", metavar="LANES,PART,BATCH,ORDER",
                     help="batched views: time only this lane count and tuning (no sweep)")
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="render the headline with the default launch shapes")
+    ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER",
+                    help="render the headline with these launch shapes (no autotune)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 8K, f16 and cage-deformation measurements")
     ap.add_argument("--cage-frames", type=int, default=300)
@@ -534,6 +539,21 @@ def main() -> int:
     params = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
     tim = StageTimings()
 
+    # launch shapes picked by the autotuner on the first orbit frame (every
+    # candidate renders the same frame; see cgs_autotune)
+    tuned = None
+    if args.tuning:
+        from cindergs_amd._lib import Tuning
+
+        st.set_tuning(Tuning(*[int(x) for x in args.tuning.split(",")]))
+        tuned = {"tuning": st.get_tuning().as_dict(), "candidates": 0}
+    elif not args.no_autotune:
+        best, table = st.autotune(*cams[0], params, reps=4)
+        tuned = {"tuning": best.as_dict(), "candidates": len(table),
+                 "best_ms": round(min(ms for _, ms in table), 4),
+                 "default_ms": round(table[0][1], 4)}
+        print(f"# autotune: {best.as_dict()} ({tuned['best_ms']} ms vs default "
+              f"{tuned['default_ms']} ms)", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         cam, comp = cams[i % 120]
         st.execute_frame_resident(cam, comp, params, tim)
@@ -611,6 +631,7 @@ def main() -> int:
         "splats_per_s": round(N / (ms_per_frame * 1e-3), 1),
         "counts": {"N": N, "S_mean": S, "E_mean": E, "T": T},
         "stages": stage_gbs,
+        "autotune": tuned,
         "kernels_ms_per_frame": {k: round(v, 4) for k, v in sorted(kern.items())},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -620,7 +641,8 @@ def main() -> int:
     }
     st.close()
     if not args.no_traffic:
-        tr = hbm_traffic(dom)
+        tstr = ",".join(str(v) for v in tuned["tuning"].values()) if tuned else ""
+        tr = hbm_traffic(dom, tuning=tstr)
         if tr is not None:
             result["roofline"]["traffic"] = int(tr["bytes"])
             result["roofline"]["traffic_pmc"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}

@@ -266,7 +266,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
   const uint32_t radix = 1u << db;
   const uint32_t passes = (bits + db - 1u) / db;
   const uint32_t parts_alloc = cgs_radix_num_parts(b.cap);  // status rows per pass
-  const uint32_t pk = part_keys == 8192u ? 8192u : 16384u;
+  const uint32_t pk = part_keys == 8192u ? 8192u : part_keys == 10240u ? 10240u : 16384u;
   const uint32_t grid = (b.cap + pk - 1u) / pk;
   if (!hist_given) {
     if (prof) prof->mark("k_radix_hist", s);
@@ -297,11 +297,15 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
     if (db == 8u) {
       if (pk == 16384u)
         hipLaunchKernelGGL((k_radix_onesweep<1024, 8>), dim3(grid), dim3(1024), 0, s, A);
+      else if (pk == 10240u)
+        hipLaunchKernelGGL((k_radix_onesweep<640, 8>), dim3(grid), dim3(640), 0, s, A);
       else
         hipLaunchKernelGGL((k_radix_onesweep<512, 8>), dim3(grid), dim3(512), 0, s, A);
     } else {
       if (pk == 16384u)
         hipLaunchKernelGGL((k_radix_onesweep<1024, 4>), dim3(grid), dim3(1024), 0, s, A);
+      else if (pk == 10240u)
+        hipLaunchKernelGGL((k_radix_onesweep<640, 4>), dim3(grid), dim3(640), 0, s, A);
       else
         hipLaunchKernelGGL((k_radix_onesweep<512, 4>), dim3(grid), dim3(512), 0, s, A);
     }
