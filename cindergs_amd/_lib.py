@@ -171,6 +171,8 @@ SIGNATURES = {
     "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
     "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
+    "cgs_mvc_deform_scene": [_vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _u32,
+                             C.c_float, C.c_float, _vp, _vp, _vp, _vp, C.POINTER(_u64)],
     "cgs_mlp_create": [_vp, _u32, C.c_float, _u64, C.POINTER(_vp)],
     "cgs_mlp_destroy": [_vp],
     "cgs_mlp_param_count": [_vp],

@@ -384,6 +384,28 @@ void cgs_launch_mvc_select(const float* winding, const float* residual, float to
                            uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
                            uint32_t* block_counts, hipStream_t s);
 uint32_t cgs_mvc_select_blocks(uint32_t n);
+// deform_scene_cpu (mvc.rs:564-749) after the per-point position and Jacobian
+struct CgsMvcSceneArgs {
+  const uint32_t* mask;      // [n] bound flags, or null (all bound)
+  const float* def_pos;      // [3n] x'
+  const float* def_F;        // [9n] F (column-major)
+  const float* rest_pos;     // [3n]
+  const float* rest_cov;     // [6n]
+  const float* rest_opa;     // [n]
+  const float* rest_sh;      // [48n]
+  const double* sh_tables;   // CGS_SH_TABLE_DOUBLES (cgs_sh_basis_tables)
+  uint32_t n, sh_degree;
+  float tau, mip;
+  float* out_pos;            // [3 * 2n]
+  float* out_cov;            // [6 * 2n]
+  float* out_opa;            // [2n]
+  float* out_sh;             // [48 * 2n]
+  uint32_t* flags;           // [n] split flags
+  float* child_b;            // [10n] second child of split splats (pos, cov, opacity)
+};
+void cgs_launch_mvc_scene(const CgsMvcSceneArgs& a, hipStream_t s);
+void cgs_launch_mvc_scene_children(const CgsMvcSceneArgs& a, const uint32_t* ranks,
+                                   hipStream_t s);
 void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
                            const float* rest_pos, const float* dverts, uint32_t nv, uint32_t n,
                            float* out_pos, float* out_F, hipStream_t s);

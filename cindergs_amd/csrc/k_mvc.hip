@@ -19,6 +19,7 @@
 // Transcendentals are the device library's asinf / sinf / atan2f, so the
 // weights agree with the reference to rounding, not bit for bit.
 #include "cgs_internal.h"
+#include "cgs_scene_math.h"
 
 namespace {
 
@@ -307,6 +308,78 @@ __global__ __launch_bounds__(256) void k_mvc_select(const float* winding, const 
   if (threadIdx.x == 0) block_counts[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
 }
 
+
+// deform_scene_cpu (mvc.rs:564-749) per splat, from the point's deformed
+// position x' and Jacobian F (k_mvc_deform): bound splats get Sigma' =
+// F Sigma F^T and their SH rotated by the polar rotation of F (rest opacity
+// kept); unbound ones keep the rest record.  A bound splat whose largest
+// stretch of F exceeds `tau` is split along the major axis of Sigma' (child A
+// replaces it, child B goes to the scratch record of i, flags[i] = 1); the mip
+// filter applies to every record that is not split and to both children.
+__global__ __launch_bounds__(64) void k_mvc_scene(CgsMvcSceneArgs A) {
+  const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+  if (i >= A.n) return;
+  const bool bound = A.mask ? A.mask[i] != 0u : true;
+  float cov[6], sh[48];
+  for (int k = 0; k < 6; ++k) cov[k] = A.rest_cov[(size_t)i * 6 + k];
+  for (int k = 0; k < 48; ++k) sh[k] = A.rest_sh[(size_t)i * 48 + k];
+  float alpha = A.rest_opa[i];
+  float pos[3];
+  bool split = false;
+  if (bound) {
+    float F[9];
+    for (int k = 0; k < 3; ++k) pos[k] = A.def_pos[(size_t)i * 3 + k];
+    for (int k = 0; k < 9; ++k) F[k] = A.def_F[(size_t)i * 9 + k];
+    float R[9], S[9], FS[9], Ft[9], FSF[9];
+    cgs_polar_rotation(F, R);
+    cgs_cov_to_m3(cov, S);
+    cgs_m3_mul(F, S, FS);
+    cgs_m3_transpose(F, Ft);
+    cgs_m3_mul(FS, Ft, FSF);
+    cgs_m3_to_cov(FSF, cov);
+    cgs_sh_rotate(sh, R, (int)A.sh_degree, A.sh_tables);
+    split = A.tau > 0.0f && cgs_max_stretch(F) > A.tau;
+  } else {
+    for (int k = 0; k < 3; ++k) pos[k] = A.rest_pos[(size_t)i * 3 + k];
+  }
+  if (!split) {
+    cgs_mip_filter_3d(cov, &alpha, A.mip);
+  } else {
+    float child[6], off[3], cb[6];
+    cgs_split_major(cov, child, off);
+    float aa = alpha, ab = alpha;
+    for (int k = 0; k < 6; ++k) cb[k] = child[k];
+    cgs_mip_filter_3d(child, &aa, A.mip);
+    cgs_mip_filter_3d(cb, &ab, A.mip);
+    float* b = A.child_b + (size_t)i * 10;
+    for (int k = 0; k < 3; ++k) b[k] = pos[k] + off[k];
+    for (int k = 0; k < 6; ++k) b[3 + k] = cb[k];
+    b[9] = ab;
+    for (int k = 0; k < 3; ++k) pos[k] = pos[k] + (-off[k]);
+    for (int k = 0; k < 6; ++k) cov[k] = child[k];
+    alpha = aa;
+  }
+  A.flags[i] = split ? 1u : 0u;
+  for (int k = 0; k < 3; ++k) A.out_pos[(size_t)i * 3 + k] = pos[k];
+  for (int k = 0; k < 6; ++k) A.out_cov[(size_t)i * 6 + k] = cov[k];
+  A.out_opa[i] = alpha;
+  for (int k = 0; k < 48; ++k) A.out_sh[(size_t)i * 48 + k] = sh[k];
+}
+
+// child B of split splat i goes to record n + ranks[i] (ranks = exclusive scan
+// of the flags: children in parent order), with the parent's rotated SH
+__global__ __launch_bounds__(256) void k_mvc_scene_children(CgsMvcSceneArgs A,
+                                                            const uint32_t* ranks) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= A.n || A.flags[i] == 0u) return;
+  const size_t j = (size_t)A.n + ranks[i];
+  const float* b = A.child_b + (size_t)i * 10;
+  for (int k = 0; k < 3; ++k) A.out_pos[j * 3 + k] = b[k];
+  for (int k = 0; k < 6; ++k) A.out_cov[j * 6 + k] = b[3 + k];
+  A.out_opa[j] = b[9];
+  for (int k = 0; k < 48; ++k) A.out_sh[j * 48 + k] = A.out_sh[(size_t)i * 48 + k];
+}
+
 }  // namespace
 
 uint32_t cgs_mvc_select_blocks(uint32_t n) { return (n + 255u) / 256u; }
@@ -339,4 +412,15 @@ void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
   if (n == 0) return;
   hipLaunchKernelGGL(k_mvc_deform, dim3(n), dim3(64), 0, s, w, g, mask, rest_pos, dverts, nv, n,
                      out_pos, out_F);
+}
+
+void cgs_launch_mvc_scene(const CgsMvcSceneArgs& a, hipStream_t s) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_mvc_scene, dim3((a.n + 63u) / 64u), dim3(64), 0, s, a);
+}
+
+void cgs_launch_mvc_scene_children(const CgsMvcSceneArgs& a, const uint32_t* ranks,
+                                   hipStream_t s) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_mvc_scene_children, dim3((a.n + 255u) / 256u), dim3(256), 0, s, a, ranks);
 }

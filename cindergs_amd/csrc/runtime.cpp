@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/cindergs.h"
+#include "cgs_scene_math.h"
 #include "cgs_internal.h"
 
 namespace {
@@ -190,6 +191,9 @@ struct cgs_context {
   FrameArena fa;
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
+  // MVC scene deformation: per-call working set and the SH rotation tables
+  DevBuf mvc_work, mvc_tables;
+  bool mvc_tables_ready = false;
 
   // ---- last frame ----
   uint32_t last_w = 0, last_h = 0;
@@ -738,7 +742,8 @@ void cgs_destroy(cgs_context* c) {
   DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
                     &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
-                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d};
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
+                    &c->mvc_tables};
   for (DevBuf* b : bufs) release(*b);
   c->fa.release_all();
   for (auto& e : c->ev)
@@ -1871,6 +1876,87 @@ int cgs_mvc_deform(cgs_context* c, const float* d_weights, const float* d_gradie
   return CGS_OK;
 }
 
+
+
+int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_gradients,
+                         const uint32_t* d_inside, const float* deformed_vertices,
+                         uint32_t num_vertices, uint64_t n, const float* d_rest_pos,
+                         const float* d_rest_cov, const float* d_rest_opa, const float* d_rest_sh,
+                         uint32_t sh_degree, float split_threshold, float mip_filter_3d_var,
+                         float* d_out_pos, float* d_out_cov, float* d_out_opa, float* d_out_sh,
+                         uint64_t* out_count) {
+  TRY(ensure_ctx(c));
+  if (!out_count) return fail(CGS_ERR_INVALID_ARG, "null out_count");
+  *out_count = 0;
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 30)) return fail(CGS_ERR_INVALID_ARG, "n too large");
+  if (!d_weights || !d_gradients || !deformed_vertices || num_vertices == 0 || !d_rest_pos ||
+      !d_rest_cov || !d_rest_opa || !d_rest_sh || !d_out_pos || !d_out_cov || !d_out_opa ||
+      !d_out_sh)
+    return fail(CGS_ERR_INVALID_ARG, "null argument (the weight gradients are required)");
+  const size_t nn = (size_t)n;
+  if (!c->mvc_tables_ready) {
+    double h_tables[CGS_SH_TABLE_DOUBLES];
+    cgs_sh_basis_tables(h_tables);
+    TRY(ensure_words(c->mvc_tables, CGS_SH_TABLE_DOUBLES * 2));
+    HIPCHK(hipMemcpy(c->mvc_tables.p, h_tables, sizeof(h_tables), hipMemcpyHostToDevice));
+    c->mvc_tables_ready = true;
+  }
+  // working set (4-byte words, grown 1.5x and kept by the context): deformed
+  // vertices [3 nv], x' [3n], F [9n], split flags [n], ranks [n], second
+  // children [10n]
+  const size_t nv_words = ((size_t)num_vertices * 3 + 63) & ~(size_t)63;
+  TRY(ensure_words(c->mvc_work, nv_words + nn * (3 + 9 + 1 + 1 + 10)));
+  float* dv = P<float>(c->mvc_work);
+  float* def_pos = dv + nv_words;
+  float* def_F = def_pos + nn * 3;
+  uint32_t* flags = reinterpret_cast<uint32_t*>(def_F + nn * 9);
+  uint32_t* ranks = flags + nn;
+  float* child_b = reinterpret_cast<float*>(ranks + nn);
+  int rc = CGS_OK;
+  hipError_t e = hipMemcpyAsync(dv, deformed_vertices, (size_t)num_vertices * 12,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    // x' = sum w v', F = sum v' grad(w)^T (evaluate_position, local_jacobian)
+    cgs_launch_mvc_deform(d_weights, d_gradients, d_inside, d_rest_pos, dv, num_vertices,
+                          (uint32_t)n, def_pos, def_F, c->stream);
+    CgsMvcSceneArgs A;
+    A.mask = d_inside;
+    A.def_pos = def_pos;
+    A.def_F = def_F;
+    A.rest_pos = d_rest_pos;
+    A.rest_cov = d_rest_cov;
+    A.rest_opa = d_rest_opa;
+    A.rest_sh = d_rest_sh;
+    A.sh_tables = P<double>(c->mvc_tables);
+    A.n = (uint32_t)n;
+    A.sh_degree = sh_degree > 3 ? 3 : sh_degree;
+    A.tau = split_threshold;
+    A.mip = mip_filter_3d_var;
+    A.out_pos = d_out_pos;
+    A.out_cov = d_out_cov;
+    A.out_opa = d_out_opa;
+    A.out_sh = d_out_sh;
+    A.flags = flags;
+    A.child_b = child_b;
+    cgs_launch_mvc_scene(A, c->stream);
+    e = hipGetLastError();
+    uint32_t splits = 0;
+    if (e == hipSuccess) rc = cgs_exclusive_scan_u32(c, flags, ranks, n, &splits);
+    if (e == hipSuccess && rc == CGS_OK && splits) {
+      cgs_launch_mvc_scene_children(A, ranks, c->stream);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && rc == CGS_OK) {
+      e = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess) *out_count = n + splits;
+    }
+  }
+  if (rc != CGS_OK) return rc;
+  if (e != hipSuccess)
+    return fail(CGS_ERR_EXECUTION, std::string("mvc deform scene: ") + hipGetErrorString(e));
+  return CGS_OK;
+}
 
 }  // extern "C"
 

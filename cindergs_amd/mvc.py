@@ -3,8 +3,9 @@
 Host mirror of PolyCageMesh (src/deformation/mvc.rs): the cage data, the
 signed volume / closed-manifold checks (mvc.rs:290-324) and calls into the
 HIP library for the exact MVC binding (bind_gaussians, 498-541), the inside
-test (winding_number, 326-363) and the deformation of bound points
-(evaluate_position + local_jacobian, 66-78 / 473-481).
+test (winding_number, 326-363), the deformation of bound points
+(evaluate_position + local_jacobian, 66-78 / 473-481) and of whole scenes
+(deform_scene_cpu, 564-749).
 
 The workload cage is a subdivided icosphere (642 vertices, 1280 triangles:
 three 1-to-4 subdivisions of an icosahedron) of radius 1.2 with seeded radial
@@ -145,10 +146,38 @@ class MvcGpu:
         dv = np.ascontiguousarray(deformed_vertices, dtype=np.float32)
         pos = self.state.to_device(np.zeros((n, 3), np.float32))
         F = self.state.to_device(np.zeros((n, 9), np.float32)) if jacobians else None
-        check(self._lib.cgs_mvc_deform(self.state._h, w.ptr, g.ptr if g else None, m.ptr,
-                                       d_rest.ptr, _ptr(dv), self.cage.num_vertices, n, pos.ptr,
+        check(self._lib.cgs_mvc_deform(self.state._h, w.ptr, g.ptr if g else None,
+                                       m.ptr if m is not None else None, d_rest.ptr, _ptr(dv), self.cage.num_vertices, n, pos.ptr,
                                        F.ptr if F else None))
         return pos, F
+
+    def deform_scene(self, w: DeviceBuffer, g: DeviceBuffer, m, rest: dict,
+                     deformed_vertices: np.ndarray, n: int, sh_degree: int = 3,
+                     split_threshold: float = 0.0, mip_filter_3d_var: float = 0.0,
+                     out: dict | None = None):
+        """PolyCageMesh::deform_scene_cpu (mvc.rs:564-749) on the device.
+
+        rest: device buffers "pos" [n,3], "cov" [n,6], "opa" [n], "sh" [n,48];
+        m: the inside flags of the binding (or None: every splat bound).  Returns
+        (count, out) where out holds device buffers "pos", "cov", "opa", "sh"
+        of 2n records (preallocated ones may be passed in), the first `count`
+        valid: the n deformed records, then the second child of each split
+        splat in parent order."""
+        dv = np.ascontiguousarray(deformed_vertices, dtype=np.float32)
+        if out is None:
+            cap = max(2 * n, 1)
+            out = {"pos": self.state.to_device(np.zeros((cap, 3), np.float32)),
+                   "cov": self.state.to_device(np.zeros((cap, 6), np.float32)),
+                   "opa": self.state.to_device(np.zeros(cap, np.float32)),
+                   "sh": self.state.to_device(np.zeros((cap, 48), np.float32))}
+        count = C.c_uint64(0)
+        check(self._lib.cgs_mvc_deform_scene(
+            self.state._h, w.ptr if w is not None else None, g.ptr if g is not None else None,
+            m.ptr if m is not None else None, _ptr(dv),
+            self.cage.num_vertices, n, rest["pos"].ptr, rest["cov"].ptr, rest["opa"].ptr,
+            rest["sh"].ptr, int(sh_degree), float(split_threshold), float(mip_filter_3d_var),
+            out["pos"].ptr, out["cov"].ptr, out["opa"].ptr, out["sh"].ptr, C.byref(count)))
+        return int(count.value), out
 
 
 def sample_inside(cage: PolyCage, n: int, seed: int = 4, inside_fn=None,

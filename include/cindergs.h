@@ -285,6 +285,28 @@ int cgs_mvc_deform(cgs_context* ctx, const float* d_weights, const float* d_grad
                    const float* deformed_vertices, uint32_t num_vertices, uint64_t n,
                    float* d_out_positions, float* d_out_jacobians);
 
+/* PolyCageMesh::deform_scene_cpu (mvc.rs:564-749) on the GPU: for each of
+ * the n splats of a binding table, x' = sum_j w_j v'_j and F = sum_j v'_j
+ * grad(w_j)^T; a bound splat gets Sigma' = F Sigma F^T and its SH (degree
+ * sh_degree, record layout of cgs_sync_scene) rotated by the polar rotation
+ * of F (higham_polar_decomposition, polar.rs:13-42; ShRotator,
+ * sh_rotation.rs:372-456), keeping its rest opacity; an unbound one keeps its
+ * rest record.  With split_threshold > 0 a bound splat whose largest stretch
+ * of F exceeds it is split along the major axis of Sigma'
+ * (split_gaussian_along_major_axis, math/covariance.rs:343-381): the first
+ * child replaces it and the second is appended after the n records in parent
+ * order.  The 3D mip filter (288-304) applies to every unsplit record and to
+ * both children.  Inputs and outputs in device memory (outputs hold up to 2n
+ * records), deformed_vertices [3*num_vertices] on the host; the weight
+ * gradients are required; *out_count receives n + the number of splits. */
+int cgs_mvc_deform_scene(cgs_context* ctx, const float* d_weights, const float* d_gradients,
+                         const uint32_t* d_inside, const float* deformed_vertices,
+                         uint32_t num_vertices, uint64_t n, const float* d_rest_pos,
+                         const float* d_rest_cov, const float* d_rest_opa, const float* d_rest_sh,
+                         uint32_t sh_degree, float split_threshold, float mip_filter_3d_var,
+                         float* d_out_pos, float* d_out_cov, float* d_out_opa, float* d_out_sh,
+                         uint64_t* out_count);
+
 /* ---- MVC surrogate network (no counterpart in the reference) ------------------
  * A coordinate MLP that stands in for exact MVC when binding many points to one
  * cage: x -> [sin 2pi Bx, cos 2pi Bx] (B 32 x 3, N(0, sigma^2)) -> 4 x

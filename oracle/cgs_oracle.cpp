@@ -17,6 +17,8 @@
 //   Stage 4  src/render/kernels/compositor.rs:721-883
 //   MVC      src/deformation/mvc.rs:340-541 (inside test, exact 3D MVC,
 //            central-difference gradients)
+//   MVC scene src/deformation/mvc.rs:564-749 (deform_scene_cpu: polar
+//            rotation, F Sigma F^T, SH rotation, strain split, 3D mip filter)
 //
 // Every routine is a template over the scalar type: the float instantiation is
 // the FP32 oracle (elementary functions from cgs_detmath.h so that the result
@@ -38,6 +40,15 @@
 //                              1146-1319), polar identities, FP64 twin
 //   MVC                     : partition of unity, linear reproduction,
 //                              FP64 twin, finite-difference consistency
+//   MVC scene (per-splat math of cgs_scene_math.h, shared with the kernel):
+//                              SH rotation orthogonal / matches the rotated
+//                              direction under this file's eval_sh / identity
+//                              no-op / degree 0 untouched (sh_rotation.rs
+//                              484-566), polar identities (polar.rs:116-205),
+//                              eigen roundtrip, mip attenuation, split quarters
+//                              the major variance (covariance.rs:388-417), and
+//                              the whole record against a float64 restatement
+//                              built on SVD / eigh / least squares
 #include <stdint.h>
 #include <string.h>
 #include <math.h>
@@ -49,6 +60,7 @@
 #endif
 
 #include "../cindergs_amd/csrc/cgs_detmath.h"
+#include "../cindergs_amd/csrc/cgs_scene_math.h"
 
 #define ORC_API extern "C" __attribute__((visibility("default")))
 
@@ -1369,4 +1381,120 @@ ORC_API void orc_mvc_weights(const float* verts, uint32_t nv, const uint32_t* tr
       mvc_weights<float>(verts, nv, tris, nt, p, (float*)out_w + i * nv, dist.data(), dir.data());
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// MVC scene deformation: PolyCageMesh::deform_scene_cpu (mvc.rs:564-749)
+// ---------------------------------------------------------------------------
+//
+// Given each point's deformed position x' and Jacobian F (evaluate_position /
+// local_jacobian), restated in the reference's three passes: (1) per point the
+// deformed record (mvc.rs:600-667), (2) the split flags (669-697), (3) one
+// sequential walk that mip-filters unsplit records and replaces each split
+// one by its first child while appending the second (699-740).
+
+ORC_API uint64_t orc_mvc_scene(const float* def_pos, const float* def_F, const uint32_t* mask,
+                               const float* rest_pos, const float* rest_cov,
+                               const float* rest_opa, const float* rest_sh, uint64_t n,
+                               uint32_t sh_degree, float tau, float mip, float* out_pos,
+                               float* out_cov, float* out_opa, float* out_sh) {
+  if (n == 0) return 0;
+  static double tables[CGS_SH_TABLE_DOUBLES];
+  cgs_sh_basis_tables(tables);
+  const int degree = sh_degree > 3 ? 3 : (int)sh_degree;
+  std::vector<float> positions(n * 3), covs(n * 6), opas(n), shs(n * 48);
+  for (uint64_t i = 0; i < n; ++i) {
+    const bool bound = mask ? mask[i] != 0u : true;
+    for (int k = 0; k < 48; ++k) shs[i * 48 + k] = rest_sh[i * 48 + k];
+    opas[i] = rest_opa[i];
+    if (!bound) {
+      for (int k = 0; k < 3; ++k) positions[i * 3 + k] = rest_pos[i * 3 + k];
+      for (int k = 0; k < 6; ++k) covs[i * 6 + k] = rest_cov[i * 6 + k];
+      continue;
+    }
+    const float* F = def_F + i * 9;
+    float R[9], S[9], FS[9], Ft[9], C[9];
+    cgs_polar_rotation(F, R);
+    cgs_cov_to_m3(rest_cov + i * 6, S);
+    cgs_m3_mul(F, S, FS);
+    cgs_m3_transpose(F, Ft);
+    cgs_m3_mul(FS, Ft, C);
+    for (int k = 0; k < 3; ++k) positions[i * 3 + k] = def_pos[i * 3 + k];
+    cgs_m3_to_cov(C, &covs[i * 6]);
+    if (degree > 0) cgs_sh_rotate(&shs[i * 48], R, degree, tables);
+  }
+  std::vector<uint8_t> needs_split;
+  if (tau > 0.0f) {
+    needs_split.assign(n, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+      if (mask && mask[i] == 0u) continue;
+      needs_split[i] = cgs_max_stretch(def_F + i * 9) > tau ? 1 : 0;
+    }
+  }
+  uint64_t total = n;
+  for (uint64_t i = 0; i < n; ++i) {
+    const bool split = tau > 0.0f && needs_split[i] != 0;
+    float cov[6];
+    for (int k = 0; k < 6; ++k) cov[k] = covs[i * 6 + k];
+    float alpha = opas[i];
+    if (!split) {
+      if (mip > 0.0f) {
+        cgs_mip_filter_3d(cov, &alpha, mip);
+        for (int k = 0; k < 6; ++k) covs[i * 6 + k] = cov[k];
+        opas[i] = alpha;
+      }
+      continue;
+    }
+    float child[6], off[3];
+    cgs_split_major(cov, child, off);
+    float cov_a[6], cov_b[6], alpha_a = alpha, alpha_b = alpha;
+    for (int k = 0; k < 6; ++k) cov_a[k] = cov_b[k] = child[k];
+    if (mip > 0.0f) {
+      cgs_mip_filter_3d(cov_a, &alpha_a, mip);
+      cgs_mip_filter_3d(cov_b, &alpha_b, mip);
+    }
+    float center[3] = {positions[i * 3], positions[i * 3 + 1], positions[i * 3 + 2]};
+    for (int k = 0; k < 3; ++k) positions[i * 3 + k] = center[k] + (-off[k]);
+    for (int k = 0; k < 6; ++k) covs[i * 6 + k] = cov_a[k];
+    opas[i] = alpha_a;
+    for (int k = 0; k < 3; ++k) positions.push_back(center[k] + off[k]);
+    for (int k = 0; k < 6; ++k) covs.push_back(cov_b[k]);
+    opas.push_back(alpha_b);
+    for (int k = 0; k < 48; ++k) shs.push_back(shs[i * 48 + k]);
+    ++total;
+  }
+  memcpy(out_pos, positions.data(), total * 12);
+  memcpy(out_cov, covs.data(), total * 24);
+  memcpy(out_opa, opas.data(), total * 4);
+  memcpy(out_sh, shs.data(), total * 192);
+  return total;
+}
+
+// the per-splat pieces, for their own checks
+ORC_API void orc_sh_tables(double* out) { cgs_sh_basis_tables(out); }
+ORC_API void orc_sh_band_matrix(const float* R, int l, float* M) {
+  double t[CGS_SH_TABLE_DOUBLES];
+  cgs_sh_basis_tables(t);
+  cgs_sh_band_matrix(R, l, t, M);
+}
+ORC_API void orc_sh_rotate(float* sh, const float* R, int degree) {
+  double t[CGS_SH_TABLE_DOUBLES];
+  cgs_sh_basis_tables(t);
+  cgs_sh_rotate(sh, R, degree, t);
+}
+ORC_API void orc_polar_rotation(const float* J, float* R) { cgs_polar_rotation(J, R); }
+ORC_API void orc_sym_eigen(const float* cov, float* evals, float* basis) {
+  cgs_sym_eigen(cov, evals, basis);
+}
+ORC_API float orc_max_stretch(const float* J) { return cgs_max_stretch(J); }
+ORC_API void orc_split_major(const float* cov, float* child_cov, float* offset) {
+  cgs_split_major(cov, child_cov, offset);
+}
+ORC_API void orc_mip_filter_3d(float* cov, float* alpha, float var) {
+  cgs_mip_filter_3d(cov, alpha, var);
+}
+// Stage 2's SH colour (eval_sh, FP64 twin): clamp(0.5 + sum c Y(d), 0, 1)
+ORC_API void orc_eval_sh64(const double* coeffs, const double* dir, uint32_t degree,
+                           double* out) {
+  eval_sh<double>(coeffs, dir[0], dir[1], dir[2], degree, out);
 }

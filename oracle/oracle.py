@@ -53,6 +53,17 @@ def lib() -> C.CDLL:
             "orc_tile_ranges": ([_vp, _u64, _u32, _u32, _vp], None),
             "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _vp, _i, _vp, _vp],
                                  None),
+            "orc_mvc_scene": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp,
+                               _vp, _vp], _u64),
+            "orc_sh_tables": ([_vp], None),
+            "orc_sh_band_matrix": ([_vp, _i, _vp], None),
+            "orc_sh_rotate": ([_vp, _vp, _i], None),
+            "orc_polar_rotation": ([_vp, _vp], None),
+            "orc_sym_eigen": ([_vp, _vp, _vp], None),
+            "orc_max_stretch": ([_vp], _f),
+            "orc_split_major": ([_vp, _vp, _vp], None),
+            "orc_mip_filter_3d": ([_vp, _vp, _f], None),
+            "orc_eval_sh64": ([_vp, _vp, _u32, _vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -372,3 +383,92 @@ def mvc_weights(verts, tris, points, fp64=False):
     L.orc_mvc_weights(_p(v), v.shape[0], _p(t), t.shape[0], _p(p), p.shape[0], 1 if fp64 else 0,
                       _p(w))
     return w
+
+
+# ---- MVC scene deformation (deform_scene_cpu, mvc.rs:564-749) ----
+# 3x3 matrices cross this boundary as numpy [3, 3] arrays; the library stores
+# them column-major (float[9], column k = m[3k..3k+2]).
+
+def _m3_in(m):
+    return np.ascontiguousarray(np.asarray(m, np.float32).T).reshape(9)
+
+
+def _m3_out(a):
+    return a.reshape(3, 3).T.copy()
+
+
+def mvc_scene(def_pos, def_F, mask, rest_pos, rest_cov, rest_opa, rest_sh, sh_degree=3,
+              tau=0.0, mip=0.0):
+    """PolyCageMesh::deform_scene_cpu after x' and F (def_F: [n, 9] column-major,
+    as the library's Jacobians): (pos [m,3], cov [m,6], opa [m], sh [m,48])
+    with m = n + splits, children appended in parent order."""
+    n = int(np.asarray(rest_opa).shape[0])
+    f = lambda a, k: np.ascontiguousarray(a, dtype=np.float32).reshape(n, k)
+    dp, dF, rp, rc, rs = f(def_pos, 3), f(def_F, 9), f(rest_pos, 3), f(rest_cov, 6), f(rest_sh, 48)
+    ro = np.ascontiguousarray(rest_opa, dtype=np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint32)
+    op, oc = np.zeros((2 * n, 3), np.float32), np.zeros((2 * n, 6), np.float32)
+    oo, osh = np.zeros(2 * n, np.float32), np.zeros((2 * n, 48), np.float32)
+    total = lib().orc_mvc_scene(_p(dp), _p(dF), _p(m), _p(rp), _p(rc), _p(ro), _p(rs), n,
+                                int(sh_degree), float(tau), float(mip), _p(op), _p(oc), _p(oo),
+                                _p(osh))
+    return op[:total], oc[:total], oo[:total], osh[:total]
+
+
+def sh_tables():
+    t = np.zeros(218, np.float64)
+    lib().orc_sh_tables(_p(t))
+    return t
+
+
+def sh_band_matrix(R, l):
+    """(2l+1) x (2l+1) rotation matrix of SH band l for the rotation R."""
+    M = np.zeros((2 * l + 1) ** 2, np.float32)
+    lib().orc_sh_band_matrix(_p(_m3_in(R)), int(l), _p(M))
+    return M.reshape(2 * l + 1, 2 * l + 1)
+
+
+def sh_rotate(sh, R, degree):
+    out = np.ascontiguousarray(sh, dtype=np.float32).copy()
+    lib().orc_sh_rotate(_p(out), _p(_m3_in(R)), int(degree))
+    return out
+
+
+def polar_rotation(J):
+    R = np.zeros(9, np.float32)
+    lib().orc_polar_rotation(_p(_m3_in(J)), _p(R))
+    return _m3_out(R)
+
+
+def sym_eigen(cov):
+    """(eigenvalues descending [3], basis [3,3] with the eigenvectors as columns)"""
+    ev, B = np.zeros(3, np.float32), np.zeros(9, np.float32)
+    lib().orc_sym_eigen(_p(np.ascontiguousarray(cov, dtype=np.float32)), _p(ev), _p(B))
+    return ev, _m3_out(B)
+
+
+def max_stretch(J):
+    return float(lib().orc_max_stretch(_p(_m3_in(J))))
+
+
+def split_major(cov):
+    """(child covariance [6], offset [3]): children at -offset and +offset"""
+    c, o = np.zeros(6, np.float32), np.zeros(3, np.float32)
+    lib().orc_split_major(_p(np.ascontiguousarray(cov, dtype=np.float32)), _p(c), _p(o))
+    return c, o
+
+
+def mip_filter_3d(cov, alpha, var):
+    c = np.ascontiguousarray(cov, dtype=np.float32).copy()
+    a = np.array([alpha], np.float32)
+    lib().orc_mip_filter_3d(_p(c), _p(a), float(var))
+    return c, float(a[0])
+
+
+def eval_sh64(coeffs, direction, degree):
+    """Stage 2's SH colour in float64: clamp(0.5 + sum c Y(d), 0, 1) per channel."""
+    out = np.zeros(3, np.float64)
+    lib().orc_eval_sh64(_p(np.ascontiguousarray(coeffs, dtype=np.float64)),
+                        _p(np.ascontiguousarray(direction, dtype=np.float64)), int(degree),
+                        _p(out))
+    return out

@@ -10,13 +10,13 @@ surface, 7e-5 on positions, 1.6e-2 on the central-difference Jacobian);
 deformed positions and Jacobians equal to float64 sums over the same weights
 to 1e-5 relative."""
 import json
-import math
 import os
 
 import numpy as np
 import pytest
 
-from cindergs_amd.mvc import MvcGpu, PolyCage, icosphere
+from cage_presets import preset as _preset
+from cindergs_amd.mvc import MvcGpu, icosphere
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -62,26 +62,6 @@ def test_icosphere_bind_matches_fp64_oracle(gpu, oracle_lib):
     assert np.all(W[~M] == 0) and np.all(G[~M] == 0)
     for b in (dp, w, g, m):
         b.free()
-
-
-def _preset(kind, a, b):
-    m = FIX["mvc_cages"]
-    if kind == "box":
-        lo, hi = a, b
-        V = [[lo[0], lo[1], lo[2]], [hi[0], lo[1], lo[2]], [hi[0], hi[1], lo[2]],
-             [lo[0], hi[1], lo[2]], [lo[0], lo[1], hi[2]], [hi[0], lo[1], hi[2]],
-             [hi[0], hi[1], hi[2]], [lo[0], hi[1], hi[2]]]
-        return PolyCage(np.array(V, np.float32), np.array(m["box_triangles"]))
-    c, r = np.array(a, np.float32), np.float32(max(abs(b), 1e-4))
-    if kind == "octahedron":
-        d = [[r, 0, 0], [-r, 0, 0], [0, r, 0], [0, -r, 0], [0, 0, r], [0, 0, -r]]
-        return PolyCage(c + np.array(d, np.float32), np.array(m["octahedron_triangles"]))
-    phi = (1.0 + math.sqrt(5.0)) * 0.5
-    raw = np.array([[-1, phi, 0], [1, phi, 0], [-1, -phi, 0], [1, -phi, 0], [0, -1, phi],
-                    [0, 1, phi], [0, -1, -phi], [0, 1, -phi], [phi, 0, -1], [phi, 0, 1],
-                    [-phi, 0, -1], [-phi, 0, 1]], np.float32)
-    raw = raw / np.linalg.norm(raw, axis=1, keepdims=True)
-    return PolyCage(c + raw * r, np.array(m["icosahedron_triangles"]))
 
 
 def test_reference_mvc_cases_on_gpu(gpu):

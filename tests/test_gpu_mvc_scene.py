@@ -1,0 +1,156 @@
+"""cgs_mvc_deform_scene (PolyCageMesh::deform_scene_cpu, src/deformation/
+mvc.rs:564-749) against the oracle's sequential restatement.
+
+The oracle runs on the device's own x' and F (cgs_mvc_deform over the same
+binding), so every output is compared bit for bit: positions, covariances,
+opacities, SH records, the record count and the order of the appended
+children.  The oracle itself is pinned on the CPU against float64 and the
+reference's unit tests (test_oracle_scene.py)."""
+import numpy as np
+import pytest
+
+from cage_presets import FIX, scene_case
+from cindergs_amd._lib import GpuError
+from cindergs_amd.mvc import MvcGpu, icosphere
+
+pytestmark = pytest.mark.gpu
+
+
+def _ball_points(n, r, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1, keepdims=True) * r * rng.uniform(0, 1, (n, 1)) ** (1 / 3)
+            ).astype(np.float32)
+
+
+def _rest_records(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, 3, 3)) * 0.03
+    C = A @ A.transpose(0, 2, 1) + np.eye(3) * 1e-5
+    cov = np.stack([C[:, 0, 0], C[:, 0, 1], C[:, 0, 2], C[:, 1, 1], C[:, 1, 2], C[:, 2, 2]], 1)
+    return (cov.astype(np.float32), rng.uniform(0.05, 0.99, n).astype(np.float32),
+            (rng.uniform(-1, 1, (n, 48)) * 0.5).astype(np.float32))
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def _run(gpu, oracle_lib, cage, Vd, pts, cov, opa, sh, degree, tau, mip, use_mask=True):
+    n = pts.shape[0]
+    mg = MvcGpu(gpu, cage)
+    dp = gpu.to_device(pts)
+    w, g, m = mg.bind(dp, n)
+    mask = m if use_mask else None
+    pos, F = mg.deform(w, g, mask, dp, Vd, n)
+    rest = {"pos": dp, "cov": gpu.to_device(cov), "opa": gpu.to_device(opa),
+            "sh": gpu.to_device(sh)}
+    count, out = mg.deform_scene(w, g, mask, rest, Vd, n, degree, tau, mip)
+    got = [out[k].numpy()[:count] for k in ("pos", "cov", "opa", "sh")]
+    M = m.numpy() if use_mask else None
+    ref = oracle_lib.mvc_scene(pos.numpy(), F.numpy(), M, pts, cov, opa, sh, degree, tau, mip)
+    extra = {"F": F.numpy(), "mask": m.numpy(), "pos": pos.numpy()}
+    for b in [dp, w, g, m, pos, F, rest["cov"], rest["opa"], rest["sh"]] + list(out.values()):
+        b.free()
+    return count, got, ref, extra
+
+
+@pytest.mark.parametrize("degree,split,mip,use_mask", [
+    (3, True, 0.0025, True), (3, False, 0.0, True), (2, True, 0.0, True),
+    (1, False, 0.001, False), (0, True, 0.0025, True)])
+def test_deform_scene_matches_oracle_bit_exact(gpu, oracle_lib, degree, split, mip, use_mask):
+    cage = icosphere()
+    inside = _ball_points(2000, 1.0, 5)
+    outside = (_ball_points(48, 1.0, 6) * 0.1 + np.array([0, 0, 2.5], np.float32))
+    pts = np.ascontiguousarray(np.concatenate([inside, outside]), dtype=np.float32)
+    n = pts.shape[0]
+    rng = np.random.default_rng(8)
+    th = 0.5
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    V = cage.vertices.astype(np.float64)
+    # a twist-and-stretch: stretch along x grows with height, plus noise
+    stretch = 1.0 + 0.9 * (V[:, 2:3] + 1.2) / 2.4
+    Vd = np.concatenate([V[:, :1] * stretch, V[:, 1:2], V[:, 2:3] * 0.9], 1) @ Rz.T
+    Vd = (Vd + rng.uniform(-0.03, 0.03, V.shape)).astype(np.float32)
+    cov, opa, sh = _rest_records(n, 9)
+    tau = 0.0
+    if split:
+        # the median largest stretch of the device's own Jacobians: about half split
+        mg = MvcGpu(gpu, cage)
+        dp = gpu.to_device(pts)
+        w, g, m = mg.bind(dp, n)
+        _, F = mg.deform(w, g, m, dp, Vd, n)
+        Fh, M = F.numpy(), m.numpy().astype(bool)
+        for b in (dp, w, g, m, F):
+            b.free()
+        s = np.array([oracle_lib.max_stretch(f.reshape(3, 3).T) for f in Fh[M]])
+        tau = float(np.median(s))
+    print(f"[case] mvc deform_scene n={n} degree={degree} tau={tau:.4f} mip={mip} "
+          f"mask={use_mask}", flush=True)
+    count, got, ref, extra = _run(gpu, oracle_lib, cage, Vd, pts, cov, opa, sh, degree, tau, mip,
+                                  use_mask)
+    assert count == ref[0].shape[0]
+    if split:
+        assert n + 500 < count < n + 1500
+    else:
+        assert count == n
+    for name, a, b in zip(("pos", "cov", "opa", "sh"), got, ref):
+        np.testing.assert_array_equal(_bits(a), _bits(b), err_msg=name)
+    if use_mask:
+        ub = np.where(extra["mask"] == 0)[0]
+        assert ub.size == 48
+        np.testing.assert_array_equal(got[0][ub], pts[ub])
+        np.testing.assert_array_equal(got[3][ub], sh[ub])
+
+
+@pytest.mark.parametrize("idx", [0, 1])
+def test_reference_deform_scene_cases(gpu, oracle_lib, idx):
+    """mvc.rs:854-884 (split count 3, outside splat kept) and
+    integration_tests.rs:1755-1801 (translated box moves x 0.2 -> 0.6)."""
+    case = FIX["mvc_scene"][idx]
+    cage, Vd, pos, cov, opa, sh = scene_case(case)
+    print(f"[case] mvc deform_scene reference {case['name']}", flush=True)
+    count, got, ref, extra = _run(gpu, oracle_lib, cage, Vd, pos, cov, opa, sh,
+                                  case["sh_degree"], case["split_threshold"],
+                                  case["mip_filter_3d_var"])
+    np.testing.assert_array_equal(extra["mask"].astype(bool), case["bound_mask"])
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(_bits(a), _bits(b))
+    if idx == 0:
+        assert count == case["expected_count"]
+        assert abs(got[0][1, 0] - case["outside_x"]) < case["tol"]
+    else:
+        assert count == 1
+        assert abs(got[0][0, 0] - case["expected_x"]) < case["tol"]
+
+
+def test_deform_scene_empty_repeat_and_errors(gpu, oracle_lib):
+    cage = icosphere()
+    mg = MvcGpu(gpu, cage)
+    print("[case] mvc deform_scene empty / repeat / errors", flush=True)
+    # n = 0: nothing to do, count 0
+    tiny = {k: gpu.to_device(np.zeros(8, np.float32)) for k in ("pos", "cov", "opa", "sh")}
+    count, out0 = mg.deform_scene(None, None, None, tiny, cage.vertices, 0, 3, 1.5, 0.0)
+    assert count == 0
+    for b in list(tiny.values()) + list(out0.values()):
+        b.free()
+    # the same call twice into the same outputs gives the same records
+    pts = _ball_points(300, 0.9, 11)
+    n = pts.shape[0]
+    dp = gpu.to_device(pts)
+    w, g, m = mg.bind(dp, n)
+    cov, opa, sh = _rest_records(n, 12)
+    rest = {"pos": dp, "cov": gpu.to_device(cov), "opa": gpu.to_device(opa),
+            "sh": gpu.to_device(sh)}
+    Vd = (cage.vertices * np.array([1.6, 1.0, 0.8], np.float32)).astype(np.float32)
+    c1, out = mg.deform_scene(w, g, m, rest, Vd, n, 3, 1.4, 0.002)
+    first = [out[k].numpy()[:c1] for k in ("pos", "cov", "opa", "sh")]
+    c2, out = mg.deform_scene(w, g, m, rest, Vd, n, 3, 1.4, 0.002, out=out)
+    assert c1 == c2 and c1 > n
+    for a, k in zip(first, ("pos", "cov", "opa", "sh")):
+        np.testing.assert_array_equal(_bits(a), _bits(out[k].numpy()[:c2]))
+    # the weight gradients are required
+    with pytest.raises(GpuError):
+        mg.deform_scene(w, None, m, rest, Vd, n, 3, 1.4, 0.0, out=out)
+    for b in [dp, w, g, m, rest["cov"], rest["opa"], rest["sh"]] + list(out.values()):
+        b.free()
