@@ -351,6 +351,83 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int):
     return res
 
 
+def mvc_scene(n_points: int, reps: int, cpu_points: int):
+    """PolyCageMesh::deform_scene_cpu on the GPU (cgs_mvc_deform_scene) for
+    n_points splats bound to the 642-vertex icosphere cage: rest covariances
+    of scales U(0.005, 0.03), SH degree 3, the cage stretched along x by
+    1 + 0.8 (z + 1.2) / 2.4 and squashed 0.9 along z, split threshold 1.4,
+    3D mip filter 0.0025.  Output buffers hold 2n records and are allocated
+    before the timed calls; the oracle's sequential restatement is timed on
+    the first cpu_points splats of the same binding (one core)."""
+    from cindergs_amd import GpuPipelineState
+    from cindergs_amd.mvc import MvcGpu, icosphere, sample_inside
+
+    st = GpuPipelineState(0)
+    lib = st._lib
+    cage = icosphere(3, 1.2, 0.15)
+    ex = MvcGpu(st, cage)
+    nv = cage.num_vertices
+    pts = sample_inside(cage, n_points, seed=31, inside_fn=lambda p: _winding(ex, st, p))
+    rng = np.random.Generator(np.random.PCG64(77))
+    sc = rng.uniform(0.005, 0.03, (n_points, 3))
+    cov = np.zeros((n_points, 6), np.float32)
+    cov[:, 0], cov[:, 3], cov[:, 5] = sc[:, 0] ** 2, sc[:, 1] ** 2, sc[:, 2] ** 2
+    opa = rng.uniform(0.05, 0.99, n_points).astype(np.float32)
+    sh = (rng.uniform(-1, 1, (n_points, 48)) * 0.4).astype(np.float32)
+    V = cage.vertices.astype(np.float64)
+    Vd = np.concatenate([V[:, :1] * (1.0 + 0.8 * (V[:, 2:3] + 1.2) / 2.4), V[:, 1:2],
+                         V[:, 2:3] * 0.9], 1).astype(np.float32)
+    d_pts = st.to_device(pts)
+    w, g, m = ex.bind(d_pts, n_points)
+    rest = {"pos": d_pts, "cov": st.to_device(cov), "opa": st.to_device(opa),
+            "sh": st.to_device(sh)}
+    tau, mip = 1.4, 0.0025
+    count, out = ex.deform_scene(w, g, m, rest, Vd, n_points, 3, tau, mip)  # warm-up
+    st.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        count, out = ex.deform_scene(w, g, m, rest, Vd, n_points, 3, tau, mip, out=out)
+    st.synchronize()
+    gpu_ms = 1000 * (time.perf_counter() - t) / reps
+    # device times of the last call's launches (HIP events on the context
+    # stream); k_mvc_deform streams the binding: 16 nv bytes per bound splat
+    # (weights + gradients) and 52 per splat (mask, x', F), plus the vertices
+    kern = {k: round(v, 4) for k, v in st.kernel_times()}
+    n_bound = int(m.numpy().astype(bool).sum())
+    dbytes = 16 * nv * n_bound + 52 * n_points + 12 * nv
+    dms = kern.get("k_mvc_deform", 0.0)
+    res = {"workload": f"{n_points} splats bound to a {nv}-vertex cage, SH degree 3, "
+                       f"split threshold {tau}, mip {mip}",
+           "ms": round(gpu_ms, 3), "splats_per_s": round(n_points / (gpu_ms / 1000)),
+           "records_out": count, "split_fraction": round((count - n_points) / n_points, 4),
+           "kernels_ms": kern,
+           "k_mvc_deform_roofline": {
+               "bound": "hbm", "bytes_per_launch": dbytes,
+               "achieved": round(dbytes / (dms * 1e-3) / 1e9, 1) if dms > 0 else None,
+               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(dbytes / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if dms > 0 else None}}
+    # the oracle on a bounded sample, over the device's own x' and F
+    k = min(cpu_points, n_points)
+    if k > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # the checker, timed here as the CPU baseline of this leg
+
+        pos, F = ex.deform(w, g, m, d_pts, Vd, n_points)
+        xh, Fh, mh = pos.numpy()[:k], F.numpy()[:k], m.numpy()[:k]
+        pos.free()
+        F.free()
+        oracle.lib()
+        t = time.perf_counter()
+        oracle.mvc_scene(xh, Fh, mh, pts[:k], cov[:k], opa[:k], sh[:k], 3, tau, mip)
+        cpu_s = time.perf_counter() - t
+        res["cpu_oracle"] = {"splats_per_s": round(k / cpu_s), "cores": 1, "kind": "port",
+                             "sample": f"first {k} splats of the same binding"}
+    for b in [d_pts, w, g, m, rest["cov"], rest["opa"], rest["sh"]] + list(out.values()):
+        b.free()
+    st.close()
+    return res
+
+
 def _winding(ex, st, p):
     d = st.to_device(p)
     w = ex.winding(d, p.shape[0])
@@ -488,10 +565,13 @@ def main() -> int:
     ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER",
                     help="render the headline with these launch shapes (no autotune)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the 8K, f16 and cage-deformation measurements")
+                    help="skip the 8K, f16, cage-deformation and MVC scene measurements")
     ap.add_argument("--cage-frames", type=int, default=300)
     ap.add_argument("--mvc", type=int, default=0, metavar="POINTS",
                     help="also time exact vs surrogate MVC binding of POINTS points")
+    ap.add_argument("--mvc-scene", type=int, default=1_073_930, metavar="SPLATS",
+                    help="MVC scene deformation of SPLATS bound splats (0 = skip; part of the extras)")
+    ap.add_argument("--mvc-scene-only", action="store_true")
     ap.add_argument("--mvc-steps", type=int, default=20000)
     ap.add_argument("--mvc-train", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -526,6 +606,9 @@ def main() -> int:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         print(json.dumps({"batched": batched_views(args.batch_views, args.batch_reps, lanes,
                                                    fixed=fixed)}), flush=True)
+        return 0
+    if args.mvc_scene_only:
+        print(json.dumps({"mvc_scene": mvc_scene(args.mvc_scene, 5, 16384)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -653,6 +736,9 @@ def main() -> int:
         print(f"# f16: {result['f16']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
         result["cage"] = cage_frames(scene, args.cage_frames)
         print(f"# cage: {result['cage']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
+        if args.mvc_scene > 0:
+            result["mvc_scene"] = mvc_scene(args.mvc_scene, 5, 16384)
+            print(f"# mvc scene: {result['mvc_scene']['ms']} ms", file=sys.stderr, flush=True)
     if args.batch_views > 0:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes, fixed=fixed)

@@ -393,7 +393,6 @@ struct CgsMvcSceneArgs {
   const float* rest_cov;     // [6n]
   const float* rest_opa;     // [n]
   const float* rest_sh;      // [48n]
-  const double* sh_tables;   // CGS_SH_TABLE_DOUBLES (cgs_sh_basis_tables)
   uint32_t n, sh_degree;
   float tau, mip;
   float* out_pos;            // [3 * 2n]

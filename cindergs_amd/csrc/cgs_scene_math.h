@@ -475,3 +475,7 @@ static inline void cgs_sh_basis_tables(double* tables) {
       }
   }
 }
+
+// The same rotation as straight-line code (generated by gen_sh_rotation.py):
+// cgs_sh_rotate_fast(sh, R, degree, tables) gives the bits of cgs_sh_rotate.
+#include "cgs_sh_rotation_gen.h"

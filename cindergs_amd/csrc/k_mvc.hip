@@ -236,13 +236,34 @@ __global__ __launch_bounds__(64) void k_mvc_winding(CgsMvcArgs A, float* out) {
 }
 
 // evaluate_position (mvc.rs:473-481) and MvcBindings::local_jacobian (66-78)
-// for every bound point: x' = sum_j w_j v'_j, F = sum_j v'_j g_j^T, in vertex
-// order; one wave per point, lanes over vertices, fixed-order reduction.
-__global__ __launch_bounds__(64) void k_mvc_deform(const float* w, const float* g,
-                                                   const uint32_t* mask, const float* rest_pos,
-                                                   const float* dverts, uint32_t nv, uint32_t n,
-                                                   float* out_pos, float* out_F) {
-  const uint32_t i = blockIdx.x, lane = threadIdx.x;
+// for every bound point: x' = sum_j w_j v'_j, F = sum_j v'_j g_j^T.  One wave
+// per point (four per block), lanes over vertices with 12-byte loads of the
+// gradient and vertex triples, then a reduce-scatter butterfly that leaves
+// the 12 sums (x' and the 9 entries of F) on 12 lanes after 14 cross-lane
+// exchanges.  The reduction order is fixed, so the result is reproducible.
+struct F3 {
+  float x, y, z;
+};
+
+// lanes whose `bit` is clear keep the first half of a[0..2h), the others the
+// second half, each adding its partner's copy of the half it keeps
+template <int H>
+__device__ __forceinline__ void keep_half(float* a, uint32_t lane, int bit) {
+  const bool hi = (lane & (uint32_t)bit) != 0u;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float send = hi ? a[k] : a[H + k];
+    const float keep = hi ? a[H + k] : a[k];
+    a[k] = keep + __shfl_xor(send, bit, 64);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mvc_deform(const float* w, const float* g,
+                                                    const uint32_t* mask, const float* rest_pos,
+                                                    const float* dverts, uint32_t nv, uint32_t n,
+                                                    float* out_pos, float* out_F) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6);
   if (i >= n) return;
   if (mask && mask[i] == 0u) {
     if (lane < 3) out_pos[(size_t)i * 3 + lane] = rest_pos[(size_t)i * 3 + lane];
@@ -250,37 +271,45 @@ __global__ __launch_bounds__(64) void k_mvc_deform(const float* w, const float* 
     return;
   }
   const float* wr = w + (size_t)i * nv;
-  const float* gr = g ? g + (size_t)i * nv * 3 : nullptr;
-  float px = 0, py = 0, pz = 0;
-  float F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t v = lane; v < nv; v += 64u) {
-    const float vx = dverts[v * 3], vy = dverts[v * 3 + 1], vz = dverts[v * 3 + 2];
-    const float wv = wr[v];
-    px += vx * wv;
-    py += vy * wv;
-    pz += vz * wv;
-    if (gr) {
-      const float g0 = gr[(size_t)v * 3], g1 = gr[(size_t)v * 3 + 1], g2 = gr[(size_t)v * 3 + 2];
-      // column-major F: col k = sum v' * g[k]
-      F[0] += vx * g0; F[1] += vy * g0; F[2] += vz * g0;
-      F[3] += vx * g1; F[4] += vy * g1; F[5] += vz * g1;
-      F[6] += vx * g2; F[7] += vy * g2; F[8] += vz * g2;
-    }
-  }
-  px = wave_sum(px);
-  py = wave_sum(py);
-  pz = wave_sum(pz);
-  if (lane == 0) {
-    out_pos[(size_t)i * 3] = px;
-    out_pos[(size_t)i * 3 + 1] = py;
-    out_pos[(size_t)i * 3 + 2] = pz;
-  }
-  if (out_F && gr) {
+  const F3* gr = (g && out_F) ? reinterpret_cast<const F3*>(g + (size_t)i * nv * 3) : nullptr;
+  const F3* vv = reinterpret_cast<const F3*>(dverts);
+  // a[0..2] = x', a[3 + 3c + r] = F (column-major), a[12..15] padding
+  float a[16];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const float s = wave_sum(F[k]);
-      if (lane == 0) out_F[(size_t)i * 9 + k] = s;
+  for (int k = 0; k < 16; ++k) a[k] = 0.0f;
+  if (gr) {
+    for (uint32_t v = lane; v < nv; v += 64u) {
+      const F3 p = vv[v];
+      const float wv = __builtin_nontemporal_load(wr + v);
+      const F3 gv = gr[v];
+      a[0] += p.x * wv; a[1] += p.y * wv; a[2] += p.z * wv;
+      a[3] += p.x * gv.x; a[4] += p.y * gv.x; a[5] += p.z * gv.x;
+      a[6] += p.x * gv.y; a[7] += p.y * gv.y; a[8] += p.z * gv.y;
+      a[9] += p.x * gv.z; a[10] += p.y * gv.z; a[11] += p.z * gv.z;
     }
+  } else {
+    for (uint32_t v = lane; v < nv; v += 64u) {
+      const F3 p = vv[v];
+      const float wv = __builtin_nontemporal_load(wr + v);
+      a[0] += p.x * wv; a[1] += p.y * wv; a[2] += p.z * wv;
+    }
+  }
+  // 16 values -> 8 (bit 32) -> 4 (bit 16) -> 2 (bit 8) -> 1 (bit 4), then the
+  // last two butterflies over bits 2 and 1: lane l ends with the sum of
+  // value index ((l >> 5) & 1) * 8 + ((l >> 4) & 1) * 4 + ((l >> 3) & 1) * 2 + ((l >> 2) & 1)
+  keep_half<8>(a, lane, 32);
+  keep_half<4>(a, lane, 16);
+  keep_half<2>(a, lane, 8);
+  keep_half<1>(a, lane, 4);
+  a[0] += __shfl_xor(a[0], 2, 64);
+  a[0] += __shfl_xor(a[0], 1, 64);
+  if ((lane & 3u) == 0u) {
+    const uint32_t k = ((lane >> 5) & 1u) * 8u + ((lane >> 4) & 1u) * 4u + ((lane >> 3) & 1u) * 2u +
+                       ((lane >> 2) & 1u);
+    if (k < 3u)
+      out_pos[(size_t)i * 3 + k] = a[0];
+    else if (k < 12u && out_F && gr)
+      out_F[(size_t)i * 9 + (k - 3u)] = a[0];
   }
 }
 
@@ -337,7 +366,7 @@ __global__ __launch_bounds__(64) void k_mvc_scene(CgsMvcSceneArgs A) {
     cgs_m3_transpose(F, Ft);
     cgs_m3_mul(FS, Ft, FSF);
     cgs_m3_to_cov(FSF, cov);
-    cgs_sh_rotate(sh, R, (int)A.sh_degree, A.sh_tables);
+    cgs_sh_rotate_fast(sh, R, (int)A.sh_degree);
     split = A.tau > 0.0f && cgs_max_stretch(F) > A.tau;
   } else {
     for (int k = 0; k < 3; ++k) pos[k] = A.rest_pos[(size_t)i * 3 + k];
@@ -410,8 +439,8 @@ void cgs_launch_mvc_deform(const float* w, const float* g, const uint32_t* mask,
                            const float* rest_pos, const float* dverts, uint32_t nv, uint32_t n,
                            float* out_pos, float* out_F, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mvc_deform, dim3(n), dim3(64), 0, s, w, g, mask, rest_pos, dverts, nv, n,
-                     out_pos, out_F);
+  hipLaunchKernelGGL(k_mvc_deform, dim3((n + 3u) / 4u), dim3(256), 0, s, w, g, mask, rest_pos,
+                     dverts, nv, n, out_pos, out_F);
 }
 
 void cgs_launch_mvc_scene(const CgsMvcSceneArgs& a, hipStream_t s) {

@@ -25,7 +25,6 @@
 #include <vector>
 
 #include "../../include/cindergs.h"
-#include "cgs_scene_math.h"
 #include "cgs_internal.h"
 
 namespace {
@@ -191,9 +190,8 @@ struct cgs_context {
   FrameArena fa;
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
-  // MVC scene deformation: per-call working set and the SH rotation tables
-  DevBuf mvc_work, mvc_tables;
-  bool mvc_tables_ready = false;
+  // MVC scene deformation: per-call working set
+  DevBuf mvc_work;
 
   // ---- last frame ----
   uint32_t last_w = 0, last_h = 0;
@@ -742,8 +740,7 @@ void cgs_destroy(cgs_context* c) {
   DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
                     &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
-                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
-                    &c->mvc_tables};
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work};
   for (DevBuf* b : bufs) release(*b);
   c->fa.release_all();
   for (auto& e : c->ev)
@@ -1894,14 +1891,9 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
       !d_rest_cov || !d_rest_opa || !d_rest_sh || !d_out_pos || !d_out_cov || !d_out_opa ||
       !d_out_sh)
     return fail(CGS_ERR_INVALID_ARG, "null argument (the weight gradients are required)");
+  if (reinterpret_cast<uintptr_t>(d_out_sh) % 16u != 0u)
+    return fail(CGS_ERR_INVALID_ARG, "d_out_sh must be 16-byte aligned");
   const size_t nn = (size_t)n;
-  if (!c->mvc_tables_ready) {
-    double h_tables[CGS_SH_TABLE_DOUBLES];
-    cgs_sh_basis_tables(h_tables);
-    TRY(ensure_words(c->mvc_tables, CGS_SH_TABLE_DOUBLES * 2));
-    HIPCHK(hipMemcpy(c->mvc_tables.p, h_tables, sizeof(h_tables), hipMemcpyHostToDevice));
-    c->mvc_tables_ready = true;
-  }
   // working set (4-byte words, grown 1.5x and kept by the context): deformed
   // vertices [3 nv], x' [3n], F [9n], split flags [n], ranks [n], second
   // children [10n]
@@ -1916,8 +1908,13 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
   int rc = CGS_OK;
   hipError_t e = hipMemcpyAsync(dv, deformed_vertices, (size_t)num_vertices * 12,
                                 hipMemcpyHostToDevice, c->stream);
+  // per-launch device times of this call, for cgs_kernel_times
+  c->prof.on = true;
+  c->prof.n = 0;
+  c->prof_count = 0;
   if (e == hipSuccess) {
     // x' = sum w v', F = sum v' grad(w)^T (evaluate_position, local_jacobian)
+    c->prof.mark("k_mvc_deform", c->stream);
     cgs_launch_mvc_deform(d_weights, d_gradients, d_inside, d_rest_pos, dv, num_vertices,
                           (uint32_t)n, def_pos, def_F, c->stream);
     CgsMvcSceneArgs A;
@@ -1928,7 +1925,6 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
     A.rest_cov = d_rest_cov;
     A.rest_opa = d_rest_opa;
     A.rest_sh = d_rest_sh;
-    A.sh_tables = P<double>(c->mvc_tables);
     A.n = (uint32_t)n;
     A.sh_degree = sh_degree > 3 ? 3 : sh_degree;
     A.tau = split_threshold;
@@ -1939,19 +1935,31 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
     A.out_sh = d_out_sh;
     A.flags = flags;
     A.child_b = child_b;
+    c->prof.mark("k_mvc_scene", c->stream);
     cgs_launch_mvc_scene(A, c->stream);
     e = hipGetLastError();
     uint32_t splits = 0;
+    c->prof.mark("k_scan", c->stream);
     if (e == hipSuccess) rc = cgs_exclusive_scan_u32(c, flags, ranks, n, &splits);
     if (e == hipSuccess && rc == CGS_OK && splits) {
+      c->prof.mark("k_mvc_scene_children", c->stream);
       cgs_launch_mvc_scene_children(A, ranks, c->stream);
       e = hipGetLastError();
     }
+    c->prof.mark("end", c->stream);
     if (e == hipSuccess && rc == CGS_OK) {
       e = hipStreamSynchronize(c->stream);
       if (e == hipSuccess) *out_count = n + splits;
     }
+    if (e == hipSuccess && rc == CGS_OK) {
+      for (int i = 0; i + 1 < c->prof.n; ++i) {
+        c->prof_names[i] = c->prof.names[i];
+        c->prof_ms[i] = elapsed(c->prof.ev[i], c->prof.ev[i + 1]);
+      }
+      c->prof_count = c->prof.n > 0 ? c->prof.n - 1 : 0;
+    }
   }
+  c->prof.on = false;
   if (rc != CGS_OK) return rc;
   if (e != hipSuccess)
     return fail(CGS_ERR_EXECUTION, std::string("mvc deform scene: ") + hipGetErrorString(e));

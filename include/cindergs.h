@@ -137,7 +137,8 @@ int cgs_framebuffer_device_ptr(cgs_context* ctx, void** ptr, uint32_t* padded_ro
 
 /* Per-launch device times of the last cgs_execute_frame run with
  * profile_stages >= 2 (the reference's measure_stage, gpu_state.rs:575-608,
- * at kernel granularity): names[32*i] (NUL-terminated) and ms[i]. */
+ * at kernel granularity), or of the last cgs_mvc_deform_scene call:
+ * names[32*i] (NUL-terminated) and ms[i]. */
 int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, uint32_t* count);
 
 /* ---- launch tuning ----
@@ -297,8 +298,9 @@ int cgs_mvc_deform(cgs_context* ctx, const float* d_weights, const float* d_grad
  * child replaces it and the second is appended after the n records in parent
  * order.  The 3D mip filter (288-304) applies to every unsplit record and to
  * both children.  Inputs and outputs in device memory (outputs hold up to 2n
- * records), deformed_vertices [3*num_vertices] on the host; the weight
- * gradients are required; *out_count receives n + the number of splits. */
+ * records; d_out_sh 16-byte aligned, as cgs_malloc returns), deformed_vertices
+ * [3*num_vertices] on the host; the weight gradients are required; *out_count
+ * receives n + the number of splits. */
 int cgs_mvc_deform_scene(cgs_context* ctx, const float* d_weights, const float* d_gradients,
                          const uint32_t* d_inside, const float* deformed_vertices,
                          uint32_t num_vertices, uint64_t n, const float* d_rest_pos,

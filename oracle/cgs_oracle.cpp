@@ -1482,6 +1482,10 @@ ORC_API void orc_sh_rotate(float* sh, const float* R, int degree) {
   cgs_sh_basis_tables(t);
   cgs_sh_rotate(sh, R, degree, t);
 }
+// the kernel's straight-line form of the same rotation (bit-identical)
+ORC_API void orc_sh_rotate_fast(float* sh, const float* R, int degree) {
+  cgs_sh_rotate_fast(sh, R, degree);
+}
 ORC_API void orc_polar_rotation(const float* J, float* R) { cgs_polar_rotation(J, R); }
 ORC_API void orc_sym_eigen(const float* cov, float* evals, float* basis) {
   cgs_sym_eigen(cov, evals, basis);

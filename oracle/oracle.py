@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
             "orc_sh_tables": ([_vp], None),
             "orc_sh_band_matrix": ([_vp, _i, _vp], None),
             "orc_sh_rotate": ([_vp, _vp, _i], None),
+            "orc_sh_rotate_fast": ([_vp, _vp, _i], None),
             "orc_polar_rotation": ([_vp, _vp], None),
             "orc_sym_eigen": ([_vp, _vp, _vp], None),
             "orc_max_stretch": ([_vp], _f),
@@ -431,6 +432,13 @@ def sh_band_matrix(R, l):
 def sh_rotate(sh, R, degree):
     out = np.ascontiguousarray(sh, dtype=np.float32).copy()
     lib().orc_sh_rotate(_p(out), _p(_m3_in(R)), int(degree))
+    return out
+
+
+def sh_rotate_fast(sh, R, degree):
+    """The kernel's straight-line form of sh_rotate (cgs_sh_rotation_gen.h)."""
+    out = np.ascontiguousarray(sh, dtype=np.float32).copy()
+    lib().orc_sh_rotate_fast(_p(out), _p(_m3_in(R)), int(degree))
     return out
 
 

@@ -118,6 +118,18 @@ def test_deform_kernel_matches_float64_sums(gpu):
     ref = W @ Vd.astype(np.float64)
     assert np.allclose(P[M], ref[M], rtol=1e-5, atol=1e-6)
     np.testing.assert_array_equal(P[~M], p[~M])
+    # F = sum_j v'_j g_j^T, column-major [n, 9]
+    Fd = F.numpy().reshape(-1, 3, 3).transpose(0, 2, 1)
+    Fref = np.einsum("jr,njc->nrc", Vd.astype(np.float64), G.reshape(-1, cage.num_vertices, 3))
+    # f32 sums of 642 terms: within 1e-6 of the sum of magnitudes
+    mag = np.einsum("jr,njc->nrc", np.abs(Vd.astype(np.float64)),
+                    np.abs(G.reshape(-1, cage.num_vertices, 3)))
+    assert np.all(np.abs(Fd[M] - Fref[M]) <= 1e-6 * mag[M] + 1e-7)
+    np.testing.assert_array_equal(Fd[~M], np.broadcast_to(np.eye(3, dtype=np.float32),
+                                                          Fd[~M].shape))
+    # positions alone (no Jacobian output) give the same positions
+    pos2, _ = mg.deform(w, g, m, dp, Vd, p.shape[0], jacobians=False)
+    np.testing.assert_array_equal(pos2.numpy(), P)
     Fr = np.einsum("njk,jl->nkl", G, Vd.astype(np.float64)).reshape(-1, 9)  # column-major
     assert np.allclose(F.numpy()[M], Fr[M], rtol=1e-4, atol=1e-5)
     np.testing.assert_array_equal(F.numpy()[~M][0], np.eye(3).reshape(-1).astype(np.float32))

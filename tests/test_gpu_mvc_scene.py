@@ -6,6 +6,8 @@ binding), so every output is compared bit for bit: positions, covariances,
 opacities, SH records, the record count and the order of the appended
 children.  The oracle itself is pinned on the CPU against float64 and the
 reference's unit tests (test_oracle_scene.py)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -147,10 +149,18 @@ def test_deform_scene_empty_repeat_and_errors(gpu, oracle_lib):
     first = [out[k].numpy()[:c1] for k in ("pos", "cov", "opa", "sh")]
     c2, out = mg.deform_scene(w, g, m, rest, Vd, n, 3, 1.4, 0.002, out=out)
     assert c1 == c2 and c1 > n
+    # the call's launches are timed (cgs_kernel_times)
+    kt = gpu.kernel_times()
+    assert [k for k, _ in kt] == ["k_mvc_deform", "k_mvc_scene", "k_scan",
+                                  "k_mvc_scene_children"]
+    assert all(ms > 0 for _, ms in kt)
     for a, k in zip(first, ("pos", "cov", "opa", "sh")):
         np.testing.assert_array_equal(_bits(a), _bits(out[k].numpy()[:c2]))
-    # the weight gradients are required
+    # the weight gradients are required, and 16-byte aligned SH output
     with pytest.raises(GpuError):
         mg.deform_scene(w, None, m, rest, Vd, n, 3, 1.4, 0.0, out=out)
+    shifted = type("Shifted", (), {"ptr": ctypes.c_void_p(out["sh"].ptr.value + 4)})()
+    with pytest.raises(GpuError, match="aligned"):
+        mg.deform_scene(w, g, m, rest, Vd, n, 3, 1.4, 0.0, out=dict(out, sh=shifted))
     for b in [dp, w, g, m, rest["cov"], rest["opa"], rest["sh"]] + list(out.values()):
         b.free()
