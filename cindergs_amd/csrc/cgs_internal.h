@@ -314,7 +314,8 @@ void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row
 // hist_given the caller has already filled b.hist for the passes (pass p's
 // 2^digit_bits bins at b.hist + p * 2^digit_bits).  part_keys: 16384 (one
 // 1024-thread block per CU), 10240 (640 threads: ~256 partitions for 2.5M
-// keys) or 8192 (two 512-thread blocks per CU);
+// keys), 8192 (two 512-thread blocks per CU) or 4096 (256 threads, four
+// blocks per CU);
 // digit_bits: 8 (4 passes over 32 bits) or 4 (8 passes).  Every choice gives
 // the same stable order.
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,

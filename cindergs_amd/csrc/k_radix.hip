@@ -17,10 +17,10 @@
 namespace {
 
 constexpr int RI = 16;           // keys per thread
-constexpr int RP_MIN = 512 * RI; // smallest partition (status arrays are sized for it)
+constexpr int RP_MIN = 256 * RI; // smallest partition (status arrays are sized for it)
 constexpr int RADIX = 256;       // digits of the widest digit (8 bits)
 constexpr int MAX_PASSES = 4;    // passes of 8-bit digits over 32 bits
-constexpr int HIST_BLOCKS = 128; // partial histograms of the standalone sort
+constexpr int HIST_BLOCKS = 512; // partial histograms of the standalone sort
 
 __device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift, uint32_t db) {
   const uint32_t w = bits - shift >= db ? db : bits - shift;
@@ -28,7 +28,9 @@ __device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift, ui
 }
 
 // Partial digit histograms of all passes, one row of MAX_PASSES*256 words per
-// block (no global atomics: same-address device atomics serialise).
+// block (no global atomics: same-address device atomics serialise).  Each
+// thread reads four keys per step (one 16-byte load when the four are in
+// range) and adds every digit of every pass to the block's LDS histogram.
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* d_n,
                                                     uint32_t n_static, uint32_t cap,
                                                     uint32_t bits, uint32_t db,
@@ -39,26 +41,23 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
   for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += 256u) s_h[i] = 0u;
   __syncthreads();
   const uint32_t n = min(d_n ? *d_n : n_static, cap);
-  // Equal digits within a wave are merged first (8 ballots give the lanes that
-  // share a digit), so each distinct digit costs one LDS atomic per wave.
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t stride = gridDim.x * 256u;
-  for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += stride) {
-    const uint32_t i = base + lane;
-    const bool valid = i < n;
-    const uint32_t k = valid ? keys[i] : 0u;
-    const uint64_t vmask = __ballot(valid);
-    for (uint32_t p = 0; p < passes; ++p) {
-      const uint32_t sh = p * db;
-      const uint32_t d = (k >> sh) & digit_mask(bits, sh, db);
-      uint64_t m = vmask;
-      for (uint32_t b = 0; b < db; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
+  const uint32_t stride = gridDim.x * 1024u;
+  for (uint32_t i0 = blockIdx.x * 1024u + threadIdx.x * 4u; i0 < n; i0 += stride) {
+    uint32_t k[4];
+    if (i0 + 3u < n && (reinterpret_cast<uintptr_t>(keys + i0) & 15u) == 0u) {
+      const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+      k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < n ? keys[i0 + j] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (i0 + (uint32_t)j >= n) break;
+      for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t sh = p * db;
+        atomicAdd(&s_h[p * radix + ((k[j] >> sh) & digit_mask(bits, sh, db))], 1u);
       }
-      if (valid && cgs::mask_rank(m) == 0u)
-        atomicAdd(&s_h[p * radix + d], (uint32_t)__popcll(m));
     }
   }
   __syncthreads();
@@ -266,7 +265,10 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
   const uint32_t radix = 1u << db;
   const uint32_t passes = (bits + db - 1u) / db;
   const uint32_t parts_alloc = cgs_radix_num_parts(b.cap);  // status rows per pass
-  const uint32_t pk = part_keys == 8192u ? 8192u : part_keys == 10240u ? 10240u : 16384u;
+  const uint32_t pk = part_keys == 4096u    ? 4096u
+                      : part_keys == 8192u  ? 8192u
+                      : part_keys == 10240u ? 10240u
+                                            : 16384u;
   const uint32_t grid = (b.cap + pk - 1u) / pk;
   if (!hist_given) {
     if (prof) prof->mark("k_radix_hist", s);
@@ -299,15 +301,19 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
         hipLaunchKernelGGL((k_radix_onesweep<1024, 8>), dim3(grid), dim3(1024), 0, s, A);
       else if (pk == 10240u)
         hipLaunchKernelGGL((k_radix_onesweep<640, 8>), dim3(grid), dim3(640), 0, s, A);
-      else
+      else if (pk == 8192u)
         hipLaunchKernelGGL((k_radix_onesweep<512, 8>), dim3(grid), dim3(512), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_radix_onesweep<256, 8>), dim3(grid), dim3(256), 0, s, A);
     } else {
       if (pk == 16384u)
         hipLaunchKernelGGL((k_radix_onesweep<1024, 4>), dim3(grid), dim3(1024), 0, s, A);
       else if (pk == 10240u)
         hipLaunchKernelGGL((k_radix_onesweep<640, 4>), dim3(grid), dim3(640), 0, s, A);
-      else
+      else if (pk == 8192u)
         hipLaunchKernelGGL((k_radix_onesweep<512, 4>), dim3(grid), dim3(512), 0, s, A);
+      else
+        hipLaunchKernelGGL((k_radix_onesweep<256, 4>), dim3(grid), dim3(256), 0, s, A);
     }
     kin = A.keys_out;
     vin = A.vals_out;

@@ -1,6 +1,6 @@
 """Launch tuning (cgs_tuning, cgs_autotune) on the GPU.
 
-Bar: every tuning candidate (radix partition 16384/10240/8192 keys, 8- or 4-bit
+Bar: every tuning candidate (radix partition 16384/10240/8192/4096 keys, 8- or 4-bit
 digits, compositor batch 256/128/64, heaviest-tiles-first order on/off) gives
 bit-identical tile counts, scan offsets, sorted keys, sorted ids, tile ranges
 and RGBA8 to the default setting, which the parity tests hold to the oracle;
@@ -53,7 +53,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
         print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
         gpu.set_tuning(c)
         _same(ref, _frame(gpu, cam, comp, scene.count), f"candidate {i}")
-    assert len(seen) == n == 36
+    assert len(seen) == n == 48
     gpu.set_tuning(default)
 
 
@@ -64,7 +64,7 @@ def test_partition_sizes_on_64bit_keys(gpu):
     comp = syn.compositor(7680, 4320, (0.0, 0.0, -3.0))
     default = gpu.get_tuning()
     ref = _frame(gpu, cam, comp, scene.count)
-    for pk, batch in ((8192, 128), (10240, 256), (16384, 64)):
+    for pk, batch in ((8192, 128), (10240, 256), (16384, 64), (4096, 256)):
         t = Tuning(pk, 8, batch, 0)
         print(f"[case] 8K partition {pk} batch {batch}", flush=True)
         gpu.set_tuning(t)
@@ -91,7 +91,7 @@ def test_autotune_keeps_a_candidate_and_the_frame(gpu):
 def test_invalid_tuning_is_rejected(gpu):
     from cindergs_amd._lib import GpuError
 
-    for bad in (Tuning(4096, 8, 256, 1), Tuning(12288, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
+    for bad in (Tuning(2048, 8, 256, 1), Tuning(12288, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
                 Tuning(16384, 8, 256, 2)):
         with pytest.raises(GpuError):
             gpu.set_tuning(bad)
