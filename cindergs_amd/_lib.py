@@ -19,6 +19,7 @@ CGS_ERR_UNSUPPORTED = 3
 CGS_ERR_EXECUTION = 4
 CGS_ERR_SHORT_READBACK = 5
 CGS_ERR_INVALID_ARG = 6
+CGS_ERR_FORMAT = 7
 
 STATUS_NAMES = {
     CGS_ERR_RUNTIME: "Runtime",
@@ -27,7 +28,35 @@ STATUS_NAMES = {
     CGS_ERR_EXECUTION: "Execution",
     CGS_ERR_SHORT_READBACK: "ShortReadback",
     CGS_ERR_INVALID_ARG: "InvalidArgument",
+    CGS_ERR_FORMAT: "Format",
 }
+
+# PlyError kinds in PlyLayout.error (include/cindergs.h)
+PLY_ERRORS = {
+    1: "MissingEndHeader", 2: "InvalidMagic", 3: "InvalidHeaderUtf8", 4: "UnsupportedFormat",
+    5: "MissingFormat", 6: "MissingVertexElement", 7: "UnsupportedLayout",
+    8: "UnsupportedListProperty", 9: "UnsupportedElement", 10: "MissingProperty",
+    11: "InvalidPropertyType", 12: "UnsupportedShLayout", 13: "TruncatedPayload",
+}
+
+
+class PlyLayout(C.Structure):
+    """cgs_ply_layout: PlyHeader (ply_loader.rs:125-140) as a decode table."""
+
+    _fields_ = [
+        ("vertex_count", C.c_uint64),
+        ("payload_offset", C.c_uint64),
+        ("vertex_stride", C.c_uint32),
+        ("format", C.c_uint32),
+        ("sh_degree", C.c_uint32),
+        ("canonical", C.c_uint32),
+        ("num_properties", C.c_uint32),
+        ("error", C.c_uint32),
+        ("error_detail", C.c_uint64 * 2),
+        ("slot_offset", C.c_uint32 * 62),
+        ("slot_type", C.c_uint8 * 62),
+        ("pad", C.c_uint8 * 2),
+    ]
 
 
 class CameraUniforms(C.Structure):
@@ -173,6 +202,11 @@ SIGNATURES = {
     "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
     "cgs_mvc_deform_scene": [_vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _u32,
                              C.c_float, C.c_float, _vp, _vp, _vp, _vp, C.POINTER(_u64)],
+    "cgs_ply_parse_header": [_vp, _u64, _vp],
+    "cgs_ply_load": [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp],
+    "cgs_activate_raw": [_vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp],
+    "cgs_quantize_scene": [_vp, _vp, _vp, _u64, _u32, _vp, _vp, _vp],
+    "cgs_export_raw": [_vp, _vp, _vp, _vp, _vp, _u64, _vp],
     "cgs_mlp_create": [_vp, _u32, C.c_float, _u64, C.POINTER(_vp)],
     "cgs_mlp_destroy": [_vp],
     "cgs_mlp_param_count": [_vp],

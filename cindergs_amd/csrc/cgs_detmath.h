@@ -18,6 +18,11 @@
 //   acos: half-angle identities onto asin(z), |z| <= 1/2, asin Taylor series
 //         through z^21.
 //   cos : Taylor series through x^18, valid on [-pi, pi].
+//   atan2: |y|/|x| or |x|/|y| in [0, 1]; above tan(pi/8), atan t =
+//         pi/4 + atan((t-1)/(t+1)); atan u by its Taylor series through u^17
+//         on |u| <= tan(pi/8); then the octant and quadrant identities.
+//   f32 -> f16: IEEE round to nearest, ties to even, on the bit patterns
+//         (integer operations only), subnormal halves included.
 // Accuracy is pinned by tests against double-precision libm (a few ulp).
 #pragma once
 
@@ -170,3 +175,66 @@ CGS_HD float cgs_cosf(float x) {
 }
 
 CGS_HD float cgs_sqrtf(float x) { return __builtin_sqrtf(x); }
+
+// atan on [0, 1] (see the method above)
+CGS_HD float cgs_atan_unit(float t) {
+  const bool red = t > 0.414213562373095049f;
+  const float u = red ? (t - 1.0f) / (t + 1.0f) : t;
+  const float z = u * u;
+  const float p = 1.0f + z * (-0.333333333333333333f +
+                  z * (0.2f +
+                  z * (-0.142857142857142857f +
+                  z * (0.111111111111111111f +
+                  z * (-0.0909090909090909091f +
+                  z * (0.0769230769230769231f +
+                  z * (-0.0666666666666666667f +
+                  z * 0.0588235294117647059f)))))));
+  const float r = u * p;
+  return red ? 0.785398163397448310f + r : r;
+}
+
+// atan2(y, x) with the quadrant conventions of C's atan2 for finite inputs
+CGS_HD float cgs_atan2f(float y, float x) {
+  if (x != x || y != y) return x + y;
+  const float ax = x < 0.0f ? -x : x, ay = y < 0.0f ? -y : y;
+  const bool yneg = (cgs_f2u(y) >> 31) != 0u, xneg = (cgs_f2u(x) >> 31) != 0u;
+  float a;
+  if (ax == 0.0f && ay == 0.0f) {
+    a = xneg ? CGS_PI_F : 0.0f;
+  } else if (ay > ax) {
+    a = CGS_HALF_PI_F - cgs_atan_unit(ax / ay);
+    if (xneg) a = CGS_PI_F - a;
+  } else {
+    a = cgs_atan_unit(ay / ax);
+    if (xneg) a = CGS_PI_F - a;
+  }
+  return yneg ? -a : a;
+}
+
+// f32 -> IEEE binary16 bits, round to nearest even; NaN stays a quiet NaN
+// with the top payload bits, overflow gives infinity
+CGS_HD uint16_t cgs_f32_to_f16(float f) {
+  const uint32_t x = cgs_f2u(f);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) {
+    if (ax == 0x7F800000u) return (uint16_t)(sign | 0x7C00u);
+    return (uint16_t)(sign | 0x7E00u | ((ax >> 13) & 0x03FFu));
+  }
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // >= 65520 rounds to inf
+  if (ax >= 0x38800000u) {
+    // normal half: rebias the exponent by 127 - 15 and round the 13 dropped bits
+    const uint32_t r = ax - 0x38000000u + 0x0FFFu + ((ax >> 13) & 1u);
+    return (uint16_t)(sign | (r >> 13));
+  }
+  // subnormal half: round |f| / 2^-24 to an integer
+  const uint32_t e = ax >> 23;
+  if (e < 102u) return (uint16_t)sign;  // below 2^-25: rounds to zero
+  const uint32_t m = (ax & 0x007FFFFFu) | 0x00800000u;
+  const uint32_t sh = 126u - e;  // 14..24
+  uint32_t q = m >> sh;
+  const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+  if (rem > half || (rem == half && (q & 1u))) ++q;
+  return (uint16_t)(sign | q);
+}
+

@@ -472,9 +472,35 @@ int cgs_mlp_infer_residual(struct cgs_mlp* M, const float* d_x, uint64_t n, floa
 hipStream_t cgs_mlp_stream(struct cgs_mlp* M);
 uint32_t cgs_mlp_outputs(struct cgs_mlp* M);
 
+// ---- scene ingest (k_ingest.hip, ply.cpp) ----
+// decode + activation of n raw vertex records (a PLY payload) into the SoA
+struct CgsActivateArgs {
+  const uint8_t* records;    // n * stride bytes, 4-byte aligned
+  uint64_t n;
+  uint32_t stride;           // bytes per record
+  uint32_t le;               // little-endian scalars
+  uint32_t per_block;        // records per block (<= 256, per_block * stride <= 65536)
+  uint32_t word_path;        // every present slot is a 4-byte aligned LE float32 and stride % 4 == 0
+  uint32_t slot_offset[62];
+  uint8_t slot_type[62];
+  float* pos;                // [3n]
+  float* cov;                // [6n]
+  float* opa;                // [n]
+  float* sh;                 // [48n]
+};
+void cgs_launch_activate_raw(const CgsActivateArgs& a, hipStream_t s);
+void cgs_launch_quantize(const float* cov, const float* sh, uint64_t n, uint32_t degree,
+                         uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16, hipStream_t s);
+void cgs_launch_export_raw(const float* pos, const float* cov, const float* opa,
+                           const float* sh, uint64_t n, float* records, hipStream_t s);
+
 // error reporting and context access shared by the host files
 int cgs_fail(int code, const char* msg);
 hipStream_t cgs_context_stream(struct cgs_context* c);
+// the context's device made current (for ABI files outside runtime.cpp)
+int cgs_context_enter(struct cgs_context* c);
+// a device buffer of at least `bytes`, owned by the context (grown 1.5x, kept)
+int cgs_context_staging(struct cgs_context* c, uint64_t bytes, void** p);
 
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved

@@ -192,6 +192,8 @@ struct cgs_context {
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
   // MVC scene deformation: per-call working set
   DevBuf mvc_work;
+  // scene ingest: the vertex payload of a PLY file
+  DevBuf ingest;
 
   // ---- last frame ----
   uint32_t last_w = 0, last_h = 0;
@@ -740,7 +742,8 @@ void cgs_destroy(cgs_context* c) {
   DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
                     &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
-                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work};
+                    &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
+                    &c->ingest};
   for (DevBuf* b : bufs) release(*b);
   c->fa.release_all();
   for (auto& e : c->ev)
@@ -1969,6 +1972,13 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
 }  // extern "C"
 
 int cgs_fail(int code, const char* msg) { return fail(code, msg ? msg : "error"); }
+int cgs_context_enter(cgs_context* c) { return ensure_ctx(c); }
+int cgs_context_staging(cgs_context* c, uint64_t bytes, void** p) {
+  TRY(ensure_ctx(c));
+  TRY(ensure_words(c->ingest, (size_t)((bytes + 3u) / 4u)));
+  *p = c->ingest.p;
+  return CGS_OK;
+}
 hipStream_t cgs_context_stream(cgs_context* c) { return c ? c->stream : nullptr; }
 
 

@@ -60,6 +60,20 @@ class QuantizedGaussianSceneSoa:
     opacities: np.ndarray              # [n] f32
     sh_coeffs_f16_packed: np.ndarray    # [n, words] u32
     sh_degree: int
+    octahedral_rotations: Optional[np.ndarray] = None  # [n] u32 (soa.rs:369-402)
+
+    def total_bytes(self) -> int:
+        """Quantised footprint (soa.rs:581-589)."""
+        rot = 0 if self.octahedral_rotations is None else self.octahedral_rotations.size * 4
+        return (self.positions.size + self.covariances_f16_packed.size + self.opacities.size
+                + self.sh_coeffs_f16_packed.size) * 4 + rot
+
+    def unquantized_bytes(self) -> int:
+        return self.count * (3 + 6 + 1 + 48) * 4
+
+    def compression_ratio(self) -> float:
+        q = self.total_bytes()
+        return 1.0 if q == 0 else self.unquantized_bytes() / q
 
     @property
     def count(self) -> int:

@@ -30,7 +30,8 @@ typedef enum cgs_status {
   CGS_ERR_UNSUPPORTED = 3,
   CGS_ERR_EXECUTION = 4,
   CGS_ERR_SHORT_READBACK = 5,
-  CGS_ERR_INVALID_ARG = 6
+  CGS_ERR_INVALID_ARG = 6,
+  CGS_ERR_FORMAT = 7 /* malformed or unsupported input file (PlyError, ply_loader.rs:157-222) */
 } cgs_status;
 
 typedef struct cgs_context cgs_context;
@@ -309,6 +310,90 @@ int cgs_mvc_deform_scene(cgs_context* ctx, const float* d_weights, const float* 
                          uint32_t sh_degree, float split_threshold, float mip_filter_3d_var,
                          float* d_out_pos, float* d_out_cov, float* d_out_opa, float* d_out_sh,
                          uint64_t* out_count);
+
+/* ---- scene ingest and quantised storage ---------------------------------------
+ * PlyLoader (src/scene/ply_loader.rs:601-716), GaussianSceneSoa::push_raw /
+ * from_raw_vertices_par (soa.rs:113-189), QuantizedGaussianSceneSoa::
+ * from_soa_with_degree (454-500) and to_raw_vertices (306-328). */
+
+/* PlyFormat (ply_loader.rs:12-16): only the binary ones load */
+#define CGS_PLY_BINARY_LE 1u
+#define CGS_PLY_BINARY_BE 2u
+/* PlyError variants (ply_loader.rs:157-222), in cgs_ply_layout.error */
+#define CGS_PLY_OK 0u
+#define CGS_PLY_MISSING_END_HEADER 1u
+#define CGS_PLY_INVALID_MAGIC 2u
+#define CGS_PLY_INVALID_HEADER_UTF8 3u
+#define CGS_PLY_UNSUPPORTED_FORMAT 4u
+#define CGS_PLY_MISSING_FORMAT 5u
+#define CGS_PLY_MISSING_VERTEX_ELEMENT 6u
+#define CGS_PLY_UNSUPPORTED_LAYOUT 7u
+#define CGS_PLY_UNSUPPORTED_LIST 8u
+#define CGS_PLY_UNSUPPORTED_ELEMENT 9u  /* error_detail[0] = record count */
+#define CGS_PLY_MISSING_PROPERTY 10u
+#define CGS_PLY_INVALID_PROPERTY_TYPE 11u
+#define CGS_PLY_UNSUPPORTED_SH_LAYOUT 12u /* error_detail[0] = f_rest count */
+#define CGS_PLY_TRUNCATED_PAYLOAD 13u    /* error_detail = needed, available bytes */
+
+/* PlyHeader (ply_loader.rs:125-140) as a decode table: for each of the 62
+ * floats of a RawPlyVertex (raw_vertex.rs:16-37: position, normal, f_dc,
+ * f_rest 15 per channel, opacity, log-scale, quaternion w x y z), the byte
+ * offset in a vertex record and the scalar type of the property that fills it
+ * (slot_type 0: absent, the float stays 0; 1-8: int8, uint8, int16, uint16,
+ * int32, uint32, float32, float64).  A compact SH layout (9 or 24 f_rest) is
+ * scattered per channel as parse_vertex_payload does (540-598). */
+typedef struct cgs_ply_layout {
+  uint64_t vertex_count;
+  uint64_t payload_offset;   /* bytes before the vertex payload */
+  uint32_t vertex_stride;    /* bytes per vertex record */
+  uint32_t format;           /* CGS_PLY_BINARY_LE / _BE */
+  uint32_t sh_degree;        /* 0 / 9 / 24 / 45 f_rest give 0..3 */
+  uint32_t canonical;        /* the 62 float32 properties in RawPlyVertex order, LE */
+  uint32_t num_properties;
+  uint32_t error;            /* CGS_PLY_* of the last failure, 0 on success */
+  uint64_t error_detail[2];
+  uint32_t slot_offset[62];
+  uint8_t slot_type[62];
+  uint8_t pad[2];
+} cgs_ply_layout;
+
+/* parse_header (ply_loader.rs:290-468) of a PLY file in host memory; on a
+ * malformed header returns CGS_ERR_FORMAT with layout->error set and the
+ * reference's message in cgs_last_error(). */
+int cgs_ply_parse_header(const uint8_t* bytes, uint64_t len, cgs_ply_layout* layout);
+
+/* PlyLoader::load_from_bytes (622-654) into device memory: the header is
+ * parsed on the host (layout may be NULL, or the result of
+ * cgs_ply_parse_header on the same bytes), the vertex payload is copied to
+ * the device as is, and one kernel decodes and activates every record:
+ * d_pos [3n], d_cov [6n], d_opa [n], d_sh [48n] (n = vertex_count).  A payload
+ * shorter than n * stride gives CGS_PLY_TRUNCATED_PAYLOAD. */
+int cgs_ply_load(cgs_context* ctx, const uint8_t* bytes, uint64_t len,
+                 const cgs_ply_layout* layout, float* d_pos, float* d_cov, float* d_opa,
+                 float* d_sh);
+
+/* from_raw_vertices_par (soa.rs:137-189) on a payload already in device memory
+ * (d_records: n records of layout->vertex_stride bytes, 4-byte aligned). */
+int cgs_activate_raw(cgs_context* ctx, const void* d_records, const cgs_ply_layout* layout,
+                     uint64_t n, float* d_pos, float* d_cov, float* d_opa, float* d_sh);
+
+/* QuantizedGaussianSceneSoa::from_soa_with_degree (soa.rs:454-500) on the
+ * device: per splat three f16 pairs of the covariance (pack_f16_pair, round
+ * to nearest even), the octahedral rotation of its principal axes
+ * (decompose_covariance_3d, encode_octahedral_rotation, 369-402), and the
+ * degree-trimmed f16 SH words ((degree + 1)^2 * 3 halves, rounded up to
+ * whole words; 2 / 6 / 14 / 24 per splat for degree 0..3).  Positions and
+ * opacities are unchanged and stay with the caller. */
+int cgs_quantize_scene(cgs_context* ctx, const float* d_cov, const float* d_sh, uint64_t n,
+                       uint32_t sh_degree, uint32_t* d_cov_f16, uint32_t* d_rot_oct,
+                       uint32_t* d_sh_f16);
+
+/* GaussianSceneSoa::to_raw_vertices (soa.rs:306-328) on the device: n
+ * RawPlyVertex records (248 bytes: logit opacity, log-scales and the WXYZ
+ * quaternion of decompose_covariance_3d, zero normals) into d_records, the
+ * payload of PlyLoader::write_binary_ply_bytes (674-705). */
+int cgs_export_raw(cgs_context* ctx, const float* d_pos, const float* d_cov,
+                   const float* d_opa, const float* d_sh, uint64_t n, void* d_records);
 
 /* ---- MVC surrogate network (no counterpart in the reference) ------------------
  * A coordinate MLP that stands in for exact MVC when binding many points to one

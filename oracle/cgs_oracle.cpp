@@ -19,6 +19,8 @@
 //            central-difference gradients)
 //   MVC scene src/deformation/mvc.rs:564-749 (deform_scene_cpu: polar
 //            rotation, F Sigma F^T, SH rotation, strain split, 3D mip filter)
+//   Ingest   src/scene/ply_loader.rs:540-598 (parse_vertex_payload),
+//            soa.rs:113-126 (push_raw), 454-500 (quantize), 306-328 (export)
 //
 // Every routine is a template over the scalar type: the float instantiation is
 // the FP32 oracle (elementary functions from cgs_detmath.h so that the result
@@ -40,6 +42,12 @@
 //                              1146-1319), polar identities, FP64 twin
 //   MVC                     : partition of unity, linear reproduction,
 //                              FP64 twin, finite-difference consistency
+//   ingest (cgs_ingest_math.h, shared with the kernels): decode against the
+//                              tests' own struct unpacking, activation against
+//                              a float64 closed form (R(q) diag(s^2) R^T,
+//                              sigmoid, exp), f16 against numpy's float16,
+//                              atan2 against libm, the octahedral code by its
+//                              decode round trip, export by reload
 //   MVC scene (per-splat math of cgs_scene_math.h, shared with the kernel):
 //                              SH rotation orthogonal / matches the rotated
 //                              direction under this file's eval_sh / identity
@@ -61,6 +69,7 @@
 
 #include "../cindergs_amd/csrc/cgs_detmath.h"
 #include "../cindergs_amd/csrc/cgs_scene_math.h"
+#include "../cindergs_amd/csrc/cgs_ingest_math.h"
 
 #define ORC_API extern "C" __attribute__((visibility("default")))
 
@@ -1501,4 +1510,46 @@ ORC_API void orc_mip_filter_3d(float* cov, float* alpha, float var) {
 ORC_API void orc_eval_sh64(const double* coeffs, const double* dir, uint32_t degree,
                            double* out) {
   eval_sh<double>(coeffs, dir[0], dir[1], dir[2], degree, out);
+}
+
+// ---------------------------------------------------------------------------
+// Scene ingest: parse_vertex_payload + push_raw (ply_loader.rs:540-598,
+// soa.rs:113-126), sequentially, record by record
+// ---------------------------------------------------------------------------
+
+ORC_API void orc_decode_activate(const uint8_t* payload, uint64_t n, uint32_t stride, int le,
+                                 const uint32_t* slot_offset, const uint8_t* slot_type,
+                                 float* raw_out, float* pos, float* cov, float* opa, float* sh) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint8_t* rec = payload + i * stride;
+    float raw[CGS_RAW_FLOATS];
+    for (uint32_t k = 0; k < CGS_RAW_FLOATS; ++k)
+      raw[k] = slot_type[k] ? cgs_pt_read(rec + slot_offset[k], slot_type[k], le != 0) : 0.0f;
+    if (raw_out) memcpy(raw_out + i * CGS_RAW_FLOATS, raw, sizeof(raw));
+    cgs_activate_record(raw, pos + i * 3, cov + i * 6, opa + i, sh + i * 48);
+  }
+}
+
+ORC_API void orc_quantize(const float* cov, const float* sh, uint64_t n, uint32_t degree,
+                          uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16) {
+  const uint32_t deg = degree > 3u ? 3u : degree, words = cgs_sh_words(deg);
+  for (uint64_t i = 0; i < n; ++i)
+    cgs_quantize_record(cov + i * 6, sh + i * 48, deg, cov_f16 + i * 3, rot + i,
+                        sh_f16 + i * words);
+}
+
+ORC_API void orc_export_raw(const float* pos, const float* cov, const float* opa,
+                            const float* sh, uint64_t n, float* raw) {
+  for (uint64_t i = 0; i < n; ++i)
+    cgs_export_record(pos + i * 3, cov + i * 6, opa[i], sh + i * 48, raw + i * CGS_RAW_FLOATS);
+}
+
+ORC_API void orc_f32_to_f16(const float* in, uint64_t n, uint16_t* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = cgs_f32_to_f16(in[i]);
+}
+ORC_API float orc_atan2f(float y, float x) { return cgs_atan2f(y, x); }
+ORC_API void orc_quat_from_m3(const float* m, float* q) { cgs_quat_from_m3(m, q); }
+ORC_API uint32_t orc_encode_octahedral(const float* q) { return cgs_encode_octahedral(q); }
+ORC_API void orc_decompose_covariance(const float* cov, float* scale, float* q) {
+  cgs_decompose_covariance(cov, scale, q);
 }

@@ -65,6 +65,15 @@ def lib() -> C.CDLL:
             "orc_split_major": ([_vp, _vp, _vp], None),
             "orc_mip_filter_3d": ([_vp, _vp, _f], None),
             "orc_eval_sh64": ([_vp, _vp, _u32, _vp], None),
+            "orc_decode_activate": ([_vp, _u64, _u32, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                                    None),
+            "orc_quantize": ([_vp, _vp, _u64, _u32, _vp, _vp, _vp], None),
+            "orc_export_raw": ([_vp, _vp, _vp, _vp, _u64, _vp], None),
+            "orc_f32_to_f16": ([_vp, _u64, _vp], None),
+            "orc_atan2f": ([_f, _f], _f),
+            "orc_quat_from_m3": ([_vp, _vp], None),
+            "orc_encode_octahedral": ([_vp], _u32),
+            "orc_decompose_covariance": ([_vp, _vp, _vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -480,3 +489,74 @@ def eval_sh64(coeffs, direction, degree):
                         _p(np.ascontiguousarray(direction, dtype=np.float64)), int(degree),
                         _p(out))
     return out
+
+
+# ---- scene ingest (ply_loader.rs:540-598, soa.rs:113-126, 306-328, 454-500) ----
+
+def decode_activate(payload, n, stride, little_endian, slot_offset, slot_type):
+    """parse_vertex_payload + push_raw restated over a slot table: (raw [n,62],
+    pos [n,3], cov [n,6], opa [n], sh [n,48])."""
+    pl = np.frombuffer(bytes(payload), np.uint8)
+    so = np.ascontiguousarray(slot_offset, dtype=np.uint32)
+    st = np.ascontiguousarray(slot_type, dtype=np.uint8)
+    raw = np.zeros((n, 62), np.float32)
+    pos, cov = np.zeros((n, 3), np.float32), np.zeros((n, 6), np.float32)
+    opa, sh = np.zeros(n, np.float32), np.zeros((n, 48), np.float32)
+    lib().orc_decode_activate(_p(np.ascontiguousarray(pl)), n, stride, 1 if little_endian else 0,
+                              _p(so), _p(st), _p(raw), _p(pos), _p(cov), _p(opa), _p(sh))
+    return raw, pos, cov, opa, sh
+
+
+def sh_words(degree):
+    bases = {0: 1, 1: 4, 2: 9}.get(int(degree), 16)
+    return (bases * 3 + 1) // 2
+
+
+def quantize(cov, sh, degree):
+    """from_soa_with_degree per splat: (cov_f16 [n,3] u32, rot [n] u32, sh_f16 [n,words] u32)"""
+    c = np.ascontiguousarray(cov, dtype=np.float32).reshape(-1, 6)
+    s = np.ascontiguousarray(sh, dtype=np.float32).reshape(-1, 48)
+    n, w = c.shape[0], sh_words(degree)
+    cw, r, sw = np.zeros((n, 3), np.uint32), np.zeros(n, np.uint32), np.zeros((n, w), np.uint32)
+    lib().orc_quantize(_p(c), _p(s), n, int(degree), _p(cw), _p(r), _p(sw))
+    return cw, r, sw
+
+
+def export_raw(pos, cov, opa, sh):
+    """to_raw_vertices: [n, 62] float records"""
+    p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    n = p.shape[0]
+    c = np.ascontiguousarray(cov, dtype=np.float32).reshape(n, 6)
+    o = np.ascontiguousarray(opa, dtype=np.float32).reshape(n)
+    s = np.ascontiguousarray(sh, dtype=np.float32).reshape(n, 48)
+    raw = np.zeros((n, 62), np.float32)
+    lib().orc_export_raw(_p(p), _p(c), _p(o), _p(s), n, _p(raw))
+    return raw
+
+
+def f32_to_f16_bits(x):
+    a = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    out = np.zeros(a.shape[0], np.uint16)
+    lib().orc_f32_to_f16(_p(a), a.shape[0], _p(out))
+    return out
+
+
+def atan2f(y, x):
+    return float(lib().orc_atan2f(float(y), float(x)))
+
+
+def quat_from_m3(m):
+    """Quat::from_mat3 of a [3,3] matrix: (x, y, z, w)"""
+    q = np.zeros(4, np.float32)
+    lib().orc_quat_from_m3(_p(_m3_in(m)), _p(q))
+    return q
+
+
+def encode_octahedral(q_xyzw):
+    return int(lib().orc_encode_octahedral(_p(np.ascontiguousarray(q_xyzw, dtype=np.float32))))
+
+
+def decompose_covariance(cov):
+    s, q = np.zeros(3, np.float32), np.zeros(4, np.float32)
+    lib().orc_decompose_covariance(_p(np.ascontiguousarray(cov, dtype=np.float32)), _p(s), _p(q))
+    return s, q
