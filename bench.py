@@ -428,6 +428,88 @@ def mvc_scene(n_points: int, reps: int, cpu_points: int):
     return res
 
 
+def ingest(scene, reps: int = 5, cpu_records: int = 65536):
+    """Scene ingest of the headline scene: its canonical binary PLY (248-byte
+    RawPlyVertex records, written by cgs_export_raw) loaded by cgs_ply_load
+    from host memory (PCIe included), then activated again from records
+    already in HBM (cgs_activate_raw), quantised at SH degree 3
+    (cgs_quantize_scene) and exported (cgs_export_raw).  Kernel times are HIP
+    events on the context stream; algorithmic bytes per splat: activation
+    248 read + 232 written, quantize 216 + 112, export 232 + 248.  The oracle's
+    decode + activation runs on the first cpu_records records (one core)."""
+    import ctypes as C
+
+    from cindergs_amd import GpuPipelineState
+    from cindergs_amd._lib import check
+    from cindergs_amd.ply import PlyLoader, parse_header
+
+    st = GpuPipelineState(0)
+    n = scene.count
+    data = PlyLoader.write_binary_ply_bytes(scene, st)
+    layout = parse_header(data)
+    payload = np.frombuffer(data, dtype=np.uint8, offset=int(layout.payload_offset))
+    d_rec = st.to_device(payload)
+    outs = [st.to_device(np.zeros(n * k, np.float32)) for k in (3, 6, 1, 48)]
+    qouts = [st.to_device(np.zeros(n * k, np.uint32)) for k in (3, 1, 24)]
+    d_raw = st.to_device(np.zeros(n * 62, np.float32))
+    lib, h = st._lib, st._h
+    src = np.frombuffer(data, dtype=np.uint8)
+    res = {"workload": f"{n} splats, canonical binary PLY ({len(data) / 1e6:.1f} MB)"}
+
+    def timed(fn, name):
+        fn()
+        st.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        st.synchronize()
+        wall = 1000 * (time.perf_counter() - t) / reps
+        kt = dict(st.kernel_times())
+        return wall, kt.get(name)
+
+    wall_load, _ = timed(lambda: check(lib.cgs_ply_load(
+        h, src.ctypes.data_as(C.c_void_p), src.nbytes, C.byref(layout), outs[0].ptr, outs[1].ptr,
+        outs[2].ptr, outs[3].ptr)), "k_activate_raw")
+    res["ply_load_ms_from_host"] = round(wall_load, 3)
+    for key, fn, kname, bpp in (
+            ("activate", lambda: check(lib.cgs_activate_raw(h, d_rec.ptr, C.byref(layout), n,
+                                                            outs[0].ptr, outs[1].ptr,
+                                                            outs[2].ptr, outs[3].ptr)),
+             "k_activate_raw", 480),
+            ("quantize", lambda: check(lib.cgs_quantize_scene(h, outs[1].ptr, outs[3].ptr, n, 3,
+                                                              qouts[0].ptr, qouts[1].ptr,
+                                                              qouts[2].ptr)),
+             "k_quantize", 328),
+            ("export", lambda: check(lib.cgs_export_raw(h, outs[0].ptr, outs[1].ptr, outs[2].ptr,
+                                                        outs[3].ptr, n, d_raw.ptr)),
+             "k_export_raw", 480)):
+        wall, kms = timed(fn, kname)
+        ent = {"ms": round(wall, 3), "kernel": kname, "kernel_ms": round(kms, 4) if kms else None}
+        if kms:
+            gbs = bpp * n / (kms * 1e-3) / 1e9
+            ent["roofline"] = {"bound": "hbm", "bytes_per_launch": bpp * n,
+                               "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        res[key] = ent
+    k = min(cpu_records, n)
+    if k:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # the checker, timed here as the CPU baseline of this leg
+
+        so = np.array(layout.slot_offset, np.uint32)
+        ty = np.array(layout.slot_type, np.uint8)
+        oracle.lib()
+        t = time.perf_counter()
+        oracle.decode_activate(payload[:k * 248].tobytes(), k, 248, True, so, ty)
+        cpu_s = time.perf_counter() - t
+        res["cpu_oracle"] = {"splats_per_s": round(k / cpu_s), "cores": 1, "kind": "port",
+                             "sample": f"first {k} records of the same PLY"}
+    for b in [d_rec, d_raw] + outs + qouts:
+        b.free()
+    st.close()
+    return res
+
+
 def _winding(ex, st, p):
     d = st.to_device(p)
     w = ex.winding(d, p.shape[0])
@@ -565,13 +647,14 @@ def main() -> int:
     ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER",
                     help="render the headline with these launch shapes (no autotune)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the 8K, f16, cage-deformation and MVC scene measurements")
+                    help="skip the 8K, f16, cage-deformation, MVC scene and ingest measurements")
     ap.add_argument("--cage-frames", type=int, default=300)
     ap.add_argument("--mvc", type=int, default=0, metavar="POINTS",
                     help="also time exact vs surrogate MVC binding of POINTS points")
     ap.add_argument("--mvc-scene", type=int, default=1_073_930, metavar="SPLATS",
                     help="MVC scene deformation of SPLATS bound splats (0 = skip; part of the extras)")
     ap.add_argument("--mvc-scene-only", action="store_true")
+    ap.add_argument("--ingest-only", action="store_true")
     ap.add_argument("--mvc-steps", type=int, default=20000)
     ap.add_argument("--mvc-train", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -609,6 +692,9 @@ def main() -> int:
         return 0
     if args.mvc_scene_only:
         print(json.dumps({"mvc_scene": mvc_scene(args.mvc_scene, 5, 16384)}), flush=True)
+        return 0
+    if args.ingest_only:
+        print(json.dumps({"ingest": ingest(syn.scene_clusters(args.splats))}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -739,6 +825,9 @@ def main() -> int:
         if args.mvc_scene > 0:
             result["mvc_scene"] = mvc_scene(args.mvc_scene, 5, 16384)
             print(f"# mvc scene: {result['mvc_scene']['ms']} ms", file=sys.stderr, flush=True)
+        result["ingest"] = ingest(scene)
+        print(f"# ingest: activate {result['ingest']['activate']['kernel_ms']} ms", file=sys.stderr,
+              flush=True)
     if args.batch_views > 0:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes, fixed=fixed)

@@ -206,9 +206,11 @@ CGS_HD uint32_t cgs_pack_f16_pair(float a, float b) {
   return (uint32_t)cgs_f32_to_f16(a) | ((uint32_t)cgs_f32_to_f16(b) << 16);
 }
 
-// one splat: cov_f16 [3], the octahedral rotation, sh_f16 [cgs_sh_words]
-CGS_HD void cgs_quantize_record(const float* cov, const float* sh, uint32_t degree,
-                                uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16) {
+// one splat at SH degree DEG: cov_f16 [3], the octahedral rotation,
+// sh_f16 [cgs_sh_words(DEG)] (indexes resolved at compile time)
+template <uint32_t DEG>
+CGS_HD void cgs_quantize_record_t(const float* cov, const float* sh, uint32_t* cov_f16,
+                                  uint32_t* rot, uint32_t* sh_f16) {
   cov_f16[0] = cgs_pack_f16_pair(cov[0], cov[1]);
   cov_f16[1] = cgs_pack_f16_pair(cov[2], cov[3]);
   cov_f16[2] = cgs_pack_f16_pair(cov[4], cov[5]);
@@ -216,21 +218,35 @@ CGS_HD void cgs_quantize_record(const float* cov, const float* sh, uint32_t degr
   cgs_decompose_covariance(cov, scale, q);
   *rot = cgs_encode_octahedral(q);
   // DC, then the first (bases - 1) rest coefficients of each channel, zero padded
-  const uint32_t rest = cgs_sh_bases(degree) - 1u, words = cgs_sh_words(degree);
+  constexpr uint32_t bases = DEG == 0u ? 1u : DEG == 1u ? 4u : DEG == 2u ? 9u : 16u;
+  constexpr uint32_t rest = bases - 1u, words = (bases * 3u + 1u) / 2u;
+#pragma unroll
   for (uint32_t w = 0; w < words; ++w) {
     float pair[2];
+#pragma unroll
     for (uint32_t h = 0; h < 2u; ++h) {
       const uint32_t i = 2u * w + h;
       float v = 0.0f;
       if (i < 3u) {
         v = sh[i];
-      } else if (i < 3u + 3u * rest) {
-        const uint32_t ch = (i - 3u) / rest, k = (i - 3u) % rest;
+      } else if (rest > 0u && i < 3u + 3u * rest) {
+        const uint32_t ch = (i - 3u) / (rest > 0u ? rest : 1u);
+        const uint32_t k = (i - 3u) % (rest > 0u ? rest : 1u);
         v = sh[3u + ch * 15u + k];
       }
       pair[h] = v;
     }
     sh_f16[w] = cgs_pack_f16_pair(pair[0], pair[1]);
+  }
+}
+
+CGS_HD void cgs_quantize_record(const float* cov, const float* sh, uint32_t degree,
+                                uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16) {
+  switch (degree) {
+    case 0u: cgs_quantize_record_t<0u>(cov, sh, cov_f16, rot, sh_f16); break;
+    case 1u: cgs_quantize_record_t<1u>(cov, sh, cov_f16, rot, sh_f16); break;
+    case 2u: cgs_quantize_record_t<2u>(cov, sh, cov_f16, rot, sh_f16); break;
+    default: cgs_quantize_record_t<3u>(cov, sh, cov_f16, rot, sh_f16); break;
   }
 }
 

@@ -479,7 +479,7 @@ struct CgsActivateArgs {
   uint64_t n;
   uint32_t stride;           // bytes per record
   uint32_t le;               // little-endian scalars
-  uint32_t per_block;        // records per block (<= 256, per_block * stride <= 65536)
+  uint32_t per_block;        // records per block (<= 128, per_block * stride <= 65528)
   uint32_t word_path;        // every present slot is a 4-byte aligned LE float32 and stride % 4 == 0
   uint32_t slot_offset[62];
   uint8_t slot_type[62];
@@ -501,6 +501,10 @@ hipStream_t cgs_context_stream(struct cgs_context* c);
 int cgs_context_enter(struct cgs_context* c);
 // a device buffer of at least `bytes`, owned by the context (grown 1.5x, kept)
 int cgs_context_staging(struct cgs_context* c, uint64_t bytes, void** p);
+// per-launch HIP events on the context stream, read by cgs_kernel_times
+void cgs_context_prof_begin(struct cgs_context* c);
+void cgs_context_prof_mark(struct cgs_context* c, const char* name);
+void cgs_context_prof_end(struct cgs_context* c, bool ok);
 
 struct CgsTetArgs {
   const float* verts;     // [3*V] interleaved

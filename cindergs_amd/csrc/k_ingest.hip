@@ -2,26 +2,60 @@
 //
 //  * k_activate_raw: PlyLoader::load_from_bytes + GaussianSceneSoa::
 //    from_raw_vertices_par (src/scene/ply_loader.rs:622-654, soa.rs:137-189)
-//    for any binary vertex layout.  A block stages the bytes of up to 256
-//    consecutive records in LDS with coalesced 4-byte loads (the payload is
-//    read from HBM once, whatever its layout), then each thread decodes its
-//    record through the slot table (offset + scalar type of each of the 62
-//    RawPlyVertex floats) and activates it (cgs_activate_record).  When every
-//    present slot is an aligned little-endian float32 (the canonical 248-byte
-//    3DGS record, or a reordered all-float one) the decode is one LDS word
-//    read per slot; otherwise it assembles the bytes.
+//    for any binary vertex layout.  A 128-thread block stages the bytes of up
+//    to 128 consecutive records in LDS with coalesced 16-byte loads (the
+//    payload is read from HBM once, whatever its layout), each thread decodes
+//    its record through the slot table (offset + scalar type of each of the
+//    62 RawPlyVertex floats) and activates it (cgs_activate_record), and the
+//    block's outputs go back through the same LDS so that every global store
+//    is a coalesced 16-byte vector.  When every present slot is an aligned
+//    little-endian float32 (the canonical 248-byte 3DGS record, or a
+//    reordered all-float one) the decode is one LDS word read per slot;
+//    otherwise it assembles the bytes.
 //  * k_quantize: QuantizedGaussianSceneSoa::from_soa_with_degree (soa.rs:
 //    454-500), one thread per splat: f16 pairs, octahedral rotation,
 //    degree-trimmed f16 SH.
 //  * k_export_raw: GaussianSceneSoa::to_raw_vertices (soa.rs:306-328), one
-//    thread per splat, 248-byte records written as 31 float2 stores.
+//    thread per splat, 248-byte records.
+//  Both are bound by the per-splat eigen solve in double, not by HBM.
 #include "cgs_internal.h"
 #include "cgs_ingest_math.h"
 
 namespace {
 
+constexpr uint32_t AT = 128;  // threads (and at most records) per activation block
+constexpr uint32_t OUT_WORDS = 60;  // pos 3 + cov 7 + opacity 1 + SH 49 per record in LDS
+
+// copies cnt records of W words, held in LDS at an odd stride S (no bank
+// conflicts when thread t writes record t), to W*cnt consecutive words in
+// global memory: 16-byte stores when W % 4 == 0, 8-byte when W % 2 == 0 (and
+// dst is aligned for them), else words
+template <uint32_t W, uint32_t S, typename T>
+__device__ __forceinline__ void lds_to_global(const uint32_t* lds, T* dst, uint32_t cnt,
+                                              uint32_t t) {
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  const uintptr_t a = reinterpret_cast<uintptr_t>(d);
+  if (W % 4u == 0u && (a & 15u) == 0u) {
+    const uint32_t q = cnt * (W / 4u);
+    for (uint32_t k = t; k < q; k += AT) {
+      const uint32_t r = k / (W / 4u), c = 4u * (k % (W / 4u));
+      const uint32_t* p = lds + r * S + c;
+      reinterpret_cast<uint4*>(d)[k] = make_uint4(p[0], p[1], p[2], p[3]);
+    }
+  } else if (W % 2u == 0u && (a & 7u) == 0u) {
+    const uint32_t q = cnt * (W / 2u);
+    for (uint32_t k = t; k < q; k += AT) {
+      const uint32_t r = k / (W / 2u), c = 2u * (k % (W / 2u));
+      const uint32_t* p = lds + r * S + c;
+      reinterpret_cast<uint2*>(d)[k] = make_uint2(p[0], p[1]);
+    }
+  } else {
+    for (uint32_t k = t; k < W * cnt; k += AT) d[k] = lds[(k / W) * S + (k % W)];
+  }
+}
+
 template <bool WORDS>
-__global__ __launch_bounds__(256) void k_activate_raw(CgsActivateArgs A) {
+__global__ __launch_bounds__(AT) void k_activate_raw(CgsActivateArgs A) {
   extern __shared__ uint32_t s_words[];
   const uint64_t r0 = (uint64_t)blockIdx.x * A.per_block;
   if (r0 >= A.n) return;
@@ -29,63 +63,118 @@ __global__ __launch_bounds__(256) void k_activate_raw(CgsActivateArgs A) {
   const uint64_t total = A.n * A.stride;
   const uint64_t b0 = r0 * A.stride, b1 = b0 + (uint64_t)cnt * A.stride;
   const uint64_t w0 = b0 / 4u, w1 = (b1 + 3u) / 4u;
+  const uint32_t t = threadIdx.x;
   const uint32_t* src = reinterpret_cast<const uint32_t*>(A.records);
-  for (uint64_t w = w0 + threadIdx.x; w < w1; w += 256u) {
-    uint32_t v;
-    if (4u * w + 4u <= total) {
-      v = __builtin_nontemporal_load(src + w);
-    } else {  // the payload's last partial word: bytes inside it only
-      v = 0u;
-      for (uint32_t k = 0; k < 4u; ++k)
-        if (4u * w + k < total) v |= (uint32_t)A.records[4u * w + k] << (8u * k);
+  // stage the records: 16-byte loads over the aligned middle, words at the ends
+  {
+    const uint64_t q0 = (w0 + 3u) / 4u, q1 = w1 / 4u;  // whole uint4 of the range
+    const bool vec = q1 > q0 && 16u * q1 <= total &&
+                     (reinterpret_cast<uintptr_t>(A.records) & 15u) == 0u;
+    const uint64_t head_end = vec ? 4u * q0 : w1, tail_start = vec ? 4u * q1 : w1;
+    for (uint64_t w = w0 + t; w < head_end; w += AT) {
+      uint32_t v;
+      if (4u * w + 4u <= total) {
+        v = __builtin_nontemporal_load(src + w);
+      } else {
+        v = 0u;
+        for (uint32_t k = 0; k < 4u; ++k)
+          if (4u * w + k < total) v |= (uint32_t)A.records[4u * w + k] << (8u * k);
+      }
+      s_words[w - w0] = v;
     }
-    s_words[w - w0] = v;
+    if (vec) {
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4* src4 = reinterpret_cast<const u32x4*>(A.records);
+      for (uint64_t q = q0 + t; q < q1; q += AT) {
+        const u32x4 v = __builtin_nontemporal_load(src4 + q);
+        const uint64_t o = 4u * q - w0;
+        s_words[o] = v.x; s_words[o + 1u] = v.y; s_words[o + 2u] = v.z; s_words[o + 3u] = v.w;
+      }
+      for (uint64_t w = tail_start + t; w < w1; w += AT) {
+        uint32_t v;
+        if (4u * w + 4u <= total) {
+          v = __builtin_nontemporal_load(src + w);
+        } else {
+          v = 0u;
+          for (uint32_t k = 0; k < 4u; ++k)
+            if (4u * w + k < total) v |= (uint32_t)A.records[4u * w + k] << (8u * k);
+        }
+        s_words[w - w0] = v;
+      }
+    }
   }
   __syncthreads();
-  const uint32_t t = threadIdx.x;
-  if (t >= cnt) return;
-  const uint32_t rec_byte = (uint32_t)(b0 - 4u * w0) + t * A.stride;
-  float raw[CGS_RAW_FLOATS];
-  if (WORDS) {
-    const uint32_t rw = rec_byte / 4u;
+  float out[58];
+  if (t < cnt) {
+    const uint32_t rec_byte = (uint32_t)(b0 - 4u * w0) + t * A.stride;
+    float raw[CGS_RAW_FLOATS];
+    if (WORDS) {
+      const uint32_t rw = rec_byte / 4u;
 #pragma unroll
-    for (uint32_t k = 0; k < CGS_RAW_FLOATS; ++k)
-      raw[k] = A.slot_type[k] ? cgs_u2f(s_words[rw + A.slot_offset[k] / 4u]) : 0.0f;
-  } else {
-    const uint8_t* rec = reinterpret_cast<const uint8_t*>(s_words) + rec_byte;
-    for (uint32_t k = 0; k < CGS_RAW_FLOATS; ++k)
-      raw[k] = A.slot_type[k] ? cgs_pt_read(rec + A.slot_offset[k], A.slot_type[k], A.le != 0u)
-                              : 0.0f;
+      for (uint32_t k = 0; k < CGS_RAW_FLOATS; ++k)
+        raw[k] = A.slot_type[k] ? cgs_u2f(s_words[rw + A.slot_offset[k] / 4u]) : 0.0f;
+    } else {
+      const uint8_t* rec = reinterpret_cast<const uint8_t*>(s_words) + rec_byte;
+      for (uint32_t k = 0; k < CGS_RAW_FLOATS; ++k)
+        raw[k] = A.slot_type[k] ? cgs_pt_read(rec + A.slot_offset[k], A.slot_type[k],
+                                              A.le != 0u)
+                                : 0.0f;
+    }
+    cgs_activate_record(raw, out, out + 3, out + 9, out + 10);
   }
-  float pos[3], cov[6], opa, sh[48];
-  cgs_activate_record(raw, pos, cov, &opa, sh);
-  const uint64_t i = r0 + t;
-  for (int k = 0; k < 3; ++k) A.pos[i * 3 + k] = pos[k];
-  float2* c2 = reinterpret_cast<float2*>(A.cov + i * 6);
-  for (int k = 0; k < 3; ++k) c2[k] = make_float2(cov[2 * k], cov[2 * k + 1]);
-  A.opa[i] = opa;
-  float4* s4 = reinterpret_cast<float4*>(A.sh + i * 48);
-  for (int k = 0; k < 12; ++k)
-    s4[k] = make_float4(sh[4 * k], sh[4 * k + 1], sh[4 * k + 2], sh[4 * k + 3]);
+  __syncthreads();  // the records are consumed: the LDS now holds the outputs
+  // planes of the block at odd strides: pos [cnt*3] | cov [cnt*7] | opacity
+  // [cnt] | SH [cnt*49]
+  uint32_t* o_pos = s_words;
+  uint32_t* o_cov = o_pos + 3u * cnt;
+  uint32_t* o_opa = o_cov + 7u * cnt;
+  uint32_t* o_sh = o_opa + cnt;
+  if (t < cnt) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o_pos[3u * t + k] = cgs_f2u(out[k]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o_cov[7u * t + k] = cgs_f2u(out[3 + k]);
+    o_opa[t] = cgs_f2u(out[9]);
+#pragma unroll
+    for (int k = 0; k < 48; ++k) o_sh[49u * t + k] = cgs_f2u(out[10 + k]);
+  }
+  __syncthreads();
+  lds_to_global<3u, 3u>(o_pos, A.pos + r0 * 3u, cnt, t);
+  lds_to_global<6u, 7u>(o_cov, A.cov + r0 * 6u, cnt, t);
+  lds_to_global<1u, 1u>(o_opa, A.opa + r0, cnt, t);
+  lds_to_global<48u, 49u>(o_sh, A.sh + r0 * 48u, cnt, t);
 }
 
+// quantize and export: one thread per splat, direct loads and stores.  Their
+// cost is the per-splat eigen solve in double (decompose_covariance_3d), so
+// occupancy matters more than coalescing: staging them through LDS like the
+// activation measured 2.2x (quantize) and 1.4x (export) slower.
+template <uint32_t DEG>
 __global__ __launch_bounds__(256) void k_quantize(const float* cov, const float* sh, uint64_t n,
-                                                  uint32_t degree, uint32_t* cov_f16,
-                                                  uint32_t* rot, uint32_t* sh_f16) {
+                                                  uint32_t* cov_f16, uint32_t* rot,
+                                                  uint32_t* sh_f16) {
+  constexpr uint32_t words = DEG == 0u ? 2u : DEG == 1u ? 6u : DEG == 2u ? 14u : 24u;
   const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
   float c[6], s[48];
-  for (int k = 0; k < 6; ++k) c[k] = cov[i * 6 + k];
+  const float2* c2 = reinterpret_cast<const float2*>(cov + i * 6);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float2 v = c2[k];
+    c[2 * k] = v.x; c[2 * k + 1] = v.y;
+  }
   const float4* s4 = reinterpret_cast<const float4*>(sh + i * 48);
+#pragma unroll
   for (int k = 0; k < 12; ++k) {
     const float4 v = s4[k];
     s[4 * k] = v.x; s[4 * k + 1] = v.y; s[4 * k + 2] = v.z; s[4 * k + 3] = v.w;
   }
-  uint32_t cw[3], r, sw[24];
-  cgs_quantize_record(c, s, degree, cw, &r, sw);
+  uint32_t cw[3], r, sw[words];
+  cgs_quantize_record_t<DEG>(c, s, cw, &r, sw);
+#pragma unroll
   for (int k = 0; k < 3; ++k) cov_f16[i * 3 + k] = cw[k];
   rot[i] = r;
-  const uint32_t words = cgs_sh_words(degree);
+#pragma unroll
   for (uint32_t k = 0; k < words; ++k) sh_f16[i * words + k] = sw[k];
 }
 
@@ -112,18 +201,29 @@ __global__ __launch_bounds__(256) void k_export_raw(const float* pos, const floa
 void cgs_launch_activate_raw(const CgsActivateArgs& a, hipStream_t s) {
   if (a.n == 0) return;
   const uint64_t blocks = (a.n + a.per_block - 1u) / a.per_block;
-  const size_t lds = (((size_t)a.per_block * a.stride + 3u) / 4u + 1u) * 4u;
+  // LDS: the staged records (plus the partial words at both ends), reused for
+  // the outputs of the block
+  const size_t in_words = ((size_t)a.per_block * a.stride + 3u) / 4u + 1u;
+  const size_t out_words = (size_t)a.per_block * OUT_WORDS;
+  const size_t lds = (in_words > out_words ? in_words : out_words) * 4u;
   if (a.word_path)
-    hipLaunchKernelGGL(k_activate_raw<true>, dim3((uint32_t)blocks), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_activate_raw<true>, dim3((uint32_t)blocks), dim3(AT), lds, s, a);
   else
-    hipLaunchKernelGGL(k_activate_raw<false>, dim3((uint32_t)blocks), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_activate_raw<false>, dim3((uint32_t)blocks), dim3(AT), lds, s, a);
 }
 
 void cgs_launch_quantize(const float* cov, const float* sh, uint64_t n, uint32_t degree,
                          uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_quantize, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, s, cov, sh,
-                     n, degree, cov_f16, rot, sh_f16);
+  const dim3 grid((uint32_t)((n + 255u) / 256u));
+  if (degree == 0u)
+    hipLaunchKernelGGL(k_quantize<0u>, grid, dim3(256), 0, s, cov, sh, n, cov_f16, rot, sh_f16);
+  else if (degree == 1u)
+    hipLaunchKernelGGL(k_quantize<1u>, grid, dim3(256), 0, s, cov, sh, n, cov_f16, rot, sh_f16);
+  else if (degree == 2u)
+    hipLaunchKernelGGL(k_quantize<2u>, grid, dim3(256), 0, s, cov, sh, n, cov_f16, rot, sh_f16);
+  else
+    hipLaunchKernelGGL(k_quantize<3u>, grid, dim3(256), 0, s, cov, sh, n, cov_f16, rot, sh_f16);
 }
 
 void cgs_launch_export_raw(const float* pos, const float* cov, const float* opa,

@@ -341,7 +341,7 @@ int activate_args(const cgs_ply_layout* L, CgsActivateArgs* A) {
   }
   A->stride = L->vertex_stride;
   A->le = L->format == CGS_PLY_BINARY_LE ? 1u : 0u;
-  A->per_block = (uint32_t)std::min<uint64_t>(256u, (65536u - 8u) / L->vertex_stride);
+  A->per_block = (uint32_t)std::min<uint64_t>(128u, (65536u - 8u) / L->vertex_stride);
   A->word_path = words ? 1u : 0u;
   std::memcpy(A->slot_offset, L->slot_offset, sizeof(A->slot_offset));
   std::memcpy(A->slot_type, L->slot_type, sizeof(A->slot_type));
@@ -391,8 +391,12 @@ int cgs_activate_raw(cgs_context* c, const void* d_records, const cgs_ply_layout
   A.sh = d_sh;
   if ((n + A.per_block - 1u) / A.per_block > 0x7FFFFFFFull)
     return cgs_fail(CGS_ERR_INVALID_ARG, "too many records");
+  cgs_context_prof_begin(c);
+  cgs_context_prof_mark(c, "k_activate_raw");
   cgs_launch_activate_raw(A, cgs_context_stream(c));
-  return finish(c, "activate raw");
+  const int r = finish(c, "activate raw");
+  cgs_context_prof_end(c, r == CGS_OK);
+  return r;
 }
 
 int cgs_ply_load(cgs_context* c, const uint8_t* bytes, uint64_t len, const cgs_ply_layout* layout,
@@ -434,11 +438,16 @@ int cgs_quantize_scene(cgs_context* c, const float* d_cov, const float* d_sh, ui
   if (n == 0) return CGS_OK;
   if (!d_cov || !d_sh || !d_cov_f16 || !d_rot_oct || !d_sh_f16)
     return cgs_fail(CGS_ERR_INVALID_ARG, "null buffer");
-  if (misaligned(d_sh, 16)) return cgs_fail(CGS_ERR_INVALID_ARG, "d_sh must be 16-byte aligned");
+  if (misaligned(d_sh, 16) || misaligned(d_cov, 8))
+    return cgs_fail(CGS_ERR_INVALID_ARG, "d_sh must be 16-byte and d_cov 8-byte aligned");
   if (n >= (1ull << 31) * 256ull) return cgs_fail(CGS_ERR_INVALID_ARG, "n too large");
+  cgs_context_prof_begin(c);
+  cgs_context_prof_mark(c, "k_quantize");
   cgs_launch_quantize(d_cov, d_sh, n, sh_degree > 3u ? 3u : sh_degree, d_cov_f16, d_rot_oct,
                       d_sh_f16, cgs_context_stream(c));
-  return finish(c, "quantize");
+  const int r = finish(c, "quantize");
+  cgs_context_prof_end(c, r == CGS_OK);
+  return r;
 }
 
 int cgs_export_raw(cgs_context* c, const float* d_pos, const float* d_cov, const float* d_opa,
@@ -451,9 +460,13 @@ int cgs_export_raw(cgs_context* c, const float* d_pos, const float* d_cov, const
   if (misaligned(d_sh, 16) || misaligned(d_records, 8))
     return cgs_fail(CGS_ERR_INVALID_ARG, "d_sh must be 16-byte and d_records 8-byte aligned");
   if (n >= (1ull << 31) * 256ull) return cgs_fail(CGS_ERR_INVALID_ARG, "n too large");
+  cgs_context_prof_begin(c);
+  cgs_context_prof_mark(c, "k_export_raw");
   cgs_launch_export_raw(d_pos, d_cov, d_opa, d_sh, n, static_cast<float*>(d_records),
                         cgs_context_stream(c));
-  return finish(c, "export raw");
+  const int r = finish(c, "export raw");
+  cgs_context_prof_end(c, r == CGS_OK);
+  return r;
 }
 
 }  // extern "C"

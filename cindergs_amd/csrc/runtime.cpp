@@ -1973,6 +1973,25 @@ int cgs_mvc_deform_scene(cgs_context* c, const float* d_weights, const float* d_
 
 int cgs_fail(int code, const char* msg) { return fail(code, msg ? msg : "error"); }
 int cgs_context_enter(cgs_context* c) { return ensure_ctx(c); }
+void cgs_context_prof_begin(cgs_context* c) {
+  c->prof.on = true;
+  c->prof.n = 0;
+  c->prof_count = 0;
+}
+void cgs_context_prof_mark(cgs_context* c, const char* name) { c->prof.mark(name, c->stream); }
+void cgs_context_prof_end(cgs_context* c, bool ok) {
+  if (ok && c->prof.on) {
+    c->prof.mark("end", c->stream);
+    if (hipStreamSynchronize(c->stream) == hipSuccess) {
+      for (int i = 0; i + 1 < c->prof.n; ++i) {
+        c->prof_names[i] = c->prof.names[i];
+        c->prof_ms[i] = elapsed(c->prof.ev[i], c->prof.ev[i + 1]);
+      }
+      c->prof_count = c->prof.n > 0 ? c->prof.n - 1 : 0;
+    }
+  }
+  c->prof.on = false;
+}
 int cgs_context_staging(cgs_context* c, uint64_t bytes, void** p) {
   TRY(ensure_ctx(c));
   TRY(ensure_words(c->ingest, (size_t)((bytes + 3u) / 4u)));
