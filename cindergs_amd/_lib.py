@@ -196,6 +196,7 @@ SIGNATURES = {
     "cgs_set_capture": [_vp, _u32],
     "cgs_read_deformed": [_vp, _vp, _vp, _vp, _vp, C.POINTER(_u32)],
     "cgs_read_tet_transforms": [_vp, _vp, _vp, _vp],
+    "cgs_export_deformed_scene": [_vp, _u32, _vp, _vp, _vp, _vp, _u64, C.POINTER(_u64)],
     "cgs_bind_tets": [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],

@@ -143,9 +143,11 @@ class TetMesh:
         """bind_point for every position: first tet in index order with penalty
         <= 1e-4, else minimal penalty.  On a grid cage only the tetrahedra of
         the 27 cells around the point's cell can contain it (jitter < half a
-        cell), so those are the candidates searched, in index order."""
-        assert self.cells is not None, "cell search needs a grid cage"
+        cell), so those are the candidates searched, in index order; any other
+        mesh is scanned whole."""
         pos = np.asarray(positions, dtype=np.float32)
+        if self.cells is None:
+            return self._bind_scan(pos)
         n = pos.shape[0]
         nx, ny, nz = self.cells
         cell = np.floor((pos - self.lo) / ((self.hi - self.lo) / np.array(self.cells))).astype(np.int64)
@@ -174,6 +176,29 @@ class TetMesh:
         tet = np.where(found, first_t, best_t).astype(np.uint32)
         w = np.where(found[:, None], first_w, best_w).astype(np.float32)
         return tet, np.ascontiguousarray(w)
+
+    def _bind_scan(self, pos: np.ndarray):
+        """bind_point over every tet in index order (mod.rs:267-305)."""
+        n = pos.shape[0]
+        best_t = np.zeros(n, np.int64)
+        best_pen = np.full(n, np.inf, np.float32)
+        best_w = np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1))
+        found = np.zeros(n, bool)
+        first_t = np.zeros(n, np.int64)
+        first_w = np.zeros((n, 4), np.float32)
+        for t in range(self.num_tets):
+            w = self.barycentric(pos, np.full(n, t, np.int64))
+            pen = np.maximum(-w, 0).sum(1).astype(np.float32)
+            hit = ~found & (pen <= 1e-4)
+            first_t[hit] = t
+            first_w[hit] = w[hit]
+            found |= hit
+            better = pen < best_pen
+            best_pen = np.where(better, pen, best_pen)
+            best_t = np.where(better, t, best_t)
+            best_w[better] = w[better]
+        tet = np.where(found, first_t, best_t).astype(np.uint32)
+        return tet, np.ascontiguousarray(np.where(found[:, None], first_w, best_w), np.float32)
 
     def cage(self, vertices: np.ndarray, bind_tet: np.ndarray, bind_w: np.ndarray) -> TetCage:
         return TetCage(self.elements, self.inv_dm, np.ascontiguousarray(vertices, dtype=np.float32),

@@ -493,6 +493,23 @@ void cgs_launch_quantize(const float* cov, const float* sh, uint64_t n, uint32_t
                          uint32_t* cov_f16, uint32_t* rot, uint32_t* sh_f16, hipStream_t s);
 void cgs_launch_export_raw(const float* pos, const float* cov, const float* opa,
                            const float* sh, uint64_t n, float* records, hipStream_t s);
+// the deformed scene of the last Stage 1 run as a SoA, SH rotated per tet
+#define CGS_SH_MATS 83u  // 9 + 25 + 49 floats: the band 1-3 matrices of one rotation
+struct CgsExportDeformedArgs {
+  const float* geo;          // Stage 1 output, blocked K=10
+  const float* sh;           // rest SH, blocked K=48 f32
+  const uint32_t* bind_tet;  // [num_rest]
+  const uint32_t* split_meta;  // [2 * splits] (parent, tet)
+  const float* mats;         // [num_tets * CGS_SH_MATS]
+  uint32_t num_rest, count, num_tets, degree;
+  float* pos;                // [3 count]
+  float* cov;                // [6 count]
+  float* opa;                // [count]
+  float* out_sh;             // [48 count], 16-byte aligned
+};
+void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, const double* tables,
+                                float* mats, hipStream_t s);
+void cgs_launch_export_deformed(const CgsExportDeformedArgs& a, hipStream_t s);
 
 // error reporting and context access shared by the host files
 int cgs_fail(int code, const char* msg);

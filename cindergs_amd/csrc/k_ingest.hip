@@ -196,7 +196,66 @@ __global__ __launch_bounds__(256) void k_export_raw(const float* pos, const floa
   for (int k = 0; k < 31; ++k) out[k] = make_float2(raw[2 * k], raw[2 * k + 1]);
 }
 
+// ShRotator::new for every tet (sh_rotation.rs:404-425): the f32 rotation
+// matrices of bands 1-3 of the tet's polar rotation, 9 + 25 + 49 floats per
+// tet, built once and shared by every splat bound to it
+__global__ __launch_bounds__(64) void k_tet_sh_matrices(const float* rot, uint32_t num_tets,
+                                                        const double* tables, float* mats) {
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= num_tets) return;
+  float R[9];
+  for (int k = 0; k < 9; ++k) R[k] = rot[(size_t)t * 9 + k];
+  float* m = mats + (size_t)t * CGS_SH_MATS;
+  cgs_sh_band_matrix(R, 1, tables, m);
+  cgs_sh_band_matrix(R, 2, tables, m + 9);
+  cgs_sh_band_matrix(R, 3, tables, m + 34);
+}
+
+// one record of the deformed scene for export: the Stage 1 geometry, and the
+// SH of its source splat (a split child's parent) rotated by ShRotator::apply
+// (sh_rotation.rs:427-446) of its tet; unbound records keep their SH
+__global__ __launch_bounds__(256) void k_export_deformed(CgsExportDeformedArgs A) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= A.count) return;
+  for (uint32_t k = 0; k < 3u; ++k) A.pos[(size_t)i * 3 + k] = A.geo[cgs_blk(i, k, CGS_GEO_K)];
+  for (uint32_t k = 0; k < 6u; ++k)
+    A.cov[(size_t)i * 6 + k] = A.geo[cgs_blk(i, 3u + k, CGS_GEO_K)];
+  A.opa[i] = A.geo[cgs_blk(i, CGS_CH_OPA, CGS_GEO_K)];
+  uint32_t src = i, tet = A.num_tets;
+  if (i < A.num_rest) {
+    tet = A.bind_tet[i];
+  } else {
+    src = A.split_meta[2u * (i - A.num_rest)];
+    tet = A.split_meta[2u * (i - A.num_rest) + 1u];
+  }
+  float sh[48];
+  for (uint32_t k = 0; k < 48u; ++k) sh[k] = A.sh[cgs_blk(src, k, CGS_SH_K)];
+  if (A.degree > 0u && tet < A.num_tets) {
+    const float* m = A.mats + (size_t)tet * CGS_SH_MATS;
+    for (uint32_t ch = 0; ch < 3u; ++ch) {
+      const uint32_t rest = 3u + ch * 15u;
+      cgs_sh_apply_block(sh, rest, 3, m);
+      if (A.degree >= 2u) cgs_sh_apply_block(sh, rest + 3u, 5, m + 9);
+      if (A.degree >= 3u) cgs_sh_apply_block(sh, rest + 8u, 7, m + 34);
+    }
+  }
+  float4* o = reinterpret_cast<float4*>(A.out_sh + (size_t)i * 48);
+  for (int k = 0; k < 12; ++k) o[k] = make_float4(sh[4 * k], sh[4 * k + 1], sh[4 * k + 2], sh[4 * k + 3]);
+}
+
 }  // namespace
+
+void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, const double* tables,
+                                float* mats, hipStream_t s) {
+  if (num_tets == 0) return;
+  hipLaunchKernelGGL(k_tet_sh_matrices, dim3((num_tets + 63u) / 64u), dim3(64), 0, s, rot,
+                     num_tets, tables, mats);
+}
+
+void cgs_launch_export_deformed(const CgsExportDeformedArgs& a, hipStream_t s) {
+  if (a.count == 0) return;
+  hipLaunchKernelGGL(k_export_deformed, dim3((a.count + 255u) / 256u), dim3(256), 0, s, a);
+}
 
 void cgs_launch_activate_raw(const CgsActivateArgs& a, hipStream_t s) {
   if (a.n == 0) return;

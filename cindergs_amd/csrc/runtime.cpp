@@ -26,6 +26,7 @@
 
 #include "../../include/cindergs.h"
 #include "cgs_internal.h"
+#include "cgs_scene_math.h"
 
 namespace {
 
@@ -194,6 +195,9 @@ struct cgs_context {
   DevBuf mvc_work;
   // scene ingest: the vertex payload of a PLY file
   DevBuf ingest;
+  // export of a deformed scene: SH basis tables (doubles) and per-tet matrices
+  DevBuf sh_tables, tet_shm;
+  bool sh_tables_ready = false;
 
   // ---- last frame ----
   uint32_t last_w = 0, last_h = 0;
@@ -743,7 +747,7 @@ void cgs_destroy(cgs_context* c) {
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
                     &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
                     &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
-                    &c->ingest};
+                    &c->ingest, &c->sh_tables, &c->tet_shm};
   for (DevBuf* b : bufs) release(*b);
   c->fa.release_all();
   for (auto& e : c->ev)
@@ -1223,6 +1227,60 @@ int cgs_read_deformed(cgs_context* c, float* pos, float* cov, float* opa, uint32
     if (opa) opa[i] = g[cgs_blk(i, CGS_CH_OPA, CGS_GEO_K)];
   }
   if (split_count) *split_count = splits;
+  return CGS_OK;
+}
+
+int cgs_export_deformed_scene(cgs_context* c, uint32_t sh_degree, float* d_pos, float* d_cov,
+                              float* d_opa, float* d_sh, uint64_t capacity, uint64_t* count) {
+  TRY(ensure_ctx(c));
+  if (!count) return fail(CGS_ERR_INVALID_ARG, "null count");
+  *count = 0;
+  if (!c->has_cages || !c->def_geo.p || !c->rot.p)
+    return fail(CGS_ERR_INVALID_ARG, "no deformation has run");
+  if (c->f16) return fail(CGS_ERR_UNSUPPORTED, "cage deformation of the f16 storage variant");
+  if (c->n_rest == 0) return CGS_OK;
+  uint32_t splits = 0;
+  HIPCHK(hipMemcpyAsync(&splits, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const uint64_t m = (uint64_t)c->n_rest + splits;
+  if (m > c->def_cap) return fail(CGS_ERR_EXECUTION, "split count exceeds the deformed arena");
+  if (!d_pos && !d_cov && !d_opa && !d_sh) {  // a query of the record count
+    *count = m;
+    return CGS_OK;
+  }
+  if (!d_pos || !d_cov || !d_opa || !d_sh) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  if (reinterpret_cast<uintptr_t>(d_sh) % 16u != 0u)
+    return fail(CGS_ERR_INVALID_ARG, "d_sh must be 16-byte aligned");
+  if (m > capacity) return fail(CGS_ERR_INVALID_ARG, "capacity below the deformed splat count");
+  const uint32_t deg = sh_degree > 3u ? 3u : sh_degree;
+  if (!c->sh_tables_ready) {
+    double h[CGS_SH_TABLE_DOUBLES];
+    cgs_sh_basis_tables(h);
+    TRY(ensure_words(c->sh_tables, CGS_SH_TABLE_DOUBLES * 2));
+    HIPCHK(hipMemcpy(c->sh_tables.p, h, sizeof(h), hipMemcpyHostToDevice));
+    c->sh_tables_ready = true;
+  }
+  TRY(ensure_words(c->tet_shm, (size_t)c->num_tets * CGS_SH_MATS));
+  cgs_launch_tet_sh_matrices(P<float>(c->rot), c->num_tets, P<double>(c->sh_tables),
+                             P<float>(c->tet_shm), c->stream);
+  CgsExportDeformedArgs A;
+  A.geo = P<float>(c->def_geo);
+  A.sh = P<float>(c->rest_sh);
+  A.bind_tet = P<uint32_t>(c->bind_tet);
+  A.split_meta = P<uint32_t>(c->split_meta);
+  A.mats = P<float>(c->tet_shm);
+  A.num_rest = c->n_rest;
+  A.count = (uint32_t)m;
+  A.num_tets = c->num_tets;
+  A.degree = deg;
+  A.pos = d_pos;
+  A.cov = d_cov;
+  A.opa = d_opa;
+  A.out_sh = d_sh;
+  cgs_launch_export_deformed(A, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *count = m;
   return CGS_OK;
 }
 

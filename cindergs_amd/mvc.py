@@ -18,6 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from .cages import TetMesh
 from .pipeline import DeviceBuffer, GpuPipelineState, _ptr, check
 
 SEED_BASE = 0xC1DE5EED00000000
@@ -54,6 +55,30 @@ class PolyCage:
 
     def diagonal(self) -> float:
         return float(np.linalg.norm(self.vertices.max(0) - self.vertices.min(0)))
+
+    def to_tet_mesh(self):
+        """PolyCageMesh::to_tet_mesh (mvc.rs:751-779): the vertices plus their f32
+        centroid (summed in vertex order, times 1/n), and one tet per triangle
+        fanned to the centroid, (i0, i1, i2, c) when det[v1-v0, v2-v0, c-v0] >= 0,
+        else (i0, i2, i1, c).  The determinant is glam's z . (x cross y) in f32."""
+        f = np.float32
+        V = self.vertices
+        acc = np.zeros(3, f)
+        for v in V:
+            acc = (acc + v).astype(f)
+        centroid = (acc * (f(1.0) / f(V.shape[0]))).astype(f)
+        v0, v1, v2 = V[self.triangles[:, 0]], V[self.triangles[:, 1]], V[self.triangles[:, 2]]
+        x, y, z = (v1 - v0).astype(f), (v2 - v0).astype(f), (centroid - v0).astype(f)
+        cx = x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
+        cy = x[:, 2] * y[:, 0] - x[:, 0] * y[:, 2]
+        cz = x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0]
+        det = (z[:, 0] * cx + z[:, 1] * cy) + z[:, 2] * cz
+        c = np.full(self.triangles.shape[0], V.shape[0], np.uint32)
+        t = self.triangles
+        keep = det >= 0
+        elems = np.stack([t[:, 0], np.where(keep, t[:, 1], t[:, 2]),
+                          np.where(keep, t[:, 2], t[:, 1]), c], 1).astype(np.uint32)
+        return TetMesh(np.concatenate([V, centroid[None]]).astype(f), elems)
 
 
 def icosahedron(radius: float = 1.0) -> PolyCage:

@@ -262,6 +262,29 @@ class GpuPipelineState:
                                           C.byref(cnt)))
         return pos, cov, opa, meta[: cnt.value]
 
+    def export_deformed_scene(self, sh_degree: int = 3) -> GaussianSceneSoa:
+        """The deformed scene of the last frame with cages as deform_scene_cpu returns
+        it (cpu_reference.rs:97-137, 166-182, 394-402), built on the device: Stage 1's
+        positions, covariances and opacities, and each record's source SH rotated
+        by its tet's ShRotator (sh_rotation.rs:396-446)."""
+        cnt = C.c_uint64()
+        check(self._lib.cgs_export_deformed_scene(self._h, sh_degree, None, None, None, None,
+                                                  0, C.byref(cnt)))
+        m = cnt.value
+        shapes = ((m, 3), (m, 6), (m,), (m, 48))
+        if m == 0:
+            return GaussianSceneSoa(*[np.zeros(s, np.float32) for s in shapes])
+        bufs = [self.to_device(np.zeros(s, np.float32)) for s in shapes]
+        try:
+            check(self._lib.cgs_export_deformed_scene(self._h, sh_degree,
+                                                      *[b.ptr for b in bufs], m,
+                                                      C.byref(cnt)))
+            out = [b.numpy() for b in bufs]
+        finally:
+            for b in bufs:
+                b.free()
+        return GaussianSceneSoa(*out)
+
     def read_tet_transforms(self, num_tets: int):
         J = np.zeros((num_tets, 9), dtype=np.float32)
         R = np.zeros((num_tets, 9), dtype=np.float32)

@@ -250,6 +250,21 @@ int cgs_read_deformed(cgs_context* ctx, float* positions, float* covariances, fl
 int cgs_read_tet_transforms(cgs_context* ctx, float* jacobians, float* rotations,
                             float* max_stretch);
 
+/* The deformed scene of the last cgs_execute_frame with cages as a
+ * GaussianSceneSoa for export, as the CPU path's deform_scene_cpu writes it
+ * (src/deformation/cpu_reference.rs:97-137, 166-182): the Stage 1 positions,
+ * covariances and opacities the renderer used, and each splat's SH (a split
+ * child's: its parent's) rotated by the polar rotation of its tet with
+ * ShRotator (src/render/rasterizer/sh_rotation.rs:396-446), whose matrices
+ * are built once per tet; unbound splats keep their SH.  Bands above
+ * sh_degree are copied as they are.  Outputs in device memory with room for
+ * `capacity` records (d_sh 16-byte aligned); *count receives the number of
+ * records (rest splats + split children).  With all four outputs null the
+ * call only sets *count.  The f16 storage variant has no cage deformation:
+ * CGS_ERR_UNSUPPORTED. */
+int cgs_export_deformed_scene(cgs_context* ctx, uint32_t sh_degree, float* d_pos, float* d_cov,
+                              float* d_opa, float* d_sh, uint64_t capacity, uint64_t* count);
+
 /* TetMesh::bind_gaussians / bind_gaussians_with_bounds (src/deformation/
  * mod.rs:263-323, bind_point 267-310) on the GPU: for each of the n positions
  * (xyz), the first tetrahedron in index order whose barycentric penalty

@@ -1479,6 +1479,46 @@ ORC_API uint64_t orc_mvc_scene(const float* def_pos, const float* def_F, const u
   return total;
 }
 
+// The SH of deform_scene_cpu's output (cpu_reference.rs:141-147, 166-182,
+// 368-369): ShRotator::new once per tet (sh_rotation.rs:412-430), applied to
+// each bound record's SH (sh_rotation.rs:433-446); unbound records and tets
+// out of range keep the rest SH; split child s (split_meta[2s] = parent,
+// [2s+1] = its tet) copies the rotated SH of its parent.  rot: [T][9]
+// column-major.
+ORC_API void orc_deformed_sh(const float* rest_sh, uint32_t n, const float* rot,
+                             uint32_t num_tets, const uint32_t* bind_tet,
+                             const uint32_t* split_meta, uint32_t splits, int degree,
+                             float* out_sh) {
+  double t[CGS_SH_TABLE_DOUBLES];
+  cgs_sh_basis_tables(t);
+  if (degree > 3) degree = 3;
+  std::vector<float> mats;
+  if (degree > 0) {
+    mats.resize((size_t)num_tets * 83);
+    for (uint32_t k = 0; k < num_tets; ++k) {
+      float* m = &mats[(size_t)k * 83];
+      cgs_sh_band_matrix(rot + (size_t)k * 9, 1, t, m);
+      if (degree >= 2) cgs_sh_band_matrix(rot + (size_t)k * 9, 2, t, m + 9);
+      if (degree >= 3) cgs_sh_band_matrix(rot + (size_t)k * 9, 3, t, m + 34);
+    }
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    float* o = out_sh + (size_t)i * 48;
+    memcpy(o, rest_sh + (size_t)i * 48, 192);
+    const uint32_t tet = bind_tet[i];
+    if (degree == 0 || tet >= num_tets) continue;
+    const float* m = &mats[(size_t)tet * 83];
+    for (int ch = 0; ch < 3; ++ch) {
+      const int rest = 3 + ch * 15;
+      cgs_sh_apply_block(o, rest, 3, m);
+      if (degree >= 2) cgs_sh_apply_block(o, rest + 3, 5, m + 9);
+      if (degree >= 3) cgs_sh_apply_block(o, rest + 8, 7, m + 34);
+    }
+  }
+  for (uint32_t s = 0; s < splits; ++s)
+    memcpy(out_sh + (size_t)(n + s) * 48, out_sh + (size_t)split_meta[2 * s] * 48, 192);
+}
+
 // the per-splat pieces, for their own checks
 ORC_API void orc_sh_tables(double* out) { cgs_sh_basis_tables(out); }
 ORC_API void orc_sh_band_matrix(const float* R, int l, float* M) {

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "orc_mvc_scene": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp,
                                _vp, _vp], _u64),
             "orc_sh_tables": ([_vp], None),
+            "orc_deformed_sh": ([_vp, _u32, _vp, _u32, _vp, _vp, _u32, _i, _vp], None),
             "orc_sh_band_matrix": ([_vp, _i, _vp], None),
             "orc_sh_rotate": ([_vp, _vp, _i], None),
             "orc_sh_rotate_fast": ([_vp, _vp, _i], None),
@@ -423,6 +424,21 @@ def mvc_scene(def_pos, def_F, mask, rest_pos, rest_cov, rest_opa, rest_sh, sh_de
                                 int(sh_degree), float(tau), float(mip), _p(op), _p(oc), _p(oo),
                                 _p(osh))
     return op[:total], oc[:total], oo[:total], osh[:total]
+
+
+def deformed_sh(rest_sh, rot, bind_tet, split_meta, degree):
+    """SH of deform_scene_cpu's output (rest records, then split children in
+    split_meta order); rot: [T, 9] column-major per-tet polar rotations."""
+    rs = np.ascontiguousarray(rest_sh, dtype=np.float32).reshape(-1, 48)
+    n = rs.shape[0]
+    R = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, 9)
+    bt = np.ascontiguousarray(bind_tet, dtype=np.uint32).reshape(n)
+    meta = np.ascontiguousarray(split_meta, dtype=np.uint32).reshape(-1, 2)
+    s = meta.shape[0]
+    out = np.zeros((n + s, 48), np.float32)
+    lib().orc_deformed_sh(_p(rs), n, _p(R), R.shape[0], _p(bt), _p(meta if s else None), s,
+                          int(degree), _p(out))
+    return out
 
 
 def sh_tables():

@@ -1,0 +1,186 @@
+"""GPU export of a tet-deformed scene (cgs_export_deformed_scene): the
+GaussianSceneSoa deform_scene_cpu returns (src/deformation/cpu_reference.rs:
+97-137, 166-182, 394-402) built from the frame's own Stage 1 output.
+
+Bar: positions / covariances / opacities bit-equal to the frame's deformed
+splats (themselves bit-exact against the oracle in test_gpu_deform); SH
+bit-exact against the oracle's ShRotator restatement applied with the
+frame's per-tet rotations; the exported scene rendered without cages gives
+the cage-deformed frame's keys, ids and ranges exactly and its RGBA within
+1/255 (colour from rotated coefficients at d against rest coefficients at
+R^T d).  The two-splat case mirrors
+test_deliverable_deformation_end_to_end_pipeline (tests/integration_tests.rs:
+1320-1400).
+"""
+import numpy as np
+import pytest
+
+from cindergs_amd import _lib
+from cindergs_amd import synthetic as syn
+from cindergs_amd.cages import CageSpringSimulator, TetMesh
+from cindergs_amd.pipeline import GaussianSceneSoa
+
+pytestmark = pytest.mark.gpu
+
+
+def _cage_for(scene, steps=10, impulse=0.2):
+    lo, hi = scene.positions.min(0), scene.positions.max(0)
+    mesh = TetMesh.jittered_grid(lo, hi, (8, 8, 8))
+    tet, w = mesh.bind_gaussians(scene.positions)
+    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim.impulse(impulse, 0.05)
+    for _ in range(steps):
+        sim.step()
+    return mesh, mesh.cage(sim.x, tet, w)
+
+
+@pytest.mark.parametrize("tau,degree", [(0.0, 3), (1.02, 3), (1.02, 1), (0.0, 0)])
+def test_export_matches_oracle(gpu, oracle_lib, tau, degree):
+    import oracle
+
+    scene = syn.scene_gaussian_ball(20_000, k=41)
+    mesh, cage = _cage_for(scene)
+    cam, comp = syn.view_config1()
+    print(f"[case] export tau={tau} degree={degree}", flush=True)
+    gpu.execute_frame_gpu(scene, cage, cam, comp, strain_split_threshold=tau)
+    pos, cov, opa, meta = gpu.read_deformed(scene.count)
+    _, R, _ = gpu.read_tet_transforms(mesh.num_tets)
+    out = gpu.export_deformed_scene(degree)
+    assert out.count == scene.count + meta.shape[0]
+    if tau > 0:
+        assert meta.shape[0] > 0
+    np.testing.assert_array_equal(out.positions.view(np.uint32), pos.view(np.uint32))
+    np.testing.assert_array_equal(out.covariances_3d.view(np.uint32), cov.view(np.uint32))
+    np.testing.assert_array_equal(out.opacities.view(np.uint32), opa.view(np.uint32))
+    exp = oracle.deformed_sh(scene.sh_coeffs, R, cage.bind_tet, meta, degree)
+    np.testing.assert_array_equal(out.sh_coeffs.view(np.uint32), exp.view(np.uint32))
+    # and against the oracle's own Stage 1 (its rotations and split order)
+    d = oracle.deform(cage, scene, tau)
+    exp2 = oracle.deformed_sh(scene.sh_coeffs, d["R"], cage.bind_tet, d["split_meta"], degree)
+    np.testing.assert_array_equal(out.sh_coeffs.view(np.uint32), exp2.view(np.uint32))
+
+
+def test_exported_scene_renders_like_the_cage_frame(gpu):
+    scene = syn.scene_gaussian_ball(20_000, k=42)
+    _, cage = _cage_for(scene)
+    cam, comp = syn.view_config1()
+    tiles = comp.tiles_x * comp.tiles_y
+    print("[case] export render equivalence", flush=True)
+    gpu.set_capture(True)
+    gpu.execute_frame_gpu(scene, cage, cam, comp, strain_split_threshold=1.02,
+                          force_stage3=True)
+    s3_cage = gpu.read_stage3(2 * scene.count, tiles)
+    img_cage = gpu.readback_framebuffer_rgba8(800, 600)
+    out = gpu.export_deformed_scene(comp.sh_degree)
+    assert out.count > scene.count
+    gpu.execute_frame_gpu(out, None, cam, comp, force_stage3=True)
+    s3 = gpu.read_stage3(out.count, tiles)
+    img = gpu.readback_framebuffer_rgba8(800, 600)
+    gpu.set_capture(False)
+    assert s3["n"] == s3_cage["n"] > 0
+    np.testing.assert_array_equal(s3["keys"], s3_cage["keys"])
+    np.testing.assert_array_equal(s3["ids"], s3_cage["ids"])
+    np.testing.assert_array_equal(s3["ranges"], s3_cage["ranges"])
+    diff = np.abs(img.astype(np.int16) - img_cage.astype(np.int16))
+    assert diff.max() <= 1
+    assert img[..., 3].max() > 0
+
+
+def test_export_of_an_f16_scene_is_unsupported(gpu):
+    """Cage deformation of the f16 storage variant is refused (UnsupportedTarget),
+    so is its export."""
+    scene = syn.scene_gaussian_ball(2_000, k=43)
+    _, cage = _cage_for(scene, steps=2)
+    cam, comp = syn.view_config1()
+    gpu.execute_frame_gpu(scene, cage, cam, comp)
+    gpu.sync_scene(syn.quantize_f16(scene, 2))
+    with pytest.raises(_lib.GpuError):
+        gpu.export_deformed_scene(2)
+
+
+def test_two_splat_box_cage_end_to_end(gpu, oracle_lib):
+    """integration_tests.rs:1320-1400: a box cage scaled by 1.2 and moved +0.1 in
+    x carries two splats; the export holds both, with 48 SH each, and renders
+    with both splats surviving.  A uniform scale has R = I, so the SH stay put
+    up to the polar iteration's rounding."""
+    import oracle
+
+    sh0 = np.zeros(48, np.float32)
+    sh0[0] = 1.0
+    sh1 = np.zeros(48, np.float32)
+    sh1[1] = 1.0
+    sh1[5] = 0.3
+    scene = GaussianSceneSoa(np.array([[0, 0, 2.0], [0.5, 0.5, 2.5]], np.float32),
+                             np.array([[0.04, 0, 0, 0.04, 0, 0.04],
+                                       [0.02, 0, 0, 0.02, 0, 0.02]], np.float32),
+                             np.array([0.9, 0.8], np.float32), np.stack([sh0, sh1]))
+    mesh = TetMesh.jittered_grid(np.array([-2, -2, 0.0]), np.array([2, 2, 4.0]), (2, 2, 2),
+                                 jitter=0.0)
+    tet, w = mesh.bind_gaussians(scene.positions)
+    assert (tet != 0xFFFFFFFF).all()
+    moved = mesh.rest_vertices * 1.2 + np.array([0.1, 0, 0], np.float32)
+    cage = mesh.cage(moved.astype(np.float32), tet, w)
+    cam, comp = syn.view_config1()
+    gpu.execute_frame_gpu(scene, cage, cam, comp)
+    out = gpu.export_deformed_scene(2)
+    assert out.count == 2 and out.sh_coeffs.shape == (2, 48)
+    d = oracle.deform(cage, scene)
+    np.testing.assert_array_equal(out.positions.view(np.uint32), d["pos"].view(np.uint32))
+    assert np.abs(out.sh_coeffs - scene.sh_coeffs).max() < 1e-5
+    exp = oracle.deformed_sh(scene.sh_coeffs, d["R"], tet, np.zeros((0, 2), np.uint32), 2)
+    np.testing.assert_array_equal(out.sh_coeffs.view(np.uint32), exp.view(np.uint32))
+    gpu.execute_frame_gpu(out, None, cam, comp)
+    attrs, ext = gpu.read_projected(2)
+    assert (ext != 0).all()
+
+
+def test_export_errors(gpu):
+    scene = syn.scene_gaussian_ball(2_000, k=44)
+    _, cage = _cage_for(scene, steps=2)
+    cam, comp = syn.view_config1()
+    gpu.execute_frame_gpu(scene, None, cam, comp)
+    with pytest.raises(_lib.GpuError):
+        gpu.export_deformed_scene(3)  # no deformation has run
+    gpu.execute_frame_gpu(scene, cage, cam, comp)
+    import ctypes as C
+    cnt = C.c_uint64()
+    bufs = [gpu.to_device(np.zeros(s, np.float32)) for s in ((4, 3), (4, 6), (4,), (4, 48))]
+    try:
+        with pytest.raises(_lib.GpuError):
+            _lib.check(gpu._lib.cgs_export_deformed_scene(gpu._h, 3, *[b.ptr for b in bufs], 4,
+                                                          C.byref(cnt)))
+    finally:
+        for b in bufs:
+            b.free()
+    assert gpu.export_deformed_scene(3).count == scene.count
+
+
+def test_centroid_fan_cage_drives_the_tet_path(gpu, oracle_lib):
+    """integration_tests.rs:1790-1820: PolyCageMesh::to_tet_mesh feeds the tet
+    pipeline; the GPU binding equals the index-order scan, and a cage moved by
+    +0.4 in x moves the bound splats by +0.4 (exactly as the oracle's Stage 1)."""
+    import cage_presets
+    import oracle
+
+    poly = cage_presets.preset("octahedron", [0.0, 0.0, 0.0], 2.0)
+    tm = poly.to_tet_mesh()
+    rng = np.random.default_rng(9)
+    p = rng.uniform(-0.5, 0.5, (500, 3)).astype(np.float32)
+    scene = GaussianSceneSoa(p, np.tile(np.array([0.002, 0, 0, 0.002, 0, 0.002], np.float32),
+                                        (500, 1)),
+                             np.full(500, 0.7, np.float32),
+                             (rng.uniform(-1, 1, (500, 48)) * 0.1).astype(np.float32))
+    tet, w = tm.bind_gaussians_gpu(gpu, p)
+    tet_cpu, w_cpu = tm.bind_gaussians(p)
+    np.testing.assert_array_equal(tet, tet_cpu)
+    np.testing.assert_array_equal(w.view(np.uint32), w_cpu.view(np.uint32))
+    moved = (tm.rest_vertices + np.array([0.4, 0, 0], np.float32)).astype(np.float32)
+    cage = tm.cage(moved, tet, w)
+    cam, comp = syn.view_config1()
+    gpu.execute_frame_gpu(scene, cage, cam, comp)
+    out = gpu.export_deformed_scene(3)
+    d = oracle.deform(cage, scene)
+    np.testing.assert_array_equal(out.positions.view(np.uint32), d["pos"].view(np.uint32))
+    assert np.abs(out.positions - (p + np.array([0.4, 0, 0], np.float32))).max() < 1e-5
+    # a pure translation: R = I, SH within the polar iteration's rounding
+    assert np.abs(out.sh_coeffs - scene.sh_coeffs).max() < 1e-5

@@ -496,3 +496,96 @@ def test_generated_tables_match_the_runtime_construction(oracle_lib):
         gen.main()
         fresh = open(os.path.join(d, "cgs_sh_rotation_gen.h")).read()
     assert fresh == open(os.path.join(csrc, "cgs_sh_rotation_gen.h")).read()
+
+
+def _rand_rot(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_deformed_scene_sh(oracle_lib, degree):
+    """The SH of deform_scene_cpu's output (cpu_reference.rs:141-147, 368-369):
+    a bound record's colour at direction d is the rest colour at R^T d of its
+    tet (float64 evaluator), each record equals sh_rotate of its own SH, bands
+    above the degree, unbound records and tets out of range keep their bits,
+    and a split child carries its parent's rotated SH."""
+    rng = np.random.default_rng(31 + degree)
+    T, n = 6, 40
+    Rs_ = [f32m(_rand_rot(rng)) for _ in range(T - 1)] + [f32m(-np.eye(3))]
+    rot = np.stack([R.T.reshape(9) for R in Rs_]).astype(F32)
+    sh = (rng.uniform(-1, 1, (n, 48)) * 0.05).astype(F32)
+    bind = rng.integers(0, T, n).astype(np.uint32)
+    bind[::7] = 0xFFFFFFFF
+    bind[3] = T + 2
+    meta = np.array([[5, bind[5]], [1, bind[1]], [7, bind[7]]], np.uint32)
+    out = oracle_lib.deformed_sh(sh, rot, bind, meta, degree)
+    assert out.shape == (n + 3, 48)
+    bases = {0: 1, 1: 4, 2: 9, 3: 16}[degree]
+    for i in range(n):
+        if bind[i] >= T or degree == 0:
+            np.testing.assert_array_equal(out[i].view(np.uint32), sh[i].view(np.uint32))
+            continue
+        R = Rs_[bind[i]]
+        np.testing.assert_array_equal(out[i].view(np.uint32),
+                                      oracle_lib.sh_rotate(sh[i], R, degree).view(np.uint32))
+        for c in range(3):
+            tail = out[i, 3 + 15 * c + bases - 1: 3 + 15 * (c + 1)]
+            np.testing.assert_array_equal(tail, sh[i, 3 + 15 * c + bases - 1: 3 + 15 * (c + 1)])
+        for d in rng.normal(size=(8, 3)):
+            d /= np.linalg.norm(d)
+            exp = oracle_lib.eval_sh64(sh[i], R.astype(np.float64).T @ d, degree)
+            got = oracle_lib.eval_sh64(out[i], d, degree)
+            assert np.abs(got - exp).max() < 2e-7
+    for s, (p, _) in enumerate(meta):
+        np.testing.assert_array_equal(out[n + s].view(np.uint32), out[p].view(np.uint32))
+
+
+@pytest.mark.parametrize("kind,a,b", [("octahedron", [0.0, 0.0, 0.0], 2.0),
+                                      ("box", [-1.0, -0.5, 0.0], [1.0, 0.5, 2.0]),
+                                      ("icosahedron", [0.3, 0.0, -0.2], 1.5)])
+def test_to_tet_mesh(kind, a, b):
+    """test_mvc_deform_scene_and_to_tet_mesh (mvc.rs:855-860): one tet per
+    triangle, one extra vertex; every tet has positive volume, the fan covers
+    the cage's volume exactly (star-shaped cages), and a point bound inside
+    it follows a translated cage exactly."""
+    import cage_presets
+
+    cage = cage_presets.preset(kind, a, b)
+    tm = cage.to_tet_mesh()
+    assert tm.num_tets == cage.triangles.shape[0]
+    assert tm.num_vertices == cage.num_vertices + 1
+    np.testing.assert_allclose(tm.rest_vertices[-1], cage.vertices.astype(np.float64).mean(0),
+                               atol=1e-6)
+    V = tm.rest_vertices.astype(np.float64)
+    E = tm.elements
+    vol = np.einsum("ij,ij->i", np.cross(V[E[:, 1]] - V[E[:, 0]], V[E[:, 2]] - V[E[:, 0]]),
+                    V[E[:, 3]] - V[E[:, 0]]) / 6.0
+    assert (vol > 0).all()
+    assert abs(vol.sum() - cage.signed_volume()) < 1e-5 * abs(cage.signed_volume())
+    # the centroid is the 4th corner of every tet
+    assert (E[:, 3] == cage.num_vertices).all()
+    pts = tm.rest_vertices[-1][None] + np.array([[0.1, 0.2, 0.0], [-0.2, 0.1, 0.15]], F32)
+    tet, w = tm.bind_gaussians(pts)
+    assert (np.maximum(-w, 0).sum(1) <= 1e-4).all()
+    moved = (tm.rest_vertices + np.array([0.4, 0, 0], F32)).astype(F32)
+    got = (w[:, :, None] * moved[E[tet]]).sum(1)
+    np.testing.assert_allclose(got, pts + np.array([0.4, 0, 0]), atol=1e-5)
+
+
+def test_to_tet_mesh_flips_inward_triangles():
+    """A triangle wound inward gets its last two corners swapped (mvc.rs:769-774)."""
+    import cage_presets
+
+    cage = cage_presets.preset("octahedron", [0.0, 0.0, 0.0], 1.0)
+    t = cage.triangles.copy()
+    t[0] = t[0][[0, 2, 1]]
+    from cindergs_amd.mvc import PolyCage
+
+    flipped = PolyCage(cage.vertices, t)
+    a, b = cage.to_tet_mesh(), flipped.to_tet_mesh()
+    np.testing.assert_array_equal(a.elements, b.elements)
