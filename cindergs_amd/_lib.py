@@ -100,6 +100,18 @@ class FrameParams(C.Structure):
     ]
 
 
+class SortFrame(C.Structure):
+    """cgs_sort_frame: the frame fields SortCache::take_if_reusable_for_frame keys on
+    (src/render/pipeline.rs:119-145)."""
+    _fields_ = [
+        ("splat_count", C.c_uint64),
+        ("focal_x", C.c_float),
+        ("focal_y", C.c_float),
+        ("split_threshold", C.c_float),
+        ("mip_var", C.c_float),
+    ]
+
+
 class StageTimings(C.Structure):
     """GpuStageTimings (src/render/gpu_state.rs:513-534)."""
 
@@ -197,6 +209,18 @@ SIGNATURES = {
     "cgs_read_deformed": [_vp, _vp, _vp, _vp, _vp, C.POINTER(_u32)],
     "cgs_read_tet_transforms": [_vp, _vp, _vp, _vp],
     "cgs_export_deformed_scene": [_vp, _u32, _vp, _vp, _vp, _vp, _u64, C.POINTER(_u64)],
+    "cgs_sort_cache_create": [C.POINTER(_vp)],
+    "cgs_sort_cache_destroy": [_vp],
+    "cgs_sort_cache_can_reuse": [_vp, _vp, _vp, _u64, _u32, _u32, C.POINTER(_u32)],
+    "cgs_sort_cache_take_if_reusable": [_vp, _vp, _vp, _u64, _u32, _u32, C.POINTER(_u32)],
+    "cgs_sort_cache_take_if_reusable_for_frame": [_vp, _vp, _vp, _u64, _u32, _u32, _vp,
+                                                  C.POINTER(_u32)],
+    "cgs_sort_cache_commit": [_vp, _vp, _vp, _u64, _u32, _u32],
+    "cgs_sort_cache_commit_for_frame": [_vp, _vp, _vp, _u64, _u32, _u32, _vp],
+    "cgs_sort_cache_invalidate": [_vp],
+    "cgs_sort_cache_reuse_count": [_vp, C.POINTER(_u64)],
+    "cgs_set_sort_hysteresis": [_vp, _u32],
+    "cgs_sort_hysteresis_reuse_count": [_vp, C.POINTER(_u64)],
     "cgs_bind_tets": [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
@@ -251,7 +275,8 @@ def load() -> C.CDLL:
         fn.argtypes = args
         if name == "cgs_last_error":
             fn.restype = C.c_char_p
-        elif name in ("cgs_destroy", "cgs_mlp_destroy", "cgs_batch_destroy"):
+        elif name in ("cgs_destroy", "cgs_mlp_destroy", "cgs_batch_destroy",
+                      "cgs_sort_cache_destroy"):
             fn.restype = None
         elif name == "cgs_mlp_param_count":
             fn.restype = _u64

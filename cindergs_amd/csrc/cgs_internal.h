@@ -198,6 +198,7 @@ struct CgsProjectArgs {
   uint32_t num_tets;
   uint32_t num_rest;
   uint32_t max_gaussians;
+  uint32_t zero_culled;  // write zero records for culled / inactive splats
   float* geom;       // [8*max]
   float* color;      // [4*max]
   uint32_t* extents; // [max]
@@ -513,6 +514,7 @@ void cgs_launch_export_deformed(const CgsExportDeformedArgs& a, hipStream_t s);
 
 // error reporting and context access shared by the host files
 int cgs_fail(int code, const char* msg);
+
 hipStream_t cgs_context_stream(struct cgs_context* c);
 // the context's device made current (for ABI files outside runtime.cpp)
 int cgs_context_enter(struct cgs_context* c);

@@ -239,6 +239,10 @@ __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
   if (idx >= active) {
     A.extents[idx] = 0u;
+    if (A.zero_culled) {
+      float4* g = reinterpret_cast<float4*>(A.geom + (size_t)idx * 8);
+      g[0] = g[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     return;
   }
   const CgsSceneDev& S = A.scene;
@@ -264,6 +268,9 @@ __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
     g[0] = g0;
     g[1] = g1;
     reinterpret_cast<float4*>(A.color)[idx] = make_float4(cr, cg, cbl, 0.0f);
+  } else if (A.zero_culled) {
+    float4* g = reinterpret_cast<float4*>(A.geom + (size_t)idx * 8);
+    g[0] = g[1] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   A.extents[idx] = ext;
 }

@@ -27,6 +27,7 @@
 #include "../../include/cindergs.h"
 #include "cgs_internal.h"
 #include "cgs_scene_math.h"
+#include "cgs_sort_cache.h"
 
 namespace {
 
@@ -207,6 +208,10 @@ struct cgs_context {
   bool s3_valid = false;
   Stage3Key s3_key{};
   uint32_t s3_survivors = 0;
+  uint32_t s3_max_g = 0;       // splat slots when the held tile order was sorted
+  // SortCache hysteresis mode (cgs_set_sort_hysteresis)
+  bool hysteresis = false;
+  cgs_sort_cache sort_cache;
   CgsProf prof;
   int prof_count = 0;
   double prof_ms[CgsProf::MAX];
@@ -656,7 +661,7 @@ int project_args(const cgs_context* c, const SceneFrame& F, const cgs_compositor
 // Stage 2 of one view into the arena a
 int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t s,
            CgsProf* prof, const cgs_camera_uniforms* cam, const cgs_compositor_uniforms* comp,
-           const cgs_frame_params* prm) {
+           const cgs_frame_params* prm, bool zero_culled = false) {
   const uint32_t max_g = F.max_g;
   TRY(ensure_words(a.geom, (size_t)8 * max_g));
   TRY(ensure_words(a.color, (size_t)4 * max_g));
@@ -671,6 +676,7 @@ int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t
   PA.geom = P<float>(a.geom);
   PA.color = P<float>(a.color);
   PA.extents = P<uint32_t>(a.extents);
+  PA.zero_culled = zero_culled ? 1u : 0u;
   prof->mark("k_project", s);
   cgs_launch_project(PA, s);
   HIPCHK(hipGetLastError());
@@ -779,12 +785,28 @@ int cgs_invalidate(cgs_context* c) {
   c->deformed_valid = false;
   c->deform_generation++;
   c->s3_valid = false;
+  c->sort_cache.invalidate();
   return CGS_OK;
 }
 
 int cgs_invalidate_stage3(cgs_context* c) {
   if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
   c->s3_valid = false;
+  c->sort_cache.invalidate();
+  return CGS_OK;
+}
+
+int cgs_set_sort_hysteresis(cgs_context* c, uint32_t enable) {
+  if (!c) return fail(CGS_ERR_INVALID_ARG, "null context");
+  c->hysteresis = enable != 0;
+  c->sort_cache = cgs_sort_cache();
+  c->s3_valid = false;
+  return CGS_OK;
+}
+
+int cgs_sort_hysteresis_reuse_count(cgs_context* c, uint64_t* count) {
+  if (!c || !count) return fail(CGS_ERR_INVALID_ARG, "null argument");
+  *count = c->sort_cache.reuse_count;
   return CGS_OK;
 }
 
@@ -969,10 +991,23 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
 
   // ---- Stage 3 cache key (gpu_state.rs:956-981) ----
   const Stage3Key key = stage3_key(c, cam, g, prm);
-  const bool use_cache = !prm->force_stage3 && c->s3_valid && c->s3_key == key;
+  bool use_cache = !prm->force_stage3 && c->s3_valid && c->s3_key == key;
+  // or the SortCache hysteresis of the CPU fallback frame (pipeline.rs:552-600):
+  // cage state = the vertices of this frame's cages, flattened
+  const cgs_sort_frame sf = {c->n_rest, cam->focal_x, cam->focal_y,
+                             prm->strain_split_threshold, prm->mip_filter_3d_var};
+  const float* cage_xyz = c->has_cages ? c->last_verts.data() : nullptr;
+  const uint64_t cage_nv = c->has_cages ? c->last_verts.size() / 3 : 0;
+  if (c->hysteresis) {
+    const bool hit = c->sort_cache.take(c->sort_cache.params_match(sf), cam->view_matrix,
+                                        cage_xyz, cage_nv, g.W, g.H);
+    use_cache = hit && !prm->force_stage3 && c->s3_valid && c->s3_max_g == max_g;
+  }
 
   // ---- Stage 2 ----
-  TRY(stage2(c, F, c->fa, s, &c->prof, cam, comp, prm));
+  // (a splat culled now but present in a reused tile order gets a record of
+  // zero opacity, so that it contributes nothing)
+  TRY(stage2(c, F, c->fa, s, &c->prof, cam, comp, prm, c->hysteresis && use_cache));
   if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
 
   // ---- Stage 3 ----
@@ -1007,10 +1042,12 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
     c->s3_key = key;
     c->s3_valid = true;
     c->s3_survivors = survivors;
+    c->s3_max_g = max_g;
     T.stage3_reruns = (uint32_t)reruns;
   } else {
     T.stage3_cached = 1;
   }
+  if (c->hysteresis) c->sort_cache.commit_frame(sf, cam->view_matrix, cage_xyz, cage_nv, g.W, g.H);
   c->active = N + (F.split_possible ? splits : 0u);
   if (c->prof.on) {
     c->prof.mark("end", s);

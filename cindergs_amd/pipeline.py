@@ -285,6 +285,18 @@ class GpuPipelineState:
                 b.free()
         return GaussianSceneSoa(*out)
 
+    def set_sort_hysteresis(self, enable: bool):
+        """The CPU fallback frame's SortCache hysteresis (pipeline.rs:539-600) inside
+        execute_frame_gpu: reuse the previous tile order while the camera and the
+        cages move by at most 1e-5."""
+        check(self._lib.cgs_set_sort_hysteresis(self._h, 1 if enable else 0))
+
+    @property
+    def sort_reuse_count(self) -> int:
+        n = C.c_uint64()
+        check(self._lib.cgs_sort_hysteresis_reuse_count(self._h, C.byref(n)))
+        return n.value
+
     def read_tet_transforms(self, num_tets: int):
         J = np.zeros((num_tets, 9), dtype=np.float32)
         R = np.zeros((num_tets, 9), dtype=np.float32)
@@ -349,6 +361,99 @@ class GpuPipelineState:
         """gpu_radix_sort_u64_pairs (radix_sort.rs:377-477), in place."""
         check(self._lib.cgs_radix_sort_u64_pairs(self._h, tiles.ptr, depths.ptr, ids.ptr, n,
                                                  tile_bits))
+
+
+def _view16(view) -> np.ndarray:
+    v = np.ascontiguousarray(view, dtype=np.float32).reshape(-1)
+    if v.shape[0] != 16:
+        raise ValueError("a view matrix has 16 floats")
+    return v
+
+
+def _cage_xyz(cage_state):
+    c = np.ascontiguousarray(cage_state, dtype=np.float32).reshape(-1, 3)
+    return c, c.shape[0]
+
+
+class SortCache:
+    """SortCache (src/render/pipeline.rs:66-193) over the C ABI's cgs_sort_cache.
+    The binning (any object: the reference's TileBinningOutputs) is held here;
+    the native object decides reuse and counts reuses."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        p = C.c_void_p()
+        check(self._lib.cgs_sort_cache_create(C.byref(p)))
+        self._h = p
+        self.binning = None
+
+    def close(self):
+        if self._h:
+            self._lib.cgs_sort_cache_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _frame(splat_count, focal_x, focal_y, split_threshold, mip_var):
+        return _lib.SortFrame(int(splat_count), float(focal_x), float(focal_y),
+                              float(split_threshold), float(mip_var))
+
+    def can_reuse(self, view_matrix, cage_state, width: int, height: int) -> bool:
+        v, (c, n) = _view16(view_matrix), _cage_xyz(cage_state)
+        r = C.c_uint32()
+        check(self._lib.cgs_sort_cache_can_reuse(self._h, _ptr(v), _ptr(c), n, width, height,
+                                                 C.byref(r)))
+        return bool(r.value)
+
+    def _take(self, reused: int):
+        b, self.binning = self.binning, None
+        return b if reused else None
+
+    def take_if_reusable(self, view_matrix, cage_state, width: int, height: int):
+        v, (c, n) = _view16(view_matrix), _cage_xyz(cage_state)
+        r = C.c_uint32()
+        check(self._lib.cgs_sort_cache_take_if_reusable(self._h, _ptr(v), _ptr(c), n, width,
+                                                        height, C.byref(r)))
+        return self._take(r.value)
+
+    def take_if_reusable_for_frame(self, view_matrix, cage_state, width: int, height: int,
+                                   splat_count: int, focal_x: float, focal_y: float,
+                                   split_threshold: float, mip_var: float):
+        v, (c, n) = _view16(view_matrix), _cage_xyz(cage_state)
+        f = self._frame(splat_count, focal_x, focal_y, split_threshold, mip_var)
+        r = C.c_uint32()
+        check(self._lib.cgs_sort_cache_take_if_reusable_for_frame(
+            self._h, _ptr(v), _ptr(c), n, width, height, C.byref(f), C.byref(r)))
+        return self._take(r.value)
+
+    def commit(self, view_matrix, cage_state, width: int, height: int, binning):
+        v, (c, n) = _view16(view_matrix), _cage_xyz(cage_state)
+        check(self._lib.cgs_sort_cache_commit(self._h, _ptr(v), _ptr(c), n, width, height))
+        self.binning = binning
+
+    def commit_for_frame(self, view_matrix, cage_state, width: int, height: int,
+                         splat_count: int, focal_x: float, focal_y: float,
+                         split_threshold: float, mip_var: float, binning):
+        v, (c, n) = _view16(view_matrix), _cage_xyz(cage_state)
+        f = self._frame(splat_count, focal_x, focal_y, split_threshold, mip_var)
+        check(self._lib.cgs_sort_cache_commit_for_frame(self._h, _ptr(v), _ptr(c), n, width,
+                                                        height, C.byref(f)))
+        self.binning = binning
+
+    def invalidate(self):
+        check(self._lib.cgs_sort_cache_invalidate(self._h))
+        self.binning = None
+
+    @property
+    def reuse_count(self) -> int:
+        n = C.c_uint64()
+        check(self._lib.cgs_sort_cache_reuse_count(self._h, C.byref(n)))
+        return n.value
 
 
 class DeviceBuffer:

@@ -91,9 +91,62 @@ const char* cgs_last_error(void);
 int cgs_device_name(cgs_context* ctx, char* buf, size_t len);
 int cgs_synchronize(cgs_context* ctx);
 
-/* GpuPipelineState::invalidate / invalidate_stage3 (gpu_state.rs:565-573) */
+/* GpuPipelineState::invalidate / invalidate_stage3 (gpu_state.rs:565-573);
+ * both also drop the context's sort cache (see cgs_set_sort_hysteresis) */
 int cgs_invalidate(cgs_context* ctx);
 int cgs_invalidate_stage3(cgs_context* ctx);
+
+/* ---- temporal hysteresis for Stage 3: SortCache (src/render/pipeline.rs:
+ *      40-63, 66-193) ----
+ * A host object with the reference's methods.  It keeps the view matrix, the
+ * flattened cage vertices (xyz per vertex), the frame size and, for the
+ * *_for_frame forms, the splat count and the bits of focal_x, focal_y, the
+ * split threshold and the mip variance of the last committed binning.  A
+ * later frame takes it back when the view moved by at most 1e-5 (largest
+ * absolute element difference), no cage coordinate moved by more than 1e-5
+ * and the rest is equal.  The binning (TileBinningOutputs) stays with the
+ * caller; the object records whether one is held: take_* clears that, a
+ * commit sets it.  No device work. */
+typedef struct cgs_sort_cache cgs_sort_cache;
+typedef struct cgs_sort_frame {
+  uint64_t splat_count;
+  float focal_x, focal_y, split_threshold, mip_var;
+} cgs_sort_frame;
+int cgs_sort_cache_create(cgs_sort_cache** out);   /* SortCache::default */
+void cgs_sort_cache_destroy(cgs_sort_cache* cache);
+int cgs_sort_cache_can_reuse(const cgs_sort_cache* cache, const float view[16],
+                             const float* cage_xyz, uint64_t num_cage_vertices, uint32_t width,
+                             uint32_t height, uint32_t* reusable);              /* 85-101 */
+int cgs_sort_cache_take_if_reusable(cgs_sort_cache* cache, const float view[16],
+                                    const float* cage_xyz, uint64_t num_cage_vertices,
+                                    uint32_t width, uint32_t height, uint32_t* reused); /* 103-117 */
+int cgs_sort_cache_take_if_reusable_for_frame(cgs_sort_cache* cache, const float view[16],
+                                              const float* cage_xyz, uint64_t num_cage_vertices,
+                                              uint32_t width, uint32_t height,
+                                              const cgs_sort_frame* frame,
+                                              uint32_t* reused);                /* 119-145 */
+int cgs_sort_cache_commit(cgs_sort_cache* cache, const float view[16], const float* cage_xyz,
+                          uint64_t num_cage_vertices, uint32_t width, uint32_t height); /* 147-161 */
+int cgs_sort_cache_commit_for_frame(cgs_sort_cache* cache, const float view[16],
+                                    const float* cage_xyz, uint64_t num_cage_vertices,
+                                    uint32_t width, uint32_t height,
+                                    const cgs_sort_frame* frame);               /* 163-184 */
+int cgs_sort_cache_invalidate(cgs_sort_cache* cache);                           /* 186-192 */
+int cgs_sort_cache_reuse_count(const cgs_sort_cache* cache, uint64_t* count);   /* reuse_count */
+
+/* The same hysteresis inside cgs_execute_frame, as the reference's CPU
+ * fallback frame applies it (execute_cpu_fallback_frame, pipeline.rs:539-600):
+ * with enable = 1 each frame asks the context's sort cache
+ * (take_if_reusable_for_frame with the frame's view matrix, cage vertices,
+ * size, splat count, focal lengths, split threshold and mip variance); on a
+ * hit Stages 1-2 run as usual, Stage 3 is skipped and Stage 4 composites the
+ * new projection in the previous frame's tile order (stage3_cached = 1); a
+ * splat culled in the new frame but present in that order contributes
+ * nothing.  Every frame then commits.  enable = 0 (the default) restores the
+ * exact Stage 3 cache key of gpu_state.rs:454-499.  Either call starts a
+ * fresh cache (reuse count 0). */
+int cgs_set_sort_hysteresis(cgs_context* ctx, uint32_t enable);
+int cgs_sort_hysteresis_reuse_count(cgs_context* ctx, uint64_t* count);
 
 /* GpuSceneState::sync_scene_and_cages, scene part (gpu_state.rs:182-213):
  * GaussianSceneSoa (src/scene/soa.rs:30-45) — positions[3n] xyz, covariances
