@@ -247,12 +247,64 @@ def cage_frames(scene, frames: int = 300):
         st.sync_cages(mesh.cage(sim.x, tet, w), scene.count)
         st.execute_frame_resident(*cams[f % 120], prof, tim)
         s1.append(tim.stage1_deform_ms)
+    exp = export_deformed(st, scene, mesh.num_tets, tet)
     st.close()
     return {"workload": f"{scene.count} splats bound to {mesh.num_tets} tets, spring-driven "
                         f"cage, strain split 1.02, 1920x1080, {frames} frames",
             "ms_per_frame": round(ms, 4), "stage1_ms": round(float(np.mean(s1)), 4),
             "stage1_ran_frames": ran, "split_children_mean": int(np.mean(splits)),
-            "gpu_tet_binding_ms": round(bind_ms, 1)}
+            "gpu_tet_binding_ms": round(bind_ms, 1), "export": exp}
+
+
+def export_deformed(st, scene, num_tets: int, bind_tet, reps: int = 5):
+    """cgs_export_deformed_scene after the last cage frame: the deformed scene as
+    deform_scene_cpu returns it, SH rotated per tet.  Algorithmic bytes per
+    record: geometry 40 + SH 192 + tet 4 read (a split child reads its 8-byte
+    split record instead of the tet), 40 + 192 written; the per-tet matrices
+    (83 floats per tet) stay in cache.  CPU baseline: the oracle's rotation of
+    the same records' SH on one core."""
+    import ctypes as C
+
+    cnt = C.c_uint64()
+    st._lib.cgs_export_deformed_scene(st._h, 3, None, None, None, None, 0, C.byref(cnt))
+    m = cnt.value
+    bufs = [st.to_device(np.zeros(s, np.float32)) for s in ((m, 3), (m, 6), (m,), (m, 48))]
+    times = {}
+    try:
+        for r in range(reps + 1):
+            from cindergs_amd._lib import check
+            check(st._lib.cgs_export_deformed_scene(st._h, 3, *[b.ptr for b in bufs], m,
+                                                    C.byref(cnt)))
+            if r:
+                for name, v in st.kernel_times():
+                    times.setdefault(name, []).append(v)
+        _, R, _ = st.read_tet_transforms(num_tets)
+        _, _, _, meta = st.read_deformed(scene.count)
+    finally:
+        for b in bufs:
+            b.free()
+    kms = {k: round(float(np.mean(v)), 4) for k, v in times.items()}
+    splits = m - scene.count
+    bpr = 40 + 192 + 4 + 40 + 192
+    nbytes = bpr * m + 4 * splits
+    res = {"records": int(m), "split_children": int(splits), "kernel_ms": kms}
+    ke = kms.get("k_export_deformed")
+    if ke:
+        gbs = nbytes / (ke * 1e-3) / 1e9
+        res["roofline"] = {"bound": "hbm", "bytes_per_launch": int(nbytes),
+                           "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, timed here as the CPU baseline of this leg
+
+    oracle.lib()
+    k = min(m, 262144)
+    t = time.perf_counter()
+    oracle.deformed_sh(scene.sh_coeffs[:k], R, bind_tet[:k], np.zeros((0, 2), np.uint32), 3)
+    cpu_s = time.perf_counter() - t
+    res["cpu_oracle"] = {"records_per_s": round(k / cpu_s), "cores": 1, "kind": "port",
+                         "sample": f"SH of the first {k} records"}
+    return res
 
 
 def mvc_binding(n_points: int, train_steps: int, n_train: int):
@@ -655,6 +707,8 @@ def main() -> int:
                     help="MVC scene deformation of SPLATS bound splats (0 = skip; part of the extras)")
     ap.add_argument("--mvc-scene-only", action="store_true")
     ap.add_argument("--ingest-only", action="store_true")
+    ap.add_argument("--cage-only", action="store_true",
+                    help="only the cage-deformation leg (frames, Stage 1, scene export)")
     ap.add_argument("--mvc-steps", type=int, default=20000)
     ap.add_argument("--mvc-train", type=int, default=2_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -695,6 +749,10 @@ def main() -> int:
         return 0
     if args.ingest_only:
         print(json.dumps({"ingest": ingest(syn.scene_clusters(args.splats))}), flush=True)
+        return 0
+    if args.cage_only:
+        print(json.dumps({"cage": cage_frames(syn.scene_clusters(args.splats), args.cage_frames)}),
+              flush=True)
         return 0
 
     t0 = time.perf_counter()

@@ -495,7 +495,11 @@ void cgs_launch_quantize(const float* cov, const float* sh, uint64_t n, uint32_t
 void cgs_launch_export_raw(const float* pos, const float* cov, const float* opa,
                            const float* sh, uint64_t n, float* records, hipStream_t s);
 // the deformed scene of the last Stage 1 run as a SoA, SH rotated per tet
-#define CGS_SH_MATS 83u  // 9 + 25 + 49 floats: the band 1-3 matrices of one rotation
+// the band 1-3 matrices of one rotation (9, 25, 49 floats) at float offsets
+// 0, 12 and 40, each padded to whole float4 (16-byte loads)
+#define CGS_SH_MATS 92u
+#define CGS_SH_M2 12u
+#define CGS_SH_M3 40u
 struct CgsExportDeformedArgs {
   const float* geo;          // Stage 1 output, blocked K=10
   const float* sh;           // rest SH, blocked K=48 f32
@@ -508,8 +512,8 @@ struct CgsExportDeformedArgs {
   float* opa;                // [count]
   float* out_sh;             // [48 count], 16-byte aligned
 };
-void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, const double* tables,
-                                float* mats, hipStream_t s);
+void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, float* mats,
+                                hipStream_t s);
 void cgs_launch_export_deformed(const CgsExportDeformedArgs& a, hipStream_t s);
 
 // error reporting and context access shared by the host files

@@ -203,7 +203,9 @@ def groups(l):
     return sorted(out.values(), key=lambda g: g[0])
 
 
-def band(em, l):
+def band(em, l, matrix_only=False):
+    """Band l's rotation applied to sh (or, with matrix_only, its matrix stored
+    row-major into M, as cgs_sh_band_matrix stores it)."""
     rows, n = 2 * l + 1, nmono(l)
     ob, op = table_offset(l, 0), table_offset(l, 1)
     em.lines.append(f"  // band {l}: rest slots {l * l - 1}..{l * l + 2 * l - 1} of each channel")
@@ -236,6 +238,11 @@ def band(em, l):
             # M[j][i] = (float) D[i][j]
             for j in range(rows):
                 m[j, i] = em.tmp(f"(float){d[j]}", "float")
+    if matrix_only:
+        for j in range(rows):
+            for i in range(rows):
+                em.lines.append(f"  M[{j * rows + i}] = {m[j, i]};")
+        return
     # c'[j] = sum over i ascending of M[j][i] c[i], per channel, in f32
     for ch in range(3):
         orig = [em.tmp(f"sh[{3 + ch * 15 + base + i}]", "float") for i in range(rows)]
@@ -262,6 +269,16 @@ def main():
         em = Emitter()
         band(em, l)
         out.append(f"CGS_HD void cgs_sh_rotate_band{l}(float* sh, const float* R) {{")
+        for a in range(3):
+            for c in range(3):
+                out.append(f"  const double fm{a}{c} = (double)R[{3 * a + c}];")
+        out += em.lines
+        out += ["}", ""]
+    for l in (1, 2, 3):
+        em = Emitter()
+        band(em, l, matrix_only=True)
+        out.append(f"// cgs_sh_band_matrix(R, {l}, tables, M) with the same operations")
+        out.append(f"CGS_HD void cgs_sh_band{l}_matrix(const float* R, float* M) {{")
         for a in range(3):
             for c in range(3):
                 out.append(f"  const double fm{a}{c} = (double)R[{3 * a + c}];")

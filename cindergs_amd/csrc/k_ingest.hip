@@ -199,62 +199,90 @@ __global__ __launch_bounds__(256) void k_export_raw(const float* pos, const floa
 // ShRotator::new for every tet (sh_rotation.rs:404-425): the f32 rotation
 // matrices of bands 1-3 of the tet's polar rotation, 9 + 25 + 49 floats per
 // tet, built once and shared by every splat bound to it
+// ShRotator::new per tet (sh_rotation.rs:412-430): thread (t, band l =
+// blockIdx.y + 1) writes band l's matrix of tet t, with the generated
+// straight-line form of cgs_sh_band_matrix (same operations, same values)
 __global__ __launch_bounds__(64) void k_tet_sh_matrices(const float* rot, uint32_t num_tets,
-                                                        const double* tables, float* mats) {
+                                                        float* mats) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   if (t >= num_tets) return;
   float R[9];
   for (int k = 0; k < 9; ++k) R[k] = rot[(size_t)t * 9 + k];
   float* m = mats + (size_t)t * CGS_SH_MATS;
-  cgs_sh_band_matrix(R, 1, tables, m);
-  cgs_sh_band_matrix(R, 2, tables, m + 9);
-  cgs_sh_band_matrix(R, 3, tables, m + 34);
+  if (blockIdx.y == 0) {
+    cgs_sh_band1_matrix(R, m);
+    for (uint32_t k = 9; k < CGS_SH_M2; ++k) m[k] = 0.0f;  // the pads
+  } else if (blockIdx.y == 1) {
+    cgs_sh_band2_matrix(R, m + CGS_SH_M2);
+    for (uint32_t k = CGS_SH_M2 + 25u; k < CGS_SH_M3; ++k) m[k] = 0.0f;
+  } else {
+    cgs_sh_band3_matrix(R, m + CGS_SH_M3);
+    for (uint32_t k = CGS_SH_M3 + 49u; k < CGS_SH_MATS; ++k) m[k] = 0.0f;
+  }
 }
 
 // one record of the deformed scene for export: the Stage 1 geometry, and the
 // SH of its source splat (a split child's parent) rotated by ShRotator::apply
-// (sh_rotation.rs:427-446) of its tet; unbound records keep their SH
-__global__ __launch_bounds__(256) void k_export_deformed(CgsExportDeformedArgs A) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= A.count) return;
-  for (uint32_t k = 0; k < 3u; ++k) A.pos[(size_t)i * 3 + k] = A.geo[cgs_blk(i, k, CGS_GEO_K)];
-  for (uint32_t k = 0; k < 6u; ++k)
-    A.cov[(size_t)i * 6 + k] = A.geo[cgs_blk(i, 3u + k, CGS_GEO_K)];
-  A.opa[i] = A.geo[cgs_blk(i, CGS_CH_OPA, CGS_GEO_K)];
-  uint32_t src = i, tet = A.num_tets;
-  if (i < A.num_rest) {
-    tet = A.bind_tet[i];
-  } else {
-    src = A.split_meta[2u * (i - A.num_rest)];
-    tet = A.split_meta[2u * (i - A.num_rest) + 1u];
-  }
-  float sh[48];
-  for (uint32_t k = 0; k < 48u; ++k) sh[k] = A.sh[cgs_blk(src, k, CGS_SH_K)];
-  if (A.degree > 0u && tet < A.num_tets) {
-    const float* m = A.mats + (size_t)tet * CGS_SH_MATS;
-    for (uint32_t ch = 0; ch < 3u; ++ch) {
-      const uint32_t rest = 3u + ch * 15u;
-      cgs_sh_apply_block(sh, rest, 3, m);
-      if (A.degree >= 2u) cgs_sh_apply_block(sh, rest + 3u, 5, m + 9);
-      if (A.degree >= 3u) cgs_sh_apply_block(sh, rest + 8u, 7, m + 34);
+// (sh_rotation.rs:427-446) of its tet; unbound records keep their SH.  The
+// outputs go through LDS at odd strides so that the block writes its rows of
+// every output array with coalesced stores.
+__global__ __launch_bounds__(AT) void k_export_deformed(CgsExportDeformedArgs A) {
+  __shared__ uint32_t o_sh[AT * 49u];
+  __shared__ uint32_t o_pos[AT * 3u], o_cov[AT * 7u], o_opa[AT];
+  const uint32_t t = threadIdx.x, r0 = blockIdx.x * AT, i = r0 + t;
+  const uint32_t cnt = min(AT, A.count - r0);
+  if (i < A.count) {
+    for (uint32_t k = 0; k < 3u; ++k) o_pos[3u * t + k] = cgs_f2u(A.geo[cgs_blk(i, k, CGS_GEO_K)]);
+    for (uint32_t k = 0; k < 6u; ++k)
+      o_cov[7u * t + k] = cgs_f2u(A.geo[cgs_blk(i, 3u + k, CGS_GEO_K)]);
+    o_opa[t] = cgs_f2u(A.geo[cgs_blk(i, CGS_CH_OPA, CGS_GEO_K)]);
+    uint32_t src = i, tet = A.num_tets;
+    if (i < A.num_rest) {
+      tet = A.bind_tet[i];
+    } else {
+      src = A.split_meta[2u * (i - A.num_rest)];
+      tet = A.split_meta[2u * (i - A.num_rest) + 1u];
     }
+    float sh[48];
+    for (uint32_t k = 0; k < 48u; ++k) sh[k] = A.sh[cgs_blk(src, k, CGS_SH_K)];
+    if (A.degree > 0u && tet < A.num_tets) {
+      // band by band (the bands are disjoint blocks, so the per-channel order
+      // of ShRotator::apply gives the same values), each matrix fetched with
+      // 16-byte loads: the records' tets differ from lane to lane
+      const float4* m4 = reinterpret_cast<const float4*>(A.mats + (size_t)tet * CGS_SH_MATS);
+      float M[52];
+      for (uint32_t q = 0; q < 3u; ++q) reinterpret_cast<float4*>(M)[q] = m4[q];
+      for (uint32_t ch = 0; ch < 3u; ++ch) cgs_sh_apply_block(sh, 3u + ch * 15u, 3, M);
+      if (A.degree >= 2u) {
+        for (uint32_t q = 0; q < 7u; ++q) reinterpret_cast<float4*>(M)[q] = m4[CGS_SH_M2 / 4u + q];
+        for (uint32_t ch = 0; ch < 3u; ++ch) cgs_sh_apply_block(sh, 6u + ch * 15u, 5, M);
+      }
+      if (A.degree >= 3u) {
+        for (uint32_t q = 0; q < 13u; ++q) reinterpret_cast<float4*>(M)[q] = m4[CGS_SH_M3 / 4u + q];
+        for (uint32_t ch = 0; ch < 3u; ++ch) cgs_sh_apply_block(sh, 11u + ch * 15u, 7, M);
+      }
+    }
+    for (uint32_t k = 0; k < 48u; ++k) o_sh[49u * t + k] = cgs_f2u(sh[k]);
   }
-  float4* o = reinterpret_cast<float4*>(A.out_sh + (size_t)i * 48);
-  for (int k = 0; k < 12; ++k) o[k] = make_float4(sh[4 * k], sh[4 * k + 1], sh[4 * k + 2], sh[4 * k + 3]);
+  __syncthreads();
+  lds_to_global<3u, 3u>(o_pos, A.pos + (size_t)r0 * 3u, cnt, t);
+  lds_to_global<6u, 7u>(o_cov, A.cov + (size_t)r0 * 6u, cnt, t);
+  lds_to_global<1u, 1u>(o_opa, A.opa + r0, cnt, t);
+  lds_to_global<48u, 49u>(o_sh, A.out_sh + (size_t)r0 * 48u, cnt, t);
 }
 
 }  // namespace
 
-void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, const double* tables,
-                                float* mats, hipStream_t s) {
+void cgs_launch_tet_sh_matrices(const float* rot, uint32_t num_tets, float* mats,
+                                hipStream_t s) {
   if (num_tets == 0) return;
-  hipLaunchKernelGGL(k_tet_sh_matrices, dim3((num_tets + 63u) / 64u), dim3(64), 0, s, rot,
-                     num_tets, tables, mats);
+  hipLaunchKernelGGL(k_tet_sh_matrices, dim3((num_tets + 63u) / 64u, 3u), dim3(64), 0, s, rot,
+                     num_tets, mats);
 }
 
 void cgs_launch_export_deformed(const CgsExportDeformedArgs& a, hipStream_t s) {
   if (a.count == 0) return;
-  hipLaunchKernelGGL(k_export_deformed, dim3((a.count + 255u) / 256u), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_export_deformed, dim3((a.count + AT - 1u) / AT), dim3(AT), 0, s, a);
 }
 
 void cgs_launch_activate_raw(const CgsActivateArgs& a, hipStream_t s) {

@@ -196,9 +196,8 @@ struct cgs_context {
   DevBuf mvc_work;
   // scene ingest: the vertex payload of a PLY file
   DevBuf ingest;
-  // export of a deformed scene: SH basis tables (doubles) and per-tet matrices
-  DevBuf sh_tables, tet_shm;
-  bool sh_tables_ready = false;
+  // export of a deformed scene: per-tet SH band matrices
+  DevBuf tet_shm;
 
   // ---- last frame ----
   uint32_t last_w = 0, last_h = 0;
@@ -753,7 +752,7 @@ void cgs_destroy(cgs_context* c) {
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
                     &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
                     &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
-                    &c->ingest, &c->sh_tables, &c->tet_shm};
+                    &c->ingest, &c->tet_shm};
   for (DevBuf* b : bufs) release(*b);
   c->fa.release_all();
   for (auto& e : c->ev)
@@ -1290,16 +1289,10 @@ int cgs_export_deformed_scene(cgs_context* c, uint32_t sh_degree, float* d_pos, 
     return fail(CGS_ERR_INVALID_ARG, "d_sh must be 16-byte aligned");
   if (m > capacity) return fail(CGS_ERR_INVALID_ARG, "capacity below the deformed splat count");
   const uint32_t deg = sh_degree > 3u ? 3u : sh_degree;
-  if (!c->sh_tables_ready) {
-    double h[CGS_SH_TABLE_DOUBLES];
-    cgs_sh_basis_tables(h);
-    TRY(ensure_words(c->sh_tables, CGS_SH_TABLE_DOUBLES * 2));
-    HIPCHK(hipMemcpy(c->sh_tables.p, h, sizeof(h), hipMemcpyHostToDevice));
-    c->sh_tables_ready = true;
-  }
   TRY(ensure_words(c->tet_shm, (size_t)c->num_tets * CGS_SH_MATS));
-  cgs_launch_tet_sh_matrices(P<float>(c->rot), c->num_tets, P<double>(c->sh_tables),
-                             P<float>(c->tet_shm), c->stream);
+  cgs_context_prof_begin(c);  // per-launch device times, for cgs_kernel_times
+  c->prof.mark("k_tet_sh_matrices", c->stream);
+  cgs_launch_tet_sh_matrices(P<float>(c->rot), c->num_tets, P<float>(c->tet_shm), c->stream);
   CgsExportDeformedArgs A;
   A.geo = P<float>(c->def_geo);
   A.sh = P<float>(c->rest_sh);
@@ -1314,8 +1307,12 @@ int cgs_export_deformed_scene(cgs_context* c, uint32_t sh_degree, float* d_pos, 
   A.cov = d_cov;
   A.opa = d_opa;
   A.out_sh = d_sh;
+  c->prof.mark("k_export_deformed", c->stream);
   cgs_launch_export_deformed(A, c->stream);
-  HIPCHK(hipGetLastError());
+  const hipError_t e = hipGetLastError();
+  cgs_context_prof_end(c, e == hipSuccess);
+  if (e != hipSuccess)
+    return fail(CGS_ERR_EXECUTION, std::string("export deformed scene: ") + hipGetErrorString(e));
   HIPCHK(hipStreamSynchronize(c->stream));
   *count = m;
   return CGS_OK;

@@ -192,8 +192,8 @@ int cgs_framebuffer_device_ptr(cgs_context* ctx, void** ptr, uint32_t* padded_ro
 /* Per-launch device times of the last cgs_execute_frame run with
  * profile_stages >= 2 (the reference's measure_stage, gpu_state.rs:575-608,
  * at kernel granularity), or of the last cgs_mvc_deform_scene,
- * cgs_activate_raw (and cgs_ply_load), cgs_quantize_scene or cgs_export_raw
- * call: names[32*i] (NUL-terminated) and ms[i]. */
+ * cgs_activate_raw (and cgs_ply_load), cgs_quantize_scene, cgs_export_raw or
+ * cgs_export_deformed_scene call: names[32*i] (NUL-terminated) and ms[i]. */
 int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, uint32_t* count);
 
 /* ---- launch tuning ----

@@ -1535,6 +1535,12 @@ ORC_API void orc_sh_rotate(float* sh, const float* R, int degree) {
 ORC_API void orc_sh_rotate_fast(float* sh, const float* R, int degree) {
   cgs_sh_rotate_fast(sh, R, degree);
 }
+// and of the band matrices
+ORC_API void orc_sh_band_matrix_fast(const float* R, int l, float* M) {
+  if (l == 1) cgs_sh_band1_matrix(R, M);
+  else if (l == 2) cgs_sh_band2_matrix(R, M);
+  else cgs_sh_band3_matrix(R, M);
+}
 ORC_API void orc_polar_rotation(const float* J, float* R) { cgs_polar_rotation(J, R); }
 ORC_API void orc_sym_eigen(const float* cov, float* evals, float* basis) {
   cgs_sym_eigen(cov, evals, basis);

@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
             "orc_sh_band_matrix": ([_vp, _i, _vp], None),
             "orc_sh_rotate": ([_vp, _vp, _i], None),
             "orc_sh_rotate_fast": ([_vp, _vp, _i], None),
+            "orc_sh_band_matrix_fast": ([_vp, _i, _vp], None),
             "orc_polar_rotation": ([_vp, _vp], None),
             "orc_sym_eigen": ([_vp, _vp, _vp], None),
             "orc_max_stretch": ([_vp], _f),
@@ -451,6 +452,13 @@ def sh_band_matrix(R, l):
     """(2l+1) x (2l+1) rotation matrix of SH band l for the rotation R."""
     M = np.zeros((2 * l + 1) ** 2, np.float32)
     lib().orc_sh_band_matrix(_p(_m3_in(R)), int(l), _p(M))
+    return M.reshape(2 * l + 1, 2 * l + 1)
+
+
+def sh_band_matrix_fast(R, l):
+    """The kernel's straight-line form of sh_band_matrix (cgs_sh_rotation_gen.h)."""
+    M = np.zeros((2 * l + 1) ** 2, np.float32)
+    lib().orc_sh_band_matrix_fast(_p(_m3_in(R)), int(l), _p(M))
     return M.reshape(2 * l + 1, 2 * l + 1)
 
 

@@ -475,6 +475,21 @@ def test_generated_sh_rotation_is_bit_identical_to_the_loop_form(oracle_lib):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_generated_band_matrices_are_bit_identical_to_the_loop_form(oracle_lib):
+    """The export kernel's per-tet matrices (cgs_sh_band{1,2,3}_matrix) equal
+    cgs_sh_band_matrix bit for bit over rotations, reflections, axis flips and
+    random matrices."""
+    rng = np.random.default_rng(17)
+    mats = [np.eye(3), -np.eye(3), np.diag([1.0, -1.0, 1.0]), rot_z(0.5) @ rot_x(1.0)]
+    mats += [_rand_rot(rng) for _ in range(150)] + [rng.normal(size=(3, 3)) for _ in range(50)]
+    for R in mats:
+        R32 = f32m(R)
+        for l in (1, 2, 3):
+            a = oracle_lib.sh_band_matrix(R32, l)
+            b = oracle_lib.sh_band_matrix_fast(R32, l)
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_generated_tables_match_the_runtime_construction(oracle_lib):
     """The generator's B and B^+ (emitted as literals) equal the 218 doubles of
     cgs_sh_basis_tables bit for bit, and the committed header is what the
