@@ -232,7 +232,38 @@ __device__ __forceinline__ void sh_direction(const CgsProjectArgs& A, float px, 
   }
 }
 
-template <bool F16>
+// The SH coefficients a splat of degree DEG uses, loaded together once the
+// splat survives (the degree is a template argument, so the bands above it
+// are neither loaded nor evaluated).
+template <int DEG>
+struct ShRec {
+  static constexpr int NB = (DEG + 1) * (DEG + 1);  // bases per channel
+  float c[3 * NB];
+};
+// record index of basis j of channel ch: dc r g b, then 15 rest per channel
+__device__ __forceinline__ constexpr uint32_t sh_k(int ch, int j) {
+  return j == 0 ? (uint32_t)ch : (uint32_t)(3 + ch * 15 + (j - 1));
+}
+template <bool F16, int DEG>
+__device__ __forceinline__ void load_rec(const CgsSceneDev& S, uint32_t i, ShRec<DEG>& R) {
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+    for (int j = 0; j < ShRec<DEG>::NB; ++j)
+      R.c[ch * ShRec<DEG>::NB + j] = ld_sh<F16>(S, sh_k(ch, j), i);
+}
+template <int DEG>
+struct ShView {
+  const ShRec<DEG>& R;
+  // coefficient k of the record (k constant after inlining)
+  __device__ __forceinline__ float operator()(uint32_t k) const {
+    const int ch = k < 3u ? (int)k : (int)((k - 3u) / 15u);
+    const int j = k < 3u ? 0 : (int)((k - 3u) % 15u) + 1;
+    return R.c[ch * ShRec<DEG>::NB + j];
+  }
+};
+
+template <bool F16, int DEG>
 __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
   const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
   if (idx >= A.max_gaussians) return;
@@ -253,17 +284,18 @@ __global__ __launch_bounds__(256) void k_project(CgsProjectArgs A) {
   float cov[6];
 #pragma unroll
   for (uint32_t k = 0; k < 6; ++k) cov[k] = ld_cov<F16>(S, k, idx);
+  uint32_t sh_idx, tet_raw;
+  sh_source(A, idx, sh_idx, tet_raw);
+  ShRec<DEG> R;
   float4 g0, g1;
   const uint32_t ext =
       project_geometry(A.camera, px, py, pz, op, cov, A.mip, A.min_opacity, A.min_radius, g0, g1);
   if (ext != 0u) {
-    uint32_t sh_idx, tet_raw;
-    sh_source(A, idx, sh_idx, tet_raw);
+    load_rec<F16, DEG>(S, sh_idx, R);
     float dx, dy, dz;
     sh_direction(A, px, py, pz, A.cam_pos, tet_raw, dx, dy, dz);
     float cr, cg, cbl;
-    eval_sh([&](uint32_t k) { return ld_sh<F16>(S, k, sh_idx); }, dx, dy, dz, A.sh_degree, cr,
-            cg, cbl);
+    eval_sh(ShView<DEG>{R}, dx, dy, dz, (uint32_t)DEG, cr, cg, cbl);
     float4* g = reinterpret_cast<float4*>(A.geom + (size_t)idx * 8);
     g[0] = g0;
     g[1] = g1;
@@ -350,9 +382,19 @@ void cgs_launch_project_views(const CgsProjectArgs& a, const CgsViewProjection* 
 
 void cgs_launch_project(const CgsProjectArgs& a, hipStream_t s) {
   if (a.max_gaussians == 0) return;
-  const uint32_t blocks = (a.max_gaussians + 255u) / 256u;
-  if (a.scene.sh_h)
-    hipLaunchKernelGGL(k_project<true>, dim3(blocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_project<false>, dim3(blocks), dim3(256), 0, s, a);
+  const dim3 g((a.max_gaussians + 255u) / 256u), b(256);
+  const uint32_t d = a.sh_degree > 3u ? 3u : a.sh_degree;
+#define KP(F, D) hipLaunchKernelGGL((k_project<F, D>), g, b, 0, s, a)
+  if (a.scene.sh_h) {
+    if (d == 0) KP(true, 0);
+    else if (d == 1) KP(true, 1);
+    else if (d == 2) KP(true, 2);
+    else KP(true, 3);
+  } else {
+    if (d == 0) KP(false, 0);
+    else if (d == 1) KP(false, 1);
+    else if (d == 2) KP(false, 2);
+    else KP(false, 3);
+  }
+#undef KP
 }
