@@ -276,6 +276,18 @@ __device__ __forceinline__ void hist_key(uint32_t* s_hist, uint32_t key, uint32_
   for (uint32_t p = 0; p * db < bits; ++p)
     atomicAdd(&s_hist[(p << db) + ((key >> (p * db)) & dmask(bits, p * db, db))], 1u);
 }
+// The digits of the passes p >= first_p only (one entry), and of the passes
+// below first_p, made only of a splat's own bits, once with its entry count.
+__device__ __forceinline__ void hist_key_from(uint32_t* s_hist, uint32_t key, uint32_t bits,
+                                              uint32_t db, uint32_t first_p) {
+  for (uint32_t p = first_p; p * db < bits; ++p)
+    atomicAdd(&s_hist[(p << db) + ((key >> (p * db)) & dmask(bits, p * db, db))], 1u);
+}
+__device__ __forceinline__ void hist_const(uint32_t* s_hist, uint32_t key, uint32_t bits,
+                                           uint32_t db, uint32_t passes, uint32_t count) {
+  for (uint32_t p = 0; p < passes; ++p)
+    atomicAdd(&s_hist[(p << db) + ((key >> (p * db)) & dmask(bits, p * db, db))], count);
+}
 // Histograms of the wave walk's stored entries.  The lanes of a chunk hold
 // tiles of one or a few splats, so the digits made only of a splat's depth
 // bits (the low `const_bits` of the key) are shared by a whole run of lanes:
@@ -363,6 +375,9 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     uint32_t cur = off;
     const uint32_t kb = cgs_f2u(b.d1.w);
     const uint32_t mask = A.small_masks ? (uint32_t)A.small_masks[s] : 0u;
+    // passes whose digits hold depth bits only (the splat's own): counted once
+    // below with the splat's stored entries; the others per entry
+    const uint32_t cpass = A.mode64 ? 4u : min(A.depth_bits, hbits) / db;
     uint32_t k = 0u;
     for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
       for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
@@ -372,16 +387,21 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
           if (cur < A.cap) {
             store_entry(A, cur, tile, kb, s);
             if (hist) {
-              if (!A.mode64) {
-                hist_key(s_hist, (tile << A.depth_bits) | kb, hbits, db);
-              } else {
-                hist_key(s_hist, kb, 32u, 8u);
+              if (!A.mode64)
+                hist_key_from(s_hist, (tile << A.depth_bits) | kb, hbits, db, cpass);
+              else
                 hist_key(s_hist + 1024, tile, A.tile_bits, 8u);
-              }
             }
           }
           ++cur;
         }
+    const uint32_t stored = min(cur, A.cap) - min(off, A.cap);
+    if (hist && stored != 0u) {
+      if (!A.mode64)
+        hist_const(s_hist, kb, hbits, db, cpass, stored);
+      else
+        hist_const(s_hist, kb, 32u, 8u, 4u, stored);
+    }
   }
   // big boxes: the wave walks them together; a hit goes to its splat's next
   // slot plus the hits of the same splat on the lanes before it
