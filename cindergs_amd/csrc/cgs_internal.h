@@ -246,6 +246,10 @@ struct CgsBinArgs {
   uint8_t* small_masks;      // [max] per splat with a box of <= 8 tiles: bit k = tile k hit
   uint64_t* chunk_masks;     // [num_parts * 16 * 32] per wave: the hit masks of its first
                              // 32 wave-walk chunks
+  // words k_count clears for the kernels after it (sort tickets and look-back
+  // status, tile ranges), so that no fill launch precedes it: three spans
+  uint32_t* zero_at[3];
+  uint32_t zero_len[3];
 };
 
 // Optional per-launch timing: mark(name) records an event before a launch;

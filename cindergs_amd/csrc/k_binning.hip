@@ -224,6 +224,11 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
   WaveBig& W = s_w[wave];
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
   const uint32_t s = blockIdx.x * (uint32_t)BP + t;
+  // the spans the later kernels of the frame need cleared (none of them is
+  // read or written by this kernel)
+  for (int z = 0; z < 3; ++z)
+    for (uint32_t i = blockIdx.x * (uint32_t)BT + t; i < A.zero_len[z]; i += gridDim.x * (uint32_t)BT)
+      A.zero_at[z][i] = 0u;
   const BinSplat b = load_bin_splat(A, s, active);
   // small boxes: the thread walks them; bit k of the splat's mask = box tile
   // k (row-major) is hit

@@ -355,8 +355,16 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   const FrameLayout L = frame_layout(max_g, cap);
   TRY(ensure_words(a.ctrl3, L.words));
   uint32_t* ctrl = P<uint32_t>(a.ctrl3);
-  cgs_launch_fill_u32(ctrl, 0u, L.zero_words, s);
-  cgs_launch_fill_u32(P<uint32_t>(a.ranges), 0u, (uint64_t)g.T * 2, s);
+  // to clear: the frame's scalars, sort tickets and digit histograms (up to
+  // the per-block sums, which k_count writes whole), the sort's look-back
+  // status (the second set only for 64-bit keys) and the tile ranges.  k_count
+  // clears them; without splats there is no k_count and they are filled here.
+  uint32_t* zero_at[3] = {ctrl, ctrl + L.rA_status, P<uint32_t>(a.ranges)};
+  const uint32_t zero_len[3] = {(uint32_t)L.block_sums,
+                                (uint32_t)((a.mode64 ? 2u : 1u) * (L.rB_status - L.rA_status)),
+                                g.T * 2u};
+  if (cgs_bin_num_parts(max_g) == 0)
+    for (int z = 0; z < 3; ++z) cgs_launch_fill_u32(zero_at[z], 0u, zero_len[z], s);
   TRY(ensure_words(a.cnt_dbg, max_g));  // per-splat tile counts (always needed)
   TRY(ensure_words(a.hitm, cgs_bin_mask_words(max_g)));
   a.captured = false;
@@ -373,6 +381,10 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
 
   CgsBinArgs B;
   memset(&B, 0, sizeof(B));
+  for (int z = 0; z < 3; ++z) {
+    B.zero_at[z] = zero_at[z];
+    B.zero_len[z] = zero_len[z];
+  }
   B.geom = a.geom_in ? a.geom_in : P<float>(a.geom);
   B.extents = a.extents_in ? a.extents_in : P<uint32_t>(a.extents);
   B.split_count = split_count;
