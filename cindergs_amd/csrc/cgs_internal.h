@@ -247,7 +247,7 @@ struct CgsBinArgs {
   uint32_t digit_bits;       // 32-bit mode: digit width of the sort (8 or 4); 64-bit: 8
   // hit results k_count leaves for k_emit (both null: k_emit re-tests):
   uint8_t* small_masks;      // [max] per splat with a box of <= 8 tiles: bit k = tile k hit
-  uint64_t* chunk_masks;     // [num_parts * 16 * 32] per wave: the hit masks of its first
+  uint64_t* chunk_masks;     // [num_parts * 8 * 32] per wave: the hit masks of its first
                              // 32 wave-walk chunks
   // words k_count clears for the kernels after it (sort tickets and look-back
   // status, tile ranges), so that no fill launch precedes it: three spans
@@ -316,8 +316,10 @@ uint32_t cgs_radix_num_parts(uint32_t cap);
 uint32_t cgs_radix_hist_blocks();
 size_t cgs_radix_control_words(uint32_t cap);
 // hist[b] = sum_r partial[r*row_words + b] for b < bins
+// hist[0..bins) = column sums of partial[rows][row_words]; zeroed: hist is
+// already zero (else it is cleared first)
 void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row_words,
-                            uint32_t bins, uint32_t* hist, hipStream_t s);
+                            uint32_t bins, uint32_t* hist, hipStream_t s, bool zeroed);
 // Returns 1 if the sorted result is in the *_alt buffers, else 0.  With
 // hist_given the caller has already filled b.hist for the passes (pass p's
 // 2^digit_bits bins at b.hist + p * 2^digit_bits).  part_keys: 16384 (one

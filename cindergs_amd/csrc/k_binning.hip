@@ -21,7 +21,9 @@
 
 namespace {
 
-constexpr int BT = 1024;      // threads per block (16 wave64s), one splat per thread
+// threads per block (8 wave64s), one splat per thread: 512 measured faster
+// than 1024 (k_emit -27 % at 1080p, -31 % on 65M-entry 4K views) and than 256
+constexpr int BT = 512;
 constexpr int BP = BT;        // splats per block
 constexpr uint32_t SMALL_BOX = 8;  // boxes of at most this many tiles: walked by their thread
 constexpr uint32_t KCHUNK = 32;    // hit masks kept per wave for the wave walk

@@ -67,33 +67,28 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
 
 // hist[b] = sum over rows of partial[row][b], b < bins: each block owns 16 bins,
 // its 16 row-chunks are summed in parallel and combined in LDS.
+// hist[b] += sum over the rows of a slice of partial[row][b]: block (x, y)
+// sums bins [64x, 64x + 64) over the y-th slice of the rows, four rows at a
+// time (one wave per row, lanes over bins: 256-byte row reads), and adds its
+// totals to hist, which must start at zero.
 __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* partial, uint32_t rows,
                                                      uint32_t row_words, uint32_t bins,
                                                      uint32_t* hist) {
-  __shared__ uint32_t s_sum[16][17];
-  const uint32_t t = threadIdx.x;
-  const uint32_t bin = blockIdx.x * 16u + (t & 15u);
-  const uint32_t chunk = t >> 4;
+  __shared__ uint32_t s_sum[4][64];
+  const uint32_t t = threadIdx.x, lane = t & 63u, grp = t >> 6;
+  const uint32_t bin = blockIdx.x * 64u + lane;
+  const uint32_t per = (rows + gridDim.y - 1u) / gridDim.y;
+  const uint32_t r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
   uint32_t acc = 0;
   if (bin < bins) {
-    uint32_t r = chunk;
-    // 8 independent loads in flight per iteration
-    for (; r + 7u * 16u < rows; r += 8u * 16u) {
-      uint32_t v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(r + u * 16u) * row_words + bin];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += v[u];
-    }
-    for (; r < rows; r += 16u) acc += partial[(size_t)r * row_words + bin];
+#pragma unroll 4
+    for (uint32_t r = r0 + grp; r < r1; r += 4u) acc += partial[(size_t)r * row_words + bin];
   }
-  s_sum[chunk][t & 15u] = acc;
+  s_sum[grp][lane] = acc;
   __syncthreads();
-  if (t < 16u && blockIdx.x * 16u + t < bins) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) v += s_sum[c][t];
-    hist[blockIdx.x * 16u + t] = v;
+  if (t < 64u && bin < bins) {
+    const uint32_t v = s_sum[0][t] + s_sum[1][t] + s_sum[2][t] + s_sum[3][t];
+    if (v != 0u) atomicAdd(&hist[bin], v);
   }
 }
 
@@ -250,10 +245,15 @@ size_t cgs_radix_control_words(uint32_t cap) {
 }
 
 void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row_words,
-                            uint32_t bins, uint32_t* hist, hipStream_t s) {
-  if (rows == 0 || bins == 0) return;
-  hipLaunchKernelGGL(k_hist_reduce, dim3((bins + 15u) / 16u), dim3(256), 0, s, partial, rows,
-                     row_words, bins, hist);
+                            uint32_t bins, uint32_t* hist, hipStream_t s, bool zeroed) {
+  if (bins == 0) return;
+  if (!zeroed) cgs_launch_fill_u32(hist, 0u, bins, s);
+  if (rows == 0) return;
+  // slices of >= 32 rows, at most 64 of them
+  uint32_t slices = rows / 32u;
+  slices = slices < 1u ? 1u : slices > 64u ? 64u : slices;
+  hipLaunchKernelGGL(k_hist_reduce, dim3((bins + 63u) / 64u, slices), dim3(256), 0, s, partial,
+                     rows, row_words, bins, hist);
 }
 
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
@@ -276,7 +276,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
                        b.cap, bits, db, b.hist_partial);
     if (prof) prof->mark("k_hist_reduce", s);
     cgs_launch_hist_reduce(b.hist_partial, HIST_BLOCKS, MAX_PASSES * RADIX, passes * radix,
-                           b.hist, s);
+                           b.hist, s, false);
   }
   const uint32_t* kin = b.keys;
   const uint32_t* vin = iota_vals ? nullptr : b.vals;

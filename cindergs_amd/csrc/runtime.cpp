@@ -422,7 +422,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
                                       : (1u << db) * ((sort_bits + db - 1u) / db);
   prof->mark("k_hist_reduce", s);
   cgs_launch_hist_reduce(ctrl + L.bin_hist, L.bin_parts, B.hist_words, hist_bins, ctrl + L.histA,
-                         s);
+                         s, true);  // cleared with the frame's control words
 
   const uint32_t* d_n = ctrl + L.total;
   if (!a.mode64) {
