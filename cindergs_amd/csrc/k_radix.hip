@@ -16,7 +16,7 @@
 
 namespace {
 
-constexpr int RI = 16;           // keys per thread
+constexpr int RI = 16;           // keys per thread (12 for the 12288-key partitions)
 constexpr int RP_MIN = 256 * RI; // smallest partition (status arrays are sized for it)
 constexpr int RADIX = 256;       // digits of the widest digit (8 bits)
 constexpr int MAX_PASSES = 4;    // passes of 8-bit digits over 32 bits
@@ -107,10 +107,10 @@ struct PassArgs {
   uint32_t* ticket;
 };
 
-// One pass over partitions of RT*RI keys with DB-bit digits.
-template <int RT, int DB>
+// One pass over partitions of RT*KI keys with DB-bit digits (KI keys per thread).
+template <int RT, int DB, int KI = RI>
 __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
-  constexpr int RP = RT * RI;
+  constexpr int RP = RT * KI;
   constexpr int RW = RT / 64;
   constexpr int NB = 1 << DB;  // digits
   __shared__ uint32_t s_keys[RP];
@@ -140,15 +140,15 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   // offs: each key's rank among the wave's keys of its digit (< 1024, or
   // 0xFFFF for no key), two per register, so that keys, values and ranks of
   // the 1024-thread shape fit its 128 VGPRs without spilling
-  uint32_t keys[RI], vals[RI], offs[RI / 2];
-  const uint32_t wbase = base + wave * (uint32_t)(RI * 64);
+  uint32_t keys[KI], vals[KI], offs[KI / 2];
+  const uint32_t wbase = base + wave * (uint32_t)(KI * 64);
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
+  for (int i = 0; i < KI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     keys[i] = idx < n ? A.keys_in[idx] : 0u;
   }
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
+  for (int i = 0; i < KI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     const bool valid = idx < n;
     const uint32_t d = (keys[i] >> A.shift) & dmask;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   // the values are loaded once the keys are ranked: their latency overlaps the
   // digit scans and the look-back, and they are not live during the ranking
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
+  for (int i = 0; i < KI; ++i) {
     const uint32_t idx = wbase + (uint32_t)i * 64u + lane;
     vals[i] = A.vals_in ? (idx < n ? A.vals_in[idx] : 0u) : idx;
   }
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
 
   // stage keys and values in LDS in digit order
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
+  for (int i = 0; i < KI; ++i) {
     const uint32_t rank = (offs[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
     if (rank != 0xFFFFu) {
       const uint32_t d = (keys[i] >> A.shift) & dmask;
@@ -268,6 +268,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
   const uint32_t pk = part_keys == 4096u    ? 4096u
                       : part_keys == 8192u  ? 8192u
                       : part_keys == 10240u ? 10240u
+                      : part_keys == 12288u ? 12288u
                                             : 16384u;
   const uint32_t grid = (b.cap + pk - 1u) / pk;
   if (!hist_given) {
@@ -299,6 +300,8 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
     if (db == 8u) {
       if (pk == 16384u)
         hipLaunchKernelGGL((k_radix_onesweep<1024, 8>), dim3(grid), dim3(1024), 0, s, A);
+      else if (pk == 12288u)
+        hipLaunchKernelGGL((k_radix_onesweep<1024, 8, 12>), dim3(grid), dim3(1024), 0, s, A);
       else if (pk == 10240u)
         hipLaunchKernelGGL((k_radix_onesweep<640, 8>), dim3(grid), dim3(640), 0, s, A);
       else if (pk == 8192u)
@@ -308,6 +311,8 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
     } else {
       if (pk == 16384u)
         hipLaunchKernelGGL((k_radix_onesweep<1024, 4>), dim3(grid), dim3(1024), 0, s, A);
+      else if (pk == 12288u)
+        hipLaunchKernelGGL((k_radix_onesweep<1024, 4, 12>), dim3(grid), dim3(1024), 0, s, A);
       else if (pk == 10240u)
         hipLaunchKernelGGL((k_radix_onesweep<640, 4>), dim3(grid), dim3(640), 0, s, A);
       else if (pk == 8192u)

@@ -695,9 +695,11 @@ int stage2(const cgs_context* c, const SceneFrame& F, FrameArena& a, hipStream_t
 }
 
 int check_tuning(const cgs_tuning* t) {
-  if (t->radix_partition_keys != 16384u && t->radix_partition_keys != 10240u &&
-      t->radix_partition_keys != 8192u && t->radix_partition_keys != 4096u)
-    return fail(CGS_ERR_INVALID_ARG, "radix_partition_keys must be 16384, 10240, 8192 or 4096");
+  if (t->radix_partition_keys != 16384u && t->radix_partition_keys != 12288u &&
+      t->radix_partition_keys != 10240u && t->radix_partition_keys != 8192u &&
+      t->radix_partition_keys != 4096u)
+    return fail(CGS_ERR_INVALID_ARG,
+                "radix_partition_keys must be 16384, 12288, 10240, 8192 or 4096");
   if (t->radix_digit_bits != 8u && t->radix_digit_bits != 4u)
     return fail(CGS_ERR_INVALID_ARG, "radix_digit_bits must be 8 or 4");
   if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u)
@@ -1098,16 +1100,17 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 
 // ---- launch tuning ----
 
-uint32_t cgs_tuning_candidates(void) { return 48u; }
+uint32_t cgs_tuning_candidates(void) { return 60u; }
 
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
-  if (!out || i >= 48u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
-  static const uint32_t parts[4] = {16384u, 10240u, 8192u, 4096u}, digits[2] = {8u, 4u},
-                        batches[3] = {256u, 128u, 64u}, orders[2] = {1u, 0u};
-  out->radix_partition_keys = parts[i % 4u];
-  out->radix_digit_bits = digits[(i / 4u) % 2u];
-  out->composite_batch = batches[(i / 8u) % 3u];
-  out->tile_order = orders[i / 24u];
+  if (!out || i >= 60u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  static const uint32_t parts[5] = {16384u, 12288u, 10240u, 8192u, 4096u},
+                        digits[2] = {8u, 4u}, batches[3] = {256u, 128u, 64u},
+                        orders[2] = {1u, 0u};
+  out->radix_partition_keys = parts[i % 5u];
+  out->radix_digit_bits = digits[(i / 5u) % 2u];
+  out->composite_batch = batches[(i / 10u) % 3u];
+  out->tile_order = orders[i / 30u];
   return CGS_OK;
 }
 

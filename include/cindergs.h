@@ -205,9 +205,9 @@ int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, ui
  * per-context settings that cgs_autotune picks by timing.  Defaults:
  * {16384, 8, 256, 1}. */
 typedef struct cgs_tuning {
-  uint32_t radix_partition_keys; /* 16384, 10240, 8192 or 4096 keys per sort
-                                    partition (1024, 640, 512 or 256 threads x
-                                    16 keys) */
+  uint32_t radix_partition_keys; /* 16384, 12288, 10240, 8192 or 4096 keys per
+                                    sort partition (1024 threads x 16 or 12
+                                    keys; 640, 512 or 256 threads x 16) */
   uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
   uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step */
   uint32_t tile_order;           /* 1: heaviest tiles composited first */

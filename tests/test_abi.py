@@ -65,8 +65,8 @@ def test_tuning_candidates_cover_every_knob():
 
     n = tuning_candidates_count()
     combos = {tuple(tuning_candidate(i).as_dict().values()) for i in range(n)}
-    assert n == len(combos) == 4 * 2 * 3 * 2
-    assert {c[0] for c in combos} == {16384, 10240, 8192, 4096}
+    assert n == len(combos) == 5 * 2 * 3 * 2
+    assert {c[0] for c in combos} == {16384, 12288, 10240, 8192, 4096}
     assert {c[1] for c in combos} == {8, 4}
     assert {c[2] for c in combos} == {256, 128, 64}
     assert {c[3] for c in combos} == {0, 1}

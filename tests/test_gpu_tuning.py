@@ -53,7 +53,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
         print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
         gpu.set_tuning(c)
         _same(ref, _frame(gpu, cam, comp, scene.count), f"candidate {i}")
-    assert len(seen) == n == 48
+    assert len(seen) == n == 60
     gpu.set_tuning(default)
 
 
@@ -64,7 +64,7 @@ def test_partition_sizes_on_64bit_keys(gpu):
     comp = syn.compositor(7680, 4320, (0.0, 0.0, -3.0))
     default = gpu.get_tuning()
     ref = _frame(gpu, cam, comp, scene.count)
-    for pk, batch in ((8192, 128), (10240, 256), (16384, 64), (4096, 256)):
+    for pk, batch in ((8192, 128), (10240, 256), (16384, 64), (4096, 256), (12288, 128)):
         t = Tuning(pk, 8, batch, 0)
         print(f"[case] 8K partition {pk} batch {batch}", flush=True)
         gpu.set_tuning(t)
@@ -91,7 +91,7 @@ def test_autotune_keeps_a_candidate_and_the_frame(gpu):
 def test_invalid_tuning_is_rejected(gpu):
     from cindergs_amd._lib import GpuError
 
-    for bad in (Tuning(2048, 8, 256, 1), Tuning(12288, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
+    for bad in (Tuning(2048, 8, 256, 1), Tuning(12000, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
                 Tuning(16384, 8, 256, 2)):
         with pytest.raises(GpuError):
             gpu.set_tuning(bad)
