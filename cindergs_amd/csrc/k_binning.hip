@@ -464,25 +464,42 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint6
 }
 
 // identify_tile_ranges_kernel (binning.rs:344-379); shift = depth_bits for
-// packed keys, 0 for sorted tile ids (383-417).  Ranges must be zeroed.
+// packed keys, 0 for sorted tile ids (383-417).  Ranges must be zeroed.  Each
+// thread takes four consecutive entries (one 16-byte load; `sorted` is
+// 16-byte aligned) and the entry before them.
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* sorted, const uint32_t* d_n,
                                                      uint32_t cap, uint32_t shift,
                                                      uint32_t total_tiles, uint32_t* ranges) {
   const uint32_t n = min(*d_n, cap);
-  const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
-  if (idx >= n) return;
-  const uint32_t tile = sorted[idx] >> shift;
-  if (tile >= total_tiles) return;
-  if (idx == 0u) {
-    ranges[tile * 2u] = 0u;
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t i0 = q * 4u;
+  if (i0 >= n) return;
+  uint32_t k[4];
+  if (i0 + 4u <= n) {
+    const uint4 v = reinterpret_cast<const uint4*>(sorted)[q];
+    k[0] = v.x;
+    k[1] = v.y;
+    k[2] = v.z;
+    k[3] = v.w;
   } else {
-    const uint32_t prev = sorted[idx - 1u] >> shift;
-    if (tile != prev) {
-      if (prev < total_tiles) ranges[prev * 2u + 1u] = idx;
-      ranges[tile * 2u] = idx;
-    }
+    for (uint32_t j = 0; j < 4u; ++j) k[j] = i0 + j < n ? sorted[i0 + j] : 0u;
   }
-  if (idx + 1u == n) ranges[tile * 2u + 1u] = n;
+  uint32_t prev = i0 != 0u ? sorted[i0 - 1u] >> shift : 0u;
+  for (uint32_t j = 0; j < 4u; ++j) {
+    const uint32_t idx = i0 + j;
+    if (idx >= n) break;
+    const uint32_t tile = k[j] >> shift;
+    if (tile < total_tiles) {
+      if (idx == 0u) {
+        ranges[tile * 2u] = 0u;
+      } else if (tile != prev) {
+        if (prev < total_tiles) ranges[prev * 2u + 1u] = idx;
+        ranges[tile * 2u] = idx;
+      }
+      if (idx + 1u == n) ranges[tile * 2u + 1u] = n;
+    }
+    prev = tile;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_gather2(const uint32_t* perm, const uint32_t* a_in,
@@ -536,8 +553,9 @@ void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_
                             uint32_t shift, uint32_t total_tiles, uint32_t* ranges,
                             hipStream_t s) {
   if (cap == 0) return;
-  hipLaunchKernelGGL(k_tile_ranges, dim3((cap + 255u) / 256u), dim3(256), 0, s, sorted, d_n, cap,
-                     shift, total_tiles, ranges);
+  const uint32_t quads = (cap + 3u) / 4u;
+  hipLaunchKernelGGL(k_tile_ranges, dim3((quads + 255u) / 256u), dim3(256), 0, s, sorted, d_n,
+                     cap, shift, total_tiles, ranges);
 }
 
 void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
