@@ -332,6 +332,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
                           CgsProf* prof = nullptr, uint32_t part_keys = 16384,
                           uint32_t digit_bits = 8);
+// a_out[i] = a_in[perm[i]], b_out likewise; perm, a_out, b_out 16-byte aligned
 void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
                         hipStream_t s);

@@ -506,12 +506,26 @@ __global__ __launch_bounds__(256) void k_gather2(const uint32_t* perm, const uin
                                                  const uint32_t* b_in, uint32_t* a_out,
                                                  uint32_t* b_out, const uint32_t* d_n,
                                                  uint32_t cap) {
+  // four entries per thread: one 16-byte load of the permutation, eight
+  // independent gathers in flight, two 16-byte stores (perm, a_out and b_out
+  // 16-byte aligned)
   const uint32_t n = min(*d_n, cap);
-  const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
-  if (idx >= n) return;
-  const uint32_t p = perm[idx];
-  a_out[idx] = a_in[p];
-  b_out[idx] = b_in[p];
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t i0 = q * 4u;
+  if (i0 >= n) return;
+  if (i0 + 4u <= n) {
+    const uint4 p = reinterpret_cast<const uint4*>(perm)[q];
+    const uint4 a = make_uint4(a_in[p.x], a_in[p.y], a_in[p.z], a_in[p.w]);
+    const uint4 b = make_uint4(b_in[p.x], b_in[p.y], b_in[p.z], b_in[p.w]);
+    reinterpret_cast<uint4*>(a_out)[q] = a;
+    reinterpret_cast<uint4*>(b_out)[q] = b;
+  } else {
+    for (uint32_t idx = i0; idx < n; ++idx) {
+      const uint32_t p = perm[idx];
+      a_out[idx] = a_in[p];
+      b_out[idx] = b_in[p];
+    }
+  }
 }
 
 }  // namespace
@@ -562,6 +576,7 @@ void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
                         hipStream_t s) {
   if (cap == 0) return;
-  hipLaunchKernelGGL(k_gather2, dim3((cap + 255u) / 256u), dim3(256), 0, s, perm, a_in, b_in,
+  const uint32_t quads = (cap + 3u) / 4u;
+  hipLaunchKernelGGL(k_gather2, dim3((quads + 255u) / 256u), dim3(256), 0, s, perm, a_in, b_in,
                      a_out, b_out, d_n, cap);
 }

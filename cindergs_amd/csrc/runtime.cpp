@@ -1510,10 +1510,13 @@ int cgs_radix_sort_u64_pairs(cgs_context* c, uint32_t* tiles, uint32_t* depths, 
   if (!tiles || !depths || !ids) return fail(CGS_ERR_INVALID_ARG, "null buffer");
   const uint32_t m = (uint32_t)n;
   // 1) stable sort by depth with the entry index as payload (4 passes: result in place)
-  TRY(ensure_words(c->prim_alt_d, (size_t)m * 3));
+  // three arrays of m words, each starting 16-byte aligned (the gather and
+  // the sort use 16-byte accesses)
+  const size_t mp = ((size_t)m + 3u) & ~(size_t)3u;
+  TRY(ensure_words(c->prim_alt_d, mp * 3));
   uint32_t* perm = P<uint32_t>(c->prim_alt_d);
-  uint32_t* t2 = perm + m;
-  uint32_t* i2 = perm + 2 * (size_t)m;
+  uint32_t* t2 = perm + mp;
+  uint32_t* i2 = perm + 2 * mp;
   int alt = 0;
   TRY(radix_prim(c, depths, perm, m, 32, true, &alt));
   if (alt) {

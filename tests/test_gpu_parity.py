@@ -98,9 +98,10 @@ def test_radix_sort_stability_full_32_bits(gpu):
         dv.free()
 
 
-def test_radix_sort_u64_fallback_reference_vector(gpu):
-    """gpu_parity_tests.rs:613-661: n=4096, 19 tiles, 5 tile bits."""
-    n = 4096
+@pytest.mark.parametrize("n", [4096, 4093, 1003])
+def test_radix_sort_u64_fallback_reference_vector(gpu, n):
+    """gpu_parity_tests.rs:613-661: n=4096, 19 tiles, 5 tile bits (and the same
+    formula at sizes that are not a multiple of 4: the 16-byte gather's tail)."""
     i = np.arange(n)
     tiles = ((i * 73 + 5) % 19).astype(np.uint32)
     depths = (np.float32(0.5) + ((i * 19 + 3) % 100).astype(np.float32) * np.float32(0.1)).view(np.uint32)
