@@ -547,8 +547,11 @@ struct CgsTetArgs {
   float* jac;             // [9*T]
   float* rot;             // [9*T]
   float* stretch;         // [T]
+  float* pack;            // [CGS_TET_PACK*T] the per-tet record k_deform reads
   uint32_t num_tets;
 };
+// per tet: corners v0..v3 (xyz), J (9, column-major), stretch, 2 pad floats
+#define CGS_TET_PACK 24u
 void cgs_launch_tet_transforms(const CgsTetArgs& a, hipStream_t s);
 
 struct CgsDeformArgs {
@@ -556,8 +559,9 @@ struct CgsDeformArgs {
   const uint32_t* elems;
   const float* jac;
   const float* stretch;
+  const float* tet_pack;     // [CGS_TET_PACK*T] (see CgsTetArgs)
   const uint32_t* bind_tet;  // [n]
-  const float* bind_w;       // [4n]
+  const float* bind_w;       // [4n], 16-byte aligned
   const float* rest_geo;     // blocked K=10
   float* out_geo;            // blocked K=10, capacity 2n when splitting
   uint32_t* split_meta;      // [2n] or null

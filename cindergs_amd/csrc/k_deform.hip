@@ -23,9 +23,12 @@ __global__ __launch_bounds__(64) void k_tet_transforms(CgsTetArgs A) {
   const uint32_t v0 = e[0], v1 = e[1], v2 = e[2], v3 = e[3];
   const float* V = A.verts;
   const float x0x = V[v0 * 3], x0y = V[v0 * 3 + 1], x0z = V[v0 * 3 + 2];
-  const float ds00 = V[v1 * 3] - x0x, ds10 = V[v1 * 3 + 1] - x0y, ds20 = V[v1 * 3 + 2] - x0z;
-  const float ds01 = V[v2 * 3] - x0x, ds11 = V[v2 * 3 + 1] - x0y, ds21 = V[v2 * 3 + 2] - x0z;
-  const float ds02 = V[v3 * 3] - x0x, ds12 = V[v3 * 3 + 1] - x0y, ds22 = V[v3 * 3 + 2] - x0z;
+  const float x1x = V[v1 * 3], x1y = V[v1 * 3 + 1], x1z = V[v1 * 3 + 2];
+  const float x2x = V[v2 * 3], x2y = V[v2 * 3 + 1], x2z = V[v2 * 3 + 2];
+  const float x3x = V[v3 * 3], x3y = V[v3 * 3 + 1], x3z = V[v3 * 3 + 2];
+  const float ds00 = x1x - x0x, ds10 = x1y - x0y, ds20 = x1z - x0z;
+  const float ds01 = x2x - x0x, ds11 = x2y - x0y, ds21 = x2z - x0z;
+  const float ds02 = x3x - x0x, ds12 = x3y - x0y, ds22 = x3z - x0z;
   const float* M = A.inv_dm + (size_t)idx * 9;
   const float a00 = M[0], a10 = M[1], a20 = M[2], a01 = M[3], a11 = M[4], a21 = M[5];
   const float a02 = M[6], a12 = M[7], a22 = M[8];
@@ -43,6 +46,7 @@ __global__ __launch_bounds__(64) void k_tet_transforms(CgsTetArgs A) {
   J[6] = j02; J[7] = j12; J[8] = j22;
 
   // largest stretch from C = J^T J (trigonometric eigenvalue formula)
+  float stretch_v = 0.0f;
   const float c00 = j00 * j00 + j10 * j10 + j20 * j20;
   const float c01 = j00 * j01 + j10 * j11 + j20 * j21;
   const float c02 = j00 * j02 + j10 * j12 + j20 * j22;
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(64) void k_tet_transforms(CgsTetArgs A) {
   const float p1 = c01 * c01 + c02 * c02 + c12 * c12;
   if (p1 <= 1e-10f) {
     const float dmax = cgs_fmax(c00, cgs_fmax(c11, c22));
-    A.stretch[idx] = cgs_sqrtf(cgs_fmax(dmax, 0.0f));
+    stretch_v = cgs_sqrtf(cgs_fmax(dmax, 0.0f));
   } else {
     const float q = (c00 + c11 + c22) / 3.0f;
     const float d0 = c00 - q, d1 = c11 - q, d2 = c22 - q;
@@ -66,7 +70,7 @@ __global__ __launch_bounds__(64) void k_tet_transforms(CgsTetArgs A) {
     const float r = cgs_fclamp(0.5f * det_b, -1.0f, 1.0f);
     const float phi = cgs_acosf(r) / 3.0f;
     const float emax = q + 2.0f * p * cgs_cosf(phi);
-    A.stretch[idx] = cgs_sqrtf(cgs_fmax(emax, 0.0f));
+    stretch_v = cgs_sqrtf(cgs_fmax(emax, 0.0f));
   }
 
   // Newton-Schulz (Higham) polar rotation, 5 steps, chirality fix
@@ -117,6 +121,16 @@ __global__ __launch_bounds__(64) void k_tet_transforms(CgsTetArgs A) {
   }
   R[0] = r00; R[1] = r10; R[2] = r20; R[3] = r01; R[4] = r11; R[5] = r21;
   R[6] = r02; R[7] = r12; R[8] = r22;
+  A.stretch[idx] = stretch_v;
+  // what k_deform reads of the tet, in six 16-byte words: the four deformed
+  // corners, J (column-major) and the stretch
+  float4* P4 = reinterpret_cast<float4*>(A.pack + (size_t)idx * CGS_TET_PACK);
+  P4[0] = make_float4(x0x, x0y, x0z, x1x);
+  P4[1] = make_float4(x1y, x1z, x2x, x2y);
+  P4[2] = make_float4(x2z, x3x, x3y, x3z);
+  P4[3] = make_float4(j00, j10, j20, j01);
+  P4[4] = make_float4(j11, j21, j02, j12);
+  P4[5] = make_float4(j22, stretch_v, 0.0f, 0.0f);
 }
 
 __device__ __forceinline__ float det3s(float xx, float xy, float xz, float yy, float yz,
@@ -175,17 +189,18 @@ __device__ __forceinline__ bool deform_splat(const CgsDeformArgs& A, uint32_t id
         store_splat(A, idx, x, y, z, s_xx, s_xy, s_xz, s_yy, s_yz, s_zz, alpha);
       }
     } else {
-      const float* W = A.bind_w + (size_t)idx * 4;
-      const float w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3];
-      const uint32_t* E = A.elems + (size_t)tet_raw * 4;
-      const float* V = A.verts;
-      const uint32_t v0 = E[0], v1 = E[1], v2 = E[2], v3 = E[3];
-      px = w0 * V[v0 * 3] + w1 * V[v1 * 3] + w2 * V[v2 * 3] + w3 * V[v3 * 3];
-      py = w0 * V[v0 * 3 + 1] + w1 * V[v1 * 3 + 1] + w2 * V[v2 * 3 + 1] + w3 * V[v3 * 3 + 1];
-      pz = w0 * V[v0 * 3 + 2] + w1 * V[v1 * 3 + 2] + w2 * V[v2 * 3 + 2] + w3 * V[v3 * 3 + 2];
-      const float* J = A.jac + (size_t)tet_raw * 9;
-      const float j00 = J[0], j10 = J[1], j20 = J[2], j01 = J[3], j11 = J[4], j21 = J[5];
-      const float j02 = J[6], j12 = J[7], j22 = J[8];
+      // the splat's weights and its tet's packed record (corners, J, stretch):
+      // seven 16-byte loads
+      const float4 W = *reinterpret_cast<const float4*>(A.bind_w + (size_t)idx * 4);
+      const float w0 = W.x, w1 = W.y, w2 = W.z, w3 = W.w;
+      const float4* T4 = reinterpret_cast<const float4*>(A.tet_pack + (size_t)tet_raw * CGS_TET_PACK);
+      const float4 q0 = T4[0], q1 = T4[1], q2 = T4[2], q3 = T4[3], q4 = T4[4], q5 = T4[5];
+      px = w0 * q0.x + w1 * q0.w + w2 * q1.z + w3 * q2.y;
+      py = w0 * q0.y + w1 * q1.x + w2 * q1.w + w3 * q2.z;
+      pz = w0 * q0.z + w1 * q1.y + w2 * q2.x + w3 * q2.w;
+      const float j00 = q3.x, j10 = q3.y, j20 = q3.z, j01 = q3.w, j11 = q4.x, j21 = q4.y;
+      const float j02 = q4.z, j12 = q4.w, j22 = q5.x;
+      const float tet_stretch = q5.y;
       const float k00 = j00 * s_xx + j01 * s_xy + j02 * s_xz;
       const float k01 = j00 * s_xy + j01 * s_yy + j02 * s_yz;
       const float k02 = j00 * s_xz + j01 * s_yz + j02 * s_zz;
@@ -210,7 +225,7 @@ __device__ __forceinline__ bool deform_splat(const CgsDeformArgs& A, uint32_t id
       const float safe_base = cgs_fclamp(1.0f - alpha, 1e-4f, 0.999f);
       const float alpha_p =
           cgs_fclamp(1.0f - cgs_powf_pos(safe_base, 1.0f / s_det), 0.05f, 0.99f);
-      split = A.split_threshold > 0.0f && A.stretch[tet_raw] > A.split_threshold;
+      split = A.split_threshold > 0.0f && tet_stretch > A.split_threshold;
       if (!split) {
         if (mip > 0.0f) {
           const float det_orig = cgs_fmax(det3s(m00, m01, m02, m11, m12, m22), 1e-24f);

@@ -186,7 +186,7 @@ struct cgs_context {
   uint64_t deform_generation = 0;
 
   // ---- arena ----
-  DevBuf jac, rot, stretch, def_geo, split_meta;
+  DevBuf jac, rot, stretch, tet_pack, def_geo, split_meta;
   uint32_t def_cap = 0;  // splat capacity of def_geo
   DevBuf ctrl1;  // [split_count, ticket, pad, pad][u64 status...]
   FrameArena fa;
@@ -569,6 +569,7 @@ int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* 
     TRY(ensure_words(c->jac, (size_t)c->num_tets * 9));
     TRY(ensure_words(c->rot, (size_t)c->num_tets * 9));
     TRY(ensure_words(c->stretch, c->num_tets));
+    TRY(ensure_words(c->tet_pack, (size_t)c->num_tets * CGS_TET_PACK));
     if (c->def_cap < F.max_g) {
       c->def_cap = F.max_g;
       c->deformed_valid = false;
@@ -586,6 +587,7 @@ int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* 
       TA.jac = P<float>(c->jac);
       TA.rot = P<float>(c->rot);
       TA.stretch = P<float>(c->stretch);
+      TA.pack = P<float>(c->tet_pack);
       TA.num_tets = c->num_tets;
       prof->mark("k_tet_transforms", s);
       cgs_launch_tet_transforms(TA, s);
@@ -594,6 +596,7 @@ int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* 
       D.elems = P<uint32_t>(c->elems);
       D.jac = P<float>(c->jac);
       D.stretch = P<float>(c->stretch);
+      D.tet_pack = P<float>(c->tet_pack);
       D.bind_tet = P<uint32_t>(c->bind_tet);
       D.bind_w = P<float>(c->bind_w);
       D.rest_geo = P<float>(c->rest_geo);
@@ -764,7 +767,7 @@ void cgs_destroy(cgs_context* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->rest_geo, &c->rest_cov_h, &c->rest_sh, &c->rest_sh_h, &c->staging,
                     &c->elems, &c->inv_dm, &c->bind_tet, &c->bind_w, &c->verts, &c->jac, &c->rot,
-                    &c->stretch, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
+                    &c->stretch, &c->tet_pack, &c->def_geo, &c->split_meta, &c->ctrl1, &c->prim_ctrl,
                     &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
                     &c->ingest, &c->tet_shm};
   for (DevBuf* b : bufs) release(*b);
