@@ -22,6 +22,8 @@ import sys
 import tempfile
 import time
 
+import math
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -221,7 +223,7 @@ def cage_frames(scene, frames: int = 300):
     t = time.perf_counter()
     tet, w = mesh.bind_gaussians_gpu(st, scene.positions)
     bind_ms = 1000 * (time.perf_counter() - t)
-    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping_c=4.0)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping=math.exp(-4.0 / 60.0))
     sim.impulse(0.2, 0.05)
     params = FrameParams(1.02, 0.0, 1.0 / 255.0, 0.5, 0, 0)
     tim = StageTimings()

@@ -5,8 +5,9 @@ Follows src/deformation/mod.rs (TetMesh::new 202-227: D_m = [X1-X0, X2-X0,
 X3-X0], inverse stored column-major, identity when |det| <= 1e-8;
 bind_point 267-310: the first tetrahedron in index order whose barycentric
 penalty sum(max(-w_k, 0)) is <= 1e-4, else the one with the smallest penalty)
-and src/deformation/dynamics.rs (CageSpringSimulator::step 249-277:
-semi-implicit Euler, a = k (x_rest - x) / m, v = (v + a dt) * exp(-c dt)).
+and src/deformation/dynamics.rs (CageSpringSimulator, 74-316: semi-implicit
+Euler, a = k (x_rest - x) / m, v = (v + a dt) * exp(-c dt), pins and
+collision planes).
 
 The grid cage is a jittered Freudenthal (Kuhn) subdivision: each cell of an
 nx x ny x nz grid over the scene's bounds is split into 6 tetrahedra sharing
@@ -206,39 +207,220 @@ class TetMesh:
                        np.ascontiguousarray(bind_w, dtype=np.float32))
 
 
+F32 = np.float32
+
+
+def _dot3(a, b):
+    """glam Vec3::dot: (x*x' + y*y') + z*z' in f32"""
+    return F32(F32(F32(a[0] * b[0]) + F32(a[1] * b[1])) + F32(a[2] * b[2]))
+
+
+class CollisionPlane:
+    """CollisionPlane (src/deformation/dynamics.rs:3-72): the half-space
+    n . x >= offset, with restitution and tangential friction on contact."""
+
+    def __init__(self, normal, offset: float, restitution: float, friction: float):
+        n = np.asarray(normal, F32).reshape(3)
+        ls = _dot3(n, n)
+        if ls > F32(1e-8):
+            n = (n * (F32(1.0) / np.sqrt(ls, dtype=F32))).astype(F32)  # Vec3::normalize
+        else:
+            n = np.array([0, 1, 0], F32)
+        self.normal = n
+        self.offset = F32(offset)
+        self.restitution = F32(min(max(restitution, 0.0), 1.0))
+        self.friction = F32(min(max(friction, 0.0), 1.0))
+
+    @classmethod
+    def ground_y(cls, y: float) -> "CollisionPlane":
+        return cls((0.0, 1.0, 0.0), y, 0.25, 0.2)
+
+    def signed_distance(self, point) -> np.float32:
+        return F32(_dot3(self.normal, np.asarray(point, F32)) - self.offset)
+
+    def clamp_point(self, point) -> np.ndarray:
+        p = np.asarray(point, F32)
+        d = self.signed_distance(p)
+        return (p - self.normal * d).astype(F32) if d < 0 else p
+
+    def resolve_contact(self, pos: np.ndarray, vel: np.ndarray) -> bool:
+        """resolve_contact (47-60) on one vertex, in place."""
+        d = self.signed_distance(pos)
+        if d >= 0:
+            return False
+        pos -= self.normal * d
+        vn = _dot3(vel, self.normal)
+        if vn < 0:
+            v_norm = (self.normal * vn).astype(F32)
+            v_tan = (vel - v_norm).astype(F32)
+            vel[:] = (v_tan * (F32(1.0) - self.friction) - v_norm * self.restitution).astype(F32)
+        return True
+
+
 class CageSpringSimulator:
-    """CageSpringSimulator::step (src/deformation/dynamics.rs:249-277), f32."""
+    """CageSpringSimulator (src/deformation/dynamics.rs:74-316), in f32.
 
-    MAX_DT = 0.033
+    Hookean springs towards the rest cage, semi-implicit Euler with the
+    frame-rate independent damping gamma = exp(-c dt), c = -ln(retention) * 60
+    (`damping` is the velocity retention per 1/60 s frame), dt clamped to
+    MAX_DT, transient (`pinned_index`) and persistent pins, and collision
+    planes.  Defaults are the reference's (stiffness 180, retention 0.92, mass
+    1).  The cage workload's spring (stiffness 40, c = 4 /s) is
+    `CageSpringSimulator(rest, stiffness=40.0, damping=math.exp(-4.0 / 60.0))`.
+    """
 
-    def __init__(self, rest: np.ndarray, stiffness: float = 40.0, damping_c: float = 4.0,
+    MAX_DT = F32(0.033)
+    DAMPING_REFERENCE_HZ = F32(60.0)
+
+    def __init__(self, rest: np.ndarray, stiffness: float = 180.0, damping: float = 0.92,
                  mass: float = 1.0):
-        self.rest = np.asarray(rest, dtype=np.float32).copy()
-        self.x = self.rest.copy()
-        self.v = np.zeros_like(self.rest)
-        self.k = np.float32(stiffness)
-        self.c = np.float32(damping_c)  # continuous damping coefficient (1/s)
-        self.mass = np.float32(mass)
+        self.rest_positions = np.ascontiguousarray(rest, dtype=F32).reshape(-1, 3).copy()
+        self.current_positions = self.rest_positions.copy()
+        self.velocities = np.zeros_like(self.rest_positions)
+        self.stiffness = F32(stiffness)
+        self.damping = F32(damping)
+        self.mass = F32(mass)
+        self.pinned_index = None
+        self.pinned_vertices = np.zeros(self.rest_positions.shape[0], bool)
+        self.collision_planes: list[CollisionPlane] = []
 
-    def impulse(self, magnitude: float = 0.2, fraction: float = 0.05, seed: int = 3):
-        """Outward radial velocity pulse on a seeded subset of the vertices."""
+    # -- the names the renderer's callers use ---------------------------------
+    @property
+    def x(self) -> np.ndarray:
+        """a copy of the current positions (step updates them in place)"""
+        return self.current_positions.copy()
+
+    @property
+    def rest(self) -> np.ndarray:
+        return self.rest_positions
+
+    @staticmethod
+    def damping_coefficient(damping: float) -> np.float32:
+        """-ln(clamp(retention, 1e-4, 0.9999)) * 60 (138-141)"""
+        r = F32(min(max(F32(damping), F32(1.0e-4)), F32(0.9999)))
+        return F32(-np.log(r, dtype=F32) * CageSpringSimulator.DAMPING_REFERENCE_HZ)
+
+    @staticmethod
+    def frame_damping(damping: float, dt: float) -> np.float32:
+        """exp(-c dt) (145-148)"""
+        c = CageSpringSimulator.damping_coefficient(damping)
+        return F32(np.exp(F32(-c * F32(dt)), dtype=F32))
+
+    def is_vertex_pinned(self, i: int) -> bool:
+        return self.pinned_index == i or bool(self.pinned_vertices[i])
+
+    def pin_vertex(self, i: int) -> None:
+        if i < self.pinned_vertices.size:
+            self.pinned_vertices[i] = True
+            self.velocities[i] = 0
+
+    def unpin_vertex(self, i: int) -> None:
+        if i < self.pinned_vertices.size:
+            self.pinned_vertices[i] = False
+
+    def toggle_pin_vertex(self, i: int) -> bool:
+        if i >= self.pinned_vertices.size:
+            return False
+        self.pinned_vertices[i] = not self.pinned_vertices[i]
+        if self.pinned_vertices[i]:
+            self.velocities[i] = 0
+        return bool(self.pinned_vertices[i])
+
+    def clear_pins(self) -> None:
+        self.pinned_vertices[:] = False
+
+    def pinned_count(self) -> int:
+        m = self.pinned_vertices.copy()
+        if self.pinned_index is not None and self.pinned_index < m.size:
+            m[self.pinned_index] = True
+        return int(m.sum())
+
+    def add_collision_plane(self, plane: CollisionPlane) -> None:
+        self.collision_planes.append(plane)
+
+    def clear_collision_planes(self) -> None:
+        self.collision_planes.clear()
+
+    def project_point_to_collision_planes(self, point) -> np.ndarray:
+        p = np.asarray(point, F32)
+        for pl in self.collision_planes:
+            p = pl.clamp_point(p)
+        return p
+
+    def set_pinned(self, pinned) -> None:
+        self.pinned_index = pinned
+        if pinned is not None and pinned < self.velocities.shape[0]:
+            self.velocities[pinned] = 0
+
+    def set_pinned_position(self, i: int, pos) -> None:
+        if i < self.current_positions.shape[0]:
+            self.current_positions[i] = self.project_point_to_collision_planes(pos)
+            self.velocities[i] = 0
+
+    def step(self, dt: float) -> np.ndarray:
+        """step (249-277): the per-vertex update of the reference, vectorised
+        over the vertices with the same f32 operations in the same order."""
+        dt_sim = F32(min(F32(dt), self.MAX_DT))
+        if dt_sim <= 0:
+            return self.current_positions
+        inv_mass = F32(F32(1.0) / max(self.mass, F32(0.001)))
+        k = self.stiffness
+        gamma = self.frame_damping(self.damping, dt_sim)
+        free = ~self.pinned_vertices.copy()
+        if self.pinned_index is not None and self.pinned_index < free.size:
+            free[self.pinned_index] = False
+        self.velocities[~free] = 0
+        x, v = self.current_positions, self.velocities
+        disp = self.rest_positions[free] - x[free]
+        accel = disp * F32(k * inv_mass)
+        v[free] = (v[free] + accel * dt_sim) * gamma
+        x[free] = x[free] + v[free] * dt_sim
+        if self.collision_planes:
+            for i in np.nonzero(free)[0]:
+                for pl in self.collision_planes:
+                    pl.resolve_contact(x[i], v[i])
+        return x
+
+    def trigger_jiggle_impulse(self, magnitude: float) -> None:
+        """Outward radial velocity pulse on every unpinned vertex (280-306)."""
+        n = self.rest_positions.shape[0]
+        if n == 0:
+            return
+        c = np.zeros(3, F32)
+        for p in self.rest_positions:
+            c = (c + p).astype(F32)
+        c = (c / F32(n)).astype(F32)
+        mag = F32(magnitude)
+        for i in range(n):
+            if self.is_vertex_pinned(i):
+                continue
+            d = (self.rest_positions[i] - c).astype(F32)
+            ln = np.sqrt(_dot3(d, d), dtype=F32)
+            rcp = F32(1.0) / ln if ln > 0 else F32(np.inf)
+            d = (d * rcp).astype(F32) if np.isfinite(rcp) and rcp > 0 else np.zeros(3, F32)
+            imp = d * mag if _dot3(d, d) > F32(1e-4) else np.array([0, mag, 0], F32)
+            self.velocities[i] = (self.velocities[i] + imp).astype(F32)
+
+    def reset(self) -> None:
+        self.current_positions[:] = self.rest_positions
+        self.velocities[:] = 0
+        self.pinned_index = None
+
+    def positions_array(self) -> np.ndarray:
+        return self.current_positions.copy()
+
+    def impulse(self, magnitude: float = 0.2, fraction: float = 0.05, seed: int = 3) -> None:
+        """The cage workload's excitation (no reference counterpart): an outward
+        radial velocity pulse of `magnitude` on a seeded `fraction` of the
+        vertices, directions as in trigger_jiggle_impulse."""
         rng = np.random.Generator(np.random.PCG64(0xC1DE5EED00000000 + seed + 100))
-        n = self.rest.shape[0]
+        n = self.rest_positions.shape[0]
         pick = rng.choice(n, size=max(1, int(round(fraction * n))), replace=False)
-        centroid = self.rest.mean(0)
-        d = self.rest[pick] - centroid
+        centroid = self.rest_positions.mean(0)
+        d = self.rest_positions[pick] - centroid
         ln = np.linalg.norm(d, axis=1, keepdims=True)
-        d = np.where(ln > 1e-2, d / np.maximum(ln, 1e-12), np.array([0, 1, 0], dtype=np.float32))
-        self.v[pick] += (d * magnitude).astype(np.float32)
-
-    def step(self, dt: float = 1.0 / 60.0):
-        dt = np.float32(min(dt, self.MAX_DT))
-        gamma = np.float32(math.exp(-float(self.c) * float(dt)))
-        inv_m = np.float32(1.0) / max(self.mass, np.float32(0.001))
-        accel = (self.rest - self.x) * (self.k * inv_m)
-        self.v = ((self.v + accel * dt) * gamma).astype(np.float32)
-        self.x = (self.x + self.v * dt).astype(np.float32)
-        return self.x
+        d = np.where(ln > 1e-2, d / np.maximum(ln, 1e-12), np.array([0, 1, 0], dtype=F32))
+        self.velocities[pick] += (d * magnitude).astype(F32)
 
 
 def auto_bind_gpu(state, meshes, positions: np.ndarray):

@@ -333,8 +333,9 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
                           CgsProf* prof = nullptr, uint32_t part_keys = 16384,
                           uint32_t digit_bits = 8);
-// a_out[i] = a_in[perm[i]], b_out likewise; perm, a_out, b_out 16-byte aligned
-void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
+// a_out[i] = a_in[perm[i]], b_out likewise; perm, a_out, b_out 16-byte
+// aligned (false, nothing launched, otherwise)
+bool cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
                         hipStream_t s);
 void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_t cap,
@@ -557,9 +558,7 @@ void cgs_launch_tet_transforms(const CgsTetArgs& a, hipStream_t s);
 struct CgsDeformArgs {
   const float* verts;
   const uint32_t* elems;
-  const float* jac;
-  const float* stretch;
-  const float* tet_pack;     // [CGS_TET_PACK*T] (see CgsTetArgs)
+  const float* tet_pack;     // [CGS_TET_PACK*T] (see CgsTetArgs), 16-byte aligned
   const uint32_t* bind_tet;  // [n]
   const float* bind_w;       // [4n], 16-byte aligned
   const float* rest_geo;     // blocked K=10
@@ -574,5 +573,6 @@ struct CgsDeformArgs {
   uint32_t num_tets;
 };
 uint32_t cgs_deform_num_parts(uint32_t n);
-void cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s);
+// false (nothing launched) when tet_pack or bind_w is not 16-byte aligned
+bool cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s);
 }

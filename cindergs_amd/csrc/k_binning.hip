@@ -572,11 +572,13 @@ void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_
                      cap, shift, total_tiles, ranges);
 }
 
-void cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
+bool cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,
                         uint32_t* a_out, uint32_t* b_out, const uint32_t* d_n, uint32_t cap,
                         hipStream_t s) {
-  if (cap == 0) return;
+  if (((uintptr_t)perm | (uintptr_t)a_out | (uintptr_t)b_out) & 15u) return false;
+  if (cap == 0) return true;
   const uint32_t quads = (cap + 3u) / 4u;
   hipLaunchKernelGGL(k_gather2, dim3((quads + 255u) / 256u), dim3(256), 0, s, perm, a_in, b_in,
                      a_out, b_out, d_n, cap);
+  return true;
 }

@@ -379,8 +379,11 @@ void cgs_launch_tet_transforms(const CgsTetArgs& a, hipStream_t s) {
 
 uint32_t cgs_deform_num_parts(uint32_t n) { return (n + 256u * DI - 1u) / (256u * DI); }
 
-void cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s) {
+bool cgs_launch_deform(const CgsDeformArgs& a, hipStream_t s) {
+  // the tet record and the weights are read with 16-byte loads
+  if (((uintptr_t)a.tet_pack | (uintptr_t)a.bind_w) & 15u) return false;
   const uint32_t parts = cgs_deform_num_parts(a.num_gaussians);
-  if (parts == 0) return;
+  if (parts == 0) return true;
   hipLaunchKernelGGL(k_deform, dim3(parts), dim3(256), 0, s, a);
+  return true;
 }

@@ -412,8 +412,12 @@ int cgs_ply_load(cgs_context* c, const uint8_t* bytes, uint64_t len, const cgs_p
   const uint64_t n = layout->vertex_count;
   if (n == 0) return CGS_OK;
   if (!bytes || layout->payload_offset > len) return cgs_fail(CGS_ERR_INVALID_ARG, "bad bytes");
-  const uint64_t needed = n * layout->vertex_stride, avail = len - layout->payload_offset;
-  if (avail < needed) {
+  const uint64_t avail = len - layout->payload_offset;
+  // n * stride can wrap for a huge vertex count: compare by division first
+  const uint64_t stride = layout->vertex_stride;
+  const bool too_many = stride != 0 && n > avail / stride;
+  const uint64_t needed = too_many ? UINT64_MAX : n * stride;
+  if (too_many || avail < needed) {
     cgs_ply_layout err = *layout;
     return ply_fail(&err, CGS_PLY_TRUNCATED_PAYLOAD,
                     "vertex payload is truncated: need " + std::to_string(needed) +

@@ -463,8 +463,9 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
     uint32_t* gv = altA ? P<uint32_t>(a.e_perm) : P<uint32_t>(a.e_alt_v);
     // with 4 passes altA == 0: perm in e_perm, gather into the alt pair
     prof->mark("k_gather2", s);
-    cgs_launch_gather2(perm, P<uint32_t>(a.e_keys), P<uint32_t>(a.e_ids), gk, gv, d_n, cap,
-                       s);
+    if (!cgs_launch_gather2(perm, P<uint32_t>(a.e_keys), P<uint32_t>(a.e_ids), gk, gv, d_n, cap,
+                            s))
+      return fail(CGS_ERR_INVALID_ARG, "internal: gather buffers not 16-byte aligned");
     CgsRadixBuffers Bt;
     Bt.keys = gk;
     Bt.vals = gv;
@@ -594,8 +595,6 @@ int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* 
       CgsDeformArgs D;
       D.verts = P<float>(c->verts);
       D.elems = P<uint32_t>(c->elems);
-      D.jac = P<float>(c->jac);
-      D.stretch = P<float>(c->stretch);
       D.tet_pack = P<float>(c->tet_pack);
       D.bind_tet = P<uint32_t>(c->bind_tet);
       D.bind_w = P<float>(c->bind_w);
@@ -610,7 +609,8 @@ int stage1(cgs_context* c, const cgs_frame_params* prm, hipStream_t s, CgsProf* 
       D.num_gaussians = N;
       D.num_tets = c->num_tets;
       prof->mark("k_deform", s);
-      cgs_launch_deform(D, s);
+      if (!cgs_launch_deform(D, s))
+        return fail(CGS_ERR_INVALID_ARG, "internal: tet records or weights not 16-byte aligned");
       HIPCHK(hipGetLastError());
       c->deformed_valid = true;
       c->deform_generation++;
@@ -1530,7 +1530,8 @@ int cgs_radix_sort_u64_pairs(cgs_context* c, uint32_t* tiles, uint32_t* depths, 
   TRY(ensure_words(c->staging, 1));
   uint32_t* d_n = P<uint32_t>(c->staging);
   cgs_launch_fill_u32(d_n, m, 1, c->stream);
-  cgs_launch_gather2(perm, tiles, ids, t2, i2, d_n, m, c->stream);
+  if (!cgs_launch_gather2(perm, tiles, ids, t2, i2, d_n, m, c->stream))
+    return fail(CGS_ERR_INVALID_ARG, "internal: gather buffers not 16-byte aligned");
   // 3) stable sort by tile id
   TRY(radix_prim(c, t2, i2, m, tile_bits ? tile_bits : 1, false, &alt));
   const uint32_t* st = alt ? P<uint32_t>(c->prim_alt_k) : t2;

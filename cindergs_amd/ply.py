@@ -79,12 +79,23 @@ class PlyLoader:
         """(count, {"pos", "cov", "opa", "sh"} device buffers) of a PLY in memory."""
         layout = parse_header(data)
         n = int(layout.vertex_count)
+        p, ln, keep = _bytes_ptr(data)
+        avail = max(0, ln - int(layout.payload_offset))
+        if n * int(layout.vertex_stride) > avail:
+            # checked before any buffer is sized by the (possibly huge) count;
+            # the library applies the same test, without 64-bit wrap-around
+            need = min(n * int(layout.vertex_stride), (1 << 64) - 1)
+            layout.error = 13
+            layout.error_detail[0] = need
+            layout.error_detail[1] = avail
+            raise PlyError(_lib.CGS_ERR_FORMAT,
+                           f"vertex payload is truncated: need {need} bytes, file provides {avail}",
+                           layout.error, layout.error_detail)
         cap = max(n, 1)
         out = {"pos": state.to_device(np.zeros((cap, 3), np.float32)),
                "cov": state.to_device(np.zeros((cap, 6), np.float32)),
                "opa": state.to_device(np.zeros(cap, np.float32)),
                "sh": state.to_device(np.zeros((cap, 48), np.float32))}
-        p, ln, keep = _bytes_ptr(data)
         rc = state._lib.cgs_ply_load(state._h, p, ln, C.byref(layout), out["pos"].ptr,
                                      out["cov"].ptr, out["opa"].ptr, out["sh"].ptr)
         del keep

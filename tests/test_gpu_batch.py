@@ -8,6 +8,8 @@ scenes and resolutions in one batch, an empty scene (background frame), a
 cage-deformed scene with splitting, the 64-bit key path (8K view), graph
 reuse on an unchanged batch and re-capture after a change.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -160,10 +162,10 @@ def test_batch_deformed_scene_and_8k_view():
     lo, hi = scene.positions.min(0), scene.positions.max(0)
     mesh = TetMesh.jittered_grid(lo, hi, (8, 8, 8))
     tet, w = mesh.bind_gaussians(scene.positions)
-    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping=math.exp(-4.0 / 60.0))
     sim.impulse(0.2, 0.05)
     for _ in range(8):
-        sim.step()
+        sim.step(1.0 / 60.0)
     cage = mesh.cage(sim.x, tet, w)
     plain = syn.scene_gaussian_ball(15_000, k=37)
     states = _states([scene, plain], cages=[cage, None])

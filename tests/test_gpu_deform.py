@@ -8,6 +8,8 @@ Mirrors test_deformation_kernel_matches_cpu_reference /
 test_gpu_deform_with_splitting_matches_cpu_strain_detection
 (tests/gpu_parity_tests.rs:165-228, 314-371) with tighter bars.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -21,11 +23,44 @@ def _cage_for(scene, steps=10, impulse=0.2):
     lo, hi = scene.positions.min(0), scene.positions.max(0)
     mesh = TetMesh.jittered_grid(lo, hi, (10, 10, 10))
     tet, w = mesh.bind_gaussians(scene.positions)
-    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping=math.exp(-4.0 / 60.0))
     sim.impulse(impulse, 0.05)
     for _ in range(steps):
-        sim.step()
+        sim.step(1.0 / 60.0)
     return mesh, mesh.cage(sim.x, tet, w)
+
+
+def _rel_rows(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(b).max(1), 1e-30)
+    return np.abs(a - b).max(1) / scale
+
+
+def _check_fp64(n, pos, cov, opa, meta, d64, stretch32, tau):
+    """The device's Stage 1 records against the FP64 twin, 1e-5 relative per
+    record.  A splat split on one side only must sit on the threshold: its
+    tet's stretch within 1e-5 relative of tau; such splats are counted and
+    otherwise skipped (their records are different objects)."""
+    par32 = {int(p): k for k, p in enumerate(meta[:, 0])} if meta.size else {}
+    par64 = {int(p): k for k, p in enumerate(d64["split_meta"][:, 0])} if d64["splits"] else {}
+    only_one = set(par32) ^ set(par64)
+    for p in only_one:
+        tet = int(meta[par32[p], 1]) if p in par32 else int(d64["split_meta"][par64[p], 1])
+        assert abs(float(stretch32[tet]) - tau) <= 1e-5 * tau, (p, stretch32[tet], tau)
+    assert len(only_one) <= max(1, n // 10_000), len(only_one)
+    keep = np.ones(n, bool)
+    keep[list(only_one)] = False
+    assert _rel_rows(pos[:n][keep], d64["pos"][:n][keep]).max() < 1e-5
+    assert _rel_rows(cov[:n][keep], d64["cov"][:n][keep]).max() < 1e-5
+    assert np.abs(opa[:n][keep] - d64["opa"][:n][keep]).max() < 1e-5
+    both = sorted(set(par32) & set(par64))
+    if both:
+        i32 = np.array([n + par32[p] for p in both])
+        i64 = np.array([n + par64[p] for p in both])
+        assert _rel_rows(pos[i32], d64["pos"][i64]).max() < 1e-5
+        assert _rel_rows(cov[i32], d64["cov"][i64]).max() < 1e-5
+        assert np.abs(opa[i32] - d64["opa"][i64]).max() < 1e-5
 
 
 @pytest.mark.parametrize("tau,mip", [(0.0, 0.0), (1.02, 0.0), (1.02, 1e-4), (0.0, 2e-4)])
@@ -55,13 +90,11 @@ def test_deformation_matches_oracle(gpu, oracle_lib, tau, mip):
     np.testing.assert_array_equal(pos.view(np.uint32), d["pos"].view(np.uint32))
     np.testing.assert_array_equal(cov.view(np.uint32), d["cov"].view(np.uint32))
     np.testing.assert_array_equal(opa.view(np.uint32), d["opa"].view(np.uint32))
-    # FP32 against the FP64 twin: within 1e-5 relative (the contract's bar)
+    # FP32 against the FP64 twin: within 1e-5 relative (the contract's bar),
+    # per record (the error of a position or covariance over its own largest
+    # entry); split children are matched by parent
     d64 = oracle.deform(cage, scene, tau, mip, fp64=True)
-    if d64["splits"] == d["splits"]:
-        scale_p = np.abs(d64["pos"]).max()
-        assert np.abs(pos - d64["pos"]).max() / scale_p < 1e-5
-        cn = np.abs(d64["cov"]).max(1, keepdims=True)
-        assert (np.abs(cov - d64["cov"]) / cn).max() < 1e-5
+    _check_fp64(scene.count, pos, cov, opa, meta, d64, d["stretch"], tau)
     # the frame rendered from the deformed splats
     n_act = scene.count + d["splits"]
     attrs, ext = gpu.read_projected(n_act)

@@ -12,6 +12,8 @@ R^T d).  The two-splat case mirrors
 test_deliverable_deformation_end_to_end_pipeline (tests/integration_tests.rs:
 1320-1400).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -27,10 +29,10 @@ def _cage_for(scene, steps=10, impulse=0.2):
     lo, hi = scene.positions.min(0), scene.positions.max(0)
     mesh = TetMesh.jittered_grid(lo, hi, (8, 8, 8))
     tet, w = mesh.bind_gaussians(scene.positions)
-    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping=math.exp(-4.0 / 60.0))
     sim.impulse(impulse, 0.05)
     for _ in range(steps):
-        sim.step()
+        sim.step(1.0 / 60.0)
     return mesh, mesh.cage(sim.x, tet, w)
 
 

@@ -122,6 +122,13 @@ def test_truncated_empty_and_errors(gpu):
     with pytest.raises(PlyError) as e:
         PlyLoader.load_to_device(gpu, text + bytes(8))
     assert e.value.ply_kind == "TruncatedPayload" and e.value.detail == (2 * 14 * 4, 8)
+    # a vertex count whose payload size wraps 64 bits is truncated, not a
+    # wrapped small payload
+    huge = (1 << 64) // (14 * 4) + 1
+    text = header_text("binary_little_endian 1.0", huge, STANDARD).encode()
+    with pytest.raises(PlyError) as e:
+        PlyLoader.load_to_device(gpu, text + bytes(64))
+    assert e.value.ply_kind == "TruncatedPayload" and e.value.detail[1] == 64
     n, out = PlyLoader.load_to_device(gpu, header_text("binary_little_endian 1.0", 0,
                                                        STANDARD).encode())
     assert n == 0

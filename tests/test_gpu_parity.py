@@ -14,14 +14,14 @@ from cindergs_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
-def _frame_vs_oracle(gpu, oracle_lib, scene, cam, comp, label):
+def _frame_vs_oracle(gpu, oracle_lib, scene, cam, comp, label, mip=0.0):
     import oracle
 
     print(f"[case] {label}: n={scene.count} {comp.screen_width}x{comp.screen_height}", flush=True)
     gpu.set_capture(True)
-    t = gpu.execute_frame_gpu(scene, None, cam, comp, force_stage3=True)
+    t = gpu.execute_frame_gpu(scene, None, cam, comp, mip_filter_3d_var=mip, force_stage3=True)
     gpu.set_capture(False)
-    ref = oracle.render_frame(scene, cam, comp)
+    ref = oracle.render_frame(scene, cam, comp, mip=mip)
     n = scene.count
     W, H = comp.screen_width, comp.screen_height
     T = ((W + 15) // 16) * ((H + 15) // 16)
@@ -136,6 +136,17 @@ def test_frame_1p07m_1080p_orbit_matches_oracle(gpu, oracle_lib):
     for frame in (0, 37):
         cam, comp = syn.view_orbit(frame)
         _frame_vs_oracle(gpu, oracle_lib, scene, cam, comp, f"1.07M 1080p orbit frame {frame}")
+
+
+@pytest.mark.parametrize("mip", [1e-4, 2e-3])
+def test_frame_with_3d_mip_filter_and_no_cages(gpu, oracle_lib, mip):
+    """project.rs:1114-1133: without cages the 3D mip filter runs in Stage 2
+    (stage2_mip = mip, gpu_state.rs:894): Sigma + var I and the opacity scaled
+    by sqrt(det Sigma / det(Sigma + var I)), culling splats it pushes below
+    min_opacity."""
+    scene = syn.scene_gaussian_ball(100_000, k=14)
+    cam, comp = syn.view_config1()
+    _frame_vs_oracle(gpu, oracle_lib, scene, cam, comp, f"100k mip={mip}", mip=mip)
 
 
 def test_frame_8k_uses_64bit_keys(gpu, oracle_lib):

@@ -10,6 +10,7 @@ tile order (splats culled now: zero records); above the bound, or after
 invalidate_stage3, the frame equals a fresh one.
 """
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -96,10 +97,10 @@ def test_cage_jitter_reuses_the_tile_order(gpu, oracle_lib):
     lo, hi = scene.positions.min(0), scene.positions.max(0)
     mesh = TetMesh.jittered_grid(lo, hi, (8, 8, 8))
     tet, w = mesh.bind_gaussians(scene.positions)
-    sim = CageSpringSimulator(mesh.rest_vertices)
+    sim = CageSpringSimulator(mesh.rest_vertices, stiffness=40.0, damping=math.exp(-4.0 / 60.0))
     sim.impulse(0.2, 0.05)
     for _ in range(5):
-        sim.step()
+        sim.step(1.0 / 60.0)
     cage = mesh.cage(sim.x, tet, w)
     v2 = cage.vertices.copy()
     v2[:, 1] += np.float32(0.5e-5)
