@@ -253,6 +253,7 @@ SIGNATURES = {
     "cgs_memcpy_d2h": [_vp, _vp, _vp, _u64],
     "cgs_memcpy_d2d": [_vp, _vp, _vp, _u64],
     "cgs_exclusive_scan_u32": [_vp, _vp, _vp, _u64, C.POINTER(_u32)],
+    "cgs_exclusive_scan_u32_noalloc": [_vp, _vp, _vp, _u64, C.POINTER(_u32)],
     "cgs_radix_sort_u32_pairs": [_vp, _vp, _vp, _u64, _u32],
     "cgs_radix_sort_u64_pairs": [_vp, _vp, _vp, _vp, _u64, _u32],
 }

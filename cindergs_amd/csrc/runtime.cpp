@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,31 @@ uint64_t hash_sample(uint64_t h, const uint32_t* data, size_t len) {
 
 const uint64_t FNV_OFFSET = 0xcbf29ce484222325ull;
 
+// Scratch of cgs_exclusive_scan_u32_noalloc: [0] total, [1] partition ticket,
+// [4 ..) one u64 look-back status word per 4096-element partition.  A copy
+// lives on every GPU; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (kept in g_scratch).
+constexpr uint32_t NOALLOC_SCAN_WORDS = 4u + 2u * (CGS_SCAN_NOALLOC_MAX_N / 4096u);
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCAN_WORDS];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[MAX_DEVICES] = {};
+
+// the scratch address on `device` (which must be current), resolved once
+int scratch_for_device(int device, uint32_t** out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(CGS_ERR_UNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CGS_ERR_RUNTIME, std::string("scratch symbol lookup failed: ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch[device];
+  return CGS_OK;
+}
+
 struct Stage3Key {
   uint64_t deform_generation;
   uint32_t view_bits[16];
@@ -191,6 +217,7 @@ struct cgs_context {
   DevBuf ctrl1;  // [split_count, ticket, pad, pad][u64 status...]
   FrameArena fa;
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
+  uint32_t* scan_scratch = nullptr;  // this device's copy of `scratch` (never the symbol itself)
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
   // MVC scene deformation: per-call working set
   DevBuf mvc_work;
@@ -757,6 +784,13 @@ int cgs_create(int device, cgs_context** out) {
     return fail(CGS_ERR_RUNTIME, "pinned allocation failed");
   }
   memset(c->h_counters, 0, 64);
+  const int rs = scratch_for_device(device, &c->scan_scratch);
+  if (rs != CGS_OK) {
+    (void)hipHostFree(c->h_counters);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rs;
+  }
   *out = c;
   return CGS_OK;
 }
@@ -1460,6 +1494,30 @@ int cgs_exclusive_scan_u32(cgs_context* c, const uint32_t* d_in, uint32_t* d_out
   HIPCHK(hipStreamSynchronize(c->stream));
   tot = c->h_counters[8];
   if (total) *total = tot;
+  return CGS_OK;
+}
+
+int cgs_exclusive_scan_u32_noalloc(cgs_context* c, const uint32_t* d_in, uint32_t* d_out,
+                                   uint64_t n, uint32_t* total) {
+  TRY(ensure_ctx(c));
+  if (n > CGS_SCAN_NOALLOC_MAX_N)
+    return fail(CGS_ERR_INVALID_ARG, "n exceeds CGS_SCAN_NOALLOC_MAX_N (2^28)");
+  if (n && (!d_in || !d_out)) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  uint32_t* ctrl = c->scan_scratch;
+  if (!ctrl) return fail(CGS_ERR_RUNTIME, "scan scratch not resolved");
+  const uint32_t parts = cgs_scan_num_parts((uint32_t)n);
+  const size_t words = 4 + 2 * (size_t)(parts ? parts : 1);
+  // the previous call's total, ticket and status words: cleared by a kernel
+  // (no memset node, so the call can be captured into a graph)
+  cgs_launch_fill_u32(ctrl, 0u, words, c->stream);
+  cgs_launch_scan(d_in, d_out, (uint32_t)n, ctrl, reinterpret_cast<uint64_t*>(ctrl + 4), ctrl + 1,
+                  c->stream);
+  HIPCHK(hipGetLastError());
+  if (total) {
+    HIPCHK(hipMemcpyAsync(c->h_counters + 9, ctrl, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *total = c->h_counters[9];
+  }
   return CGS_OK;
 }
 

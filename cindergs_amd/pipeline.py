@@ -351,6 +351,16 @@ class GpuPipelineState:
         check(self._lib.cgs_exclusive_scan_u32(self._h, d_in.ptr, d_out.ptr, n, C.byref(t)))
         return t.value
 
+    def exclusive_scan_total_noalloc(self, d_in: "DeviceBuffer", d_out: "DeviceBuffer", n: int,
+                                     want_total: bool = True):
+        """The scan without any allocation (cgs_exclusive_scan_u32_noalloc):
+        the total, or None (and an asynchronous call) with want_total False."""
+        t = C.c_uint32()
+        check(self._lib.cgs_exclusive_scan_u32_noalloc(self._h, d_in.ptr if d_in else None,
+                                                       d_out.ptr if d_out else None, n,
+                                                       C.byref(t) if want_total else None))
+        return t.value if want_total else None
+
     def radix_sort_u32_pairs(self, keys: "DeviceBuffer", vals: "DeviceBuffer", n: int,
                              sort_bits: int):
         """gpu_radix_sort_u32_pairs (radix_sort.rs:292-368), in place."""

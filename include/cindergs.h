@@ -521,6 +521,19 @@ int cgs_memcpy_d2d(cgs_context* ctx, void* dst, const void* src, uint64_t bytes)
 /* gpu_exclusive_scan_total (src/render/kernels/scan.rs:312-333). */
 int cgs_exclusive_scan_u32(cgs_context* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
                            uint32_t* total);
+/* No counterpart in the reference.  cgs_exclusive_scan_u32 for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same outputs (d_out and the total).
+ * Its scratch (the total, the partition ticket and one 8-byte look-back status
+ * word per 4096 elements: 512 KiB) is a fixed array in the library's device
+ * image, one copy per GPU, held until the process ends; no call allocates.
+ * n <= CGS_SCAN_NOALLOC_MAX_N, else CGS_ERR_INVALID_ARG.  With total NULL the
+ * call is asynchronous on the context stream; otherwise it waits and reads the
+ * total.  Every context on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+#define CGS_SCAN_NOALLOC_MAX_N (1u << 28)
+int cgs_exclusive_scan_u32_noalloc(cgs_context* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                   uint64_t n, uint32_t* total);
 /* gpu_radix_sort_u32_pairs (radix_sort.rs:292-368): stable sort of (key, value)
  * on the low sort_bits key bits, in place. */
 int cgs_radix_sort_u32_pairs(cgs_context* ctx, uint32_t* d_keys, uint32_t* d_vals, uint64_t n,
