@@ -650,8 +650,9 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
         if tu.radix_digit_bits != 8 or i == 0:
             continue
         wall, gms, stats = run_batch(best[0], tu)
-        key = "lanes={} part={} batch={} order={}".format(
-            best[0], tu.radix_partition_keys, tu.composite_batch, tu.tile_order)
+        key = "lanes={} sort={} part={} batch={} order={}".format(
+            best[0], "bucket" if tu.tile_sort else "radix", tu.radix_partition_keys,
+            tu.composite_batch, tu.tile_order)
         sweep[key] = round(1000 * wall, 3)
         print(f"# batched {key}: {1000 * wall:.2f} ms", file=sys.stderr, flush=True)
         if wall < best[1]:

@@ -141,6 +141,7 @@ class Tuning(C.Structure):
         ("radix_digit_bits", C.c_uint32),
         ("composite_batch", C.c_uint32),
         ("tile_order", C.c_uint32),
+        ("tile_sort", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -254,7 +254,52 @@ struct CgsBinArgs {
   // status, tile ranges), so that no fill launch precedes it: three spans
   uint32_t* zero_at[3];
   uint32_t zero_len[3];
+  // bucket mode (tile_hist null: the LSD mode, one sub-block per block)
+  uint32_t* tile_hist;        // [grid][tiles_pad / 2] u16-pair tile counts per block (k_count)
+  const uint32_t* tile_base;  // [grid][tiles_pad] bucket base of every (block, tile) (k_emit)
+  uint64_t* bucket;           // [cap] (key << 32 | splat id), grouped by tile (k_emit)
+  uint32_t tiles_pad;         // multiple of 128
+  uint32_t subs;              // sub-blocks of 512 splats per block
+  uint32_t parts;             // sub-blocks in all
 };
+
+// ---- Stage 3 in bucket mode (k_tilesort.hip) ----
+#define CGS_SEG_WINDOW 2048u  // entries per sort window; larger tiles are sorted in chunks
+#define CGS_SEG_CHUNK 4096u   // entries per chunk of a large tile
+struct CgsBucketArgs {
+  const uint32_t* hist;   // [rows][tiles_pad / 2] u16-pair tile counts of the binning blocks
+  uint32_t rows, tiles, tiles_pad, cap;
+  uint32_t big;           // tiles with more entries are sorted in chunks
+  uint32_t window;        // CGS_SEG_WINDOW
+  uint32_t* base;         // [rows][tiles_pad] bucket base of every (block, tile)
+  uint32_t* tile_start;   // [tiles + 1] monotone tile starts
+  uint32_t* chunk_start;  // [tiles + 1] first chunk of every tile (large tiles)
+  uint32_t* ranges;       // [2 tiles] (start, end), (0, 0) when empty
+  uint32_t* win_first;    // [win_slots] first tile starting at or after k x window
+  uint32_t win_slots;
+  uint32_t* chunk_tile;   // [chunk_slots] tile of every chunk
+  uint32_t chunk_slots;
+  uint64_t* status;       // [tiles_pad / 128] look-back, zeroed
+  uint32_t* ticket;       // zeroed
+  uint32_t* stats;        // [4] zeroed: max tile entries, merge rounds, entries, chunks
+};
+struct CgsSegSortArgs {
+  uint64_t* bucket;       // [cap] entries grouped by tile (from k_emit)
+  uint64_t* alt;          // [cap] chunk runs of the large tiles
+  uint32_t* keys;         // [cap] sorted keys (output)
+  uint32_t* ids;          // [cap] sorted splat ids (output)
+  const uint32_t* tile_start;
+  const uint32_t* chunk_start;
+  const uint32_t* win_first;
+  const uint32_t* chunk_tile;
+  const uint32_t* stats;
+  uint32_t tiles, cap, window, windows, big, chunk_slots;
+};
+uint32_t cgs_seg_sort_windows(uint32_t cap);
+struct CgsProf;
+void cgs_launch_bucket_scan(const CgsBucketArgs& a, hipStream_t s);
+void cgs_launch_seg_sort(const CgsSegSortArgs& a, uint32_t chunk_blocks, uint32_t rounds,
+                         hipStream_t s, CgsProf* prof);
 
 // Optional per-launch timing: mark(name) records an event before a launch;
 // the time of launch i is the interval between marks i and i+1.
@@ -295,7 +340,11 @@ uint32_t cgs_bin_num_parts(uint32_t max_gaussians);
 size_t cgs_bin_mask_words(uint32_t max_gaussians);
 void cgs_bin_masks(uint32_t* buf, uint32_t max_gaussians, uint8_t** small_masks,
                    uint64_t** chunk_masks);
-void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof = nullptr);
+// tiles the bucket mode's LDS histograms hold (larger frames use the LSD mode)
+#define CGS_BUCKET_MAX_TILES 32768u
+uint32_t cgs_bin_grid(const CgsBinArgs& a);
+void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof = nullptr,
+                    const CgsBucketArgs* between = nullptr);
 void cgs_launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
                      uint64_t* status, uint32_t* ticket, hipStream_t s);
 uint32_t cgs_scan_num_parts(uint32_t n);

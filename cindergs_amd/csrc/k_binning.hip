@@ -27,6 +27,10 @@ constexpr int BT = 512;
 constexpr int BP = BT;        // splats per block
 constexpr uint32_t SMALL_BOX = 8;  // boxes of at most this many tiles: walked by their thread
 constexpr uint32_t KCHUNK = 32;    // hit masks kept per wave for the wave walk
+// words of the bucket mode's LDS tile histogram: 8192 tiles (1080p: 8160) or
+// 32768 tiles (2160p: 32400)
+constexpr int BIN_HW_SMALL = 4096;
+constexpr int BIN_HW_LARGE = 16384;
 
 struct TileBox {
   uint32_t tx0, tx1, ty0, ty1;
@@ -218,56 +222,84 @@ __device__ __forceinline__ uint64_t run_of_head(uint64_t R, uint32_t lane) {
   return lo & (~0ull << lane);
 }
 
-// count_tiles_kernel (binning.rs:90-164): per-splat tile counts + per-block sums
+// one entry of a tile histogram of u16 pairs (tile t in the low / high half
+// of word t / 2; a block covers at most 65535 splats, so no half overflows)
+__device__ __forceinline__ void tile_hist_add(uint32_t* h, uint32_t tile) {
+  atomicAdd(&h[tile >> 1], 1u << ((tile & 1u) << 4));
+}
+
+// count_tiles_kernel (binning.rs:90-164): per-splat tile counts and per
+// sub-block sums; in bucket mode (HW > 0) also the block's tile histogram,
+// written as row blockIdx.x of tile_hist.  A block takes `subs` consecutive
+// sub-blocks of BP splats.
+template <int HW>
 __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
   __shared__ WaveBig s_w[BT / 64];
   __shared__ uint32_t s_tmp[BT / 64];
+  __shared__ uint32_t s_th[HW > 0 ? HW : 1];
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   WaveBig& W = s_w[wave];
   const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
-  const uint32_t s = blockIdx.x * (uint32_t)BP + t;
   // the spans the later kernels of the frame need cleared (none of them is
   // read or written by this kernel)
   for (int z = 0; z < 3; ++z)
     for (uint32_t i = blockIdx.x * (uint32_t)BT + t; i < A.zero_len[z]; i += gridDim.x * (uint32_t)BT)
       A.zero_at[z][i] = 0u;
-  const BinSplat b = load_bin_splat(A, s, active);
-  // small boxes: the thread walks them; bit k of the splat's mask = box tile
-  // k (row-major) is hit
-  uint32_t cnt = 0;
-  if (b.area != 0u && b.area <= SMALL_BOX) {
-    uint32_t mask = 0u, k = 0u;
-    for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
-      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
-        if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) mask |= 1u << k;
-    cnt = (uint32_t)__popc(mask);
-    if (A.small_masks) A.small_masks[s] = (uint8_t)mask;
+  if (HW > 0) {
+    for (uint32_t i = t; i < (uint32_t)HW; i += BT) s_th[i] = 0u;
+    __syncthreads();
   }
-  // big boxes: the wave walks them together; the hit masks of its first
-  // KCHUNK chunks are kept for k_emit
-  uint32_t nv, slot;
-  const uint32_t total = stage_big(W, b, s, 0u, lane, nv, slot);
-  uint64_t* masks =
-      A.chunk_masks ? A.chunk_masks + ((size_t)blockIdx.x * (BT / 64) + wave) * KCHUNK : nullptr;
-  uint32_t cp = 0;
-  for (uint32_t base = 0; base < total; base += 64u) {
-    uint32_t sp, tx, ty;
-    bool hit;
-    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
-    const uint64_t Hm = __ballot(hit);
-    if (masks && base / 64u < KCHUNK && lane == 0u) masks[base / 64u] = Hm;
-    if (((R >> lane) & 1ull) && base + lane < total)
-      W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
-    wave_sync_lds();
+  const uint32_t sb0 = blockIdx.x * A.subs, sb1 = min(sb0 + A.subs, A.parts);
+  for (uint32_t sb = sb0; sb < sb1; ++sb) {
+    const uint32_t s = sb * (uint32_t)BP + t;
+    const BinSplat b = load_bin_splat(A, s, active);
+    // small boxes: the thread walks them; bit k of the splat's mask = box tile
+    // k (row-major) is hit
+    uint32_t cnt = 0;
+    if (b.area != 0u && b.area <= SMALL_BOX) {
+      uint32_t mask = 0u, k = 0u;
+      for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
+        for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
+          if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+            mask |= 1u << k;
+            if (HW > 0) tile_hist_add(s_th, ty * A.tiles_x + tx);
+          }
+      cnt = (uint32_t)__popc(mask);
+      if (A.small_masks) A.small_masks[s] = (uint8_t)mask;
+    }
+    // big boxes: the wave walks them together; the hit masks of its first
+    // KCHUNK chunks are kept for k_emit
+    uint32_t nv, slot;
+    const uint32_t total = stage_big(W, b, s, 0u, lane, nv, slot);
+    uint64_t* masks =
+        A.chunk_masks ? A.chunk_masks + ((size_t)sb * (BT / 64) + wave) * KCHUNK : nullptr;
+    uint32_t cp = 0;
+    for (uint32_t base = 0; base < total; base += 64u) {
+      uint32_t sp, tx, ty;
+      bool hit;
+      const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit);
+      const uint64_t Hm = __ballot(hit);
+      if (HW > 0 && hit) tile_hist_add(s_th, ty * A.tiles_x + tx);
+      if (masks && base / 64u < KCHUNK && lane == 0u) masks[base / 64u] = Hm;
+      if (((R >> lane) & 1ull) && base + lane < total)
+        W.acc[sp] += (uint32_t)__popcll(Hm & run_of_head(R, lane));
+      wave_sync_lds();
+    }
+    if (b.area > SMALL_BOX) cnt = W.acc[slot];
+    if (s < A.max_gaussians) A.counts[s] = cnt;
+    uint32_t tot, stot;
+    (void)cgs::block_exclusive_scan<BT>(cnt, s_tmp, tot);
+    (void)cgs::block_exclusive_scan<BT>(b.surv ? 1u : 0u, s_tmp, stot);
+    if (t == 0) {
+      A.block_sums[sb] = tot;
+      A.block_surv[sb] = stot;
+    }
   }
-  if (b.area > SMALL_BOX) cnt = W.acc[slot];
-  if (s < A.max_gaussians) A.counts[s] = cnt;
-  uint32_t tot, stot;
-  (void)cgs::block_exclusive_scan<BT>(cnt, s_tmp, tot);
-  (void)cgs::block_exclusive_scan<BT>(b.surv ? 1u : 0u, s_tmp, stot);
-  if (t == 0) {
-    A.block_sums[blockIdx.x] = tot;
-    A.block_surv[blockIdx.x] = stot;
+  if (HW > 0) {
+    __syncthreads();
+    const uint32_t hw = A.tiles_pad / 2u;
+    uint32_t* row = A.tile_hist + (size_t)blockIdx.x * hw;
+    for (uint32_t i = t; i < hw; i += BT) row[i] = s_th[i];
   }
 }
 
@@ -337,118 +369,140 @@ __device__ __forceinline__ void store_entry(const CgsBinArgs& A, uint32_t pos, u
   A.ids[pos] = id;
 }
 
+// bucket mode: the entry goes to its tile's bucket at the (block, tile) base
+// plus its rank among the block's entries of that tile (LDS cursor)
+__device__ __forceinline__ void bucket_put(const CgsBinArgs& A, uint32_t* cur, uint32_t tile,
+                                           uint32_t kb, uint32_t id) {
+  const uint32_t sh = (tile & 1u) << 4;
+  const uint32_t r = (atomicAdd(&cur[tile >> 1], 1u << sh) >> sh) & 0xFFFFu;
+  const uint32_t slot = A.tile_base[(size_t)blockIdx.x * A.tiles_pad + tile] + r;
+  if (slot < A.cap) A.bucket[slot] = ((uint64_t)((tile << A.depth_bits) | kb) << 32) | id;
+}
+
 // gpu_exclusive_scan_total + emit_tile_keys(_u64)_kernel (scan.rs:198-333,
-// binning.rs:168-328).  The prefix of block b is the sum of the block sums
-// before it, reduced here by the block itself (no look-back chain, no ticket).
-// Only stored entries are histogrammed: the sort reads min(total, cap)
-// entries and its digit offsets must add up to that.
+// binning.rs:168-328).  The prefix of sub-block sb is the sum of the
+// sub-block sums before it, reduced here by the block itself (no look-back
+// chain, no ticket).  LSD mode: entries in emission order (splat-major) and
+// the digit histograms of the sort; only stored entries are histogrammed (the
+// sort reads min(total, cap) entries and its digit offsets must add up to
+// that).  Bucket mode (HW > 0): entries into their tile buckets.
+template <int HW>
 __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
   __shared__ WaveBig s_w[BT / 64];
   __shared__ uint32_t s_tmp[BT / 64];
-  __shared__ uint32_t s_hist[2048];
+  __shared__ uint32_t s_hist[HW > 0 ? HW : 2048];  // digit histograms, or tile cursors
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t blk = blockIdx.x;
   WaveBig& W = s_w[wave];
-  if (A.hist_partial)
-    for (uint32_t i = t; i < A.hist_words; i += BT) s_hist[i] = 0u;
-  // prefix of this block and, in the last block, the totals
-  uint32_t pre = 0, all_surv = 0;
-  const bool last = blk == gridDim.x - 1;
-  for (uint32_t j = t; j < blk; j += BT) pre += A.block_sums[j];
-  if (last)
-    for (uint32_t j = t; j < gridDim.x; j += BT) all_surv += A.block_surv[j];
-  uint32_t pre_tot, surv_tot;
-  (void)cgs::block_exclusive_scan<BT>(pre, s_tmp, pre_tot);
-  (void)cgs::block_exclusive_scan<BT>(all_surv, s_tmp, surv_tot);
-  const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
-  const uint32_t s = blk * (uint32_t)BP + t;
-  const uint32_t c = s < A.max_gaussians ? A.counts[s] : 0u;
-  uint32_t btot;
-  const uint32_t off = pre_tot + cgs::block_exclusive_scan<BT>(c, s_tmp, btot);
-  if (last && t == 0) {
-    A.total[0] = pre_tot + btot;
-    A.total[1] = surv_tot;
-  }
-  if (A.offsets_out && s < A.max_gaussians) A.offsets_out[s] = off;
-  const bool hist = A.hist_partial != nullptr;
+  const bool hist = HW == 0 && A.hist_partial != nullptr;
+  const uint32_t hclear = HW > 0 ? (uint32_t)HW : (hist ? A.hist_words : 0u);
+  for (uint32_t i = t; i < hclear; i += BT) s_hist[i] = 0u;
   const uint32_t hbits = A.mode64 ? 32u : A.sort_bits;
   const uint32_t db = A.digit_bits == 4u ? 4u : 8u;
-  // staged exactly as k_count staged it (its walk's chunks, whose hit masks
-  // are reused here, depend on every big box, hit or not)
-  const BinSplat b = load_bin_splat(A, s, active);
-  // small boxes: the thread walks them (the tiles k_count found hit, when it
-  // kept their mask)
-  if (b.area != 0u && b.area <= SMALL_BOX) {
-    uint32_t cur = off;
-    const uint32_t kb = cgs_f2u(b.d1.w);
-    const uint32_t mask = A.small_masks ? (uint32_t)A.small_masks[s] : 0u;
-    // passes whose digits hold depth bits only (the splat's own): counted once
-    // below with the splat's stored entries; the others per entry
-    const uint32_t cpass = A.mode64 ? 4u : min(A.depth_bits, hbits) / db;
-    uint32_t k = 0u;
-    for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
-      for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
-        if (A.small_masks ? ((mask >> k) & 1u) != 0u
-                          : hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
-          const uint32_t tile = ty * A.tiles_x + tx;
-          if (cur < A.cap) {
-            store_entry(A, cur, tile, kb, s);
-            if (hist) {
-              if (!A.mode64)
-                hist_key_from(s_hist, (tile << A.depth_bits) | kb, hbits, db, cpass);
-              else
-                hist_key(s_hist + 1024, tile, A.tile_bits, 8u);
+  const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
+  const uint32_t sb0 = blk * A.subs, sb1 = min(sb0 + A.subs, A.parts);
+  for (uint32_t sb = sb0; sb < sb1; ++sb) {
+    // prefix of this sub-block and, in the last one, the totals
+    uint32_t pre = 0, all_surv = 0;
+    const bool last = sb == A.parts - 1u;
+    for (uint32_t j = t; j < sb; j += BT) pre += A.block_sums[j];
+    if (last)
+      for (uint32_t j = t; j < A.parts; j += BT) all_surv += A.block_surv[j];
+    uint32_t pre_tot, surv_tot;
+    (void)cgs::block_exclusive_scan<BT>(pre, s_tmp, pre_tot);
+    (void)cgs::block_exclusive_scan<BT>(all_surv, s_tmp, surv_tot);
+    const uint32_t s = sb * (uint32_t)BP + t;
+    const uint32_t c = s < A.max_gaussians ? A.counts[s] : 0u;
+    uint32_t btot;
+    const uint32_t off = pre_tot + cgs::block_exclusive_scan<BT>(c, s_tmp, btot);
+    if (last && t == 0) {
+      A.total[0] = pre_tot + btot;
+      A.total[1] = surv_tot;
+    }
+    if (A.offsets_out && s < A.max_gaussians) A.offsets_out[s] = off;
+    // staged exactly as k_count staged it (its walk's chunks, whose hit masks
+    // are reused here, depend on every big box, hit or not)
+    const BinSplat b = load_bin_splat(A, s, active);
+    // small boxes: the thread walks them (the tiles k_count found hit, when it
+    // kept their mask)
+    if (b.area != 0u && b.area <= SMALL_BOX) {
+      uint32_t cur = off;
+      const uint32_t kb = cgs_f2u(b.d1.w);
+      const uint32_t mask = A.small_masks ? (uint32_t)A.small_masks[s] : 0u;
+      // passes whose digits hold depth bits only (the splat's own): counted once
+      // below with the splat's stored entries; the others per entry
+      const uint32_t cpass = A.mode64 ? 4u : min(A.depth_bits, hbits) / db;
+      uint32_t k = 0u;
+      for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
+        for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
+          if (A.small_masks ? ((mask >> k) & 1u) != 0u
+                            : hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+            const uint32_t tile = ty * A.tiles_x + tx;
+            if (HW > 0) {
+              bucket_put(A, s_hist, tile, kb, s);
+            } else if (cur < A.cap) {
+              store_entry(A, cur, tile, kb, s);
+              if (hist) {
+                if (!A.mode64)
+                  hist_key_from(s_hist, (tile << A.depth_bits) | kb, hbits, db, cpass);
+                else
+                  hist_key(s_hist + 1024, tile, A.tile_bits, 8u);
+              }
             }
+            ++cur;
           }
-          ++cur;
-        }
-    const uint32_t stored = min(cur, A.cap) - min(off, A.cap);
-    if (hist && stored != 0u) {
-      if (!A.mode64)
-        hist_const(s_hist, kb, hbits, db, cpass, stored);
-      else
-        hist_const(s_hist, kb, 32u, 8u, 4u, stored);
-    }
-  }
-  // big boxes: the wave walks them together; a hit goes to its splat's next
-  // slot plus the hits of the same splat on the lanes before it
-  uint32_t nv, slot;
-  const uint32_t total = stage_big(W, b, s, off, lane, nv, slot);
-  const uint64_t* masks =
-      A.chunk_masks ? A.chunk_masks + ((size_t)blk * (BT / 64) + wave) * KCHUNK : nullptr;
-  uint32_t cp = 0;
-  for (uint32_t base = 0; base < total; base += 64u) {
-    uint32_t sp, tx, ty;
-    bool hit;
-    const bool known = masks && base / 64u < KCHUNK;
-    const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit, !known,
-                                  known ? masks[base / 64u] : 0ull);
-    const uint64_t Hm = __ballot(hit);
-    const bool act = base + lane < total;
-    uint32_t pos = 0, id = 0, kb = 0;
-    if (act) kb = cgs_f2u(W.d1[sp].w);  // the splat's depth bits, valid on every active lane
-    if (hit) {
-      pos = W.acc[sp] + (uint32_t)__popcll(Hm & run_before(R, lane));
-      id = W.box[sp].w;
-    }
-    const bool head = ((R >> lane) & 1ull) && act;
-    const uint64_t run = head ? run_of_head(R, lane) : 0ull;
-    wave_sync_lds();
-    if (head) W.acc[sp] += (uint32_t)__popcll(Hm & run);
-    const bool store = hit && pos < A.cap;
-    const uint64_t Hs = __ballot(store);
-    const uint32_t tile = ty * A.tiles_x + tx;
-    if (store) store_entry(A, pos, tile, kb, id);
-    if (hist) {
-      if (!A.mode64) {
-        hist_walk(s_hist, (tile << A.depth_bits) | kb, hbits, A.depth_bits, db, store, Hs, head,
-                  run, lane);
-      } else {
-        hist_walk(s_hist, kb, 32u, 32u, 8u, store, Hs, head, run, lane);
-        hist_walk(s_hist + 1024, tile, A.tile_bits, 0u, 8u, store, Hs, head, run, lane);
+      const uint32_t stored = min(cur, A.cap) - min(off, A.cap);
+      if (hist && stored != 0u) {
+        if (!A.mode64)
+          hist_const(s_hist, kb, hbits, db, cpass, stored);
+        else
+          hist_const(s_hist, kb, 32u, 8u, 4u, stored);
       }
     }
-    wave_sync_lds();
+    // big boxes: the wave walks them together; a hit goes to its splat's next
+    // slot plus the hits of the same splat on the lanes before it
+    uint32_t nv, slot;
+    const uint32_t total = stage_big(W, b, s, off, lane, nv, slot);
+    const uint64_t* masks =
+        A.chunk_masks ? A.chunk_masks + ((size_t)sb * (BT / 64) + wave) * KCHUNK : nullptr;
+    uint32_t cp = 0;
+    for (uint32_t base = 0; base < total; base += 64u) {
+      uint32_t sp, tx, ty;
+      bool hit;
+      const bool known = masks && base / 64u < KCHUNK;
+      const uint64_t R = walk_chunk(A, W, base, total, nv, lane, cp, sp, tx, ty, hit, !known,
+                                    known ? masks[base / 64u] : 0ull);
+      const uint64_t Hm = __ballot(hit);
+      const bool act = base + lane < total;
+      uint32_t pos = 0, id = 0, kb = 0;
+      if (act) kb = cgs_f2u(W.d1[sp].w);  // the splat's depth bits, valid on every active lane
+      if (hit) {
+        pos = W.acc[sp] + (uint32_t)__popcll(Hm & run_before(R, lane));
+        id = W.box[sp].w;
+      }
+      const bool head = ((R >> lane) & 1ull) && act;
+      const uint64_t run = head ? run_of_head(R, lane) : 0ull;
+      wave_sync_lds();
+      if (head) W.acc[sp] += (uint32_t)__popcll(Hm & run);
+      const uint32_t tile = ty * A.tiles_x + tx;
+      if (HW > 0) {
+        if (hit) bucket_put(A, s_hist, tile, kb, id);
+      } else {
+        const bool store = hit && pos < A.cap;
+        const uint64_t Hs = __ballot(store);
+        if (store) store_entry(A, pos, tile, kb, id);
+        if (hist) {
+          if (!A.mode64) {
+            hist_walk(s_hist, (tile << A.depth_bits) | kb, hbits, A.depth_bits, db, store, Hs,
+                      head, run, lane);
+          } else {
+            hist_walk(s_hist, kb, 32u, 32u, 8u, store, Hs, head, run, lane);
+            hist_walk(s_hist + 1024, tile, A.tile_bits, 0u, 8u, store, Hs, head, run, lane);
+          }
+        }
+      }
+      wave_sync_lds();
+    }
   }
   if (hist) {
     __syncthreads();
@@ -554,13 +608,36 @@ void cgs_bin_masks(uint32_t* buf, uint32_t max_gaussians, uint8_t** small_masks,
   *chunk_masks = reinterpret_cast<uint64_t*>(buf + small);
 }
 
-void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof) {
-  const uint32_t parts = cgs_bin_num_parts(a.max_gaussians);
-  if (parts == 0) return;
+uint32_t cgs_bin_grid(const CgsBinArgs& a) {
+  return a.subs ? (a.parts + a.subs - 1u) / a.subs : 0u;
+}
+
+// LSD mode: k_count, k_emit.  Bucket mode: k_count, `between` (the column
+// scan), k_emit.
+void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof,
+                    const CgsBucketArgs* between) {
+  const uint32_t grid = cgs_bin_grid(a);
+  if (grid == 0) return;
+  const uint32_t hw = a.tiles_pad / 2u;
+  const int mode = !a.tile_hist ? 0 : hw <= (uint32_t)BIN_HW_SMALL ? 1 : 2;
   if (prof) prof->mark("k_count", s);
-  hipLaunchKernelGGL(k_count, dim3(parts), dim3(BT), 0, s, a);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_count<0>, dim3(grid), dim3(BT), 0, s, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_count<BIN_HW_SMALL>, dim3(grid), dim3(BT), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_count<BIN_HW_LARGE>, dim3(grid), dim3(BT), 0, s, a);
+  if (between) {
+    if (prof) prof->mark("k_bucket_scan", s);
+    cgs_launch_bucket_scan(*between, s);
+  }
   if (prof) prof->mark("k_emit", s);
-  hipLaunchKernelGGL(k_emit, dim3(parts), dim3(BT), 0, s, a);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_emit<0>, dim3(grid), dim3(BT), 0, s, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_emit<BIN_HW_SMALL>, dim3(grid), dim3(BT), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_emit<BIN_HW_LARGE>, dim3(grid), dim3(BT), 0, s, a);
 }
 
 void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_t cap,

@@ -21,10 +21,13 @@
 //    the blend is sequential; the blend is branch-free (select, not if/break);
 //  * a wave stops when none of its pixels is alive, the tile stops after a
 //    batch in which no pixel is alive (__syncthreads_or);
-//  * the conic is stored pre-scaled by -log2(e)/2 (b by -log2(e)) so that
-//    q2 = log2(e) q takes two FMAs and e^q is one hardware exp2; the blend
-//    folds T (1 - alpha) into T - alpha T.  These change only rounding (the
-//    colour parity bound is 1/255);
+//  * q = -1/2 (a dx^2 + 2 b dx dy + c dy^2) is computed with the reference's
+//    operations in the reference's order, on the conic stored as -a/2, -b,
+//    -c/2 (power-of-two scalings: the same bits): the cut q >= -4, where the
+//    blended alpha jumps by up to 0.99 e^-4, falls on the same side as in the
+//    oracle; e^q is the hardware exp2 of q log2(e) and the blend folds
+//    T (1 - alpha) into T - alpha T, which change only rounding (the colour
+//    parity bound is 1/255);
 //  * block b composites tile b: the hardware deals consecutive blocks to the 8
 //    XCDs in turn, so XCD k gets the tile columns x = k (mod 8) (the row
 //    length 120 at 1080p is a multiple of 8) and the dense middle of the
@@ -61,20 +64,20 @@ __device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float 
 
 // One splat's compositor inputs as staged in LDS.
 struct Splat {
-  float4 g;  // -a/2, -b, -c/2 scaled by log2(e), opacity (0 when <= 1/255)
+  float4 g;  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
   float4 m;  // mu_x, mu_y, r, g
   float b;   // blue
 };
 
 constexpr float LOG2E = 1.4426950408889634f;
-constexpr float Q2_MIN = -4.0f * LOG2E;  // q >= -4 in base-2 units
 
-// log2(e) * min(q, 0) with q = -1/2 (a dx^2 + 2 b dx dy + c dy^2), as
-// dx (a' dx + b' dy) + c' dy^2 (two fused multiply-adds)
-__device__ __forceinline__ float splat_q2(const Splat& s, float pcx, float pcy) {
+// min(q, 0) with q = -1/2 (a dx^2 + 2 b dx dy + c dy^2) in the order of
+// compositor.rs (a dx dx + 2 b dx dy + c dy dy, left to right, then the
+// factor): with the factor folded into the staged conic the products and sums
+// round to the same bits (no fused operations: -ffp-contract=off)
+__device__ __forceinline__ float splat_q(const Splat& s, float pcx, float pcy) {
   const float dx = pcx - s.m.x, dy = pcy - s.m.y;
-  const float u = __builtin_fmaf(s.g.y, dy, s.g.x * dx);
-  const float q = __builtin_fmaf(u, dx, s.g.z * dy * dy);
+  const float q = (s.g.x * dx * dx + s.g.y * dx * dy) + s.g.z * dy * dy;
   return fminf(q, 0.0f);
 }
 
@@ -82,9 +85,9 @@ __device__ __forceinline__ float splat_q2(const Splat& s, float pcx, float pcy) 
 // w = alpha T, C += w c, T -= w (= T (1 - alpha)); T < 1e-3 saturates the
 // pixel (T = 0) after the splat is added.  A pixel that is not alive has
 // ok = false, so w = 0 and its T stays 0 (saturated) or 1 (off screen).
-__device__ __forceinline__ void blend(const Splat& s, float q2, float a, bool valid, bool& alive,
+__device__ __forceinline__ void blend(const Splat& s, float q, float a, bool valid, bool& alive,
                                       float& T, float& cr, float& cg, float& cb) {
-  const bool ok = valid && alive && q2 >= Q2_MIN && a >= ALPHA_MIN;
+  const bool ok = valid && alive && q >= -4.0f && a >= ALPHA_MIN;
   const float w = (ok ? a : 0.0f) * T;
   cr = __builtin_fmaf(w, s.m.z, cr);
   cg = __builtin_fmaf(w, s.m.w, cg);
@@ -141,8 +144,20 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* ranges, uin
   }
 }
 
-// CB splats per batch (64, 128 or 256), staged by threads t < CB
-template <int CB>
+// Splats per step of the inner loop and the next batch's prefetch, measured
+// on the 1.07M-splat 1080p orbit (k_composite, batch 256): 4 splats with
+// prefetch 216 us (82 VGPRs, 5 waves per SIMD), without 219 us; 2 splats
+// 246 us with or without (64-66 VGPRs, 7-8 waves per SIMD).  The wave is
+// VALU bound (about 26 VALU instructions per splat evaluation, most of them
+// the reference's q and the sequential blend), so the independent q / e^q
+// chains of 4 splats are worth more than the occupancy of 2.
+constexpr int COMPOSITE_STEP = 4;
+constexpr bool COMPOSITE_PREFETCH = true;
+
+// CB splats per batch (64, 128 or 256), staged by threads t < CB; STEP
+// splats walked per step of the inner loop; PF: the records of the next batch
+// are loaded while this one is composited
+template <int CB, int STEP, bool PF>
 __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   // staged batch: plane 0 = Splat::g, 1 = Splat::m, 2 = (blue, -, -, -); the
   // quadrant lists hold byte offsets into a plane, so a splat's three reads
@@ -170,13 +185,13 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   const float4* geom4 = reinterpret_cast<const float4*>(A.geom);
   const float4* color4 = reinterpret_cast<const float4*>(A.color);
 
-  // records of the batch about to be staged, loaded one batch ahead.  The
-  // loads are unconditional, at indexes clamped to the tile's last entry, so
-  // that nothing waits for them before they are used a batch later.
+  // with PF, the records of the batch about to be staged are loaded one batch
+  // ahead.  The loads are unconditional, at indexes clamped to the tile's last
+  // entry, so that nothing waits for them before they are used a batch later.
   float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, rc = r0;
   uint32_t g_ahead = 0u;
   const uint32_t last = empty ? rs : re - 1u;
-  if (!empty && loader) {
+  if (PF && !empty && loader) {
     const uint32_t g = A.sorted_ids[min(rs + t, last)];
     r0 = geom4[(size_t)g * 2];
     r1 = geom4[(size_t)g * 2 + 1];
@@ -190,10 +205,16 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
     // ---- stage the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
     bool q0 = false, q1 = false, q2 = false, q3 = false;
+    if (!PF && loader && e < re) {
+      const uint32_t g = A.sorted_ids[e];
+      r0 = geom4[(size_t)g * 2];
+      r1 = geom4[(size_t)g * 2 + 1];
+      rc = color4[g];
+    }
     if (loader && e < re) {
       // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
       const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
-      s_stage[0][t] = make_float4(-0.5f * LOG2E * r0.z, -LOG2E * r0.w, -0.5f * LOG2E * r1.x, op);
+      s_stage[0][t] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
       s_stage[1][t] = make_float4(r0.x, r0.y, rc.x, rc.y);
       reinterpret_cast<float*>(&s_stage[2][t])[0] = rc.z;
       q0 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by, 8u, 8u,
@@ -213,7 +234,7 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
       s_mask[3][wave] = m3;
     }
     // the next batch's records are loaded while this batch is composited
-    if (loader) {
+    if (PF && loader) {
       const uint32_t en = e + CB;
       r0 = geom4[(size_t)g_ahead * 2];
       r1 = geom4[(size_t)g_ahead * 2 + 1];
@@ -233,27 +254,35 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
     cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
     __builtin_amdgcn_wave_barrier();
 
-    // ---- walk it four splats at a time, front to back ----
+    // ---- walk it STEP splats at a time, front to back ----
     if (__ballot(alive) != 0ull) {
 #pragma unroll 1
-      for (uint32_t i = 0; i < cnt; i += 4u) {
-        const uint4 id4 = *reinterpret_cast<const uint4*>(&s_list[wave][i]);
+      for (uint32_t i = 0; i < cnt; i += (uint32_t)STEP) {
+        uint32_t k[STEP];
+        if (STEP == 4) {
+          const uint4 id4 = *reinterpret_cast<const uint4*>(&s_list[wave][i]);
+          k[0] = id4.x;
+          k[1 % STEP] = id4.y;
+          k[2 % STEP] = id4.z;
+          k[3 % STEP] = id4.w;
+        } else {
+          const uint2 id2 = *reinterpret_cast<const uint2*>(&s_list[wave][i]);
+          k[0] = id2.x;
+          k[1 % STEP] = id2.y;
+        }
         // wave-uniform: entries past cnt repeat the first (always valid) one
-        const bool v1 = i + 1u < cnt, v2 = i + 2u < cnt, v3 = i + 3u < cnt;
-        const uint32_t k0 = id4.x, k1 = v1 ? id4.y : id4.x, k2 = v2 ? id4.z : id4.x,
-                       k3 = v3 ? id4.w : id4.x;
-        const Splat S0 = staged<CB>(s_stage, k0), S1 = staged<CB>(s_stage, k1);
-        const Splat S2 = staged<CB>(s_stage, k2), S3 = staged<CB>(s_stage, k3);
-        const float qa = splat_q2(S0, pcx, pcy), qb = splat_q2(S1, pcx, pcy);
-        const float qc = splat_q2(S2, pcx, pcy), qd = splat_q2(S3, pcx, pcy);
-        const float aa = fminf(0.99f, S0.g.w * __builtin_amdgcn_exp2f(qa));
-        const float ab = fminf(0.99f, S1.g.w * __builtin_amdgcn_exp2f(qb));
-        const float ac = fminf(0.99f, S2.g.w * __builtin_amdgcn_exp2f(qc));
-        const float ad = fminf(0.99f, S3.g.w * __builtin_amdgcn_exp2f(qd));
-        blend(S0, qa, aa, true, alive, T, cr, cg, cb);
-        blend(S1, qb, ab, v1, alive, T, cr, cg, cb);
-        blend(S2, qc, ac, v2, alive, T, cr, cg, cb);
-        blend(S3, qd, ad, v3, alive, T, cr, cg, cb);
+        bool v[STEP];
+        Splat S[STEP];
+        float q[STEP], al[STEP];
+#pragma unroll
+        for (int j = 0; j < STEP; ++j) {
+          v[j] = i + (uint32_t)j < cnt;
+          S[j] = staged<CB>(s_stage, v[j] ? k[j] : k[0]);
+          q[j] = splat_q(S[j], pcx, pcy);
+          al[j] = fminf(0.99f, S[j].g.w * __builtin_amdgcn_exp2f(q[j] * LOG2E));
+        }
+#pragma unroll
+        for (int j = 0; j < STEP; ++j) blend(S[j], q[j], al[j], v[j], alive, T, cr, cg, cb);
         if (__ballot(alive) == 0ull) break;
       }
     }
@@ -288,10 +317,12 @@ void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* pro
     A.order = nullptr;  // tile order (8K frames: one block would take too long)
   }
   if (prof) prof->mark("k_composite", s);
+  constexpr int ST = COMPOSITE_STEP;
+  constexpr bool PF = COMPOSITE_PREFETCH;
   if (batch == 64u)
-    hipLaunchKernelGGL(k_composite<64>, dim3(total), dim3(CT), 0, s, A);
+    hipLaunchKernelGGL((k_composite<64, ST, PF>), dim3(total), dim3(CT), 0, s, A);
   else if (batch == 128u)
-    hipLaunchKernelGGL(k_composite<128>, dim3(total), dim3(CT), 0, s, A);
+    hipLaunchKernelGGL((k_composite<128, ST, PF>), dim3(total), dim3(CT), 0, s, A);
   else
-    hipLaunchKernelGGL(k_composite<256>, dim3(total), dim3(CT), 0, s, A);
+    hipLaunchKernelGGL((k_composite<256, ST, PF>), dim3(total), dim3(CT), 0, s, A);
 }

@@ -171,7 +171,20 @@ struct FrameArena {
   uint32_t* sorted_ids = nullptr;
   bool capture = false;
   bool captured = false;
-  cgs_tuning tune = {16384u, 8u, 256u, 1u};
+  cgs_tuning tune = {16384u, 8u, 256u, 1u, 0u};
+  // bucket mode (k_tilesort.hip): histograms, bases, buckets, the sort's
+  // tables and control words, and the launch plan of the segmented sort
+  DevBuf bk_hist, bk_base, bk_bucket, bk_alt, bk_tab, bk_ctl;
+  bool bucket = false;             // the last Stage 3 ran in bucket mode
+  uint32_t bk_chunk_blocks = 0;    // chunk blocks / merge rounds it launched
+  uint32_t bk_rounds = 0;
+  bool bk_planned = false;         // the plan of the next launch: chunks / rounds
+  uint32_t bk_need_chunks = 0, bk_need_rounds = 0;
+  // what the last frame of each geometry (tiles, splat slots) needed; a view
+  // batch sets the plan of each view itself (plan_external)
+  std::vector<std::pair<uint64_t, std::pair<uint32_t, uint32_t>>> bk_plans;
+  uint64_t bk_key = 0;
+  bool plan_external = false;
   // projected records read by Stages 3-4 when they live outside the arena
   // (the view batch's multi-view projection); null: geom / color / extents
   const float* geom_in = nullptr;
@@ -179,11 +192,14 @@ struct FrameArena {
   const uint32_t* extents_in = nullptr;
 
   void release_all() {
-    DevBuf* bufs[] = {&geom, &color, &extents, &e_keys, &e_ids, &e_depths, &e_alt_k,
-                      &e_alt_v, &e_perm, &ranges, &out_rgba, &cnt_dbg, &off_dbg, &comp_order,
-                      &hitm, &ctrl3};
+    DevBuf* bufs[] = {&geom,    &color,  &extents, &e_keys,   &e_ids,     &e_depths,
+                      &e_alt_k, &e_alt_v, &e_perm,  &ranges,   &out_rgba,  &cnt_dbg,
+                      &off_dbg, &comp_order, &hitm, &ctrl3,    &bk_hist,   &bk_base,
+                      &bk_bucket, &bk_alt, &bk_tab, &bk_ctl};
     for (DevBuf* b : bufs) release(*b);
     entry_cap = 0;
+    bk_planned = false;
+    bk_plans.clear();
   }
 };
 
@@ -359,6 +375,187 @@ void init_entry_cap(FrameArena& a, uint32_t max_g) {
   a.entry_cap = (uint32_t)guess;
 }
 
+uint32_t ceil_log2_u32(uint32_t v) { return v <= 1u ? 0u : 32u - (uint32_t)__builtin_clz(v - 1u); }
+
+// control words of the bucket mode: [0, 4) stats (max tile entries, merge
+// rounds, entries, chunks), [4] ticket, then one u64 look-back status per
+// 128-tile group
+size_t bucket_ctl_words(uint32_t tiles_pad) { return 8u + 2u * (size_t)(tiles_pad / 128u); }
+
+// buffer sizes of the bucket mode for max_g splat slots, T tiles and an entry
+// capacity of cap
+struct BucketSizes {
+  uint32_t tiles_pad, parts, subs, grid, windows, win_slots, chunk_slots;
+  size_t tab_words, ctl_words;
+};
+BucketSizes bucket_sizes(uint32_t max_g, uint32_t T, uint32_t cap) {
+  BucketSizes z;
+  z.tiles_pad = (T + 127u) / 128u * 128u;
+  z.parts = cgs_bin_num_parts(max_g);
+  // about 512 blocks; one block covers at most 127 x 512 < 65536 splats (the
+  // u16 histogram halves)
+  z.subs = (z.parts + 511u) / 512u;
+  if (z.subs < 1u) z.subs = 1u;
+  if (z.subs > 127u) z.subs = 127u;
+  z.grid = (z.parts + z.subs - 1u) / z.subs;
+  z.windows = cgs_seg_sort_windows(cap);
+  z.win_slots = z.windows + 1u;
+  z.chunk_slots = cap / CGS_SEG_CHUNK + cap / CGS_SEG_WINDOW + 2u;
+  z.tab_words = 2u * ((size_t)T + 1u) + z.win_slots + z.chunk_slots;
+  z.ctl_words = bucket_ctl_words(z.tiles_pad);
+  return z;
+}
+
+bool bucket_mode(const FrameArena& a, uint32_t T, uint32_t max_g) {
+  return a.tune.tile_sort && T <= CGS_BUCKET_MAX_TILES && cgs_bin_num_parts(max_g) != 0;
+}
+
+int size_bucket(FrameArena& a, uint32_t max_g, uint32_t T) {
+  const uint32_t cap = a.entry_cap;
+  const BucketSizes z = bucket_sizes(max_g, T, cap);
+  TRY(ensure_words(a.bk_bucket, (size_t)cap * 2u));
+  TRY(ensure_words(a.bk_alt, (size_t)cap * 2u));
+  TRY(ensure_words(a.bk_hist, (size_t)z.grid * (z.tiles_pad / 2u)));
+  TRY(ensure_words(a.bk_base, (size_t)z.grid * z.tiles_pad));
+  TRY(ensure_words(a.bk_tab, z.tab_words));
+  TRY(ensure_words(a.bk_ctl, z.ctl_words));
+  return CGS_OK;
+}
+
+// Stage 3 in bucket mode (k_count with tile histograms, k_bucket_scan,
+// k_emit into tile buckets, k_seg_sort and its merge rounds): the same keys,
+// ids and ranges as the LSD sort.  The segmented sort's grid and merge rounds
+// follow the previous frame's needs; the host re-runs a frame that needed
+// more (bucket_rerun).
+int launch_stage3_bucket(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& g,
+                         float near_plane, uint32_t num_rest, const uint32_t* split_count,
+                         uint32_t max_g) {
+  const uint32_t cap = a.entry_cap;
+  const uint32_t T = g.T;
+  const BucketSizes z = bucket_sizes(max_g, T, cap);
+  const uint32_t tiles_pad = z.tiles_pad, parts = z.parts, subs = z.subs, grid = z.grid;
+  const uint32_t windows = z.windows, win_slots = z.win_slots, chunk_slots = z.chunk_slots;
+  const size_t ctl_words = z.ctl_words;
+  TRY(ensure_words(a.e_keys, cap));
+  TRY(ensure_words(a.e_ids, cap));
+  TRY(ensure_words(a.ranges, (size_t)T * 2));
+  TRY(size_bucket(a, max_g, T));
+  const FrameLayout L = frame_layout(max_g, cap);
+  TRY(ensure_words(a.ctrl3, L.words));
+  uint32_t* ctrl = P<uint32_t>(a.ctrl3);
+  TRY(ensure_words(a.cnt_dbg, max_g));
+  TRY(ensure_words(a.hitm, cgs_bin_mask_words(max_g)));
+  a.captured = false;
+  if (a.capture) {
+    TRY(ensure_words(a.off_dbg, max_g));
+    a.captured = true;
+  }
+  uint32_t tile_bits = T <= 1 ? 1u : 32u - (uint32_t)__builtin_clz(T - 1u);
+  uint32_t depth_bits = 32u - tile_bits;
+  if (depth_bits > 28u) depth_bits = 28u;
+  a.depth_bits = depth_bits;
+
+  uint32_t* tab = P<uint32_t>(a.bk_tab);
+  uint32_t* tile_start = tab;
+  uint32_t* chunk_start = tab + (T + 1u);
+  uint32_t* win_first = tab + 2u * (T + 1u);
+  uint32_t* chunk_tile = win_first + win_slots;
+  uint32_t* bctl = P<uint32_t>(a.bk_ctl);
+
+  CgsBinArgs B;
+  memset(&B, 0, sizeof(B));
+  // cleared by k_count: the frame's scalars, and the bucket control words
+  B.zero_at[0] = ctrl;
+  B.zero_len[0] = (uint32_t)L.block_sums;
+  B.zero_at[1] = bctl;
+  B.zero_len[1] = (uint32_t)ctl_words;
+  B.zero_at[2] = bctl;
+  B.zero_len[2] = 0u;
+  B.geom = a.geom_in ? a.geom_in : P<float>(a.geom);
+  B.extents = a.extents_in ? a.extents_in : P<uint32_t>(a.extents);
+  B.split_count = split_count;
+  B.num_rest = num_rest;
+  B.max_gaussians = max_g;
+  B.width = g.W;
+  B.height = g.H;
+  B.tiles_x = g.tiles_x;
+  B.tiles_y = g.tiles_y;
+  B.mode64 = 0u;
+  float np = near_plane < 1e-6f ? 1e-6f : near_plane;
+  B.base_depth_bits = fbits(np) & 0x7F800000u;
+  B.depth_bits = depth_bits;
+  B.cap = cap;
+  B.ids = P<uint32_t>(a.e_ids);  // not written in bucket mode
+  B.counts = P<uint32_t>(a.cnt_dbg);
+  B.offsets_out = a.capture ? P<uint32_t>(a.off_dbg) : nullptr;
+  cgs_bin_masks(P<uint32_t>(a.hitm), max_g, &B.small_masks, &B.chunk_masks);
+  B.total = ctrl + L.total;
+  B.block_sums = ctrl + L.block_sums;
+  B.block_surv = ctrl + L.block_surv;
+  B.hist_partial = nullptr;
+  B.sort_bits = tile_bits + depth_bits > 32u ? 32u : tile_bits + depth_bits;
+  B.tile_bits = tile_bits;
+  B.digit_bits = 8u;
+  B.tile_hist = P<uint32_t>(a.bk_hist);
+  B.tile_base = P<uint32_t>(a.bk_base);
+  B.bucket = P<uint64_t>(a.bk_bucket);
+  B.tiles_pad = tiles_pad;
+  B.subs = subs;
+  B.parts = parts;
+
+  CgsBucketArgs K;
+  K.hist = P<uint32_t>(a.bk_hist);
+  K.rows = grid;
+  K.tiles = T;
+  K.tiles_pad = tiles_pad;
+  K.cap = cap;
+  K.big = CGS_SEG_WINDOW;
+  K.window = CGS_SEG_WINDOW;
+  K.base = P<uint32_t>(a.bk_base);
+  K.tile_start = tile_start;
+  K.chunk_start = chunk_start;
+  K.ranges = P<uint32_t>(a.ranges);
+  K.win_first = win_first;
+  K.win_slots = win_slots;
+  K.chunk_tile = chunk_tile;
+  K.chunk_slots = chunk_slots;
+  K.stats = bctl;
+  K.ticket = bctl + 4;
+  K.status = reinterpret_cast<uint64_t*>(bctl + 8);
+  cgs_launch_bin(B, s, prof, &K);
+
+  CgsSegSortArgs S;
+  S.bucket = P<uint64_t>(a.bk_bucket);
+  S.alt = P<uint64_t>(a.bk_alt);
+  S.keys = P<uint32_t>(a.e_keys);
+  S.ids = P<uint32_t>(a.e_ids);
+  S.tile_start = tile_start;
+  S.chunk_start = chunk_start;
+  S.win_first = win_first;
+  S.chunk_tile = chunk_tile;
+  S.stats = bctl;
+  S.tiles = T;
+  S.cap = cap;
+  S.window = CGS_SEG_WINDOW;
+  S.windows = windows;
+  S.big = CGS_SEG_WINDOW;
+  S.chunk_slots = chunk_slots;
+  // the previous frame's chunks with a quarter more room, and its merge rounds
+  uint32_t cb = chunk_slots, rounds = 3u;
+  if (a.bk_planned) {
+    const uint64_t want = (uint64_t)a.bk_need_chunks + a.bk_need_chunks / 4u + 64u;
+    cb = want < chunk_slots ? (uint32_t)want : chunk_slots;
+    rounds = a.bk_need_rounds;
+  }
+  a.bk_chunk_blocks = cb;
+  a.bk_rounds = rounds;
+  cgs_launch_seg_sort(S, cb, rounds, s, prof);
+  a.sorted_keys = P<uint32_t>(a.e_keys);
+  a.sorted_ids = P<uint32_t>(a.e_ids);
+  HIPCHK(hipGetLastError());
+  return CGS_OK;
+}
+
 // Stage 3 on the projected splats in the arena (geom/extents, max_g slots,
 // active = num_rest + *split_count).  Launch-only; the caller checks the
 // total against the capacity after synchronising.
@@ -370,6 +567,20 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   a.mode64 = g.T > 65535u;
   init_entry_cap(a, max_g);
   const uint32_t cap = a.entry_cap;
+  const bool bucket = bucket_mode(a, g.T, max_g);
+  a.bucket = bucket;
+  if (bucket && !a.plan_external) {
+    // the plan this geometry's last frame left
+    a.bk_key = ((uint64_t)g.T << 32) | max_g;
+    a.bk_planned = false;
+    for (const auto& p : a.bk_plans)
+      if (p.first == a.bk_key) {
+        a.bk_planned = true;
+        a.bk_need_chunks = p.second.first;
+        a.bk_need_rounds = p.second.second;
+      }
+  }
+  if (bucket) return launch_stage3_bucket(a, s, prof, g, near_plane, num_rest, split_count, max_g);
   TRY(ensure_words(a.e_keys, cap));
   TRY(ensure_words(a.e_ids, cap));
   TRY(ensure_words(a.e_alt_k, cap));
@@ -441,6 +652,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   if (sort_bits > 32) sort_bits = 32;
   B.sort_bits = sort_bits;
   B.tile_bits = tile_bits;
+  B.subs = 1u;
+  B.parts = cgs_bin_num_parts(max_g);
   const uint32_t db = a.mode64 ? 8u : a.tune.radix_digit_bits;
   B.digit_bits = db;
   cgs_launch_bin(B, s, prof);
@@ -542,11 +755,39 @@ int launch_stage4(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   return CGS_OK;
 }
 
+// After a Stage 3 in bucket mode: records the chunks and merge rounds the
+// frame needed (stats = the bucket control words read back); true when it
+// needed more than were launched, so that its Stage 3 must run again.
+bool bucket_rerun(FrameArena& a, const uint32_t* stats) {
+  if (!a.bucket) return false;
+  const bool more = stats[3] > a.bk_chunk_blocks || stats[1] > a.bk_rounds;
+  a.bk_need_chunks = stats[3];
+  a.bk_need_rounds = stats[1];
+  a.bk_planned = true;
+  if (!a.plan_external) {
+    bool found = false;
+    for (auto& p : a.bk_plans)
+      if (p.first == a.bk_key) {
+        p.second = {stats[3], stats[1]};
+        found = true;
+      }
+    if (!found) {
+      if (a.bk_plans.size() >= 64) a.bk_plans.erase(a.bk_plans.begin());
+      a.bk_plans.push_back({a.bk_key, {stats[3], stats[1]}});
+    }
+  }
+  return more;
+}
+
 // read the entry total, the survivors and the split count (one small copy + wait):
-// h_counters[2] = entries, [3] = survivors, [4] = split children
+// h_counters[2] = entries, [3] = survivors, [4] = split children, [12..16) the
+// bucket mode's stats
 int read_counters(cgs_context* c, bool with_split) {
   const uint32_t* ctrl = P<uint32_t>(c->fa.ctrl3);
   HIPCHK(hipMemcpyAsync(c->h_counters, ctrl, 4 * 4, hipMemcpyDeviceToHost, c->stream));
+  if (c->fa.bucket)
+    HIPCHK(hipMemcpyAsync(c->h_counters + 12, c->fa.bk_ctl.p, 4 * 4, hipMemcpyDeviceToHost,
+                          c->stream));
   if (with_split && c->ctrl1.p)
     HIPCHK(hipMemcpyAsync(c->h_counters + 4, c->ctrl1.p, 4, hipMemcpyDeviceToHost, c->stream));
   else
@@ -735,6 +976,7 @@ int check_tuning(const cgs_tuning* t) {
   if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u)
     return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128 or 256");
   if (t->tile_order > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_order must be 0 or 1");
+  if (t->tile_sort > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_sort must be 0 or 1");
   return CGS_OK;
 }
 
@@ -1073,12 +1315,16 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   uint32_t survivors = use_cache ? c->s3_survivors : c->h_counters[3];
   if (!use_cache) {
     uint32_t total = c->h_counters[2];
-    // the tile-entry arena overflowed: grow and re-run Stages 3-4
+    // the tile-entry arena overflowed (grow it), or the segmented sort needed
+    // more chunk blocks or merge rounds than it launched: re-run Stages 3-4
     int reruns = 0;
-    while (total > c->fa.entry_cap && reruns < 4) {
-      uint64_t want = (uint64_t)total + total / 2;
-      if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
-      c->fa.entry_cap = (uint32_t)want;
+    bool more = bucket_rerun(c->fa, c->h_counters + 12);
+    while ((total > c->fa.entry_cap || more) && reruns < 4) {
+      if (total > c->fa.entry_cap) {
+        uint64_t want = (uint64_t)total + total / 2;
+        if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
+        c->fa.entry_cap = (uint32_t)want;
+      }
       TRY(launch_stage3(c->fa, s, &c->prof, g, cam->near_plane, N, F.split_count, max_g));
       if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
       TRY(launch_stage4(c->fa, s, &c->prof, g, comp));
@@ -1086,8 +1332,11 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
       TRY(read_counters(c, c->has_cages));
       total = c->h_counters[2];
       survivors = c->h_counters[3];
+      more = bucket_rerun(c->fa, c->h_counters + 12);
       reruns++;
     }
+    if (total > c->fa.entry_cap || more)
+      return fail(CGS_ERR_EXECUTION, "Stage 3 did not settle after 4 re-runs (internal error)");
     c->n_entries = total;
     c->s3_key = key;
     c->s3_valid = true;
@@ -1137,17 +1386,29 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 
 // ---- launch tuning ----
 
-uint32_t cgs_tuning_candidates(void) { return 60u; }
+// the 60 choices of the LSD radix sort (partition x digit width x compositor
+// batch x tile order), then the 6 of the bucket mode (batch x order)
+uint32_t cgs_tuning_candidates(void) { return 66u; }
 
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
-  if (!out || i >= 60u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  if (!out || i >= 66u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
   static const uint32_t parts[5] = {16384u, 12288u, 10240u, 8192u, 4096u},
                         digits[2] = {8u, 4u}, batches[3] = {256u, 128u, 64u},
                         orders[2] = {1u, 0u};
+  if (i >= 60u) {
+    i -= 60u;
+    out->radix_partition_keys = 16384u;
+    out->radix_digit_bits = 8u;
+    out->composite_batch = batches[i % 3u];
+    out->tile_order = orders[i / 3u];
+    out->tile_sort = 1u;
+    return CGS_OK;
+  }
   out->radix_partition_keys = parts[i % 5u];
   out->radix_digit_bits = digits[(i / 5u) % 2u];
   out->composite_batch = batches[(i / 10u) % 3u];
   out->tile_order = orders[i / 30u];
+  out->tile_sort = 0u;
   return CGS_OK;
 }
 
@@ -1421,7 +1682,9 @@ int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
     TRY(read_counters(c, false));
     const uint32_t total = c->h_counters[2];
     c->n_entries = total;
-    if (total <= c->fa.entry_cap) break;
+    const bool more = bucket_rerun(c->fa, c->h_counters + 12);
+    if (total <= c->fa.entry_cap && !more) break;
+    if (total <= c->fa.entry_cap) continue;
     uint64_t want = (uint64_t)total + total / 2;
     if (want >= (1ull << 30)) return fail(CGS_ERR_EXECUTION, "tile entries exceed 2^30");
     c->fa.entry_cap = (uint32_t)want;
@@ -2204,7 +2467,8 @@ struct cgs_batch {
   std::vector<hipEvent_t> lane_done;
   std::vector<FrameArena> lanes;
   DevBuf fb;        // every view's framebuffer, 256-byte aligned slices
-  DevBuf counters;  // [view][4]: control words 0-3 of the view's Stage 3
+  DevBuf counters;  // [view][8]: control words 0-3 of the view's Stage 3, then
+                    // its bucket-mode stats (zero in the LSD mode)
   uint32_t* h_counters = nullptr;
   size_t h_counter_words = 0;
   std::vector<size_t> fb_off;  // words
@@ -2228,6 +2492,10 @@ struct cgs_batch {
   // deal the views to lanes by their measured weight
   uint64_t views_hash = 0;
   std::vector<uint32_t> last_entries;
+  // per view, the segmented sort's needs in the last replay (bucket mode)
+  // and what the captured graph launches
+  std::vector<uint8_t> plan_known, view_bucket;
+  std::vector<uint32_t> plan_chunks, plan_rounds, launched_chunks, launched_rounds;
 };
 
 namespace {
@@ -2260,6 +2528,7 @@ int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_
   TRY(ensure_words(a.ranges, (size_t)T * 2));
   TRY(ensure_words(a.comp_order, T));
   TRY(ensure_words(a.ctrl3, frame_layout(max_g, cap).words));
+  if (bucket_mode(a, T, max_g)) TRY(size_bucket(a, max_g, T));
   return CGS_OK;
 }
 
@@ -2277,11 +2546,12 @@ int batch_enqueue_view(cgs_batch* b, cgs_context* c, const SceneFrame& F, FrameA
                        uint32_t v) {
   const FrameGeom g = frame_geom(comp);
   uint32_t* out = P<uint32_t>(b->fb) + b->fb_off[v];
-  uint32_t* slot = P<uint32_t>(b->counters) + 4 * (size_t)v;
+  uint32_t* slot = P<uint32_t>(b->counters) + 8 * (size_t)v;
+  b->view_bucket[v] = 0;
   if (F.N == 0) {
     // background frame (gpu_state.rs:658-704)
     TRY(background_arena(a, g, s));
-    cgs_launch_fill_u32(slot, 0u, 4, s);
+    cgs_launch_fill_u32(slot, 0u, 8, s);
     return launch_stage4(a, s, &b->noprof, g, comp, out);
   }
   if (b->shared_projection) {
@@ -2293,9 +2563,22 @@ int batch_enqueue_view(cgs_batch* b, cgs_context* c, const SceneFrame& F, FrameA
   } else {
     TRY(stage2(c, F, a, s, &b->noprof, cam, comp, prm));
   }
+  // the segmented sort's plan is this view's, from its last replay
+  a.plan_external = true;
+  a.bk_planned = b->plan_known[v] != 0;
+  a.bk_need_chunks = b->plan_chunks[v];
+  a.bk_need_rounds = b->plan_rounds[v];
   int rc = launch_stage3(a, s, &b->noprof, g, cam->near_plane, F.N, F.split_count, F.max_g);
   if (rc == CGS_OK) {
     cgs_launch_copy_words(P<uint32_t>(a.ctrl3), slot, 4, s);
+    if (a.bucket) {
+      cgs_launch_copy_words(P<uint32_t>(a.bk_ctl), slot + 4, 4, s);
+      b->view_bucket[v] = 1;
+      b->launched_chunks[v] = a.bk_chunk_blocks;
+      b->launched_rounds[v] = a.bk_rounds;
+    } else {
+      cgs_launch_fill_u32(slot + 4, 0u, 4, s);
+    }
     rc = launch_stage4(a, s, &b->noprof, g, comp, out);
   }
   a.geom_in = a.color_in = nullptr;
@@ -2486,15 +2769,23 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     words += ((size_t)G[v].padded_row * G[v].H + 63) / 64 * 64;
   }
   TRY(ensure_words(b->fb, words));
-  TRY(ensure_words(b->counters, 4 * (size_t)(n_views ? n_views : 1)));
-  if (b->h_counter_words < 4 * (size_t)n_views) {
+  TRY(ensure_words(b->counters, 8 * (size_t)(n_views ? n_views : 1)));
+  if (b->h_counter_words < 8 * (size_t)n_views) {
     if (b->h_counters) (void)hipHostFree(b->h_counters);
     b->h_counters = nullptr;
     b->h_counter_words = 0;
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_counters), 16 * (size_t)n_views,
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_counters), 32 * (size_t)n_views,
                          hipHostMallocDefault));
-    b->h_counter_words = 4 * (size_t)n_views;
+    b->h_counter_words = 8 * (size_t)n_views;
   }
+  if (b->plan_known.size() != n_views || b->views_hash != vh) {
+    b->plan_known.assign(n_views, 0);
+    b->plan_chunks.assign(n_views, 0);
+    b->plan_rounds.assign(n_views, 0);
+  }
+  b->view_bucket.assign(n_views, 0);
+  b->launched_chunks.assign(n_views, 0);
+  b->launched_rounds.assign(n_views, 0);
   if (n_views == 0) {
     if (stats) *stats = St;
     return CGS_OK;
@@ -2583,14 +2874,18 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
       h = fnv_buf(h, c->split_meta);
     }
     for (const FrameArena& a : b->lanes) {
-      const DevBuf* bufs[] = {&a.geom, &a.color, &a.extents, &a.cnt_dbg, &a.e_keys, &a.e_ids,
-                              &a.e_alt_k, &a.e_alt_v, &a.e_depths, &a.e_perm, &a.ranges,
-                              &a.comp_order, &a.hitm, &a.ctrl3};
+      const DevBuf* bufs[] = {&a.geom,      &a.color,     &a.extents, &a.cnt_dbg, &a.e_keys,
+                              &a.e_ids,     &a.e_alt_k,   &a.e_alt_v, &a.e_depths, &a.e_perm,
+                              &a.ranges,    &a.comp_order, &a.hitm,   &a.ctrl3,   &a.bk_hist,
+                              &a.bk_base,   &a.bk_bucket, &a.bk_alt,  &a.bk_tab,  &a.bk_ctl};
       for (const DevBuf* d : bufs) h = fnv_buf(h, *d);
       h = fnv_u64(h, a.entry_cap);
     }
     h = fnv_buf(h, b->fb);
     h = fnv_buf(h, b->counters);
+    h = fnv_bytes(h, b->plan_known.data(), n_views);
+    h = fnv_bytes(h, b->plan_chunks.data(), 4 * (size_t)n_views);
+    h = fnv_bytes(h, b->plan_rounds.data(), 4 * (size_t)n_views);
     h = fnv_u64(h, b->shared_projection ? 1u : 0u);
     h = fnv_buf(h, b->proj);
     h = fnv_buf(h, b->vproj);
@@ -2607,7 +2902,7 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     HIPCHK(hipEventRecord(b->t0, b->stream));
     HIPCHK(hipGraphLaunch(b->exec, b->stream));
     HIPCHK(hipEventRecord(b->t1, b->stream));
-    HIPCHK(hipMemcpyAsync(b->h_counters, b->counters.p, 16 * (size_t)n_views,
+    HIPCHK(hipMemcpyAsync(b->h_counters, b->counters.p, 32 * (size_t)n_views,
                           hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     St.gpu_ms += elapsed(b->t0, b->t1);
@@ -2615,18 +2910,28 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
     std::vector<uint32_t> need(L, 0);
     bool overflow = false;
     uint64_t entries = 0;
+    bool replan = false;
     for (uint32_t v = 0; v < n_views; ++v) {
-      const uint32_t total = b->h_counters[4 * v + 2];
+      const uint32_t total = b->h_counters[8 * v + 2];
       entries += total;
       FrameArena& a = b->lanes[b->view_lane[v]];
       need[b->view_lane[v]] = std::max(need[b->view_lane[v]], total);
       if (total > a.entry_cap) overflow = true;
+      if (b->view_bucket[v]) {
+        // the segmented sort's needs: a view that needed more chunk blocks or
+        // merge rounds than its graph launched is planned anew and replayed
+        const uint32_t* st = b->h_counters + 8 * v + 4;
+        if (st[3] > b->launched_chunks[v] || st[1] > b->launched_rounds[v]) replan = true;
+        b->plan_known[v] = 1;
+        b->plan_chunks[v] = st[3];
+        b->plan_rounds[v] = st[1];
+      }
     }
     St.total_tile_entries = entries;
     b->views_hash = vh;
     b->last_entries.resize(n_views);
-    for (uint32_t v = 0; v < n_views; ++v) b->last_entries[v] = b->h_counters[4 * v + 2];
-    if (!overflow) break;
+    for (uint32_t v = 0; v < n_views; ++v) b->last_entries[v] = b->h_counters[8 * v + 2];
+    if (!overflow && !replan) break;
     if (attempt >= 4) return fail(CGS_ERR_EXECUTION, "tile-entry arena still overflowing");
     for (uint32_t k = 0; k < L; ++k) {
       FrameArena& a = b->lanes[k];
@@ -2681,10 +2986,10 @@ int cgs_batch_readback_rgba8(cgs_batch* b, uint32_t view, uint8_t* dst, uint32_t
 int cgs_batch_view_counts(cgs_batch* b, uint32_t view, uint32_t* tile_entries,
                           uint32_t* survivors) {
   if (!b) return fail(CGS_ERR_INVALID_ARG, "null batch");
-  if (view >= b->fb_off.size() || 4 * (size_t)view >= b->h_counter_words)
+  if (view >= b->fb_off.size() || 8 * (size_t)view >= b->h_counter_words)
     return fail(CGS_ERR_INVALID_ARG, "view index out of range");
-  if (tile_entries) *tile_entries = b->h_counters[4 * view + 2];
-  if (survivors) *survivors = b->h_counters[4 * view + 3];
+  if (tile_entries) *tile_entries = b->h_counters[8 * view + 2];
+  if (survivors) *survivors = b->h_counters[8 * view + 3];
   return CGS_OK;
 }
 

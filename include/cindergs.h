@@ -203,7 +203,7 @@ int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, ui
  * reference fixes its shapes in code (radix_sort.rs:292-368: 4-bit digits,
  * 256-key blocks; compositor.rs:721-883: batches of 256); here they are
  * per-context settings that cgs_autotune picks by timing.  Defaults:
- * {16384, 8, 256, 1}. */
+ * {16384, 8, 256, 1, 0}; the radix fields matter only with tile_sort = 0. */
 typedef struct cgs_tuning {
   uint32_t radix_partition_keys; /* 16384, 12288, 10240, 8192 or 4096 keys per
                                     sort partition (1024 threads x 16 or 12
@@ -211,6 +211,11 @@ typedef struct cgs_tuning {
   uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
   uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step */
   uint32_t tile_order;           /* 1: heaviest tiles composited first */
+  uint32_t tile_sort;            /* 1: entries written into per-tile buckets and each
+                                    tile sorted by (key, splat id) in LDS; 0: one
+                                    stable LSD radix sort of all entries (the
+                                    reference's method).  Frames of more than 32768
+                                    tiles always take the radix sort. */
 } cgs_tuning;
 
 int cgs_set_tuning(cgs_context* ctx, const cgs_tuning* tuning);

@@ -58,17 +58,23 @@ def test_no_device_maps_to_noadapter():
 
 
 def test_tuning_candidates_cover_every_knob():
-    """The autotuner's sweep (host-only entry points, no GPU needed): every
-    combination of partition size, digit width, compositor batch and tile
-    order exactly once."""
+    """The autotuner's sweep (host-only entry points, no GPU needed): the
+    bucket sort with every compositor batch and tile order, and the LSD radix
+    sort with every combination of partition size, digit width, compositor
+    batch and tile order, each exactly once."""
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
     n = tuning_candidates_count()
-    combos = {tuple(tuning_candidate(i).as_dict().values()) for i in range(n)}
-    assert n == len(combos) == 5 * 2 * 3 * 2
-    assert {c[0] for c in combos} == {16384, 12288, 10240, 8192, 4096}
-    assert {c[1] for c in combos} == {8, 4}
-    assert {c[2] for c in combos} == {256, 128, 64}
-    assert {c[3] for c in combos} == {0, 1}
+    combos = [tuple(tuning_candidate(i).as_dict().values()) for i in range(n)]
+    assert n == len(set(combos)) == 6 + 5 * 2 * 3 * 2
+    bucket = [c for c in combos if c[4] == 1]
+    radix = [c for c in combos if c[4] == 0]
+    assert len(bucket) == 6 and {(c[2], c[3]) for c in bucket} == {
+        (b, o) for b in (256, 128, 64) for o in (0, 1)}
+    assert {c[0] for c in radix} == {16384, 12288, 10240, 8192, 4096}
+    assert {c[1] for c in radix} == {8, 4}
+    assert {c[2] for c in radix} == {256, 128, 64}
+    assert {c[3] for c in radix} == {0, 1}
     assert tuning_candidate(0).as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
-                                             "composite_batch": 256, "tile_order": 1}
+                                             "composite_batch": 256, "tile_order": 1,
+                                             "tile_sort": 0}

@@ -1,7 +1,8 @@
 """Launch tuning (cgs_tuning, cgs_autotune) on the GPU.
 
-Bar: every tuning candidate (radix partition 16384/10240/8192/4096 keys, 8- or 4-bit
-digits, compositor batch 256/128/64, heaviest-tiles-first order on/off) gives
+Bar: every tuning candidate (the per-tile bucket sort, or the LSD radix sort
+with partitions of 16384/12288/10240/8192/4096 keys and 8- or 4-bit digits;
+compositor batch 256/128/64, heaviest-tiles-first order on/off) gives
 bit-identical tile counts, scan offsets, sorted keys, sorted ids, tile ranges
 and RGBA8 to the default setting, which the parity tests hold to the oracle;
 the 64-bit key path (8K) likewise for the partition sizes; the autotuner keeps
@@ -43,7 +44,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
     cam, comp = syn.view_orbit(7, 1280, 720)
     default = gpu.get_tuning()
     assert default.as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
-                                 "composite_batch": 256, "tile_order": 1}
+                                 "composite_batch": 256, "tile_order": 1, "tile_sort": 0}
     ref = _frame(gpu, cam, comp, scene.count)
     n = tuning_candidates_count()
     seen = set()
@@ -53,7 +54,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
         print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
         gpu.set_tuning(c)
         _same(ref, _frame(gpu, cam, comp, scene.count), f"candidate {i}")
-    assert len(seen) == n == 60
+    assert len(seen) == n == 66
     gpu.set_tuning(default)
 
 
@@ -92,6 +93,6 @@ def test_invalid_tuning_is_rejected(gpu):
     from cindergs_amd._lib import GpuError
 
     for bad in (Tuning(2048, 8, 256, 1), Tuning(12000, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
-                Tuning(16384, 8, 256, 2)):
+                Tuning(16384, 8, 256, 2), Tuning(16384, 8, 256, 1, 2)):
         with pytest.raises(GpuError):
             gpu.set_tuning(bad)
