@@ -50,16 +50,18 @@ def _kernel_key(name: str) -> str:
     return name.split("(")[0].split("<")[0].strip()
 
 
-def pmc_per_dispatch(counter: str, bench_args: list, timeout_s: int = 90) -> dict:
-    """Mean value of one rocprofv3 PMC counter per dispatch, per kernel, over a
-    child run of this script (the program itself after `--`, as rocprofv3
-    requires).  Empty dict when rocprofv3 is absent or the run fails."""
+def pmc_counters(counters: list, bench_args: list, timeout_s: int = 90) -> dict:
+    """Mean value per dispatch of rocprofv3 PMC counters (one pass: at most 8 SQ
+    counters), per kernel, over a child run of this script (the program itself
+    after `--`, as rocprofv3 requires): {counter: {kernel: mean}}, plus
+    "VGPR_Count" and "LDS_Block_Size" from the same rows.  Empty dict when
+    rocprofv3 is absent or the run fails."""
     exe = shutil.which("rocprofv3")
     if exe is None:
         return {}
     out = tempfile.mkdtemp(prefix="cgs_pmc_")
     try:
-        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "c", "--",
+        cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "c", "--",
                sys.executable, os.path.abspath(__file__)] + bench_args
         subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                        timeout=timeout_s, check=True)
@@ -68,14 +70,65 @@ def pmc_per_dispatch(counter: str, bench_args: list, timeout_s: int = 90) -> dic
         for f in files:
             with open(f, newline="") as fh:
                 for row in csv.DictReader(fh):
-                    if row.get("Counter_Name") == counter:
-                        vals.setdefault(_kernel_key(row["Kernel_Name"]), []).append(
-                            float(row["Counter_Value"]))
-        return {k: float(np.mean(v)) for k, v in vals.items()}
+                    k = _kernel_key(row["Kernel_Name"])
+                    vals.setdefault(row["Counter_Name"], {}).setdefault(k, []).append(
+                        float(row["Counter_Value"]))
+                    for col in ("VGPR_Count", "LDS_Block_Size"):
+                        if col in row:
+                            vals.setdefault(col, {}).setdefault(k, []).append(float(row[col]))
+        return {c: {k: float(np.mean(v)) for k, v in d.items()} for c, d in vals.items()}
     except (subprocess.SubprocessError, OSError, KeyError, ValueError):
         return {}
     finally:
         shutil.rmtree(out, ignore_errors=True)
+
+
+def pmc_per_dispatch(counter: str, bench_args: list, timeout_s: int = 90) -> dict:
+    """Mean value of one rocprofv3 PMC counter per dispatch, per kernel."""
+    return pmc_counters([counter], bench_args, timeout_s).get(counter, {})
+
+
+# FP32 VALU issue of MI355X: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (a wave64
+# instruction takes 4 cycles), i.e. the 157.3 TFLOP/s datasheet figure without
+# its factors 2 (packed FP32) and 2 (FMA)
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+
+
+def composite_valu(tuning: str, per_launch_ms: float) -> dict | None:
+    """Stage 4's compute roofline from one rocprofv3 PMC child run of the
+    headline frames: k_composite's VALU instructions, its exp2 instructions
+    (one per pixel-splat evaluation: every lane of a walk step evaluates one
+    splat at its pixel), LDS instructions and bank-conflict cycles, waves, and
+    the occupancy its VGPRs allow."""
+    run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
+           "--no-traffic", "--no-extras", "--batch-views", "0", "--mvc", "0"]
+    run += ["--tuning", tuning] if tuning else ["--no-autotune"]
+    c = pmc_counters(["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_LDS",
+                      "SQ_LDS_BANK_CONFLICT", "SQ_WAVES"], run)
+    k = "k_composite"
+    try:
+        valu = c["SQ_INSTS_VALU"][k]
+        trans = c["SQ_INSTS_VALU_TRANS_F32"][k]
+        vgpr = c.get("VGPR_Count", {}).get(k)
+    except KeyError:
+        return None
+    evals = trans * 64.0
+    lane_ops_s = valu * 64.0 / (per_launch_ms * 1e-3)
+    res = {"pixel_splat_evals": int(evals),
+           "evals_per_s": round(evals / (per_launch_ms * 1e-3), 1),
+           "valu_insts_per_eval": round(valu / max(trans, 1.0), 2),
+           "valu_lane_ops_per_s": round(lane_ops_s, 1),
+           "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+           "valu_frac": round(lane_ops_s / VALU_PEAK_LANE_OPS, 4),
+           "lds_insts": int(c["SQ_INSTS_LDS"][k]),
+           "lds_bank_conflict_cycles": int(c["SQ_LDS_BANK_CONFLICT"][k]),
+           "waves": int(c["SQ_WAVES"][k])}
+    if vgpr:
+        # 512 VGPRs per SIMD lane, allocated in granules of 8; at most 8 waves
+        g = int(-(-vgpr // 8) * 8)
+        res["vgprs"] = int(vgpr)
+        res["occupancy_waves_per_simd"] = min(8, 512 // g)
+    return res
 
 
 def hbm_traffic(kernel: str, calib_mib: int = 256, tuning: str = "") -> dict | None:
@@ -85,7 +138,7 @@ def hbm_traffic(kernel: str, calib_mib: int = 256, tuning: str = "") -> dict | N
     # the headline workload only: the other legs launch the same kernels on
     # other sizes and would enter the per-dispatch mean
     run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
-           "--no-traffic", "--no-extras", "--batch-views", "0"]
+           "--no-traffic", "--no-extras", "--batch-views", "0", "--mvc", "0"]
     run += ["--tuning", tuning] if tuning else ["--no-autotune"]
     cal = ["--calibrate-copy", str(calib_mib)]
     nbytes = float(calib_mib << 20)
@@ -309,11 +362,12 @@ def export_deformed(st, scene, num_tets: int, bind_tet, reps: int = 5):
     return res
 
 
-def mvc_binding(n_points: int, train_steps: int, n_train: int):
+def mvc_binding(n_points: int, train_steps: int, n_train: int, n_err: int = 200_000):
     """Exact MVC binding (weights + gradients) of n_points inside a 642-vertex
     icosphere cage, and the surrogate network trained on exact weights, with
     the residual fallback (tolerance 1e-3 of the cage diagonal).  Output
-    buffers are allocated before the timed calls."""
+    buffers are allocated before the timed calls; the position errors are
+    measured on the first n_err points."""
     import ctypes as C
 
     from cindergs_amd import GpuPipelineState
@@ -369,7 +423,8 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int):
         check(lib.cgs_memcpy_d2h(st._h, _ptr(out), d, out.nbytes))
         return out
 
-    we, wsu, wnet = host(d_w, n_points), host(d_ws, n_points), host(d_wi, n_points)
+    n_e = min(n_err, n_points)
+    we, wsu, wnet = host(d_w, n_e), host(d_ws, n_e), host(d_wi, n_e)
     # deformed-position error under a seeded 0.1 perturbation of the cage vertices
     rng = np.random.Generator(np.random.PCG64(99))
     dv = (cage.vertices + rng.uniform(-0.1, 0.1, cage.vertices.shape)).astype(np.float32)
@@ -386,9 +441,10 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int):
     # the points it keeps)
     A = np.concatenate([np.ones((1, nv)), cage.vertices.T.astype(np.float64)], 0)
     Minv = np.linalg.inv(A @ A.T)
-    rhs = np.concatenate([np.ones((n_points, 1)), pts[:n_points].astype(np.float64)], 1)
+    rhs = np.concatenate([np.ones((n_e, 1)), pts[:n_e].astype(np.float64)], 1)
     wproj = (wnet + ((rhs - wnet.astype(np.float64) @ A.T) @ Minv @ A)).astype(np.float32)
-    res = {"workload": f"{n_points} points in a {nv}-vertex cage, weights + gradients",
+    res = {"workload": f"{n_points} points in a {nv}-vertex cage, weights + gradients; "
+                       f"position errors over the first {n_e} points",
            "exact_ms": round(exact_ms, 2), "surrogate_bind_ms": round(sur_ms, 2),
            "surrogate_inference_ms": round(inf_ms, 2),
            "fallback_fraction": round(fb.value / n_points, 4), "train_steps": train_steps,
@@ -699,13 +755,13 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="render the headline with the default launch shapes")
-    ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER",
+    ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER,SORT",
                     help="render the headline with these launch shapes (no autotune)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 8K, f16, cage-deformation, MVC scene and ingest measurements")
     ap.add_argument("--cage-frames", type=int, default=300)
-    ap.add_argument("--mvc", type=int, default=0, metavar="POINTS",
-                    help="also time exact vs surrogate MVC binding of POINTS points")
+    ap.add_argument("--mvc", type=int, default=4_000_000, metavar="POINTS",
+                    help="time exact vs surrogate MVC binding of POINTS points (0 = skip)")
     ap.add_argument("--mvc-scene", type=int, default=1_073_930, metavar="SPLATS",
                     help="MVC scene deformation of SPLATS bound splats (0 = skip; part of the extras)")
     ap.add_argument("--mvc-scene-only", action="store_true")
@@ -876,6 +932,10 @@ def main() -> int:
         if tr is not None:
             result["roofline"]["traffic"] = int(tr["bytes"])
             result["roofline"]["traffic_pmc"] = {k: tr[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+        if "k_composite" in kern:
+            cv = composite_valu(tstr, kern["k_composite"] / launches["k_composite"])
+            if cv is not None:
+                result["stages"]["stage4"].update(cv)
     if not args.no_extras:
         result["frame_8k"] = frame_8k(scene)
         print(f"# 8K: {result['frame_8k']['ms_per_frame']} ms/frame", file=sys.stderr, flush=True)
