@@ -395,12 +395,17 @@ def export_deformed(st, scene, num_tets: int, bind_tet, reps: int = 5):
     return res
 
 
-def mvc_binding(n_points: int, train_steps: int, n_train: int, n_err: int = 200_000):
+def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.3, 0.55),
+                residual_tol: float = 1e-2, n_err: int = 200_000):
     """Exact MVC binding (weights + gradients) of n_points inside a 642-vertex
-    icosphere cage, and the surrogate network trained on exact weights, with
-    the residual fallback (tolerance 1e-3 of the cage diagonal).  Output
-    buffers are allocated before the timed calls; the position errors are
-    measured on the first n_err points."""
+    icosphere cage against the learned binding: the network (Fourier features
+    -> 4 x 256 -> 642) trained on the far share of MVC's sums (triangles more
+    than `near` away from the point, targets from cgs_mvc_far_share) plus the
+    exactly evaluated near sums, with the residual fallback (tolerance
+    residual_tol of the cage diagonal) to exact MVC.  Output buffers are
+    allocated before the timed calls; the position errors (deformed positions
+    under a seeded 0.1 perturbation of the cage, against exact MVC) are
+    measured on the first n_err points, the Jacobian errors likewise."""
     import ctypes as C
 
     from cindergs_amd import GpuPipelineState
@@ -424,7 +429,6 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, n_err: int = 200_
     d_pts = st.to_device(pts[:n_points])
     d_w, d_g, d_m = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12), dalloc(n_points * 4)
     d_ws, d_gs, d_ms = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12), dalloc(n_points * 4)
-    d_wi, d_gi = dalloc(n_points * nv * 4), dalloc(n_points * nv * 12)
     cv, ct = _ptr(cage.vertices), _ptr(cage.triangles)
     st.synchronize()
     t = time.perf_counter()
@@ -432,60 +436,95 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, n_err: int = 200_
     st.synchronize()
     exact_ms = 1000 * (time.perf_counter() - t)
     d_tr = st.to_device(pts[n_points:])
-    d_wt = dalloc(n_train * nv * 4)
-    check(lib.cgs_mvc_bind_exact(st._h, cv, nv, ct, nt, d_tr.ptr, n_train, d_wt, None, None))
+    d_qt = dalloc(n_train * nv * 4)
+    t = time.perf_counter()
+    check(lib.cgs_mvc_far_share(st._h, cv, nv, ct, nt, near[0], near[1], d_tr.ptr, n_train, d_qt))
+    st.synchronize()
+    targets_s = time.perf_counter() - t
     net = MvcSurrogate(st, nv, fourier_sigma=1.0, seed=7)
+    net.set_near_field(*near)
     st.synchronize()
     t = time.perf_counter()
-    check(lib.cgs_mlp_train(net._h, d_tr.ptr, d_wt, n_train, train_steps, 8192, 1e-3, None))
+    segments = 8
+    losses = []
+    for k in range(segments):
+        lr = 1e-3 * (0.51 + 0.49 * np.cos(np.pi * k / segments))
+        out = np.zeros(train_steps // segments, np.float32)
+        check(lib.cgs_mlp_train(net._h, d_tr.ptr, d_qt, n_train, train_steps // segments, 8192,
+                                float(lr), _ptr(out)))
+        losses.append(out)
     st.synchronize()
     train_s = time.perf_counter() - t
     fb = C.c_uint64()
+    # untimed first call (device buffers of the network's chunks), then timed
+    check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, min(n_points, 65536),
+                           residual_tol, d_ws, d_gs, d_ms, C.byref(fb)))
+    st.synchronize()
     t = time.perf_counter()
-    check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, n_points, 1e-3, d_ws, d_gs,
-                           d_ms, C.byref(fb)))
+    check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, n_points, residual_tol, d_ws,
+                           d_gs, d_ms, C.byref(fb)))
     st.synchronize()
     sur_ms = 1000 * (time.perf_counter() - t)
-    t = time.perf_counter()
-    check(lib.cgs_mlp_infer(net._h, d_pts.ptr, n_points, d_wi, d_gi))
-    st.synchronize()
-    inf_ms = 1000 * (time.perf_counter() - t)
+    n_e = min(n_err, n_points)
 
-    def host(d, n):
-        out = np.empty((n, nv), np.float32)
+    def host(d, n, k=1):
+        out = np.empty((n, nv * k), np.float32)
         check(lib.cgs_memcpy_d2h(st._h, _ptr(out), d, out.nbytes))
         return out
 
-    n_e = min(n_err, n_points)
-    we, wsu, wnet = host(d_w, n_e), host(d_ws, n_e), host(d_wi, n_e)
-    # deformed-position error under a seeded 0.1 perturbation of the cage vertices
+    we, wsu = host(d_w, n_e), host(d_ws, n_e)
+    ge, gsu = host(d_g, n_e, 3).reshape(n_e, nv, 3), host(d_gs, n_e, 3).reshape(n_e, nv, 3)
+    # the same points without the fallback (every point from the network)
+    fb_none = C.c_uint64()
+    d_wn, d_gn = dalloc(n_e * nv * 4), dalloc(n_e * nv * 12)
+    check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, n_e, 1e30, d_wn, d_gn, d_ms,
+                           C.byref(fb_none)))
+    wnf = host(d_wn, n_e)
     rng = np.random.Generator(np.random.PCG64(99))
     dv = (cage.vertices + rng.uniform(-0.1, 0.1, cage.vertices.shape)).astype(np.float32)
     pe = we @ dv
+    Fe = np.matmul(ge.transpose(0, 2, 1), dv)
     diag = cage.diagonal()
+    # the residual test against the error it is meant to catch: the network's
+    # far shares of the same points through the near/far combination give
+    # each point's residual and its (projected) weights
+    d_q, d_r = dalloc(n_e * nv * 4), dalloc(n_e * 4)
+    check(lib.cgs_mlp_infer(net._h, d_pts.ptr, n_e, d_q, None))
+    check(lib.cgs_mvc_nearfar_bind(st._h, cv, nv, ct, nt, near[0], near[1], d_pts.ptr, n_e, d_q,
+                                   None, d_wn, None, d_ms, d_r))
+    resid = np.empty(n_e, np.float32)
+    check(lib.cgs_memcpy_d2h(st._h, _ptr(resid), d_r, resid.nbytes))
+    e_nf = np.linalg.norm(pe - host(d_wn, n_e) @ dv, axis=1) / diag
+    sweep = {}
+    for tol in (1e-3, 3e-3, 1e-2, 3e-2, 1e-1):
+        keep = resid <= tol * diag
+        ek = e_nf[keep]
+        sweep[f"{tol:g}"] = {"fallback": round(1.0 - float(keep.mean()), 4),
+                             "kept_max": float(ek.max()) if ek.size else 0.0,
+                             "kept_p99": float(np.quantile(ek, 0.99)) if ek.size else 0.0}
+    check(lib.cgs_free(st._h, d_q))
+    check(lib.cgs_free(st._h, d_r))
 
-    def stats(wx):
-        e = np.linalg.norm(pe - wx @ dv, axis=1) / diag
+    def stats(e):
         return {"max": float(e.max()), "p99": float(np.quantile(e, 0.99)),
                 "mean": float(e.mean())}
 
-    # the network's weights after the linear-precision projection
-    # w + A^T (A A^T)^-1 ([1; x] - A w), A = [1^T; V^T] (what the bind applies to
-    # the points it keeps)
-    A = np.concatenate([np.ones((1, nv)), cage.vertices.T.astype(np.float64)], 0)
-    Minv = np.linalg.inv(A @ A.T)
-    rhs = np.concatenate([np.ones((n_e, 1)), pts[:n_e].astype(np.float64)], 1)
-    wproj = (wnet + ((rhs - wnet.astype(np.float64) @ A.T) @ Minv @ A)).astype(np.float32)
     res = {"workload": f"{n_points} points in a {nv}-vertex cage, weights + gradients; "
-                       f"position errors over the first {n_e} points",
+                       f"errors over the first {n_e} points",
            "exact_ms": round(exact_ms, 2), "surrogate_bind_ms": round(sur_ms, 2),
-           "surrogate_inference_ms": round(inf_ms, 2),
+           "speedup": round(exact_ms / sur_ms, 2),
+           "near_field": list(near), "residual_tol": residual_tol,
            "fallback_fraction": round(fb.value / n_points, 4), "train_steps": train_steps,
            "train_samples": n_train, "train_s": round(train_s, 2),
-           "position_error_over_diag": {"bind (with fallback)": stats(wsu),
-                                        "network alone": stats(wnet),
-                                        "network + projection": stats(wproj)}}
-    for p in (d_w, d_g, d_m, d_ws, d_gs, d_ms, d_wi, d_gi, d_wt):
+           "targets_s": round(targets_s, 2),
+           "final_loss": float(np.mean(losses[-1][-50:])),
+           "position_error_over_diag": {
+               "bind (with fallback)": stats(np.linalg.norm(pe - wsu @ dv, axis=1) / diag),
+               "without fallback": stats(np.linalg.norm(pe - wnf @ dv, axis=1) / diag)},
+           "residual_sweep": sweep,
+           "jacobian_error_max_abs": stats(
+               np.abs(np.matmul(gsu.transpose(0, 2, 1), dv) - Fe).reshape(n_e, -1).max(1))}
+    for p in (d_w, d_g, d_m, d_ws, d_gs, d_ms, d_qt, d_wn, d_gn):
         check(lib.cgs_free(st._h, p))
     d_pts.free()
     d_tr.free()

@@ -171,6 +171,7 @@ _vp = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
 _int = C.c_int
+_f = C.c_float
 
 # (name, argtypes) of every entry point declared in include/cindergs.h
 SIGNATURES = {
@@ -224,6 +225,10 @@ SIGNATURES = {
     "cgs_sort_hysteresis_reuse_count": [_vp, C.POINTER(_u64)],
     "cgs_bind_tets": [_vp, _vp, _u32, _vp, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
     "cgs_mvc_bind_exact": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp, _vp, _vp],
+    "cgs_mvc_far_share": [_vp, _vp, _u32, _vp, _u32, _f, _f, _vp, _u64, _vp],
+    "cgs_mvc_nearfar_bind": [_vp, _vp, _u32, _vp, _u32, _f, _f, _vp, _u64, _vp, _vp, _vp, _vp, _vp,
+                             _vp],
+    "cgs_mlp_set_near_field": [_vp, _f, _f],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
     "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
     "cgs_mvc_deform_scene": [_vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _u32,

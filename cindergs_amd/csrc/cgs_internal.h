@@ -447,6 +447,44 @@ void cgs_launch_mvc_select(const float* winding, const float* residual, float to
                            uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
                            uint32_t* block_counts, hipStream_t s);
 uint32_t cgs_mvc_select_blocks(uint32_t n);
+// Near/far split of the MVC sums (k_mvc.hip): triangle t's terms at point x
+// carry the weight phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)) (c_t
+// the triangle's centroid): the near sums u_near = sum_t phi_t mu_t are
+// evaluated exactly, the far share q = (u - u_near) / sum(u) is learned.
+// Training targets: q * q_scale per point, [n x nv], degenerate points zero.
+void cgs_launch_mvc_far_share(const CgsMvcArgs& a, const float* cent, float r0, float r1,
+                              float q_scale, float* out, hipStream_t s);
+// winding numbers, one point per lane (the same sum as winding_number,
+// mvc.rs:326-347, accumulated in triangle order)
+void cgs_launch_mvc_winding_lane(const float* verts, const uint32_t* tris, uint32_t nt,
+                                 const float* pts, uint32_t n, float* out, hipStream_t s);
+struct CgsNearArgs {
+  const float* verts;          // [nv][3]
+  uint32_t nv;
+  const uint32_t* tris;        // [nt][3]
+  uint32_t nt;
+  const float* cent;           // [nt][3] triangle centroids
+  const uint32_t* cell_start;  // [cells + 1] candidate lists: every triangle whose
+  const uint32_t* cell_list;   //   centroid is within r1 + h + cell half-diagonal
+  float lo[3], inv_cell;       //   of the cell centre, ascending
+  uint32_t dim[3];
+  float r0, r1, h, tau;        // tau: scale below which the near term fades out
+  const float* pts;            // [n][3]
+  uint32_t n;
+  const float* winding;        // [n] or null (every point inside)
+  const float* Q;              // [n x ldq] far share * q_scale
+  const float* DQ;             // [3n x ldq] its derivatives (row 3p + axis) or null
+  uint32_t ldq;
+  float q_scale;
+  float minv[16];              // (A A^T)^-1, A = [1; v] (4 x nv), row-major
+  float* out_w;                // [n][nv]
+  float* out_g;                // [n][nv][3] or null (then DQ unused)
+  float* residual;             // [n]: |sum w v - x| before the projection; +inf
+                               //   where the near sums hit a vertex or a face
+  uint32_t* inside;            // [n] or null
+};
+#define CGS_NEARFAR_MAX_VERTICES 2048u
+void cgs_launch_mvc_nearfar(const CgsNearArgs& a, hipStream_t s);
 // deform_scene_cpu (mvc.rs:564-749) after the per-point position and Jacobian
 struct CgsMvcSceneArgs {
   const uint32_t* mask;      // [n] bound flags, or null (all bound)
@@ -509,7 +547,24 @@ void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const f
                         uint16_t* A0, uint16_t* A0t, uint32_t ldt, uint16_t* T0, hipStream_t s);
 void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,
                          uint32_t P, uint32_t V, uint16_t* dZ, uint16_t* dZt, uint32_t ldt,
-                         float* row_loss, hipStream_t s);
+                         float* row_loss, bool affine, hipStream_t s);
+// Whole-network inference in one kernel (k_mlp.hip): Fourier features, four
+// hidden layers and the output layer with the activations resident in LDS.
+// Z [P x ldz] = outputs (no normalisation), TZ [3P x ldz] = their derivatives
+// along x, y, z (row 3p + axis) or null.
+struct CgsMlpInferArgs {
+  const float* X;                       // [P x 3]
+  uint32_t P;
+  const float* fourier;                 // [32 x 3]
+  const uint16_t* wb[CGS_MLP_LAYERS];   // bf16 [rows x cols]
+  const float* w;                       // master weights (bias column)
+  uint64_t off[CGS_MLP_LAYERS];
+  uint32_t n_out_p;                     // output rows, a multiple of 32
+  float* Z;
+  float* TZ;
+  uint32_t ldz;
+};
+void cgs_launch_mlp_infer(const CgsMlpInferArgs& a, hipStream_t s);
 void cgs_launch_sum(const float* v, uint32_t n, float div, float* out, hipStream_t s);
 void cgs_launch_adam(const CgsAdamArgs& a, bool update, hipStream_t s);
 void cgs_launch_sample(uint32_t* idx, uint32_t P, uint64_t n, uint64_t seed, uint32_t step,
@@ -524,6 +579,7 @@ struct CgsMlpPostArgs {
   float* out_w;         // [P x V]
   float* out_g;         // [P x V x 3] or null
   float* residual;      // [P] or null
+  int raw;              // 1: outputs and tangents copied as they are
 };
 void cgs_launch_mlp_post(const CgsMlpPostArgs& a, hipStream_t s);
 
@@ -531,6 +587,9 @@ struct cgs_mlp;
 int cgs_mlp_infer_residual(struct cgs_mlp* M, const float* d_x, uint64_t n, float* d_w,
                            float* d_grad, const float* d_cage, const float* minv16,
                            float* d_residual);
+int cgs_mlp_infer_nearfar(struct cgs_mlp* M, const float* d_x, uint64_t n, float* d_w,
+                          float* d_grad, const CgsNearArgs& near);
+bool cgs_mlp_near(struct cgs_mlp* M, float* r0, float* r1);
 hipStream_t cgs_mlp_stream(struct cgs_mlp* M);
 uint32_t cgs_mlp_outputs(struct cgs_mlp* M);
 

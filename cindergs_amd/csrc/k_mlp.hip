@@ -143,12 +143,15 @@ __device__ __forceinline__ float wave_sum64(float v) {
 
 // Affine normalisation, squared error and its gradient, one wave per row:
 // w = z + (1 - sum z)/V, e = w - target, loss_row = sum e^2,
-// dL/dz_j = g_j - mean(g) with g = 2 e / (P V).  Writes dZ [P x ldz] and its
+// dL/dz_j = g_j - mean(g) with g = 2 e / (P V).  With affine = 0 the outputs
+// are compared as they are (w = z, dL/dz = g): the far-field share of a
+// near/far network has no sum constraint.  Writes dZ [P x ldz] and its
 // transpose [ldz(+32) x P] in bf16 (columns >= V stay zero).
 __global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
                                                   const float* targets, const uint32_t* idx,
                                                   uint32_t P, uint32_t V, uint16_t* dZ,
-                                                  uint16_t* dZt, uint32_t ldt, float* row_loss) {
+                                                  uint16_t* dZt, uint32_t ldt, float* row_loss,
+                                                  int affine) {
   const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
   if (p >= P) return;
   const float* z = Z + (size_t)p * ldz;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
   float s = 0.0f;
   for (uint32_t j = lane; j < V; j += 64u) s += z[j];
   s = wave_sum64(s);
-  const float shift = (1.0f - s) / (float)V;
+  const float shift = affine ? (1.0f - s) / (float)V : 0.0f;
   const float scale = 2.0f / ((float)P * (float)V);
   float gsum = 0.0f, l2 = 0.0f;
   for (uint32_t j = lane; j < V; j += 64u) {
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
   }
   gsum = wave_sum64(gsum);
   l2 = wave_sum64(l2);
-  const float gmean = gsum / (float)V;
+  const float gmean = affine ? gsum / (float)V : 0.0f;
   for (uint32_t j = lane; j < V; j += 64u) {
     const float e = (z[j] + shift) - tg[j];
     const uint16_t d = f2bf(scale * e - gmean);
@@ -249,6 +252,149 @@ __global__ __launch_bounds__(256) void k_sample(uint32_t* idx, uint32_t P, uint6
   idx[i] = (uint32_t)(x % n);
 }
 
+// Whole-network inference, one block of eight waves per 128 activation rows.
+// With tangents (TAN) a block takes 32 points and row 4p + a holds point p's
+// activations (a = 0) or their derivative along axis a - 1 (a = 1..3);
+// without, 128 points, one row each.  The rows live in LDS as bf16
+// (128 x 256, 64 KiB, 16-byte chunks XOR-swizzled by row so that the sixteen
+// rows of an MFMA fragment read fall on different banks); every layer is
+// v_mfma_f32_16x16x32_bf16 over the resident rows, each wave producing 32
+// output columns for all 128 rows (eight waves: the 256 hidden columns in
+// one pass), its weight fragments read from global memory (the weights,
+// 0.8 MiB in bf16, stay in L2) one K step ahead.  Measured on MI355X against
+// four waves of 64 columns (0.73 ms per 65536 points with tangents) and a
+// 2 x 2 split of rows and columns (0.88 ms): 0.47 ms, the weight-fragment
+// loads per wave being what limits the wider tiles.  The MFMA output layout
+// puts rows 4q..4q+3 of one column in one lane, i.e. a point's value and its
+// three tangents, so the ReLU mask of the tangents is applied in registers.
+// The activations never leave the chip; only the output layer is written, as
+// f32.
+__device__ __forceinline__ uint32_t act_idx(uint32_t row, uint32_t k) {
+  return row * 256u + ((((k >> 3) ^ (row & 7u)) << 3) | (k & 7u));
+}
+
+template <bool TAN>
+__global__ __launch_bounds__(512) void k_mlp_infer(CgsMlpInferArgs A) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[128 * 256];
+  constexpr uint32_t PTS = TAN ? 32u : 128u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t r = lane & 15u, kq = (lane >> 4) * 8u;
+  const uint32_t p0 = blockIdx.x * PTS;
+  const float TWO_PI = 6.283185307179586f;
+  for (uint32_t t = threadIdx.x; t < PTS * 32u; t += 512u) {
+    const uint32_t pl = t >> 5, k = t & 31u, p = p0 + pl;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (p < A.P) {
+      x = A.X[(size_t)p * 3];
+      y = A.X[(size_t)p * 3 + 1];
+      z = A.X[(size_t)p * 3 + 2];
+    }
+    const float bb[3] = {A.fourier[k * 3], A.fourier[k * 3 + 1], A.fourier[k * 3 + 2]};
+    const float sarg = TWO_PI * ((bb[0] * x + bb[1] * y) + bb[2] * z);
+    float sn, cs;
+    sincosf(sarg, &sn, &cs);
+    const uint32_t row = TAN ? pl * 4u : pl;
+    act[act_idx(row, k)] = f2bf(sn);
+    act[act_idx(row, 32u + k)] = f2bf(cs);
+    if (TAN) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        act[act_idx(row + 1u + a, k)] = f2bf(TWO_PI * bb[a] * cs);
+        act[act_idx(row + 1u + a, 32u + k)] = f2bf(-TWO_PI * bb[a] * sn);
+      }
+    }
+  }
+  __syncthreads();
+  const bf16x8 zero = {};
+  f32x4 acc[8][2];
+#pragma unroll 1
+  for (int l = 0; l < CGS_MLP_LAYERS; ++l) {
+    const uint32_t K = l == 0 ? 64u : 256u;
+    const uint32_t N = l == CGS_MLP_LAYERS - 1 ? A.n_out_p : 256u;
+    const uint16_t* W = A.wb[l];
+    const float* bias = A.w + A.off[l] + K;
+#pragma unroll 1
+    for (uint32_t c0 = wave * 32u; c0 < N; c0 += 256u) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      // weight fragments one K step ahead
+      bf16x8 b[2], bn[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t col = c0 + j * 16u + r;
+        b[j] = col < N ? *reinterpret_cast<const bf16x8*>(W + (size_t)col * K + kq) : zero;
+      }
+      for (uint32_t k = 0; k < K; k += 32u) {
+        const uint32_t kn = k + 32u < K ? k + 32u : k;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const uint32_t col = c0 + j * 16u + r;
+          bn[j] = col < N ? *reinterpret_cast<const bf16x8*>(W + (size_t)col * K + kn + kq) : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[act_idx(i * 16u + r, k + kq)]);
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = bn[j];
+      }
+      if (l == CGS_MLP_LAYERS - 1) {
+        // output layer: f32 rows of Z (values) and TZ (tangents)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const uint32_t col = c0 + j * 16u + (lane & 15u);
+            if (col >= N) continue;
+            const float bv = bias[(size_t)col * (K + 32u)];
+            if (TAN) {
+              const uint32_t p = p0 + i * 4u + (lane >> 4);
+              if (p >= A.P) continue;
+              A.Z[(size_t)p * A.ldz + col] = acc[i][j][0] + bv;
+              if (A.TZ) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) A.TZ[((size_t)p * 3 + a) * A.ldz + col] = acc[i][j][1 + a];
+              }
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const uint32_t p = p0 + i * 16u + (lane >> 4) * 4u + q;
+                if (p < A.P) A.Z[(size_t)p * A.ldz + col] = acc[i][j][q] + bv;
+              }
+            }
+          }
+      }
+    }
+    if (l == CGS_MLP_LAYERS - 1) break;
+    __syncthreads();  // every wave has read this layer's input rows
+    const uint32_t c0 = wave * 32u;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t col = c0 + j * 16u + (lane & 15u);
+        const uint32_t row0 = i * 16u + (lane >> 4) * 4u;
+        const float bv = bias[(size_t)col * (K + 32u)];
+        if (TAN) {
+          const float pre = acc[i][j][0] + bv;
+          const bool on = pre > 0.0f;
+          act[act_idx(row0, col)] = f2bf(on ? pre : 0.0f);
+#pragma unroll
+          for (int a = 1; a < 4; ++a) act[act_idx(row0 + a, col)] = f2bf(on ? acc[i][j][a] : 0.0f);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) act[act_idx(row0 + q, col)] = f2bf(fmaxf(acc[i][j][q] + bv, 0.0f));
+        }
+      }
+    __syncthreads();
+  }
+}
+
 // Surrogate outputs of P points (one wave per point): w = z + (1 - sum z)/V
 // and, from the tangent rows, dw/dx_a = t_a - (sum t_a)/V.  With a cage: the
 // reconstruction residual |sum_j w_j v_j - x| of these weights (the fallback
@@ -264,7 +410,7 @@ __global__ __launch_bounds__(256) void k_mlp_post(CgsMlpPostArgs A) {
   float s = 0.0f;
   for (uint32_t j = lane; j < V; j += 64u) s += z[j];
   s = wave_sum64(s);
-  const float shift = (1.0f - s) / (float)V;
+  const float shift = A.raw ? 0.0f : (1.0f - s) / (float)V;
   float* wrow = A.out_w + (size_t)p * V;
   float* grow = A.out_g ? A.out_g + (size_t)p * V * 3 : nullptr;
   if (A.TZ && grow) {
@@ -272,7 +418,7 @@ __global__ __launch_bounds__(256) void k_mlp_post(CgsMlpPostArgs A) {
       const float* t = A.TZ + ((size_t)p * 3 + a) * A.ldz;
       float ts = 0.0f;
       for (uint32_t j = lane; j < V; j += 64u) ts += t[j];
-      ts = wave_sum64(ts) / (float)V;
+      ts = A.raw ? 0.0f : wave_sum64(ts) / (float)V;
       for (uint32_t j = lane; j < V; j += 64u) grow[(size_t)j * 3 + a] = t[j] - ts;
     }
   }
@@ -346,11 +492,19 @@ void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const f
                      A0t, ldt, T0);
 }
 
+void cgs_launch_mlp_infer(const CgsMlpInferArgs& a, hipStream_t s) {
+  if (a.P == 0) return;
+  if (a.TZ)
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3((a.P + 31u) / 32u), dim3(512), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3((a.P + 127u) / 128u), dim3(512), 0, s, a);
+}
+
 void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,
                          uint32_t P, uint32_t V, uint16_t* dZ, uint16_t* dZt, uint32_t ldt,
-                         float* row_loss, hipStream_t s) {
+                         float* row_loss, bool affine, hipStream_t s) {
   hipLaunchKernelGGL(k_mlp_loss, dim3((P + 3u) / 4u), dim3(256), 0, s, Z, ldz, targets, idx, P, V,
-                     dZ, dZt, ldt, row_loss);
+                     dZ, dZt, ldt, row_loss, affine ? 1 : 0);
 }
 
 void cgs_launch_sum(const float* v, uint32_t n, float div, float* out, hipStream_t s) {

@@ -409,6 +409,410 @@ __global__ __launch_bounds__(256) void k_mvc_scene_children(CgsMvcSceneArgs A,
   for (int k = 0; k < 48; ++k) A.out_sh[j * 48 + k] = A.out_sh[(size_t)i * 48 + k];
 }
 
+// ---------------------------------------------------------------------------
+// Near/far split for the learned binding.  Triangle t's terms at x get the
+// weight phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)), c_t its
+// centroid: 1 within r0, 0 beyond r1.  The near sums u_near = sum_t phi_t
+// mu_t (the singular, fast-varying part next to the cage) are evaluated
+// exactly; the far sums are smooth and are what the network learns, as the
+// share q = u_far / (u_near + u_far) of the normalised weights.
+__device__ __forceinline__ float near_phi(float r, float r0, float r1) {
+  const float t = cgs_fclamp((r1 - r) / (r1 - r0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// Far-share training targets, one wave per point: the loop of mvc_eval with
+// each triangle's terms scaled by 1 - phi_t before the per-vertex sums, the
+// normaliser over all terms.  Points on a vertex or a face, or with a zero
+// normaliser, get a zero row (they do not occur among interior samples).
+__global__ __launch_bounds__(64) void k_mvc_far_share(CgsMvcArgs A, const float* cent, float r0,
+                                                      float r1, float q_scale, float* out) {
+  extern __shared__ float4 s_dyn[];
+  float4* s_dir = s_dyn;
+  float* s_mu = reinterpret_cast<float*>(s_dir + A.nv);
+  const uint32_t lane = threadIdx.x, i = blockIdx.x;
+  if (i >= A.n) return;
+  const V3f x = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
+  float* row = out + (size_t)i * A.nv;
+  uint32_t bad = 0u;
+  for (uint32_t v = lane; v < A.nv; v += 64u) {
+    const V3f diff = vsub(vert(A.verts, v), x);
+    const float d = vlen(diff);
+    if (d < 1e-6f) bad = 1u;
+    s_dir[v] = make_float4(diff.x / d, diff.y / d, diff.z / d, d);
+  }
+  if (__ballot(bad != 0u) != 0ull) {
+    for (uint32_t v = lane; v < A.nv; v += 64u) row[v] = 0.0f;
+    return;
+  }
+  __builtin_amdgcn_wave_barrier();
+  float part = 0.0f;
+  for (uint32_t t = lane; t < A.nt; t += 64u) {
+    const uint32_t i0 = A.tris[t * 3], i1 = A.tris[t * 3 + 1], i2 = A.tris[t * 3 + 2];
+    const float4 d0 = s_dir[i0], d1 = s_dir[i1], d2 = s_dir[i2];
+    const V3f u0 = {d0.x, d0.y, d0.z}, u1 = {d1.x, d1.y, d1.z}, u2 = {d2.x, d2.y, d2.z};
+    const float th0 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u1, u2)) * 0.5f, 0.0f, 1.0f));
+    const float th1 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u2, u0)) * 0.5f, 0.0f, 1.0f));
+    const float th2 = 2.0f * asinf(cgs_fclamp(vlen(vsub(u0, u1)) * 0.5f, 0.0f, 1.0f));
+    const float h = 0.5f * ((th0 + th1) + th2);
+    float mu0 = 0.0f, mu1 = 0.0f, mu2 = 0.0f;
+    if (fabsf(MVC_PI - h) < 1e-5f) {
+      bad = 1u;
+    } else {
+      const float st0 = sinf(th0), st1 = sinf(th1), st2 = sinf(th2);
+      if (!(fabsf(st0) < 1e-6f || fabsf(st1) < 1e-6f || fabsf(st2) < 1e-6f)) {
+        const float sh = sinf(h);
+        const float c0 = cgs_fclamp((2.0f * sh * sinf(h - th0)) / (st1 * st2) - 1.0f, -1.0f, 1.0f);
+        const float c1 = cgs_fclamp((2.0f * sh * sinf(h - th1)) / (st2 * st0) - 1.0f, -1.0f, 1.0f);
+        const float c2 = cgs_fclamp((2.0f * sh * sinf(h - th2)) / (st0 * st1) - 1.0f, -1.0f, 1.0f);
+        const float sg = vdot(u2, vcross(u0, u1)) >= 0.0f ? 1.0f : -1.0f;
+        const float s0 = sg * sqrtf(cgs_fmax(1.0f - c0 * c0, 0.0f));
+        const float s1 = sg * sqrtf(cgs_fmax(1.0f - c1 * c1, 0.0f));
+        const float s2 = sg * sqrtf(cgs_fmax(1.0f - c2 * c2, 0.0f));
+        if (!(fabsf(s0) <= 1e-6f || fabsf(s1) <= 1e-6f || fabsf(s2) <= 1e-6f)) {
+          mu0 = (th0 - c1 * th2 - c2 * th1) / (2.0f * d0.w * st1 * s2);
+          mu1 = (th1 - c2 * th0 - c0 * th2) / (2.0f * d1.w * st2 * s0);
+          mu2 = (th2 - c0 * th1 - c1 * th0) / (2.0f * d2.w * st0 * s1);
+        }
+      }
+    }
+    part += (mu0 + mu1) + mu2;
+    const V3f c = vert(cent, t);
+    const float far = 1.0f - near_phi(vlen(vsub(x, c)), r0, r1);
+    s_mu[t * 3] = far * mu0;
+    s_mu[t * 3 + 1] = far * mu1;
+    s_mu[t * 3 + 2] = far * mu2;
+  }
+  const float total = wave_sum(part);
+  const bool skip = __ballot(bad != 0u) != 0ull || !(fabsf(total) > 1e-12f);
+  __builtin_amdgcn_wave_barrier();
+  const float scale = skip ? 0.0f : q_scale / total;
+  for (uint32_t v = lane; v < A.nv; v += 64u) {
+    float acc = 0.0f;
+    for (uint32_t j = A.vt_start[v]; j < A.vt_start[v + 1]; ++j) acc += s_mu[A.vt_list[j]];
+    row[v] = acc * scale;
+  }
+}
+
+// winding_number (mvc.rs:326-347) with one point per lane: the triangle loop
+// index is the same on every lane, so its vertices come through scalar loads
+// and the whole wave works on 64 points at once.
+__global__ __launch_bounds__(256) void k_mvc_winding_lane(const float* verts, const uint32_t* tris,
+                                                          uint32_t nt, const float* pts, uint32_t n,
+                                                          float* out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t ic = i < n ? i : n - 1u;
+  const V3f p = {pts[(size_t)ic * 3], pts[(size_t)ic * 3 + 1], pts[(size_t)ic * 3 + 2]};
+  float sum = 0.0f;
+  bool on_vertex = false;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const V3f a = vsub(vert(verts, tris[t * 3]), p);
+    const V3f b = vsub(vert(verts, tris[t * 3 + 1]), p);
+    const V3f c = vsub(vert(verts, tris[t * 3 + 2]), p);
+    const float la = vlen(a), lb = vlen(b), lc = vlen(c);
+    if (la < 1e-7f || lb < 1e-7f || lc < 1e-7f) {
+      on_vertex = true;
+      continue;
+    }
+    const float det = vdot(a, vcross(b, c));
+    const float den = ((la * lb * lc + vdot(a, b) * lc) + vdot(b, c) * la) + vdot(c, a) * lb;
+    sum += 2.0f * atan2f(det, den);
+  }
+  if (i < n) out[i] = on_vertex ? 1.0f : sum / (4.0f * MVC_PI);
+}
+
+// JSW terms of one triangle at x: the loop body of mvc_eval with the
+// directions computed for this triangle alone and every sine and cosine taken
+// from the chord lengths l_i = |u_j - u_k| instead of sinf: with
+// a_i = theta_i / 2 = asin(l_i / 2), sin a_i = l_i / 2, cos a_i =
+// sqrt((1 - l_i/2)(1 + l_i/2)), sin theta_i = 2 sin a_i cos a_i,
+// cos theta_i = 1 - 2 sin^2 a_i, and h = a_0 + a_1 + a_2 by the addition
+// formulas (the same quantities, three asinf per triangle instead of three
+// asinf and seven sinf).  Returns 0 when the triangle adds nothing, 1 with mu
+// set, 2 when x is on a vertex or on the triangle.
+__device__ __forceinline__ int jsw_terms(V3f x, V3f v0, V3f v1, V3f v2, float* mu) {
+  const V3f e0 = vsub(v0, x), e1 = vsub(v1, x), e2 = vsub(v2, x);
+  const float d0 = vlen(e0), d1 = vlen(e1), d2 = vlen(e2);
+  if (d0 < 1e-6f || d1 < 1e-6f || d2 < 1e-6f) return 2;
+  const V3f u0 = {e0.x / d0, e0.y / d0, e0.z / d0};
+  const V3f u1 = {e1.x / d1, e1.y / d1, e1.z / d1};
+  const V3f u2 = {e2.x / d2, e2.y / d2, e2.z / d2};
+  const float s0h = cgs_fclamp(vlen(vsub(u1, u2)) * 0.5f, 0.0f, 1.0f);
+  const float s1h = cgs_fclamp(vlen(vsub(u2, u0)) * 0.5f, 0.0f, 1.0f);
+  const float s2h = cgs_fclamp(vlen(vsub(u0, u1)) * 0.5f, 0.0f, 1.0f);
+  const float c0h = sqrtf((1.0f - s0h) * (1.0f + s0h));
+  const float c1h = sqrtf((1.0f - s1h) * (1.0f + s1h));
+  const float c2h = sqrtf((1.0f - s2h) * (1.0f + s2h));
+  const float th0 = 2.0f * asinf(s0h), th1 = 2.0f * asinf(s1h), th2 = 2.0f * asinf(s2h);
+  const float h = 0.5f * ((th0 + th1) + th2);
+  if (fabsf(MVC_PI - h) < 1e-5f) return 2;
+  const float st0 = 2.0f * s0h * c0h, st1 = 2.0f * s1h * c1h, st2 = 2.0f * s2h * c2h;
+  if (fabsf(st0) < 1e-6f || fabsf(st1) < 1e-6f || fabsf(st2) < 1e-6f) return 0;
+  const float ct0 = 1.0f - 2.0f * s0h * s0h, ct1 = 1.0f - 2.0f * s1h * s1h, ct2 = 1.0f - 2.0f * s2h * s2h;
+  const float s01 = s0h * c1h + c0h * s1h, c01 = c0h * c1h - s0h * s1h;
+  const float sh = s01 * c2h + c01 * s2h, ch = c01 * c2h - s01 * s2h;
+  const float c0 = cgs_fclamp((2.0f * sh * (sh * ct0 - ch * st0)) / (st1 * st2) - 1.0f, -1.0f, 1.0f);
+  const float c1 = cgs_fclamp((2.0f * sh * (sh * ct1 - ch * st1)) / (st2 * st0) - 1.0f, -1.0f, 1.0f);
+  const float c2 = cgs_fclamp((2.0f * sh * (sh * ct2 - ch * st2)) / (st0 * st1) - 1.0f, -1.0f, 1.0f);
+  const float sg = vdot(u2, vcross(u0, u1)) >= 0.0f ? 1.0f : -1.0f;
+  const float s0 = sg * sqrtf(cgs_fmax(1.0f - c0 * c0, 0.0f));
+  const float s1 = sg * sqrtf(cgs_fmax(1.0f - c1 * c1, 0.0f));
+  const float s2 = sg * sqrtf(cgs_fmax(1.0f - c2 * c2, 0.0f));
+  if (fabsf(s0) <= 1e-6f || fabsf(s1) <= 1e-6f || fabsf(s2) <= 1e-6f) return 0;
+  mu[0] = (th0 - c1 * th2 - c2 * th1) / (2.0f * d0 * st1 * s2);
+  mu[1] = (th1 - c2 * th0 - c0 * th2) / (2.0f * d1 * st2 * s0);
+  mu[2] = (th2 - c0 * th1 - c1 * th0) / (2.0f * d2 * st0 * s1);
+  return 1;
+}
+
+// The learned binding's weights, one wave per point:
+//   w = (1 - sum q) u_near U_near / (U_near^2 + tau^2) + q,
+// u_near = sum over the point's near triangles of phi_t mu_t, U_near its sum,
+// q = the network's far share.  The factor U / (U^2 + tau^2) is 1 / U_near
+// wherever |U_near| >> tau and fades the near term out continuously as
+// U_near -> 0 at the near shell's outer edge; tau is far below any interior
+// point's weight sum (>= 4 pi / diagonal), so the near term is short by at
+// most tau / (2 sum u) of the weights' sum there.  Gradients: central
+// differences (step h) of the weights with the network's share linearised
+// from its tangents, q(p +- h e_a) = q +- h dq_a:
+//   g_a = (1 - Q) (un+ - un-) / 2h - dQ_a (un+ + un-) / 2 + dq_a.
+// The residual |sum w v - x| is taken before the linear-precision projection
+// w' = w + A^T Minv ([1; x] - A w) (and g'_a likewise with e_a), which the
+// rows receive.
+//
+// Layout: the triangles of the point's grid cell within reach are compacted
+// into LDS, then the (triangle, evaluation point) pairs are spread over the
+// lanes so that every lane does about the same number of JSW evaluations;
+// the per-vertex sums of the seven evaluations live in a small LDS hash table
+// keyed by vertex (NEAR_SLOTS slots, open addressing; a point whose near
+// triangles touch more vertices than fit is flagged for the exact path), so
+// a wave needs 9 KiB of LDS whatever the cage size.  The network's rows are
+// loaded into registers (VPL vertices per lane) before the near loop, and
+// the weights stay there until they are written.
+constexpr uint32_t NEAR_SLOTS = 256u, NEAR_TRIS = 256u, NEAR_EMPTY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t near_hash(uint32_t v) { return (v * 0x9E3779B1u) >> 24; }
+
+// slot of vertex v (inserted when absent); NEAR_EMPTY when the table is full
+__device__ __forceinline__ uint32_t near_slot_insert(uint32_t* keys, uint32_t v) {
+  uint32_t s = near_hash(v);
+  for (uint32_t k = 0; k < NEAR_SLOTS; ++k) {
+    const uint32_t old = atomicCAS(&keys[s], NEAR_EMPTY, v);
+    if (old == NEAR_EMPTY || old == v) return s;
+    s = (s + 1u) & (NEAR_SLOTS - 1u);
+  }
+  return NEAR_EMPTY;
+}
+
+__device__ __forceinline__ uint32_t near_slot_find(const uint32_t* keys, uint32_t v) {
+  uint32_t s = near_hash(v);
+  for (uint32_t k = 0; k < NEAR_SLOTS; ++k) {
+    const uint32_t key = keys[s];
+    if (key == v) return s;
+    if (key == NEAR_EMPTY) return NEAR_EMPTY;
+    s = (s + 1u) & (NEAR_SLOTS - 1u);
+  }
+  return NEAR_EMPTY;
+}
+
+template <int VPL>
+__global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
+  __shared__ uint32_t s_key[NEAR_SLOTS];
+  __shared__ float s_val[7][NEAR_SLOTS];
+  __shared__ uint32_t s_tri[NEAR_TRIS];
+  const uint32_t lane = threadIdx.x, i = blockIdx.x;
+  if (i >= A.n) return;
+  const uint32_t nv = A.nv;
+  const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
+  float* row = A.out_w + (size_t)i * nv;
+  float* grow = A.out_g ? A.out_g + (size_t)i * nv * 3 : nullptr;
+  const bool in = A.winding ? fabsf(A.winding[i]) >= 0.5f : true;
+  if (A.inside && lane == 0) A.inside[i] = in ? 1u : 0u;
+  if (!in) {
+    for (uint32_t v = lane; v < nv; v += 64u) row[v] = 0.0f;
+    if (grow)
+      for (uint32_t v = lane; v < 3 * nv; v += 64u) grow[v] = 0.0f;
+    if (lane == 0) A.residual[i] = 0.0f;
+    return;
+  }
+  const float zs = 1.0f / A.q_scale;
+  const bool tan = grow && A.DQ;
+  const float* q = A.Q + (size_t)i * A.ldq;
+  const uint32_t evals = grow ? 7u : 1u;
+  for (uint32_t k = lane; k < NEAR_SLOTS; k += 64u) {
+    s_key[k] = NEAR_EMPTY;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) s_val[e][k] = 0.0f;
+  }
+  float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  uint32_t degenerate = 0u;
+  const float fx = (p.x - A.lo[0]) * A.inv_cell, fy = (p.y - A.lo[1]) * A.inv_cell,
+              fz = (p.z - A.lo[2]) * A.inv_cell;
+  const bool in_grid = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)A.dim[0] &&
+                       fy < (float)A.dim[1] && fz < (float)A.dim[2];
+  uint32_t c0 = 0u, c1 = 0u;
+  if (in_grid) {
+    const uint32_t cx = min((uint32_t)fx, A.dim[0] - 1u), cy = min((uint32_t)fy, A.dim[1] - 1u),
+                   cz = min((uint32_t)fz, A.dim[2] - 1u);
+    const uint32_t cell = (cz * A.dim[1] + cy) * A.dim[0] + cx;
+    c0 = A.cell_start[cell];
+    c1 = A.cell_start[cell + 1];
+  }
+  const float reach = A.r1 + A.h;
+  // chunks of up to NEAR_TRIS candidates: compact those within reach, then
+  // the (triangle, evaluation) pairs over the lanes
+  for (uint32_t base = c0; base < c1; base += NEAR_TRIS) {
+    const uint32_t end = min(c1, base + NEAR_TRIS);
+    uint32_t m = 0u;
+    __syncthreads();
+    for (uint32_t j0 = base; j0 < end; j0 += 64u) {
+      const uint32_t j = j0 + lane;
+      uint32_t t = 0u;
+      bool keep = false;
+      if (j < end) {
+        t = A.cell_list[j];
+        keep = vlen(vsub(p, vert(A.cent, t))) < reach;
+      }
+      const uint64_t bal = __ballot(keep);
+      if (keep) s_tri[m + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = t;
+      m += (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    for (uint32_t item = lane; item < m * evals; item += 64u) {
+      const uint32_t t = s_tri[item / evals];
+      const int e = (int)(item % evals);
+      const V3f c = vert(A.cent, t);
+      V3f x = p;
+      if (e > 0) {
+        const float step = (e & 1) ? A.h : -A.h;
+        const int axis = (e - 1) >> 1;
+        if (axis == 0) x.x += step;
+        else if (axis == 1) x.y += step;
+        else x.z += step;
+      }
+      const float phi = near_phi(vlen(vsub(x, c)), A.r0, A.r1);
+      if (!(phi > 0.0f)) continue;
+      const uint32_t iv[3] = {A.tris[t * 3], A.tris[t * 3 + 1], A.tris[t * 3 + 2]};
+      float mu[3];
+      const int k = jsw_terms(x, vert(A.verts, iv[0]), vert(A.verts, iv[1]), vert(A.verts, iv[2]), mu);
+      if (k == 2) degenerate = 1u;
+      if (k != 1) continue;
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        const uint32_t sl = near_slot_insert(s_key, iv[c3]);
+        if (sl == NEAR_EMPTY) {
+          degenerate = 1u;
+        } else {
+          atomicAdd(&s_val[e][sl], phi * mu[c3]);
+        }
+      }
+#pragma unroll
+      for (int ee = 0; ee < 7; ++ee)
+        if (ee == e) U[ee] += phi * ((mu[0] + mu[1]) + mu[2]);
+    }
+  }
+  const bool degen = __ballot(degenerate != 0u) != 0ull;
+  float inv[7];
+  const float tau2 = A.tau * A.tau;
+#pragma unroll
+  for (int e = 0; e < 7; ++e) {
+    const float Ue = (uint32_t)e < evals ? wave_sum(U[e]) : 0.0f;
+    inv[e] = Ue / (Ue * Ue + tau2);
+  }
+  float wv[VPL], gv[3][VPL];
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    const uint32_t v = lane + 64u * k;
+    wv[k] = v < nv ? q[v] * zs : 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      gv[a][k] = (tan && v < nv) ? A.DQ[((size_t)i * 3 + a) * A.ldq + v] * zs : 0.0f;
+  }
+  float sq = 0.0f, sdq[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    sq += wv[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sdq[a] += gv[a][k];
+  }
+  const float keep = 1.0f - wave_sum(sq), inv_2h = 0.5f / A.h;
+  float dQ[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) dQ[a] = tan ? wave_sum(sdq[a]) : 0.0f;
+  __syncthreads();
+  // the combined rows and the sums of the projection (sum w, sum w v; the
+  // same for each g_a)
+  float m4[4] = {0.f, 0.f, 0.f, 0.f}, mg[3][4] = {{0.f}};
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    const uint32_t v = lane + 64u * k;
+    if (v >= nv) continue;
+    const uint32_t sl = near_slot_find(s_key, v);
+    const V3f cv = vert(A.verts, v);
+    const float un = sl != NEAR_EMPTY ? s_val[0][sl] * inv[0] : 0.0f;
+    const float w = keep * un + wv[k];
+    wv[k] = w;
+    m4[0] += w;
+    m4[1] += w * cv.x;
+    m4[2] += w * cv.y;
+    m4[3] += w * cv.z;
+    if (grow) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float up = sl != NEAR_EMPTY ? s_val[1 + 2 * a][sl] * inv[1 + 2 * a] : 0.0f;
+        const float um = sl != NEAR_EMPTY ? s_val[2 + 2 * a][sl] * inv[2 + 2 * a] : 0.0f;
+        const float g = (keep * ((up - um) * inv_2h) - dQ[a] * (0.5f * (up + um))) + gv[a][k];
+        gv[a][k] = g;
+        mg[a][0] += g;
+        mg[a][1] += g * cv.x;
+        mg[a][2] += g * cv.y;
+        mg[a][3] += g * cv.z;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m4[k] = wave_sum(m4[k]);
+  if (grow)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) mg[a][k] = wave_sum(mg[a][k]);
+  if (lane == 0) {
+    const float rx = m4[1] - p.x, ry = m4[2] - p.y, rz = m4[3] - p.z;
+    A.residual[i] = degen ? __builtin_inff() : sqrtf((rx * rx + ry * ry) + rz * rz);
+  }
+  const float r[4] = {1.0f - m4[0], p.x - m4[1], p.y - m4[2], p.z - m4[3]};
+  float lam[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    lam[k] = ((A.minv[k * 4] * r[0] + A.minv[k * 4 + 1] * r[1]) + A.minv[k * 4 + 2] * r[2]) +
+             A.minv[k * 4 + 3] * r[3];
+  float la[3][4] = {{0.f}};
+  if (grow)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float e4[4] = {-mg[a][0], (a == 0 ? 1.0f : 0.0f) - mg[a][1],
+                           (a == 1 ? 1.0f : 0.0f) - mg[a][2], (a == 2 ? 1.0f : 0.0f) - mg[a][3]};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        la[a][k] = ((A.minv[k * 4] * e4[0] + A.minv[k * 4 + 1] * e4[1]) + A.minv[k * 4 + 2] * e4[2]) +
+                   A.minv[k * 4 + 3] * e4[3];
+    }
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    const uint32_t v = lane + 64u * k;
+    if (v >= nv) continue;
+    const V3f cv = vert(A.verts, v);
+    row[v] = wv[k] + (((lam[0] + lam[1] * cv.x) + lam[2] * cv.y) + lam[3] * cv.z);
+    if (grow) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        grow[(size_t)v * 3 + a] =
+            gv[a][k] + (((la[a][0] + la[a][1] * cv.x) + la[a][2] * cv.y) + la[a][3] * cv.z);
+    }
+  }
+}
+
 }  // namespace
 
 uint32_t cgs_mvc_select_blocks(uint32_t n) { return (n + 255u) / 256u; }
@@ -428,6 +832,33 @@ size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt) {
 void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s) {
   if (a.n == 0) return;
   hipLaunchKernelGGL(k_mvc_bind, dim3(a.n), dim3(64), cgs_mvc_lds_bytes(a.nv, a.nt), s, a);
+}
+
+void cgs_launch_mvc_far_share(const CgsMvcArgs& a, const float* cent, float r0, float r1,
+                              float q_scale, float* out, hipStream_t s) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(k_mvc_far_share, dim3(a.n), dim3(64), (size_t)a.nv * 16 + (size_t)a.nt * 12, s,
+                     a, cent, r0, r1, q_scale, out);
+}
+
+void cgs_launch_mvc_winding_lane(const float* verts, const uint32_t* tris, uint32_t nt,
+                                 const float* pts, uint32_t n, float* out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mvc_winding_lane, dim3((n + 255u) / 256u), dim3(256), 0, s, verts, tris, nt,
+                     pts, n, out);
+}
+
+void cgs_launch_mvc_nearfar(const CgsNearArgs& a, hipStream_t s) {
+  if (a.n == 0) return;
+  const size_t lds = 0;
+  if (a.nv <= 256)
+    hipLaunchKernelGGL(k_mvc_nearfar<4>, dim3(a.n), dim3(64), lds, s, a);
+  else if (a.nv <= 512)
+    hipLaunchKernelGGL(k_mvc_nearfar<8>, dim3(a.n), dim3(64), lds, s, a);
+  else if (a.nv <= 1024)
+    hipLaunchKernelGGL(k_mvc_nearfar<16>, dim3(a.n), dim3(64), lds, s, a);
+  else
+    hipLaunchKernelGGL(k_mvc_nearfar<32>, dim3(a.n), dim3(64), lds, s, a);
 }
 
 void cgs_launch_mvc_winding(const CgsMvcArgs& a, float* out, hipStream_t s) {

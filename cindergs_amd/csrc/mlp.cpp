@@ -53,11 +53,11 @@ struct cgs_mlp {
   Dev<uint32_t> idx;
   Dev<uint16_t> a0, a0t, hb[4], ht[4], dz, dzt, gb[2], gt[2];
   Dev<float> z, row_loss, losses;
-  // inference buffers (CHUNK points)
+  // inference buffers (CHUNK points): outputs and their tangents
   bool inf_ready = false;
-  Dev<uint16_t> ia0, it0, ih[2], itg[2];
-  Dev<float> iz, itz, cage;
-  uint32_t cage_nv = 0;
+  Dev<float> iz, itz;
+  // near/far split radii (0, 0: the network stands for the whole weights)
+  float near_r0 = 0.0f, near_r1 = 0.0f;
   std::vector<void*> owned;
 };
 
@@ -174,7 +174,7 @@ int forward_backward(cgs_mlp* M, const float* X, const float* T, const uint32_t*
     cgs_launch_gemm_nt(g, s);
   }
   cgs_launch_mlp_loss(M->z.p, M->n_out_p, T, idx, P, M->n_out, M->dz.p, M->dzt.p, P,
-                      d_loss ? M->row_loss.p : nullptr, s);
+                      d_loss ? M->row_loss.p : nullptr, !(M->near_r1 > 0.0f), s);
   if (d_loss) cgs_launch_sum(M->row_loss.p, P, (float)P * (float)M->n_out, d_loss, s);
   // backward: G5 = dZ; dW_l = G_{l+1}^T [H_l | 1]; G_l = (G_{l+1} W_l) * (H_l > 0)
   const uint16_t* gB = M->dz.p;   // G_{l+1} [P x rows_l]
@@ -203,62 +203,58 @@ int forward_backward(cgs_mlp* M, const float* X, const float* T, const uint32_t*
 
 int ensure_infer(cgs_mlp* M) {
   if (M->inf_ready) return CGS_OK;
-  TRYM(alloc(M, M->ia0, (size_t)CHUNK * NIN, false));
-  TRYM(alloc(M, M->it0, (size_t)3 * CHUNK * NIN, false));
-  for (int k = 0; k < 2; ++k) {
-    TRYM(alloc(M, M->ih[k], (size_t)CHUNK * NH, false));
-    TRYM(alloc(M, M->itg[k], (size_t)3 * CHUNK * NH, false));
-  }
   TRYM(alloc(M, M->iz, (size_t)CHUNK * M->n_out_p, false));
   TRYM(alloc(M, M->itz, (size_t)3 * CHUNK * M->n_out_p, false));
   M->inf_ready = true;
   return CGS_OK;
 }
 
-// w (and dw/dx) of points X[0..n) into rows of out_w [n x n_out] (out_g
-// [n x n_out x 3]); with a cage, residuals |sum w v - x| into residual[n]
+CgsMlpInferArgs infer_args(cgs_mlp* M) {
+  CgsMlpInferArgs a;
+  memset(&a, 0, sizeof(a));
+  a.fourier = M->fourier.p;
+  for (int l = 0; l < CGS_MLP_LAYERS; ++l) {
+    a.wb[l] = M->wb[l].p;
+    a.off[l] = M->off[l];
+  }
+  a.w = M->w.p;
+  a.n_out_p = M->n_out_p;
+  a.ldz = M->n_out_p;
+  return a;
+}
+
+// Network outputs of points X[0..n) in chunks of CHUNK (k_mlp_infer), then
+// per chunk either the affine normalisation (+ residual and projection with a
+// cage, k_mlp_post) into out_w / out_g, or, with `near`, the near/far
+// combination (k_mvc_nearfar) whose per-point pointers are offset here.
 int infer(cgs_mlp* M, const float* X, uint64_t n, float* out_w, float* out_g, const float* cage,
-          const float* minv16, float* residual) {
+          const float* minv16, float* residual, const CgsNearArgs* near) {
   TRYM(ensure_infer(M));
   hipStream_t s = M->s;
+  CgsMlpInferArgs ia = infer_args(M);
   for (uint64_t b = 0; b < n; b += CHUNK) {
     const uint32_t Q = (uint32_t)((n - b) < CHUNK ? (n - b) : CHUNK);
     const float* x = X + b * 3;
-    cgs_launch_fourier(x, nullptr, Q, M->fourier.p, M->ia0.p, nullptr, 0,
-                       out_g ? M->it0.p : nullptr, s);
-    const uint16_t* hin = M->ia0.p;
-    const uint16_t* tin = M->it0.p;
-    for (int l = 0; l < CGS_MLP_LAYERS; ++l) {
-      const int k = l & 1;
-      CgsGemmArgs g = gemm(hin, M->cols[l], M->wb[l].p, M->cols[l], Q, M->rows[l], M->cols[l]);
-      g.bias = M->w.p + M->off[l] + M->cols[l];
-      g.bias_stride = M->cols[l] + 32;
-      if (l < 4) {
-        g.relu = 1;
-        g.Cb = M->ih[k].p;
-        g.ldcb = NH;
-      } else {
-        g.Cf = M->iz.p;
-        g.ldcf = M->n_out_p;
-      }
-      cgs_launch_gemm_nt(g, s);
-      if (out_g) {
-        CgsGemmArgs t = gemm(tin, M->cols[l], M->wb[l].p, M->cols[l], 3 * Q, M->rows[l],
-                             M->cols[l]);
-        if (l < 4) {
-          t.mask = M->ih[k].p;
-          t.ldm = NH;
-          t.mask_div = 3;
-          t.Cb = M->itg[k].p;
-          t.ldcb = NH;
-        } else {
-          t.Cf = M->itz.p;
-          t.ldcf = M->n_out_p;
-        }
-        cgs_launch_gemm_nt(t, s);
-        tin = M->itg[k].p;
-      }
-      hin = M->ih[k].p;
+    ia.X = x;
+    ia.P = Q;
+    ia.Z = M->iz.p;
+    ia.TZ = out_g ? M->itz.p : nullptr;
+    cgs_launch_mlp_infer(ia, s);
+    if (near) {
+      CgsNearArgs na = *near;
+      na.pts = x;
+      na.n = Q;
+      na.winding = near->winding ? near->winding + b : nullptr;
+      na.Q = M->iz.p;
+      na.DQ = out_g ? M->itz.p : nullptr;
+      na.ldq = M->n_out_p;
+      na.q_scale = (float)M->n_out;
+      na.out_w = out_w + b * M->n_out;
+      na.out_g = out_g ? out_g + b * M->n_out * 3 : nullptr;
+      na.residual = near->residual + b;
+      na.inside = near->inside ? near->inside + b : nullptr;
+      cgs_launch_mvc_nearfar(na, s);
+      continue;
     }
     CgsMlpPostArgs pa;
     memset(&pa, 0, sizeof(pa));
@@ -273,6 +269,7 @@ int infer(cgs_mlp* M, const float* X, uint64_t n, float* out_w, float* out_g, co
     pa.out_w = out_w + b * M->n_out;
     pa.out_g = out_g ? out_g + b * M->n_out * 3 : nullptr;
     pa.residual = residual ? residual + b : nullptr;
+    pa.raw = (M->near_r1 > 0.0f && !cage) ? 1 : 0;
     cgs_launch_mlp_post(pa, s);
   }
   MCHK(hipGetLastError());
@@ -424,10 +421,20 @@ int cgs_mlp_gradient(cgs_mlp* M, const float* d_x, const float* d_targets, uint3
   return CGS_OK;
 }
 
+int cgs_mlp_set_near_field(cgs_mlp* M, float r0, float r1) {
+  if (!M) return cgs_fail(CGS_ERR_INVALID_ARG, "null network");
+  const bool off = r0 == 0.0f && r1 == 0.0f;
+  if (!off && !(r0 >= 0.0f && r1 > r0 && r1 < 1e6f))
+    return cgs_fail(CGS_ERR_INVALID_ARG, "near field radii need 0 <= r0 < r1 (or both 0)");
+  M->near_r0 = r0;
+  M->near_r1 = r1;
+  return CGS_OK;
+}
+
 int cgs_mlp_infer(cgs_mlp* M, const float* d_x, uint64_t n, float* d_w, float* d_grad) {
   if (!M || !d_x || !d_w) return cgs_fail(CGS_ERR_INVALID_ARG, "null argument");
   if (n == 0) return CGS_OK;
-  TRYM(infer(M, d_x, n, d_w, d_grad, nullptr, nullptr, nullptr));
+  TRYM(infer(M, d_x, n, d_w, d_grad, nullptr, nullptr, nullptr, nullptr));
   MCHK(hipStreamSynchronize(M->s));
   return CGS_OK;
 }
@@ -437,7 +444,16 @@ int cgs_mlp_infer(cgs_mlp* M, const float* d_x, uint64_t n, float* d_w, float* d
 // used by the surrogate binding in runtime.cpp
 int cgs_mlp_infer_residual(cgs_mlp* M, const float* d_x, uint64_t n, float* d_w, float* d_grad,
                            const float* d_cage, const float* minv16, float* d_residual) {
-  return infer(M, d_x, n, d_w, d_grad, d_cage, minv16, d_residual);
+  return infer(M, d_x, n, d_w, d_grad, d_cage, minv16, d_residual, nullptr);
+}
+int cgs_mlp_infer_nearfar(cgs_mlp* M, const float* d_x, uint64_t n, float* d_w, float* d_grad,
+                          const CgsNearArgs& near) {
+  return infer(M, d_x, n, d_w, d_grad, nullptr, nullptr, nullptr, &near);
+}
+bool cgs_mlp_near(cgs_mlp* M, float* r0, float* r1) {
+  *r0 = M->near_r0;
+  *r1 = M->near_r1;
+  return M->near_r1 > 0.0f;
 }
 hipStream_t cgs_mlp_stream(cgs_mlp* M) { return M->s; }
 uint32_t cgs_mlp_outputs(cgs_mlp* M) { return M->n_out; }

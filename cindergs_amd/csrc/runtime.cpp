@@ -203,6 +203,9 @@ struct FrameArena {
   }
 };
 
+struct NearCache;
+static void near_cache_free(NearCache* n);
+
 struct cgs_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -234,6 +237,7 @@ struct cgs_context {
   FrameArena fa;
   uint32_t* h_counters = nullptr;  // pinned [4]: survivors, split_count, total, pad
   uint32_t* scan_scratch = nullptr;  // this device's copy of `scratch` (never the symbol itself)
+  NearCache* near_cache = nullptr;  // the last near/far binding's cage buffers and grid
   DevBuf prim_ctrl, prim_alt_k, prim_alt_v, prim_alt_d;
   // MVC scene deformation: per-call working set
   DevBuf mvc_work;
@@ -1047,6 +1051,8 @@ void cgs_destroy(cgs_context* c) {
                     &c->prim_alt_k, &c->prim_alt_v, &c->prim_alt_d, &c->mvc_work,
                     &c->ingest, &c->tet_shm};
   for (DevBuf* b : bufs) release(*b);
+  near_cache_free(c->near_cache);
+  c->near_cache = nullptr;
   c->fa.release_all();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -2162,11 +2168,316 @@ int cgs_mvc_winding(cgs_context* c, const float* cage_vertices, uint32_t num_ver
   return CGS_OK;
 }
 
+// ---- near/far split of the MVC sums (the learned binding) ----
+
+// (A A^T)^-1 of A = [1 ... 1; v_1 ... v_nv] (4 x nv), in double, row-major
+// into minv; false when the vertices are coplanar.
+static bool linear_precision_minv(const float* verts, uint32_t nv, float minv[16]) {
+  double G[4][4] = {{0}};
+  for (uint32_t v = 0; v < nv; ++v) {
+    const double a[4] = {1.0, verts[(size_t)v * 3], verts[(size_t)v * 3 + 1], verts[(size_t)v * 3 + 2]};
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) G[i][k] += a[i] * a[k];
+  }
+  double aug[4][8];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 8; ++k) aug[i][k] = k < 4 ? G[i][k] : (k - 4 == i ? 1.0 : 0.0);
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    for (int r2 = col + 1; r2 < 4; ++r2)
+      if (std::fabs(aug[r2][col]) > std::fabs(aug[piv][col])) piv = r2;
+    if (std::fabs(aug[piv][col]) < 1e-12) return false;
+    for (int k = 0; k < 8; ++k) std::swap(aug[col][k], aug[piv][k]);
+    const double d = aug[col][col];
+    for (int k = 0; k < 8; ++k) aug[col][k] /= d;
+    for (int r2 = 0; r2 < 4; ++r2)
+      if (r2 != col) {
+        const double f = aug[r2][col];
+        for (int k = 0; k < 8; ++k) aug[r2][k] -= f * aug[col][k];
+      }
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k) minv[i * 4 + k] = (float)aug[i][4 + k];
+  return true;
+}
+
+static float cage_diagonal(const float* verts, uint32_t nv) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = cgs_fmin(lo[k], verts[(size_t)v * 3 + k]);
+      hi[k] = cgs_fmax(hi[k], verts[(size_t)v * 3 + k]);
+    }
+  return sqrtf(((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1])) +
+               (hi[2] - lo[2]) * (hi[2] - lo[2]));
+}
+
+static std::vector<float> triangle_centroids(const float* verts, const uint32_t* tris, uint32_t nt) {
+  std::vector<float> cent((size_t)nt * 3);
+  for (uint32_t t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k)
+      cent[(size_t)t * 3 + k] = ((verts[(size_t)tris[t * 3] * 3 + k] + verts[(size_t)tris[t * 3 + 1] * 3 + k]) +
+                                 verts[(size_t)tris[t * 3 + 2] * 3 + k]) / 3.0f;
+  return cent;
+}
+
+// A uniform grid over the cage's bounding box (32 cells along the longest
+// side) listing, per cell, every triangle whose centroid lies within
+// r1 + h + (cell half-diagonal) of the cell centre: a superset of the
+// triangles with phi > 0 at any point of the cell or h away from it.
+struct NearGrid {
+  void* d_cent = nullptr;
+  void* d_start = nullptr;
+  void* d_list = nullptr;
+  float lo[3] = {0.f, 0.f, 0.f}, inv_cell = 0.f;
+  uint32_t dim[3] = {1u, 1u, 1u};
+  ~NearGrid() {
+    for (void* p : {d_cent, d_start, d_list})
+      if (p) (void)hipFree(p);
+  }
+};
+
+static int near_grid_build(cgs_context* c, const float* verts, uint32_t nv, const uint32_t* tris,
+                           uint32_t nt, float r1, float h, NearGrid& G) {
+  const std::vector<float> cent = triangle_centroids(verts, tris, nt);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = cgs_fmin(lo[k], verts[(size_t)v * 3 + k]);
+      hi[k] = cgs_fmax(hi[k], verts[(size_t)v * 3 + k]);
+    }
+  const float ext = cgs_fmax(cgs_fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  if (!(ext > 0.0f)) return fail(CGS_ERR_INVALID_ARG, "degenerate cage bounding box");
+  const float cell = ext / 32.0f;
+  for (int k = 0; k < 3; ++k) {
+    G.lo[k] = lo[k];
+    G.dim[k] = (uint32_t)((hi[k] - lo[k]) / cell) + 1u;
+  }
+  G.inv_cell = 1.0f / cell;
+  const size_t cells = (size_t)G.dim[0] * G.dim[1] * G.dim[2];
+  const double R = (double)r1 + (double)h + 0.5 * (double)cell * std::sqrt(3.0);
+  std::vector<uint32_t> start(cells + 1, 0u), list;
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<uint32_t> fill;
+    if (pass == 1) {
+      for (size_t i = 0; i < cells; ++i) start[i + 1] += start[i];
+      if (start[cells] > (1u << 28)) return fail(CGS_ERR_INVALID_ARG, "near-field lists too large");
+      list.resize(start[cells] ? start[cells] : 1);
+      fill.assign(start.begin(), start.end() - 1);
+    }
+    for (uint32_t t = 0; t < nt; ++t) {
+      int a0[3], a1[3];
+      for (int k = 0; k < 3; ++k) {
+        const double ck = cent[(size_t)t * 3 + k];
+        a0[k] = std::max(0, (int)std::floor((ck - R - lo[k]) / cell));
+        a1[k] = std::min((int)G.dim[k] - 1, (int)std::floor((ck + R - lo[k]) / cell));
+      }
+      for (int z = a0[2]; z <= a1[2]; ++z)
+        for (int y = a0[1]; y <= a1[1]; ++y)
+          for (int x = a0[0]; x <= a1[0]; ++x) {
+            const double dx = lo[0] + (x + 0.5) * cell - cent[(size_t)t * 3];
+            const double dy = lo[1] + (y + 0.5) * cell - cent[(size_t)t * 3 + 1];
+            const double dz = lo[2] + (z + 0.5) * cell - cent[(size_t)t * 3 + 2];
+            if (dx * dx + dy * dy + dz * dz > R * R) continue;
+            const size_t ci = ((size_t)z * G.dim[1] + y) * G.dim[0] + x;
+            if (pass == 0)
+              start[ci + 1]++;
+            else
+              list[fill[ci]++] = t;
+          }
+    }
+  }
+  HIPCHK(hipMalloc(&G.d_cent, (size_t)nt * 12));
+  HIPCHK(hipMalloc(&G.d_start, (cells + 1) * 4));
+  HIPCHK(hipMalloc(&G.d_list, list.size() * 4));
+  HIPCHK(hipMemcpyAsync(G.d_cent, cent.data(), (size_t)nt * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(G.d_start, start.data(), (cells + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(G.d_list, list.data(), list.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+  return CGS_OK;
+}
+
+static int check_near_radii(float r0, float r1) {
+  if (!(r0 >= 0.0f && r1 > r0 && r1 < 1e6f))
+    return fail(CGS_ERR_INVALID_ARG, "near field radii need 0 <= r0 < r1");
+  return CGS_OK;
+}
+
+// Training targets of the near/far network: the far share q of the exact
+// weights (u - u_near) / sum u, times num_vertices, per point.
+int cgs_mvc_far_share(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                      const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                      float near_r1, const float* d_points, uint64_t n, float* d_out) {
+  TRY(ensure_ctx(c));
+  TRY(check_near_radii(near_r0, near_r1));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!d_points || !d_out) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  MvcCage M;
+  TRY(mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M));
+  const std::vector<float> cent = triangle_centroids(cage_vertices, triangles, num_triangles);
+  void* d_cent = nullptr;
+  HIPCHK(hipMalloc(&d_cent, cent.size() * 4));
+  hipError_t e = hipMemcpyAsync(d_cent, cent.data(), cent.size() * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    CgsMvcArgs A;
+    memset(&A, 0, sizeof(A));
+    A.verts = (const float*)M.d_verts;
+    A.nv = num_vertices;
+    A.tris = (const uint32_t*)M.d_tris;
+    A.nt = num_triangles;
+    A.vt_start = (const uint32_t*)M.d_start;
+    A.vt_list = (const uint32_t*)M.d_list;
+    A.pts = d_points;
+    A.n = (uint32_t)n;
+    cgs_launch_mvc_far_share(A, (const float*)d_cent, near_r0, near_r1, (float)num_vertices, d_out,
+                             c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_cent);
+  if (e != hipSuccess) return fail(CGS_ERR_EXECUTION, std::string("far share: ") + hipGetErrorString(e));
+  return CGS_OK;
+}
+
+// The cage buffers and the near-field grid of the last near/far binding,
+// kept by the context and reused while the cage (vertices, triangles) and the
+// radii stay the same: the grid costs more host time than a small binding.
+struct NearCache {
+  uint64_t key = 0;
+  MvcCage M;
+  NearGrid G;
+  float minv[16];
+};
+
+static void near_cache_free(NearCache* n) { delete n; }
+
+static uint64_t near_key(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float r0,
+                         float r1) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto mix = [&](const void* p, size_t bytes) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t k = 0; k < bytes; ++k) {
+      h ^= b[k];
+      h *= 0x100000001b3ull;
+    }
+  };
+  mix(&nv, 4);
+  mix(&nt, 4);
+  mix(&r0, 4);
+  mix(&r1, 4);
+  mix(verts, (size_t)nv * 12);
+  mix(tris, (size_t)nt * 12);
+  return h;
+}
+
+// Shared set-up of a near/far binding: the cached cage buffers and grid,
+// the winding numbers of every point (one point per lane) and the kernel
+// arguments.
+struct NearSetup {
+  NearCache* C = nullptr;  // owned by the context
+  void* d_wind = nullptr;
+  CgsNearArgs a;
+  ~NearSetup() {
+    if (d_wind) (void)hipFree(d_wind);
+  }
+};
+
+static int near_setup(cgs_context* c, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                      float r0, float r1, const float* d_points, uint64_t n, NearSetup& S) {
+  TRY(check_near_radii(r0, r1));
+  if (nv > CGS_NEARFAR_MAX_VERTICES)
+    return fail(CGS_ERR_INVALID_ARG, "the near/far binding takes cages of at most 2048 vertices");
+  if (!verts || !tris) return fail(CGS_ERR_INVALID_ARG, "null cage");
+  const uint64_t key = near_key(verts, nv, tris, nt, r0, r1);
+  if (!c->near_cache || c->near_cache->key != key) {
+    near_cache_free(c->near_cache);
+    c->near_cache = nullptr;
+    NearCache* C = new NearCache();
+    int rc = mvc_upload(c, verts, nv, tris, nt, C->M);
+    if (rc == CGS_OK) rc = near_grid_build(c, verts, nv, tris, nt, r1, C->M.h, C->G);
+    if (rc == CGS_OK && !linear_precision_minv(verts, nv, C->minv))
+      rc = fail(CGS_ERR_INVALID_ARG, "cage vertices are coplanar");
+    if (rc != CGS_OK) {
+      delete C;
+      return rc;
+    }
+    C->key = key;
+    c->near_cache = C;
+  }
+  NearCache* C = c->near_cache;
+  S.C = C;
+  memset(&S.a, 0, sizeof(S.a));
+  memcpy(S.a.minv, C->minv, sizeof(S.a.minv));
+  HIPCHK(hipMalloc(&S.d_wind, n * 4));
+  cgs_launch_mvc_winding_lane((const float*)C->M.d_verts, (const uint32_t*)C->M.d_tris, nt, d_points,
+                              (uint32_t)n, (float*)S.d_wind, c->stream);
+  CgsNearArgs& a = S.a;
+  a.verts = (const float*)C->M.d_verts;
+  a.nv = nv;
+  a.tris = (const uint32_t*)C->M.d_tris;
+  a.nt = nt;
+  a.cent = (const float*)C->G.d_cent;
+  a.cell_start = (const uint32_t*)C->G.d_start;
+  a.cell_list = (const uint32_t*)C->G.d_list;
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = C->G.lo[k];
+    a.dim[k] = C->G.dim[k];
+  }
+  a.inv_cell = C->G.inv_cell;
+  a.r0 = r0;
+  a.r1 = r1;
+  a.h = C->M.h;
+  // every interior point has sum u >= 4 pi / diagonal; tau is 1e-4 of that,
+  // so the near term is short by at most 5e-5 of the weights' sum where it
+  // fades out (the projection restores the sum)
+  a.tau = 1.2566371e-3f / cgs_fmax(cage_diagonal(verts, nv), 1e-6f);
+  a.pts = d_points;
+  a.n = (uint32_t)n;
+  a.winding = (const float*)S.d_wind;
+  return CGS_OK;
+}
+
+// The near/far combination with caller-supplied far shares: d_q [n x nv]
+// (q x nv, as cgs_mvc_far_share writes them) and d_dq [3n x nv] (row
+// 3p + axis, the derivatives of those rows) or NULL.  Outputs as
+// cgs_mvc_bind_exact plus d_residual [n] (|sum w v - x| before the
+// projection; +inf where the near sums meet a vertex or a face).
+int cgs_mvc_nearfar_bind(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                         const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                         float near_r1, const float* d_points, uint64_t n, const float* d_q,
+                         const float* d_dq, float* d_weights, float* d_gradients,
+                         uint32_t* d_inside, float* d_residual) {
+  TRY(ensure_ctx(c));
+  if (n == 0) return CGS_OK;
+  if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
+  if (!d_points || !d_q || !d_weights || !d_residual) return fail(CGS_ERR_INVALID_ARG, "null buffer");
+  NearSetup S;
+  TRY(near_setup(c, cage_vertices, num_vertices, triangles, num_triangles, near_r0, near_r1, d_points,
+                 n, S));
+  CgsNearArgs a = S.a;
+  a.Q = d_q;
+  a.DQ = d_dq;
+  a.ldq = num_vertices;
+  a.q_scale = (float)num_vertices;
+  a.out_w = d_weights;
+  a.out_g = d_gradients;
+  a.residual = d_residual;
+  a.inside = d_inside;
+  cgs_launch_mvc_nearfar(a, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
+
 // The surrogate binding: the network's weights and gradients for every
 // point, exact MVC (the reference routine) for every point whose
 // reconstruction residual |sum_j w_j v_j - x| exceeds residual_tol times the
 // cage's bounding-box diagonal, zero rows outside the cage (same inside test
-// as the exact binding).  Same outputs as cgs_mvc_bind_exact.
+// as the exact binding).  Same outputs as cgs_mvc_bind_exact.  A network
+// with a near field (cgs_mlp_set_near_field) gives the far share only: the
+// near sums are evaluated exactly per point (k_mvc_nearfar) and the inside
+// test runs one point per lane.
 int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices,
                  uint32_t num_vertices, const uint32_t* triangles, uint32_t num_triangles,
                  const float* d_points, uint64_t n, float residual_tol, float* d_weights,
@@ -2181,17 +2492,13 @@ int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices
     return CGS_OK;
   }
   if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
-  MvcCage M;
-  TRY(mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M));
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t v = 0; v < num_vertices; ++v)
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = cgs_fmin(lo[k], cage_vertices[(size_t)v * 3 + k]);
-      hi[k] = cgs_fmax(hi[k], cage_vertices[(size_t)v * 3 + k]);
-    }
-  const float diag = sqrtf(((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1])) +
-                           (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  float r0 = 0.0f, r1 = 0.0f;
+  const bool near = cgs_mlp_near(net, &r0, &r1);
+  const float diag = cage_diagonal(cage_vertices, num_vertices);
   const uint32_t nb = cgs_mvc_select_blocks((uint32_t)n);
+  NearSetup S;
+  MvcCage M0;
+  MvcCage* M = &M0;
   void *d_wind = nullptr, *d_res = nullptr, *d_flags = nullptr, *d_counts = nullptr;
   auto cleanup = [&]() {
     for (void* p : {d_wind, d_res, d_flags, d_counts})
@@ -2199,65 +2506,61 @@ int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices
   };
   int rc = CGS_OK;
   do {
-    if (hipMalloc(&d_wind, n * 4) != hipSuccess || hipMalloc(&d_res, n * 4) != hipSuccess ||
-        hipMalloc(&d_flags, n * 4) != hipSuccess || hipMalloc(&d_counts, (size_t)nb * 4) != hipSuccess) {
+    if (hipMalloc(&d_res, n * 4) != hipSuccess || hipMalloc(&d_flags, n * 4) != hipSuccess ||
+        hipMalloc(&d_counts, (size_t)nb * 4) != hipSuccess) {
       rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (surrogate binding)");
       break;
     }
-    CgsMvcArgs A;
-    memset(&A, 0, sizeof(A));
-    A.verts = (const float*)M.d_verts;
-    A.nv = num_vertices;
-    A.tris = (const uint32_t*)M.d_tris;
-    A.nt = num_triangles;
-    A.vt_start = (const uint32_t*)M.d_start;
-    A.vt_list = (const uint32_t*)M.d_list;
-    A.pts = d_points;
-    A.n = (uint32_t)n;
-    A.h = M.h;
-    cgs_launch_mvc_winding(A, (float*)d_wind, c->stream);
-    // (A A^T)^-1 for the linear-precision projection, A = [1; v] (4 x nv)
-    double G[4][4] = {{0}};
-    for (uint32_t v = 0; v < num_vertices; ++v) {
-      const double a[4] = {1.0, cage_vertices[(size_t)v * 3], cage_vertices[(size_t)v * 3 + 1],
-                           cage_vertices[(size_t)v * 3 + 2]};
-      for (int i = 0; i < 4; ++i)
-        for (int k = 0; k < 4; ++k) G[i][k] += a[i] * a[k];
-    }
-    double aug[4][8];
-    for (int i = 0; i < 4; ++i)
-      for (int k = 0; k < 8; ++k) aug[i][k] = k < 4 ? G[i][k] : (k - 4 == i ? 1.0 : 0.0);
-    bool singular = false;
-    for (int col = 0; col < 4 && !singular; ++col) {
-      int piv = col;
-      for (int r2 = col + 1; r2 < 4; ++r2)
-        if (std::fabs(aug[r2][col]) > std::fabs(aug[piv][col])) piv = r2;
-      if (std::fabs(aug[piv][col]) < 1e-12) {
-        singular = true;
+    const float* wind = nullptr;
+    if (near) {
+      if ((rc = near_setup(c, cage_vertices, num_vertices, triangles, num_triangles, r0, r1, d_points,
+                           n, S)) != CGS_OK)
+        break;
+      M = &S.C->M;
+      wind = (const float*)S.d_wind;
+      CgsNearArgs a = S.a;
+      a.residual = (float*)d_res;
+      a.inside = d_inside;
+      if ((rc = cgs_mlp_infer_nearfar(net, d_points, n, d_weights, d_gradients, a)) != CGS_OK) break;
+    } else {
+      if ((rc = mvc_upload(c, cage_vertices, num_vertices, triangles, num_triangles, M0)) != CGS_OK) break;
+      if (hipMalloc(&d_wind, n * 4) != hipSuccess) {
+        rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (surrogate binding)");
         break;
       }
-      for (int k = 0; k < 8; ++k) std::swap(aug[col][k], aug[piv][k]);
-      const double d = aug[col][col];
-      for (int k = 0; k < 8; ++k) aug[col][k] /= d;
-      for (int r2 = 0; r2 < 4; ++r2)
-        if (r2 != col) {
-          const double f = aug[r2][col];
-          for (int k = 0; k < 8; ++k) aug[r2][k] -= f * aug[col][k];
-        }
+      CgsMvcArgs W;
+      memset(&W, 0, sizeof(W));
+      W.verts = (const float*)M0.d_verts;
+      W.nv = num_vertices;
+      W.tris = (const uint32_t*)M0.d_tris;
+      W.nt = num_triangles;
+      W.pts = d_points;
+      W.n = (uint32_t)n;
+      cgs_launch_mvc_winding(W, (float*)d_wind, c->stream);
+      wind = (const float*)d_wind;
+      float minv[16];
+      if (!linear_precision_minv(cage_vertices, num_vertices, minv)) {
+        rc = fail(CGS_ERR_INVALID_ARG, "cage vertices are coplanar");
+        break;
+      }
+      rc = cgs_mlp_infer_residual(net, d_points, n, d_weights, d_gradients, (const float*)M0.d_verts,
+                                  minv, (float*)d_res);
+      if (rc != CGS_OK) break;
     }
-    if (singular) {
-      rc = fail(CGS_ERR_INVALID_ARG, "cage vertices are coplanar");
-      break;
-    }
-    float minv[16];
-    for (int i = 0; i < 4; ++i)
-      for (int k = 0; k < 4; ++k) minv[i * 4 + k] = (float)aug[i][4 + k];
-    rc = cgs_mlp_infer_residual(net, d_points, n, d_weights, d_gradients,
-                                (const float*)M.d_verts, minv, (float*)d_res);
-    if (rc != CGS_OK) break;
-    cgs_launch_mvc_select((const float*)d_wind, (const float*)d_res, residual_tol * diag,
-                          (uint32_t)n, num_vertices, d_inside, (uint32_t*)d_flags, d_weights,
-                          d_gradients, (uint32_t*)d_counts, c->stream);
+    cgs_launch_mvc_select(wind, (const float*)d_res, residual_tol * diag, (uint32_t)n, num_vertices,
+                          d_inside, (uint32_t*)d_flags, d_weights, d_gradients, (uint32_t*)d_counts,
+                          c->stream);
+    CgsMvcArgs A;
+    memset(&A, 0, sizeof(A));
+    A.verts = (const float*)M->d_verts;
+    A.nv = num_vertices;
+    A.tris = (const uint32_t*)M->d_tris;
+    A.nt = num_triangles;
+    A.vt_start = (const uint32_t*)M->d_start;
+    A.vt_list = (const uint32_t*)M->d_list;
+    A.pts = d_points;
+    A.n = (uint32_t)n;
+    A.h = M->h;
     A.out_w = d_weights;
     A.out_g = d_gradients;
     A.mask = d_inside;

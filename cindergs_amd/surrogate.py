@@ -1,11 +1,13 @@
 """The MVC surrogate network on the GPU (cgs_mlp_* entry points).
 
-A coordinate MLP (Fourier features -> 4 x 256 ReLU -> n_out, affine-normalised
-so the weights sum to 1) trained on exact MVC weights and used to bind many
-points to one closed cage, with exact MVC as the fallback for any point whose
-reconstruction residual |sum_j w_j v_j - x| is above a threshold (the
+A coordinate MLP (Fourier features -> 4 x 256 ReLU -> n_out) used to bind
+many points to one closed cage, with exact MVC as the fallback for any point
+whose reconstruction residual |sum_j w_j v_j - x| is above a threshold (the
 reference has no learned component: it always evaluates exact MVC,
-src/deformation/mvc.rs:498-541).
+src/deformation/mvc.rs:498-541).  Either the network stands for the whole
+weights (affine-normalised so they sum to 1, trained on exact MVC weights), or,
+with a near field, for the far share of MVC's sums only (trained on
+MvcGpu.far_share targets) while the near sums are evaluated exactly.
 """
 from __future__ import annotations
 
@@ -28,6 +30,7 @@ class MvcSurrogate:
         check(self._lib.cgs_mlp_create(state._h, n_out, fourier_sigma, seed, C.byref(h)))
         self._h = h
         self.n_out = n_out
+        self.near_field = (0.0, 0.0)
         self.count = int(self._lib.cgs_mlp_param_count(h))
         rows = (C.c_uint32 * LAYERS)()
         cols = (C.c_uint32 * LAYERS)()
@@ -63,6 +66,13 @@ class MvcSurrogate:
         r, c, o = self.rows[l], self.cols[l], self.offsets[l]
         blk = params[o:o + r * (c + 32)].reshape(r, c + 32)
         return blk[:, :c], blk[:, c]
+
+    def set_near_field(self, r0: float, r1: float):
+        """Near/far split (cgs_mlp_set_near_field): the network then learns the
+        far share of the weights (targets from MvcGpu.far_share) and the
+        binding adds the exactly evaluated near sums.  (0, 0) switches it off."""
+        check(self._lib.cgs_mlp_set_near_field(self._h, float(r0), float(r1)))
+        self.near_field = (float(r0), float(r1))
 
     # ---- training and inference ------------------------------------------
     def train(self, d_x: DeviceBuffer, d_t: DeviceBuffer, n: int, steps: int, batch: int = 8192,

@@ -495,11 +495,39 @@ int cgs_mlp_train(cgs_mlp* net, const float* d_x, const float* d_targets, uint64
  * multiple of 64), without an update */
 int cgs_mlp_gradient(cgs_mlp* net, const float* d_x, const float* d_targets, uint32_t n,
                      float* grad, float* loss);
-/* weights [n*n_out] and, if d_grad, their input derivatives [n*n_out*3] */
+/* weights [n*n_out] and, if d_grad, their input derivatives [n*n_out*3]; with a
+ * near field set, the raw outputs (far share x n_out) and their derivatives */
 int cgs_mlp_infer(cgs_mlp* net, const float* d_x, uint64_t n, float* d_w, float* d_grad);
+/* Near/far split (r0 < r1, both 0 to switch it off).  MVC's sums over the
+ * cage triangles split with phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)),
+ * c_t the triangle's centroid: the near sums u_near = sum_t phi_t mu_t are
+ * evaluated exactly at bind time, the network learns the far share
+ * q = (u - u_near) / sum u (scaled by n_out; targets from cgs_mvc_far_share,
+ * plain squared error, no affine normalisation) and the binding's weights are
+ * w = (1 - sum q) u_near / sum u_near + q (|sum u_near| floored at a small tau,
+ * sign kept), projected onto linear
+ * precision. */
+int cgs_mlp_set_near_field(cgs_mlp* net, float r0, float r1);
+/* Training targets for a near/far network: q x num_vertices per point,
+ * d_out [n x num_vertices] (zero rows for points on a vertex or a face). */
+int cgs_mvc_far_share(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
+                      const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                      float near_r1, const float* d_points, uint64_t n, float* d_out);
+/* The near/far combination of cgs_mlp_bind with the far shares given by the
+ * caller: d_q [n x num_vertices] (as cgs_mvc_far_share writes them), d_dq
+ * [3n x num_vertices] (row 3p + axis: their derivatives) or NULL.  Outputs as
+ * cgs_mvc_bind_exact plus d_residual [n]: |sum w v - x| before the
+ * linear-precision projection (+inf where the near sums meet a vertex or a
+ * face).  Exact far shares give exact MVC weights. */
+int cgs_mvc_nearfar_bind(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
+                         const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                         float near_r1, const float* d_points, uint64_t n, const float* d_q,
+                         const float* d_dq, float* d_weights, float* d_gradients,
+                         uint32_t* d_inside, float* d_residual);
 /* The surrogate for PolyCageMesh::bind_gaussians (mvc.rs:498-541): same outputs
  * as cgs_mvc_bind_exact (zero rows outside), the network's weights and input
- * derivatives for inside points whose residual |sum_j w_j v_j - x| is at most
+ * derivatives (with a near field: combined with the exact near sums) for
+ * inside points whose residual |sum_j w_j v_j - x| is at most
  * residual_tol x (cage bounding-box diagonal), exact MVC for the others;
  * fallback_count receives how many points took the exact path. */
 int cgs_mlp_bind(cgs_context* ctx, cgs_mlp* net, const float* cage_vertices,

@@ -1293,6 +1293,73 @@ static void mvc_weights(const float* V, uint32_t nv, const uint32_t* F, uint32_t
   }
 }
 
+// Far share of the MVC sums for the near/far network: triangle t's terms
+// scaled by 1 - phi_t(x), phi_t = smoothstep((r1 - |x - c_t|) / (r1 - r0)),
+// c_t = (v0 + v1 + v2) / 3 in f32; normalised by the full sum, times nv.
+// Points on a vertex or a face, or with a zero sum, give a zero row.
+template <class T>
+static void mvc_far_share(const float* V, uint32_t nv, const uint32_t* F, uint32_t nt, const T x[3],
+                          T r0, T r1, T* q, T* dist, W3<T>* dir) {
+  typedef Tr<T> m;
+  const T PI_ = (T)3.14159265358979323846f;
+  const W3<T> X = {x[0], x[1], x[2]};
+  for (uint32_t k = 0; k < nv; ++k) q[k] = 0;
+  for (uint32_t i = 0; i < nv; ++i) {
+    const W3<T> diff = w_sub(w_at<T>(V, i), X);
+    const T d = w_len(diff);
+    if (d < (T)1e-6f) return;
+    dist[i] = d;
+    dir[i] = {diff.x / d, diff.y / d, diff.z / d};
+  }
+  T total = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t i0 = F[t * 3], i1 = F[t * 3 + 1], i2 = F[t * 3 + 2];
+    const W3<T> u0 = dir[i0], u1 = dir[i1], u2 = dir[i2];
+    const T th0 = (T)2 * m::asin_(w_clamp(w_len(w_sub(u1, u2)) * (T)0.5, (T)0, (T)1));
+    const T th1 = (T)2 * m::asin_(w_clamp(w_len(w_sub(u2, u0)) * (T)0.5, (T)0, (T)1));
+    const T th2 = (T)2 * m::asin_(w_clamp(w_len(w_sub(u0, u1)) * (T)0.5, (T)0, (T)1));
+    const T h = (T)0.5 * ((th0 + th1) + th2);
+    if (fabs((double)(PI_ - h)) < (double)(T)1e-5f) {
+      for (uint32_t k = 0; k < nv; ++k) q[k] = 0;
+      return;
+    }
+    const T s_t0 = m::sin_(th0), s_t1 = m::sin_(th1), s_t2 = m::sin_(th2);
+    if (fabs((double)s_t0) < (double)(T)1e-6f || fabs((double)s_t1) < (double)(T)1e-6f ||
+        fabs((double)s_t2) < (double)(T)1e-6f)
+      continue;
+    const T sin_h = m::sin_(h);
+    const T c0 = w_clamp(((T)2 * sin_h * m::sin_(h - th0)) / (s_t1 * s_t2) - (T)1, (T)-1, (T)1);
+    const T c1 = w_clamp(((T)2 * sin_h * m::sin_(h - th1)) / (s_t2 * s_t0) - (T)1, (T)-1, (T)1);
+    const T c2 = w_clamp(((T)2 * sin_h * m::sin_(h - th2)) / (s_t0 * s_t1) - (T)1, (T)-1, (T)1);
+    const T sgn = w_dot(u2, w_cross(u0, u1)) >= (T)0 ? (T)1 : (T)-1;
+    const T s0 = sgn * m::sqrt_(w_max((T)1 - c0 * c0, (T)0));
+    const T s1 = sgn * m::sqrt_(w_max((T)1 - c1 * c1, (T)0));
+    const T s2 = sgn * m::sqrt_(w_max((T)1 - c2 * c2, (T)0));
+    if (fabs((double)s0) <= (double)(T)1e-6f || fabs((double)s1) <= (double)(T)1e-6f ||
+        fabs((double)s2) <= (double)(T)1e-6f)
+      continue;
+    const T mu0 = (th0 - c1 * th2 - c2 * th1) / ((T)2 * dist[i0] * s_t1 * s2);
+    const T mu1 = (th1 - c2 * th0 - c0 * th2) / ((T)2 * dist[i1] * s_t2 * s0);
+    const T mu2 = (th2 - c0 * th1 - c1 * th0) / ((T)2 * dist[i2] * s_t0 * s1);
+    total += (mu0 + mu1) + mu2;
+    float cf[3];
+    for (int k = 0; k < 3; ++k)
+      cf[k] = ((V[(size_t)i0 * 3 + k] + V[(size_t)i1 * 3 + k]) + V[(size_t)i2 * 3 + k]) / 3.0f;
+    const W3<T> dc = {X.x - (T)cf[0], X.y - (T)cf[1], X.z - (T)cf[2]};
+    const T tt = w_clamp((r1 - w_len(dc)) / (r1 - r0), (T)0, (T)1);
+    const T far = (T)1 - tt * tt * ((T)3 - (T)2 * tt);
+    q[i0] += far * mu0;
+    q[i1] += far * mu1;
+    q[i2] += far * mu2;
+  }
+  if (!(fabs((double)total) > (double)(T)1e-12f)) {
+    for (uint32_t k = 0; k < nv; ++k) q[k] = 0;
+    return;
+  }
+  const T scale = (T)nv / total;
+  for (uint32_t k = 0; k < nv; ++k) q[k] *= scale;
+}
+
 // jacobian_step_size (mvc.rs:485-495)
 template <class T> static T mvc_step(const float* V, uint32_t nv) {
   T lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1393,6 +1460,25 @@ ORC_API void orc_mvc_weights(const float* verts, uint32_t nv, const uint32_t* tr
 }
 
 // ---------------------------------------------------------------------------
+// far shares of n points: [n*nv], times nv (fp64: double output)
+ORC_API void orc_mvc_far_share(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                               const float* pts, uint64_t n, float r0, float r1, int fp64, void* out) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    if (fp64) {
+      std::vector<double> dist(nv);
+      std::vector<W3<double>> dir(nv);
+      const double p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_far_share<double>(verts, nv, tris, nt, p, r0, r1, (double*)out + i * nv, dist.data(), dir.data());
+    } else {
+      std::vector<float> dist(nv);
+      std::vector<W3<float>> dir(nv);
+      const float p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_far_share<float>(verts, nv, tris, nt, p, r0, r1, (float*)out + i * nv, dist.data(), dir.data());
+    }
+  }
+}
+
 // MVC scene deformation: PolyCageMesh::deform_scene_cpu (mvc.rs:564-749)
 // ---------------------------------------------------------------------------
 //

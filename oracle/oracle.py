@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
             "orc_tile_ranges": ([_vp, _u64, _u32, _u32, _vp], None),
             "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _vp, _i, _vp, _vp],
                                  None),
+            "orc_mvc_far_share": ([_vp, _u32, _vp, _u32, _vp, _u64, _f, _f, _i, _vp], None),
             "orc_mvc_scene": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp,
                                _vp, _vp], _u64),
             "orc_sh_tables": ([_vp], None),
@@ -367,6 +368,17 @@ def mvc_bind(verts, tris, points, fp64=False, gradients=True):
     m = np.zeros(n, np.uint8)
     L.orc_mvc_bind(_p(v), nv, _p(t), t.shape[0], _p(p), n, 1 if fp64 else 0, _p(w), _p(g), _p(m))
     return w, g, m.astype(bool)
+
+
+def mvc_far_share(verts, tris, points, r0, r1, fp64=True):
+    """Far share of MVC's sums (triangle terms scaled by 1 - phi_t), times nv."""
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.uint32)
+    p = np.ascontiguousarray(points, np.float32)
+    out = np.zeros((p.shape[0], v.shape[0]), np.float64 if fp64 else np.float32)
+    lib().orc_mvc_far_share(_p(v), v.shape[0], _p(t), t.shape[0], _p(p), p.shape[0], r0, r1,
+                            1 if fp64 else 0, _p(out))
+    return out
 
 
 def mvc_winding(verts, tris, points, fp64=False):
