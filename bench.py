@@ -395,7 +395,7 @@ def export_deformed(st, scene, num_tets: int, bind_tet, reps: int = 5):
     return res
 
 
-def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.3, 0.55),
+def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
                 residual_tol: float = 1e-2, n_err: int = 200_000):
     """Exact MVC binding (weights + gradients) of n_points inside a 642-vertex
     icosphere cage against the learned binding: the network (Fourier features
@@ -842,6 +842,10 @@ def main() -> int:
                     help="only the cage-deformation leg (frames, Stage 1, scene export)")
     ap.add_argument("--mvc-steps", type=int, default=20000)
     ap.add_argument("--mvc-train", type=int, default=2_000_000)
+    ap.add_argument("--mvc-near", default="0.25,0.5", metavar="R0,R1",
+                    help="near/far split radii of the learned binding")
+    ap.add_argument("--mvc-tol", type=float, default=1e-2,
+                    help="residual tolerance of the learned binding (x cage diagonal)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -1025,7 +1029,9 @@ def main() -> int:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         result["batched"] = batched_views(args.batch_views, args.batch_reps, lanes, fixed=fixed)
     if args.mvc > 0:
-        result["mvc"] = mvc_binding(args.mvc, args.mvc_steps, args.mvc_train)
+        near = tuple(float(v) for v in args.mvc_near.split(","))
+        result["mvc"] = mvc_binding(args.mvc, args.mvc_steps, args.mvc_train, near=near,
+                                    residual_tol=args.mvc_tol)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.warmup, args.cpu_budget)
     else:

@@ -436,17 +436,19 @@ struct CgsMvcArgs {
   float* out_w;             // [n][nv]
   float* out_g;             // [n][nv][3] or null
   uint32_t* mask;           // [n] inside flags or null
-  const uint32_t* only;     // [n] points to bind (others untouched) or null = all
+  const uint32_t* list;     // [n] indices of the points to bind (others untouched),
+                            //   or null: points 0..n-1
 };
 size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt);
 void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s);
 void cgs_launch_mvc_winding(const CgsMvcArgs& a, float* out, hipStream_t s);
-// inside = |winding| >= 0.5; flag = inside && residual > tol; rows of points
-// outside zeroed (w [n x nv], g [n x nv x 3] or null); block counts of flags
+// inside = |winding| >= 0.5 (winding given) or the flags already in
+// `inside` (winding null); points inside whose residual exceeds tol are
+// appended to list (count in *count, zeroed by the caller; the order is
+// arbitrary); rows of points outside zeroed (w [n x nv], g [n x nv x 3] or null)
 void cgs_launch_mvc_select(const float* winding, const float* residual, float tol, uint32_t n,
-                           uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
-                           uint32_t* block_counts, hipStream_t s);
-uint32_t cgs_mvc_select_blocks(uint32_t n);
+                           uint32_t nv, uint32_t* inside, float* w, float* g, uint32_t* list,
+                           uint32_t* count, hipStream_t s);
 // Near/far split of the MVC sums (k_mvc.hip): triangle t's terms at point x
 // carry the weight phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)) (c_t
 // the triangle's centroid): the near sums u_near = sum_t phi_t mu_t are
@@ -471,9 +473,15 @@ struct CgsNearArgs {
   float r0, r1, h, tau;        // tau: scale below which the near term fades out
   const float* pts;            // [n][3]
   uint32_t n;
-  const float* winding;        // [n] or null (every point inside)
+  const float* winding;        // [n] or null: the inside test is the crossing
+                               //   parity below
+  const uint32_t* cell_inside; // [cells] 1 = cell centre inside the cage (parity
+                               //   reference), 2 = undecided (points there are
+                               //   flagged for the exact path)
   const float* Q;              // [n x ldq] far share * q_scale
   const float* DQ;             // [3n x ldq] its derivatives (row 3p + axis) or null
+  const uint16_t* Qh;          // the same in bf16 (used when set)
+  const uint16_t* DQh;
   uint32_t ldq;
   float q_scale;
   float minv[16];              // (A A^T)^-1, A = [1; v] (4 x nv), row-major
@@ -562,6 +570,8 @@ struct CgsMlpInferArgs {
   uint32_t n_out_p;                     // output rows, a multiple of 32
   float* Z;
   float* TZ;
+  uint16_t* Zh;                         // bf16 outputs instead (when set)
+  uint16_t* TZh;
   uint32_t ldz;
 };
 void cgs_launch_mlp_infer(const CgsMlpInferArgs& a, hipStream_t s);

@@ -355,6 +355,15 @@ __global__ __launch_bounds__(512) void k_mlp_infer(CgsMlpInferArgs A) {
             if (TAN) {
               const uint32_t p = p0 + i * 4u + (lane >> 4);
               if (p >= A.P) continue;
+              if (A.Zh) {
+                A.Zh[(size_t)p * A.ldz + col] = f2bf(acc[i][j][0] + bv);
+                if (A.TZh) {
+#pragma unroll
+                  for (int a = 0; a < 3; ++a)
+                    A.TZh[((size_t)p * 3 + a) * A.ldz + col] = f2bf(acc[i][j][1 + a]);
+                }
+                continue;
+              }
               A.Z[(size_t)p * A.ldz + col] = acc[i][j][0] + bv;
               if (A.TZ) {
 #pragma unroll
@@ -364,7 +373,11 @@ __global__ __launch_bounds__(512) void k_mlp_infer(CgsMlpInferArgs A) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) {
                 const uint32_t p = p0 + i * 16u + (lane >> 4) * 4u + q;
-                if (p < A.P) A.Z[(size_t)p * A.ldz + col] = acc[i][j][q] + bv;
+                if (p >= A.P) continue;
+                if (A.Zh)
+                  A.Zh[(size_t)p * A.ldz + col] = f2bf(acc[i][j][q] + bv);
+                else
+                  A.Z[(size_t)p * A.ldz + col] = acc[i][j][q] + bv;
               }
             }
           }
@@ -494,7 +507,7 @@ void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const f
 
 void cgs_launch_mlp_infer(const CgsMlpInferArgs& a, hipStream_t s) {
   if (a.P == 0) return;
-  if (a.TZ)
+  if (a.TZ || a.TZh)
     hipLaunchKernelGGL(k_mlp_infer<true>, dim3((a.P + 31u) / 32u), dim3(512), 0, s, a);
   else
     hipLaunchKernelGGL(k_mlp_infer<false>, dim3((a.P + 127u) / 128u), dim3(512), 0, s, a);

@@ -189,9 +189,8 @@ __global__ __launch_bounds__(64) void k_mvc_bind(CgsMvcArgs A) {
   float* s_w = s_mu + 3 * A.nt;
   float* s_w2 = s_w + A.nv;
   const uint32_t lane = threadIdx.x;
-  const uint32_t i = blockIdx.x;
-  if (i >= A.n) return;
-  if (A.only && A.only[i] == 0u) return;
+  if (blockIdx.x >= A.n) return;
+  const uint32_t i = A.list ? A.list[blockIdx.x] : blockIdx.x;
   const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
   float* row = A.out_w + (size_t)i * A.nv;
   float* grow = A.out_g ? A.out_g + (size_t)i * A.nv * 3 : nullptr;
@@ -315,26 +314,18 @@ __global__ __launch_bounds__(256) void k_mvc_deform(const float* w, const float*
 
 __global__ __launch_bounds__(256) void k_mvc_select(const float* winding, const float* residual,
                                                     float tol, uint32_t n, uint32_t nv,
-                                                    uint32_t* inside, uint32_t* flags, float* w,
-                                                    float* g, uint32_t* block_counts) {
-  __shared__ uint32_t s_c[4];
+                                                    uint32_t* inside, float* w, float* g,
+                                                    uint32_t* list, uint32_t* count) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  uint32_t f = 0u;
-  if (i < n) {
-    const bool in = fabsf(winding[i]) >= 0.5f;
-    f = (in && residual[i] > tol) ? 1u : 0u;
-    inside[i] = in ? 1u : 0u;
-    flags[i] = f;
-    if (!in) {
-      for (uint32_t v = 0; v < nv; ++v) w[(size_t)i * nv + v] = 0.0f;
-      if (g)
-        for (uint32_t v = 0; v < 3 * nv; ++v) g[(size_t)i * nv * 3 + v] = 0.0f;
-    }
+  if (i >= n) return;
+  const bool in = winding ? fabsf(winding[i]) >= 0.5f : inside[i] != 0u;
+  if (winding) inside[i] = in ? 1u : 0u;
+  if (in && residual[i] > tol) list[atomicAdd(count, 1u)] = i;
+  if (!in) {
+    for (uint32_t v = 0; v < nv; ++v) w[(size_t)i * nv + v] = 0.0f;
+    if (g)
+      for (uint32_t v = 0; v < 3 * nv; ++v) g[(size_t)i * nv * 3 + v] = 0.0f;
   }
-  const uint32_t c = (uint32_t)__popcll(__ballot(f != 0u));
-  if ((threadIdx.x & 63u) == 0) s_c[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) block_counts[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
 }
 
 
@@ -416,6 +407,8 @@ __global__ __launch_bounds__(256) void k_mvc_scene_children(CgsMvcSceneArgs A,
 // mu_t (the singular, fast-varying part next to the cage) are evaluated
 // exactly; the far sums are smooth and are what the network learns, as the
 // share q = u_far / (u_near + u_far) of the normalised weights.
+__device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 __device__ __forceinline__ float near_phi(float r, float r0, float r1) {
   const float t = cgs_fclamp((r1 - r) / (r1 - r0), 0.0f, 1.0f);
   return t * t * (3.0f - 2.0f * t);
@@ -615,6 +608,35 @@ __device__ __forceinline__ uint32_t near_slot_find(const uint32_t* keys, uint32_
   return NEAR_EMPTY;
 }
 
+// Does the segment [p, c] cross triangle (v0, v1, v2)?  Moller-Trumbore:
+// 0 = no, 1 = yes, 2 = too close to call (the segment passes within 1e-5,
+// in barycentric or segment-length terms, of an edge, a vertex or its own
+// end points, or runs parallel to the triangle's plane inside it).
+__device__ __forceinline__ int segment_crosses(V3f p, V3f c, V3f v0, V3f v1, V3f v2) {
+  const float EPS = 1e-5f;
+  const V3f d = vsub(c, p), e1 = vsub(v1, v0), e2 = vsub(v2, v0);
+  const V3f pv = vcross(d, e2);
+  const float det = vdot(e1, pv);
+  const float scale = vlen(e1) * vlen(e2) * vlen(d);
+  const V3f tv = vsub(p, v0);
+  if (fabsf(det) <= 1e-7f * scale) {
+    // parallel: undecided only if the segment lies in the plane
+    const V3f nrm = vcross(e1, e2);
+    const float dist = fabsf(vdot(tv, nrm)) / fmaxf(vlen(nrm), 1e-30f);
+    return dist <= EPS * vlen(d) ? 2 : 0;
+  }
+  const float inv = 1.0f / det;
+  const float u = vdot(tv, pv) * inv;
+  const V3f qv = vcross(tv, e1);
+  const float v = vdot(d, qv) * inv;
+  const float t = vdot(e2, qv) * inv;
+  const float w = 1.0f - u - v;
+  const bool near_in = u > -EPS && v > -EPS && w > -EPS && t > -EPS && t < 1.0f + EPS;
+  if (!near_in) return 0;
+  const bool in = u > EPS && v > EPS && w > EPS && t > EPS && t < 1.0f - EPS;
+  return in ? 1 : 2;
+}
+
 template <int VPL>
 __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
   __shared__ uint32_t s_key[NEAR_SLOTS];
@@ -626,7 +648,46 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
   const V3f p = {A.pts[(size_t)i * 3], A.pts[(size_t)i * 3 + 1], A.pts[(size_t)i * 3 + 2]};
   float* row = A.out_w + (size_t)i * nv;
   float* grow = A.out_g ? A.out_g + (size_t)i * nv * 3 : nullptr;
-  const bool in = A.winding ? fabsf(A.winding[i]) >= 0.5f : true;
+  const float fx = (p.x - A.lo[0]) * A.inv_cell, fy = (p.y - A.lo[1]) * A.inv_cell,
+              fz = (p.z - A.lo[2]) * A.inv_cell;
+  const bool in_grid = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)A.dim[0] &&
+                       fy < (float)A.dim[1] && fz < (float)A.dim[2];
+  uint32_t c0 = 0u, c1 = 0u, cell = 0u;
+  V3f cc = p;
+  if (in_grid) {
+    const uint32_t cx = min((uint32_t)fx, A.dim[0] - 1u), cy = min((uint32_t)fy, A.dim[1] - 1u),
+                   cz = min((uint32_t)fz, A.dim[2] - 1u);
+    cell = (cz * A.dim[1] + cy) * A.dim[0] + cx;
+    c0 = A.cell_start[cell];
+    c1 = A.cell_start[cell + 1];
+    const float cs = 1.0f / A.inv_cell;
+    cc = {A.lo[0] + ((float)cx + 0.5f) * cs, A.lo[1] + ((float)cy + 0.5f) * cs,
+          A.lo[2] + ((float)cz + 0.5f) * cs};
+  }
+  // inside test: the winding numbers when given, else the parity of the
+  // segment from p to its cell's centre (whose side is known) against the
+  // cell's triangles (every triangle meeting the cell is in its list);
+  // undecided points go to the exact path with the inside flag set
+  bool in;
+  uint32_t degenerate = 0u;
+  if (A.winding) {
+    in = fabsf(A.winding[i]) >= 0.5f;
+  } else if (!in_grid) {
+    in = false;  // outside the bounding box
+  } else {
+    const uint32_t ref = A.cell_inside[cell];
+    uint32_t crossings = 0u, unsure = ref > 1u ? 1u : 0u;
+    for (uint32_t j = c0 + lane; j < c1; j += 64u) {
+      const uint32_t t = A.cell_list[j];
+      const int k = segment_crosses(p, cc, vert(A.verts, A.tris[t * 3]), vert(A.verts, A.tris[t * 3 + 1]),
+                                    vert(A.verts, A.tris[t * 3 + 2]));
+      crossings += k == 1 ? 1u : 0u;
+      unsure |= k == 2 ? 1u : 0u;
+    }
+    const bool odd = ((uint32_t)wave_sum((float)crossings) & 1u) != 0u;
+    degenerate = __ballot(unsure != 0u) != 0ull ? 1u : 0u;
+    in = degenerate ? true : ((ref == 1u) != odd);
+  }
   if (A.inside && lane == 0) A.inside[i] = in ? 1u : 0u;
   if (!in) {
     for (uint32_t v = lane; v < nv; v += 64u) row[v] = 0.0f;
@@ -636,8 +697,8 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
     return;
   }
   const float zs = 1.0f / A.q_scale;
-  const bool tan = grow && A.DQ;
-  const float* q = A.Q + (size_t)i * A.ldq;
+  const bool tan = grow && (A.DQ || A.DQh);
+  const size_t qrow = (size_t)i * A.ldq;
   const uint32_t evals = grow ? 7u : 1u;
   for (uint32_t k = lane; k < NEAR_SLOTS; k += 64u) {
     s_key[k] = NEAR_EMPTY;
@@ -645,19 +706,6 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
     for (int e = 0; e < 7; ++e) s_val[e][k] = 0.0f;
   }
   float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  uint32_t degenerate = 0u;
-  const float fx = (p.x - A.lo[0]) * A.inv_cell, fy = (p.y - A.lo[1]) * A.inv_cell,
-              fz = (p.z - A.lo[2]) * A.inv_cell;
-  const bool in_grid = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)A.dim[0] &&
-                       fy < (float)A.dim[1] && fz < (float)A.dim[2];
-  uint32_t c0 = 0u, c1 = 0u;
-  if (in_grid) {
-    const uint32_t cx = min((uint32_t)fx, A.dim[0] - 1u), cy = min((uint32_t)fy, A.dim[1] - 1u),
-                   cz = min((uint32_t)fz, A.dim[2] - 1u);
-    const uint32_t cell = (cz * A.dim[1] + cy) * A.dim[0] + cx;
-    c0 = A.cell_start[cell];
-    c1 = A.cell_start[cell + 1];
-  }
   const float reach = A.r1 + A.h;
   // chunks of up to NEAR_TRIS candidates: compact those within reach, then
   // the (triangle, evaluation) pairs over the lanes
@@ -723,10 +771,12 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
 #pragma unroll
   for (int k = 0; k < VPL; ++k) {
     const uint32_t v = lane + 64u * k;
-    wv[k] = v < nv ? q[v] * zs : 0.0f;
+    wv[k] = v >= nv ? 0.0f : (A.Qh ? bf16_f32(A.Qh[qrow + v]) : A.Q[qrow + v]) * zs;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-      gv[a][k] = (tan && v < nv) ? A.DQ[((size_t)i * 3 + a) * A.ldq + v] * zs : 0.0f;
+    for (int a = 0; a < 3; ++a) {
+      const size_t t = ((size_t)i * 3 + a) * A.ldq + v;
+      gv[a][k] = (tan && v < nv) ? (A.DQh ? bf16_f32(A.DQh[t]) : A.DQ[t]) * zs : 0.0f;
+    }
   }
   float sq = 0.0f, sdq[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -815,14 +865,12 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
 
 }  // namespace
 
-uint32_t cgs_mvc_select_blocks(uint32_t n) { return (n + 255u) / 256u; }
-
 void cgs_launch_mvc_select(const float* winding, const float* residual, float tol, uint32_t n,
-                           uint32_t nv, uint32_t* inside, uint32_t* flags, float* w, float* g,
-                           uint32_t* block_counts, hipStream_t s) {
+                           uint32_t nv, uint32_t* inside, float* w, float* g, uint32_t* list,
+                           uint32_t* count, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mvc_select, dim3(cgs_mvc_select_blocks(n)), dim3(256), 0, s, winding,
-                     residual, tol, n, nv, inside, flags, w, g, block_counts);
+  hipLaunchKernelGGL(k_mvc_select, dim3((n + 255u) / 256u), dim3(256), 0, s, winding, residual, tol,
+                     n, nv, inside, w, g, list, count);
 }
 
 size_t cgs_mvc_lds_bytes(uint32_t nv, uint32_t nt) {

@@ -56,6 +56,7 @@ struct cgs_mlp {
   // inference buffers (CHUNK points): outputs and their tangents
   bool inf_ready = false;
   Dev<float> iz, itz;
+  Dev<uint16_t> izh, itzh;  // bf16 copies for the near/far binding
   // near/far split radii (0, 0: the network stands for the whole weights)
   float near_r0 = 0.0f, near_r1 = 0.0f;
   std::vector<void*> owned;
@@ -205,6 +206,8 @@ int ensure_infer(cgs_mlp* M) {
   if (M->inf_ready) return CGS_OK;
   TRYM(alloc(M, M->iz, (size_t)CHUNK * M->n_out_p, false));
   TRYM(alloc(M, M->itz, (size_t)3 * CHUNK * M->n_out_p, false));
+  TRYM(alloc(M, M->izh, (size_t)CHUNK * M->n_out_p, false));
+  TRYM(alloc(M, M->itzh, (size_t)3 * CHUNK * M->n_out_p, false));
   M->inf_ready = true;
   return CGS_OK;
 }
@@ -232,6 +235,37 @@ int infer(cgs_mlp* M, const float* X, uint64_t n, float* out_w, float* out_g, co
   TRYM(ensure_infer(M));
   hipStream_t s = M->s;
   CgsMlpInferArgs ia = infer_args(M);
+  if (near) {
+    // one stream: running the network on a second stream alongside the
+    // near/far kernel was measured slower on MI355X, the two contending for
+    // the same CUs.  The far shares pass between them in bf16 (half the
+    // traffic of f32; the network computes in bf16 anyway).
+    for (uint64_t b = 0; b < n; b += CHUNK) {
+      const uint32_t Q = (uint32_t)((n - b) < CHUNK ? (n - b) : CHUNK);
+      ia.X = X + b * 3;
+      ia.P = Q;
+      ia.Zh = M->izh.p;
+      ia.TZh = out_g ? M->itzh.p : nullptr;
+      cgs_launch_mlp_infer(ia, s);
+      CgsNearArgs na = *near;
+      na.pts = X + b * 3;
+      na.n = Q;
+      na.winding = near->winding ? near->winding + b : nullptr;
+      na.Q = nullptr;
+      na.DQ = nullptr;
+      na.Qh = M->izh.p;
+      na.DQh = out_g ? M->itzh.p : nullptr;
+      na.ldq = M->n_out_p;
+      na.q_scale = (float)M->n_out;
+      na.out_w = out_w + b * M->n_out;
+      na.out_g = out_g ? out_g + b * M->n_out * 3 : nullptr;
+      na.residual = near->residual + b;
+      na.inside = near->inside ? near->inside + b : nullptr;
+      cgs_launch_mvc_nearfar(na, s);
+    }
+    MCHK(hipGetLastError());
+    return CGS_OK;
+  }
   for (uint64_t b = 0; b < n; b += CHUNK) {
     const uint32_t Q = (uint32_t)((n - b) < CHUNK ? (n - b) : CHUNK);
     const float* x = X + b * 3;
@@ -240,22 +274,6 @@ int infer(cgs_mlp* M, const float* X, uint64_t n, float* out_w, float* out_g, co
     ia.Z = M->iz.p;
     ia.TZ = out_g ? M->itz.p : nullptr;
     cgs_launch_mlp_infer(ia, s);
-    if (near) {
-      CgsNearArgs na = *near;
-      na.pts = x;
-      na.n = Q;
-      na.winding = near->winding ? near->winding + b : nullptr;
-      na.Q = M->iz.p;
-      na.DQ = out_g ? M->itz.p : nullptr;
-      na.ldq = M->n_out_p;
-      na.q_scale = (float)M->n_out;
-      na.out_w = out_w + b * M->n_out;
-      na.out_g = out_g ? out_g + b * M->n_out * 3 : nullptr;
-      na.residual = near->residual + b;
-      na.inside = near->inside ? near->inside + b : nullptr;
-      cgs_launch_mvc_nearfar(na, s);
-      continue;
-    }
     CgsMlpPostArgs pa;
     memset(&pa, 0, sizeof(pa));
     pa.Z = M->iz.p;

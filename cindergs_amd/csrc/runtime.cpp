@@ -2137,7 +2137,7 @@ int cgs_mvc_bind_exact(cgs_context* c, const float* cage_vertices, uint32_t num_
   A.out_w = d_weights;
   A.out_g = d_gradients;
   A.mask = d_inside;
-  A.only = nullptr;
+  A.list = nullptr;
   cgs_launch_mvc_bind(A, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2348,7 +2348,67 @@ struct NearCache {
   MvcCage M;
   NearGrid G;
   float minv[16];
+  bool parity = false;          // every triangle's circumradius <= r1 + h
+  void* d_cell_inside = nullptr;  // [cells]: 1 inside, 0 outside, 2 undecided
+  ~NearCache() {
+    if (d_cell_inside) (void)hipFree(d_cell_inside);
+  }
 };
+
+// The side of every grid cell's centre (winding numbers, one point per lane)
+// for the crossing-parity inside test, which needs every triangle that meets
+// a cell in the cell's list: true when no triangle reaches farther than
+// r1 + h from its centroid.
+static int near_cells_classify(cgs_context* c, const float* verts, const uint32_t* tris, uint32_t nt,
+                               float r1, NearCache& C) {
+  const std::vector<float> cent = triangle_centroids(verts, tris, nt);
+  float reach = 0.0f;
+  for (uint32_t t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const float* v = verts + (size_t)tris[t * 3 + k] * 3;
+      const float dx = v[0] - cent[t * 3], dy = v[1] - cent[t * 3 + 1], dz = v[2] - cent[t * 3 + 2];
+      reach = cgs_fmax(reach, sqrtf((dx * dx + dy * dy) + dz * dz));
+    }
+  C.parity = reach <= r1 + C.M.h;
+  if (!C.parity) return CGS_OK;
+  const NearGrid& G = C.G;
+  const size_t cells = (size_t)G.dim[0] * G.dim[1] * G.dim[2];
+  const float cell = 1.0f / G.inv_cell;
+  std::vector<float> centres(cells * 3);
+  for (uint32_t z = 0; z < G.dim[2]; ++z)
+    for (uint32_t y = 0; y < G.dim[1]; ++y)
+      for (uint32_t x = 0; x < G.dim[0]; ++x) {
+        const size_t ci = ((size_t)z * G.dim[1] + y) * G.dim[0] + x;
+        centres[ci * 3] = G.lo[0] + ((float)x + 0.5f) * cell;
+        centres[ci * 3 + 1] = G.lo[1] + ((float)y + 0.5f) * cell;
+        centres[ci * 3 + 2] = G.lo[2] + ((float)z + 0.5f) * cell;
+      }
+  void *d_c = nullptr, *d_w = nullptr;
+  std::vector<float> w(cells);
+  hipError_t e = hipMalloc(&d_c, cells * 12);
+  if (e == hipSuccess) e = hipMalloc(&d_w, cells * 4);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_c, centres.data(), cells * 12, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    cgs_launch_mvc_winding_lane((const float*)C.M.d_verts, (const uint32_t*)C.M.d_tris, nt,
+                                (const float*)d_c, (uint32_t)cells, (float*)d_w, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(w.data(), d_w, cells * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_c) (void)hipFree(d_c);
+  if (d_w) (void)hipFree(d_w);
+  if (e != hipSuccess) return fail(CGS_ERR_EXECUTION, std::string("cell classification: ") + hipGetErrorString(e));
+  std::vector<uint32_t> side(cells);
+  for (size_t i = 0; i < cells; ++i) {
+    const float a = fabsf(w[i]);
+    side[i] = a >= 0.75f ? 1u : (a <= 0.25f ? 0u : 2u);
+  }
+  HIPCHK(hipMalloc(&C.d_cell_inside, cells * 4));
+  HIPCHK(hipMemcpyAsync(C.d_cell_inside, side.data(), cells * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CGS_OK;
+}
 
 static void near_cache_free(NearCache* n) { delete n; }
 
@@ -2396,6 +2456,7 @@ static int near_setup(cgs_context* c, const float* verts, uint32_t nv, const uin
     NearCache* C = new NearCache();
     int rc = mvc_upload(c, verts, nv, tris, nt, C->M);
     if (rc == CGS_OK) rc = near_grid_build(c, verts, nv, tris, nt, r1, C->M.h, C->G);
+    if (rc == CGS_OK) rc = near_cells_classify(c, verts, tris, nt, r1, *C);
     if (rc == CGS_OK && !linear_precision_minv(verts, nv, C->minv))
       rc = fail(CGS_ERR_INVALID_ARG, "cage vertices are coplanar");
     if (rc != CGS_OK) {
@@ -2409,10 +2470,13 @@ static int near_setup(cgs_context* c, const float* verts, uint32_t nv, const uin
   S.C = C;
   memset(&S.a, 0, sizeof(S.a));
   memcpy(S.a.minv, C->minv, sizeof(S.a.minv));
-  HIPCHK(hipMalloc(&S.d_wind, n * 4));
-  cgs_launch_mvc_winding_lane((const float*)C->M.d_verts, (const uint32_t*)C->M.d_tris, nt, d_points,
-                              (uint32_t)n, (float*)S.d_wind, c->stream);
+  if (!C->parity) {
+    HIPCHK(hipMalloc(&S.d_wind, n * 4));
+    cgs_launch_mvc_winding_lane((const float*)C->M.d_verts, (const uint32_t*)C->M.d_tris, nt, d_points,
+                                (uint32_t)n, (float*)S.d_wind, c->stream);
+  }
   CgsNearArgs& a = S.a;
+  a.cell_inside = (const uint32_t*)C->d_cell_inside;
   a.verts = (const float*)C->M.d_verts;
   a.nv = nv;
   a.tris = (const uint32_t*)C->M.d_tris;
@@ -2495,29 +2559,27 @@ int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices
   float r0 = 0.0f, r1 = 0.0f;
   const bool near = cgs_mlp_near(net, &r0, &r1);
   const float diag = cage_diagonal(cage_vertices, num_vertices);
-  const uint32_t nb = cgs_mvc_select_blocks((uint32_t)n);
   NearSetup S;
   MvcCage M0;
   MvcCage* M = &M0;
-  void *d_wind = nullptr, *d_res = nullptr, *d_flags = nullptr, *d_counts = nullptr;
+  void *d_wind = nullptr, *d_res = nullptr, *d_list = nullptr, *d_count = nullptr;
   auto cleanup = [&]() {
-    for (void* p : {d_wind, d_res, d_flags, d_counts})
+    for (void* p : {d_wind, d_res, d_list, d_count})
       if (p) (void)hipFree(p);
   };
   int rc = CGS_OK;
   do {
-    if (hipMalloc(&d_res, n * 4) != hipSuccess || hipMalloc(&d_flags, n * 4) != hipSuccess ||
-        hipMalloc(&d_counts, (size_t)nb * 4) != hipSuccess) {
+    if (hipMalloc(&d_res, n * 4) != hipSuccess || hipMalloc(&d_list, n * 4) != hipSuccess ||
+        hipMalloc(&d_count, 4) != hipSuccess) {
       rc = fail(CGS_ERR_RUNTIME, "hipMalloc failed (surrogate binding)");
       break;
     }
-    const float* wind = nullptr;
+    const float* wind = nullptr;  // null: the inside flags come from the near/far kernel
     if (near) {
       if ((rc = near_setup(c, cage_vertices, num_vertices, triangles, num_triangles, r0, r1, d_points,
                            n, S)) != CGS_OK)
         break;
       M = &S.C->M;
-      wind = (const float*)S.d_wind;
       CgsNearArgs a = S.a;
       a.residual = (float*)d_res;
       a.inside = d_inside;
@@ -2547,37 +2609,41 @@ int cgs_mlp_bind(cgs_context* c, struct cgs_mlp* net, const float* cage_vertices
                                   minv, (float*)d_res);
       if (rc != CGS_OK) break;
     }
+    // points whose residual fails the test, compacted into a list, take the
+    // exact routine
+    hipError_t e = hipMemsetAsync(d_count, 0, 4, c->stream);
     cgs_launch_mvc_select(wind, (const float*)d_res, residual_tol * diag, (uint32_t)n, num_vertices,
-                          d_inside, (uint32_t*)d_flags, d_weights, d_gradients, (uint32_t*)d_counts,
+                          d_inside, d_weights, d_gradients, (uint32_t*)d_list, (uint32_t*)d_count,
                           c->stream);
-    CgsMvcArgs A;
-    memset(&A, 0, sizeof(A));
-    A.verts = (const float*)M->d_verts;
-    A.nv = num_vertices;
-    A.tris = (const uint32_t*)M->d_tris;
-    A.nt = num_triangles;
-    A.vt_start = (const uint32_t*)M->d_start;
-    A.vt_list = (const uint32_t*)M->d_list;
-    A.pts = d_points;
-    A.n = (uint32_t)n;
-    A.h = M->h;
-    A.out_w = d_weights;
-    A.out_g = d_gradients;
-    A.mask = d_inside;
-    A.only = (const uint32_t*)d_flags;
-    cgs_launch_mvc_bind(A, c->stream);
-    std::vector<uint32_t> counts(nb);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(counts.data(), d_counts, (size_t)nb * 4, hipMemcpyDeviceToHost, c->stream);
+    uint32_t count = 0;
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&count, d_count, 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && count) {
+      CgsMvcArgs A;
+      memset(&A, 0, sizeof(A));
+      A.verts = (const float*)M->d_verts;
+      A.nv = num_vertices;
+      A.tris = (const uint32_t*)M->d_tris;
+      A.nt = num_triangles;
+      A.vt_start = (const uint32_t*)M->d_start;
+      A.vt_list = (const uint32_t*)M->d_list;
+      A.pts = d_points;
+      A.n = count;
+      A.h = M->h;
+      A.out_w = d_weights;
+      A.out_g = d_gradients;
+      A.mask = d_inside;
+      A.list = (const uint32_t*)d_list;
+      cgs_launch_mvc_bind(A, c->stream);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     if (e != hipSuccess) {
       rc = fail(CGS_ERR_EXECUTION, std::string("surrogate binding: ") + hipGetErrorString(e));
       break;
     }
-    uint64_t fb = 0;
-    for (uint32_t k = 0; k < nb; ++k) fb += counts[k];
-    if (fallback_count) *fallback_count = fb;
+    if (fallback_count) *fallback_count = count;
   } while (false);
   (void)hipStreamSynchronize(c->stream);
   cleanup();
