@@ -134,7 +134,9 @@ class StageTimings(C.Structure):
 
 
 class Tuning(C.Structure):
-    """cgs_tuning: launch shapes that leave every output bit-identical."""
+    """cgs_tuning: launch shapes that leave every output bit-identical, and
+    the tile size (16 x 16 the reference's; 8 x 8, 16 x 32, 32 x 16 render the
+    same algorithm on that tile grid)."""
 
     _fields_ = [
         ("radix_partition_keys", C.c_uint32),
@@ -142,7 +144,14 @@ class Tuning(C.Structure):
         ("composite_batch", C.c_uint32),
         ("tile_order", C.c_uint32),
         ("tile_sort", C.c_uint32),
+        ("tile_w", C.c_uint32),
+        ("tile_h", C.c_uint32),
     ]
+
+    def __init__(self, radix_partition_keys=16384, radix_digit_bits=8, composite_batch=256,
+                 tile_order=1, tile_sort=0, tile_w=16, tile_h=16):
+        super().__init__(radix_partition_keys, radix_digit_bits, composite_batch, tile_order,
+                         tile_sort, tile_w, tile_h)
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -193,7 +202,8 @@ SIGNATURES = {
     "cgs_set_tuning": [_vp, C.POINTER(Tuning)],
     "cgs_get_tuning": [_vp, C.POINTER(Tuning)],
     "cgs_autotune": [_vp, C.POINTER(CameraUniforms), C.POINTER(CompositorUniforms),
-                     C.POINTER(FrameParams), _u32, C.POINTER(Tuning), C.POINTER(C.c_double), _u32],
+                     C.POINTER(FrameParams), _u32, _u32, C.POINTER(Tuning), C.POINTER(C.c_double),
+                     _u32],
     "cgs_tuning_candidates": [],
     "cgs_tuning_candidate": [_u32, C.POINTER(Tuning)],
     "cgs_batch_set_tuning": [_vp, C.POINTER(Tuning)],

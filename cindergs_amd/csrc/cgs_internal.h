@@ -226,6 +226,7 @@ struct CgsBinArgs {
   uint32_t num_rest;
   uint32_t max_gaussians;
   uint32_t width, height, tiles_x, tiles_y;
+  uint32_t tile_w, tile_h;   // tile size in pixels (16 x 16: the reference's)
   uint32_t mode64;           // 0: packed 32-bit keys, 1: (tile, depth bits) pairs
   uint32_t base_depth_bits;  // 32-bit mode
   uint32_t depth_bits;       // 32-bit mode
@@ -397,12 +398,13 @@ struct CgsCompositeArgs {
   const uint32_t* sorted_ids;
   const uint32_t* ranges;
   uint32_t width, height, tiles_x, tiles_y, padded_row;
+  uint32_t tile_w, tile_h;  // 8 x 8, 16 x 16, 16 x 32 or 32 x 16 pixels
   float bg[4];
   uint32_t* out;
   uint32_t* order;  // [tiles] dispatch order scratch (null: tile order)
 };
 // batch: splats staged per compositor step (64, 128 or 256; the same pixels
-// for every choice)
+// for every choice; 8 x 8 tiles take 64)
 void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof = nullptr,
                           uint32_t batch = 256);
 

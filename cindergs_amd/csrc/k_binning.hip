@@ -36,28 +36,34 @@ struct TileBox {
   uint32_t tx0, tx1, ty0, ty1;
 };
 
+// compute_gaussian_tile_bounds_xy (tiles/mod.rs:97-130, the GPU form in
+// binning.rs:52-63) with the tile size as a parameter: the division by the
+// tile width or height is exact for the power-of-two sizes, so at 16 x 16 the
+// box is the reference's bit for bit
 __device__ __forceinline__ TileBox tile_box(float mu_x, float mu_y, uint32_t packed,
-                                            uint32_t tiles_x, uint32_t tiles_y) {
+                                            uint32_t tiles_x, uint32_t tiles_y, float tw,
+                                            float th) {
   const float ext_x = (float)(packed & 0xFFFFu);
   const float ext_y = (float)((packed >> 16) & 0xFFFFu);
   const int32_t mtx = (int32_t)tiles_x - 1, mty = (int32_t)tiles_y - 1;
   TileBox b;
-  b.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / 16.0f)), 0, mtx);
-  b.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / 16.0f)), 0, mtx);
-  b.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / 16.0f)), 0, mty);
-  b.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / 16.0f)), 0, mty);
+  b.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / tw)), 0, mtx);
+  b.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / tw)), 0, mtx);
+  b.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / th)), 0, mty);
+  b.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / th)), 0, mty);
   return b;
 }
 
-// ellipse_intersects_tile_gpu (binning.rs:35-83); d0 = (mu_x, mu_y, a, b),
-// d1 = (c, alpha, q_cut, -)
+// ellipse_intersects_tile_gpu (binning.rs:35-83) on the tile's pixel-centre
+// rectangle; d0 = (mu_x, mu_y, a, b), d1 = (c, alpha, q_cut, -)
 __device__ __forceinline__ bool hits(const float4& d0, const float4& d1, uint32_t tx,
-                                     uint32_t ty, uint32_t W, uint32_t H) {
+                                     uint32_t ty, uint32_t W, uint32_t H, uint32_t tw,
+                                     uint32_t th) {
   const float mu_x = d0.x, mu_y = d0.y, a = d0.z, b = d0.w, c = d1.x;
   if (d1.y < (1.0f / 255.0f) || W == 0u || H == 0u) return false;
-  const uint32_t xs = tx * 16u, ys = ty * 16u;
+  const uint32_t xs = tx * tw, ys = ty * th;
   if (xs >= W || ys >= H) return false;
-  const uint32_t xe = min(xs + 15u, W - 1u), ye = min(ys + 15u, H - 1u);
+  const uint32_t xe = min(xs + tw - 1u, W - 1u), ye = min(ys + th - 1u, H - 1u);
   const float dx0 = ((float)xs + 0.5f) - mu_x;
   const float dx1 = ((float)xe + 0.5f) - mu_x;
   const float dy0 = ((float)ys + 0.5f) - mu_y;
@@ -104,7 +110,7 @@ __device__ __forceinline__ BinSplat load_bin_splat(const CgsBinArgs& A, uint32_t
   const float4 g0 = g[0], g1 = g[1];
   b.d0 = g0;
   b.d1 = make_float4(g1.x, g1.y, g1.w, cgs_u2f(splat_key_bits(A, g1.z)));
-  b.box = tile_box(g0.x, g0.y, packed, A.tiles_x, A.tiles_y);
+  b.box = tile_box(g0.x, g0.y, packed, A.tiles_x, A.tiles_y, (float)A.tile_w, (float)A.tile_h);
   // a splat below the opacity cut hits no tile (hits() rejects it first)
   if (g1.y >= (1.0f / 255.0f))
     b.area = (b.box.tx1 - b.box.tx0 + 1u) * (b.box.ty1 - b.box.ty0 + 1u);
@@ -200,7 +206,7 @@ __device__ __forceinline__ uint64_t walk_chunk(const CgsBinArgs& A, WaveBig& W, 
     }
     tile_x = bx.x + (uint32_t)r;
     tile_y = bx.y + q;
-    hit = test ? hits(W.d0[sp], W.d1[sp], tile_x, tile_y, A.width, A.height)
+    hit = test ? hits(W.d0[sp], W.d1[sp], tile_x, tile_y, A.width, A.height, A.tile_w, A.tile_h)
                : ((known >> lane) & 1ull) != 0ull;
   }
   return M | 1ull;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
       uint32_t mask = 0u, k = 0u;
       for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
         for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
-          if (hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+          if (hits(b.d0, b.d1, tx, ty, A.width, A.height, A.tile_w, A.tile_h)) {
             mask |= 1u << k;
             if (HW > 0) tile_hist_add(s_th, ty * A.tiles_x + tx);
           }
@@ -436,7 +442,7 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
       for (uint32_t ty = b.box.ty0; ty <= b.box.ty1; ++ty)
         for (uint32_t tx = b.box.tx0; tx <= b.box.tx1; ++tx, ++k)
           if (A.small_masks ? ((mask >> k) & 1u) != 0u
-                            : hits(b.d0, b.d1, tx, ty, A.width, A.height)) {
+                            : hits(b.d0, b.d1, tx, ty, A.width, A.height, A.tile_w, A.tile_h)) {
             const uint32_t tile = ty * A.tiles_x + tx;
             if (HW > 0) {
               bucket_put(A, s_hist, tile, kb, s);

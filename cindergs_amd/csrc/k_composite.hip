@@ -1,4 +1,5 @@
-// k_composite.hip — Stage 4: 16x16 tile compositor into RGBA8.
+// k_composite.hip — Stage 4: tile compositor into RGBA8 (16x16 tiles as the
+// reference; 8x8, 16x32 and 32x16 as tuning choices).
 //
 // Follows tile_compositor_v2_kernel (src/render/kernels/compositor.rs:721-883):
 // front-to-back alpha blending of the tile's depth-sorted splats, q clamped to
@@ -8,13 +9,13 @@
 // 64 pixels (gpu_state.rs:25-27).
 //
 // MI355X design:
-//  * one 256-thread block (4 wave64s) per tile; wave w owns the 8x8 quadrant
-//    (w&1, w>>1) of the tile, one pixel per lane;
-//  * a batch of 256 splats is gathered once into LDS (two float4 + one float
-//    per splat) and, by the loading thread, tested against the 4 quadrants
-//    with the exact ellipse-vs-rectangle test the binning uses
-//    (binning.rs:35-83, on the quadrant's pixel-centre rectangle); the four
-//    results become 4 ballot masks per loading wave;
+//  * one block per tile of QX x QY quadrants of 8x8 pixels (16x16: 2 x 2,
+//    256 threads); wave w owns quadrant (w % QX, w / QX), one pixel per lane;
+//  * a batch of CB splats is gathered once into LDS (two float4 + one float
+//    per splat) and, by the loading thread, tested against the QX x QY
+//    quadrants with the exact ellipse-vs-rectangle test the binning uses
+//    (binning.rs:35-83, on the quadrant's pixel-centre rectangle); the
+//    results become one ballot mask per quadrant and loading wave;
 //  * each wave compacts the set bits of its 256-bit mask into an LDS list of
 //    splat slots (order kept) and walks the list front to back four splats
 //    per step: the four LDS reads and q / e^q chains are independent, only
@@ -39,7 +40,6 @@
 
 namespace {
 
-constexpr int CT = 256;  // threads: one per pixel of the 16x16 tile
 constexpr float ALPHA_MIN = 0.003921569f;  // the reference's literal
 
 // ellipse_intersects_tile_gpu (binning.rs:35-83) on an arbitrary pixel rectangle
@@ -154,17 +154,20 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* ranges, uin
 constexpr int COMPOSITE_STEP = 4;
 constexpr bool COMPOSITE_PREFETCH = true;
 
-// CB splats per batch (64, 128 or 256), staged by threads t < CB; STEP
-// splats walked per step of the inner loop; PF: the records of the next batch
-// are loaded while this one is composited
-template <int CB, int STEP, bool PF>
-__global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
+// Tile of QX x QY quadrants of 8x8 pixels (64 x QX x QY threads); CB splats
+// per batch (64, 128 or 256, at most the thread count), staged by threads
+// t < CB; STEP splats walked per step of the inner loop; PF: the records of
+// the next batch are loaded while this one is composited
+template <int QX, int QY, int CB, int STEP, bool PF>
+__global__ __launch_bounds__(64 * QX * QY) void k_composite(CgsCompositeArgs A) {
+  constexpr int NQ = QX * QY;
+  static_assert(CB <= 64 * NQ, "a batch is staged by one thread per splat");
   // staged batch: plane 0 = Splat::g, 1 = Splat::m, 2 = (blue, -, -, -); the
   // quadrant lists hold byte offsets into a plane, so a splat's three reads
   // share one address register
   __shared__ float4 s_stage[3][CB];
-  __shared__ uint64_t s_mask[4][CB / 64];     // [quadrant][loading wave]
-  __shared__ __align__(16) uint32_t s_list[4][CB];  // [quadrant] splats that can touch it
+  __shared__ uint64_t s_mask[NQ][CB / 64];     // [quadrant][loading wave]
+  __shared__ __align__(16) uint32_t s_list[NQ][CB];  // [quadrant] splats that can touch it
   const bool loader = threadIdx.x < (uint32_t)CB;
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
@@ -173,10 +176,10 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
   const uint32_t t = threadIdx.x;
   const uint32_t wave = t >> 6, lane = t & 63u;
-  const uint32_t qx0 = tx * 16u + (wave & 1u) * 8u, qy0 = ty * 16u + (wave >> 1) * 8u;
+  const uint32_t bx = tx * (8u * QX), by = ty * (8u * QY);
+  const uint32_t qx0 = bx + (wave % QX) * 8u, qy0 = by + (wave / QX) * 8u;
   const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
   const bool inside = px < A.width && py < A.height;
-  const uint32_t bx = tx * 16u, by = ty * 16u;
 
   const uint32_t rs = A.ranges[tile * 2u], re = A.ranges[tile * 2u + 1u];
   const bool empty = rs >= re;
@@ -204,7 +207,9 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
   for (uint32_t b0 = 0; b0 < n; b0 += CB) {
     // ---- stage the batch and test it against the 4 quadrants ----
     const uint32_t e = rs + b0 + t;
-    bool q0 = false, q1 = false, q2 = false, q3 = false;
+    bool qh[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) qh[q] = false;
     if (!PF && loader && e < re) {
       const uint32_t g = A.sorted_ids[e];
       r0 = geom4[(size_t)g * 2];
@@ -217,21 +222,15 @@ __global__ __launch_bounds__(CT) void k_composite(CgsCompositeArgs A) {
       s_stage[0][t] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
       s_stage[1][t] = make_float4(r0.x, r0.y, rc.x, rc.y);
       reinterpret_cast<float*>(&s_stage[2][t])[0] = rc.z;
-      q0 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by, 8u, 8u,
-                             A.width, A.height);
-      q1 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + 8u, by, 8u, 8u,
-                             A.width, A.height);
-      q2 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx, by + 8u, 8u, 8u,
-                             A.width, A.height);
-      q3 = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + 8u, by + 8u, 8u,
-                             8u, A.width, A.height);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        qh[q] = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + (q % QX) * 8u,
+                                  by + (q / QX) * 8u, 8u, 8u, A.width, A.height);
     }
-    const uint64_t m0 = __ballot(q0), m1 = __ballot(q1), m2 = __ballot(q2), m3 = __ballot(q3);
-    if (lane == 0 && wave < (uint32_t)(CB / 64)) {
-      s_mask[0][wave] = m0;
-      s_mask[1][wave] = m1;
-      s_mask[2][wave] = m2;
-      s_mask[3][wave] = m3;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint64_t mq = __ballot(qh[q]);
+      if (lane == 0 && wave < (uint32_t)(CB / 64)) s_mask[q][wave] = mq;
     }
     // the next batch's records are loaded while this batch is composited
     if (PF && loader) {
@@ -319,10 +318,29 @@ void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* pro
   if (prof) prof->mark("k_composite", s);
   constexpr int ST = COMPOSITE_STEP;
   constexpr bool PF = COMPOSITE_PREFETCH;
-  if (batch == 64u)
-    hipLaunchKernelGGL((k_composite<64, ST, PF>), dim3(total), dim3(CT), 0, s, A);
-  else if (batch == 128u)
-    hipLaunchKernelGGL((k_composite<128, ST, PF>), dim3(total), dim3(CT), 0, s, A);
-  else
-    hipLaunchKernelGGL((k_composite<256, ST, PF>), dim3(total), dim3(CT), 0, s, A);
+  const dim3 grid(total);
+  if (A.tile_w == 8u && A.tile_h == 8u) {
+    hipLaunchKernelGGL((k_composite<1, 1, 64, ST, PF>), grid, dim3(64), 0, s, A);
+  } else if (A.tile_w == 16u && A.tile_h == 32u) {
+    if (batch == 64u)
+      hipLaunchKernelGGL((k_composite<2, 4, 64, ST, PF>), grid, dim3(512), 0, s, A);
+    else if (batch == 128u)
+      hipLaunchKernelGGL((k_composite<2, 4, 128, ST, PF>), grid, dim3(512), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_composite<2, 4, 256, ST, PF>), grid, dim3(512), 0, s, A);
+  } else if (A.tile_w == 32u && A.tile_h == 16u) {
+    if (batch == 64u)
+      hipLaunchKernelGGL((k_composite<4, 2, 64, ST, PF>), grid, dim3(512), 0, s, A);
+    else if (batch == 128u)
+      hipLaunchKernelGGL((k_composite<4, 2, 128, ST, PF>), grid, dim3(512), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_composite<4, 2, 256, ST, PF>), grid, dim3(512), 0, s, A);
+  } else {
+    if (batch == 64u)
+      hipLaunchKernelGGL((k_composite<2, 2, 64, ST, PF>), grid, dim3(256), 0, s, A);
+    else if (batch == 128u)
+      hipLaunchKernelGGL((k_composite<2, 2, 128, ST, PF>), grid, dim3(256), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_composite<2, 2, 256, ST, PF>), grid, dim3(256), 0, s, A);
+  }
 }

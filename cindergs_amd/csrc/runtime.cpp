@@ -142,6 +142,7 @@ struct Stage3Key {
   uint64_t deform_generation;
   uint32_t view_bits[16];
   uint32_t fx, fy, cx, cy, width, height, near_bits, mip_bits, min_op_bits, min_rad_bits;
+  uint32_t tile_w, tile_h;
   bool operator==(const Stage3Key& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -171,7 +172,7 @@ struct FrameArena {
   uint32_t* sorted_ids = nullptr;
   bool capture = false;
   bool captured = false;
-  cgs_tuning tune = {16384u, 8u, 256u, 1u, 0u};
+  cgs_tuning tune = {16384u, 8u, 256u, 1u, 0u, 16u, 16u};
   // bucket mode (k_tilesort.hip): histograms, bases, buckets, the sort's
   // tables and control words, and the launch plan of the segmented sort
   DevBuf bk_hist, bk_base, bk_bucket, bk_alt, bk_tab, bk_ctl;
@@ -358,14 +359,19 @@ void scene_changed(cgs_context* c) {
 
 struct FrameGeom {
   uint32_t W, H, tiles_x, tiles_y, T, padded_row;
+  uint32_t tile_w, tile_h;  // tile size in pixels
 };
 
-FrameGeom frame_geom(const cgs_compositor_uniforms* comp) {
+// the frame's tile grid (TileGrid::new, tiles/mod.rs:39-49, with the tuned
+// tile size in place of TILE_SIZE)
+FrameGeom frame_geom(const cgs_compositor_uniforms* comp, const cgs_tuning& tune) {
   FrameGeom g;
   g.W = comp->screen_width < 1 ? 1 : comp->screen_width;
   g.H = comp->screen_height < 1 ? 1 : comp->screen_height;
-  g.tiles_x = (g.W + 15) / 16;
-  g.tiles_y = (g.H + 15) / 16;
+  g.tile_w = tune.tile_w;
+  g.tile_h = tune.tile_h;
+  g.tiles_x = (g.W + g.tile_w - 1u) / g.tile_w;
+  g.tiles_y = (g.H + g.tile_h - 1u) / g.tile_h;
   g.T = g.tiles_x * g.tiles_y;
   g.padded_row = (g.W + 63) / 64 * 64;  // padded_rgba8_row_u32 (gpu_state.rs:25-27)
   return g;
@@ -484,6 +490,8 @@ int launch_stage3_bucket(FrameArena& a, hipStream_t s, CgsProf* prof, const Fram
   B.height = g.H;
   B.tiles_x = g.tiles_x;
   B.tiles_y = g.tiles_y;
+  B.tile_w = g.tile_w;
+  B.tile_h = g.tile_h;
   B.mode64 = 0u;
   float np = near_plane < 1e-6f ? 1e-6f : near_plane;
   B.base_depth_bits = fbits(np) & 0x7F800000u;
@@ -575,7 +583,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   a.bucket = bucket;
   if (bucket && !a.plan_external) {
     // the plan this geometry's last frame left
-    a.bk_key = ((uint64_t)g.T << 32) | max_g;
+    a.bk_key = ((uint64_t)(g.tile_w * 64u + g.tile_h) << 48) | ((uint64_t)g.T << 32) | max_g;
     a.bk_planned = false;
     for (const auto& p : a.bk_plans)
       if (p.first == a.bk_key) {
@@ -636,6 +644,8 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   B.height = g.H;
   B.tiles_x = g.tiles_x;
   B.tiles_y = g.tiles_y;
+  B.tile_w = g.tile_w;
+  B.tile_h = g.tile_h;
   B.mode64 = a.mode64 ? 1u : 0u;
   float np = near_plane < 1e-6f ? 1e-6f : near_plane;
   B.base_depth_bits = fbits(np) & 0x7F800000u;
@@ -749,6 +759,8 @@ int launch_stage4(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   A.height = g.H;
   A.tiles_x = g.tiles_x;
   A.tiles_y = g.tiles_y;
+  A.tile_w = g.tile_w;
+  A.tile_h = g.tile_h;
   A.padded_row = g.padded_row;
   for (int i = 0; i < 4; ++i) A.bg[i] = comp->background_color[i];
   A.out = out;
@@ -914,6 +926,8 @@ Stage3Key stage3_key(const cgs_context* c, const cgs_camera_uniforms* cam, const
   key.cy = fbits(cam->principal_y);
   key.width = g.W;
   key.height = g.H;
+  key.tile_w = g.tile_w;
+  key.tile_h = g.tile_h;
   key.near_bits = fbits(cam->near_plane);
   key.mip_bits = fbits(prm->mip_filter_3d_var);
   key.min_op_bits = fbits(prm->min_opacity);
@@ -981,6 +995,13 @@ int check_tuning(const cgs_tuning* t) {
     return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128 or 256");
   if (t->tile_order > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_order must be 0 or 1");
   if (t->tile_sort > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_sort must be 0 or 1");
+  const uint32_t tw = t->tile_w, th = t->tile_h;
+  if (!((tw == 16u && th == 16u) || (tw == 8u && th == 8u) || (tw == 16u && th == 32u) ||
+        (tw == 32u && th == 16u)))
+    return fail(CGS_ERR_INVALID_ARG, "tile size must be 16x16, 8x8, 16x32 or 32x16");
+  // an 8 x 8 tile is one wave, which stages 64 splats per step
+  if (tw == 8u && t->composite_batch != 64u)
+    return fail(CGS_ERR_INVALID_ARG, "8x8 tiles take composite_batch 64");
   return CGS_OK;
 }
 
@@ -1248,7 +1269,7 @@ int cgs_execute_frame(cgs_context* c, const cgs_camera_uniforms* cam,
   if (!cam || !comp || !prm) return fail(CGS_ERR_INVALID_ARG, "null frame argument");
   cgs_stage_timings T;
   memset(&T, 0, sizeof(T));
-  const FrameGeom g = frame_geom(comp);
+  const FrameGeom g = frame_geom(comp, c->fa.tune);
   if (g.W != c->last_w || g.H != c->last_h) {
     c->s3_valid = false;
     c->last_w = g.W;
@@ -1392,35 +1413,75 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 
 // ---- launch tuning ----
 
-// the 60 choices of the LSD radix sort (partition x digit width x compositor
-// batch x tile order), then the 6 of the bucket mode (batch x order)
-uint32_t cgs_tuning_candidates(void) { return 66u; }
+// 16 x 16 tiles: the 60 choices of the LSD radix sort (partition x digit
+// width x compositor batch x tile order), then the 6 of the bucket mode (batch
+// x order); then the other tile sizes with fewer choices each: 8 x 8 (batch
+// 64) radix partitions 16384 / 8192 x order, then bucket x order; 16 x 32 and
+// 32 x 16 radix partitions 16384 / 8192 x batch 256 / 128 x order, then
+// bucket x batch x order
+uint32_t cgs_tuning_candidates(void) { return 96u; }
 
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
-  if (!out || i >= 66u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  if (!out || i >= 96u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
   static const uint32_t parts[5] = {16384u, 12288u, 10240u, 8192u, 4096u},
                         digits[2] = {8u, 4u}, batches[3] = {256u, 128u, 64u},
-                        orders[2] = {1u, 0u};
-  if (i >= 60u) {
+                        orders[2] = {1u, 0u}, parts2[2] = {16384u, 8192u};
+  out->radix_partition_keys = 16384u;
+  out->radix_digit_bits = 8u;
+  out->tile_w = 16u;
+  out->tile_h = 16u;
+  if (i < 60u) {
+    out->radix_partition_keys = parts[i % 5u];
+    out->radix_digit_bits = digits[(i / 5u) % 2u];
+    out->composite_batch = batches[(i / 10u) % 3u];
+    out->tile_order = orders[i / 30u];
+    out->tile_sort = 0u;
+  } else if (i < 66u) {
     i -= 60u;
-    out->radix_partition_keys = 16384u;
-    out->radix_digit_bits = 8u;
     out->composite_batch = batches[i % 3u];
     out->tile_order = orders[i / 3u];
     out->tile_sort = 1u;
-    return CGS_OK;
+  } else if (i < 72u) {
+    i -= 66u;
+    out->tile_w = out->tile_h = 8u;
+    out->composite_batch = 64u;
+    if (i < 4u) {
+      out->radix_partition_keys = parts2[i % 2u];
+      out->tile_order = orders[i / 2u];
+      out->tile_sort = 0u;
+    } else {
+      out->tile_order = orders[i - 4u];
+      out->tile_sort = 1u;
+    }
+  } else {
+    i -= 72u;
+    const bool tall = i < 12u;
+    if (!tall) i -= 12u;
+    out->tile_w = tall ? 16u : 32u;
+    out->tile_h = tall ? 32u : 16u;
+    if (i < 8u) {
+      out->radix_partition_keys = parts2[i % 2u];
+      out->composite_batch = batches[(i / 2u) % 2u];
+      out->tile_order = orders[i / 4u];
+      out->tile_sort = 0u;
+    } else {
+      i -= 8u;
+      out->composite_batch = batches[i % 2u];
+      out->tile_order = orders[i / 2u];
+      out->tile_sort = 1u;
+    }
   }
-  out->radix_partition_keys = parts[i % 5u];
-  out->radix_digit_bits = digits[(i / 5u) % 2u];
-  out->composite_batch = batches[(i / 10u) % 3u];
-  out->tile_order = orders[i / 30u];
-  out->tile_sort = 0u;
   return CGS_OK;
 }
 
 int cgs_set_tuning(cgs_context* c, const cgs_tuning* t) {
   if (!c || !t) return fail(CGS_ERR_INVALID_ARG, "bad arguments");
   TRY(check_tuning(t));
+  if (t->tile_w != c->fa.tune.tile_w || t->tile_h != c->fa.tune.tile_h) {
+    // the kept Stage 3 (and the hysteresis cache's) is for the old tile grid
+    c->s3_valid = false;
+    c->sort_cache.invalidate();
+  }
   c->fa.tune = *t;
   return CGS_OK;
 }
@@ -1433,51 +1494,78 @@ int cgs_get_tuning(cgs_context* c, cgs_tuning* t) {
 
 int cgs_autotune(cgs_context* c, const cgs_camera_uniforms* cam,
                  const cgs_compositor_uniforms* comp, const cgs_frame_params* prm, uint32_t reps,
-                 cgs_tuning* best, double* ms, uint32_t n_ms) {
+                 uint32_t flags, cgs_tuning* best, double* ms, uint32_t n_ms) {
   TRY(ensure_ctx(c));
   if (!cam || !comp || !prm) return fail(CGS_ERR_INVALID_ARG, "null frame argument");
+  if (flags & ~CGS_TUNE_TILE_SHAPES) return fail(CGS_ERR_INVALID_ARG, "unknown autotune flags");
   if (reps == 0) reps = 1;
   cgs_frame_params p = *prm;
   p.force_stage3 = 1;
   p.profile_stages = 0;
   const cgs_tuning start = c->fa.tune;
-  const FrameGeom g = frame_geom(comp);
-  // the current setting's frame is the reference every candidate must equal
-  cgs_stage_timings T;
-  TRY(cgs_execute_frame(c, cam, comp, &p, &T));
-  const uint64_t ref_entries = T.total_tile_entries;
-  std::vector<uint8_t> ref((size_t)g.W * g.H * 4), got(ref.size());
-  std::vector<uint32_t> ref_ranges((size_t)g.T * 2), got_ranges(ref_ranges.size());
-  TRY(cgs_readback_rgba8(c, ref.data(), g.W, g.H));
-  if (c->n_rest) {
-    HIPCHK(hipMemcpy(ref_ranges.data(), c->fa.ranges.p, ref_ranges.size() * 4,
-                     hipMemcpyDeviceToHost));
-  }
+  // per tile size, the frame every candidate of that size must equal: the
+  // current setting's for its size, else the size's first candidate's
+  struct SizeRef {
+    uint32_t tw, th;
+    uint64_t entries;
+    std::vector<uint8_t> rgba;
+    std::vector<uint32_t> ranges;
+  };
+  std::vector<SizeRef> refs;
+  uint32_t ran_w = start.tile_w, ran_h = start.tile_h;  // tile size of the last frame run
+  auto restore = [&](int rc) {
+    c->fa.tune = start;
+    if (ran_w != start.tile_w || ran_h != start.tile_h) {
+      c->s3_valid = false;
+      c->sort_cache.invalidate();
+    }
+    return rc;
+  };
+  // one frame of the current setting; its framebuffer, ranges and entry count
+  auto frame = [&](SizeRef& r) -> int {
+    const FrameGeom g = frame_geom(comp, c->fa.tune);
+    cgs_stage_timings T;
+    ran_w = c->fa.tune.tile_w;
+    ran_h = c->fa.tune.tile_h;
+    TRY(cgs_execute_frame(c, cam, comp, &p, &T));
+    r.tw = c->fa.tune.tile_w;
+    r.th = c->fa.tune.tile_h;
+    r.entries = T.total_tile_entries;
+    r.rgba.resize((size_t)g.W * g.H * 4);
+    TRY(cgs_readback_rgba8(c, r.rgba.data(), g.W, g.H));
+    r.ranges.assign(c->n_rest ? (size_t)g.T * 2 : 0, 0u);
+    if (c->n_rest)
+      HIPCHK(hipMemcpy(r.ranges.data(), c->fa.ranges.p, r.ranges.size() * 4,
+                       hipMemcpyDeviceToHost));
+    return CGS_OK;
+  };
+  refs.emplace_back();
+  if (int rc = frame(refs.back())) return restore(rc);
   const uint32_t n = cgs_tuning_candidates();
   double best_ms = 1e300;
   cgs_tuning best_t = start;
+  SizeRef got;
   for (uint32_t i = 0; i < n; ++i) {
     cgs_tuning cand;
     TRY(cgs_tuning_candidate(i, &cand));
+    if (ms && i < n_ms) ms[i] = -1.0;
+    if (!(flags & CGS_TUNE_TILE_SHAPES) &&
+        (cand.tile_w != start.tile_w || cand.tile_h != start.tile_h))
+      continue;
     c->fa.tune = cand;
-    int rc = cgs_execute_frame(c, cam, comp, &p, &T);
-    if (rc == CGS_OK) rc = cgs_readback_rgba8(c, got.data(), g.W, g.H);
-    if (rc == CGS_OK && c->n_rest) {
-      if (hipMemcpy(got_ranges.data(), c->fa.ranges.p, got_ranges.size() * 4,
-                    hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(CGS_ERR_EXECUTION, "range readback failed");
+    if (int rc = frame(got)) return restore(rc);
+    const SizeRef* ref = nullptr;
+    for (const SizeRef& r : refs)
+      if (r.tw == cand.tile_w && r.th == cand.tile_h) ref = &r;
+    if (!ref) {
+      refs.push_back(got);
+    } else if (got.entries != ref->entries || got.rgba != ref->rgba || got.ranges != ref->ranges) {
+      return restore(fail(CGS_ERR_EXECUTION, "a tuning candidate changed the frame (internal error)"));
     }
-    if (rc != CGS_OK) {
-      c->fa.tune = start;
-      return rc;
-    }
-    if (T.total_tile_entries != ref_entries || got != ref ||
-        (c->n_rest && got_ranges != ref_ranges)) {
-      c->fa.tune = start;
-      return fail(CGS_ERR_EXECUTION, "a tuning candidate changed the frame (internal error)");
-    }
+    cgs_stage_timings T;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    for (uint32_t r = 0; r < reps; ++r) TRY(cgs_execute_frame(c, cam, comp, &p, &T));
+    for (uint32_t r = 0; r < reps; ++r)
+      if (int rc = cgs_execute_frame(c, cam, comp, &p, &T)) return restore(rc);
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(hipEventSynchronize(c->ev[1]));
     const double t_ms = elapsed(c->ev[0], c->ev[1]) / reps;
@@ -1488,6 +1576,10 @@ int cgs_autotune(cgs_context* c, const cgs_camera_uniforms* cam,
     }
   }
   c->fa.tune = best_t;
+  if (ran_w != best_t.tile_w || ran_h != best_t.tile_h) {
+    c->s3_valid = false;
+    c->sort_cache.invalidate();
+  }
   if (best) *best = best_t;
   return CGS_OK;
 }
@@ -1680,7 +1772,7 @@ int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
                    const cgs_compositor_uniforms* comp, uint64_t* n_entries) {
   TRY(ensure_ctx(c));
   if (!cam || !comp) return fail(CGS_ERR_INVALID_ARG, "null argument");
-  const FrameGeom g = frame_geom(comp);
+  const FrameGeom g = frame_geom(comp, c->fa.tune);
   c->last_w = g.W;
   c->last_h = g.H;
   for (int attempt = 0; attempt < 5; ++attempt) {
@@ -1702,7 +1794,7 @@ int cgs_run_stage3(cgs_context* c, const cgs_camera_uniforms* cam,
 int cgs_run_stage4(cgs_context* c, const cgs_compositor_uniforms* comp) {
   TRY(ensure_ctx(c));
   if (!comp) return fail(CGS_ERR_INVALID_ARG, "null argument");
-  const FrameGeom g = frame_geom(comp);
+  const FrameGeom g = frame_geom(comp, c->fa.tune);
   if (g.tiles_x != c->fa.tiles_x || g.tiles_y != c->fa.tiles_y)
     return fail(CGS_ERR_INVALID_ARG, "compositor size differs from Stage 3");
   TRY(launch_stage4(c->fa, c->stream, &c->prof, g, comp));
@@ -2913,7 +3005,7 @@ int batch_enqueue_view(cgs_batch* b, cgs_context* c, const SceneFrame& F, FrameA
                        hipStream_t s, const cgs_camera_uniforms* cam,
                        const cgs_compositor_uniforms* comp, const cgs_frame_params* prm,
                        uint32_t v) {
-  const FrameGeom g = frame_geom(comp);
+  const FrameGeom g = frame_geom(comp, a.tune);
   uint32_t* out = P<uint32_t>(b->fb) + b->fb_off[v];
   uint32_t* slot = P<uint32_t>(b->counters) + 8 * (size_t)v;
   b->view_bucket[v] = 0;
@@ -3107,7 +3199,7 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
   std::vector<double> cost(n_views);
   std::vector<uint32_t> order(n_views);
   for (uint32_t v = 0; v < n_views; ++v) {
-    G[v] = frame_geom(&comps[v]);
+    G[v] = frame_geom(&comps[v], b->lanes[0].tune);
     const double mg = (double)SF[view_scene[v]].max_g;
     const double e = measured ? (double)b->last_entries[v] : 3.0 * mg;
     cost[v] = e + 0.5 * (double)G[v].W * G[v].H + 0.2 * mg;

@@ -193,14 +193,18 @@ class GpuPipelineState:
         check(self._lib.cgs_get_tuning(self._h, C.byref(t)))
         return t
 
-    def autotune(self, camera, compositor, params: FrameParams, reps: int = 3):
-        """Time every tuning candidate on this view (each checked to give the
-        same frame) and keep the fastest.  Returns (best, [(candidate, ms)])."""
+    def autotune(self, camera, compositor, params: FrameParams, reps: int = 3,
+                 tile_shapes: bool = False):
+        """Time the tuning candidates of the current tile size (of every size
+        with tile_shapes) on this view, each checked to give the same frame as
+        the others of its size, and keep the fastest.  Returns (best,
+        [(candidate, ms)]) with ms = -1 for the candidates not swept."""
         n = tuning_candidates_count()
         ms = (C.c_double * n)()
         best = _lib.Tuning()
+        flags = 1 if tile_shapes else 0  # CGS_TUNE_TILE_SHAPES
         check(self._lib.cgs_autotune(self._h, C.byref(camera), C.byref(compositor),
-                                     C.byref(params), reps, C.byref(best), ms, n))
+                                     C.byref(params), reps, flags, C.byref(best), ms, n))
         return best, [(tuning_candidate(i), ms[i]) for i in range(n)]
 
     def kernel_times(self):

@@ -198,37 +198,51 @@ int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, ui
 
 /* ---- launch tuning ----
  * Launch shapes that leave every output bit-identical (keys, ranges, RGBA8):
- * radix partition size and digit width of the Stage 3 sort, splats staged
- * per compositor step, and the heaviest-tiles-first compositor order.  The
- * reference fixes its shapes in code (radix_sort.rs:292-368: 4-bit digits,
- * 256-key blocks; compositor.rs:721-883: batches of 256); here they are
- * per-context settings that cgs_autotune picks by timing.  Defaults:
- * {16384, 8, 256, 1, 0}; the radix fields matter only with tile_sort = 0. */
+ * radix partition size and digit width of the Stage 3 sort, the tile sort,
+ * splats staged per compositor step, and the heaviest-tiles-first compositor
+ * order.  The reference fixes its shapes in code (radix_sort.rs:292-368:
+ * 4-bit digits, 256-key blocks; compositor.rs:721-883: batches of 256;
+ * TILE_SIZE = 16, tiles/mod.rs:10); here they are per-context settings that
+ * cgs_autotune picks by timing.  The tile size is one of them but changes the
+ * frame: with tiles of w x h pixels every output is the reference's algorithm
+ * run at that tile size (tile rectangles, tile count, and with it the split of
+ * the 32-bit key between tile and depth bits).  Defaults:
+ * {16384, 8, 256, 1, 0, 16, 16}; the radix fields matter only with
+ * tile_sort = 0. */
 typedef struct cgs_tuning {
   uint32_t radix_partition_keys; /* 16384, 12288, 10240, 8192 or 4096 keys per
                                     sort partition (1024 threads x 16 or 12
                                     keys; 640, 512 or 256 threads x 16) */
   uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
-  uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step */
+  uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step
+                                    (8 x 8 tiles: 64) */
   uint32_t tile_order;           /* 1: heaviest tiles composited first */
   uint32_t tile_sort;            /* 1: entries written into per-tile buckets and each
                                     tile sorted by (key, splat id) in LDS; 0: one
                                     stable LSD radix sort of all entries (the
                                     reference's method).  Frames of more than 32768
                                     tiles always take the radix sort. */
+  uint32_t tile_w, tile_h;       /* tile size in pixels: 16 x 16 (the reference's),
+                                    8 x 8, 16 x 32 or 32 x 16 */
 } cgs_tuning;
 
 int cgs_set_tuning(cgs_context* ctx, const cgs_tuning* tuning);
 int cgs_get_tuning(cgs_context* ctx, cgs_tuning* tuning);
 
-/* Times every cgs_tuning choice on `reps` frames of the given view (after one
- * untimed frame each), checks that each gives the same framebuffer and tile
- * entries as the current setting, and keeps the fastest.  ms[i] (optional,
- * n_ms entries) gets the mean device time of choice i in the order of
- * cgs_tuning_candidate; *best gets the chosen setting. */
+/* cgs_autotune flags: also sweep the other tile sizes (whose frames differ
+ * from the current size's; each is checked against its own first frame) */
+#define CGS_TUNE_TILE_SHAPES 1u
+
+/* Times the cgs_tuning choices of the current tile size (all sizes with
+ * CGS_TUNE_TILE_SHAPES) on `reps` frames of the given view (after one untimed
+ * frame each), checks that each gives the same framebuffer and tile entries as
+ * the current setting (as the first choice of its tile size), and keeps the
+ * fastest.  ms[i] (optional, n_ms entries) gets the mean device time of choice
+ * i in the order of cgs_tuning_candidate, or -1 for a choice not swept; *best
+ * gets the chosen setting. */
 int cgs_autotune(cgs_context* ctx, const cgs_camera_uniforms* camera,
                  const cgs_compositor_uniforms* compositor, const cgs_frame_params* params,
-                 uint32_t reps, cgs_tuning* best, double* ms, uint32_t n_ms);
+                 uint32_t reps, uint32_t flags, cgs_tuning* best, double* ms, uint32_t n_ms);
 /* candidate i (< cgs_tuning_candidates()) of the autotuner's sweep */
 uint32_t cgs_tuning_candidates(void);
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out);

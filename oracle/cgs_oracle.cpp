@@ -401,14 +401,16 @@ static uint32_t project_all(const ProjParams& P, const float* pos, const float* 
 
 // ellipse_intersects_tile_gpu (binning.rs:35-83) with the per-splat cutoff
 // q_cutoff = clamp(2 ln(255 alpha), 0, 8) hoisted (a pure function of alpha).
+// Tiles of tw x th pixels (the reference's TILE_SIZE = 16, tiles/mod.rs:10,
+// as a parameter): the tile's pixel-centre rectangle.
 template <class T>
 static bool ellipse_tile(T mu_x, T mu_y, T a, T b, T c, T alpha, T q_cut, uint32_t tx, uint32_t ty,
-                         uint32_t W, uint32_t H) {
+                         uint32_t W, uint32_t H, uint32_t tw, uint32_t th) {
   typedef Mth<T> m;
   if (alpha < (T)(1.0f / 255.0f) || W == 0 || H == 0) return false;
-  uint32_t xs = tx * 16u, ys = ty * 16u;
+  uint32_t xs = tx * tw, ys = ty * th;
   if (xs >= W || ys >= H) return false;
-  uint32_t xe = std::min(xs + 15u, W - 1u), ye = std::min(ys + 15u, H - 1u);
+  uint32_t xe = std::min(xs + tw - 1u, W - 1u), ye = std::min(ys + th - 1u, H - 1u);
   T tminx = (T)(float)xs + (T)0.5, tmaxx = (T)(float)xe + (T)0.5;
   T tminy = (T)(float)ys + (T)0.5, tmaxy = (T)(float)ye + (T)0.5;
   T dx0 = tminx - mu_x, dx1 = tmaxx - mu_x, dy0 = tminy - mu_y, dy1 = tmaxy - mu_y;
@@ -426,16 +428,19 @@ struct Bounds {
   uint32_t tx0, tx1, ty0, ty1;
 };
 
+// compute_gaussian_tile_bounds_xy (tiles/mod.rs:97-130) with TILE_SIZE_F32
+// replaced by the tile width / height
 static Bounds tile_bounds(float mu_x, float mu_y, uint32_t packed, uint32_t tiles_x,
-                          uint32_t tiles_y) {
+                          uint32_t tiles_y, uint32_t tw, uint32_t th) {
   float ext_x = (float)(packed & 0xFFFFu);
   float ext_y = (float)((packed >> 16) & 0xFFFFu);
   int32_t mtx = (int32_t)tiles_x - 1, mty = (int32_t)tiles_y - 1;
+  const float fw = (float)tw, fh = (float)th;
   Bounds r;
-  r.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / 16.0f)), 0, mtx);
-  r.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / 16.0f)), 0, mtx);
-  r.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / 16.0f)), 0, mty);
-  r.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / 16.0f)), 0, mty);
+  r.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / fw)), 0, mtx);
+  r.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / fw)), 0, mtx);
+  r.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / fh)), 0, mty);
+  r.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / fh)), 0, mty);
   return r;
 }
 
@@ -509,7 +514,8 @@ ORC_API void orc_packed_key_bits(uint32_t total_tiles, uint32_t* tile_bits, uint
 // count_tiles_kernel (binning.rs:90-164)
 ORC_API void orc_count_tiles(const float* attrs10, const uint32_t* extents, uint32_t active,
                              uint32_t max_gaussians, uint32_t W, uint32_t H, uint32_t tiles_x,
-                             uint32_t tiles_y, uint32_t* counts) {
+                             uint32_t tiles_y, uint32_t tile_w, uint32_t tile_h,
+                             uint32_t* counts) {
 #pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
   for (int64_t ii = 0; ii < (int64_t)max_gaussians; ++ii) {
     uint32_t idx = (uint32_t)ii;
@@ -518,12 +524,13 @@ ORC_API void orc_count_tiles(const float* attrs10, const uint32_t* extents, uint
       continue;
     }
     const float* a = attrs10 + (size_t)idx * 10;
-    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y, tile_w, tile_h);
     float qc = qcut_of(a[5]);
     uint32_t cnt = 0;
     for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
       for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
-        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) cnt++;
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H, tile_w,
+                                tile_h)) cnt++;
     counts[idx] = cnt;
   }
 }
@@ -543,8 +550,9 @@ ORC_API uint32_t orc_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t 
 // emit_tile_keys_kernel (binning.rs:168-250)
 ORC_API void orc_emit_keys32(const float* attrs10, const uint32_t* extents,
                              const uint32_t* offsets, uint32_t active, uint32_t W, uint32_t H,
-                             uint32_t tiles_x, uint32_t tiles_y, uint32_t base_depth_bits,
-                             uint32_t depth_bits, uint32_t* keys, uint32_t* ids) {
+                             uint32_t tiles_x, uint32_t tiles_y, uint32_t tile_w,
+                             uint32_t tile_h, uint32_t base_depth_bits, uint32_t depth_bits,
+                             uint32_t* keys, uint32_t* ids) {
 #pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
   for (int64_t ii = 0; ii < (int64_t)active; ++ii) {
     uint32_t idx = (uint32_t)ii;
@@ -553,12 +561,13 @@ ORC_API void orc_emit_keys32(const float* attrs10, const uint32_t* extents,
     uint32_t raw = cgs_f2u(cgs_fmax(a[9], 1e-6f));
     uint32_t rel = raw > base_depth_bits ? std::min(raw - base_depth_bits, 0x0FFFFFFFu) : 0u;
     uint32_t dq = rel >> (28u - depth_bits);
-    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y, tile_w, tile_h);
     float qc = qcut_of(a[5]);
     uint32_t cur = offsets[idx];
     for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
       for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
-        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) {
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H, tile_w,
+                                tile_h)) {
           uint32_t tile = ty * tiles_x + tx;
           keys[cur] = (tile << depth_bits) | dq;
           ids[cur] = idx;
@@ -570,20 +579,22 @@ ORC_API void orc_emit_keys32(const float* attrs10, const uint32_t* extents,
 // emit_tile_keys_u64_kernel (binning.rs:254-328)
 ORC_API void orc_emit_keys64(const float* attrs10, const uint32_t* extents,
                              const uint32_t* offsets, uint32_t active, uint32_t W, uint32_t H,
-                             uint32_t tiles_x, uint32_t tiles_y, uint32_t* tile_ids,
-                             uint32_t* depths, uint32_t* ids) {
+                             uint32_t tiles_x, uint32_t tiles_y, uint32_t tile_w,
+                             uint32_t tile_h, uint32_t* tile_ids, uint32_t* depths,
+                             uint32_t* ids) {
 #pragma omp parallel for schedule(static, 2048) num_threads(g_threads)
   for (int64_t ii = 0; ii < (int64_t)active; ++ii) {
     uint32_t idx = (uint32_t)ii;
     if (tiles_x == 0 || tiles_y == 0 || extents[idx] == 0) continue;
     const float* a = attrs10 + (size_t)idx * 10;
     uint32_t dbits = cgs_f2u(cgs_fmax(a[9], 0.0f));
-    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y);
+    Bounds bd = tile_bounds(a[0], a[1], extents[idx], tiles_x, tiles_y, tile_w, tile_h);
     float qc = qcut_of(a[5]);
     uint32_t cur = offsets[idx];
     for (uint32_t ty = bd.ty0; ty <= bd.ty1; ++ty)
       for (uint32_t tx = bd.tx0; tx <= bd.tx1; ++tx)
-        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H)) {
+        if (ellipse_tile<float>(a[0], a[1], a[2], a[3], a[4], a[5], qc, tx, ty, W, H, tile_w,
+                                tile_h)) {
           tile_ids[cur] = ty * tiles_x + tx;
           depths[cur] = dbits;
           ids[cur] = idx;
@@ -664,16 +675,15 @@ ORC_API void orc_tile_ranges(const uint32_t* sorted, uint64_t n, uint32_t shift,
 // ---------------------------------------------------------------------------
 template <class T, class AT>
 static void composite_impl(const AT* attrs10, const uint32_t* sorted_ids, const uint32_t* ranges,
-                           uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t padded_row,
-                           const float* bg, uint32_t* out, float* out_rgba_f) {
+                           uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t tile_w,
+                           uint32_t tile_h, uint32_t padded_row, const float* bg, uint32_t* out,
+                           float* out_rgba_f) {
   typedef Mth<T> m;
-  uint32_t tiles_y = (H + 15) / 16;
-  (void)tiles_y;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
     uint32_t py = (uint32_t)yy;
     for (uint32_t px = 0; px < W; ++px) {
-      uint32_t tile = (py / 16) * tiles_x + (px / 16);
+      uint32_t tile = (py / tile_h) * tiles_x + (px / tile_w);
       uint32_t rs = ranges[tile * 2], re = ranges[tile * 2 + 1];
       bool empty = rs >= re;
       T cr = 0, cg = 0, cb = 0, t = 1;
@@ -730,14 +740,14 @@ static void composite_impl(const AT* attrs10, const uint32_t* sorted_ids, const 
 // then be double).
 ORC_API void orc_composite_v2(const void* attrs10, const uint32_t* sorted_ids,
                               const uint32_t* ranges, uint32_t W, uint32_t H, uint32_t tiles_x,
-                              uint32_t padded_row, const float* bg4, int fp64, uint32_t* out,
-                              float* out_f) {
+                              uint32_t tile_w, uint32_t tile_h, uint32_t padded_row,
+                              const float* bg4, int fp64, uint32_t* out, float* out_f) {
   if (fp64)
     composite_impl<double, double>((const double*)attrs10, sorted_ids, ranges, W, H, tiles_x,
-                                   padded_row, bg4, out, out_f);
+                                   tile_w, tile_h, padded_row, bg4, out, out_f);
   else
     composite_impl<float, float>((const float*)attrs10, sorted_ids, ranges, W, H, tiles_x,
-                                 padded_row, bg4, out, out_f);
+                                 tile_w, tile_h, padded_row, bg4, out, out_f);
 }
 
 // ---------------------------------------------------------------------------

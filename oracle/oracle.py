@@ -42,17 +42,18 @@ def lib() -> C.CDLL:
             "orc_project_v2": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _f,
                                 _f, _f, _u32, _u32, _u32, _u32, _u32, _i, _vp, _vp], _u32),
             "orc_packed_key_bits": ([_u32, C.POINTER(_u32), C.POINTER(_u32)], None),
-            "orc_count_tiles": ([_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp], None),
-            "orc_exclusive_scan": ([_vp, _vp, _u64], _u32),
-            "orc_emit_keys32": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp,
-                                 _vp], None),
-            "orc_emit_keys64": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp],
+            "orc_count_tiles": ([_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp],
                                 None),
+            "orc_exclusive_scan": ([_vp, _vp, _u64], _u32),
+            "orc_emit_keys32": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
+                                 _u32, _vp, _vp], None),
+            "orc_emit_keys64": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp,
+                                 _vp, _vp], None),
             "orc_sort_pairs32": ([_vp, _vp, _u64, _u32], None),
             "orc_sort64": ([_vp, _vp, _vp, _u64, _u32], None),
             "orc_tile_ranges": ([_vp, _u64, _u32, _u32, _vp], None),
-            "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _vp, _i, _vp, _vp],
-                                 None),
+            "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp, _i, _vp,
+                                  _vp], None),
             "orc_mvc_far_share": ([_vp, _u32, _vp, _u32, _vp, _u64, _f, _f, _i, _vp], None),
             "orc_mvc_scene": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp,
                                _vp, _vp], _u64),
@@ -137,13 +138,19 @@ def project(scene, cam_arr, cam_pos, sh_degree=3, mip=0.0, min_opacity=1.0 / 255
     return attrs, ext, surv
 
 
-def count_tiles(attrs, ext, W, H, active=None):
+def tile_grid(W, H, tile=(16, 16)):
+    """TileGrid::new (tiles/mod.rs:39-49) for tiles of tile = (w, h) pixels."""
+    tw, th = tile
+    return (W + tw - 1) // tw, (H + th - 1) // th
+
+
+def count_tiles(attrs, ext, W, H, active=None, tile=(16, 16)):
     n = ext.shape[0]
     active = n if active is None else active
-    tx, ty = (W + 15) // 16, (H + 15) // 16
+    tx, ty = tile_grid(W, H, tile)
     out = np.zeros(n, dtype=np.uint32)
     lib().orc_count_tiles(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), active, n,
-                          W, H, tx, ty, _p(out))
+                          W, H, tx, ty, tile[0], tile[1], _p(out))
     return out
 
 
@@ -154,27 +161,28 @@ def exclusive_scan(v):
     return out, tot
 
 
-def emit32(attrs, ext, offsets, total, W, H, near, active=None):
+def emit32(attrs, ext, offsets, total, W, H, near, active=None, tile=(16, 16)):
     n = ext.shape[0]
     active = n if active is None else active
-    tx, ty = (W + 15) // 16, (H + 15) // 16
+    tx, ty = tile_grid(W, H, tile)
     _, db = packed_key_bits(tx * ty)
     keys = np.zeros(max(total, 1), dtype=np.uint32)
     ids = np.zeros(max(total, 1), dtype=np.uint32)
     lib().orc_emit_keys32(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), _p(offsets),
-                          active, W, H, tx, ty, base_depth_bits(near), db, _p(keys), _p(ids))
+                          active, W, H, tx, ty, tile[0], tile[1], base_depth_bits(near), db,
+                          _p(keys), _p(ids))
     return keys[:total], ids[:total], db
 
 
-def emit64(attrs, ext, offsets, total, W, H, active=None):
+def emit64(attrs, ext, offsets, total, W, H, active=None, tile=(16, 16)):
     n = ext.shape[0]
     active = n if active is None else active
-    tx, ty = (W + 15) // 16, (H + 15) // 16
+    tx, ty = tile_grid(W, H, tile)
     tiles = np.zeros(max(total, 1), dtype=np.uint32)
     depths = np.zeros(max(total, 1), dtype=np.uint32)
     ids = np.zeros(max(total, 1), dtype=np.uint32)
     lib().orc_emit_keys64(_p(np.ascontiguousarray(attrs, dtype=np.float32)), _p(ext), _p(offsets),
-                          active, W, H, tx, ty, _p(tiles), _p(depths), _p(ids))
+                          active, W, H, tx, ty, tile[0], tile[1], _p(tiles), _p(depths), _p(ids))
     return tiles[:total], depths[:total], ids[:total]
 
 
@@ -200,8 +208,8 @@ def tile_ranges(sorted_keys, shift, total_tiles):
     return r.reshape(-1, 2)
 
 
-def composite(attrs, sorted_ids, ranges, W, H, bg, fp64=False, want_float=False):
-    tx = (W + 15) // 16
+def composite(attrs, sorted_ids, ranges, W, H, bg, fp64=False, want_float=False, tile=(16, 16)):
+    tx, _ = tile_grid(W, H, tile)
     pad = (W + 63) // 64 * 64
     out = np.zeros(pad * H, dtype=np.uint32)
     outf = np.zeros((H, W, 4), dtype=np.float32) if want_float else None
@@ -210,15 +218,16 @@ def composite(attrs, sorted_ids, ranges, W, H, bg, fp64=False, want_float=False)
     if ids.size == 0:
         ids = np.zeros(1, dtype=np.uint32)
     lib().orc_composite_v2(_p(a), _p(ids), _p(np.ascontiguousarray(ranges, dtype=np.uint32)), W,
-                           H, tx, pad, _p(np.asarray(bg, dtype=np.float32)), 1 if fp64 else 0,
-                           _p(out), _p(outf))
+                           H, tx, tile[0], tile[1], pad, _p(np.asarray(bg, dtype=np.float32)),
+                           1 if fp64 else 0, _p(out), _p(outf))
     rgba = out.reshape(H, pad)[:, :W].copy().view(np.uint8).reshape(H, W, 4)
     return (rgba, outf) if want_float else rgba
 
 
 def render_frame(scene, cam, comp, sh_degree=None, mip=0.0, min_opacity=1.0 / 255.0,
-                 min_radius=0.5, f16=None):
-    """Stages 2-4 of execute_frame_gpu without cages, all intermediates returned."""
+                 min_radius=0.5, f16=None, tile=(16, 16)):
+    """Stages 2-4 of execute_frame_gpu without cages, all intermediates returned;
+    tile = (w, h) pixels per tile (the reference's 16 x 16 by default)."""
     from cindergs_amd.synthetic import camera_array  # plain data helper
 
     W, H = max(1, comp.screen_width), max(1, comp.screen_height)
@@ -226,22 +235,23 @@ def render_frame(scene, cam, comp, sh_degree=None, mip=0.0, min_opacity=1.0 / 25
     deg = comp.sh_degree if sh_degree is None else sh_degree
     attrs, ext, surv = project(scene, cam_arr, list(comp.camera_position), deg, mip, min_opacity,
                                min_radius, f16=f16)
-    counts = count_tiles(attrs, ext, W, H)
+    counts = count_tiles(attrs, ext, W, H, tile=tile)
     offsets, total = exclusive_scan(counts)
-    T = ((W + 15) // 16) * ((H + 15) // 16)
+    tx, ty = tile_grid(W, H, tile)
+    T = tx * ty
     tb, _ = packed_key_bits(T)
     out = {"attrs": attrs, "extents": ext, "survivors": surv, "counts": counts,
            "offsets": offsets, "total": total}
     if T <= 65535:
-        keys, ids, db = emit32(attrs, ext, offsets, total, W, H, cam.near_plane)
+        keys, ids, db = emit32(attrs, ext, offsets, total, W, H, cam.near_plane, tile=tile)
         sk, sid = sort_pairs32(keys, ids, min(32, tb + db))
         ranges = tile_ranges(sk, db, T)
     else:
-        tiles, depths, ids = emit64(attrs, ext, offsets, total, W, H)
+        tiles, depths, ids = emit64(attrs, ext, offsets, total, W, H, tile=tile)
         sk, _, sid = sort64(tiles, depths, ids, tb)
         ranges = tile_ranges(sk, 0, T)
     out.update(sorted_keys=sk, sorted_ids=sid, ranges=ranges)
-    out["rgba"] = composite(attrs, sid, ranges, W, H, list(comp.background_color))
+    out["rgba"] = composite(attrs, sid, ranges, W, H, list(comp.background_color), tile=tile)
     return out
 
 

@@ -58,23 +58,37 @@ def test_no_device_maps_to_noadapter():
 
 
 def test_tuning_candidates_cover_every_knob():
-    """The autotuner's sweep (host-only entry points, no GPU needed): the
-    bucket sort with every compositor batch and tile order, and the LSD radix
-    sort with every combination of partition size, digit width, compositor
-    batch and tile order, each exactly once."""
+    """The autotuner's sweep (host-only entry points, no GPU needed).  16 x 16
+    tiles: the bucket sort with every compositor batch and tile order, and the
+    LSD radix sort with every combination of partition size, digit width,
+    compositor batch and tile order.  8 x 8 tiles (batch 64): radix partitions
+    16384 / 8192 and the bucket sort, each with both orders.  16 x 32 and
+    32 x 16: the same with batches 256 / 128.  Each exactly once, 16 x 16
+    first."""
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
     n = tuning_candidates_count()
     combos = [tuple(tuning_candidate(i).as_dict().values()) for i in range(n)]
-    assert n == len(set(combos)) == 6 + 5 * 2 * 3 * 2
-    bucket = [c for c in combos if c[4] == 1]
-    radix = [c for c in combos if c[4] == 0]
+    assert n == len(set(combos)) == 66 + 6 + 12 + 12
+    base = [c for c in combos if c[5:] == (16, 16)]
+    assert combos[:66] == base
+    bucket = [c for c in base if c[4] == 1]
+    radix = [c for c in base if c[4] == 0]
     assert len(bucket) == 6 and {(c[2], c[3]) for c in bucket} == {
         (b, o) for b in (256, 128, 64) for o in (0, 1)}
     assert {c[0] for c in radix} == {16384, 12288, 10240, 8192, 4096}
     assert {c[1] for c in radix} == {8, 4}
     assert {c[2] for c in radix} == {256, 128, 64}
     assert {c[3] for c in radix} == {0, 1}
+    small = [c for c in combos if c[5:] == (8, 8)]
+    assert sorted(small) == sorted(
+        [(p, 8, 64, o, 0, 8, 8) for p in (16384, 8192) for o in (0, 1)]
+        + [(16384, 8, 64, o, 1, 8, 8) for o in (0, 1)])
+    for shape in ((16, 32), (32, 16)):
+        got = [c for c in combos if c[5:] == shape]
+        assert sorted(got) == sorted(
+            [(p, 8, b, o, 0) + shape for p in (16384, 8192) for b in (256, 128) for o in (0, 1)]
+            + [(16384, 8, b, o, 1) + shape for b in (256, 128) for o in (0, 1)])
     assert tuning_candidate(0).as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
                                              "composite_batch": 256, "tile_order": 1,
-                                             "tile_sort": 0}
+                                             "tile_sort": 0, "tile_w": 16, "tile_h": 16}

@@ -1,0 +1,172 @@
+"""Tile sizes other than the reference's 16 x 16 (cgs_tuning.tile_w/tile_h)
+on the GPU, against the oracle run at the same tile size.
+
+Bar (the parity tests' bar, per size): tile counts, scan offsets, sorted keys,
+sorted ids and tile ranges bit-exact, RGBA8 within 1/255, for 8 x 8, 16 x 32
+and 32 x 16 tiles with the LSD radix sort and with the tile-bucket sort (the
+two give the identical frame); 8 x 8 tiles at 3840 x 2160 (129600 tiles) take
+the 64-bit keys; the autotuner over every size renders, for the size it keeps,
+the oracle's frame at that size; a batch at 8 x 8 matches per-view frames.
+"""
+import numpy as np
+import pytest
+
+from cindergs_amd import synthetic as syn
+from cindergs_amd._lib import StageTimings, Tuning
+from cindergs_amd.pipeline import FrameParams, GpuPipelineState, ViewBatch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8), (16, 32), (32, 16)]
+
+
+def _tuning(tile, sort, batch=None):
+    b = batch or (64 if tile == (8, 8) else 256)
+    return Tuning(16384, 8, b, 1, sort, tile[0], tile[1])
+
+
+def _run(st, scene, cam, comp, tile):
+    W, H = comp.screen_width, comp.screen_height
+    T = ((W + tile[0] - 1) // tile[0]) * ((H + tile[1] - 1) // tile[1])
+    st.set_capture(True)
+    t = st.execute_frame_gpu(scene, None, cam, comp, force_stage3=True)
+    st.set_capture(False)
+    s3 = st.read_stage3(scene.count, T, with_counts=True)
+    return t, s3, st.readback_framebuffer_rgba8(W, H)
+
+
+def _check(scene, cam, comp, tile, label, sorts=(0, 1)):
+    import oracle
+
+    W, H = comp.screen_width, comp.screen_height
+    print(f"[case] tiles {tile[0]}x{tile[1]} {label}: n={scene.count} {W}x{H}", flush=True)
+    ref = oracle.render_frame(scene, cam, comp, tile=tile)
+    imgs = {}
+    for sort in sorts:
+        with GpuPipelineState(0) as st:
+            st.set_tuning(_tuning(tile, sort))
+            t, s3, img = _run(st, scene, cam, comp, tile)
+        assert s3["n"] == ref["total"] == t.total_tile_entries, (label, sort)
+        for k, rk in (("counts", "counts"), ("offsets", "offsets"), ("keys", "sorted_keys"),
+                      ("ids", "sorted_ids"), ("ranges", "ranges")):
+            np.testing.assert_array_equal(s3[k], ref[rk], err_msg=f"{label} sort={sort}: {k}")
+        d = int(np.abs(img.astype(np.int16) - ref["rgba"].astype(np.int16)).max())
+        print(f"[case] tiles {tile[0]}x{tile[1]} {label} sort={sort}: E={s3['n']} "
+              f"max|dRGBA|={d}", flush=True)
+        assert d <= 1, f"{label} sort={sort}: RGBA differs by {d}/255"
+        imgs[sort] = img
+    if len(imgs) == 2:
+        assert np.array_equal(imgs[0], imgs[1]), f"{label}: radix and bucket frames differ"
+    return ref
+
+
+@pytest.mark.parametrize("tile", SHAPES)
+def test_tile_size_matches_oracle_720p(oracle_lib, tile):
+    scene = syn.scene_clusters(150_000, k=71)
+    cam, comp = syn.view_orbit(7, 1280, 720)
+    _check(scene, cam, comp, tile, "150k 720p")
+
+
+@pytest.mark.parametrize("tile", SHAPES)
+def test_tile_size_matches_oracle_odd_screen(oracle_lib, tile):
+    """A screen of no whole tiles in either direction (partial edge tiles)."""
+    scene = syn.scene_gaussian_ball(40_000, k=72)
+    cam, comp = syn.view_orbit(2, 1001, 557)
+    _check(scene, cam, comp, tile, "40k 1001x557")
+
+
+def test_8x8_1080p_orbit_matches_oracle(oracle_lib):
+    """The 1.07M-splat 1080p orbit view: 32400 tiles of 8 x 8 (32-bit keys, 15
+    tile bits), the most the bucket sort takes."""
+    scene = syn.scene_clusters()
+    cam, comp = syn.view_orbit(0)
+    _check(scene, cam, comp, (8, 8), "1.07M 1080p")
+
+
+def test_8x8_at_2160p_uses_64bit_keys(oracle_lib):
+    scene = syn.scene_gaussian_ball(60_000, k=74)
+    cam, comp = syn.view_orbit(4, 3840, 2160)
+    assert (3840 // 8) * (2160 // 8) > 65535
+    _check(scene, cam, comp, (8, 8), "60k 2160p (64-bit keys)", sorts=(1,))
+
+
+def test_autotune_over_tile_sizes(oracle_lib):
+    import oracle
+
+    scene = syn.scene_clusters(100_000, k=75)
+    cam, comp = syn.view_orbit(3, 960, 540)
+    p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    with GpuPipelineState(0) as st:
+        st.sync_scene_and_cages(scene, None)
+        best, table = st.autotune(cam, comp, p, reps=1, tile_shapes=True)
+        print(f"[case] autotune over tile sizes: best {best.as_dict()}", flush=True)
+        assert all(ms > 0 for _, ms in table)
+        assert any(c.as_dict() == best.as_dict() for c, _ in table)
+        assert st.get_tuning().as_dict() == best.as_dict()
+        tile = (best.tile_w, best.tile_h)
+        _, s3, img = _run(st, scene, cam, comp, tile)
+        # the default sweep stays on the current tile size
+        _, table16 = st.autotune(cam, comp, p, reps=1)
+        swept = [c for c, ms in table16 if ms > 0]
+        assert swept and all((c.tile_w, c.tile_h) == tile for c in swept)
+    ref = oracle.render_frame(scene, cam, comp, tile=tile)
+    np.testing.assert_array_equal(s3["keys"], ref["sorted_keys"])
+    np.testing.assert_array_equal(s3["ids"], ref["sorted_ids"])
+    np.testing.assert_array_equal(s3["ranges"], ref["ranges"])
+    assert int(np.abs(img.astype(np.int16) - ref["rgba"].astype(np.int16)).max()) <= 1
+
+
+def test_tile_size_change_drops_the_kept_stage3():
+    """A frame after cgs_set_tuning changes the tile size runs Stage 3 on the
+    new grid, also when the view is unchanged (the kept Stage 3 is for the old
+    grid)."""
+    scene = syn.scene_clusters(60_000, k=76)
+    cam, comp = syn.view_orbit(1, 800, 600)
+    p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    imgs = {}
+    with GpuPipelineState(0) as st:
+        st.sync_scene_and_cages(scene, None)
+        for tile in ((16, 16), (8, 8), (16, 16), (32, 16)):
+            st.set_tuning(_tuning(tile, 0))
+            t = StageTimings()
+            st.execute_frame_resident(cam, comp, p, t)
+            assert t.stage3_cached == 0, tile
+            st.execute_frame_resident(cam, comp, p, t)
+            assert t.stage3_cached == 1, tile
+            img = st.readback_framebuffer_rgba8(800, 600)
+            if tile in imgs:
+                assert np.array_equal(img, imgs[tile]), tile
+            imgs[tile] = img
+    for tile in ((8, 8), (32, 16)):
+        with GpuPipelineState(0) as st:
+            st.set_tuning(_tuning(tile, 0))
+            _, _, img = _run(st, scene, cam, comp, tile)
+        assert np.array_equal(img, imgs[tile]), tile
+
+
+def test_batch_at_8x8_matches_per_view_frames():
+    scenes = [syn.scene_gaussian_ball(30_000, k=77), syn.scene_clusters(50_000, k=78)]
+    states = []
+    tune = _tuning((8, 8), 1)
+    for sc in scenes:
+        st = GpuPipelineState(0)
+        st.sync_scene_and_cages(sc, None)
+        st.set_tuning(tune)
+        states.append(st)
+    view_scene, cams, comps = syn.batch_views(5, 2, seed=17,
+                                              resolutions=((320, 240), (800, 600), (1001, 557)))
+    p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
+    with ViewBatch(0, lanes=2) as batch:
+        batch.set_tuning(tune)
+        stats = batch.execute(states, view_scene, cams, comps, p)
+        print(f"[case] batch at 8x8: {stats.as_dict()}", flush=True)
+        for v in range(len(cams)):
+            st = states[view_scene[v]]
+            t = StageTimings()
+            st.execute_frame_resident(cams[v], comps[v], p, t)
+            W, H = comps[v].screen_width, comps[v].screen_height
+            ref = st.readback_framebuffer_rgba8(W, H)
+            assert np.array_equal(batch.readback_rgba8(v), ref), f"view {v}"
+            assert batch.view_counts(v)[0] == int(t.total_tile_entries), f"view {v}"
+    for st in states:
+        st.close()

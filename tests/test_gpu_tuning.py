@@ -2,11 +2,14 @@
 
 Bar: every tuning candidate (the per-tile bucket sort, or the LSD radix sort
 with partitions of 16384/12288/10240/8192/4096 keys and 8- or 4-bit digits;
-compositor batch 256/128/64, heaviest-tiles-first order on/off) gives
-bit-identical tile counts, scan offsets, sorted keys, sorted ids, tile ranges
-and RGBA8 to the default setting, which the parity tests hold to the oracle;
-the 64-bit key path (8K) likewise for the partition sizes; the autotuner keeps
-one of the candidates and the frame it renders is unchanged.
+compositor batch 256/128/64, heaviest-tiles-first order on/off) of a tile
+size gives bit-identical tile counts, scan offsets, sorted keys, sorted ids,
+tile ranges and RGBA8 to the first candidate of that size (16 x 16: to the
+default setting, which the parity tests hold to the oracle; the other sizes
+are held to the oracle at their size in test_gpu_tile_shapes.py); the 64-bit
+key path (8K) likewise for the partition sizes; the autotuner sweeps the
+current tile size only, keeps one of its candidates and the frame it renders
+is unchanged.
 """
 import numpy as np
 import pytest
@@ -25,7 +28,8 @@ def _frame(gpu, cam, comp, n):
     gpu.execute_frame_resident(cam, comp, p, t)
     gpu.set_capture(False)
     W, H = comp.screen_width, comp.screen_height
-    T = ((W + 15) // 16) * ((H + 15) // 16)
+    tw, th = gpu.get_tuning().tile_w, gpu.get_tuning().tile_h
+    T = ((W + tw - 1) // tw) * ((H + th - 1) // th)
     s3 = gpu.read_stage3(n, T, with_counts=True)
     return s3, gpu.readback_framebuffer_rgba8(W, H)
 
@@ -44,18 +48,27 @@ def test_every_candidate_gives_the_same_frame(gpu):
     cam, comp = syn.view_orbit(7, 1280, 720)
     default = gpu.get_tuning()
     assert default.as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
-                                 "composite_batch": 256, "tile_order": 1, "tile_sort": 0}
-    ref = _frame(gpu, cam, comp, scene.count)
+                                 "composite_batch": 256, "tile_order": 1, "tile_sort": 0,
+                                 "tile_w": 16, "tile_h": 16}
+    refs = {(16, 16): _frame(gpu, cam, comp, scene.count)}
     n = tuning_candidates_count()
     seen = set()
-    for i in range(n):
-        c = tuning_candidate(i)
-        seen.add(tuple(c.as_dict().values()))
-        print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
-        gpu.set_tuning(c)
-        _same(ref, _frame(gpu, cam, comp, scene.count), f"candidate {i}")
-    assert len(seen) == n == 66
-    gpu.set_tuning(default)
+    try:
+        for i in range(n):
+            c = tuning_candidate(i)
+            seen.add(tuple(c.as_dict().values()))
+            print(f"[case] candidate {i}: {c.as_dict()}", flush=True)
+            gpu.set_tuning(c)
+            got = _frame(gpu, cam, comp, scene.count)
+            size = (c.tile_w, c.tile_h)
+            if size in refs:
+                _same(refs[size], got, f"candidate {i}")
+            else:
+                refs[size] = got
+    finally:
+        gpu.set_tuning(default)
+    assert len(seen) == n == 96
+    assert set(refs) == {(16, 16), (8, 8), (16, 32), (32, 16)}
 
 
 def test_partition_sizes_on_64bit_keys(gpu):
@@ -83,7 +96,9 @@ def test_autotune_keeps_a_candidate_and_the_frame(gpu):
     best, table = gpu.autotune(cam, comp, p, reps=2)
     print(f"[case] autotune best {best.as_dict()}", flush=True)
     assert any(c.as_dict() == best.as_dict() for c, _ in table)
-    assert all(ms > 0 for _, ms in table)
+    assert (best.tile_w, best.tile_h) == (16, 16)
+    assert all((ms > 0) == ((c.tile_w, c.tile_h) == (16, 16)) for c, ms in table)
+    assert all(ms == -1.0 for c, ms in table if (c.tile_w, c.tile_h) != (16, 16))
     assert gpu.get_tuning().as_dict() == best.as_dict()
     _same(ref, _frame(gpu, cam, comp, scene.count), "autotuned")
     gpu.set_tuning(default)
@@ -93,6 +108,8 @@ def test_invalid_tuning_is_rejected(gpu):
     from cindergs_amd._lib import GpuError
 
     for bad in (Tuning(2048, 8, 256, 1), Tuning(12000, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
-                Tuning(16384, 8, 256, 2), Tuning(16384, 8, 256, 1, 2)):
+                Tuning(16384, 8, 256, 2), Tuning(16384, 8, 256, 1, 2),
+                Tuning(16384, 8, 256, 1, 0, 32, 32), Tuning(16384, 8, 256, 1, 0, 8, 16),
+                Tuning(16384, 8, 256, 1, 0, 0, 0), Tuning(16384, 8, 128, 1, 0, 8, 8)):
         with pytest.raises(GpuError):
             gpu.set_tuning(bad)
