@@ -775,7 +775,8 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
             best = (lanes, wall, int(stats.total_tile_entries), None)
     for i in range(tuning_candidates_count() if fixed is None else 0):
         tu = tuning_candidate(i)
-        if tu.radix_digit_bits != 8 or i == 0:
+        # the reference's frames: 16 x 16 tiles only
+        if tu.radix_digit_bits != 8 or i == 0 or (tu.tile_w, tu.tile_h) != (16, 16):
             continue
         wall, gms, stats = run_batch(best[0], tu)
         key = "lanes={} sort={} part={} batch={} order={}".format(

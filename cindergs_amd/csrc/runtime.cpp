@@ -2969,8 +2969,11 @@ void batch_drop_graph(cgs_batch* b) {
   b->key = 0;
 }
 
-// every buffer a view of (max_g splat slots, T tiles) touches in Stages 2-4
-int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_records) {
+// every buffer the views of a lane touch in Stages 2-4: up to max_g splat
+// slots and T tiles, and in bucket mode up to bucket_g slots and bucket_T
+// tiles (the views of at most CGS_BUCKET_MAX_TILES tiles; 0: none)
+int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_records,
+               uint32_t bucket_g, uint32_t bucket_T) {
   init_entry_cap(a, max_g);
   const uint32_t cap = a.entry_cap;
   TRY(ensure_words(a.geom, own_records ? (size_t)8 * max_g : 8));
@@ -2989,7 +2992,7 @@ int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_
   TRY(ensure_words(a.ranges, (size_t)T * 2));
   TRY(ensure_words(a.comp_order, T));
   TRY(ensure_words(a.ctrl3, frame_layout(max_g, cap).words));
-  if (bucket_mode(a, T, max_g)) TRY(size_bucket(a, max_g, T));
+  if (bucket_T != 0 && bucket_mode(a, bucket_T, bucket_g)) TRY(size_bucket(a, bucket_g, bucket_T));
   return CGS_OK;
 }
 
@@ -3301,15 +3304,19 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
   for (int attempt = 0;; ++attempt) {
     // ---- size every lane for its largest view ----
     for (uint32_t k = 0; k < L; ++k) {
-      uint32_t mg = 1, T = 1;
+      uint32_t mg = 1, T = 1, bg = 0, bT = 0;
       bool m64 = false;
       for (uint32_t v = 0; v < n_views; ++v) {
         if (b->view_lane[v] != k) continue;
         mg = std::max(mg, SF[view_scene[v]].max_g);
         T = std::max(T, G[v].T);
         m64 = m64 || G[v].T > 65535u;
+        if (G[v].T <= CGS_BUCKET_MAX_TILES && SF[view_scene[v]].N != 0) {
+          bg = std::max(bg, SF[view_scene[v]].max_g);
+          bT = std::max(bT, G[v].T);
+        }
       }
-      TRY(size_arena(b->lanes[k], mg, T, m64, !b->shared_projection));
+      TRY(size_arena(b->lanes[k], mg, T, m64, !b->shared_projection, bg, bT));
     }
     // ---- the graph's fingerprint: every pointer and value it bakes in ----
     uint64_t h = FNV_OFFSET;

@@ -6,7 +6,9 @@ sorted ids and tile ranges bit-exact, RGBA8 within 1/255, for 8 x 8, 16 x 32
 and 32 x 16 tiles with the LSD radix sort and with the tile-bucket sort (the
 two give the identical frame); 8 x 8 tiles at 3840 x 2160 (129600 tiles) take
 the 64-bit keys; the autotuner over every size renders, for the size it keeps,
-the oracle's frame at that size; a batch at 8 x 8 matches per-view frames.
+the oracle's frame at that size; a batch in bucket mode at 8 x 8 and 16 x 16
+whose lanes also hold a view beyond the bucket sort's 32768 tiles matches
+per-view frames.
 """
 import numpy as np
 import pytest
@@ -144,10 +146,13 @@ def test_tile_size_change_drops_the_kept_stage3():
         assert np.array_equal(img, imgs[tile]), tile
 
 
-def test_batch_at_8x8_matches_per_view_frames():
+@pytest.mark.parametrize("tile,big", [((8, 8), (3840, 2160)), ((16, 16), (7680, 4320))])
+def test_batch_in_bucket_mode_matches_per_view_frames(tile, big):
+    """One lane holds views of at most 32768 tiles (bucket sort) and one of
+    more (radix sort, 64-bit keys): the captured graph's buffers cover both."""
     scenes = [syn.scene_gaussian_ball(30_000, k=77), syn.scene_clusters(50_000, k=78)]
     states = []
-    tune = _tuning((8, 8), 1)
+    tune = _tuning(tile, 1)
     for sc in scenes:
         st = GpuPipelineState(0)
         st.sync_scene_and_cages(sc, None)
@@ -155,18 +160,23 @@ def test_batch_at_8x8_matches_per_view_frames():
         states.append(st)
     view_scene, cams, comps = syn.batch_views(5, 2, seed=17,
                                               resolutions=((320, 240), (800, 600), (1001, 557)))
+    bcam = syn.camera_look_at((0.0, 0.0, -3.0), (0, 0, 0), (0, 1, 0), 60.0, *big)
+    view_scene, cams, comps = (list(view_scene) + [0], list(cams) + [bcam],
+                               list(comps) + [syn.compositor(*big, (0.0, 0.0, -3.0))])
     p = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 0, 0)
-    with ViewBatch(0, lanes=2) as batch:
-        batch.set_tuning(tune)
-        stats = batch.execute(states, view_scene, cams, comps, p)
-        print(f"[case] batch at 8x8: {stats.as_dict()}", flush=True)
-        for v in range(len(cams)):
-            st = states[view_scene[v]]
-            t = StageTimings()
-            st.execute_frame_resident(cams[v], comps[v], p, t)
-            W, H = comps[v].screen_width, comps[v].screen_height
-            ref = st.readback_framebuffer_rgba8(W, H)
-            assert np.array_equal(batch.readback_rgba8(v), ref), f"view {v}"
-            assert batch.view_counts(v)[0] == int(t.total_tile_entries), f"view {v}"
+    for lanes in (1, 2):
+        with ViewBatch(0, lanes=lanes) as batch:
+            batch.set_tuning(tune)
+            stats = batch.execute(states, view_scene, cams, comps, p)
+            print(f"[case] batch at {tile[0]}x{tile[1]} with {big[0]}x{big[1]}, {lanes} lanes: "
+                  f"{stats.as_dict()}", flush=True)
+            for v in range(len(cams)):
+                st = states[view_scene[v]]
+                t = StageTimings()
+                st.execute_frame_resident(cams[v], comps[v], p, t)
+                W, H = comps[v].screen_width, comps[v].screen_height
+                ref = st.readback_framebuffer_rgba8(W, H)
+                assert np.array_equal(batch.readback_rgba8(v), ref), f"view {v}"
+                assert batch.view_counts(v)[0] == int(t.total_tile_entries), f"view {v}"
     for st in states:
         st.close()
