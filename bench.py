@@ -480,8 +480,12 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
     check(lib.cgs_mlp_bind(st._h, net._h, cv, nv, ct, nt, d_pts.ptr, n_e, 1e30, d_wn, d_gn, d_ms,
                            C.byref(fb_none)))
     wnf = host(d_wn, n_e)
-    rng = np.random.Generator(np.random.PCG64(99))
-    dv = (cage.vertices + rng.uniform(-0.1, 0.1, cage.vertices.shape)).astype(np.float32)
+    # the evaluation cage: vertex v moved by -0.1 + 0.2 u (3 uniforms of stream
+    # (0xC1DE5EED00000004, D_PERTURB, v))
+    from cindergs_amd import rng as R
+
+    u = R.Streams(0xC1DE5EED00000004, R.D_PERTURB, cage.num_vertices).uniform(3)
+    dv = (cage.vertices + (-0.1 + 0.2 * u)).astype(np.float32)
     pe = we @ dv
     Fe = np.matmul(ge.transpose(0, 2, 1), dv)
     diag = cage.diagonal()
@@ -550,12 +554,17 @@ def mvc_scene(n_points: int, reps: int, cpu_points: int):
     ex = MvcGpu(st, cage)
     nv = cage.num_vertices
     pts = sample_inside(cage, n_points, seed=31, inside_fn=lambda p: _winding(ex, st, p))
-    rng = np.random.Generator(np.random.PCG64(77))
-    sc = rng.uniform(0.005, 0.03, (n_points, 3))
+    # point i's attributes: 52 uniforms of stream (0xC1DE5EED00000000 + 77,
+    # D_SPLAT, i): axis scales 0.005 + 0.025 u[0:3], opacity 0.05 + 0.94 u[3],
+    # SH 0.4 (2 u[4:52] - 1)
+    from cindergs_amd import rng as R
+
+    u = R.uniform(0xC1DE5EED00000000 + 77, R.D_SPLAT, n_points, 52)
+    sc = 0.005 + 0.025 * u[:, 0:3]
     cov = np.zeros((n_points, 6), np.float32)
     cov[:, 0], cov[:, 3], cov[:, 5] = sc[:, 0] ** 2, sc[:, 1] ** 2, sc[:, 2] ** 2
-    opa = rng.uniform(0.05, 0.99, n_points).astype(np.float32)
-    sh = (rng.uniform(-1, 1, (n_points, 48)) * 0.4).astype(np.float32)
+    opa = (0.05 + 0.94 * u[:, 3]).astype(np.float32)
+    sh = (0.4 * (2.0 * u[:, 4:52] - 1.0)).astype(np.float32)
     V = cage.vertices.astype(np.float64)
     Vd = np.concatenate([V[:, :1] * (1.0 + 0.8 * (V[:, 2:3] + 1.2) / 2.4), V[:, 1:2],
                          V[:, 2:3] * 0.9], 1).astype(np.float32)
@@ -806,6 +815,55 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
     return res
 
 
+def sort_baseline(st, E: int, T: int, in_frame_ms: float, reps: int = 20) -> dict | None:
+    """The vendor library's radix sort beside the library's own on a frame's
+    Stage 3 sort: hipCUB's DeviceRadixSort::SortPairs (rocPRIM) on E random
+    (u32 key, u32 value) pairs over the frame's key bits (tile bits + depth
+    bits), timed with HIP events in libcgs_sort_baseline.so, against the
+    library's in-frame sort (its k_radix_onesweep passes and k_hist_reduce;
+    the digit histograms are counted by k_emit, outside that time) and its
+    standalone cgs_radix_sort_u32_pairs on the same number of random pairs
+    (histogram pass included; host-timed with a synchronisation per sort, so
+    an upper bound)."""
+    import ctypes
+
+    path = os.path.join(ROOT, "cindergs_amd", "libcgs_sort_baseline.so")
+    if not os.path.exists(path) or E <= 0:
+        return None
+    lib = ctypes.CDLL(path)
+    lib.cgs_sortbase_rocprim.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    lib.cgs_sortbase_rocprim.restype = ctypes.c_int
+    tile_bits = max(1, int(T - 1).bit_length())
+    bits = min(32, tile_bits + min(32 - tile_bits, 28))
+    ms = ctypes.c_double()
+    if lib.cgs_sortbase_rocprim(E, bits, reps, 7, ctypes.byref(ms)) != 0:
+        return None
+    rng = np.random.default_rng(7)
+    keys = (rng.integers(0, 1 << 32, size=E, dtype=np.uint64) & ((1 << bits) - 1)).astype(np.uint32)
+    vals = np.arange(E, dtype=np.uint32)
+    times = []
+    for r in range(3 + reps):
+        dk, dv = st.to_device(keys), st.to_device(vals)
+        st.synchronize()
+        t0 = time.perf_counter()
+        st.radix_sort_u32_pairs(dk, dv, E, bits)
+        st.synchronize()
+        if r >= 3:
+            times.append(1000.0 * (time.perf_counter() - t0))
+        dk.free()
+        dv.free()
+    lib_ms = float(np.mean(times))
+    out = {"pairs": E, "key_bits": bits, "rocprim_sortpairs_ms": round(ms.value, 4),
+           "library_in_frame_ms": round(in_frame_ms, 4),
+           "library_standalone_ms_upper_bound": round(lib_ms, 4),
+           "rocprim_over_library_in_frame": round(ms.value / in_frame_ms, 3)}
+    print(f"# sort baseline: rocPRIM SortPairs {ms.value:.4f} ms vs library in-frame "
+          f"{in_frame_ms:.4f} ms, standalone <= {lib_ms:.4f} ms ({E} pairs, {bits} bits)",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1013,6 +1071,8 @@ def main() -> int:
                      "bytes_per_launch": int(hbm_kernels[dom]),
                      "avg_launch_ms": round(per_launch_ms, 5)},
     }
+    result["sort_baseline"] = sort_baseline(
+        st, E, T, kern.get("k_radix_onesweep", 0.0) + kern.get("k_hist_reduce", 0.0))
     st.close()
     if not args.no_traffic:
         tstr = ",".join(str(v) for v in tuned["tuning"].values()) if tuned else ""

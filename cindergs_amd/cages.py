@@ -20,6 +20,7 @@ import math
 
 import numpy as np
 
+from . import rng
 from .pipeline import TetCage
 
 UNBOUND = 0xFFFFFFFF
@@ -98,11 +99,13 @@ class TetMesh:
         gz = np.linspace(lo[2], hi[2], nz + 1)
         V = np.stack(np.meshgrid(gx, gy, gz, indexing="ij"), -1).reshape(-1, 3)
         h = (hi - lo) / np.array(cells)
-        rng = np.random.Generator(np.random.PCG64(0xC1DE5EED00000000 + seed))
+        # vertex v draws 3 uniforms from stream (0xC1DE5EED00000000 + seed,
+        # D_GRID, v); interior vertices move by (-jitter + 2 jitter u) h
         ijk = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
                                    indexing="ij"), -1).reshape(-1, 3)
         interior = np.all((ijk > 0) & (ijk < np.array(cells)), axis=1)
-        V[interior] += rng.uniform(-jitter, jitter, size=(interior.sum(), 3)) * h
+        u = rng.Streams(0xC1DE5EED00000000 + seed, rng.D_GRID, V.shape[0]).uniform(3)
+        V[interior] += (-jitter + 2.0 * jitter * u[interior]) * h
 
         def vid(i, j, k):
             return (i * (ny + 1) + j) * (nz + 1) + k
@@ -412,10 +415,12 @@ class CageSpringSimulator:
     def impulse(self, magnitude: float = 0.2, fraction: float = 0.05, seed: int = 3) -> None:
         """The cage workload's excitation (no reference counterpart): an outward
         radial velocity pulse of `magnitude` on a seeded `fraction` of the
-        vertices, directions as in trigger_jiggle_impulse."""
-        rng = np.random.Generator(np.random.PCG64(0xC1DE5EED00000000 + seed + 100))
+        vertices, directions as in trigger_jiggle_impulse.  Vertex v draws one
+        uniform from stream (0xC1DE5EED00000000 + seed + 100, D_IMPULSE, v);
+        the vertices with the smallest draws are picked."""
         n = self.rest_positions.shape[0]
-        pick = rng.choice(n, size=max(1, int(round(fraction * n))), replace=False)
+        u = rng.Streams(0xC1DE5EED00000000 + seed + 100, rng.D_IMPULSE, n).uniform(1)[:, 0]
+        pick = np.sort(np.argsort(u, kind="stable")[:max(1, int(round(fraction * n)))])
         centroid = self.rest_positions.mean(0)
         d = self.rest_positions[pick] - centroid
         ln = np.linalg.norm(d, axis=1, keepdims=True)

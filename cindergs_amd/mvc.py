@@ -18,6 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import rng as R
 from .cages import TetMesh
 from .pipeline import DeviceBuffer, GpuPipelineState, _ptr, check
 
@@ -130,8 +131,10 @@ def icosphere(subdivisions: int = 3, radius: float = 1.2, noise: float = 0.15,
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = nf
     V = np.array(v, dtype=np.float64)
-    rng = np.random.Generator(np.random.PCG64(SEED_BASE + seed))
-    r = radius + rng.uniform(-noise, noise, size=V.shape[0])
+    # vertex v's radius: radius - noise + 2 noise u, u from stream (SEED_BASE +
+    # seed, D_ICOSPHERE, v)
+    u = R.Streams(SEED_BASE + seed, R.D_ICOSPHERE, V.shape[0]).uniform(1)[:, 0]
+    r = radius - noise + 2.0 * noise * u
     return PolyCage((V * r[:, None]).astype(np.float32), np.array(faces, dtype=np.uint32))
 
 
@@ -235,13 +238,16 @@ class MvcGpu:
 def sample_inside(cage: PolyCage, n: int, seed: int = 4, inside_fn=None,
                   batch: int = 1 << 20) -> np.ndarray:
     """n points uniform inside the cage: rejection from the bounding box by
-    |winding number| >= 0.5 (inside_fn(points) -> winding numbers)."""
-    rng = np.random.Generator(np.random.PCG64(SEED_BASE + seed + 17))
-    lo, hi = cage.vertices.min(0), cage.vertices.max(0)
+    |winding number| >= 0.5 (inside_fn(points) -> winding numbers).  Candidate
+    c is lo + (hi - lo) u (cast to f32), u the 3 uniforms of stream
+    (SEED_BASE + seed + 17, D_SAMPLE, c); candidates are taken in order."""
+    lo, hi = cage.vertices.min(0).astype(np.float64), cage.vertices.max(0).astype(np.float64)
     out = []
     have = 0
+    first = 0
     while have < n:
-        cand = rng.uniform(lo, hi, size=(batch, 3)).astype(np.float32)
+        cand = R.uniform(SEED_BASE + seed + 17, R.D_SAMPLE, batch, 3, lo, hi, first).astype(np.float32)
+        first += batch
         keep = cand[np.abs(inside_fn(cand)) >= 0.5]
         out.append(keep)
         have += keep.shape[0]

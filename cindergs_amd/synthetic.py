@@ -2,7 +2,9 @@
 
 The reference has no generator for these distributions; the specification
 here is ours:
-  * RNG: numpy PCG64 seeded with 0xC1DE5EED0000000k for workload k.
+  * RNG: the streams of cindergs_amd.rng (SplitMix64-seeded xoshiro256**,
+    one stream per (seed, domain, index), Box-Muller normals in float64) with
+    seed 0xC1DE5EED0000000k for workload k.
   * Splats: log-scales ~ N(-4.5, 0.5); unit quaternions (PLY WXYZ order,
     src/math/covariance.rs:36-51) from a normalised N(0, I4); opacity =
     sigmoid(N(0, 1.5)); SH degree 3 (48 f32, src/render/rasterizer/sh.rs:56-61)
@@ -13,6 +15,7 @@ here is ours:
     orientation) plus 10 % uniform background in [-4,4]^3 for the 1.07M scene.
   * Cameras: glam look_at_lh (camera.rs:71-96), square pixels,
     f = H / (2 tan(fov/2)), principal point at the image centre, near 0.2.
+The draw order of every stream is stated in each generator's docstring.
 """
 from __future__ import annotations
 
@@ -20,15 +23,13 @@ import math
 
 import numpy as np
 
+from . import rng as R
 from .pipeline import (CameraUniforms, CompositorUniforms, GaussianSceneSoa,
                        QuantizedGaussianSceneSoa)
 
 SEED_BASE = 0xC1DE5EED00000000
 CONFIG2_SPLATS = 1_073_930
-
-
-def _rng(k: int) -> np.random.Generator:
-    return np.random.Generator(np.random.PCG64(SEED_BASE + k))
+ATTR_NORMALS = 56  # normals per splat for its attributes (see _attributes)
 
 
 def quat_to_mat(q: np.ndarray) -> np.ndarray:
@@ -47,47 +48,94 @@ def quat_to_mat(q: np.ndarray) -> np.ndarray:
     return R
 
 
-def _attributes(rng: np.random.Generator, n: int, means: np.ndarray) -> GaussianSceneSoa:
-    log_s = rng.normal(-4.5, 0.5, size=(n, 3))
-    s = np.maximum(np.exp(log_s), 1e-4)
-    q = rng.normal(0.0, 1.0, size=(n, 4))
-    q /= np.maximum(np.linalg.norm(q, axis=1, keepdims=True), 1e-12)
-    R = quat_to_mat(q)
-    M = R * s[:, None, :]
+def _attributes(z: np.ndarray, means: np.ndarray) -> GaussianSceneSoa:
+    """A splat's attributes from its 56 standard normals z: log-scales
+    -4.5 + 0.5 z[0:3], quaternion (WXYZ) normalise(z[3:7]), opacity
+    sigmoid(1.5 z[7]), SH DC 0.5 z[8:11], SH rest 0.05 z[11:56]."""
+    n = z.shape[0]
+    s = np.maximum(np.exp(-4.5 + 0.5 * z[:, 0:3]), 1e-4)
+    q = z[:, 3:7] / np.maximum(np.linalg.norm(z[:, 3:7], axis=1, keepdims=True), 1e-12)
+    Rm = quat_to_mat(q)
+    M = Rm * s[:, None, :]
     S = M @ np.transpose(M, (0, 2, 1))
     cov = np.stack([S[:, 0, 0], S[:, 0, 1], S[:, 0, 2], S[:, 1, 1], S[:, 1, 2], S[:, 2, 2]], 1)
-    opa = 1.0 / (1.0 + np.exp(-rng.normal(0.0, 1.5, size=n)))
+    opa = 1.0 / (1.0 + np.exp(-1.5 * z[:, 7]))
     sh = np.empty((n, 48))
-    sh[:, :3] = rng.normal(0.0, 0.5, size=(n, 3))
-    sh[:, 3:] = rng.normal(0.0, 0.05, size=(n, 45))
-    return GaussianSceneSoa(means.astype(np.float32), cov.astype(np.float32),
-                            opa.astype(np.float32), sh.astype(np.float32))
+    sh[:, :3] = 0.5 * z[:, 8:11]
+    sh[:, 3:] = 0.05 * z[:, 11:56]
+    return GaussianSceneSoa(np.ascontiguousarray(means.astype(np.float32)),
+                            np.ascontiguousarray(cov.astype(np.float32)),
+                            np.ascontiguousarray(opa.astype(np.float32)),
+                            np.ascontiguousarray(sh.astype(np.float32)))
+
+
+def _concat(parts) -> GaussianSceneSoa:
+    return GaussianSceneSoa(*(np.ascontiguousarray(np.concatenate([getattr(p, f) for p in parts]))
+                              for f in ("positions", "covariances_3d", "opacities", "sh_coeffs")))
+
+
+def _splat_block(seed: int, first: int, n: int, fn) -> GaussianSceneSoa:
+    """fn(streams) -> (means, z56) on splats first..first+n-1, in chunks."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+
+    chunk = 1 << 16
+    starts = list(range(0, n, chunk))
+
+    def one(c0):
+        m, z = fn(R.Streams(seed, R.D_SPLAT, min(chunk, n - c0), first + c0))
+        return _attributes(z, m)
+
+    if len(starts) <= 1:
+        return one(0) if n else _attributes(np.zeros((0, 56)), np.zeros((0, 3)))
+    with ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1, len(starts)))) as ex:
+        return _concat(list(ex.map(one, starts)))
 
 
 def scene_gaussian_ball(n: int = 100_000, k: int = 1) -> GaussianSceneSoa:
-    """100k-splat scene: means ~ N(0, 0.5^2 I3)."""
-    rng = _rng(k)
-    means = rng.normal(0.0, 0.5, size=(n, 3))
-    return _attributes(rng, n, means)
+    """100k-splat scene: means ~ N(0, 0.5^2 I3).  Splat i draws 59 normals
+    from stream (seed, D_SPLAT, i): mean 0.5 z[0:3], then its attributes
+    from z[3:59]."""
+    def fn(st):
+        z = st.normal(59)
+        return 0.5 * z[:, 0:3], z[:, 3:59]
+
+    return _splat_block(SEED_BASE + k, 0, n, fn)
 
 
 def scene_clusters(n: int = CONFIG2_SPLATS, k: int = 2, clusters: int = 32,
                    background: float = 0.10) -> GaussianSceneSoa:
-    """Cluster-mixture scene (1,073,930 splats by default)."""
-    rng = _rng(k)
-    n_bg = int(round(n * background))
-    n_cl = n - n_bg
-    centres = rng.uniform(-3.0, 3.0, size=(clusters, 3))
-    radii = rng.uniform(0.05, 0.6, size=(clusters, 3))
-    qc = rng.normal(0.0, 1.0, size=(clusters, 4))
+    """Cluster-mixture scene (1,073,930 splats by default).  Cluster c draws
+    from stream (seed, D_SCENE_GLOBAL, c) 6 uniforms (centre -3 + 6 u[0:3],
+    axis radii 0.05 + 0.55 u[3:6]) and then 4 normals (orientation
+    quaternion).  The first n - round(n background) splats are cluster
+    members: splat i draws from (seed, D_SPLAT, i) one uniform (cluster
+    floor(32 u)) and 59 normals (local offset z[0:3] times the radii,
+    rotated, plus the centre; attributes from z[3:59]).  The rest are
+    background: 3 uniforms (position -4 + 8 u) and 56 normals (attributes)."""
+    seed = SEED_BASE + k
+    gs = R.Streams(seed, R.D_SCENE_GLOBAL, clusters)
+    u = gs.uniform(6)
+    centres = -3.0 + 6.0 * u[:, 0:3]
+    radii = 0.05 + 0.55 * u[:, 3:6]
+    qc = gs.normal(4)
     qc /= np.linalg.norm(qc, axis=1, keepdims=True)
     Rc = quat_to_mat(qc)
-    which = rng.integers(0, clusters, size=n_cl)
-    local = rng.normal(0.0, 1.0, size=(n_cl, 3)) * radii[which]
-    pts = np.einsum("nij,nj->ni", Rc[which], local) + centres[which]
-    bg = rng.uniform(-4.0, 4.0, size=(n_bg, 3))
-    means = np.concatenate([pts, bg], 0)
-    return _attributes(rng, n, means)
+    n_bg = int(round(n * background))
+    n_cl = n - n_bg
+
+    def member(st):
+        which = np.minimum((st.uniform(1)[:, 0] * clusters).astype(np.int64), clusters - 1)
+        z = st.normal(59)
+        local = z[:, 0:3] * radii[which]
+        pts = np.einsum("nij,nj->ni", Rc[which], local) + centres[which]
+        return pts, z[:, 3:59]
+
+    def back(st):
+        pos = -4.0 + 8.0 * st.uniform(3)
+        return pos, st.normal(56)
+
+    return _concat([_splat_block(seed, 0, n_cl, member), _splat_block(seed, n_cl, n_bg, back)])
 
 
 def look_at_lh(eye, target, up) -> np.ndarray:
@@ -196,14 +244,17 @@ def batch_views(n_views: int = 64, n_scenes: int = 3, seed: int = 5,
     """Seeded orbit views for batched rendering: view v looks at the origin of
     scene v % n_scenes from azimuth ~ U(0, 2 pi), elevation ~ U(-0.4, 0.4) rad
     and distance ~ U(2.5, 5), with a resolution drawn uniformly from
-    ``resolutions`` (fov 60 deg).  Returns (view_scene, cameras, compositors)."""
-    rng = _rng(seed)
+    ``resolutions`` (fov 60 deg): view v draws 4 uniforms from stream
+    (0xC1DE5EED00000000 + seed, D_VIEW, v): resolution floor(len u0), azimuth
+    2 pi u1, elevation -0.4 + 0.8 u2, distance 2.5 + 2.5 u3.  Returns
+    (view_scene, cameras, compositors)."""
+    U = R.Streams(SEED_BASE + seed, R.D_VIEW, n_views).uniform(4)
     view_scene, cams, comps = [], [], []
     for v in range(n_views):
-        W, H = resolutions[int(rng.integers(0, len(resolutions)))]
-        az = rng.uniform(0.0, 2.0 * math.pi)
-        el = rng.uniform(-0.4, 0.4)
-        r = rng.uniform(2.5, 5.0)
+        W, H = resolutions[min(int(U[v, 0] * len(resolutions)), len(resolutions) - 1)]
+        az = 2.0 * math.pi * U[v, 1]
+        el = -0.4 + 0.8 * U[v, 2]
+        r = 2.5 + 2.5 * U[v, 3]
         eye = (r * math.cos(el) * math.sin(az), r * math.sin(el), -r * math.cos(el) * math.cos(az))
         view_scene.append(v % n_scenes)
         cams.append(camera_look_at(eye, (0, 0, 0), (0, 1, 0), 60.0, W, H))
