@@ -232,7 +232,7 @@ struct CgsBinArgs {
   uint32_t depth_bits;       // 32-bit mode
   uint32_t cap;              // capacity of the entry arrays
   uint32_t* keys;            // 32-bit mode: packed keys; 64-bit mode: tile ids
-  uint32_t* depths;          // 64-bit mode only
+  uint32_t* depths;          // 64-bit mode only (null: not written)
   uint32_t* ids;
   uint32_t* counts;          // [max] per-splat tile counts
   uint32_t* offsets_out;     // optional [max] (exclusive scan)
@@ -262,7 +262,26 @@ struct CgsBinArgs {
   uint32_t tiles_pad;         // multiple of 128
   uint32_t subs;              // sub-blocks of 512 splats per block
   uint32_t parts;             // sub-blocks in all
+  // depth-ordered emission (presort): position j of the binning takes splat
+  // order[j] (a stable sort of the splats by their key's depth bits), so the
+  // entries come out ordered by (depth, splat id) and only the tile bits are
+  // left to sort; counts, offsets and hit masks are per position, the
+  // entries' ids are splat ids.  The histogram covers the tile digits only
+  // (8-bit digits of the tile id at hist_partial[p * 256]); 64-bit mode
+  // writes no depths.  Null: positions are splat indices.
+  const uint32_t* order;
 };
+
+// presort keys: keys[s] = the depth bits of splat s's sort key (32-bit mode:
+// the packed key's depth digit; 64-bit mode: the raw f32 bits of max(tz, 0)),
+// 0 for splats that emit nothing (s >= active or extents 0), s < max_g; also
+// clears zero[0 .. zero_len) (the presort's histogram, look-back status and
+// tickets) and writes per-block digit histograms of the sort's passes (8-bit
+// digits over key_bits bits) as rows of 1024 words of `partial`.  Returns the
+// rows written (<= max_rows).
+uint32_t cgs_launch_splat_depth_keys(const CgsBinArgs& a, uint32_t* keys, uint32_t* zero,
+                                     uint32_t zero_len, uint32_t key_bits, uint32_t* partial,
+                                     uint32_t max_rows, hipStream_t s);
 
 // ---- Stage 3 in bucket mode (k_tilesort.hip) ----
 #define CGS_SEG_WINDOW 2048u  // entries per sort window; larger tiles are sorted in chunks
@@ -378,11 +397,13 @@ void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row
 // keys), 8192 (two 512-thread blocks per CU) or 4096 (256 threads, four
 // blocks per CU);
 // digit_bits: 8 (4 passes over 32 bits) or 4 (8 passes).  Every choice gives
-// the same stable order.
+// the same stable order.  begin_bit: the key bits [begin_bit, bits) are
+// sorted (the entries arrive ordered by the bits below it); the histogram of
+// pass p is that of the digit at begin_bit + p * digit_bits.
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
                           CgsProf* prof = nullptr, uint32_t part_keys = 16384,
-                          uint32_t digit_bits = 8);
+                          uint32_t digit_bits = 8, uint32_t begin_bit = 0);
 // a_out[i] = a_in[perm[i]], b_out likewise; perm, a_out, b_out 16-byte
 // aligned (false, nothing launched, otherwise)
 bool cgs_launch_gather2(const uint32_t* perm, const uint32_t* a_in, const uint32_t* b_in,

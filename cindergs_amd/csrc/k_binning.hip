@@ -88,6 +88,11 @@ __device__ __forceinline__ uint32_t splat_key_bits(const CgsBinArgs& A, float tz
   return cgs_f2u(cgs_fmax(tz, 0.0f));
 }
 
+// the splat at binning position s (identity unless the splats were presorted)
+__device__ __forceinline__ uint32_t splat_at(const CgsBinArgs& A, uint32_t s) {
+  return (A.order && s < A.max_gaussians) ? A.order[s] : s;
+}
+
 // One splat as the binning sees it (thread-owned)
 struct BinSplat {
   float4 d0, d1;  // (mu_x, mu_y, a, b), (c, alpha, q_cut, key bits)
@@ -257,8 +262,9 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
   }
   const uint32_t sb0 = blockIdx.x * A.subs, sb1 = min(sb0 + A.subs, A.parts);
   for (uint32_t sb = sb0; sb < sb1; ++sb) {
-    const uint32_t s = sb * (uint32_t)BP + t;
-    const BinSplat b = load_bin_splat(A, s, active);
+    const uint32_t s = sb * (uint32_t)BP + t;  // position (counts, masks)
+    const uint32_t sid = splat_at(A, s);       // splat
+    const BinSplat b = load_bin_splat(A, sid, active);
     // small boxes: the thread walks them; bit k of the splat's mask = box tile
     // k (row-major) is hit
     uint32_t cnt = 0;
@@ -276,7 +282,7 @@ __global__ __launch_bounds__(BT) void k_count(CgsBinArgs A) {
     // big boxes: the wave walks them together; the hit masks of its first
     // KCHUNK chunks are kept for k_emit
     uint32_t nv, slot;
-    const uint32_t total = stage_big(W, b, s, 0u, lane, nv, slot);
+    const uint32_t total = stage_big(W, b, sid, 0u, lane, nv, slot);
     uint64_t* masks =
         A.chunk_masks ? A.chunk_masks + ((size_t)sb * (BT / 64) + wave) * KCHUNK : nullptr;
     uint32_t cp = 0;
@@ -370,7 +376,7 @@ __device__ __forceinline__ void store_entry(const CgsBinArgs& A, uint32_t pos, u
     A.keys[pos] = (tile << A.depth_bits) | kb;
   } else {
     A.keys[pos] = tile;
-    A.depths[pos] = kb;
+    if (A.depths) A.depths[pos] = kb;
   }
   A.ids[pos] = id;
 }
@@ -428,7 +434,11 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     if (A.offsets_out && s < A.max_gaussians) A.offsets_out[s] = off;
     // staged exactly as k_count staged it (its walk's chunks, whose hit masks
     // are reused here, depend on every big box, hit or not)
-    const BinSplat b = load_bin_splat(A, s, active);
+    const uint32_t sid = splat_at(A, s);
+    const BinSplat b = load_bin_splat(A, sid, active);
+    // presort: the histogram of the tile digits only
+    const bool tile_hist = hist && (A.mode64 || A.order);
+    uint32_t* th = A.order ? s_hist : s_hist + 1024;
     // small boxes: the thread walks them (the tiles k_count found hit, when it
     // kept their mask)
     if (b.area != 0u && b.area <= SMALL_BOX) {
@@ -445,20 +455,18 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
                             : hits(b.d0, b.d1, tx, ty, A.width, A.height, A.tile_w, A.tile_h)) {
             const uint32_t tile = ty * A.tiles_x + tx;
             if (HW > 0) {
-              bucket_put(A, s_hist, tile, kb, s);
+              bucket_put(A, s_hist, tile, kb, sid);
             } else if (cur < A.cap) {
-              store_entry(A, cur, tile, kb, s);
-              if (hist) {
-                if (!A.mode64)
-                  hist_key_from(s_hist, (tile << A.depth_bits) | kb, hbits, db, cpass);
-                else
-                  hist_key(s_hist + 1024, tile, A.tile_bits, 8u);
-              }
+              store_entry(A, cur, tile, kb, sid);
+              if (tile_hist)
+                hist_key(th, tile, A.tile_bits, 8u);
+              else if (hist)
+                hist_key_from(s_hist, (tile << A.depth_bits) | kb, hbits, db, cpass);
             }
             ++cur;
           }
       const uint32_t stored = min(cur, A.cap) - min(off, A.cap);
-      if (hist && stored != 0u) {
+      if (hist && stored != 0u && !A.order) {
         if (!A.mode64)
           hist_const(s_hist, kb, hbits, db, cpass, stored);
         else
@@ -468,7 +476,7 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     // big boxes: the wave walks them together; a hit goes to its splat's next
     // slot plus the hits of the same splat on the lanes before it
     uint32_t nv, slot;
-    const uint32_t total = stage_big(W, b, s, off, lane, nv, slot);
+    const uint32_t total = stage_big(W, b, sid, off, lane, nv, slot);
     const uint64_t* masks =
         A.chunk_masks ? A.chunk_masks + ((size_t)sb * (BT / 64) + wave) * KCHUNK : nullptr;
     uint32_t cp = 0;
@@ -498,12 +506,12 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
         const uint64_t Hs = __ballot(store);
         if (store) store_entry(A, pos, tile, kb, id);
         if (hist) {
-          if (!A.mode64) {
+          if (!A.mode64 && !A.order) {
             hist_walk(s_hist, (tile << A.depth_bits) | kb, hbits, A.depth_bits, db, store, Hs,
                       head, run, lane);
           } else {
-            hist_walk(s_hist, kb, 32u, 32u, 8u, store, Hs, head, run, lane);
-            hist_walk(s_hist + 1024, tile, A.tile_bits, 0u, 8u, store, Hs, head, run, lane);
+            if (!A.order) hist_walk(s_hist, kb, 32u, 32u, 8u, store, Hs, head, run, lane);
+            hist_walk(th, tile, A.tile_bits, 0u, 8u, store, Hs, head, run, lane);
           }
         }
       }
@@ -515,6 +523,31 @@ __global__ __launch_bounds__(BT) void k_emit(CgsBinArgs A) {
     for (uint32_t i = t; i < A.hist_words; i += BT)
       A.hist_partial[(size_t)blk * A.hist_words + i] = s_hist[i];
   }
+}
+
+// the depth bits of every splat's sort key, for the presort (0: emits
+// nothing), and the block's digit histograms of the presort's passes (8-bit
+// digits of key_bits bits) as row blockIdx.x of `partial`
+__global__ __launch_bounds__(256) void k_splat_depth_keys(CgsBinArgs A, uint32_t* keys,
+                                                          uint32_t* zero, uint32_t zero_len,
+                                                          uint32_t key_bits, uint32_t* partial) {
+  __shared__ uint32_t s_h[1024];
+  for (uint32_t i = threadIdx.x; i < 1024u; i += 256u) s_h[i] = 0u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < zero_len; i += gridDim.x * 256u)
+    zero[i] = 0u;
+  __syncthreads();
+  const uint32_t active = A.num_rest + (A.split_count ? A.split_count[0] : 0u);
+  const uint32_t passes = (key_bits + 7u) / 8u;
+  for (uint32_t s = blockIdx.x * 256u + threadIdx.x; s < A.max_gaussians;
+       s += gridDim.x * 256u) {
+    uint32_t k = 0u;
+    if (s < active && A.extents[s] != 0u) k = splat_key_bits(A, A.geom[(size_t)s * 8 + 6]);
+    keys[s] = k;
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&s_h[(p << 8) + ((k >> (8u * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 1024u; i += 256u)
+    partial[(size_t)blockIdx.x * 1024u + i] = s_h[i];
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -644,6 +677,18 @@ void cgs_launch_bin(const CgsBinArgs& a, hipStream_t s, CgsProf* prof,
     hipLaunchKernelGGL(k_emit<BIN_HW_SMALL>, dim3(grid), dim3(BT), 0, s, a);
   else
     hipLaunchKernelGGL(k_emit<BIN_HW_LARGE>, dim3(grid), dim3(BT), 0, s, a);
+}
+
+uint32_t cgs_launch_splat_depth_keys(const CgsBinArgs& a, uint32_t* keys, uint32_t* zero,
+                                     uint32_t zero_len, uint32_t key_bits, uint32_t* partial,
+                                     uint32_t max_rows, hipStream_t s) {
+  const uint32_t n = max(a.max_gaussians, zero_len);
+  if (n == 0 || max_rows == 0) return 0;
+  // about 4 splats per thread
+  const uint32_t grid = max(1u, min((n + 1023u) / 1024u, max_rows));
+  hipLaunchKernelGGL(k_splat_depth_keys, dim3(grid), dim3(256), 0, s, a, keys, zero, zero_len,
+                     key_bits, partial);
+  return grid;
 }
 
 void cgs_launch_tile_ranges(const uint32_t* sorted, const uint32_t* d_n, uint32_t cap,

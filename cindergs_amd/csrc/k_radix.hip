@@ -31,11 +31,14 @@ __device__ __forceinline__ uint32_t digit_mask(uint32_t bits, uint32_t shift, ui
 // block (no global atomics: same-address device atomics serialise).  Each
 // thread reads four keys per step (one 16-byte load when the four are in
 // range) and adds every digit of every pass to the block's LDS histogram.
+// The passes sort key bits [begin, bits): the keys are read shifted right by
+// begin, over bits - begin bits.
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const uint32_t* d_n,
                                                     uint32_t n_static, uint32_t cap,
-                                                    uint32_t bits, uint32_t db,
+                                                    uint32_t bits, uint32_t db, uint32_t begin,
                                                     uint32_t* partial) {
   __shared__ uint32_t s_h[MAX_PASSES * RADIX];
+  bits -= begin;
   const uint32_t passes = (bits + db - 1u) / db;
   const uint32_t radix = 1u << db;
   for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += 256u) s_h[i] = 0u;
@@ -46,10 +49,10 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, const 
     uint32_t k[4];
     if (i0 + 3u < n && (reinterpret_cast<uintptr_t>(keys + i0) & 15u) == 0u) {
       const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
-      k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+      k[0] = v.x >> begin; k[1] = v.y >> begin; k[2] = v.z >> begin; k[3] = v.w >> begin;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < n ? keys[i0 + j] : 0u;
+      for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < n ? keys[i0 + j] >> begin : 0u;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -258,12 +261,13 @@ void cgs_launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t row
 
 int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_t n_static,
                           uint32_t bits, bool iota_vals, bool hist_given, hipStream_t s,
-                          CgsProf* prof, uint32_t part_keys, uint32_t digit_bits) {
-  if (bits == 0 || b.cap == 0) return 0;
+                          CgsProf* prof, uint32_t part_keys, uint32_t digit_bits,
+                          uint32_t begin_bit) {
   if (bits > 32) bits = 32;
+  if (bits <= begin_bit || b.cap == 0) return 0;
   const uint32_t db = digit_bits == 4u ? 4u : 8u;
   const uint32_t radix = 1u << db;
-  const uint32_t passes = (bits + db - 1u) / db;
+  const uint32_t passes = (bits - begin_bit + db - 1u) / db;
   const uint32_t parts_alloc = cgs_radix_num_parts(b.cap);  // status rows per pass
   const uint32_t pk = part_keys == 4096u    ? 4096u
                       : part_keys == 8192u  ? 8192u
@@ -274,7 +278,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
   if (!hist_given) {
     if (prof) prof->mark("k_radix_hist", s);
     hipLaunchKernelGGL(k_radix_hist, dim3(HIST_BLOCKS), dim3(256), 0, s, b.keys, d_n, n_static,
-                       b.cap, bits, db, b.hist_partial);
+                       b.cap, bits, db, begin_bit, b.hist_partial);
     if (prof) prof->mark("k_hist_reduce", s);
     cgs_launch_hist_reduce(b.hist_partial, HIST_BLOCKS, MAX_PASSES * RADIX, passes * radix,
                            b.hist, s, false);
@@ -292,7 +296,7 @@ int cgs_launch_radix_sort(const CgsRadixBuffers& b, const uint32_t* d_n, uint32_
     A.n_static = n_static;
     A.cap = b.cap;
     A.bits = bits;
-    A.shift = p * db;
+    A.shift = begin_bit + p * db;
     A.hist = b.hist + p * radix;
     A.status = b.status + (size_t)p * parts_alloc * radix;
     A.ticket = b.tickets + p;

@@ -165,6 +165,11 @@ struct FrameArena {
   DevBuf ranges, out_rgba, cnt_dbg, off_dbg, comp_order;
   DevBuf hitm;   // hit masks from k_count to k_emit
   DevBuf ctrl3;  // frame control, see frame_layout()
+  // presort (tile_sort = 2): the splats' depth keys, their sorted order, and
+  // the control words of that sort; `order` is the order of the last Stage 3
+  // (null: it was not presorted)
+  DevBuf sp_keys, sp_vals, sp_alt_k, sp_alt_v, sp_ctrl;
+  const uint32_t* order = nullptr;
   uint32_t tiles_x = 0, tiles_y = 0, padded_row = 0;
   bool mode64 = false;
   uint32_t depth_bits = 0;
@@ -196,8 +201,10 @@ struct FrameArena {
     DevBuf* bufs[] = {&geom,    &color,  &extents, &e_keys,   &e_ids,     &e_depths,
                       &e_alt_k, &e_alt_v, &e_perm,  &ranges,   &out_rgba,  &cnt_dbg,
                       &off_dbg, &comp_order, &hitm, &ctrl3,    &bk_hist,   &bk_base,
-                      &bk_bucket, &bk_alt, &bk_tab, &bk_ctl};
+                      &bk_bucket, &bk_alt, &bk_tab, &bk_ctl, &sp_keys, &sp_vals,
+                      &sp_alt_k, &sp_alt_v, &sp_ctrl};
     for (DevBuf* b : bufs) release(*b);
+    order = nullptr;
     entry_cap = 0;
     bk_planned = false;
     bk_plans.clear();
@@ -417,7 +424,18 @@ BucketSizes bucket_sizes(uint32_t max_g, uint32_t T, uint32_t cap) {
 }
 
 bool bucket_mode(const FrameArena& a, uint32_t T, uint32_t max_g) {
-  return a.tune.tile_sort && T <= CGS_BUCKET_MAX_TILES && cgs_bin_num_parts(max_g) != 0;
+  return a.tune.tile_sort == 1u && T <= CGS_BUCKET_MAX_TILES && cgs_bin_num_parts(max_g) != 0;
+}
+
+// the presort's buffers for max_g splat slots
+int size_presort(FrameArena& a, uint32_t max_g) {
+  const size_t n = max_g ? max_g : 1u;
+  TRY(ensure_words(a.sp_keys, n));
+  TRY(ensure_words(a.sp_vals, n));
+  TRY(ensure_words(a.sp_alt_k, n));
+  TRY(ensure_words(a.sp_alt_v, n));
+  TRY(ensure_words(a.sp_ctrl, cgs_radix_control_words((uint32_t)n)));
+  return CGS_OK;
 }
 
 int size_bucket(FrameArena& a, uint32_t max_g, uint32_t T) {
@@ -577,6 +595,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   a.tiles_x = g.tiles_x;
   a.tiles_y = g.tiles_y;
   a.mode64 = g.T > 65535u;
+  a.order = nullptr;  // set again below when this frame presorts
   init_entry_cap(a, max_g);
   const uint32_t cap = a.entry_cap;
   const bool bucket = bucket_mode(a, g.T, max_g);
@@ -597,11 +616,13 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   TRY(ensure_words(a.e_ids, cap));
   TRY(ensure_words(a.e_alt_k, cap));
   TRY(ensure_words(a.e_alt_v, cap));
-  if (a.mode64) {
+  const bool presort = a.tune.tile_sort == 2u;
+  if (a.mode64 && !presort) {
     TRY(ensure_words(a.e_depths, cap));
     TRY(ensure_words(a.e_perm, cap));
   }
   TRY(ensure_words(a.ranges, (size_t)g.T * 2));
+  if (presort) TRY(size_presort(a, max_g));
   const FrameLayout L = frame_layout(max_g, cap);
   TRY(ensure_words(a.ctrl3, L.words));
   uint32_t* ctrl = P<uint32_t>(a.ctrl3);
@@ -652,7 +673,7 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   B.depth_bits = depth_bits;
   B.cap = cap;
   B.keys = P<uint32_t>(a.e_keys);
-  B.depths = a.mode64 ? P<uint32_t>(a.e_depths) : nullptr;
+  B.depths = a.mode64 && !presort ? P<uint32_t>(a.e_depths) : nullptr;
   B.ids = P<uint32_t>(a.e_ids);
   B.counts = P<uint32_t>(a.cnt_dbg);
   B.offsets_out = a.capture ? P<uint32_t>(a.off_dbg) : nullptr;
@@ -661,19 +682,55 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
   B.block_sums = ctrl + L.block_sums;
   B.block_surv = ctrl + L.block_surv;
   B.hist_partial = ctrl + L.bin_hist;
-  B.hist_words = a.mode64 ? 2048u : 1024u;
+  const uint32_t tile_passes = (tile_bits + 7u) / 8u;
+  B.hist_words = presort ? 256u * tile_passes : a.mode64 ? 2048u : 1024u;
   uint32_t sort_bits = tile_bits + depth_bits;
   if (sort_bits > 32) sort_bits = 32;
   B.sort_bits = sort_bits;
   B.tile_bits = tile_bits;
   B.subs = 1u;
   B.parts = cgs_bin_num_parts(max_g);
-  const uint32_t db = a.mode64 ? 8u : a.tune.radix_digit_bits;
+  const uint32_t db = a.mode64 || presort ? 8u : a.tune.radix_digit_bits;
   B.digit_bits = db;
+  a.order = nullptr;
+  if (presort && max_g != 0) {
+    // the splats sorted by their keys' depth bits (stable: ties by index)
+    CgsRadixBuffers SR;
+    SR.keys = P<uint32_t>(a.sp_keys);
+    SR.vals = P<uint32_t>(a.sp_vals);
+    SR.keys_alt = P<uint32_t>(a.sp_alt_k);
+    SR.vals_alt = P<uint32_t>(a.sp_alt_v);
+    uint32_t* sc = P<uint32_t>(a.sp_ctrl);
+    const size_t parts = cgs_radix_num_parts(max_g) ? cgs_radix_num_parts(max_g) : 1u;
+    const size_t status_words = 4 * parts * 256;
+    // layout: hist[1024] | status | tickets[4] | partial histograms
+    SR.hist = sc;
+    SR.status = sc + 1024;
+    SR.tickets = sc + 1024 + status_words;
+    SR.hist_partial = sc + 1024 + status_words + 4;
+    SR.cap = max_g;
+    const uint32_t key_bits = a.mode64 ? 32u : depth_bits;
+    prof->mark("k_splat_depth_keys", s);
+    const uint32_t rows = cgs_launch_splat_depth_keys(
+        B, SR.keys, SR.hist, (uint32_t)(1024 + status_words + 4), key_bits, SR.hist_partial,
+        cgs_radix_hist_blocks(), s);
+    // its launches are timed together (the entries' sort keeps its own names)
+    prof->mark("k_presort", s);
+    cgs_launch_hist_reduce(SR.hist_partial, rows, 1024u, 256u * ((key_bits + 7u) / 8u), SR.hist, s,
+                           true);
+    // partitions of 4096 splats up to 4M slots: enough blocks to fill the
+    // chip, where the entries' partitions would leave most of it idle
+    const uint32_t pk = max_g <= (1u << 22) ? 4096u : a.tune.radix_partition_keys;
+    const int alt = cgs_launch_radix_sort(SR, nullptr, max_g, key_bits, true, true, s, nullptr,
+                                          pk, 8u);
+    B.order = alt ? SR.vals_alt : SR.vals;
+    a.order = B.order;
+  }
   cgs_launch_bin(B, s, prof);
   // digit histograms of every pass, summed over the binning partitions
-  const uint32_t hist_bins = a.mode64 ? 1024u + 256u * ((tile_bits + 7u) / 8u)
-                                      : (1u << db) * ((sort_bits + db - 1u) / db);
+  const uint32_t hist_bins = presort     ? 256u * tile_passes
+                             : a.mode64 ? 1024u + 256u * tile_passes
+                                        : (1u << db) * ((sort_bits + db - 1u) / db);
   prof->mark("k_hist_reduce", s);
   cgs_launch_hist_reduce(ctrl + L.bin_hist, L.bin_parts, B.hist_words, hist_bins, ctrl + L.histA,
                          s, true);  // cleared with the frame's control words
@@ -690,13 +747,34 @@ int launch_stage3(FrameArena& a, hipStream_t s, CgsProf* prof, const FrameGeom& 
     R.status = ctrl + L.rA_status;
     R.tickets = ctrl + L.rA_tickets;
     R.cap = cap;
+    // presort: the entries are in (depth, splat) order already; the tile bits
+    // above depth_bits are left
     int alt = cgs_launch_radix_sort(R, d_n, 0, sort_bits, false, true, s, prof,
-                                    a.tune.radix_partition_keys, db);
+                                    a.tune.radix_partition_keys, db, presort ? depth_bits : 0u);
     a.sorted_keys = alt ? R.keys_alt : R.keys;
     a.sorted_ids = alt ? R.vals_alt : R.vals;
     prof->mark("k_tile_ranges", s);
     cgs_launch_tile_ranges(a.sorted_keys, d_n, cap, depth_bits, g.T, P<uint32_t>(a.ranges),
                            s);
+  } else if (presort) {
+    // 64-bit keys of presorted splats: the entries are in (depth, splat) order
+    // already, so one stable sort of (tile id, splat id) by the tile bits
+    CgsRadixBuffers Bt;
+    Bt.keys = P<uint32_t>(a.e_keys);
+    Bt.vals = P<uint32_t>(a.e_ids);
+    Bt.keys_alt = P<uint32_t>(a.e_alt_k);
+    Bt.vals_alt = P<uint32_t>(a.e_alt_v);
+    Bt.hist = ctrl + L.histA;
+    Bt.hist_partial = nullptr;
+    Bt.status = ctrl + L.rA_status;
+    Bt.tickets = ctrl + L.rA_tickets;
+    Bt.cap = cap;
+    int altB = cgs_launch_radix_sort(Bt, d_n, 0, tile_bits, false, true, s, prof,
+                                     a.tune.radix_partition_keys, 8u);
+    a.sorted_keys = altB ? Bt.keys_alt : Bt.keys;
+    a.sorted_ids = altB ? Bt.vals_alt : Bt.vals;
+    prof->mark("k_tile_ranges", s);
+    cgs_launch_tile_ranges(a.sorted_keys, d_n, cap, 0, g.T, P<uint32_t>(a.ranges), s);
   } else {
     // 64-bit fallback (gpu_radix_sort_u64_pairs, radix_sort.rs:377-477): stable
     // sort by depth with the entry index as payload, gather, stable sort by tile
@@ -994,7 +1072,7 @@ int check_tuning(const cgs_tuning* t) {
   if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u)
     return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128 or 256");
   if (t->tile_order > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_order must be 0 or 1");
-  if (t->tile_sort > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_sort must be 0 or 1");
+  if (t->tile_sort > 2u) return fail(CGS_ERR_INVALID_ARG, "tile_sort must be 0, 1 or 2");
   const uint32_t tw = t->tile_w, th = t->tile_h;
   if (!((tw == 16u && th == 16u) || (tw == 8u && th == 8u) || (tw == 16u && th == 32u) ||
         (tw == 32u && th == 16u)))
@@ -1007,6 +1085,7 @@ int check_tuning(const cgs_tuning* t) {
 
 // the arena of an empty scene's frame: no tile has entries
 int background_arena(FrameArena& a, const FrameGeom& g, hipStream_t s) {
+  a.order = nullptr;
   TRY(ensure_words(a.ranges, (size_t)g.T * 2));
   cgs_launch_fill_u32(P<uint32_t>(a.ranges), 0u, (uint64_t)g.T * 2, s);
   TRY(ensure_words(a.geom, 8));
@@ -1418,14 +1497,17 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 // x order); then the other tile sizes with fewer choices each: 8 x 8 (batch
 // 64) radix partitions 16384 / 8192 x order, then bucket x order; 16 x 32 and
 // 32 x 16 radix partitions 16384 / 8192 x batch 256 / 128 x order, then
-// bucket x batch x order
-uint32_t cgs_tuning_candidates(void) { return 96u; }
+// bucket x batch x order.  Last the presort (tile_sort 2): 16 x 16 with
+// partitions 16384 / 10240 / 8192 x batch 256 / 128 x order, 8 x 8 with
+// partitions 16384 / 8192 x order.
+uint32_t cgs_tuning_candidates(void) { return 112u; }
 
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
-  if (!out || i >= 96u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  if (!out || i >= 112u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
   static const uint32_t parts[5] = {16384u, 12288u, 10240u, 8192u, 4096u},
                         digits[2] = {8u, 4u}, batches[3] = {256u, 128u, 64u},
-                        orders[2] = {1u, 0u}, parts2[2] = {16384u, 8192u};
+                        orders[2] = {1u, 0u}, parts2[2] = {16384u, 8192u},
+                        parts3[3] = {16384u, 10240u, 8192u};
   out->radix_partition_keys = 16384u;
   out->radix_digit_bits = 8u;
   out->tile_w = 16u;
@@ -1452,6 +1534,20 @@ int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
     } else {
       out->tile_order = orders[i - 4u];
       out->tile_sort = 1u;
+    }
+  } else if (i >= 96u) {
+    i -= 96u;
+    out->tile_sort = 2u;
+    if (i < 12u) {
+      out->radix_partition_keys = parts3[i % 3u];
+      out->composite_batch = batches[(i / 3u) % 2u];
+      out->tile_order = orders[i / 6u];
+    } else {
+      i -= 12u;
+      out->tile_w = out->tile_h = 8u;
+      out->radix_partition_keys = parts2[i % 2u];
+      out->composite_batch = 64u;
+      out->tile_order = orders[i / 2u];
     }
   } else {
     i -= 72u;
@@ -1636,10 +1732,23 @@ int cgs_read_stage3(cgs_context* c, uint32_t* counts, uint32_t* offsets, uint32_
     return fail(CGS_ERR_INVALID_ARG, "counts/offsets need cgs_set_capture(ctx, 1) before the frame");
   const uint32_t E = c->n_entries;
   const size_t T = (size_t)c->fa.tiles_x * c->fa.tiles_y;
+  // after a presort the binning's arrays are per position: order[j] is the
+  // splat at position j
+  std::vector<uint32_t> order, pos_counts, pos_offsets;
+  const bool permuted = c->fa.order && c->max_g && (counts || offsets);
+  if (permuted) {
+    order.resize(c->max_g);
+    HIPCHK(hipMemcpyAsync(order.data(), c->fa.order, (size_t)c->max_g * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    if (counts) pos_counts.resize(c->max_g);
+    if (offsets) pos_offsets.resize(c->max_g);
+  }
+  uint32_t* cdst = permuted && counts ? pos_counts.data() : counts;
+  uint32_t* odst = permuted && offsets ? pos_offsets.data() : offsets;
   if (counts && c->max_g)
-    HIPCHK(hipMemcpyAsync(counts, c->fa.cnt_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(cdst, c->fa.cnt_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
   if (offsets && c->max_g)
-    HIPCHK(hipMemcpyAsync(offsets, c->fa.off_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(odst, c->fa.off_dbg.p, (size_t)c->max_g * 4, hipMemcpyDeviceToHost, c->stream));
   if (sorted_keys && E)
     HIPCHK(hipMemcpyAsync(sorted_keys, c->fa.sorted_keys, (size_t)E * 4, hipMemcpyDeviceToHost, c->stream));
   if (sorted_ids && E)
@@ -1647,6 +1756,14 @@ int cgs_read_stage3(cgs_context* c, uint32_t* counts, uint32_t* offsets, uint32_
   if (ranges && T)
     HIPCHK(hipMemcpyAsync(ranges, c->fa.ranges.p, T * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (permuted) {
+    for (uint32_t j = 0; j < c->max_g; ++j) {
+      const uint32_t sid = order[j];
+      if (sid >= c->max_g) return fail(CGS_ERR_EXECUTION, "internal: presort order out of range");
+      if (counts) counts[sid] = pos_counts[j];
+      if (offsets) offsets[sid] = pos_offsets[j];
+    }
+  }
   if (n_entries) *n_entries = E;
   return CGS_OK;
 }
@@ -2993,6 +3110,7 @@ int size_arena(FrameArena& a, uint32_t max_g, uint32_t T, bool mode64, bool own_
   TRY(ensure_words(a.comp_order, T));
   TRY(ensure_words(a.ctrl3, frame_layout(max_g, cap).words));
   if (bucket_T != 0 && bucket_mode(a, bucket_T, bucket_g)) TRY(size_bucket(a, bucket_g, bucket_T));
+  if (a.tune.tile_sort == 2u) TRY(size_presort(a, max_g));
   return CGS_OK;
 }
 
@@ -3345,7 +3463,8 @@ int cgs_execute_views_batched(cgs_batch* b, cgs_context* const* scenes, uint32_t
       const DevBuf* bufs[] = {&a.geom,      &a.color,     &a.extents, &a.cnt_dbg, &a.e_keys,
                               &a.e_ids,     &a.e_alt_k,   &a.e_alt_v, &a.e_depths, &a.e_perm,
                               &a.ranges,    &a.comp_order, &a.hitm,   &a.ctrl3,   &a.bk_hist,
-                              &a.bk_base,   &a.bk_bucket, &a.bk_alt,  &a.bk_tab,  &a.bk_ctl};
+                              &a.bk_base,   &a.bk_bucket, &a.bk_alt,  &a.bk_tab,  &a.bk_ctl,
+                              &a.sp_keys,   &a.sp_vals,   &a.sp_alt_k, &a.sp_alt_v, &a.sp_ctrl};
       for (const DevBuf* d : bufs) h = fnv_buf(h, *d);
       h = fnv_u64(h, a.entry_cap);
     }

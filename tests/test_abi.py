@@ -63,13 +63,21 @@ def test_tuning_candidates_cover_every_knob():
     LSD radix sort with every combination of partition size, digit width,
     compositor batch and tile order.  8 x 8 tiles (batch 64): radix partitions
     16384 / 8192 and the bucket sort, each with both orders.  16 x 32 and
-    32 x 16: the same with batches 256 / 128.  Each exactly once, 16 x 16
+    32 x 16: the same with batches 256 / 128.  Then the presort (tile_sort 2):
+    16 x 16 with partitions 16384 / 10240 / 8192 x batch 256 / 128 x order, and
+    8 x 8 with partitions 16384 / 8192 x order.  Each exactly once, 16 x 16
     first."""
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
     n = tuning_candidates_count()
     combos = [tuple(tuning_candidate(i).as_dict().values()) for i in range(n)]
-    assert n == len(set(combos)) == 66 + 6 + 12 + 12
+    assert n == len(set(combos)) == 66 + 6 + 12 + 12 + 16
+    presort = combos[96:]
+    assert all(c[4] == 2 for c in presort) and not any(c[4] == 2 for c in combos[:96])
+    assert sorted(presort) == sorted(
+        [(p, 8, b, o, 2, 16, 16) for p in (16384, 10240, 8192) for b in (256, 128) for o in (0, 1)]
+        + [(p, 8, 64, o, 2, 8, 8) for p in (16384, 8192) for o in (0, 1)])
+    combos = combos[:96]
     base = [c for c in combos if c[5:] == (16, 16)]
     assert combos[:66] == base
     bucket = [c for c in base if c[4] == 1]
