@@ -560,6 +560,12 @@ struct CgsGemmArgs {       // C = A B^T, A [M x K], B [N x K] bf16 (K % 32 == 0,
   uint32_t ldcb;
   uint16_t* Ct;            // bf16 transposed [N x ldct] or null (M % 4 == 0)
   uint32_t ldct;
+  // split-K (ksplit > 1): block z of the grid sums the K range z of ksplit
+  // equal 32-aligned ranges into Cf + z * split_stride (f32 partials only: no
+  // bias, ReLU, mask, bf16 or transposed output); the partials are summed by
+  // their reader in a fixed order
+  uint32_t ksplit;
+  uint64_t split_stride;
 };
 struct CgsAdamArgs {
   float* w;                // flat master parameters (layer l: rows x (cols + 32), bias in col `cols`)
@@ -572,8 +578,13 @@ struct CgsAdamArgs {
   uint16_t* wb[CGS_MLP_LAYERS];  // bf16 [rows x cols]
   uint16_t* wt[CGS_MLP_LAYERS];  // bf16 [cols x rows]
   float b1, b2, lr_t, eps_t;
+  uint32_t splits;         // the gradient is the sum of `splits` partials,
+  uint64_t split_stride;   // grad[e + s * split_stride], s = 0 .. splits-1
 };
 void cgs_launch_gemm_nt(const CgsGemmArgs& g, hipStream_t s);
+// grad[e] = sum over s < splits of grad[e + s * stride] (in place), e < total
+void cgs_launch_sum_splits(float* grad, uint64_t total, uint32_t splits, uint64_t stride,
+                           hipStream_t s);
 void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const float* Bm,
                         uint16_t* A0, uint16_t* A0t, uint32_t ldt, uint16_t* T0, hipStream_t s);
 void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,

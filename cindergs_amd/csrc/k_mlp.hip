@@ -37,54 +37,134 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// C = A B^T: a block computes a 64 x BN tile (BN = 64 or 128) with four
+// waves of 32 x BN/2, K staged through LDS in chunks of 64, two buffers:
+// while the MFMAs consume one chunk the block's global loads of the next are
+// in flight.  Rows of a staged tile are 128 bytes (eight 16-byte chunks);
+// chunk c of row r sits at c ^ (r & 7), so the 16 rows a fragment read
+// touches fall in different banks.
+constexpr uint32_t GBK = 64;
+
+__device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
+  return row * GBK + ((chunk ^ (row & 7u)) << 3);
+}
+
+template <int BN>
 __global__ __launch_bounds__(256) void k_gemm_nt(CgsGemmArgs G) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t m0 = blockIdx.y * 64u + (wave >> 1) * 32u;
-  const uint32_t n0 = blockIdx.x * 64u + (wave & 1u) * 32u;
-  const uint32_t r = lane & 15u, kq = (lane >> 4) * 8u;
-  f32x4 acc[2][2];
+  constexpr int NJ = BN / 32;                    // MFMA tiles per wave along N
+  constexpr int AL = 64 * 8 / 256;               // A chunks per thread (2)
+  constexpr int BL = BN * 8 / 256;               // B chunks per thread (2 or 4)
+  __shared__ __align__(16) uint16_t sA[2][64 * GBK];
+  __shared__ __align__(16) uint16_t sB[2][BN * GBK];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t bm = blockIdx.y * 64u, bn = blockIdx.x * (uint32_t)BN;
+  const uint32_t wm = (wave >> 1) * 32u, wn = (wave & 1u) * (uint32_t)(BN / 2);
+  uint32_t k_begin = 0u, k_end = G.K;
+  float* Cf = G.Cf;
+  if (G.ksplit > 1u) {
+    const uint32_t kc = ((G.K / 32u + G.ksplit - 1u) / G.ksplit) * 32u;
+    k_begin = min(G.K, blockIdx.z * kc);
+    k_end = min(G.K, k_begin + kc);
+    Cf += (size_t)blockIdx.z * G.split_stride;
+  }
+  // the epilogue's operands, loaded before the K loop (unconditionally, at
+  // indexes clamped into the matrix; out-of-range elements are not stored)
+  float bias_v[NJ];
+  uint16_t mask_v[2][NJ][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16x8 zero = {};
-  for (uint32_t k = 0; k < G.K; k += 32u) {
-    bf16x8 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const uint32_t row = m0 + i * 16u + r;
-      a[i] = row < G.M ? *reinterpret_cast<const bf16x8*>(G.A + (size_t)row * G.lda + k + kq)
-                       : zero;
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint32_t col = n0 + j * 16u + r;
-      b[j] = col < G.N ? *reinterpret_cast<const bf16x8*>(G.B + (size_t)col * G.ldb + k + kq)
-                       : zero;
-    }
+  for (int j = 0; j < NJ; ++j) {
+    const uint32_t col = min(bn + wn + j * 16u + (lane & 15u), G.N - 1u);
+    bias_v[j] = G.bias ? G.bias[(size_t)col * G.bias_stride] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t row = min(bm + wm + i * 16u + (lane >> 4) * 4u + (uint32_t)q, G.M - 1u);
+        mask_v[i][j][q] = G.mask ? G.mask[(size_t)(row / G.mask_div) * G.ldm + col] : (uint16_t)0;
+      }
+  }
+  // this thread's staging chunks: id = t + 256 u -> (row id >> 3, chunk id & 7)
+  const bf16x8 zero = {};
+  bf16x8 ra[AL], rb[BL];
+  auto load = [&](uint32_t k0) {
+#pragma unroll
+    for (int u = 0; u < AL; ++u) {
+      const uint32_t id = t + 256u * (uint32_t)u, row = id >> 3, k = k0 + (id & 7u) * 8u;
+      const uint32_t gr = bm + row;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(G.A + (size_t)min(gr, G.M - 1u) * G.lda +
+                                                        min(k, k_end - 8u));
+      ra[u] = gr < G.M && k < k_end ? v : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < BL; ++u) {
+      const uint32_t id = t + 256u * (uint32_t)u, row = id >> 3, k = k0 + (id & 7u) * 8u;
+      const uint32_t gc = bn + row;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(G.B + (size_t)min(gc, G.N - 1u) * G.ldb +
+                                                        min(k, k_end - 8u));
+      rb[u] = gc < G.N && k < k_end ? v : zero;
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < AL; ++u) {
+      const uint32_t id = t + 256u * (uint32_t)u;
+      *reinterpret_cast<bf16x8*>(&sA[buf][swz(id >> 3, id & 7u)]) = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < BL; ++u) {
+      const uint32_t id = t + 256u * (uint32_t)u;
+      *reinterpret_cast<bf16x8*>(&sB[buf][swz(id >> 3, id & 7u)]) = rb[u];
+    }
+  };
+  f32x4 acc[2][NJ];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint32_t fr = lane & 15u, fc = lane >> 4;  // fragment row, 16-byte chunk in a k step
+  if (k_begin < k_end) {
+    load(k_begin);
+    stage(0);
+    __syncthreads();
+    int buf = 0;
+    for (uint32_t k0 = k_begin; k0 < k_end; k0 += GBK) {
+      const bool more = k0 + GBK < k_end;  // block-uniform
+      if (more) load(k0 + GBK);
+#pragma unroll
+      for (uint32_t ks = 0; ks < GBK / 32u; ++ks) {
+        bf16x8 a[2], b[NJ];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          a[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][swz(wm + i * 16u + fr, ks * 4u + fc)]);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][swz(wn + j * 16u + fr, ks * 4u + fc)]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < NJ; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      if (more) stage(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
   // lane holds C[row0 + q][col] for q = 0..3
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint32_t col = n0 + j * 16u + (lane & 15u);
-      const uint32_t row0 = m0 + i * 16u + (lane >> 4) * 4u;
+    for (int j = 0; j < NJ; ++j) {
+      const uint32_t col = bn + wn + j * 16u + (lane & 15u);
+      const uint32_t row0 = bm + wm + i * 16u + (lane >> 4) * 4u;
       if (col >= G.N) continue;
       float v[4];
-      const float bias = G.bias ? G.bias[(size_t)col * G.bias_stride] : 0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        float x = acc[i][j][q] + bias;
+        float x = acc[i][j][q] + bias_v[j];
         if (G.relu) x = fmaxf(x, 0.0f);
         if (G.mask) {
-          const uint32_t row = row0 + q;
-          const uint16_t mv = row < G.M ? G.mask[(size_t)(row / G.mask_div) * G.ldm + col] : 0;
+          const uint16_t mv = mask_v[i][j][q];
           x = ((mv & 0x8000u) == 0u && mv != 0u) ? x : 0.0f;
         }
         v[q] = x;
@@ -93,7 +173,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(CgsGemmArgs G) {
       for (int q = 0; q < 4; ++q) {
         const uint32_t row = row0 + q;
         if (row >= G.M) continue;
-        if (G.Cf) G.Cf[(size_t)row * G.ldcf + col] = v[q];
+        if (Cf) Cf[(size_t)row * G.ldcf + col] = v[q];
         if (G.Cb) G.Cb[(size_t)row * G.ldcb + col] = f2bf(v[q]);
       }
       if (G.Ct && row0 + 3u < G.M) {
@@ -141,17 +221,20 @@ __device__ __forceinline__ float wave_sum64(float v) {
   return v;
 }
 
-// Affine normalisation, squared error and its gradient, one wave per row:
+// Affine normalisation, squared error and its gradient:
 // w = z + (1 - sum z)/V, e = w - target, loss_row = sum e^2,
 // dL/dz_j = g_j - mean(g) with g = 2 e / (P V).  With affine = 0 the outputs
 // are compared as they are (w = z, dL/dz = g): the far-field share of a
 // near/far network has no sum constraint.  Writes dZ [P x ldz] and its
 // transpose [ldz(+32) x P] in bf16 (columns >= V stay zero).
+//
+// One wave per point (lanes over the outputs): the row sums, then dZ.  The
+// transpose dZ^T is written by k_transpose_bf16 (a column of dZ is one row of
+// dZ^T; written here directly each lane would store 2 bytes to its own row).
 __global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
                                                   const float* targets, const uint32_t* idx,
                                                   uint32_t P, uint32_t V, uint16_t* dZ,
-                                                  uint16_t* dZt, uint32_t ldt, float* row_loss,
-                                                  int affine) {
+                                                  float* row_loss, int affine) {
   const uint32_t p = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
   if (p >= P) return;
   const float* z = Z + (size_t)p * ldz;
@@ -172,11 +255,28 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* Z, uint32_t ldz,
   const float gmean = affine ? gsum / (float)V : 0.0f;
   for (uint32_t j = lane; j < V; j += 64u) {
     const float e = (z[j] + shift) - tg[j];
-    const uint16_t d = f2bf(scale * e - gmean);
-    dZ[(size_t)p * ldz + j] = d;
-    dZt[(size_t)j * ldt + p] = d;
+    dZ[(size_t)p * ldz + j] = f2bf(scale * e - gmean);
   }
   if (lane == 0 && row_loss) row_loss[p] = l2;
+}
+
+// out[c * ldo + r] = in[r * ldi + c], r < R, c < C (bf16), through a 64 x 64
+// LDS tile: each wave reads and writes 128-byte row segments
+__global__ __launch_bounds__(256) void k_transpose_bf16(const uint16_t* in, uint32_t ldi,
+                                                        uint32_t R, uint32_t C, uint16_t* out,
+                                                        uint32_t ldo) {
+  __shared__ uint16_t t[64][66];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t r0 = blockIdx.y * 64u, c0 = blockIdx.x * 64u;
+  for (uint32_t i = wave; i < 64u; i += 4u) {
+    const uint32_t r = r0 + i, c = c0 + lane;
+    t[i][lane] = r < R && c < C ? in[(size_t)r * ldi + c] : (uint16_t)0;
+  }
+  __syncthreads();
+  for (uint32_t i = wave; i < 64u; i += 4u) {
+    const uint32_t c = c0 + i, r = r0 + lane;
+    if (c < C && r < R) out[(size_t)c * ldo + r] = t[lane][i];
+  }
 }
 
 // one block: out = sum(v[0..n)) / div
@@ -199,7 +299,8 @@ __global__ __launch_bounds__(1024) void k_sum(const float* v, uint32_t n, float 
 __global__ __launch_bounds__(256) void k_adam(CgsAdamArgs A) {
   const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   if (e >= A.total) return;
-  const float g = A.grad[e];
+  float g = A.grad[e];
+  for (uint32_t sp = 1; sp < A.splits; ++sp) g += A.grad[e + sp * A.split_stride];
   const float m = A.b1 * A.m[e] + (1.0f - A.b1) * g;
   const float v = A.b2 * A.v[e] + (1.0f - A.b2) * g * g;
   A.m[e] = m;
@@ -218,6 +319,16 @@ __global__ __launch_bounds__(256) void k_adam(CgsAdamArgs A) {
     A.wb[l][(size_t)row * A.cols[l] + col] = b;
     A.wt[l][(size_t)col * A.rows[l] + row] = b;
   }
+}
+
+// the split-K partials of the gradient summed into the first (in place)
+__global__ __launch_bounds__(256) void k_sum_splits(float* grad, uint64_t total, uint32_t splits,
+                                                    uint64_t stride) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (e >= total) return;
+  float g = grad[e];
+  for (uint32_t sp = 1; sp < splits; ++sp) g += grad[e + sp * stride];
+  grad[e] = g;
 }
 
 // bf16 weights and transposes from the master weights (no update)
@@ -495,7 +606,20 @@ __global__ __launch_bounds__(256) void k_mlp_post(CgsMlpPostArgs A) {
 
 void cgs_launch_gemm_nt(const CgsGemmArgs& g, hipStream_t s) {
   if (g.M == 0 || g.N == 0) return;
-  hipLaunchKernelGGL(k_gemm_nt, dim3((g.N + 63u) / 64u, (g.M + 63u) / 64u), dim3(256), 0, s, g);
+  const uint32_t z = g.ksplit > 1u ? g.ksplit : 1u;
+  // 64 x 128 tiles when they still give at least 256 blocks, else 64 x 64
+  const uint32_t m_blocks = (g.M + 63u) / 64u;
+  if ((uint64_t)m_blocks * ((g.N + 127u) / 128u) * z >= 256u)
+    hipLaunchKernelGGL(k_gemm_nt<128>, dim3((g.N + 127u) / 128u, m_blocks, z), dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL(k_gemm_nt<64>, dim3((g.N + 63u) / 64u, m_blocks, z), dim3(256), 0, s, g);
+}
+
+void cgs_launch_sum_splits(float* grad, uint64_t total, uint32_t splits, uint64_t stride,
+                           hipStream_t s) {
+  if (total == 0 || splits <= 1u) return;
+  hipLaunchKernelGGL(k_sum_splits, dim3((uint32_t)((total + 255u) / 256u)), dim3(256), 0, s, grad,
+                     total, splits, stride);
 }
 
 void cgs_launch_fourier(const float* X, const uint32_t* idx, uint32_t P, const float* Bm,
@@ -516,8 +640,11 @@ void cgs_launch_mlp_infer(const CgsMlpInferArgs& a, hipStream_t s) {
 void cgs_launch_mlp_loss(const float* Z, uint32_t ldz, const float* targets, const uint32_t* idx,
                          uint32_t P, uint32_t V, uint16_t* dZ, uint16_t* dZt, uint32_t ldt,
                          float* row_loss, bool affine, hipStream_t s) {
+  if (P == 0) return;
   hipLaunchKernelGGL(k_mlp_loss, dim3((P + 3u) / 4u), dim3(256), 0, s, Z, ldz, targets, idx, P, V,
-                     dZ, dZt, ldt, row_loss, affine ? 1 : 0);
+                     dZ, row_loss, affine ? 1 : 0);
+  hipLaunchKernelGGL(k_transpose_bf16, dim3((V + 63u) / 64u, (P + 63u) / 64u), dim3(256), 0, s, dZ,
+                     ldz, P, V, dZt, ldt);
 }
 
 void cgs_launch_sum(const float* v, uint32_t n, float div, float* out, hipStream_t s) {

@@ -22,6 +22,10 @@ constexpr uint32_t NF = 32;    // Fourier frequencies (64 features)
 constexpr uint32_t NIN = 64;
 constexpr uint32_t NH = 256;
 constexpr uint32_t CHUNK = 65536;  // inference points per pass
+// split-K partials of the weight gradients: their GEMMs have K = the batch
+// (8192) but only rows/64 x (cols+32)/64 output tiles (20 for a hidden
+// layer), too few blocks for the chip without the split
+constexpr uint32_t GRAD_SPLITS = 16;
 
 #define MCHK(expr)                                                                    \
   do {                                                                                \
@@ -102,6 +106,8 @@ CgsAdamArgs adam_args(cgs_mlp* M) {
   A.v = M->v.p;
   A.grad = M->grad.p;
   A.total = M->off[CGS_MLP_LAYERS];
+  A.splits = GRAD_SPLITS;
+  A.split_stride = M->off[CGS_MLP_LAYERS];
   for (int l = 0; l < CGS_MLP_LAYERS; ++l) {
     A.off[l] = M->off[l];
     A.rows[l] = M->rows[l];
@@ -185,6 +191,8 @@ int forward_backward(cgs_mlp* M, const float* X, const float* T, const uint32_t*
     CgsGemmArgs gw = gemm(gT, P, hT, P, M->rows[l], M->cols[l] + 32, P);
     gw.Cf = M->grad.p + M->off[l];
     gw.ldcf = M->cols[l] + 32;
+    gw.ksplit = GRAD_SPLITS;
+    gw.split_stride = M->off[CGS_MLP_LAYERS];
     cgs_launch_gemm_nt(gw, s);
     if (l == 0) break;
     const int k = l & 1;
@@ -335,7 +343,7 @@ int cgs_mlp_create(cgs_context* ctx, uint32_t n_out, float fourier_sigma, uint64
     if ((rc = alloc(M, M->w, total)) != CGS_OK) break;
     if ((rc = alloc(M, M->m, total)) != CGS_OK) break;
     if ((rc = alloc(M, M->v, total)) != CGS_OK) break;
-    if ((rc = alloc(M, M->grad, total)) != CGS_OK) break;
+    if ((rc = alloc(M, M->grad, total * GRAD_SPLITS)) != CGS_OK) break;
     if ((rc = alloc(M, M->fourier, NF * 3)) != CGS_OK) break;
     for (int l = 0; l < CGS_MLP_LAYERS && rc == CGS_OK; ++l) {
       rc = alloc(M, M->wb[l], (size_t)rows[l] * cols[l]);
@@ -433,6 +441,8 @@ int cgs_mlp_gradient(cgs_mlp* M, const float* d_x, const float* d_targets, uint3
   TRYM(ensure_train(M, n));
   TRYM(alloc(M, M->losses, 1));
   TRYM(forward_backward(M, d_x, d_targets, nullptr, n, M->losses.p));
+  cgs_launch_sum_splits(M->grad.p, M->off[CGS_MLP_LAYERS], GRAD_SPLITS, M->off[CGS_MLP_LAYERS],
+                        M->s);
   MCHK(hipMemcpyAsync(grad, M->grad.p, M->off[CGS_MLP_LAYERS] * 4, hipMemcpyDeviceToHost, M->s));
   if (loss) MCHK(hipMemcpyAsync(loss, M->losses.p, 4, hipMemcpyDeviceToHost, M->s));
   MCHK(hipStreamSynchronize(M->s));
