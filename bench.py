@@ -762,6 +762,23 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
         stt.synchronize()
     per_view_s = (time.perf_counter() - t) / reps
     print(f"# per-view: {1000 * per_view_s:.2f} ms per {n_views} views", file=sys.stderr, flush=True)
+    # the device time of the same views (HIP events around the stages of each
+    # frame, summed): against the per-view wall time it shows whether the
+    # per-view loop leaves the GPU idle between frames
+    prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 1, 0)
+    device_ms = 0.0
+    by_class = {}
+    for v in range(n_views):
+        states[view_scene[v]].execute_frame_resident(cams[v], comps[v], prof, tim)
+        ms = (tim.stage1_deform_ms + tim.stage2_project_ms + tim.stage3_bin_sort_ms
+              + tim.stage4_composite_ms)
+        device_ms += ms
+        key = f"{scenes[view_scene[v]].count} splats @ {comps[v].screen_width}x{comps[v].screen_height}"
+        c = by_class.setdefault(key, {"views": 0, "device_ms": 0.0})
+        c["views"] += 1
+        c["device_ms"] = round(c["device_ms"] + ms, 3)
+    print(f"# per-view device time: {device_ms:.2f} ms per {n_views} views", file=sys.stderr,
+          flush=True)
 
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
@@ -825,6 +842,10 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
            "scenes": [s.count for s in scenes],
            "resolutions": sorted({f"{c.screen_width}x{c.screen_height}" for c in comps}),
            "per_view_views_per_s": round(n_views / per_view_s, 1),
+           "per_view_device_ms": round(device_ms, 3),
+           "per_view_wall_ms": round(1000 * per_view_s, 3),
+           "per_view_device_ms_by_class": dict(sorted(by_class.items(),
+                                                      key=lambda kv: -kv[1]["device_ms"])),
            "batched_views_per_s": round(n_views / best[1], 1),
            "speedup": round(per_view_s / best[1], 3),
            "lanes": best[0], "tile_entries_per_batch": best[2],
