@@ -161,14 +161,16 @@ constexpr bool COMPOSITE_PREFETCH = true;
 template <int QX, int QY, int CB, int STEP, bool PF>
 __global__ __launch_bounds__(64 * QX * QY) void k_composite(CgsCompositeArgs A) {
   constexpr int NQ = QX * QY;
-  static_assert(CB <= 64 * NQ, "a batch is staged by one thread per splat");
+  constexpr int NT = 64 * NQ;                    // threads
+  constexpr int SU = CB > NT ? CB / NT : 1;      // splats staged per thread
+  static_assert(CB <= NT || CB % NT == 0, "a batch is staged by SU splats per thread");
   // staged batch: plane 0 = Splat::g, 1 = Splat::m, 2 = (blue, -, -, -); the
   // quadrant lists hold byte offsets into a plane, so a splat's three reads
   // share one address register
   __shared__ float4 s_stage[3][CB];
   __shared__ uint64_t s_mask[NQ][CB / 64];     // [quadrant][loading wave]
   __shared__ __align__(16) uint32_t s_list[NQ][CB];  // [quadrant] splats that can touch it
-  const bool loader = threadIdx.x < (uint32_t)CB;
+  const bool loader = threadIdx.x < (uint32_t)CB;  // (every thread when SU > 1)
 
   const uint32_t total_tiles = A.tiles_x * A.tiles_y;
   if (blockIdx.x >= total_tiles) return;
@@ -191,54 +193,66 @@ __global__ __launch_bounds__(64 * QX * QY) void k_composite(CgsCompositeArgs A) 
   // with PF, the records of the batch about to be staged are loaded one batch
   // ahead.  The loads are unconditional, at indexes clamped to the tile's last
   // entry, so that nothing waits for them before they are used a batch later.
-  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, rc = r0;
-  uint32_t g_ahead = 0u;
+  float4 r0[SU], r1[SU], rc[SU];
+  uint32_t g_ahead[SU];
   const uint32_t last = empty ? rs : re - 1u;
-  if (PF && !empty && loader) {
-    const uint32_t g = A.sorted_ids[min(rs + t, last)];
-    r0 = geom4[(size_t)g * 2];
-    r1 = geom4[(size_t)g * 2 + 1];
-    rc = color4[g];
-    g_ahead = A.sorted_ids[min(rs + t + (uint32_t)CB, last)];
+#pragma unroll
+  for (int u = 0; u < SU; ++u) {
+    r0[u] = r1[u] = rc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    g_ahead[u] = 0u;
+    const uint32_t slot = t + (uint32_t)(u * NT);
+    if (PF && !empty && loader) {
+      const uint32_t g = A.sorted_ids[min(rs + slot, last)];
+      r0[u] = geom4[(size_t)g * 2];
+      r1[u] = geom4[(size_t)g * 2 + 1];
+      rc[u] = color4[g];
+      g_ahead[u] = A.sorted_ids[min(rs + slot + (uint32_t)CB, last)];
+    }
   }
 
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
   bool alive = inside;
   for (uint32_t b0 = 0; b0 < n; b0 += CB) {
-    // ---- stage the batch and test it against the 4 quadrants ----
-    const uint32_t e = rs + b0 + t;
-    bool qh[NQ];
+    // ---- stage the batch and test it against the quadrants ----
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) qh[q] = false;
-    if (!PF && loader && e < re) {
-      const uint32_t g = A.sorted_ids[e];
-      r0 = geom4[(size_t)g * 2];
-      r1 = geom4[(size_t)g * 2 + 1];
-      rc = color4[g];
-    }
-    if (loader && e < re) {
-      // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
-      const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
-      s_stage[0][t] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
-      s_stage[1][t] = make_float4(r0.x, r0.y, rc.x, rc.y);
-      reinterpret_cast<float*>(&s_stage[2][t])[0] = rc.z;
+    for (int u = 0; u < SU; ++u) {
+      const uint32_t slot = t + (uint32_t)(u * NT);
+      const uint32_t e = rs + b0 + slot;
+      bool qh[NQ];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        qh[q] = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, bx + (q % QX) * 8u,
-                                  by + (q / QX) * 8u, 8u, 8u, A.width, A.height);
-    }
+      for (int q = 0; q < NQ; ++q) qh[q] = false;
+      if (!PF && loader && e < re) {
+        const uint32_t g = A.sorted_ids[e];
+        r0[u] = geom4[(size_t)g * 2];
+        r1[u] = geom4[(size_t)g * 2 + 1];
+        rc[u] = color4[g];
+      }
+      if (loader && e < re) {
+        // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
+        const float4 a0 = r0[u], a1 = r1[u], ac = rc[u];
+        const float op = a1.y > ALPHA_MIN ? a1.y : 0.0f;
+        s_stage[0][slot] = make_float4(-0.5f * a0.z, -a0.w, -0.5f * a1.x, op);
+        s_stage[1][slot] = make_float4(a0.x, a0.y, ac.x, ac.y);
+        reinterpret_cast<float*>(&s_stage[2][slot])[0] = ac.z;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const uint64_t mq = __ballot(qh[q]);
-      if (lane == 0 && wave < (uint32_t)(CB / 64)) s_mask[q][wave] = mq;
-    }
-    // the next batch's records are loaded while this batch is composited
-    if (PF && loader) {
-      const uint32_t en = e + CB;
-      r0 = geom4[(size_t)g_ahead * 2];
-      r1 = geom4[(size_t)g_ahead * 2 + 1];
-      rc = color4[g_ahead];
-      g_ahead = A.sorted_ids[min(en + (uint32_t)CB, last)];
+        for (int q = 0; q < NQ; ++q)
+          qh[q] = ellipse_hits_rect(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.w, bx + (q % QX) * 8u,
+                                    by + (q / QX) * 8u, 8u, 8u, A.width, A.height);
+      }
+      const uint32_t lw = wave + (uint32_t)(u * (NT / 64));  // the loading wave of this slot
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const uint64_t mq = __ballot(qh[q]);
+        if (lane == 0 && lw < (uint32_t)(CB / 64)) s_mask[q][lw] = mq;
+      }
+      // the next batch's records are loaded while this batch is composited
+      if (PF && loader) {
+        const uint32_t en = e + CB;
+        r0[u] = geom4[(size_t)g_ahead[u] * 2];
+        r1[u] = geom4[(size_t)g_ahead[u] * 2 + 1];
+        rc[u] = color4[g_ahead[u]];
+        g_ahead[u] = A.sorted_ids[min(en + (uint32_t)CB, last)];
+      }
     }
     __syncthreads();
 
@@ -340,6 +354,8 @@ void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* pro
       hipLaunchKernelGGL((k_composite<2, 2, 64, ST, PF>), grid, dim3(256), 0, s, A);
     else if (batch == 128u)
       hipLaunchKernelGGL((k_composite<2, 2, 128, ST, PF>), grid, dim3(256), 0, s, A);
+    else if (batch == 512u)
+      hipLaunchKernelGGL((k_composite<2, 2, 512, ST, PF>), grid, dim3(256), 0, s, A);
     else
       hipLaunchKernelGGL((k_composite<2, 2, 256, ST, PF>), grid, dim3(256), 0, s, A);
   }

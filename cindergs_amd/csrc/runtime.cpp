@@ -1069,8 +1069,11 @@ int check_tuning(const cgs_tuning* t) {
                 "radix_partition_keys must be 16384, 12288, 10240, 8192 or 4096");
   if (t->radix_digit_bits != 8u && t->radix_digit_bits != 4u)
     return fail(CGS_ERR_INVALID_ARG, "radix_digit_bits must be 8 or 4");
-  if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u)
-    return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128 or 256");
+  if (t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u &&
+      t->composite_batch != 512u)
+    return fail(CGS_ERR_INVALID_ARG, "composite_batch must be 64, 128, 256 or 512");
+  if (t->composite_batch == 512u && !(t->tile_w == 16u && t->tile_h == 16u))
+    return fail(CGS_ERR_INVALID_ARG, "composite_batch 512 needs 16x16 tiles");
   if (t->tile_order > 1u) return fail(CGS_ERR_INVALID_ARG, "tile_order must be 0 or 1");
   if (t->tile_sort > 2u) return fail(CGS_ERR_INVALID_ARG, "tile_sort must be 0, 1 or 2");
   const uint32_t tw = t->tile_w, th = t->tile_h;
@@ -1497,13 +1500,15 @@ int cgs_kernel_times(cgs_context* c, char* names, double* ms, uint32_t cap, uint
 // x order); then the other tile sizes with fewer choices each: 8 x 8 (batch
 // 64) radix partitions 16384 / 8192 x order, then bucket x order; 16 x 32 and
 // 32 x 16 radix partitions 16384 / 8192 x batch 256 / 128 x order, then
-// bucket x batch x order.  Last the presort (tile_sort 2): 16 x 16 with
+// bucket x batch x order.  Then the presort (tile_sort 2): 16 x 16 with
 // partitions 16384 / 10240 / 8192 x batch 256 / 128 x order, 8 x 8 with
-// partitions 16384 / 8192 x order.
-uint32_t cgs_tuning_candidates(void) { return 112u; }
+// partitions 16384 / 8192 x order.  Last, 16 x 16 with compositor batches of
+// 512 (two splats staged per thread): LSD partitions 16384 / 12288 / 10240 /
+// 8192 x order.
+uint32_t cgs_tuning_candidates(void) { return 120u; }
 
 int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
-  if (!out || i >= 112u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
+  if (!out || i >= 120u) return fail(CGS_ERR_INVALID_ARG, "candidate index out of range");
   static const uint32_t parts[5] = {16384u, 12288u, 10240u, 8192u, 4096u},
                         digits[2] = {8u, 4u}, batches[3] = {256u, 128u, 64u},
                         orders[2] = {1u, 0u}, parts2[2] = {16384u, 8192u},
@@ -1535,6 +1540,13 @@ int cgs_tuning_candidate(uint32_t i, cgs_tuning* out) {
       out->tile_order = orders[i - 4u];
       out->tile_sort = 1u;
     }
+  } else if (i >= 112u) {
+    i -= 112u;
+    static const uint32_t parts4[4] = {16384u, 12288u, 10240u, 8192u};
+    out->radix_partition_keys = parts4[i % 4u];
+    out->composite_batch = 512u;
+    out->tile_order = orders[i / 4u];
+    out->tile_sort = 0u;
   } else if (i >= 96u) {
     i -= 96u;
     out->tile_sort = 2u;

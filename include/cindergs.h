@@ -214,8 +214,8 @@ typedef struct cgs_tuning {
                                     sort partition (1024 threads x 16 or 12
                                     keys; 640, 512 or 256 threads x 16) */
   uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
-  uint32_t composite_batch;      /* 64, 128 or 256 splats per compositor step
-                                    (8 x 8 tiles: 64) */
+  uint32_t composite_batch;      /* 64, 128, 256 or 512 splats per compositor
+                                    step (8 x 8 tiles: 64; 512: 16 x 16 tiles) */
   uint32_t tile_order;           /* 1: heaviest tiles composited first */
   uint32_t tile_sort;            /* 1: entries written into per-tile buckets and each
                                     tile sorted by (key, splat id) in LDS; 0: one

@@ -65,14 +65,17 @@ def test_tuning_candidates_cover_every_knob():
     16384 / 8192 and the bucket sort, each with both orders.  16 x 32 and
     32 x 16: the same with batches 256 / 128.  Then the presort (tile_sort 2):
     16 x 16 with partitions 16384 / 10240 / 8192 x batch 256 / 128 x order, and
-    8 x 8 with partitions 16384 / 8192 x order.  Each exactly once, 16 x 16
-    first."""
+    8 x 8 with partitions 16384 / 8192 x order.  Last, 16 x 16 with compositor
+    batches of 512: partitions 16384 / 12288 / 10240 / 8192 x order.  Each
+    exactly once, 16 x 16 first."""
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
     n = tuning_candidates_count()
     combos = [tuple(tuning_candidate(i).as_dict().values()) for i in range(n)]
-    assert n == len(set(combos)) == 66 + 6 + 12 + 12 + 16
-    presort = combos[96:]
+    assert n == len(set(combos)) == 66 + 6 + 12 + 12 + 16 + 8
+    assert sorted(combos[112:]) == sorted(
+        [(p, 8, 512, o, 0, 16, 16) for p in (16384, 12288, 10240, 8192) for o in (0, 1)])
+    presort = combos[96:112]
     assert all(c[4] == 2 for c in presort) and not any(c[4] == 2 for c in combos[:96])
     assert sorted(presort) == sorted(
         [(p, 8, b, o, 2, 16, 16) for p in (16384, 10240, 8192) for b in (256, 128) for o in (0, 1)]

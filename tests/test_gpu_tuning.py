@@ -2,7 +2,7 @@
 
 Bar: every tuning candidate (the per-tile bucket sort, or the LSD radix sort
 with partitions of 16384/12288/10240/8192/4096 keys and 8- or 4-bit digits;
-compositor batch 256/128/64, heaviest-tiles-first order on/off; the presort
+compositor batch 512/256/128/64, heaviest-tiles-first order on/off; the presort
 by depth followed by a sort of the tile bits) of a tile size gives
 bit-identical tile counts, scan offsets (the presort's scan runs in depth
 order and is not compared), sorted keys, sorted ids, tile ranges and RGBA8 to
@@ -74,7 +74,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
                 refs[size] = got
     finally:
         gpu.set_tuning(default)
-    assert len(seen) == n == 112
+    assert len(seen) == n == 120
     assert set(refs) == {(16, 16), (8, 8), (16, 32), (32, 16)}
 
 
@@ -117,6 +117,8 @@ def test_invalid_tuning_is_rejected(gpu):
     for bad in (Tuning(2048, 8, 256, 1), Tuning(12000, 8, 256, 1), Tuning(16384, 6, 256, 1), Tuning(16384, 8, 100, 1),
                 Tuning(16384, 8, 256, 2), Tuning(16384, 8, 256, 1, 3),
                 Tuning(16384, 8, 256, 1, 0, 32, 32), Tuning(16384, 8, 256, 1, 0, 8, 16),
-                Tuning(16384, 8, 256, 1, 0, 0, 0), Tuning(16384, 8, 128, 1, 0, 8, 8)):
+                Tuning(16384, 8, 256, 1, 0, 0, 0), Tuning(16384, 8, 128, 1, 0, 8, 8),
+                Tuning(16384, 8, 512, 1, 0, 8, 8), Tuning(16384, 8, 512, 1, 0, 16, 32),
+                Tuning(16384, 8, 1024, 1)):
         with pytest.raises(GpuError):
             gpu.set_tuning(bad)
