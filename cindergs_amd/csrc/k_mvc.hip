@@ -558,6 +558,53 @@ __device__ __forceinline__ int jsw_terms(V3f x, V3f v0, V3f v1, V3f v2, float* m
   return 1;
 }
 
+// jsw_terms with the divisions of the near field as hardware reciprocals
+// (v_rcp_f32 / v_rsq_f32, about 1 ulp) and contracted multiply-adds: the same
+// quantities to a few ulp, which is far inside the learned binding's error
+// budget (its weights are not held to the exact kernel's bits).  The unit
+// directions come from one rsqrt per corner; 1 / (st_j st_k) = st_i / (st0
+// st1 st2) shares one reciprocal between the three cosines.
+__device__ __forceinline__ int jsw_terms_fast(V3f x, V3f v0, V3f v1, V3f v2, float* mu) {
+#pragma clang fp contract(fast)
+  const V3f e0 = vsub(v0, x), e1 = vsub(v1, x), e2 = vsub(v2, x);
+  const float q0 = vdot(e0, e0), q1 = vdot(e1, e1), q2 = vdot(e2, e2);
+  if (q0 < 1e-12f || q1 < 1e-12f || q2 < 1e-12f) return 2;
+  const float r0 = __builtin_amdgcn_rsqf(q0), r1 = __builtin_amdgcn_rsqf(q1),
+              r2 = __builtin_amdgcn_rsqf(q2);
+  const float d0 = q0 * r0, d1 = q1 * r1, d2 = q2 * r2;
+  const V3f u0 = {e0.x * r0, e0.y * r0, e0.z * r0};
+  const V3f u1 = {e1.x * r1, e1.y * r1, e1.z * r1};
+  const V3f u2 = {e2.x * r2, e2.y * r2, e2.z * r2};
+  const V3f a0 = vsub(u1, u2), a1 = vsub(u2, u0), a2 = vsub(u0, u1);
+  const float s0h = cgs_fclamp(__builtin_amdgcn_sqrtf(vdot(a0, a0)) * 0.5f, 0.0f, 1.0f);
+  const float s1h = cgs_fclamp(__builtin_amdgcn_sqrtf(vdot(a1, a1)) * 0.5f, 0.0f, 1.0f);
+  const float s2h = cgs_fclamp(__builtin_amdgcn_sqrtf(vdot(a2, a2)) * 0.5f, 0.0f, 1.0f);
+  const float c0h = __builtin_amdgcn_sqrtf((1.0f - s0h) * (1.0f + s0h));
+  const float c1h = __builtin_amdgcn_sqrtf((1.0f - s1h) * (1.0f + s1h));
+  const float c2h = __builtin_amdgcn_sqrtf((1.0f - s2h) * (1.0f + s2h));
+  const float th0 = 2.0f * asinf(s0h), th1 = 2.0f * asinf(s1h), th2 = 2.0f * asinf(s2h);
+  const float h = 0.5f * ((th0 + th1) + th2);
+  if (fabsf(MVC_PI - h) < 1e-5f) return 2;
+  const float st0 = 2.0f * s0h * c0h, st1 = 2.0f * s1h * c1h, st2 = 2.0f * s2h * c2h;
+  if (fabsf(st0) < 1e-6f || fabsf(st1) < 1e-6f || fabsf(st2) < 1e-6f) return 0;
+  const float ct0 = 1.0f - 2.0f * s0h * s0h, ct1 = 1.0f - 2.0f * s1h * s1h, ct2 = 1.0f - 2.0f * s2h * s2h;
+  const float s01 = s0h * c1h + c0h * s1h, c01 = c0h * c1h - s0h * s1h;
+  const float sh = s01 * c2h + c01 * s2h, ch = c01 * c2h - s01 * s2h;
+  const float rst = __builtin_amdgcn_rcpf(st0 * st1 * st2);
+  const float c0 = cgs_fclamp((2.0f * sh * (sh * ct0 - ch * st0)) * (st0 * rst) - 1.0f, -1.0f, 1.0f);
+  const float c1 = cgs_fclamp((2.0f * sh * (sh * ct1 - ch * st1)) * (st1 * rst) - 1.0f, -1.0f, 1.0f);
+  const float c2 = cgs_fclamp((2.0f * sh * (sh * ct2 - ch * st2)) * (st2 * rst) - 1.0f, -1.0f, 1.0f);
+  const float sg = vdot(u2, vcross(u0, u1)) >= 0.0f ? 1.0f : -1.0f;
+  const float s0 = sg * __builtin_amdgcn_sqrtf(cgs_fmax(1.0f - c0 * c0, 0.0f));
+  const float s1 = sg * __builtin_amdgcn_sqrtf(cgs_fmax(1.0f - c1 * c1, 0.0f));
+  const float s2 = sg * __builtin_amdgcn_sqrtf(cgs_fmax(1.0f - c2 * c2, 0.0f));
+  if (fabsf(s0) <= 1e-6f || fabsf(s1) <= 1e-6f || fabsf(s2) <= 1e-6f) return 0;
+  mu[0] = (th0 - c1 * th2 - c2 * th1) * __builtin_amdgcn_rcpf(2.0f * d0 * st1 * s2);
+  mu[1] = (th1 - c2 * th0 - c0 * th2) * __builtin_amdgcn_rcpf(2.0f * d1 * st2 * s0);
+  mu[2] = (th2 - c0 * th1 - c1 * th0) * __builtin_amdgcn_rcpf(2.0f * d2 * st0 * s1);
+  return 1;
+}
+
 // The learned binding's weights, one wave per point:
 //   w = (1 - sum q) u_near U_near / (U_near^2 + tau^2) + q,
 // u_near = sum over the point's near triangles of phi_t mu_t, U_near its sum,
@@ -642,6 +689,7 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
   __shared__ uint32_t s_key[NEAR_SLOTS];
   __shared__ float s_val[7][NEAR_SLOTS];
   __shared__ uint32_t s_tri[NEAR_TRIS];
+  __shared__ uint32_t s_tslot[NEAR_TRIS];
   const uint32_t lane = threadIdx.x, i = blockIdx.x;
   if (i >= A.n) return;
   const uint32_t nv = A.nv;
@@ -726,9 +774,25 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
       m += (uint32_t)__popcll(bal);
     }
     __syncthreads();
+    // the near triangles' vertices take their hash slots once per triangle,
+    // not once per evaluation: three 8-bit slots per triangle, bit 24 set
+    // when the table is full (the point then goes to the exact path)
+    for (uint32_t j = lane; j < m; j += 64u) {
+      const uint32_t t = s_tri[j];
+      uint32_t packed = 0u;
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) {
+        const uint32_t sl = near_slot_insert(s_key, A.tris[t * 3 + c3]);
+        packed |= sl == NEAR_EMPTY ? (1u << 24) : sl << (8 * c3);
+      }
+      if (packed >> 24) degenerate = 1u;
+      s_tslot[j] = packed;
+    }
+    __syncthreads();
     for (uint32_t item = lane; item < m * evals; item += 64u) {
-      const uint32_t t = s_tri[item / evals];
-      const int e = (int)(item % evals);
+      const uint32_t j = evals == 7u ? item / 7u : item;
+      const int e = (int)(item - j * evals);
+      const uint32_t t = s_tri[j];
       const V3f c = vert(A.cent, t);
       V3f x = p;
       if (e > 0) {
@@ -742,18 +806,15 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
       if (!(phi > 0.0f)) continue;
       const uint32_t iv[3] = {A.tris[t * 3], A.tris[t * 3 + 1], A.tris[t * 3 + 2]};
       float mu[3];
-      const int k = jsw_terms(x, vert(A.verts, iv[0]), vert(A.verts, iv[1]), vert(A.verts, iv[2]), mu);
+      const int k =
+          jsw_terms_fast(x, vert(A.verts, iv[0]), vert(A.verts, iv[1]), vert(A.verts, iv[2]), mu);
       if (k == 2) degenerate = 1u;
       if (k != 1) continue;
+      const uint32_t packed = s_tslot[j];
+      if (packed >> 24) continue;
 #pragma unroll
-      for (int c3 = 0; c3 < 3; ++c3) {
-        const uint32_t sl = near_slot_insert(s_key, iv[c3]);
-        if (sl == NEAR_EMPTY) {
-          degenerate = 1u;
-        } else {
-          atomicAdd(&s_val[e][sl], phi * mu[c3]);
-        }
-      }
+      for (int c3 = 0; c3 < 3; ++c3)
+        atomicAdd(&s_val[e][(packed >> (8 * c3)) & 0xFFu], phi * mu[c3]);
 #pragma unroll
       for (int ee = 0; ee < 7; ++ee)
         if (ee == e) U[ee] += phi * ((mu[0] + mu[1]) + mu[2]);
@@ -903,6 +964,8 @@ void cgs_launch_mvc_nearfar(const CgsNearArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_mvc_nearfar<4>, dim3(a.n), dim3(64), lds, s, a);
   else if (a.nv <= 512)
     hipLaunchKernelGGL(k_mvc_nearfar<8>, dim3(a.n), dim3(64), lds, s, a);
+  else if (a.nv <= 768)
+    hipLaunchKernelGGL(k_mvc_nearfar<12>, dim3(a.n), dim3(64), lds, s, a);
   else if (a.nv <= 1024)
     hipLaunchKernelGGL(k_mvc_nearfar<16>, dim3(a.n), dim3(64), lds, s, a);
   else
