@@ -804,11 +804,12 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
     if fixed is not None:
         from cindergs_amd._lib import Tuning
 
-        tu = Tuning(fixed[1], 8, fixed[2], fixed[3])
+        sort = fixed[4] if len(fixed) > 4 else 0
+        tu = Tuning(fixed[1], 8, fixed[2], fixed[3], sort)
         wall, gms, stats = run_batch(fixed[0], tu)
         best = (fixed[0], wall, int(stats.total_tile_entries), tu)
-        sweep[f"lanes={fixed[0]} part={fixed[1]} batch={fixed[2]} order={fixed[3]}"] = round(
-            1000 * wall, 3)
+        sweep[f"lanes={fixed[0]} sort={('radix', 'bucket', 'presort')[sort]} part={fixed[1]} "
+              f"batch={fixed[2]} order={fixed[3]}"] = round(1000 * wall, 3)
         lanes_list = []
     for lanes in lanes_list:
         wall, gms, stats = run_batch(lanes, None)
@@ -824,7 +825,7 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
             continue
         wall, gms, stats = run_batch(best[0], tu)
         key = "lanes={} sort={} part={} batch={} order={}".format(
-            best[0], "bucket" if tu.tile_sort else "radix", tu.radix_partition_keys,
+            best[0], ("radix", "bucket", "presort")[tu.tile_sort], tu.radix_partition_keys,
             tu.composite_batch, tu.tile_order)
         sweep[key] = round(1000 * wall, 3)
         print(f"# batched {key}: {1000 * wall:.2f} ms", file=sys.stderr, flush=True)
@@ -920,8 +921,9 @@ def main() -> int:
                     help="views of the batched multi-view measurement (0 = skip it)")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--batch-lanes", type=str, default="4,8,16")
-    ap.add_argument("--batch-fixed", type=str, default="", metavar="LANES,PART,BATCH,ORDER",
-                    help="batched views: time only this lane count and tuning (no sweep)")
+    ap.add_argument("--batch-fixed", type=str, default="", metavar="LANES,PART,BATCH,ORDER[,SORT]",
+                    help="batched views: time only this lane count and tuning (no sweep); "
+                         "SORT 0 radix, 1 bucket, 2 presort")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="render the headline with the default launch shapes")

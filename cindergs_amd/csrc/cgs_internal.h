@@ -114,7 +114,20 @@ __device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
 // hold smaller partition ids taken from an atomic ticket (or launched earlier),
 // so they are resident or finished and every wait ends.  Called by one thread
 // per status column.
-template <class U, int VBITS, int WIN>
+// With PUBLISHED the caller has already stored this partition's aggregate
+// (lookback_publish), as early as it was known, so that the successors'
+// walks find it sooner.
+template <class U, int VBITS>
+__device__ __forceinline__ void lookback_publish(U* status, uint32_t part, uint32_t stride,
+                                                 U aggregate) {
+  const U VMASK = ((U)1 << VBITS) - (U)1;
+  if (part != 0)
+    __hip_atomic_store(&status[(size_t)part * stride],
+                       ((U)CGS_LB_AGG << VBITS) | (aggregate & VMASK), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class U, int VBITS, int WIN, bool PUBLISHED = false>
 __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_t stride,
                                                U aggregate) {
   const U VMASK = ((U)1 << VBITS) - (U)1;
@@ -123,8 +136,8 @@ __device__ __forceinline__ U lookback_windowed(U* status, uint32_t part, uint32_
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  __hip_atomic_store(&status[(size_t)part * stride], ((U)CGS_LB_AGG << VBITS) | (aggregate & VMASK),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!PUBLISHED)
+    lookback_publish<U, VBITS>(status, part, stride, aggregate);
   U excl = 0;
   int64_t q = (int64_t)part - 1;  // nearest predecessor not yet accounted for
   while (true) {
@@ -171,6 +184,11 @@ __device__ __forceinline__ uint64_t lookback_u64(uint64_t* status, uint32_t part
 __device__ __forceinline__ uint32_t lookback_u32(uint32_t* status, uint32_t part,
                                                  uint32_t stride, uint32_t aggregate) {
   return lookback_windowed<uint32_t, 30, 4>(status, part, stride, aggregate);
+}
+// the same walk for a partition that published its aggregate already
+__device__ __forceinline__ uint32_t lookback_walk_u32(uint32_t* status, uint32_t part,
+                                                      uint32_t stride, uint32_t aggregate) {
+  return lookback_windowed<uint32_t, 30, 4, true>(status, part, stride, aggregate);
 }
 
 }  // namespace cgs

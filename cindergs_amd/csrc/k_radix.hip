@@ -116,8 +116,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   constexpr int RP = RT * KI;
   constexpr int RW = RT / 64;
   constexpr int NB = 1 << DB;  // digits
-  __shared__ uint32_t s_keys[RP];
-  __shared__ uint32_t s_vals[RP];
+  __shared__ uint2 s_kv[RP];  // (key, value) staged in digit order
   __shared__ uint32_t s_whist[RW][NB];
   __shared__ uint32_t s_lstart[NB];
   __shared__ uint32_t s_gadj[NB];
@@ -188,7 +187,9 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   }
   __syncthreads();
 
-  // per digit (thread t == digit t < 256): wave prefixes, partition count
+  // per digit (thread t == digit t < 256): wave prefixes, partition count;
+  // the count is published for the successors' look-back at once, before
+  // the block scans
   uint32_t dtotal = 0;
   if (t < (uint32_t)NB) {
 #pragma unroll
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
       s_whist[w][t] = dtotal;
       dtotal += c;
     }
+    cgs::lookback_publish<uint32_t, 30>(A.status + t, part, NB, dtotal);
   }
   uint32_t htot;
   const uint32_t lstart = cgs::block_exclusive_scan<RT>(dtotal, s_tmp, htot);
@@ -206,32 +208,34 @@ __global__ __launch_bounds__(RT) void k_radix_onesweep(PassArgs A) {
   if (t < (uint32_t)NB) {
     s_lstart[t] = lstart;
     // look-back for this digit over the partitions before this one
-    const uint32_t excl = cgs::lookback_u32(A.status + t, part, NB, dtotal);
+    const uint32_t excl = cgs::lookback_walk_u32(A.status + t, part, NB, dtotal);
     s_gadj[t] = dbase + excl - lstart;
   }
   __syncthreads();
+  // each wave's digit offsets become partition offsets (one LDS read per key
+  // below instead of two)
+  for (uint32_t i = t; i < (uint32_t)(RW * NB); i += RT) (&s_whist[0][0])[i] += s_lstart[i % NB];
+  __syncthreads();
 
-  // stage keys and values in LDS in digit order
+  // stage (key, value) pairs in LDS in digit order
 #pragma unroll
   for (int i = 0; i < KI; ++i) {
     const uint32_t rank = (offs[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
     if (rank != 0xFFFFu) {
       const uint32_t d = (keys[i] >> A.shift) & dmask;
-      const uint32_t pos = s_lstart[d] + s_whist[wave][d] + rank;
-      s_keys[pos] = keys[i];
-      s_vals[pos] = vals[i];
+      s_kv[s_whist[wave][d] + rank] = make_uint2(keys[i], vals[i]);
     }
   }
   __syncthreads();
 
   // write out: equal digits are consecutive in LDS and in the output
   for (uint32_t j = t; j < cnt_part; j += RT) {
-    const uint32_t k = s_keys[j];
-    const uint32_t d = (k >> A.shift) & dmask;
+    const uint2 kv = s_kv[j];
+    const uint32_t d = (kv.x >> A.shift) & dmask;
     const uint32_t dst = s_gadj[d] + j;
     if (dst < n) {  // always true when the histogram matches the n keys read
-      A.keys_out[dst] = k;
-      A.vals_out[dst] = s_vals[j];
+      A.keys_out[dst] = kv.x;
+      A.vals_out[dst] = kv.y;
     }
   }
 }
