@@ -108,9 +108,7 @@ def composite_valu(tuning: str, per_launch_ms: float) -> dict | None:
     (one per pixel-splat evaluation: every lane of a walk step evaluates one
     splat at its pixel), LDS instructions and bank-conflict cycles, waves, and
     the occupancy its VGPRs allow."""
-    run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
-           "--no-traffic", "--no-extras", "--batch-views", "0", "--mvc", "0"]
-    run += ["--tuning", tuning] if tuning else ["--no-autotune"]
+    run = HEADLINE_CHILD + (["--tuning", tuning] if tuning else ["--no-autotune"])
     c = pmc_counters(["SQ_INSTS_VALU", "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_LDS",
                       "SQ_LDS_BANK_CONFLICT", "SQ_WAVES"], run)
     k = "k_composite"
@@ -139,15 +137,51 @@ def composite_valu(tuning: str, per_launch_ms: float) -> dict | None:
     return res
 
 
+# a child run of the headline frames alone (no sort baseline, no other legs),
+# so that the profiler's per-kernel figures are the frame's
+HEADLINE_CHILD = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
+                  "--no-traffic", "--no-extras", "--no-sort-baseline", "--batch-views", "0",
+                  "--mvc", "0"]
+
+
+def kernel_trace(tuning: str = "", timeout_s: int = 120) -> dict:
+    """Per-kernel launches and mean duration (µs) from one rocprofv3
+    --kernel-trace --stats child run of the headline frames, by bare kernel
+    name (template instances of one kernel combined).  Empty dict when
+    rocprofv3 is absent or the run fails."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return {}
+    out = tempfile.mkdtemp(prefix="cgs_kt_")
+    try:
+        run = HEADLINE_CHILD + (["--tuning", tuning] if tuning else ["--no-autotune"])
+        cmd = [exe, "--kernel-trace", "--stats", "--output-format", "csv", "-d", out, "-o", "k",
+               "--", sys.executable, os.path.abspath(__file__)] + run
+        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=timeout_s, check=True)
+        files = glob.glob(os.path.join(out, "**", "*kernel_stats.csv"), recursive=True)
+        acc = {}
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    k = _kernel_key(row["Name"])
+                    calls, tot = acc.get(k, (0, 0.0))
+                    acc[k] = (calls + int(row["Calls"]), tot + float(row["TotalDurationNs"]))
+        return {k: {"calls": c, "avg_us": round(t / c / 1000.0, 2)} for k, (c, t) in acc.items()
+                if c}
+    except (subprocess.SubprocessError, OSError, KeyError, ValueError):
+        return {}
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
 def hbm_traffic(kernel: str, calib_mib: int = 256, tuning: str = "") -> dict | None:
     """HBM bytes per launch of `kernel` from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by what the same counter reads for a device-to-device
     copy of calib_mib MiB (calib bytes read / written = calib_mib MiB)."""
     # the headline workload only: the other legs launch the same kernels on
     # other sizes and would enter the per-dispatch mean
-    run = ["--steps", "3", "--warmup", "2", "--profile-frames", "2", "--no-cpu-baseline",
-           "--no-traffic", "--no-extras", "--batch-views", "0", "--mvc", "0"]
-    run += ["--tuning", tuning] if tuning else ["--no-autotune"]
+    run = HEADLINE_CHILD + (["--tuning", tuning] if tuning else ["--no-autotune"])
     cal = ["--calibrate-copy", str(calib_mib)]
     nbytes = float(calib_mib << 20)
     res = {}
@@ -929,6 +963,8 @@ def main() -> int:
                     help="render the headline with the default launch shapes")
     ap.add_argument("--tuning", type=str, default="", metavar="PART,DIGITS,BATCH,ORDER,SORT",
                     help="render the headline with these launch shapes (no autotune)")
+    ap.add_argument("--no-sort-baseline", action="store_true",
+                    help="skip the vendor-library sort baseline (the profiler's child runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 8K, f16, cage-deformation, MVC scene and ingest measurements")
     ap.add_argument("--cage-frames", type=int, default=300)
@@ -1047,18 +1083,24 @@ def main() -> int:
     ms_per_frame = 1000.0 * wall / args.steps
     print(f"# timed {args.steps} frames: {ms_per_frame:.3f} ms/frame", file=sys.stderr, flush=True)
 
-    # per-kernel device times over the same orbit frames (HIP events on the
-    # context stream, one profiled frame per camera)
-    prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 2, 0)
-    per_kernel = {}
-    stage_ms = {"stage2": [], "stage3": [], "stage4": []}
+    # stage times over the same orbit frames: HIP events on the context stream
+    # around each stage only (per-kernel events, below, add their own
+    # overhead to every launch they bracket)
     nprof = max(1, min(args.steps, args.profile_frames))
+    stage_ms = {"stage2": [], "stage3": [], "stage4": []}
+    prof_stages = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 1, 0)
     for i in range(nprof):
         cam, comp = cams[(args.warmup + i) % 120]
-        st.execute_frame_resident(cam, comp, prof, tim)
+        st.execute_frame_resident(cam, comp, prof_stages, tim)
         stage_ms["stage2"].append(tim.stage2_project_ms)
         stage_ms["stage3"].append(tim.stage3_bin_sort_ms)
         stage_ms["stage4"].append(tim.stage4_composite_ms)
+    # per-kernel device times over the same frames (HIP events around each launch)
+    prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 2, 0)
+    per_kernel = {}
+    for i in range(nprof):
+        cam, comp = cams[(args.warmup + i) % 120]
+        st.execute_frame_resident(cam, comp, prof, tim)
         for name, ms in st.kernel_times():
             per_kernel.setdefault(name, []).append(ms)
     # kernels launched several times per frame (radix passes) are summed per frame
@@ -1113,11 +1155,21 @@ def main() -> int:
                      "bytes_per_launch": int(hbm_kernels[dom]),
                      "avg_launch_ms": round(per_launch_ms, 5)},
     }
-    result["sort_baseline"] = sort_baseline(
-        st, E, T, kern.get("k_radix_onesweep", 0.0) + kern.get("k_hist_reduce", 0.0))
+    if not args.no_sort_baseline:
+        result["sort_baseline"] = sort_baseline(
+            st, E, T, kern.get("k_radix_onesweep", 0.0) + kern.get("k_hist_reduce", 0.0))
     st.close()
     if not args.no_traffic:
         tstr = ",".join(str(v) for v in tuned["tuning"].values()) if tuned else ""
+        kt = kernel_trace(tstr)
+        if dom in kt:
+            # the same launches timed by the profiler (kernel execution only)
+            kt_ms = kt[dom]["avg_us"] / 1000.0
+            result["roofline"]["kernel_trace_avg_ms"] = round(kt_ms, 5)
+            result["roofline"]["achieved_by_kernel_trace"] = round(
+                hbm_kernels[dom] / (kt_ms * 1e-3) / 1e9, 1)
+            result["kernels_us_by_kernel_trace"] = {
+                k: v["avg_us"] for k, v in sorted(kt.items()) if k in kern}
         tr = hbm_traffic(dom, tuning=tstr)
         if tr is not None:
             result["roofline"]["traffic"] = int(tr["bytes"])
