@@ -148,7 +148,7 @@ class Tuning(C.Structure):
         ("tile_h", C.c_uint32),
     ]
 
-    def __init__(self, radix_partition_keys=16384, radix_digit_bits=8, composite_batch=256,
+    def __init__(self, radix_partition_keys=16384, radix_digit_bits=8, composite_batch=64,
                  tile_order=1, tile_sort=0, tile_w=16, tile_h=16):
         super().__init__(radix_partition_keys, radix_digit_bits, composite_batch, tile_order,
                          tile_sort, tile_w, tile_h)

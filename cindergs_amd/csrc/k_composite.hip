@@ -11,7 +11,11 @@
 // MI355X design:
 //  * one block per tile of QX x QY quadrants of 8x8 pixels (16x16: 2 x 2,
 //    256 threads); wave w owns quadrant (w % QX, w / QX), one pixel per lane;
-//  * a batch of CB splats is gathered once into LDS (two float4 + one float
+//  * k_composite_q (the default, composite_batch 64): each quadrant's wave
+//    walks the tile's list on its own, 64 splats per batch, with no block
+//    barrier, and leaves as soon as its 64 pixels are saturated;
+//  * k_composite (batches of 128-512 per block): a batch of CB splats is
+//    gathered once into LDS (two float4 + one float
 //    per splat) and, by the loading thread, tested against the QX x QY
 //    quadrants with the exact ellipse-vs-rectangle test the binning uses
 //    (binning.rs:35-83, on the quadrant's pixel-centre rectangle); the
@@ -316,6 +320,118 @@ __global__ __launch_bounds__(64 * QX * QY) void k_composite(CgsCompositeArgs A) 
   }
 }
 
+// Tile of QX x QY quadrants of 8x8 pixels with one wave per quadrant and no
+// block barrier: each wave walks the tile's whole list on its own, 64 splats
+// per batch (one per lane: loaded, tested against the wave's own quadrant
+// with the same exact ellipse test, staged in the wave's own LDS region),
+// compacts the hits and composites them as the block-wide kernel does.  A
+// wave whose pixels are all saturated leaves at once while the tile's other
+// quadrants go on.  Each wave reads the tile's records itself; the waves of a
+// block share a CU, so the repeats are L1 / L2 hits.
+template <int QX, int QY, int STEP>
+__global__ __launch_bounds__(64 * QX * QY) void k_composite_q(CgsCompositeArgs A) {
+  constexpr int NQ = QX * QY;
+  static_assert(STEP <= 4, "the list holds 4 entries past its 64 hits");
+  // slot 64 of each wave's region is a splat of opacity 0 (it blends
+  // nothing): the list entries past a batch's hit count point at it, so the
+  // walk needs no test of the entry count
+  __shared__ float4 s_stage[NQ][3][65];
+  __shared__ __align__(16) uint32_t s_list[NQ][68];
+
+  const uint32_t total_tiles = A.tiles_x * A.tiles_y;
+  if (blockIdx.x >= total_tiles) return;
+  const uint32_t tile = A.order ? A.order[blockIdx.x] : blockIdx.x;
+  const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+  const uint32_t t = threadIdx.x;
+  const uint32_t wave = t >> 6, lane = t & 63u;
+  const uint32_t qx0 = tx * (8u * QX) + (wave % QX) * 8u, qy0 = ty * (8u * QY) + (wave / QX) * 8u;
+  const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
+  const bool inside = px < A.width && py < A.height;
+
+  const uint32_t rs = A.ranges[tile * 2u], re = A.ranges[tile * 2u + 1u];
+  const bool empty = rs >= re;
+  const uint32_t n = empty ? 0u : re - rs;
+  const float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
+  const float4* geom4 = reinterpret_cast<const float4*>(A.geom);
+  const float4* color4 = reinterpret_cast<const float4*>(A.color);
+  float4(*st)[65] = s_stage[wave];
+  const uint32_t wbase = wave * (uint32_t)sizeof(s_stage[0]);
+
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
+  bool alive = inside;
+  // a quadrant with no pixel on screen has nothing to composite
+  if (!empty && __ballot(alive) != 0ull) {
+    // the records of the batch about to be tested are loaded one batch ahead,
+    // at indexes clamped to the tile's last entry
+    const uint32_t last = re - 1u;
+    uint32_t g = A.sorted_ids[min(rs + lane, last)];
+    float4 r0 = geom4[(size_t)g * 2], r1 = geom4[(size_t)g * 2 + 1], rc = color4[g];
+    uint32_t g_ahead = A.sorted_ids[min(rs + lane + 64u, last)];
+    if (lane == 0) {
+      st[0][64] = make_float4(0.f, 0.f, 0.f, 0.f);
+      st[1][64] = make_float4(0.f, 0.f, 0.f, 0.f);
+      st[2][64] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    for (uint32_t b0 = 0; b0 < n; b0 += 64u) {
+      const uint32_t e = rs + b0 + lane;
+      bool hit = false;
+      __builtin_amdgcn_wave_barrier();
+      if (e < re) {
+        // r0 = mu_x, mu_y, a, b; r1 = c, alpha, tz, q_cut; rc = r, g, b
+        const float op = r1.y > ALPHA_MIN ? r1.y : 0.0f;
+        st[0][lane] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
+        st[1][lane] = make_float4(r0.x, r0.y, rc.x, rc.y);
+        reinterpret_cast<float*>(&st[2][lane])[0] = rc.z;
+        hit = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, qx0, qy0, 8u, 8u,
+                                A.width, A.height);
+      }
+      // the next batch's records load while this one is composited
+      r0 = geom4[(size_t)g_ahead * 2];
+      r1 = geom4[(size_t)g_ahead * 2 + 1];
+      rc = color4[g_ahead];
+      g_ahead = A.sorted_ids[min(e + 128u, last)];
+      const uint64_t m = __ballot(hit);
+      // list entries are byte offsets from s_stage[0] (this wave's region
+      // included), so the walk's LDS addresses need no add
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      if (hit) s_list[wave][cgs::mask_rank(m)] = wbase + lane * 16u;
+      if (lane < (uint32_t)STEP) s_list[wave][cnt + lane] = wbase + 64u * 16u;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+      for (uint32_t i = 0; i < cnt; i += (uint32_t)STEP) {
+        uint32_t k[STEP];
+#pragma unroll
+        for (int j = 0; j < STEP; ++j) k[j] = s_list[wave][i + (uint32_t)j];
+        Splat S[STEP];
+        float q[STEP], al[STEP];
+#pragma unroll
+        for (int j = 0; j < STEP; ++j) {
+          S[j] = staged<65>(s_stage[0], k[j]);
+          q[j] = splat_q(S[j], pcx, pcy);
+          al[j] = fminf(0.99f, S[j].g.w * __builtin_amdgcn_exp2f(q[j] * LOG2E));
+        }
+#pragma unroll
+        for (int j = 0; j < STEP; ++j) blend(S[j], q[j], al[j], true, alive, T, cr, cg, cb);
+        if (__ballot(alive) == 0ull) break;
+      }
+      if (__ballot(alive) == 0ull) break;
+    }
+  }
+
+  if (inside) {
+    const float fr = cgs_fclamp(cr + T * A.bg[0], 0.0f, 1.0f);
+    const float fg = cgs_fclamp(cg + T * A.bg[1], 0.0f, 1.0f);
+    const float fb = cgs_fclamp(cb + T * A.bg[2], 0.0f, 1.0f);
+    float fa = cgs_fclamp(1.0f - T, 0.0f, 1.0f);
+    if (empty) fa = A.bg[3];
+    const uint32_t r8 = (uint32_t)cgs_fclamp(roundf(fr * 255.0f), 0.0f, 255.0f);
+    const uint32_t g8 = (uint32_t)cgs_fclamp(roundf(fg * 255.0f), 0.0f, 255.0f);
+    const uint32_t b8 = (uint32_t)cgs_fclamp(roundf(fb * 255.0f), 0.0f, 255.0f);
+    const uint32_t a8 = (uint32_t)cgs_fclamp(roundf(fa * 255.0f), 0.0f, 255.0f);
+    A.out[(size_t)py * A.padded_row + px] = r8 | (g8 << 8) | (b8 << 16) | (a8 << 24);
+  }
+}
+
 }  // namespace
 
 void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* prof,
@@ -337,21 +453,21 @@ void cgs_launch_composite(const CgsCompositeArgs& a, hipStream_t s, CgsProf* pro
     hipLaunchKernelGGL((k_composite<1, 1, 64, ST, PF>), grid, dim3(64), 0, s, A);
   } else if (A.tile_w == 16u && A.tile_h == 32u) {
     if (batch == 64u)
-      hipLaunchKernelGGL((k_composite<2, 4, 64, ST, PF>), grid, dim3(512), 0, s, A);
+      hipLaunchKernelGGL((k_composite_q<2, 4, ST>), grid, dim3(512), 0, s, A);
     else if (batch == 128u)
       hipLaunchKernelGGL((k_composite<2, 4, 128, ST, PF>), grid, dim3(512), 0, s, A);
     else
       hipLaunchKernelGGL((k_composite<2, 4, 256, ST, PF>), grid, dim3(512), 0, s, A);
   } else if (A.tile_w == 32u && A.tile_h == 16u) {
     if (batch == 64u)
-      hipLaunchKernelGGL((k_composite<4, 2, 64, ST, PF>), grid, dim3(512), 0, s, A);
+      hipLaunchKernelGGL((k_composite_q<4, 2, ST>), grid, dim3(512), 0, s, A);
     else if (batch == 128u)
       hipLaunchKernelGGL((k_composite<4, 2, 128, ST, PF>), grid, dim3(512), 0, s, A);
     else
       hipLaunchKernelGGL((k_composite<4, 2, 256, ST, PF>), grid, dim3(512), 0, s, A);
   } else {
     if (batch == 64u)
-      hipLaunchKernelGGL((k_composite<2, 2, 64, ST, PF>), grid, dim3(256), 0, s, A);
+      hipLaunchKernelGGL((k_composite_q<2, 2, ST>), grid, dim3(256), 0, s, A);
     else if (batch == 128u)
       hipLaunchKernelGGL((k_composite<2, 2, 128, ST, PF>), grid, dim3(256), 0, s, A);
     else if (batch == 512u)

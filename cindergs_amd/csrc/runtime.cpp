@@ -177,7 +177,7 @@ struct FrameArena {
   uint32_t* sorted_ids = nullptr;
   bool capture = false;
   bool captured = false;
-  cgs_tuning tune = {16384u, 8u, 256u, 1u, 0u, 16u, 16u};
+  cgs_tuning tune = {16384u, 8u, 64u, 1u, 0u, 16u, 16u};
   // bucket mode (k_tilesort.hip): histograms, bases, buckets, the sort's
   // tables and control words, and the launch plan of the segmented sort
   DevBuf bk_hist, bk_base, bk_bucket, bk_alt, bk_tab, bk_ctl;

@@ -207,7 +207,7 @@ int cgs_kernel_times(cgs_context* ctx, char* names, double* ms, uint32_t cap, ui
  * frame: with tiles of w x h pixels every output is the reference's algorithm
  * run at that tile size (tile rectangles, tile count, and with it the split of
  * the 32-bit key between tile and depth bits).  Defaults:
- * {16384, 8, 256, 1, 0, 16, 16}; the radix fields matter only with
+ * {16384, 8, 64, 1, 0, 16, 16}; the radix fields matter only with
  * tile_sort = 0. */
 typedef struct cgs_tuning {
   uint32_t radix_partition_keys; /* 16384, 12288, 10240, 8192 or 4096 keys per
@@ -215,7 +215,11 @@ typedef struct cgs_tuning {
                                     keys; 640, 512 or 256 threads x 16) */
   uint32_t radix_digit_bits;     /* 8 or 4 (32-bit keys; the 64-bit path uses 8) */
   uint32_t composite_batch;      /* 64, 128, 256 or 512 splats per compositor
-                                    step (8 x 8 tiles: 64; 512: 16 x 16 tiles) */
+                                    step (8 x 8 tiles: 64; 512: 16 x 16 tiles).
+                                    64 on larger tiles: each 8 x 8 quadrant's
+                                    wave walks the tile's list on its own in
+                                    batches of 64, with no block barrier; the
+                                    others stage each batch once per block */
   uint32_t tile_order;           /* 1: heaviest tiles composited first */
   uint32_t tile_sort;            /* 1: entries written into per-tile buckets and each
                                     tile sorted by (key, splat id) in LDS; 0: one

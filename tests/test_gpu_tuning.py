@@ -54,7 +54,7 @@ def test_every_candidate_gives_the_same_frame(gpu):
     cam, comp = syn.view_orbit(7, 1280, 720)
     default = gpu.get_tuning()
     assert default.as_dict() == {"radix_partition_keys": 16384, "radix_digit_bits": 8,
-                                 "composite_batch": 256, "tile_order": 1, "tile_sort": 0,
+                                 "composite_batch": 64, "tile_order": 1, "tile_sort": 0,
                                  "tile_w": 16, "tile_h": 16}
     refs = {(16, 16): _frame(gpu, cam, comp, scene.count)}
     n = tuning_candidates_count()
