@@ -30,6 +30,25 @@ __device__ __forceinline__ float wave_sum(float v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Sum over the wave with DPP row operations (six adds, then lane 63 read
+// to a scalar): quad swaps, half-row and row mirrors, then the row-15 and
+// row-31 broadcasts carry the row sums to the last row.  Used where the
+// summation order is free (the learned binding); wave_sum keeps the
+// butterfly order of the exact kernels.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += dpp_f32<0xB1>(v);        // quad_perm [1, 0, 3, 2]
+  v += dpp_f32<0x4E>(v);        // quad_perm [2, 3, 0, 1]
+  v += dpp_f32<0x141>(v);       // row_half_mirror
+  v += dpp_f32<0x140>(v);       // row_mirror: every lane holds its row's sum
+  v += dpp_f32<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f32<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
@@ -625,13 +644,15 @@ __device__ __forceinline__ int jsw_terms_fast(V3f x, V3f v0, V3f v1, V3f v2, flo
 // lanes so that every lane does about the same number of JSW evaluations;
 // the per-vertex sums of the seven evaluations live in a small LDS hash table
 // keyed by vertex (NEAR_SLOTS slots, open addressing; a point whose near
-// triangles touch more vertices than fit is flagged for the exact path), so
-// a wave needs 9 KiB of LDS whatever the cage size.  The network's rows are
-// loaded into registers (VPL vertices per lane) before the near loop, and
-// the weights stay there until they are written.
-constexpr uint32_t NEAR_SLOTS = 256u, NEAR_TRIS = 256u, NEAR_EMPTY = 0xFFFFFFFFu;
+// triangles touch more vertices than fit is flagged for the exact path: the
+// ~60 near triangles of the 642-vertex cage touch ~40 vertices), so a wave
+// needs 6 KiB of LDS whatever the cage size.  The network's rows are loaded
+// into registers (VPL vertices per lane) after the near loop, and the
+// weights stay there until they are written.  Five waves per SIMD (96 VGPRs,
+// a few spilled) measured 3 % faster than four; the reductions are DPP sums.
+constexpr uint32_t NEAR_SLOTS = 128u, NEAR_TRIS = 256u, NEAR_EMPTY = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t near_hash(uint32_t v) { return (v * 0x9E3779B1u) >> 24; }
+__device__ __forceinline__ uint32_t near_hash(uint32_t v) { return (v * 0x9E3779B1u) >> 25; }
 
 // slot of vertex v (inserted when absent); NEAR_EMPTY when the table is full
 __device__ __forceinline__ uint32_t near_slot_insert(uint32_t* keys, uint32_t v) {
@@ -685,7 +706,7 @@ __device__ __forceinline__ int segment_crosses(V3f p, V3f c, V3f v0, V3f v1, V3f
 }
 
 template <int VPL>
-__global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_mvc_nearfar(CgsNearArgs A) {
   __shared__ uint32_t s_key[NEAR_SLOTS];
   __shared__ float s_val[7][NEAR_SLOTS];
   __shared__ uint32_t s_tri[NEAR_TRIS];
@@ -732,7 +753,7 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
       crossings += k == 1 ? 1u : 0u;
       unsure |= k == 2 ? 1u : 0u;
     }
-    const bool odd = ((uint32_t)wave_sum((float)crossings) & 1u) != 0u;
+    const bool odd = ((uint32_t)wave_sum_dpp((float)crossings) & 1u) != 0u;
     degenerate = __ballot(unsure != 0u) != 0ull ? 1u : 0u;
     in = degenerate ? true : ((ref == 1u) != odd);
   }
@@ -825,7 +846,7 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
   const float tau2 = A.tau * A.tau;
 #pragma unroll
   for (int e = 0; e < 7; ++e) {
-    const float Ue = (uint32_t)e < evals ? wave_sum(U[e]) : 0.0f;
+    const float Ue = (uint32_t)e < evals ? wave_sum_dpp(U[e]) : 0.0f;
     inv[e] = Ue / (Ue * Ue + tau2);
   }
   float wv[VPL], gv[3][VPL];
@@ -846,10 +867,10 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) sdq[a] += gv[a][k];
   }
-  const float keep = 1.0f - wave_sum(sq), inv_2h = 0.5f / A.h;
+  const float keep = 1.0f - wave_sum_dpp(sq), inv_2h = 0.5f / A.h;
   float dQ[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) dQ[a] = tan ? wave_sum(sdq[a]) : 0.0f;
+  for (int a = 0; a < 3; ++a) dQ[a] = tan ? wave_sum_dpp(sdq[a]) : 0.0f;
   __syncthreads();
   // the combined rows and the sums of the projection (sum w, sum w v; the
   // same for each g_a)
@@ -882,12 +903,12 @@ __global__ __launch_bounds__(64) void k_mvc_nearfar(CgsNearArgs A) {
     }
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) m4[k] = wave_sum(m4[k]);
+  for (int k = 0; k < 4; ++k) m4[k] = wave_sum_dpp(m4[k]);
   if (grow)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) mg[a][k] = wave_sum(mg[a][k]);
+      for (int k = 0; k < 4; ++k) mg[a][k] = wave_sum_dpp(mg[a][k]);
   if (lane == 0) {
     const float rx = m4[1] - p.x, ry = m4[2] - p.y, rz = m4[3] - p.z;
     A.residual[i] = degen ? __builtin_inff() : sqrtf((rx * rx + ry * ry) + rz * rz);
