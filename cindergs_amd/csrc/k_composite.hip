@@ -126,6 +126,8 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t* ranges, uin
   const uint32_t t0 = min(t * per, tiles), t1 = min(t0 + per, tiles);
   uint64_t cls = 0;  // 2 bits per owned tile
   uint32_t c01 = 0, c23 = 0;
+  // unrolled so that a thread's range loads are in flight together
+#pragma unroll 8
   for (uint32_t i = t0; i < t1; ++i) {
     const uint2 r = reinterpret_cast<const uint2*>(ranges)[i];
     const uint32_t n = r.y > r.x ? r.y - r.x : 0u;
