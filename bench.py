@@ -131,9 +131,11 @@ def composite_valu(tuning: str, per_launch_ms: float) -> dict | None:
            "lds_bank_conflict_cycles": int(c["SQ_LDS_BANK_CONFLICT"][k]),
            "waves": int(c["SQ_WAVES"][k])}
     if vgpr:
-        # 512 VGPRs per SIMD lane, allocated in granules of 8; at most 8 waves
-        g = int(-(-vgpr // 8) * 8)
-        res["vgprs"] = int(vgpr)
+        # rocprofv3's VGPR_Count counts register pairs on gfx950 (68 VGPRs in
+        # the compiler's resource report read 36); 512 VGPRs per SIMD lane,
+        # allocated in granules of 8; at most 8 waves
+        g = int(-(-(2 * vgpr) // 8) * 8)
+        res["vgprs"] = int(2 * vgpr)
         res["occupancy_waves_per_simd"] = min(8, 512 // g)
     return res
 
