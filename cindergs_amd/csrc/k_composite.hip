@@ -102,6 +102,20 @@ __device__ __forceinline__ void blend(const Splat& s, float q, float a, bool val
   alive = alive && !sat;
 }
 
+// blend for a pixel whose liveness is T > 0 (k_composite_q): the same
+// arithmetic as blend() on a live pixel; a pixel at T = 0 adds w = 0 and
+// stays at 0
+__device__ __forceinline__ void blend_t(const Splat& s, float q, float a, float& T, float& cr,
+                                        float& cg, float& cb) {
+  const bool ok = q >= -4.0f && a >= ALPHA_MIN;
+  const float w = (ok ? a : 0.0f) * T;
+  cr = __builtin_fmaf(w, s.m.z, cr);
+  cg = __builtin_fmaf(w, s.m.w, cg);
+  cb = __builtin_fmaf(w, s.b, cb);
+  const float Tn = T - w;
+  T = Tn < 0.001f ? 0.0f : Tn;
+}
+
 template <int CB>
 __device__ __forceinline__ Splat staged(const float4 (*st)[CB], uint32_t off) {
   const char* p = reinterpret_cast<const char*>(&st[0][0]) + off;
@@ -359,10 +373,11 @@ __global__ __launch_bounds__(64 * QX * QY) void k_composite_q(CgsCompositeArgs A
   float4(*st)[65] = s_stage[wave];
   const uint32_t wbase = wave * (uint32_t)sizeof(s_stage[0]);
 
-  float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
-  bool alive = inside;
+  // a pixel is live while T > 0: an off-screen pixel starts at T = 0 and a
+  // saturated one is set to 0, and a pixel at T = 0 blends nothing (w = a T)
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = inside ? 1.0f : 0.0f;
   // a quadrant with no pixel on screen has nothing to composite
-  if (!empty && __ballot(alive) != 0ull) {
+  if (!empty && __ballot(inside) != 0ull) {
     // the records of the batch about to be tested are loaded one batch ahead,
     // at indexes clamped to the tile's last entry
     const uint32_t last = re - 1u;
@@ -413,10 +428,10 @@ __global__ __launch_bounds__(64 * QX * QY) void k_composite_q(CgsCompositeArgs A
           al[j] = fminf(0.99f, S[j].g.w * __builtin_amdgcn_exp2f(q[j] * LOG2E));
         }
 #pragma unroll
-        for (int j = 0; j < STEP; ++j) blend(S[j], q[j], al[j], true, alive, T, cr, cg, cb);
-        if (__ballot(alive) == 0ull) break;
+        for (int j = 0; j < STEP; ++j) blend_t(S[j], q[j], al[j], T, cr, cg, cb);
+        if (__ballot(T > 0.0f) == 0ull) break;
       }
-      if (__ballot(alive) == 0ull) break;
+      if (__ballot(T > 0.0f) == 0ull) break;
     }
   }
 
