@@ -66,6 +66,31 @@ __device__ __forceinline__ bool ellipse_hits_rect(float mu_x, float mu_y, float 
   return cgs_fmin(q1, q2) <= qcut + 1e-4f;
 }
 
+// The same test with the two divisions as hardware reciprocals and the
+// margin doubled (2e-4): it accepts every splat the exact test accepts (a
+// reciprocal moves the line minimum y1 or x2 by about one ulp, which changes
+// q there only at second order, ~1e-8 of q).  The compositor may take a
+// superset of a quadrant's splats: each pixel still applies the exact q and
+// alpha cuts, so the frame is the same bit for bit.
+__device__ __forceinline__ bool ellipse_may_hit_rect(float mu_x, float mu_y, float a, float b,
+                                                     float c, float alpha, float qcut,
+                                                     uint32_t xs, uint32_t ys, uint32_t w,
+                                                     uint32_t h, uint32_t W, uint32_t H) {
+  if (alpha < (1.0f / 255.0f) || xs >= W || ys >= H) return false;
+  const uint32_t xe = min(xs + w - 1u, W - 1u), ye = min(ys + h - 1u, H - 1u);
+  const float dx0 = ((float)xs + 0.5f) - mu_x;
+  const float dx1 = ((float)xe + 0.5f) - mu_x;
+  const float dy0 = ((float)ys + 0.5f) - mu_y;
+  const float dy1 = ((float)ye + 0.5f) - mu_y;
+  const float x_near = cgs_fclamp(0.0f, dx0, dx1);
+  const float y1 = cgs_fclamp(-b * x_near * __builtin_amdgcn_rcpf(cgs_fmax(c, 1e-12f)), dy0, dy1);
+  const float q1 = a * x_near * x_near + 2.0f * b * x_near * y1 + c * y1 * y1;
+  const float y_near = cgs_fclamp(0.0f, dy0, dy1);
+  const float x2 = cgs_fclamp(-b * y_near * __builtin_amdgcn_rcpf(cgs_fmax(a, 1e-12f)), dx0, dx1);
+  const float q2 = a * x2 * x2 + 2.0f * b * x2 * y_near + c * y_near * y_near;
+  return cgs_fmin(q1, q2) <= qcut + 2e-4f;
+}
+
 // One splat's compositor inputs as staged in LDS.
 struct Splat {
   float4 g;  // -a/2, -b, -c/2, opacity (0 when <= 1/255)
@@ -399,8 +424,8 @@ __global__ __launch_bounds__(64 * QX * QY) void k_composite_q(CgsCompositeArgs A
         st[0][lane] = make_float4(-0.5f * r0.z, -r0.w, -0.5f * r1.x, op);
         st[1][lane] = make_float4(r0.x, r0.y, rc.x, rc.y);
         reinterpret_cast<float*>(&st[2][lane])[0] = rc.z;
-        hit = ellipse_hits_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, qx0, qy0, 8u, 8u,
-                                A.width, A.height);
+        hit = ellipse_may_hit_rect(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.w, qx0, qy0, 8u, 8u,
+                                   A.width, A.height);
       }
       // the next batch's records load while this one is composited
       r0 = geom4[(size_t)g_ahead * 2];
