@@ -37,20 +37,22 @@ struct TileBox {
 };
 
 // compute_gaussian_tile_bounds_xy (tiles/mod.rs:97-130, the GPU form in
-// binning.rs:52-63) with the tile size as a parameter: the division by the
-// tile width or height is exact for the power-of-two sizes, so at 16 x 16 the
-// box is the reference's bit for bit
+// binning.rs:52-63) with the tile size as a parameter.  The tile sizes are
+// powers of two (8, 16, 32), so x / w and x * (1 / w) are the same number
+// (1 / w is exact and the product is the exact quotient, rounded once): the
+// box is the reference's division bit for bit, without the division
+// sequence.  inv_tw, inv_th = 1 / tile width, 1 / tile height.
 __device__ __forceinline__ TileBox tile_box(float mu_x, float mu_y, uint32_t packed,
-                                            uint32_t tiles_x, uint32_t tiles_y, float tw,
-                                            float th) {
+                                            uint32_t tiles_x, uint32_t tiles_y, float inv_tw,
+                                            float inv_th) {
   const float ext_x = (float)(packed & 0xFFFFu);
   const float ext_y = (float)((packed >> 16) & 0xFFFFu);
   const int32_t mtx = (int32_t)tiles_x - 1, mty = (int32_t)tiles_y - 1;
   TileBox b;
-  b.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) / tw)), 0, mtx);
-  b.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) / tw)), 0, mtx);
-  b.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) / th)), 0, mty);
-  b.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) / th)), 0, mty);
+  b.tx0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x - ext_x) * inv_tw)), 0, mtx);
+  b.tx1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_x + ext_x) * inv_tw)), 0, mtx);
+  b.ty0 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y - ext_y) * inv_th)), 0, mty);
+  b.ty1 = (uint32_t)cgs_iclamp(cgs_f2i_sat(floorf((mu_y + ext_y) * inv_th)), 0, mty);
   return b;
 }
 
@@ -115,7 +117,9 @@ __device__ __forceinline__ BinSplat load_bin_splat(const CgsBinArgs& A, uint32_t
   const float4 g0 = g[0], g1 = g[1];
   b.d0 = g0;
   b.d1 = make_float4(g1.x, g1.y, g1.w, cgs_u2f(splat_key_bits(A, g1.z)));
-  b.box = tile_box(g0.x, g0.y, packed, A.tiles_x, A.tiles_y, (float)A.tile_w, (float)A.tile_h);
+  // tile sizes are powers of two (checked by cgs_set_tuning): exact reciprocals
+  b.box = tile_box(g0.x, g0.y, packed, A.tiles_x, A.tiles_y, 1.0f / (float)A.tile_w,
+                   1.0f / (float)A.tile_h);
   // a splat below the opacity cut hits no tile (hits() rejects it first)
   if (g1.y >= (1.0f / 255.0f))
     b.area = (b.box.tx1 - b.box.tx0 + 1u) * (b.box.ty1 - b.box.ty0 + 1u);
