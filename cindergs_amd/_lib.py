@@ -167,6 +167,8 @@ class BatchStats(C.Structure):
         ("reruns", C.c_uint32),
         ("total_tile_entries", C.c_uint64),
         ("gpu_ms", C.c_double),
+        ("presort_views", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -271,6 +271,8 @@ typedef struct cgs_batch_stats {
   uint32_t reruns;             /* replays after a lane's tile-entry arena grew */
   uint64_t total_tile_entries; /* summed over the views */
   double gpu_ms;               /* device time of the graph replay(s) */
+  uint32_t presort_views;      /* views sorted with the splats presorted by depth */
+  uint32_t reserved;
 } cgs_batch_stats;
 
 /* lanes in [1, 16]: views run on that many concurrent graph branches, each
@@ -280,7 +282,10 @@ void cgs_batch_destroy(cgs_batch* batch);
 
 /* View v renders scenes[view_scene[v]] with cameras[v] and compositors[v];
  * params are shared (profile_stages and force_stage3 are ignored: every view
- * runs Stage 3).  Waits for the views to finish. */
+ * runs Stage 3).  Waits for the views to finish.  With the LSD sort
+ * (tile_sort 0), a view whose last replay in the same batch had more than 4
+ * tile entries per splat slot is sorted with the splats presorted by depth
+ * (its keys, ids, ranges and pixels are the same). */
 int cgs_execute_views_batched(cgs_batch* batch, cgs_context* const* scenes, uint32_t n_scenes,
                               const uint32_t* view_scene, const cgs_camera_uniforms* cameras,
                               const cgs_compositor_uniforms* compositors, uint32_t n_views,

@@ -181,3 +181,32 @@ def test_batch_deformed_scene_and_8k_view():
         _check_batch(batch, states, view_scene, cams, comps, p, "deformed + 8K")
     for st in states:
         st.close()
+
+
+def test_batch_presorts_views_with_many_entries_per_splat():
+    """A view whose last replay had more than 4 tile entries per splat slot is
+    sorted with the splats presorted by depth (cgs_execute_views_batched):
+    the second call takes that path for the dense views, and every view stays
+    bit-identical to its per-view frame; views with few entries per splat keep
+    the LSD sort."""
+    # a small dense ball seen large (many tiles per splat) beside a sparse view
+    scenes = [syn.scene_gaussian_ball(6_000, k=37), syn.scene_clusters(40_000, k=38)]
+    states = _states(scenes)
+    view_scene = [0, 1, 0]
+    cams, comps = [], []
+    for eye_z, (w, h) in ((-1.2, (1920, 1080)), (-3.0, (320, 240)), (-2.0, (800, 600))):
+        eye = (0.1, 0.05, eye_z)
+        cams.append(syn.camera_look_at(eye, (0, 0, 0), (0, 1, 0), 60.0, w, h))
+        comps.append(syn.compositor(w, h, eye))
+    p = _params()
+    with ViewBatch(0, lanes=2) as batch:
+        s1 = _check_batch(batch, states, view_scene, cams, comps, p, "presort first")
+        assert s1.presort_views == 0  # nothing measured yet
+        s2 = _check_batch(batch, states, view_scene, cams, comps, p, "presort second")
+        per_splat = [batch.view_counts(v)[0] / scenes[view_scene[v]].count for v in range(3)]
+        print(f"[case] entries per splat {per_splat}", flush=True)
+        assert s2.presort_views == sum(1 for r in per_splat if r > 4.0) >= 1
+        s3 = _check_batch(batch, states, view_scene, cams, comps, p, "presort replay")
+        assert s3.graph_built == 0 and s3.presort_views == s2.presort_views
+    for st in states:
+        st.close()
