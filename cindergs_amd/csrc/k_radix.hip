@@ -3,15 +3,16 @@
 // Replaces gpu_radix_sort_u32_pairs (src/render/kernels/radix_sort.rs:292-368):
 // the reference runs 4-bit digits, each pass as histogram + 3-level scan +
 // scatter with 16 per-digit block scans.  Here a pass is ONE kernel (the
-// "onesweep" scheme): 8-bit digits; a partition of 4096 keys is ranked with
-// wave64 ballots (8 ballots give the mask of lanes sharing a digit) and
-// per-wave LDS digit counters, partitions chain their per-digit counts with a
-// decoupled look-back, keys are staged in LDS in digit order and written out
-// as contiguous runs.  All digit histograms of all passes come from one
-// upfront read of the keys (or from the binning pass that emitted them) as
-// per-block partials summed by a small reduction: no global atomics besides
-// one partition ticket per live 8192-key partition.  Ranking keeps the input order of equal digits,
-// so each pass and the whole sort are stable, which is the reference's order.
+// "onesweep" scheme): 8-bit (or 4-bit) digits; a partition of 4096 to 16384
+// keys (a tuning choice) is ranked with wave64 ballots (8 ballots give the
+// mask of lanes sharing a digit) and per-wave LDS digit counters, partitions
+// chain their per-digit counts with a decoupled look-back, keys are staged in
+// LDS in digit order and written out as contiguous runs.  All digit
+// histograms of all passes come from one upfront read of the keys (or from
+// the binning pass that emitted them) as per-block partials summed by a small
+// reduction: no global atomics besides one partition ticket per live
+// partition.  Ranking keeps the input order of equal digits, so each pass and
+// the whole sort are stable, which is the reference's order.
 #include "cgs_internal.h"
 
 namespace {
