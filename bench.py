@@ -201,6 +201,34 @@ def hbm_traffic(kernel: str, calib_mib: int = 256, tuning: str = "") -> dict | N
     return res
 
 
+def copy_bandwidth(mib: int = 2048, reps: int = 10) -> dict:
+    """A measured copy rate beside the 8 TB/s datasheet peak: the best of
+    `reps` device-to-device hipMemcpyAsync copies of `mib` MiB (read + write
+    bytes over host time around the copy and one stream synchronisation)."""
+    import ctypes as C
+
+    from cindergs_amd import GpuPipelineState
+
+    st = GpuPipelineState(0)
+    nbytes = mib << 20
+    src = st.to_device(np.zeros(nbytes // 4, dtype=np.uint32))
+    dst = st.to_device(np.zeros(nbytes // 4, dtype=np.uint32))
+    best = float("inf")
+    for _ in range(reps):
+        st.synchronize()
+        t = time.perf_counter()
+        rc = st._lib.cgs_memcpy_d2d(st._h, dst.ptr, src.ptr, C.c_uint64(nbytes))
+        best = min(best, time.perf_counter() - t)
+        if rc != 0:
+            raise RuntimeError(f"cgs_memcpy_d2d returned {rc}")
+    src.free()
+    dst.free()
+    st.close()
+    return {"GB/s": round(2.0 * nbytes / best / 1e9, 1), "bytes_moved": 2 * nbytes,
+            "method": f"device-to-device hipMemcpyAsync of {mib} MiB, best of {reps}, host-timed "
+                      "with one stream synchronisation"}
+
+
 def cpu_info() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -1164,6 +1192,13 @@ def main() -> int:
                      "avg_launch_ms": round(per_launch_ms, 5)},
     }
     if not args.no_sort_baseline:
+        # the measured copy rate beside the datasheet peak (the child runs
+        # of the profiler skip it with the sort baseline)
+        cb = copy_bandwidth()
+        result["hbm_copy"] = cb
+        for st_name in ("stage2", "stage3"):
+            result["stages"][st_name]["frac_of_copy"] = round(
+                result["stages"][st_name]["GB/s"] / cb["GB/s"], 3)
         result["sort_baseline"] = sort_baseline(
             st, E, T, kern.get("k_radix_onesweep", 0.0) + kern.get("k_hist_reduce", 0.0))
     st.close()
