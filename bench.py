@@ -479,7 +479,7 @@ def export_deformed(st, scene, num_tets: int, bind_tet, reps: int = 5):
 
 
 def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
-                residual_tol: float = 1e-2, n_err: int = 200_000):
+                residual_tol: float = 1e-2, n_err: int = 200_000, far_kind: int = 0):
     """Exact MVC binding (weights + gradients) of n_points inside a 642-vertex
     icosphere cage against the learned binding: the network (Fourier features
     -> 4 x 256 -> 642) trained on the far share of MVC's sums (triangles more
@@ -488,7 +488,9 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
     residual_tol of the cage diagonal) to exact MVC.  Output buffers are
     allocated before the timed calls; the position errors (deformed positions
     under a seeded 0.1 perturbation of the cage, against exact MVC) are
-    measured on the first n_err points, the Jacobian errors likewise."""
+    measured on the first n_err points, the Jacobian errors likewise.
+    far_kind 1 trains the network on the far sums instead of the far share
+    (cgs_mlp_set_far_kind)."""
     import ctypes as C
 
     from cindergs_amd import GpuPipelineState
@@ -521,11 +523,12 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
     d_tr = st.to_device(pts[n_points:])
     d_qt = dalloc(n_train * nv * 4)
     t = time.perf_counter()
-    check(lib.cgs_mvc_far_share(st._h, cv, nv, ct, nt, near[0], near[1], d_tr.ptr, n_train, d_qt))
+    check(lib.cgs_mvc_far_targets(st._h, cv, nv, ct, nt, near[0], near[1], d_tr.ptr, n_train,
+                                  far_kind, d_qt))
     st.synchronize()
     targets_s = time.perf_counter() - t
     net = MvcSurrogate(st, nv, fourier_sigma=1.0, seed=7)
-    net.set_near_field(*near)
+    net.set_near_field(*near, kind=far_kind)
     st.synchronize()
     t = time.perf_counter()
     segments = 8
@@ -577,8 +580,8 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
     # each point's residual and its (projected) weights
     d_q, d_r = dalloc(n_e * nv * 4), dalloc(n_e * 4)
     check(lib.cgs_mlp_infer(net._h, d_pts.ptr, n_e, d_q, None))
-    check(lib.cgs_mvc_nearfar_bind(st._h, cv, nv, ct, nt, near[0], near[1], d_pts.ptr, n_e, d_q,
-                                   None, d_wn, None, d_ms, d_r))
+    check(lib.cgs_mvc_nearfar_bind_kind(st._h, cv, nv, ct, nt, near[0], near[1], d_pts.ptr, n_e,
+                                        d_q, None, far_kind, d_wn, None, d_ms, d_r))
     resid = np.empty(n_e, np.float32)
     check(lib.cgs_memcpy_d2h(st._h, _ptr(resid), d_r, resid.nbytes))
     e_nf = np.linalg.norm(pe - host(d_wn, n_e) @ dv, axis=1) / diag
@@ -601,6 +604,7 @@ def mvc_binding(n_points: int, train_steps: int, n_train: int, near=(0.25, 0.5),
            "exact_ms": round(exact_ms, 2), "surrogate_bind_ms": round(sur_ms, 2),
            "speedup": round(exact_ms / sur_ms, 2),
            "near_field": list(near), "residual_tol": residual_tol,
+           "far_target": ("far share", "far sums")[far_kind],
            "fallback_fraction": round(fb.value / n_points, 4), "train_steps": train_steps,
            "train_samples": n_train, "train_s": round(train_s, 2),
            "targets_s": round(targets_s, 2),
@@ -791,11 +795,13 @@ def _winding(ex, st, p):
     return w
 
 
-def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
+def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None, per_view=True):
     """Views/s of the view batch (one HIP graph per batch) against per-view
     cgs_execute_frame calls over the same views: 3 scenes (200k, 1.07M, 4M
     splats, Gaussian-ball distribution), n_views seeded orbit cameras with
-    resolutions drawn from 320x240 / 800x600 / 1920x1080 / 3840x2160."""
+    resolutions drawn from 320x240 / 800x600 / 1920x1080 / 3840x2160.
+    per_view=False times the batch alone (for a kernel trace of the batch's
+    own launches); the per-view fields are then null."""
     from cindergs_amd import FrameParams, GpuPipelineState, StageTimings, ViewBatch
     from cindergs_amd import synthetic as syn
 
@@ -817,23 +823,26 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
         for v in range(n_views):
             states[view_scene[v]].execute_frame_resident(cams[v], comps[v], params, tim)
 
-    per_view_pass()  # warm-up (arena growth)
-    for stt in states:
-        stt.synchronize()
-    t = time.perf_counter()
-    for _ in range(reps):
-        per_view_pass()
-    for stt in states:
-        stt.synchronize()
-    per_view_s = (time.perf_counter() - t) / reps
-    print(f"# per-view: {1000 * per_view_s:.2f} ms per {n_views} views", file=sys.stderr, flush=True)
+    per_view_s = None
+    if per_view:
+        per_view_pass()  # warm-up (arena growth)
+        for stt in states:
+            stt.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            per_view_pass()
+        for stt in states:
+            stt.synchronize()
+        per_view_s = (time.perf_counter() - t) / reps
+        print(f"# per-view: {1000 * per_view_s:.2f} ms per {n_views} views", file=sys.stderr,
+              flush=True)
     # the device time of the same views (HIP events around the stages of each
     # frame, summed): against the per-view wall time it shows whether the
     # per-view loop leaves the GPU idle between frames
     prof = FrameParams(0.0, 0.0, 1.0 / 255.0, 0.5, 1, 0)
     device_ms = 0.0
     by_class = {}
-    for v in range(n_views):
+    for v in range(n_views if per_view else 0):
         states[view_scene[v]].execute_frame_resident(cams[v], comps[v], prof, tim)
         ms = (tim.stage1_deform_ms + tim.stage2_project_ms + tim.stage3_bin_sort_ms
               + tim.stage4_composite_ms)
@@ -842,8 +851,9 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
         c = by_class.setdefault(key, {"views": 0, "device_ms": 0.0})
         c["views"] += 1
         c["device_ms"] = round(c["device_ms"] + ms, 3)
-    print(f"# per-view device time: {device_ms:.2f} ms per {n_views} views", file=sys.stderr,
-          flush=True)
+    if per_view:
+        print(f"# per-view device time: {device_ms:.2f} ms per {n_views} views", file=sys.stderr,
+              flush=True)
 
     from cindergs_amd.pipeline import tuning_candidate, tuning_candidates_count
 
@@ -907,13 +917,13 @@ def batched_views(n_views: int, reps: int, lanes_list, splats=None, fixed=None):
     res = {"views": n_views, "reps": reps,
            "scenes": [s.count for s in scenes],
            "resolutions": sorted({f"{c.screen_width}x{c.screen_height}" for c in comps}),
-           "per_view_views_per_s": round(n_views / per_view_s, 1),
-           "per_view_device_ms": round(device_ms, 3),
-           "per_view_wall_ms": round(1000 * per_view_s, 3),
+           "per_view_views_per_s": round(n_views / per_view_s, 1) if per_view else None,
+           "per_view_device_ms": round(device_ms, 3) if per_view else None,
+           "per_view_wall_ms": round(1000 * per_view_s, 3) if per_view else None,
            "per_view_device_ms_by_class": dict(sorted(by_class.items(),
                                                       key=lambda kv: -kv[1]["device_ms"])),
            "batched_views_per_s": round(n_views / best[1], 1),
-           "speedup": round(per_view_s / best[1], 3),
+           "speedup": round(per_view_s / best[1], 3) if per_view else None,
            "lanes": best[0], "tile_entries_per_batch": best[2],
            "tuning": best[3].as_dict() if best[3] is not None else "default",
            "sweep_ms_per_batch": sweep}
@@ -985,6 +995,8 @@ def main() -> int:
     ap.add_argument("--batch-views", type=int, default=64,
                     help="views of the batched multi-view measurement (0 = skip it)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--batch-no-per-view", action="store_true",
+                    help="with --batch-only: time the batch alone, without the per-view frames")
     ap.add_argument("--batch-lanes", type=str, default="4,8,16")
     ap.add_argument("--batch-fixed", type=str, default="", metavar="LANES,PART,BATCH,ORDER[,SORT]",
                     help="batched views: time only this lane count and tuning (no sweep); "
@@ -1013,6 +1025,11 @@ def main() -> int:
                     help="near/far split radii of the learned binding")
     ap.add_argument("--mvc-tol", type=float, default=1e-2,
                     help="residual tolerance of the learned binding (x cage diagonal)")
+    ap.add_argument("--mvc-far", choices=("share", "sums"), default="sums",
+                    help="what the learned binding's network learns: the far share of the "
+                         "weights or the far sums")
+    ap.add_argument("--mvc-only", action="store_true",
+                    help="only the exact and learned MVC binding leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -1044,7 +1061,15 @@ def main() -> int:
     if args.batch_only:
         lanes = [int(x) for x in args.batch_lanes.split(",") if x]
         print(json.dumps({"batched": batched_views(args.batch_views, args.batch_reps, lanes,
+                                                   per_view=not args.batch_no_per_view,
                                                    fixed=fixed)}), flush=True)
+        return 0
+    if args.mvc_only:
+        near = tuple(float(v) for v in args.mvc_near.split(","))
+        print(json.dumps({"mvc": mvc_binding(args.mvc, args.mvc_steps, args.mvc_train, near=near,
+                                             residual_tol=args.mvc_tol,
+                                             far_kind=("share", "sums").index(args.mvc_far))}),
+              flush=True)
         return 0
     if args.mvc_scene_only:
         print(json.dumps({"mvc_scene": mvc_scene(args.mvc_scene, 5, 16384)}), flush=True)
@@ -1243,7 +1268,8 @@ def main() -> int:
     if args.mvc > 0:
         near = tuple(float(v) for v in args.mvc_near.split(","))
         result["mvc"] = mvc_binding(args.mvc, args.mvc_steps, args.mvc_train, near=near,
-                                    residual_tol=args.mvc_tol)
+                                    residual_tol=args.mvc_tol,
+                                    far_kind=("share", "sums").index(args.mvc_far))
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.warmup, args.cpu_budget)
     else:

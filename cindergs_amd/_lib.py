@@ -13,6 +13,9 @@ LIB_PATH = os.path.join(_HERE, "libcindergs_hip.so")
 
 # status codes (cgs_status)
 CGS_OK = 0
+# what a near/far network learns (cgs_mlp_set_far_kind)
+CGS_FAR_SHARE = 0
+CGS_FAR_SUMS = 1
 CGS_ERR_RUNTIME = 1
 CGS_ERR_NO_DEVICE = 2
 CGS_ERR_UNSUPPORTED = 3
@@ -241,6 +244,10 @@ SIGNATURES = {
     "cgs_mvc_nearfar_bind": [_vp, _vp, _u32, _vp, _u32, _f, _f, _vp, _u64, _vp, _vp, _vp, _vp, _vp,
                              _vp],
     "cgs_mlp_set_near_field": [_vp, _f, _f],
+    "cgs_mlp_set_far_kind": [_vp, _u32],
+    "cgs_mvc_far_targets": [_vp, _vp, _u32, _vp, _u32, _f, _f, _vp, _u64, _u32, _vp],
+    "cgs_mvc_nearfar_bind_kind": [_vp, _vp, _u32, _vp, _u32, _f, _f, _vp, _u64, _vp, _vp, _u32,
+                                  _vp, _vp, _vp, _vp],
     "cgs_mvc_winding": [_vp, _vp, _u32, _vp, _u32, _vp, _u64, _vp],
     "cgs_mvc_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp],
     "cgs_mvc_deform_scene": [_vp, _vp, _vp, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _u32,

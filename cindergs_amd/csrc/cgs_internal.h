@@ -493,10 +493,12 @@ void cgs_launch_mvc_select(const float* winding, const float* residual, float to
 // Near/far split of the MVC sums (k_mvc.hip): triangle t's terms at point x
 // carry the weight phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)) (c_t
 // the triangle's centroid): the near sums u_near = sum_t phi_t mu_t are
-// evaluated exactly, the far share q = (u - u_near) / sum(u) is learned.
-// Training targets: q * q_scale per point, [n x nv], degenerate points zero.
+// evaluated exactly, the far share q = (u - u_near) / sum(u) is learned, or
+// the far sums u_far = u - u_near themselves (sums = 1).
+// Training targets: q * q_scale (or u_far * q_scale) per point, [n x nv],
+// degenerate points zero.
 void cgs_launch_mvc_far_share(const CgsMvcArgs& a, const float* cent, float r0, float r1,
-                              float q_scale, float* out, hipStream_t s);
+                              float q_scale, int sums, float* out, hipStream_t s);
 // winding numbers, one point per lane (the same sum as winding_number,
 // mvc.rs:326-347, accumulated in triangle order)
 void cgs_launch_mvc_winding_lane(const float* verts, const uint32_t* tris, uint32_t nt,
@@ -519,12 +521,14 @@ struct CgsNearArgs {
   const uint32_t* cell_inside; // [cells] 1 = cell centre inside the cage (parity
                                //   reference), 2 = undecided (points there are
                                //   flagged for the exact path)
-  const float* Q;              // [n x ldq] far share * q_scale
+  const float* Q;              // [n x ldq] far share * q_scale (far_sums: u_far * q_scale)
   const float* DQ;             // [3n x ldq] its derivatives (row 3p + axis) or null
   const uint16_t* Qh;          // the same in bf16 (used when set)
   const uint16_t* DQh;
   uint32_t ldq;
   float q_scale;
+  uint32_t far_sums;           // 1: Q holds the far sums, w = (u_near + u_far) / sum
+  float sums_scale;            // the far sums' target scale for this cage (near setup)
   float minv[16];              // (A A^T)^-1, A = [1; v] (4 x nv), row-major
   float* out_w;                // [n][nv]
   float* out_g;                // [n][nv][3] or null (then DQ unused)

@@ -424,8 +424,9 @@ __global__ __launch_bounds__(256) void k_mvc_scene_children(CgsMvcSceneArgs A,
 // weight phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)), c_t its
 // centroid: 1 within r0, 0 beyond r1.  The near sums u_near = sum_t phi_t
 // mu_t (the singular, fast-varying part next to the cage) are evaluated
-// exactly; the far sums are smooth and are what the network learns, as the
-// share q = u_far / (u_near + u_far) of the normalised weights.
+// exactly; the far sums are smooth and are what the network learns, either
+// as the share q = u_far / (u_near + u_far) of the normalised weights or as
+// the far sums u_far themselves.
 __device__ __forceinline__ float bf16_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ float near_phi(float r, float r0, float r1) {
@@ -435,10 +436,12 @@ __device__ __forceinline__ float near_phi(float r, float r0, float r1) {
 
 // Far-share training targets, one wave per point: the loop of mvc_eval with
 // each triangle's terms scaled by 1 - phi_t before the per-vertex sums, the
-// normaliser over all terms.  Points on a vertex or a face, or with a zero
-// normaliser, get a zero row (they do not occur among interior samples).
+// normaliser over all terms (sums = 1: the far sums, not normalised).  Points
+// on a vertex or a face, or with a zero normaliser, get a zero row (they do
+// not occur among interior samples).
 __global__ __launch_bounds__(64) void k_mvc_far_share(CgsMvcArgs A, const float* cent, float r0,
-                                                      float r1, float q_scale, float* out) {
+                                                      float r1, float q_scale, int sums,
+                                                      float* out) {
   extern __shared__ float4 s_dyn[];
   float4* s_dir = s_dyn;
   float* s_mu = reinterpret_cast<float*>(s_dir + A.nv);
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(64) void k_mvc_far_share(CgsMvcArgs A, const float*
   const float total = wave_sum(part);
   const bool skip = __ballot(bad != 0u) != 0ull || !(fabsf(total) > 1e-12f);
   __builtin_amdgcn_wave_barrier();
-  const float scale = skip ? 0.0f : q_scale / total;
+  const float scale = skip ? 0.0f : sums ? q_scale : q_scale / total;
   for (uint32_t v = lane; v < A.nv; v += 64u) {
     float acc = 0.0f;
     for (uint32_t j = A.vt_start[v]; j < A.vt_start[v + 1]; ++j) acc += s_mu[A.vt_list[j]];
@@ -635,6 +638,13 @@ __device__ __forceinline__ int jsw_terms_fast(V3f x, V3f v0, V3f v1, V3f v2, flo
 // differences (step h) of the weights with the network's share linearised
 // from its tangents, q(p +- h e_a) = q +- h dq_a:
 //   g_a = (1 - Q) (un+ - un-) / 2h - dQ_a (un+ + un-) / 2 + dq_a.
+// With far sums (far_sums = 1) the network gives f = u_far instead and
+//   w = (u_near + f) / (U_near + F),  F = sum f,
+// which needs no fade: the denominator is the whole MVC sum, bounded away
+// from 0 inside the cage, and next to the cage, where u_near is singular, the
+// network's error is divided by it.  Gradients: the same central
+// differences, f(p +- h e_a) = f +- h df_a.  A sum at p below tau, or one at
+// p +- h e_a within tau of 0, flags the point for the exact path.
 // The residual |sum w v - x| is taken before the linear-precision projection
 // w' = w + A^T Minv ([1; x] - A w) (and g'_a likewise with e_a), which the
 // rows receive.
@@ -841,12 +851,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         if (ee == e) U[ee] += phi * ((mu[0] + mu[1]) + mu[2]);
     }
   }
-  const bool degen = __ballot(degenerate != 0u) != 0ull;
-  float inv[7];
+  bool degen = __ballot(degenerate != 0u) != 0ull;
+  const bool sums = A.far_sums != 0u;
+  float inv[7], Usum[7];
   const float tau2 = A.tau * A.tau;
 #pragma unroll
   for (int e = 0; e < 7; ++e) {
     const float Ue = (uint32_t)e < evals ? wave_sum_dpp(U[e]) : 0.0f;
+    Usum[e] = Ue;
     inv[e] = Ue / (Ue * Ue + tau2);
   }
   float wv[VPL], gv[3][VPL];
@@ -867,10 +879,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 #pragma unroll
     for (int a = 0; a < 3; ++a) sdq[a] += gv[a][k];
   }
-  const float keep = 1.0f - wave_sum_dpp(sq), inv_2h = 0.5f / A.h;
+  const float Fsum = wave_sum_dpp(sq), keep = 1.0f - Fsum, inv_2h = 0.5f / A.h;
   float dQ[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) dQ[a] = tan ? wave_sum_dpp(sdq[a]) : 0.0f;
+  if (sums) {
+    // reciprocals of the whole sums at p and at p +- h e_a (far part
+    // linearised); inv[] is reused for them
+    float den[7];
+    den[0] = Usum[0] + Fsum;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      den[1 + 2 * a] = Usum[1 + 2 * a] + (Fsum + A.h * dQ[a]);
+      den[2 + 2 * a] = Usum[2 + 2 * a] + (Fsum - A.h * dQ[a]);
+    }
+    // the whole sum at an interior point is >= 4 pi / diagonal (>> tau); at
+    // p +- h e_a, which may lie outside the cage, it may take either sign, as
+    // in the exact binding's differences, but not vanish
+    bool bad = !(den[0] > A.tau);
+#pragma unroll
+    for (int e = 1; e < 7; ++e) bad = bad || (grow && !(fabsf(den[e]) > A.tau));
+    degen = degen || bad;
+#pragma unroll
+    for (int e = 0; e < 7; ++e) inv[e] = bad ? 0.0f : 1.0f / den[e];
+  }
   __syncthreads();
   // the combined rows and the sums of the projection (sum w, sum w v; the
   // same for each g_a)
@@ -881,8 +913,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     if (v >= nv) continue;
     const uint32_t sl = near_slot_find(s_key, v);
     const V3f cv = vert(A.verts, v);
-    const float un = sl != NEAR_EMPTY ? s_val[0][sl] * inv[0] : 0.0f;
-    const float w = keep * un + wv[k];
+    const float wv0 = wv[k];  // the network's value (far share or far sum)
+    float w;
+    if (sums) {
+      w = ((sl != NEAR_EMPTY ? s_val[0][sl] : 0.0f) + wv0) * inv[0];
+    } else {
+      const float un = sl != NEAR_EMPTY ? s_val[0][sl] * inv[0] : 0.0f;
+      w = keep * un + wv0;
+    }
     wv[k] = w;
     m4[0] += w;
     m4[1] += w * cv.x;
@@ -891,9 +929,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     if (grow) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float up = sl != NEAR_EMPTY ? s_val[1 + 2 * a][sl] * inv[1 + 2 * a] : 0.0f;
-        const float um = sl != NEAR_EMPTY ? s_val[2 + 2 * a][sl] * inv[2 + 2 * a] : 0.0f;
-        const float g = (keep * ((up - um) * inv_2h) - dQ[a] * (0.5f * (up + um))) + gv[a][k];
+        float g;
+        if (sums) {
+          const float hd = A.h * gv[a][k];
+          const float np = (sl != NEAR_EMPTY ? s_val[1 + 2 * a][sl] : 0.0f) + (wv0 + hd);
+          const float nm = (sl != NEAR_EMPTY ? s_val[2 + 2 * a][sl] : 0.0f) + (wv0 - hd);
+          g = (np * inv[1 + 2 * a] - nm * inv[2 + 2 * a]) * inv_2h;
+        } else {
+          const float up = sl != NEAR_EMPTY ? s_val[1 + 2 * a][sl] * inv[1 + 2 * a] : 0.0f;
+          const float um = sl != NEAR_EMPTY ? s_val[2 + 2 * a][sl] * inv[2 + 2 * a] : 0.0f;
+          g = (keep * ((up - um) * inv_2h) - dQ[a] * (0.5f * (up + um))) + gv[a][k];
+        }
         gv[a][k] = g;
         mg[a][0] += g;
         mg[a][1] += g * cv.x;
@@ -965,10 +1011,10 @@ void cgs_launch_mvc_bind(const CgsMvcArgs& a, hipStream_t s) {
 }
 
 void cgs_launch_mvc_far_share(const CgsMvcArgs& a, const float* cent, float r0, float r1,
-                              float q_scale, float* out, hipStream_t s) {
+                              float q_scale, int sums, float* out, hipStream_t s) {
   if (a.n == 0) return;
   hipLaunchKernelGGL(k_mvc_far_share, dim3(a.n), dim3(64), (size_t)a.nv * 16 + (size_t)a.nt * 12, s,
-                     a, cent, r0, r1, q_scale, out);
+                     a, cent, r0, r1, q_scale, sums, out);
 }
 
 void cgs_launch_mvc_winding_lane(const float* verts, const uint32_t* tris, uint32_t nt,

@@ -62,7 +62,9 @@ struct cgs_mlp {
   Dev<float> iz, itz;
   Dev<uint16_t> izh, itzh;  // bf16 copies for the near/far binding
   // near/far split radii (0, 0: the network stands for the whole weights)
+  // and what the network learns with it (CGS_FAR_SHARE or CGS_FAR_SUMS)
   float near_r0 = 0.0f, near_r1 = 0.0f;
+  uint32_t far_kind = CGS_FAR_SHARE;
   std::vector<void*> owned;
 };
 
@@ -264,7 +266,8 @@ int infer(cgs_mlp* M, const float* X, uint64_t n, float* out_w, float* out_g, co
       na.Qh = M->izh.p;
       na.DQh = out_g ? M->itzh.p : nullptr;
       na.ldq = M->n_out_p;
-      na.q_scale = (float)M->n_out;
+      na.far_sums = M->far_kind == CGS_FAR_SUMS ? 1u : 0u;
+      na.q_scale = na.far_sums ? near->sums_scale : (float)M->n_out;
       na.out_w = out_w + b * M->n_out;
       na.out_g = out_g ? out_g + b * M->n_out * 3 : nullptr;
       na.residual = near->residual + b;
@@ -456,6 +459,14 @@ int cgs_mlp_set_near_field(cgs_mlp* M, float r0, float r1) {
     return cgs_fail(CGS_ERR_INVALID_ARG, "near field radii need 0 <= r0 < r1 (or both 0)");
   M->near_r0 = r0;
   M->near_r1 = r1;
+  return CGS_OK;
+}
+
+int cgs_mlp_set_far_kind(cgs_mlp* M, uint32_t kind) {
+  if (!M) return cgs_fail(CGS_ERR_INVALID_ARG, "null network");
+  if (kind != CGS_FAR_SHARE && kind != CGS_FAR_SUMS)
+    return cgs_fail(CGS_ERR_INVALID_ARG, "far target kind must be CGS_FAR_SHARE or CGS_FAR_SUMS");
+  M->far_kind = kind;
   return CGS_OK;
 }
 

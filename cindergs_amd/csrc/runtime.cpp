@@ -2524,13 +2524,25 @@ static int check_near_radii(float r0, float r1) {
   return CGS_OK;
 }
 
+// The far sums' target scale: num_vertices x diagonal / (4 pi), so that a
+// far sum of the size of the smallest interior weight sum (4 pi / diagonal)
+// spread over the vertices is 1.
+static float far_sums_scale(const float* verts, uint32_t nv) {
+  return (float)nv * cgs_fmax(cage_diagonal(verts, nv), 1e-6f) * 0.0795774715f;
+}
+
 // Training targets of the near/far network: the far share q of the exact
-// weights (u - u_near) / sum u, times num_vertices, per point.
-int cgs_mvc_far_share(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
-                      const uint32_t* triangles, uint32_t num_triangles, float near_r0,
-                      float near_r1, const float* d_points, uint64_t n, float* d_out) {
+// weights (u - u_near) / sum u, times num_vertices, per point
+// (CGS_FAR_SHARE), or the far sums u - u_near times far_sums_scale
+// (CGS_FAR_SUMS).
+int cgs_mvc_far_targets(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                        const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                        float near_r1, const float* d_points, uint64_t n, uint32_t kind,
+                        float* d_out) {
   TRY(ensure_ctx(c));
   TRY(check_near_radii(near_r0, near_r1));
+  if (kind != CGS_FAR_SHARE && kind != CGS_FAR_SUMS)
+    return fail(CGS_ERR_INVALID_ARG, "far target kind must be CGS_FAR_SHARE or CGS_FAR_SUMS");
   if (n == 0) return CGS_OK;
   if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
   if (!d_points || !d_out) return fail(CGS_ERR_INVALID_ARG, "null buffer");
@@ -2551,14 +2563,23 @@ int cgs_mvc_far_share(cgs_context* c, const float* cage_vertices, uint32_t num_v
     A.vt_list = (const uint32_t*)M.d_list;
     A.pts = d_points;
     A.n = (uint32_t)n;
-    cgs_launch_mvc_far_share(A, (const float*)d_cent, near_r0, near_r1, (float)num_vertices, d_out,
-                             c->stream);
+    const bool sums = kind == CGS_FAR_SUMS;
+    cgs_launch_mvc_far_share(A, (const float*)d_cent, near_r0, near_r1,
+                             sums ? far_sums_scale(cage_vertices, num_vertices) : (float)num_vertices,
+                             sums ? 1 : 0, d_out, c->stream);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d_cent);
   if (e != hipSuccess) return fail(CGS_ERR_EXECUTION, std::string("far share: ") + hipGetErrorString(e));
   return CGS_OK;
+}
+
+int cgs_mvc_far_share(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                      const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                      float near_r1, const float* d_points, uint64_t n, float* d_out) {
+  return cgs_mvc_far_targets(c, cage_vertices, num_vertices, triangles, num_triangles, near_r0,
+                             near_r1, d_points, n, CGS_FAR_SHARE, d_out);
 }
 
 // The cage buffers and the near-field grid of the last near/far binding,
@@ -2717,23 +2738,26 @@ static int near_setup(cgs_context* c, const float* verts, uint32_t nv, const uin
   // so the near term is short by at most 5e-5 of the weights' sum where it
   // fades out (the projection restores the sum)
   a.tau = 1.2566371e-3f / cgs_fmax(cage_diagonal(verts, nv), 1e-6f);
+  a.sums_scale = far_sums_scale(verts, nv);
   a.pts = d_points;
   a.n = (uint32_t)n;
   a.winding = (const float*)S.d_wind;
   return CGS_OK;
 }
 
-// The near/far combination with caller-supplied far shares: d_q [n x nv]
-// (q x nv, as cgs_mvc_far_share writes them) and d_dq [3n x nv] (row
+// The near/far combination with caller-supplied far targets: d_q [n x nv]
+// (as cgs_mvc_far_targets writes them for `kind`) and d_dq [3n x nv] (row
 // 3p + axis, the derivatives of those rows) or NULL.  Outputs as
 // cgs_mvc_bind_exact plus d_residual [n] (|sum w v - x| before the
 // projection; +inf where the near sums meet a vertex or a face).
-int cgs_mvc_nearfar_bind(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
-                         const uint32_t* triangles, uint32_t num_triangles, float near_r0,
-                         float near_r1, const float* d_points, uint64_t n, const float* d_q,
-                         const float* d_dq, float* d_weights, float* d_gradients,
-                         uint32_t* d_inside, float* d_residual) {
+int cgs_mvc_nearfar_bind_kind(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                              const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                              float near_r1, const float* d_points, uint64_t n, const float* d_q,
+                              const float* d_dq, uint32_t kind, float* d_weights,
+                              float* d_gradients, uint32_t* d_inside, float* d_residual) {
   TRY(ensure_ctx(c));
+  if (kind != CGS_FAR_SHARE && kind != CGS_FAR_SUMS)
+    return fail(CGS_ERR_INVALID_ARG, "far target kind must be CGS_FAR_SHARE or CGS_FAR_SUMS");
   if (n == 0) return CGS_OK;
   if (n >= (1ull << 31)) return fail(CGS_ERR_INVALID_ARG, "too many points");
   if (!d_points || !d_q || !d_weights || !d_residual) return fail(CGS_ERR_INVALID_ARG, "null buffer");
@@ -2744,7 +2768,8 @@ int cgs_mvc_nearfar_bind(cgs_context* c, const float* cage_vertices, uint32_t nu
   a.Q = d_q;
   a.DQ = d_dq;
   a.ldq = num_vertices;
-  a.q_scale = (float)num_vertices;
+  a.far_sums = kind == CGS_FAR_SUMS ? 1u : 0u;
+  a.q_scale = a.far_sums ? a.sums_scale : (float)num_vertices;
   a.out_w = d_weights;
   a.out_g = d_gradients;
   a.residual = d_residual;
@@ -2753,6 +2778,16 @@ int cgs_mvc_nearfar_bind(cgs_context* c, const float* cage_vertices, uint32_t nu
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return CGS_OK;
+}
+
+int cgs_mvc_nearfar_bind(cgs_context* c, const float* cage_vertices, uint32_t num_vertices,
+                         const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                         float near_r1, const float* d_points, uint64_t n, const float* d_q,
+                         const float* d_dq, float* d_weights, float* d_gradients,
+                         uint32_t* d_inside, float* d_residual) {
+  return cgs_mvc_nearfar_bind_kind(c, cage_vertices, num_vertices, triangles, num_triangles,
+                                   near_r0, near_r1, d_points, n, d_q, d_dq, CGS_FAR_SHARE,
+                                   d_weights, d_gradients, d_inside, d_residual);
 }
 
 // The surrogate binding: the network's weights and gradients for every

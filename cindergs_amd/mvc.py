@@ -169,31 +169,35 @@ class MvcGpu:
                 m.ptr))
         return w, g, m
 
-    def far_share(self, d_points: DeviceBuffer, n: int, r0: float, r1: float) -> DeviceBuffer:
-        """Far-share training targets (cgs_mvc_far_share): [n, nv] device rows
-        of nv * (u - u_near) / sum u."""
+    def far_share(self, d_points: DeviceBuffer, n: int, r0: float, r1: float,
+                  kind: int = 0) -> DeviceBuffer:
+        """Near/far training targets (cgs_mvc_far_targets): [n, nv] device rows
+        of nv * (u - u_near) / sum u (kind 0, CGS_FAR_SHARE) or of
+        (u - u_near) * nv * diagonal / (4 pi) (kind 1, CGS_FAR_SUMS)."""
         nv = self.cage.num_vertices
         out = self.state.to_device(np.zeros((n, nv), np.float32))
         if n:
-            check(self._lib.cgs_mvc_far_share(self.state._h, _ptr(self.cage.vertices), nv,
-                                              _ptr(self.cage.triangles),
-                                              self.cage.triangles.shape[0], float(r0), float(r1),
-                                              d_points.ptr, n, out.ptr))
+            check(self._lib.cgs_mvc_far_targets(self.state._h, _ptr(self.cage.vertices), nv,
+                                                _ptr(self.cage.triangles),
+                                                self.cage.triangles.shape[0], float(r0), float(r1),
+                                                d_points.ptr, n, int(kind), out.ptr))
         return out
 
     def nearfar_bind(self, d_points: DeviceBuffer, n: int, d_q: DeviceBuffer, r0: float, r1: float,
-                     d_dq=None, gradients: bool = True):
-        """The near/far combination with given far shares (cgs_mvc_nearfar_bind):
-        device (weights, gradients or None, inside, residual)."""
+                     d_dq=None, gradients: bool = True, kind: int = 0):
+        """The near/far combination with given far targets of `kind`
+        (cgs_mvc_nearfar_bind_kind): device (weights, gradients or None,
+        inside, residual)."""
         nv = self.cage.num_vertices
         w = self.state.to_device(np.zeros((n, nv), np.float32))
         g = self.state.to_device(np.zeros((n, nv, 3), np.float32)) if gradients else None
         m = self.state.to_device(np.zeros(n, np.uint32))
         r = self.state.to_device(np.zeros(n, np.float32))
-        check(self._lib.cgs_mvc_nearfar_bind(
+        check(self._lib.cgs_mvc_nearfar_bind_kind(
             self.state._h, _ptr(self.cage.vertices), nv, _ptr(self.cage.triangles),
             self.cage.triangles.shape[0], float(r0), float(r1), d_points.ptr, n, d_q.ptr,
-            d_dq.ptr if d_dq is not None else None, w.ptr, g.ptr if g else None, m.ptr, r.ptr))
+            d_dq.ptr if d_dq is not None else None, int(kind), w.ptr, g.ptr if g else None, m.ptr,
+            r.ptr))
         return w, g, m, r
 
     def deform(self, w: DeviceBuffer, g, m: DeviceBuffer, d_rest: DeviceBuffer,

@@ -31,6 +31,7 @@ class MvcSurrogate:
         self._h = h
         self.n_out = n_out
         self.near_field = (0.0, 0.0)
+        self.far_kind = 0
         self.count = int(self._lib.cgs_mlp_param_count(h))
         rows = (C.c_uint32 * LAYERS)()
         cols = (C.c_uint32 * LAYERS)()
@@ -67,12 +68,16 @@ class MvcSurrogate:
         blk = params[o:o + r * (c + 32)].reshape(r, c + 32)
         return blk[:, :c], blk[:, c]
 
-    def set_near_field(self, r0: float, r1: float):
+    def set_near_field(self, r0: float, r1: float, kind: int = 0):
         """Near/far split (cgs_mlp_set_near_field): the network then learns the
-        far share of the weights (targets from MvcGpu.far_share) and the
-        binding adds the exactly evaluated near sums.  (0, 0) switches it off."""
+        far share of the weights (kind 0) or the far sums (kind 1,
+        cgs_mlp_set_far_kind), with targets from MvcGpu.far_share(..., kind),
+        and the binding adds the exactly evaluated near sums.  (0, 0) switches
+        it off."""
         check(self._lib.cgs_mlp_set_near_field(self._h, float(r0), float(r1)))
+        check(self._lib.cgs_mlp_set_far_kind(self._h, int(kind)))
         self.near_field = (float(r0), float(r1))
+        self.far_kind = int(kind)
 
     # ---- training and inference ------------------------------------------
     def train(self, d_x: DeviceBuffer, d_t: DeviceBuffer, n: int, steps: int, batch: int = 8192,

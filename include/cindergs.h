@@ -531,11 +531,29 @@ int cgs_mlp_infer(cgs_mlp* net, const float* d_x, uint64_t n, float* d_w, float*
  * sign kept), projected onto linear
  * precision. */
 int cgs_mlp_set_near_field(cgs_mlp* net, float r0, float r1);
+/* What a near/far network learns (no counterpart in the reference, which has
+ * no learned binding).  CGS_FAR_SHARE (the default): the far share q above.
+ * CGS_FAR_SUMS: the far sums u_far = u - u_near themselves, times
+ * num_vertices x diagonal / (4 pi) (targets from cgs_mvc_far_targets), and
+ * the binding's weights are w = (u_near + u_far) / sum (u_near + u_far),
+ * projected onto linear precision: the network's error is then divided by
+ * the whole MVC sum, which is large next to the cage, where the far share
+ * changes fastest. */
+#define CGS_FAR_SHARE 0u
+#define CGS_FAR_SUMS 1u
+int cgs_mlp_set_far_kind(cgs_mlp* net, uint32_t kind);
 /* Training targets for a near/far network: q x num_vertices per point,
  * d_out [n x num_vertices] (zero rows for points on a vertex or a face). */
 int cgs_mvc_far_share(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
                       const uint32_t* triangles, uint32_t num_triangles, float near_r0,
                       float near_r1, const float* d_points, uint64_t n, float* d_out);
+/* Training targets of either kind (cgs_mlp_set_far_kind): kind CGS_FAR_SHARE
+ * is cgs_mvc_far_share; CGS_FAR_SUMS writes u_far x num_vertices x diagonal
+ * / (4 pi), diagonal the cage's bounding-box diagonal. */
+int cgs_mvc_far_targets(cgs_context* ctx, const float* cage_vertices, uint32_t num_vertices,
+                        const uint32_t* triangles, uint32_t num_triangles, float near_r0,
+                        float near_r1, const float* d_points, uint64_t n, uint32_t kind,
+                        float* d_out);
 /* The near/far combination of cgs_mlp_bind with the far shares given by the
  * caller: d_q [n x num_vertices] (as cgs_mvc_far_share writes them), d_dq
  * [3n x num_vertices] (row 3p + axis: their derivatives) or NULL.  Outputs as
@@ -547,6 +565,14 @@ int cgs_mvc_nearfar_bind(cgs_context* ctx, const float* cage_vertices, uint32_t 
                          float near_r1, const float* d_points, uint64_t n, const float* d_q,
                          const float* d_dq, float* d_weights, float* d_gradients,
                          uint32_t* d_inside, float* d_residual);
+/* The same with far targets of either kind (as cgs_mvc_far_targets writes
+ * them for `kind`). */
+int cgs_mvc_nearfar_bind_kind(cgs_context* ctx, const float* cage_vertices,
+                              uint32_t num_vertices, const uint32_t* triangles,
+                              uint32_t num_triangles, float near_r0, float near_r1,
+                              const float* d_points, uint64_t n, const float* d_q,
+                              const float* d_dq, uint32_t kind, float* d_weights,
+                              float* d_gradients, uint32_t* d_inside, float* d_residual);
 /* The surrogate for PolyCageMesh::bind_gaussians (mvc.rs:498-541): same outputs
  * as cgs_mvc_bind_exact (zero rows outside), the network's weights and input
  * derivatives (with a near field: combined with the exact near sums) for

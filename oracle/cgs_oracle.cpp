@@ -1305,11 +1305,13 @@ static void mvc_weights(const float* V, uint32_t nv, const uint32_t* F, uint32_t
 
 // Far share of the MVC sums for the near/far network: triangle t's terms
 // scaled by 1 - phi_t(x), phi_t = smoothstep((r1 - |x - c_t|) / (r1 - r0)),
-// c_t = (v0 + v1 + v2) / 3 in f32; normalised by the full sum, times nv.
-// Points on a vertex or a face, or with a zero sum, give a zero row.
+// c_t = (v0 + v1 + v2) / 3 in f32; normalised by the full sum, times nv
+// (sums_scale > 0: the far sums themselves, not normalised, times
+// sums_scale).  Points on a vertex or a face, or with a zero sum, give a
+// zero row.
 template <class T>
 static void mvc_far_share(const float* V, uint32_t nv, const uint32_t* F, uint32_t nt, const T x[3],
-                          T r0, T r1, T* q, T* dist, W3<T>* dir) {
+                          T r0, T r1, T* q, T* dist, W3<T>* dir, T sums_scale = 0) {
   typedef Tr<T> m;
   const T PI_ = (T)3.14159265358979323846f;
   const W3<T> X = {x[0], x[1], x[2]};
@@ -1366,7 +1368,7 @@ static void mvc_far_share(const float* V, uint32_t nv, const uint32_t* F, uint32
     for (uint32_t k = 0; k < nv; ++k) q[k] = 0;
     return;
   }
-  const T scale = (T)nv / total;
+  const T scale = sums_scale > (T)0 ? sums_scale : (T)nv / total;
   for (uint32_t k = 0; k < nv; ++k) q[k] *= scale;
 }
 
@@ -1485,6 +1487,29 @@ ORC_API void orc_mvc_far_share(const float* verts, uint32_t nv, const uint32_t* 
       std::vector<W3<float>> dir(nv);
       const float p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
       mvc_far_share<float>(verts, nv, tris, nt, p, r0, r1, (float*)out + i * nv, dist.data(), dir.data());
+    }
+  }
+}
+
+// far sums of n points: [n*nv], the far share's sums before the
+// normalisation, times scale (fp64: double output)
+ORC_API void orc_mvc_far_sums(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                              const float* pts, uint64_t n, float r0, float r1, float scale,
+                              int fp64, void* out) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    if (fp64) {
+      std::vector<double> dist(nv);
+      std::vector<W3<double>> dir(nv);
+      const double p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_far_share<double>(verts, nv, tris, nt, p, r0, r1, (double*)out + i * nv, dist.data(),
+                            dir.data(), (double)scale);
+    } else {
+      std::vector<float> dist(nv);
+      std::vector<W3<float>> dir(nv);
+      const float p[3] = {pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+      mvc_far_share<float>(verts, nv, tris, nt, p, r0, r1, (float*)out + i * nv, dist.data(),
+                           dir.data(), scale);
     }
   }
 }

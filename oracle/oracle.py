@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "orc_composite_v2": ([_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp, _i, _vp,
                                   _vp], None),
             "orc_mvc_far_share": ([_vp, _u32, _vp, _u32, _vp, _u64, _f, _f, _i, _vp], None),
+            "orc_mvc_far_sums": ([_vp, _u32, _vp, _u32, _vp, _u64, _f, _f, _f, _i, _vp], None),
             "orc_mvc_scene": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u32, _f, _f, _vp, _vp,
                                _vp, _vp], _u64),
             "orc_sh_tables": ([_vp], None),
@@ -389,6 +390,27 @@ def mvc_far_share(verts, tris, points, r0, r1, fp64=True):
     lib().orc_mvc_far_share(_p(v), v.shape[0], _p(t), t.shape[0], _p(p), p.shape[0], r0, r1,
                             1 if fp64 else 0, _p(out))
     return out
+
+
+def mvc_far_sums(verts, tris, points, r0, r1, scale, fp64=True):
+    """Far sums of MVC (triangle terms scaled by 1 - phi_t, not normalised),
+    times scale."""
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.uint32)
+    p = np.ascontiguousarray(points, np.float32)
+    out = np.zeros((p.shape[0], v.shape[0]), np.float64 if fp64 else np.float32)
+    lib().orc_mvc_far_sums(_p(v), v.shape[0], _p(t), t.shape[0], _p(p), p.shape[0], r0, r1,
+                           float(scale), 1 if fp64 else 0, _p(out))
+    return out
+
+
+def far_sums_scale(verts):
+    """The far sums' target scale of a cage: nv x bounding-box diagonal / (4 pi),
+    in float32 as the library computes it."""
+    v = np.asarray(verts, np.float32)
+    d = (v.max(0) - v.min(0)).astype(np.float32)
+    diag = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]).astype(np.float32)
+    return float(np.float32(v.shape[0]) * max(diag, np.float32(1e-6)) * np.float32(0.0795774715))
 
 
 def mvc_winding(verts, tris, points, fp64=False):

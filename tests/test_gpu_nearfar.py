@@ -1,13 +1,16 @@
-"""The near/far learned MVC binding on the GPU (cgs_mvc_far_share,
-cgs_mvc_nearfar_bind, cgs_mlp_bind with a near field).
+"""The near/far learned MVC binding on the GPU (cgs_mvc_far_targets,
+cgs_mvc_nearfar_bind_kind, cgs_mlp_bind with a near field), for both things
+the network can learn: the far share of the weights (kind 0) and the far
+sums (kind 1).
 
 Bars:
-* far shares: their RMS error against the oracle's float64 restatement at
+* far shares and far sums: their RMS error against the oracle's float64
+  restatement at
   most twice that of the oracle's own float32 run (the bar of the exact MVC
   kernel), their largest error at most four times (it sits at the
   worst-conditioned point next to the cage, where the order of the f32 sum
   over 1280 triangles alone moves it by that much);
-* the exact far shares fed to the near/far combination give the exact MVC
+* the exact far shares (or sums) fed to the near/far combination give the exact MVC
   binding: against the oracle's float64 weights, an error at most twice that
   of the exact kernel (cgs_mvc_bind_exact, float32 like it; both reach ~1e-4
   next to the cage), weight gradients within 2% of the exact kernel's
@@ -16,7 +19,7 @@ Bars:
   whose difference stencil meets a face, flagged for the exact path with an
   infinite residual, are at most 1% and left out);
 * a near/far network trained briefly (3000 steps on 131072 samples) on
-  far-share targets binds fresh points with the residual test at 1e-2 of the
+  far-share (or far-sum) targets binds fresh points with the residual test at 1e-2 of the
   cage diagonal: at most 5% of them on the exact path, the inside mask equal
   to the exact binding's, deformed positions (cage perturbed by U(-0.1, 0.1))
   within 1e-3 of the cage diagonal of exact MVC's at the 99th percentile and
@@ -48,22 +51,29 @@ def cage_setup(gpu):
     return cage, mg, inside_fn
 
 
-def test_far_share_matches_oracle(gpu, cage_setup, oracle_lib):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_far_share_matches_oracle(gpu, cage_setup, oracle_lib, kind):
     import oracle
 
     cage, mg, inside_fn = cage_setup
     x = sample_inside(cage, 512, seed=21, inside_fn=inside_fn, batch=1 << 14)
-    q = mg.far_share(gpu.to_device(x), x.shape[0], R0, R1).numpy()
-    ref = oracle.mvc_far_share(cage.vertices, cage.triangles, x, R0, R1)
-    f32 = oracle.mvc_far_share(cage.vertices, cage.triangles, x, R0, R1, fp64=False)
+    q = mg.far_share(gpu.to_device(x), x.shape[0], R0, R1, kind=kind).numpy()
+    if kind == 0:
+        ref = oracle.mvc_far_share(cage.vertices, cage.triangles, x, R0, R1)
+        f32 = oracle.mvc_far_share(cage.vertices, cage.triangles, x, R0, R1, fp64=False)
+    else:
+        s = oracle.far_sums_scale(cage.vertices)
+        ref = oracle.mvc_far_sums(cage.vertices, cage.triangles, x, R0, R1, s)
+        f32 = oracle.mvc_far_sums(cage.vertices, cage.triangles, x, R0, R1, s, fp64=False)
     err, err32 = np.abs(q - ref).max(), np.abs(f32 - ref).max()
     rms, rms32 = np.sqrt(((q - ref) ** 2).mean()), np.sqrt(((f32 - ref) ** 2).mean())
-    print(f"[case] far share vs float64: gpu max {err:.2e} rms {rms:.2e}; "
+    print(f"[case] far targets kind {kind} vs float64: gpu max {err:.2e} rms {rms:.2e}; "
           f"oracle float32 max {err32:.2e} rms {rms32:.2e}", flush=True)
     assert err <= 4.0 * err32 + 1e-6 and rms <= 2.0 * rms32 + 1e-7
 
 
-def test_exact_far_share_gives_exact_binding(gpu, cage_setup, oracle_lib):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_exact_far_share_gives_exact_binding(gpu, cage_setup, oracle_lib, kind):
     import oracle
 
     cage, mg, inside_fn = cage_setup
@@ -72,16 +82,16 @@ def test_exact_far_share_gives_exact_binding(gpu, cage_setup, oracle_lib):
     n = x.shape[0]
     h = float(np.clip(np.float32(cage.diagonal()) * np.float32(1e-3), 1e-5, 1e-2))
     dx = gpu.to_device(x)
-    q = mg.far_share(dx, n, R0, R1)
+    q = mg.far_share(dx, n, R0, R1, kind=kind)
     dq = np.zeros((n, 3, cage.num_vertices), np.float32)
     for a in range(3):
         e = np.zeros(3, np.float32)
         e[a] = h
-        qp = mg.far_share(gpu.to_device(x + e), n, R0, R1).numpy()
-        qm = mg.far_share(gpu.to_device(x - e), n, R0, R1).numpy()
+        qp = mg.far_share(gpu.to_device(x + e), n, R0, R1, kind=kind).numpy()
+        qm = mg.far_share(gpu.to_device(x - e), n, R0, R1, kind=kind).numpy()
         dq[:, a] = (qp - qm) / (2 * h)
     d_dq = gpu.to_device(dq.reshape(3 * n, -1))
-    w, g, m, r = mg.nearfar_bind(dx, n, q, R0, R1, d_dq=d_dq)
+    w, g, m, r = mg.nearfar_bind(dx, n, q, R0, R1, d_dq=d_dq, kind=kind)
     ew, eg, em = mg.bind(dx, n)
     inside = em.numpy().astype(bool)
     np.testing.assert_array_equal(m.numpy(), em.numpy())
@@ -98,7 +108,7 @@ def test_exact_far_share_gives_exact_binding(gpu, cage_setup, oracle_lib):
     V64 = cage.vertices.astype(np.float64)
     res = R[ok].max() / cage.diagonal()
     res_x = np.linalg.norm(EW[ok] @ V64 - x[ok], axis=1).max() / cage.diagonal()
-    print(f"[case] exact far share: {int(flagged.sum())} flagged; max|w - w64| {werr:.2e} "
+    print(f"[case] exact far targets kind {kind}: {int(flagged.sum())} flagged; max|w - w64| {werr:.2e} "
           f"(exact kernel {werr_x:.2e}); grad rel {gerr:.2e}; residual {res:.2e} "
           f"(exact kernel {res_x:.2e})", flush=True)
     assert flagged.sum() <= 0.01 * n
@@ -117,18 +127,20 @@ def _train(net, d_x, d_t, n, steps, batch=8192, segments=4):
     return np.concatenate(losses)
 
 
-def test_nearfar_network_training_and_binding(gpu, cage_setup):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_nearfar_network_training_and_binding(gpu, cage_setup, kind):
     cage, mg, inside_fn = cage_setup
     n_train = 131072
     xt = sample_inside(cage, n_train, seed=23, inside_fn=inside_fn, batch=1 << 17)
     dxt = gpu.to_device(xt)
-    qt = mg.far_share(dxt, n_train, R0, R1)
+    qt = mg.far_share(dxt, n_train, R0, R1, kind=kind)
     net = MvcSurrogate(gpu, cage.num_vertices, fourier_sigma=1.0, seed=7)
-    net.set_near_field(R0, R1)
+    net.set_near_field(R0, R1, kind=kind)
     untrained = MvcSurrogate(gpu, cage.num_vertices, fourier_sigma=1.0, seed=8)
-    untrained.set_near_field(R0, R1)
+    untrained.set_near_field(R0, R1, kind=kind)
     losses = _train(net, dxt, qt, n_train, 3000)
-    print(f"[case] near/far training loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    print(f"[case] near/far kind {kind} training loss {losses[0]:.3e} -> {losses[-1]:.3e}",
+          flush=True)
     assert np.all(np.isfinite(losses)) and losses[-1] * 10 <= losses[0]
     xe = sample_inside(cage, 16384, seed=24, inside_fn=inside_fn, batch=1 << 15)
     xe = np.concatenate([xe, np.array([[0, 0, 2.0], [2.0, 0, 0]], np.float32)])
@@ -139,7 +151,7 @@ def test_nearfar_network_training_and_binding(gpu, cage_setup):
     sw, sg, sm, fb = net.bind(cage, dxe, n, residual_tol=tol)
     _, _, _, fb_u = untrained.bind(cage, dxe, n, residual_tol=tol)
     frac = fb / n
-    print(f"[case] near/far binding: {fb} of {n} points exact ({frac:.4f}); untrained {fb_u}",
+    print(f"[case] near/far kind {kind} binding: {fb} of {n} points exact ({frac:.4f}); untrained {fb_u}",
           flush=True)
     np.testing.assert_array_equal(sm.numpy(), em.numpy())
     assert frac <= 0.05
