@@ -23,8 +23,8 @@ Bars:
   cage diagonal: at most 5% of them on the exact path, the inside mask equal
   to the exact binding's, deformed positions (cage perturbed by U(-0.1, 0.1))
   within 1e-3 of the cage diagonal of exact MVC's at the 99th percentile and
-  within 5e-3 at worst, and the untrained network sending more points to the
-  exact path than the trained one.
+  within 5e-3 at worst (1e-3 at worst with the far sums), and the untrained
+  network sending more points to the exact path than the trained one.
 """
 import numpy as np
 import pytest
@@ -165,5 +165,9 @@ def test_nearfar_network_training_and_binding(gpu, cage_setup, kind):
     print(f"[case] deformed position error / diag: max {e.max():.2e} p99 {np.quantile(e, 0.99):.2e} "
           f"mean {e.mean():.2e}; Jacobian max|dF| p99 {np.quantile(fe, 0.99):.2e}", flush=True)
     assert np.quantile(e, 0.99) <= 1e-3 and e.max() <= 5e-3
+    if kind == 1:
+        # the far sums' errors are divided by the whole MVC sum, largest
+        # next to the cage: the bar holds at every point
+        assert e.max() <= 1e-3
     net.close()
     untrained.close()
