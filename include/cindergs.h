@@ -519,7 +519,8 @@ int cgs_mlp_train(cgs_mlp* net, const float* d_x, const float* d_targets, uint64
 int cgs_mlp_gradient(cgs_mlp* net, const float* d_x, const float* d_targets, uint32_t n,
                      float* grad, float* loss);
 /* weights [n*n_out] and, if d_grad, their input derivatives [n*n_out*3]; with a
- * near field set, the raw outputs (far share x n_out) and their derivatives */
+ * near field set, the raw outputs (the far targets of the network's kind,
+ * as cgs_mvc_far_targets scales them) and their derivatives */
 int cgs_mlp_infer(cgs_mlp* net, const float* d_x, uint64_t n, float* d_w, float* d_grad);
 /* Near/far split (r0 < r1, both 0 to switch it off).  MVC's sums over the
  * cage triangles split with phi_t(x) = smoothstep((r1 - |x - c_t|) / (r1 - r0)),
